@@ -1,0 +1,143 @@
+/* pffrg.h -- C ABI of the MI355X (gfx950) pf-FRG flow right-hand side.
+ *
+ * Drop-in replacement for the internal Julia functions of PFFRGSolver.jl that
+ * evaluate the flow right-hand side, the parquet equations and the ODE stage
+ * algebra.  All entry points are plain C (ccall / ctypes / cgo compatible):
+ * pointers, sizes and doubles, no C++ types, no exceptions.
+ *
+ * Conventions
+ *  - An opaque context owns one HIP stream and a fixed set of device "slots".
+ *    Each slot holds one SU(2) action: Σ[n_sigma], bare[2][n_sites] and the
+ *    packed vertex  [comp spin,dens][channel s,t,u][q1 | q2_1 | q2_2 | q3]
+ *    with column-major arrays (site fastest), i.e. exactly the memory of the
+ *    reference's Action_su2 / Channel arrays and of its HDF5 datasets.
+ *  - Calls are synchronous with respect to the host unless stated otherwise.
+ *  - Every int-returning call returns 0 on success and a negative code on
+ *    failure; pffrg_last_error() then describes the failure.
+ *  - Tolerances are (atol, rtol) pairs as in the reference's NTuple{2,Float64}.
+ *  - Λ is in units of the normalised coupling (Launcher.jl:647-650).
+ */
+#ifndef PFFRG_H
+#define PFFRG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFFRG_OK 0
+#define PFFRG_ERR_ARG -1
+#define PFFRG_ERR_HIP -2
+#define PFFRG_ERR_STATE -3
+#define PFFRG_ERR_LIMIT -4
+
+/* arithmetic of the vertex passes */
+#define PFFRG_FP64 0        /* FP64 storage, interpolation, products and accumulation */
+#define PFFRG_FP32_MIXED 1  /* FP32 vertex copy, interpolation and products; FP64 quadrature */
+
+typedef struct pffrg_ctx pffrg_ctx;
+
+typedef struct {
+    int32_t n_sites;  /* irreducible sites, 1..64 */
+    int32_t n_sigma;  /* self-energy mesh points (num_σ + 1), <= 64 */
+    int32_t n_Omega;  /* bosonic mesh points (num_Ω + 1), <= 64 */
+    int32_t n_nu;     /* fermionic mesh points (num_ν + 1), <= 64 */
+    double S;         /* spin length (Action_su2.S) */
+    int32_t n_slots;  /* number of device action slots */
+    int32_t device;   /* HIP device ordinal */
+} pffrg_dims;
+
+/* Context lifetime.  Replaces the per-run allocations of launch_1l! (launcher_1l.jl:38-51). */
+int pffrg_create(const pffrg_dims *dims, pffrg_ctx **out);
+void pffrg_destroy(pffrg_ctx *ctx);
+const char *pffrg_last_error(const pffrg_ctx *ctx);
+
+/* Frequency meshes of the Mesh struct (Frequency/mesh.jl:16-27): σ, Ωs, νs, Ωt, νt. */
+int pffrg_set_mesh(pffrg_ctx *ctx, const double *sigma, const double *Os, const double *ns, const double *Ot,
+                   const double *nt);
+
+/* Reduced_lattice tables (Lattice/reduced.jl:15-25): exchange[n_sites], mult[n_sites],
+ * overlap as CSR: ov_off[n_sites+1] row offsets, ov_rows[3*R] = (j1, j2, multiplicity).
+ * index_base = 1 for the reference's 1-based Julia tables, 0 for 0-based ones. */
+int pffrg_set_lattice(pffrg_ctx *ctx, const int64_t *exchange, const double *mult, const int64_t *ov_off,
+                      const int64_t *ov_rows, int32_t index_base);
+
+/* Host <-> device transfer of one action slot; NULL skips a part. */
+int pffrg_upload_action(pffrg_ctx *ctx, int32_t slot, const double *Sigma, const double *bare, const double *gamma);
+int pffrg_download_action(pffrg_ctx *ctx, int32_t slot, double *Sigma, double *bare, double *gamma);
+int64_t pffrg_gamma_length(const pffrg_ctx *ctx);
+
+int pffrg_set_precision(pffrg_ctx *ctx, int32_t mode);
+
+/* compute_dΣ!  (Flow/flow_self.jl:2-20): da.Σ[2:end] from a. */
+int pffrg_compute_dsigma(pffrg_ctx *ctx, double Lambda, int32_t a, int32_t da, const double *sigma_tol);
+
+/* compute_dΣ_corr!  (Flow/flow_self.jl:23-50): multiloop self-energy corrections. */
+int pffrg_compute_dsigma_corr(pffrg_ctx *ctx, double Lambda, int32_t a, int32_t da, int32_t da_sigma,
+                              const double *sigma_tol);
+
+/* compute_dΓ_1l!  (Flow/compute.jl:57-112): Katanin one-loop vertex flow into da
+ * (reads da.Σ, so compute_dsigma must run first). */
+int pffrg_compute_dgamma_1l(pffrg_ctx *ctx, double Lambda, int32_t a, int32_t da, int32_t eval,
+                            const double *gamma_tol);
+
+/* compute_dΓ_2l!  (Flow/compute.jl:115-174). */
+int pffrg_compute_dgamma_2l(pffrg_ctx *ctx, double Lambda, int32_t a, int32_t da, int32_t da_l, int32_t eval,
+                            const double *gamma_tol);
+
+/* compute_dΓ_ml!  (Flow/compute.jl:177-286). */
+int pffrg_compute_dgamma_ml(pffrg_ctx *ctx, double Lambda, int32_t loops, int32_t a, int32_t da, int32_t da_l,
+                            int32_t da_c, int32_t da_temp, int32_t da_sigma, int32_t eval, const double *gamma_tol);
+
+/* compute_Σ!  (Flow/SDE.jl:58-96): Schwinger-Dyson self energy of a1 into a2 (and the
+ * reduced s bubble into a2's s channel q1/q2_2). */
+int pffrg_compute_sigma(pffrg_ctx *ctx, double Lambda, int32_t a1, int32_t a2, int32_t eval, const double *gamma_tol,
+                        const double *sigma_tol);
+
+/* compute_Γ!  (Flow/compute.jl:2-54): Bethe-Salpeter right side of a1 into a2; uses the
+ * boundary corrections left by the preceding pffrg_compute_sigma, as the reference does. */
+int pffrg_compute_gamma(pffrg_ctx *ctx, double Lambda, int32_t a1, int32_t a2, int32_t eval,
+                        const double *gamma_tol);
+
+/* symmetrize! / symmetrize_add_to!  (Action/action_lib/action_su2.jl:141-250). */
+int pffrg_symmetrize(pffrg_ctx *ctx, int32_t slot);
+int pffrg_symmetrize_add_to(pffrg_ctx *ctx, int32_t a_l, int32_t a);
+
+/* Action algebra (Action/Action.jl:180-337): vertex_only = 1 for the *_Γ! variants.
+ * replace_with!(dst, src); mult_with!(slot, fac); mult_with_add_to!(src, fac, dst): dst += fac*src. */
+int pffrg_replace_with(pffrg_ctx *ctx, int32_t dst, int32_t src, int32_t vertex_only);
+int pffrg_mult_with(pffrg_ctx *ctx, int32_t slot, double fac, int32_t vertex_only);
+int pffrg_mult_with_add_to(pffrg_ctx *ctx, int32_t src, double fac, int32_t dst, int32_t vertex_only);
+
+/* get_abs_max  (Action/Action.jl:346-359): max |q3| over components and channels, in FP64. */
+int pffrg_abs_max(pffrg_ctx *ctx, int32_t slot, double *out);
+
+/* Σ of one slot (n_sigma doubles) for the host-side parquet damping (parquet.jl:57-61). */
+int pffrg_get_sigma(pffrg_ctx *ctx, int32_t slot, double *Sigma);
+int pffrg_set_sigma(pffrg_ctx *ctx, int32_t slot, const double *Sigma);
+
+/* Measurement hooks (no reference counterpart; bench and tests only). */
+int pffrg_synchronize(pffrg_ctx *ctx);
+int pffrg_reset_stats(pffrg_ctx *ctx);
+/* stats[0] = quadrature nodes evaluated by vertex passes, stats[1] = vertex-pass elements,
+ * stats[2] = vertex-pass launches, stats[3] = vertex bytes gathered by the interpolations */
+int pffrg_get_stats(pffrg_ctx *ctx, int64_t *stats);
+/* boundary-correction factors corrs[Ω][channel][side] (2*3*n_Omega doubles) of the last pass */
+int pffrg_get_corrs(pffrg_ctx *ctx, double *corrs);
+/* integrand of one output element (pass 0..4 = Katanin, left, central, BSE, reduced SDE; channel
+ * 0..2 = s,t,u; kernel 1..4; 0-based w indices, -1 for unused) at n loop frequencies v, without
+ * quadrature: out[k][comp][site] (n*2*n_sites doubles) */
+int pffrg_probe_integrand(pffrg_ctx *ctx, int32_t pass, double Lambda, int32_t a, int32_t da, int32_t ch,
+                          int32_t kernel, int32_t w1, int32_t w2, int32_t w3, int32_t n, const double *v,
+                          double *out);
+/* enable = 1: time every vertex-pass launch with HIP events on the context stream */
+int pffrg_set_timing(pffrg_ctx *ctx, int32_t enable);
+/* total milliseconds and launch count of the vertex-pass kernel since the last reset */
+int pffrg_get_timing(pffrg_ctx *ctx, double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFFRG_H */

@@ -1,0 +1,1117 @@
+// pffrg_oracle.cpp -- FP64 CPU restatement of the PFFRGSolver.jl SU(2) flow right-hand side.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the parity oracle and the CPU baseline
+// ("port") for the HIP library.  Only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg may load it.  It is never part of the product path.
+//
+// Every routine follows one reference function; the citation is given as
+// file:line relative to the reference repository root.  Arithmetic is kept in
+// the reference's operation order (left-associative products, sequential
+// accumulation) so the GPU FP64 path can be compared to near machine precision.
+//
+// Data layout (same as the reference's column-major Julia arrays, site fastest):
+//   action.gamma = [comp 0..1][channel s,t,u][ q1 | q2_1 | q2_2 | q3 ]
+//     q1  [site, w]          -> site + ns*w
+//     q2_x[site, w, v]       -> site + ns*(w + nO*v)
+//     q3  [site, w, v, vp]   -> site + ns*(w + nO*(v + nn*vp))
+//   action.bare  = [comp][site];  action.Sigma = [n_sigma]
+// Lattice tables are 0-based here (the reference's are 1-based).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+extern "C" {
+
+typedef struct {
+    int32_t n_sites, n_sigma, n_Omega, n_nu;
+    double S;
+    const double *sigma, *Os, *ns, *Ot, *nt;  // meshes, lengths n_sigma / n_Omega / n_nu
+    const int64_t *exchange;                   // [n_sites], 0-based
+    const double *mult;                        // [n_sites]
+    const int64_t *ov_off;                     // [n_sites+1] row offsets into ov_rows
+    const int64_t *ov_rows;                    // [3*rows] (j1, j2 0-based, multiplicity)
+} orc_system;
+
+typedef struct {
+    double *Sigma;  // [n_sigma]
+    double *bare;   // [2*n_sites]
+    double *gamma;  // packed channels, see header
+} orc_action;
+
+}  // extern "C"
+
+namespace {
+
+const double PI = 3.14159265358979323846;
+
+// ---------------------------------------------------------------- Param (param.jl)
+struct Param {
+    int lo, hi;
+    double wl, wh;
+};
+
+// get_indices + get_param, Frequency/param.jl:26-86 (linear search kept as in the reference)
+inline Param get_param(double val, const double *list, int n) {
+    Param p{0, 0, 0.0, 0.0};
+    if (val >= list[n - 1]) {
+        p.lo = p.hi = n - 1;
+    } else {
+        int idx = 0;
+        while (val > list[idx]) ++idx;
+        p.lo = p.hi = idx;
+        if (val < list[idx]) p.lo -= 1;
+    }
+    if (p.lo < p.hi) {
+        double d = 1.0 / (list[p.hi] - list[p.lo]);
+        p.wl = d * (list[p.hi] - val);
+        p.wh = d * (val - list[p.lo]);
+    } else {
+        p.wl = 1.0;
+    }
+    return p;
+}
+
+// ---------------------------------------------------------------- Buffer (buffer.jl)
+struct Buffer {
+    bool exch, map, sgn_mu, sgn_nu;
+    int kernel;
+    Param p1, p2, p3;
+};
+
+inline Buffer buffer_empty() { return Buffer{false, false, false, false, 0, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}}; }
+
+// get_buffer, Frequency/buffer.jl:51-86
+inline Buffer get_buffer(double w, double v, double vp, const double *Om, const double *nu, int nO, int nn,
+                         bool exch, bool map, bool smu, bool snu) {
+    Buffer b = buffer_empty();
+    if (Om[nO - 1] < std::fabs(w)) return b;
+    b.exch = exch; b.map = map; b.sgn_mu = smu; b.sgn_nu = snu;
+    if (nu[nn - 1] < std::fabs(v)) {
+        if (nu[nn - 1] < std::fabs(vp)) {
+            b.kernel = 1; b.p1 = get_param(w, Om, nO);
+        } else {
+            b.kernel = 3; b.p1 = get_param(w, Om, nO); b.p3 = get_param(vp, nu, nn);
+        }
+    } else {
+        if (nu[nn - 1] < std::fabs(vp)) {
+            b.kernel = 2; b.p1 = get_param(w, Om, nO); b.p2 = get_param(v, nu, nn);
+        } else {
+            b.kernel = 4; b.p1 = get_param(w, Om, nO); b.p2 = get_param(v, nu, nn); b.p3 = get_param(vp, nu, nn);
+        }
+    }
+    return b;
+}
+
+// get_buffer_s, Frequency/buffer.jl:88-124
+inline Buffer buffer_s(const orc_system &y, double w, double v, double vp) {
+    bool ex = false, mp = false, smu = false, snu = false;
+    if (w < 0.0) { w *= -1.0; ex = !ex; }
+    if (v < 0.0) { v *= -1.0; ex = !ex; mp = !mp; smu = !smu; }
+    if (vp < 0.0) { vp *= -1.0; mp = !mp; snu = !snu; }
+    return get_buffer(w, v, vp, y.Os, y.ns, y.n_Omega, y.n_nu, ex, mp, smu, snu);
+}
+
+// get_buffer_t, Frequency/buffer.jl:127-160 (sgn_munu is unused by SU(2))
+inline Buffer buffer_t(const orc_system &y, double w, double v, double vp) {
+    bool smu = false, snu = false;
+    if (w < 0.0) { w *= -1.0; }
+    if (v < 0.0) { v *= -1.0; smu = !smu; }
+    if (vp < 0.0) { vp *= -1.0; snu = !snu; }
+    return get_buffer(w, v, vp, y.Ot, y.nt, y.n_Omega, y.n_nu, false, false, smu, snu);
+}
+
+// get_buffer_u, Frequency/buffer.jl:163-199
+inline Buffer buffer_u(const orc_system &y, double w, double v, double vp) {
+    bool ex = false, mp = false, snu = false;
+    if (w < 0.0) { w *= -1.0; ex = !ex; }
+    if (v < 0.0) { v *= -1.0; ex = !ex; mp = !mp; snu = !snu; }
+    if (vp < 0.0) { vp *= -1.0; mp = !mp; snu = !snu; }
+    return get_buffer(w, v, vp, y.Os, y.ns, y.n_Omega, y.n_nu, ex, mp, false, snu);
+}
+
+// ---------------------------------------------------------------- Channel access (channel.jl)
+struct Dims {
+    int ns, nO, nn;
+    long q1, q21, q22, q3, ch, comp;
+};
+
+inline Dims dims_of(const orc_system &y) {
+    Dims d;
+    d.ns = y.n_sites; d.nO = y.n_Omega; d.nn = y.n_nu;
+    d.q1 = 0;
+    d.q21 = (long)d.ns * d.nO;
+    d.q22 = d.q21 + (long)d.ns * d.nO * d.nn;
+    d.q3 = d.q22 + (long)d.ns * d.nO * d.nn;
+    d.ch = d.q3 + (long)d.ns * d.nO * d.nn * d.nn;
+    d.comp = 3 * d.ch;
+    return d;
+}
+
+inline const double *chan(const Dims &d, const double *gamma, int comp, int ch) { return gamma + comp * d.comp + ch * d.ch; }
+inline double *chan(const Dims &d, double *gamma, int comp, int ch) { return gamma + comp * d.comp + ch * d.ch; }
+
+// get_q1 / get_q2_1 / get_q2_2 / get_q3 and get_channel, Action/channel.jl:42-327
+inline double get_channel(const Dims &d, const double *c, const Buffer &b, int site) {
+    const int ns = d.ns, nO = d.nO, nn = d.nn;
+    const Param &p1 = b.p1, &p2 = b.p2, &p3 = b.p3;
+    if (b.kernel == 1) {
+        const double *q = c + d.q1;
+        double val = p1.wl * q[site + ns * p1.lo];
+        val += p1.wh * q[site + ns * p1.hi];
+        return val;
+    } else if (b.kernel == 2 || b.kernel == 3) {
+        const double *q = c + (b.kernel == 2 ? d.q21 : d.q22);
+        const Param &pv = (b.kernel == 2) ? p2 : p3;
+        auto at = [&](int w, int v) { return q[site + (long)ns * (w + (long)nO * v)]; };
+        double val = p1.wl * pv.wl * at(p1.lo, pv.lo);
+        val += p1.wh * pv.wl * at(p1.hi, pv.lo);
+        val += p1.wl * pv.wh * at(p1.lo, pv.hi);
+        val += p1.wh * pv.wh * at(p1.hi, pv.hi);
+        return val;
+    } else if (b.kernel == 4) {
+        const double *q = c + d.q3;
+        auto at = [&](int w, int v, int vp) { return q[site + (long)ns * (w + (long)nO * (v + (long)nn * vp))]; };
+        double val = p1.wl * p2.wl * p3.wl * at(p1.lo, p2.lo, p3.lo);
+        val += p1.wh * p2.wl * p3.wl * at(p1.hi, p2.lo, p3.lo);
+        val += p1.wl * p2.wh * p3.wl * at(p1.lo, p2.hi, p3.lo);
+        val += p1.wh * p2.wh * p3.wl * at(p1.hi, p2.hi, p3.lo);
+        val += p1.wl * p2.wl * p3.wh * at(p1.lo, p2.lo, p3.hi);
+        val += p1.wh * p2.wl * p3.wh * at(p1.hi, p2.lo, p3.hi);
+        val += p1.wl * p2.wh * p3.wh * at(p1.lo, p2.hi, p3.hi);
+        val += p1.wh * p2.wh * p3.wh * at(p1.hi, p2.hi, p3.hi);
+        return val;
+    }
+    return 0.0;
+}
+
+// apply_flags_su2, Action/action_lib/action_su2.jl:73-92
+inline double flag_sign(const Buffer &b, int comp) {
+    double sgn = 1.0;
+    if (comp == 1 && b.sgn_mu) sgn *= -1.0;
+    if (comp == 1 && b.sgn_nu) sgn *= -1.0;
+    return sgn;
+}
+
+struct Mask {
+    bool s, t, u;
+};
+const Mask ALL{true, true, true};
+
+// get_Γ_comp, Action/Action.jl:29-107 (scalar, one site)
+inline double gamma_comp(const orc_system &y, const Dims &d, const orc_action &a, int comp, int site, const Buffer &bs,
+                         const Buffer &bt, const Buffer &bu, Mask m) {
+    double val = a.bare[comp * d.ns + site];
+    if (m.s) {
+        int s = bs.exch ? (int)y.exchange[site] : site;
+        double sgn = flag_sign(bs, comp);
+        val += sgn * get_channel(d, chan(d, a.gamma, comp, bs.map ? 2 : 0), bs, s);
+    }
+    if (m.t) {
+        int s = bt.exch ? (int)y.exchange[site] : site;
+        double sgn = flag_sign(bt, comp);
+        val += sgn * get_channel(d, chan(d, a.gamma, comp, 1), bt, s);
+    }
+    if (m.u) {
+        int s = bu.exch ? (int)y.exchange[site] : site;
+        double sgn = flag_sign(bu, comp);
+        val += sgn * get_channel(d, chan(d, a.gamma, comp, bu.map ? 0 : 2), bu, s);
+    }
+    return val;
+}
+
+// get_Γ_comp_avx! + get_Γ_avx!, Action/Action.jl:110-167 and action_su2.jl:115-134.
+// temp[comp*ns + i]; the bare value is never permuted, the channels are.
+inline void gamma_vec(const orc_system &y, const Dims &d, const orc_action &a, const Buffer &bs, const Buffer &bt,
+                      const Buffer &bu, Mask m, double *temp) {
+    for (int comp = 0; comp < 2; ++comp) {
+        double *t = temp + comp * d.ns;
+        for (int i = 0; i < d.ns; ++i) t[i] = a.bare[comp * d.ns + i];
+        if (m.s && bs.kernel) {
+            double sgn = flag_sign(bs, comp);
+            const double *c = chan(d, a.gamma, comp, bs.map ? 2 : 0);
+            for (int i = 0; i < d.ns; ++i) t[i] += sgn * get_channel(d, c, bs, bs.exch ? (int)y.exchange[i] : i);
+        }
+        if (m.t && bt.kernel) {
+            double sgn = flag_sign(bt, comp);
+            const double *c = chan(d, a.gamma, comp, 1);
+            for (int i = 0; i < d.ns; ++i) t[i] += sgn * get_channel(d, c, bt, bt.exch ? (int)y.exchange[i] : i);
+        }
+        if (m.u && bu.kernel) {
+            double sgn = flag_sign(bu, comp);
+            const double *c = chan(d, a.gamma, comp, bu.map ? 0 : 2);
+            for (int i = 0; i < d.ns; ++i) t[i] += sgn * get_channel(d, c, bu, bu.exch ? (int)y.exchange[i] : i);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- self energy + propagators
+// get_Σ, Action/Action.jl:8-26
+inline double get_sigma(const orc_system &y, const double *Sig, double w) {
+    const int n = y.n_sigma;
+    if (std::fabs(w) <= y.sigma[n - 1]) {
+        Param p = get_param(std::fabs(w), y.sigma, n);
+        double sg = (w > 0.0) ? 1.0 : ((w < 0.0) ? -1.0 : w);  // Julia sign(): sign(±0.0) = ±0.0
+        return sg * (p.wl * Sig[p.lo] + p.wh * Sig[p.hi]);
+    }
+    return y.sigma[n - 1] * Sig[n - 1] / w;
+}
+
+// get_G_bare, Flow/bubbles.jl:2-14
+inline double G_bare(double L, double w) {
+    double val = 0.0;
+    if (std::fabs(w) > 1e-8) val = -std::expm1(-(w * w) / (L * L)) / w;
+    return val;
+}
+
+// get_G, Flow/bubbles.jl:17-42
+inline double G_full(const orc_system &y, double L, double w, const double *Sig) {
+    double val = 0.0;
+    if (std::fabs(w) > 1e-8) {
+        double s = get_sigma(y, Sig, w);
+        double G0 = G_bare(L, w);
+        double denom;
+        if (w * s >= 0.0) denom = 1.0 / G0 + s;
+        else denom = 1.0 / G0;
+        val = 1.0 / denom;
+    }
+    return val;
+}
+
+// get_S, Flow/bubbles.jl:45-61
+inline double S_single(const orc_system &y, double L, double w, const double *Sig) {
+    double val = 0.0;
+    if (std::fabs(w) > 1e-8) {
+        double G = G_full(y, L, w, Sig);
+        double G0 = G_bare(L, w);
+        double r = G / G0;
+        val = r * r * std::exp(-(w * w) / (L * L)) * 2.0 * w / (L * L * L);
+    }
+    return val;
+}
+
+// get_propagator_kat, Flow/bubbles.jl:64-80
+inline double prop_kat(const orc_system &y, double L, double w1, double w2, const double *Sig, const double *dSig) {
+    double dS = get_sigma(y, dSig, w1);
+    double S1 = S_single(y, L, w1, Sig);
+    double G1 = G_full(y, L, w1, Sig);
+    double G2 = G_full(y, L, w2, Sig);
+    return (S1 + G1 * G1 * dS) * G2 / (2.0 * PI);
+}
+
+// get_propagator, Flow/bubbles.jl:83-96
+inline double prop(const orc_system &y, double L, double w1, double w2, const double *Sig) {
+    double G1 = G_full(y, L, w1, Sig);
+    double G2 = G_full(y, L, w2, Sig);
+    return G1 * G2 / (2.0 * PI);
+}
+
+// ---------------------------------------------------------------- adaptive Gauss-Kronrod (QuadGK.jl 2.x)
+// QuadGK.jl is an un-vendored dependency ("QuadGK" compat "2.4", Project.toml).  Its
+// published algorithm is restated: order-7 Gauss / 15-point Kronrod rule, infinite
+// endpoints mapped by x = t/(1-t^2) (both infinite) or x = s0 -/+ t/(1-t) (one
+// infinite), initial segments at the given breakpoints, then repeated bisection of
+// the segment with the largest error estimate until E <= atol or E <= rtol*|I|.
+// Segments are kept in insertion order (the popped segment is replaced by its left
+// half, the right half appended) and re-summed in that order at the end; the same
+// policy is used by the HIP library.  A segment cap bounds memory on both sides.
+const double GK_X[8] = {0.991455371120812639206854697526329, 0.949107912342758524526189684047851,
+                        0.864864423359769072789712788640926, 0.741531185599394439863864773280788,
+                        0.586087235467691130294144845693013, 0.405845151377397166906606412076961,
+                        0.207784955007898467600689403773245, 0.000000000000000000000000000000000};
+const double GK_WK[8] = {0.022935322010529224963732008058970, 0.063092092629978553290700663189204,
+                         0.104790010322250183839876322541518, 0.140653259715525918745189590510238,
+                         0.169004726639267902826583426598550, 0.190350578064785409913256402421014,
+                         0.204432940075298892414161999234649, 0.209482141084727828012999174891714};
+const double GK_WG[4] = {0.129484966168869693270611432679082, 0.279705391489276667901467771423780,
+                         0.381830050505118944950369775488975, 0.417959183673469387755102040816327};
+const int GK_MAX_SEGS = 512;
+
+struct Seg {
+    double a, b, I, E;
+};
+
+template <class F>
+Seg gk_rule(const F &f, double a, double b) {
+    // QuadGK evalrule for odd order (n = 7): Gauss nodes are the even Kronrod nodes
+    double s = 0.5 * (b - a);
+    double Ig = 0.0, Ik = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        double xg = GK_X[2 * i + 1], xk = GK_X[2 * i];
+        double fg = f(a + (1.0 - xg) * s) + f(a + (1.0 + xg) * s);
+        double fk = f(a + (1.0 - xk) * s) + f(a + (1.0 + xk) * s);
+        Ig += fg * GK_WG[i];
+        Ik += fg * GK_WK[2 * i + 1] + fk * GK_WK[2 * i];
+    }
+    double f0 = f(a + s);
+    Ig += f0 * GK_WG[3];
+    Ik += f0 * GK_WK[7] + (f(a + (1.0 - GK_X[6]) * s) + f(a + (1.0 + GK_X[6]) * s)) * GK_WK[6];
+    double Iks = Ik * s, Igs = Ig * s;
+    return Seg{a, b, Iks, std::fabs(Iks - Igs)};
+}
+
+template <class F>
+double gk_adapt(const F &f, const double *bp, int nbp, double atol, double rtol) {
+    std::vector<Seg> segs;
+    segs.reserve(GK_MAX_SEGS);
+    double I = 0.0, E = 0.0;
+    for (int i = 0; i + 1 < nbp; ++i) segs.push_back(gk_rule(f, bp[i], bp[i + 1]));
+    I = segs[0].I; E = segs[0].E;
+    for (size_t i = 1; i < segs.size(); ++i) { I += segs[i].I; E += segs[i].E; }
+    if (E <= atol || E <= rtol * std::fabs(I)) return I;
+    while (E > atol && E > rtol * std::fabs(I) && (int)segs.size() < GK_MAX_SEGS) {
+        int k = 0;
+        for (int i = 1; i < (int)segs.size(); ++i)
+            if (segs[i].E > segs[k].E) k = i;
+        Seg s = segs[k];
+        double mid = 0.5 * (s.a + s.b);
+        Seg s1 = gk_rule(f, s.a, mid), s2 = gk_rule(f, mid, s.b);
+        I = (I - s.I) + s1.I + s2.I;
+        E = (E - s.E) + s1.E + s2.E;
+        segs[k] = s1;
+        segs.push_back(s2);
+    }
+    I = segs[0].I;
+    for (size_t i = 1; i < segs.size(); ++i) I += segs[i].I;
+    return I;
+}
+
+// quadgk over (-Inf, x1, ..., Inf): both endpoints infinite
+template <class F>
+double quadgk_inf_inf(const F &f, const double *inner, int ninner, double atol, double rtol) {
+    std::vector<double> bp;
+    bp.push_back(-1.0);
+    for (int i = 0; i < ninner; ++i) {
+        double x = inner[i];
+        bp.push_back(2.0 * x / (1.0 + std::hypot(1.0, 2.0 * x)));
+    }
+    bp.push_back(1.0);
+    auto g = [&](double t) {
+        double t2 = t * t;
+        double den = 1.0 / (1.0 - t2);
+        return f(t * den) * (1.0 + t2) * den * den;
+    };
+    return gk_adapt(g, bp.data(), (int)bp.size(), atol, rtol);
+}
+
+// quadgk over (s0, Inf) (upper infinite): x = s0 + t/(1-t), t in [0,1]
+template <class F>
+double quadgk_to_inf(const F &f, double s0, double atol, double rtol) {
+    double bp[2] = {0.0, 1.0};
+    auto g = [&](double t) {
+        double den = 1.0 / (1.0 - t);
+        return f(s0 + t * den) * den * den;
+    };
+    return gk_adapt(g, bp, 2, atol, rtol);
+}
+
+// quadgk over (-Inf, s0) (lower infinite): x = s0 - t/(1-t), t in [0,1]
+template <class F>
+double quadgk_from_inf(const F &f, double s0, double atol, double rtol) {
+    double bp[2] = {0.0, 1.0};
+    auto g = [&](double t) {
+        double den = 1.0 / (1.0 - t);
+        return f(s0 - t * den) * den * den;
+    };
+    return gk_adapt(g, bp, 2, atol, rtol);
+}
+
+// ---------------------------------------------------------------- adaptive Simpson (quadrature.jl)
+// buffers are (2 x ns) matrices stored comp-fastest: b[comp + 2*site]
+inline double norm2(const double *b, int n) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += b[i] * b[i];
+    return std::sqrt(s);
+}
+
+// simps!, Flow/quadrature.jl:27-80 (residual! at :2-24 inlined)
+template <class F>
+void simps(const F &f, double *b1, double *b2, int n, double a, double b, double atol, double rtol, long *nodes) {
+    for (int i = 0; i < n; ++i) b1[i] = 0.0;
+    double m = 0.5 * (a + b);
+    f(b1, a, 1.0 * (b - a) / 6.0);
+    f(b1, m, 2.0 * (b - a) / 3.0);
+    f(b1, b, 1.0 * (b - a) / 6.0);
+    *nodes += 3;
+    for (int i = 0; i < n; ++i) b2[i] = 0.5 * b1[i];
+    auto residual = [&](long nn) {
+        double h = 0.5 * (b - a) / nn;
+        for (long i = 1; i <= nn; ++i) f(b2, a + (2.0 * i - 1.0) * h, +4.0 * h / 3.0);
+        for (long i = 1; i <= nn / 2; ++i) f(b2, a + (4.0 * i - 2.0) * h, -2.0 * h / 3.0);
+        *nodes += nn + nn / 2;
+    };
+    residual(2);
+    long nn = 4;
+    while (true) {
+        for (int i = 0; i < n; ++i) b1[i] *= -1.0 / 15.0;
+        for (int i = 0; i < n; ++i) b1[i] += 16.0 / 15.0 * b2[i];
+        double n1 = norm2(b1, n), n2 = norm2(b2, n);
+        for (int i = 0; i < n; ++i) b1[i] -= b2[i];
+        double adiff = norm2(b1, n);
+        double rdiff = adiff / std::max(n1, n2);
+        if (adiff < atol || rdiff < rtol || nn > (1L << 20)) {
+            for (int i = 0; i < n; ++i) b1[i] += b2[i];
+            break;
+        }
+        for (int i = 0; i < n; ++i) b1[i] = b2[i];
+        for (int i = 0; i < n; ++i) b2[i] = 0.5 * b1[i];
+        residual(nn);
+        nn *= 2;
+    }
+}
+
+// integrate_lin!, Flow/quadrature.jl:84-108
+thread_local long *g_rec_nodes = nullptr;  // optional per-sub-interval recording (diagnostics)
+thread_local double *g_rec_res = nullptr;
+
+template <class F>
+void integrate_lin(const F &f, double *t1, double *t2, double *t3, int n, double a, double b, int eval, double atol,
+                   double rtol, long *nodes) {
+    if (b > a) {
+        double h = (b - a) / eval;
+        for (int i = 1; i <= eval; ++i) {
+            long before = *nodes;
+            simps(f, t2, t3, n, a + (i - 1) * h, a + i * h, atol, rtol, nodes);
+            if (g_rec_nodes) { *g_rec_nodes++ = *nodes - before; *g_rec_res++ = t2[0]; }
+            for (int k = 0; k < n; ++k) t1[k] += t2[k];
+        }
+    } else if (g_rec_nodes) {
+        for (int i = 0; i < eval; ++i) { *g_rec_nodes++ = 0; *g_rec_res++ = 0.0; }
+    }
+}
+
+// integrate_log!, Flow/quadrature.jl:113-139 (sgn maps [a,b] to [-b,-a])
+template <class F>
+void integrate_log(const F &f, double *t1, double *t2, double *t3, int n, double a, double b, int eval, double atol,
+                   double rtol, double sgn, long *nodes) {
+    if (b > a) {
+        double xi = std::pow(b / a, 1.0 / eval);
+        auto g = [&](double *buf, double x, double dx) { f(buf, sgn * x, dx); };
+        for (int i = 1; i <= eval; ++i) {
+            long before = *nodes;
+            simps(g, t2, t3, n, std::pow(xi, i - 1) * a, std::pow(xi, i) * a, atol, rtol, nodes);
+            if (g_rec_nodes) { *g_rec_nodes++ = *nodes - before; *g_rec_res++ = t2[0]; }
+            for (int k = 0; k < n; ++k) t1[k] += t2[k];
+        }
+    } else if (g_rec_nodes) {
+        for (int i = 0; i < eval; ++i) { *g_rec_nodes++ = 0; *g_rec_res++ = 0.0; }
+    }
+}
+
+// ---------------------------------------------------------------- pass types
+enum PassType { P_KAT = 0, P_LEFT = 1, P_CENTRAL = 2, P_BSE = 3, P_REDUCED = 4 };
+
+struct PassCtx {
+    const orc_system *y;
+    Dims d;
+    double L;
+    int pass;
+    const orc_action *a;   // propagator action and "full" vertex
+    const orc_action *da;  // kat: dΣ source; left: left vertex; central: right vertex (da_l)
+    const double *corrs;   // [side][ch][nO], side-fastest as in corrs[2,3,nΩ]
+};
+
+inline double corr_at(const PassCtx &c, int side, int ch, int w1) { return c.corrs[side + 2 * (ch + 3 * w1)]; }
+
+// per-channel bosonic pair (w1, w2) for the bubble: s (flow_su2_s.jl:18), t (flow_su2_t.jl:18), u (flow_su2_u.jl:18)
+inline void bubble_args(int ch, double W, double v, double &x1, double &x2) {
+    if (ch == 0) { x1 = v + 0.5 * W; x2 = 0.5 * W - v; }
+    else if (ch == 1) { x1 = v + 0.5 * W; x2 = v - 0.5 * W; }
+    else { x1 = v - 0.5 * W; x2 = v + 0.5 * W; }
+}
+
+inline double bubble(const PassCtx &c, int ch, double W, double v) {
+    double x1, x2;
+    bubble_args(ch, W, v, x1, x2);
+    const orc_system &y = *c.y;
+    if (c.pass == P_KAT) return prop_kat(y, c.L, x1, x2, c.a->Sigma, c.da->Sigma) + prop_kat(y, c.L, x2, x1, c.a->Sigma, c.da->Sigma);
+    return -prop(y, c.L, x1, x2, c.a->Sigma);
+}
+
+// Integrands: compute_{s,t,u}_{kat,left,central}! (Flow/flow_lib/flow_su2/flow_su2_{s,t,u}.jl),
+// compute_{s,t,u}_BSE! (Flow/parquet_lib/parquet_su2/BSE_su2_{s,t,u}.jl) and
+// compute_s_reduced! (Flow/parquet_lib/parquet_su2/SDE_su2.jl:2-43).
+struct Integrand {
+    const PassCtx *c;
+    int ch;
+    double W, vw, vwp;  // element frequencies (Ω, ν, ν')
+    mutable std::vector<double> t1, t2;  // vertex caches [comp*ns + i]
+
+    void operator()(double *buff, double v, double dv) const {
+        const PassCtx &C = *c;
+        const orc_system &y = *C.y;
+        const Dims &d = C.d;
+        const int ns = d.ns;
+        const double p = bubble(C, ch, W, v);
+        const orc_action &A = *C.a;
+        const orc_action &D = *C.da;
+        if (ch == 0) {
+            const double s = W, vs = vw, vsp = vwp;
+            Buffer bs2 = buffer_s(y, s, v, vsp);
+            Buffer bt2 = buffer_t(y, -v - vsp, 0.5 * (s + v - vsp), 0.5 * (s - v + vsp));
+            Buffer bu2 = buffer_u(y, v - vsp, 0.5 * (s + v + vsp), 0.5 * (s - v - vsp));
+            if (C.pass == P_REDUCED) {
+                gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data());
+                for (int i = 0; i < ns; ++i) {
+                    double v1s = A.bare[i], v1d = A.bare[ns + i];
+                    double v2s = t2[i], v2d = t2[ns + i];
+                    double Gs = -p * (-2.0 * v1s * v2s + v1s * v2d + v1d * v2s);
+                    double Gd = -p * (3.0 * v1s * v2s + v1d * v2d);
+                    buff[2 * i] += dv * Gs;
+                    buff[2 * i + 1] += dv * Gd;
+                }
+                return;
+            }
+            Buffer bs1 = buffer_s(y, s, vs, -v);
+            Buffer bt1 = buffer_t(y, v - vs, 0.5 * (s + v + vs), 0.5 * (s - v - vs));
+            Buffer bu1 = buffer_u(y, v + vs, 0.5 * (s - v + vs), 0.5 * (s + v - vs));
+            switch (C.pass) {
+                case P_KAT: gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
+                case P_LEFT: gamma_vec(y, d, D, bs1, bt1, bu1, Mask{false, true, true}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
+                case P_CENTRAL: gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, D, bs2, bt2, bu2, Mask{true, false, false}, t2.data()); break;
+                case P_BSE: gamma_vec(y, d, A, bs1, bt1, bu1, Mask{false, true, true}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
+            }
+            for (int i = 0; i < ns; ++i) {
+                double v1s = t1[i], v1d = t1[ns + i], v2s = t2[i], v2d = t2[ns + i];
+                double Gs = -p * (-2.0 * v1s * v2s + v1s * v2d + v1d * v2s);
+                double Gd = -p * (3.0 * v1s * v2s + v1d * v2d);
+                buff[2 * i] += dv * Gs;
+                buff[2 * i + 1] += dv * Gd;
+            }
+        } else if (ch == 2) {
+            const double u = W, vu = vw, vup = vwp;
+            Buffer bs1 = buffer_s(y, v + vu, 0.5 * (u - v + vu), 0.5 * (-u - v + vu));
+            Buffer bt1 = buffer_t(y, v - vu, 0.5 * (u + v + vu), 0.5 * (-u + v + vu));
+            Buffer bu1 = buffer_u(y, u, vu, v);
+            Buffer bs2 = buffer_s(y, v + vup, 0.5 * (u + v - vup), 0.5 * (-u + v - vup));
+            Buffer bt2 = buffer_t(y, -v + vup, 0.5 * (u + v + vup), 0.5 * (-u + v + vup));
+            Buffer bu2 = buffer_u(y, u, v, vup);
+            switch (C.pass) {
+                case P_KAT: gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
+                case P_LEFT: gamma_vec(y, d, D, bs1, bt1, bu1, Mask{true, true, false}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
+                case P_CENTRAL: gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, D, bs2, bt2, bu2, Mask{false, false, true}, t2.data()); break;
+                case P_BSE: gamma_vec(y, d, A, bs1, bt1, bu1, Mask{true, true, false}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
+            }
+            for (int i = 0; i < ns; ++i) {
+                double v1s = t1[i], v1d = t1[ns + i], v2s = t2[i], v2d = t2[ns + i];
+                double Gs = -p * (2.0 * v1s * v2s + v1s * v2d + v1d * v2s);
+                double Gd = -p * (3.0 * v1s * v2s + v1d * v2d);
+                buff[2 * i] += dv * Gs;
+                buff[2 * i + 1] += dv * Gd;
+            }
+        } else {
+            const double t = W, vt = vw, vtp = vwp;
+            Buffer bs1 = buffer_s(y, v + vt, 0.5 * (-t - v + vt), 0.5 * (-t + v - vt));
+            Buffer bt1 = buffer_t(y, t, vt, v);
+            Buffer bu1 = buffer_u(y, -v + vt, 0.5 * (-t + v + vt), 0.5 * (t + v + vt));
+            Buffer bs2 = buffer_s(y, v + vtp, 0.5 * (-t + v - vtp), 0.5 * (-t - v + vtp));
+            Buffer bt2 = buffer_t(y, t, v, vtp);
+            Buffer bu2 = buffer_u(y, v - vtp, 0.5 * (-t + v + vtp), 0.5 * (t + v + vtp));
+            Buffer bs3 = buffer_s(y, v + vt, 0.5 * (t + v - vt), 0.5 * (-t + v - vt));
+            Buffer bt3 = buffer_t(y, -v + vt, 0.5 * (t + v + vt), 0.5 * (-t + v + vt));
+            Buffer bu3 = buffer_u(y, t, v, vt);
+            Buffer bs4 = buffer_s(y, v + vtp, 0.5 * (t - v + vtp), 0.5 * (-t - v + vtp));
+            Buffer bt4 = buffer_t(y, v - vtp, 0.5 * (t + v + vtp), 0.5 * (-t + v + vtp));
+            Buffer bu4 = buffer_u(y, t, vtp, v);
+            double v3s, v3d, v4s, v4d;
+            switch (C.pass) {
+                case P_KAT:
+                    v3s = gamma_comp(y, d, A, 0, 0, bs3, bt3, bu3, ALL); v3d = gamma_comp(y, d, A, 1, 0, bs3, bt3, bu3, ALL);
+                    v4s = gamma_comp(y, d, A, 0, 0, bs4, bt4, bu4, ALL); v4d = gamma_comp(y, d, A, 1, 0, bs4, bt4, bu4, ALL);
+                    gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data());
+                    break;
+                case P_LEFT:
+                    v3s = gamma_comp(y, d, D, 0, 0, bs3, bt3, bu3, Mask{true, true, false}); v3d = gamma_comp(y, d, D, 1, 0, bs3, bt3, bu3, Mask{true, true, false});
+                    v4s = gamma_comp(y, d, A, 0, 0, bs4, bt4, bu4, ALL); v4d = gamma_comp(y, d, A, 1, 0, bs4, bt4, bu4, ALL);
+                    gamma_vec(y, d, D, bs1, bt1, bu1, Mask{true, false, true}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data());
+                    break;
+                case P_CENTRAL:
+                    v3s = gamma_comp(y, d, A, 0, 0, bs3, bt3, bu3, ALL); v3d = gamma_comp(y, d, A, 1, 0, bs3, bt3, bu3, ALL);
+                    v4s = gamma_comp(y, d, D, 0, 0, bs4, bt4, bu4, Mask{false, false, true}); v4d = gamma_comp(y, d, D, 1, 0, bs4, bt4, bu4, Mask{false, false, true});
+                    gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, D, bs2, bt2, bu2, Mask{false, true, false}, t2.data());
+                    break;
+                default:  // P_BSE
+                    v3s = gamma_comp(y, d, A, 0, 0, bs3, bt3, bu3, Mask{true, true, false}); v3d = gamma_comp(y, d, A, 1, 0, bs3, bt3, bu3, Mask{true, true, false});
+                    v4s = gamma_comp(y, d, A, 0, 0, bs4, bt4, bu4, ALL); v4d = gamma_comp(y, d, A, 1, 0, bs4, bt4, bu4, ALL);
+                    gamma_vec(y, d, A, bs1, bt1, bu1, Mask{true, false, true}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data());
+                    break;
+            }
+            const double S2 = 2.0 * y.S;
+            for (int i = 0; i < ns; ++i) {
+                double v1s = t1[i], v1d = t1[ns + i], v2s = t2[i], v2d = t2[ns + i];
+                double Gs = -p * (-1.0 * v1s * v4s + v1s * v4d - 1.0 * v3s * v2s + v3d * v2s);
+                double Gd = -p * (3.0 * v1d * v4s + v1d * v4d + 3.0 * v3s * v2d + v3d * v2d);
+                for (long r = y.ov_off[i]; r < y.ov_off[i + 1]; ++r) {
+                    int j1 = (int)y.ov_rows[3 * r], j2 = (int)y.ov_rows[3 * r + 1];
+                    double m = (double)y.ov_rows[3 * r + 2];
+                    double w1s = t1[j1], w1d = t1[ns + j1], w2s = t2[j2], w2d = t2[ns + j2];
+                    Gs += -p * (-2.0) * m * S2 * w1s * w2s;
+                    Gd += -p * (-2.0) * m * S2 * w1d * w2d;
+                }
+                buff[2 * i] += dv * Gs;
+                buff[2 * i + 1] += dv * Gd;
+            }
+        }
+    }
+};
+
+// get_kernel_args, Frequency/mesh.jl:30-79 (u uses the s meshes); indices 0-based, -1 = "Inf"
+inline void kernel_args(const orc_system &y, int ch, int kernel, int w1, int w2, int w3, double &W, double &v, double &vp) {
+    const double *Om = (ch == 1) ? y.Ot : y.Os;
+    const double *nu = (ch == 1) ? y.nt : y.ns;
+    W = Om[w1];
+    v = (kernel == 2 || kernel == 4) ? nu[w2] : INFINITY;
+    vp = (kernel == 3 || kernel == 4) ? nu[w3] : INFINITY;
+}
+
+// compute_channel_{s,t,u}_{kat,left,central}! (Flow/flow_{s,t,u}.jl), compute_channel_*_BSE!
+// (Flow/BSE_*.jl) and compute_channel_s_reduced! (Flow/SDE.jl:2-56): one output element.
+long compute_element(const PassCtx &C, int ch, int kernel, int w1, int w2, int w3, orc_action &out, double *t1, double *t2,
+                     double *t3, int eval, double atol, double rtol) {
+    const orc_system &y = *C.y;
+    const Dims &d = C.d;
+    const int n = 2 * d.ns;
+    for (int i = 0; i < n; ++i) t1[i] = 0.0;
+    double W, vw, vwp;
+    if (C.pass == P_REDUCED) {
+        // SDE.jl:13-19: kernel 1 -> (Ωs[w1], Inf); else (Ωs[w1], νs[w3])
+        W = y.Os[w1]; vw = 0.0; vwp = (kernel == 1) ? INFINITY : y.ns[w3];
+    } else {
+        kernel_args(y, ch, kernel, w1, w2, w3, W, vw, vwp);
+    }
+    Integrand f{&C, ch, W, vw, vwp, std::vector<double>(n), std::vector<double>(n)};
+    const double ref = C.L + 0.5 * W;
+    const double val = (ch == 1) ? (y.Ot[d.nO - 1] + y.nt[d.nn - 1]) : (y.Os[d.nO - 1] + y.ns[d.nn - 1]);
+    long nodes = 0;
+    integrate_log(f, t1, t2, t3, n, 2.0 * ref, 1.0 * val, eval, atol, rtol, -1.0, &nodes);
+    integrate_lin(f, t1, t2, t3, n, -2.0 * ref, 0.0 * ref, eval, atol, rtol, &nodes);
+    integrate_lin(f, t1, t2, t3, n, 0.0 * ref, 2.0 * ref, eval, atol, rtol, &nodes);
+    integrate_log(f, t1, t2, t3, n, 2.0 * ref, 1.0 * val, eval, atol, rtol, 1.0, &nodes);
+    f(t1, -val, corr_at(C, 0, ch, w1));
+    f(t1, val, corr_at(C, 1, ch, w1));
+    nodes += 2;
+    // parse result (flow_s.jl:41-51)
+    for (int comp = 0; comp < 2; ++comp) {
+        double *c = chan(d, out.gamma, comp, ch);
+        long base;
+        if (kernel == 1) base = d.q1 + (long)d.ns * w1;
+        else if (kernel == 2) base = d.q21 + (long)d.ns * (w1 + (long)d.nO * w2);
+        else if (kernel == 3) base = d.q22 + (long)d.ns * (w1 + (long)d.nO * w3);
+        else base = d.q3 + (long)d.ns * (w1 + (long)d.nO * (w2 + (long)d.nn * w3));
+        for (int i = 0; i < d.ns; ++i) c[base + i] = t1[comp + 2 * i];
+    }
+    return nodes;
+}
+
+struct Elem {
+    int ch, kernel, w1, w2, w3;
+};
+
+// element enumeration of compute_Γ!/compute_dΓ_*! (Flow/compute.jl:2-286): half passes
+// have w2 >= w3 for q3, full (left) passes all (w2, w3)
+std::vector<Elem> elements(const Dims &d, bool full) {
+    std::vector<Elem> e;
+    for (int w1 = 0; w1 < d.nO; ++w1) {
+        for (int ch = 0; ch < 3; ++ch) e.push_back({ch, 1, w1, -1, -1});
+        for (int w3 = 0; w3 < d.nn; ++w3) {
+            for (int ch = 0; ch < 3; ++ch) e.push_back({ch, 2, w1, w3, -1});
+            for (int ch = 0; ch < 3; ++ch) e.push_back({ch, 3, w1, -1, w3});
+            for (int w2 = full ? 0 : w3; w2 < d.nn; ++w2)
+                for (int ch = 0; ch < 3; ++ch) e.push_back({ch, 4, w1, w2, w3});
+        }
+    }
+    return e;
+}
+
+long run_pass(const PassCtx &C, const std::vector<Elem> &E, orc_action &out, int eval, double atol, double rtol) {
+    long total = 0;
+    const int n = 2 * C.d.ns;
+#pragma omp parallel reduction(+ : total)
+    {
+        std::vector<double> t1(n), t2(n), t3(n);
+#pragma omp for schedule(dynamic, 1)
+        for (long k = 0; k < (long)E.size(); ++k) {
+            const Elem &e = E[k];
+            total += compute_element(C, e.ch, e.kernel, e.w1, e.w2, e.w3, out, t1.data(), t2.data(), t3.data(), eval, atol, rtol);
+        }
+    }
+    return total;
+}
+
+// symmetrize!, Action/action_lib/action_su2.jl:141-173
+void symmetrize(const orc_system &y, const Dims &d, orc_action &a) {
+    for (int comp = 0; comp < 2; ++comp)
+        for (int ch = 0; ch < 3; ++ch) {
+            double *q3 = chan(d, a.gamma, comp, ch) + d.q3;
+            for (int v = 0; v < d.nn; ++v)
+                for (int vp = v + 1; vp < d.nn; ++vp)
+                    for (int w = 0; w < d.nO; ++w)
+                        for (int i = 0; i < d.ns; ++i) {
+                            int src = (ch == 2) ? i : (int)y.exchange[i];
+                            q3[i + (long)d.ns * (w + (long)d.nO * (v + (long)d.nn * vp))] =
+                                q3[src + (long)d.ns * (w + (long)d.nO * (vp + (long)d.nn * v))];
+                        }
+        }
+}
+
+// symmetrize_add_to!, Action/action_lib/action_su2.jl:176-250
+void symmetrize_add_to(const orc_system &y, const Dims &d, const orc_action &al, orc_action &a) {
+    for (int comp = 0; comp < 2; ++comp)
+        for (int ch = 0; ch < 3; ++ch) {
+            const double *l = chan(d, al.gamma, comp, ch);
+            double *o = chan(d, a.gamma, comp, ch);
+            auto ex = [&](int i) { return (ch == 2) ? i : (int)y.exchange[i]; };
+            for (int w = 0; w < d.nO; ++w)
+                for (int i = 0; i < d.ns; ++i)
+                    o[d.q1 + i + (long)d.ns * w] += l[d.q1 + i + (long)d.ns * w] + l[d.q1 + ex(i) + (long)d.ns * w];
+            for (int v = 0; v < d.nn; ++v)
+                for (int w = 0; w < d.nO; ++w)
+                    for (int i = 0; i < d.ns; ++i) {
+                        long k = i + (long)d.ns * (w + (long)d.nO * v), kx = ex(i) + (long)d.ns * (w + (long)d.nO * v);
+                        double a21 = l[d.q21 + k] + l[d.q22 + kx];
+                        double a22 = l[d.q22 + k] + l[d.q21 + kx];
+                        o[d.q21 + k] += a21;
+                        o[d.q22 + k] += a22;
+                    }
+            for (int vp = 0; vp < d.nn; ++vp)
+                for (int v = 0; v < d.nn; ++v)
+                    for (int w = 0; w < d.nO; ++w)
+                        for (int i = 0; i < d.ns; ++i) {
+                            long k = i + (long)d.ns * (w + (long)d.nO * (v + (long)d.nn * vp));
+                            long kx = ex(i) + (long)d.ns * (w + (long)d.nO * (vp + (long)d.nn * v));
+                            o[d.q3 + k] += l[d.q3 + k] + l[d.q3 + kx];
+                        }
+        }
+}
+
+long gamma_size(const Dims &d) { return 2 * d.comp; }
+
+// compute_corrs_kat! / compute_corrs!, Flow/bubbles.jl:99-220
+void compute_corrs(const orc_system &y, double L, const orc_action &a, const orc_action *da, bool kat, double *corrs,
+                   double atol, double rtol) {
+    const int nO = y.n_Omega, nn = y.n_nu;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < nO; ++i) {
+        for (int ch = 0; ch < 3; ++ch) {
+            const double W = (ch == 1) ? y.Ot[i] : y.Os[i];
+            const double val = (ch == 1) ? (y.Ot[nO - 1] + y.nt[nn - 1]) : (y.Os[nO - 1] + y.ns[nn - 1]);
+            auto f = [&](double v) {
+                double x1, x2;
+                bubble_args(ch, W, v, x1, x2);
+                if (kat) return prop_kat(y, L, x1, x2, a.Sigma, da->Sigma) + prop_kat(y, L, x2, x1, a.Sigma, da->Sigma);
+                return -prop(y, L, x1, x2, a.Sigma);
+            };
+            double bm = f(-val), bp = f(val);
+            corrs[0 + 2 * (ch + 3 * i)] = 0.0;
+            corrs[1 + 2 * (ch + 3 * i)] = 0.0;
+            if (std::fabs(bm) > 1e-8) corrs[0 + 2 * (ch + 3 * i)] = quadgk_from_inf(f, -val, atol, rtol) / bm;
+            if (std::fabs(bp) > 1e-8) corrs[1 + 2 * (ch + 3 * i)] = quadgk_to_inf(f, val, atol, rtol) / bp;
+        }
+    }
+}
+
+// vertex buffers shared by the self-energy kernels, Flow/flow_lib/flow_su2/flow_su2_self.jl:11-19
+struct SigmaBufs {
+    Buffer b1s, b1t, b1u, b2s, b2t, b2u;
+};
+inline SigmaBufs sigma_bufs(const orc_system &y, double w, double v) {
+    SigmaBufs b;
+    b.b1s = buffer_s(y, v + w, 0.5 * (-v + w), 0.5 * (v - w));
+    b.b1t = buffer_t(y, 0.0, w, v);
+    b.b1u = buffer_u(y, -v + w, 0.5 * (v + w), 0.5 * (v + w));
+    b.b2s = buffer_s(y, v + w, 0.5 * (-v + w), 0.5 * (-v + w));
+    b.b2t = buffer_t(y, v - w, 0.5 * (v + w), 0.5 * (v + w));
+    b.b2u = buffer_u(y, 0.0, w, v);
+    return b;
+}
+
+// loop value 3Γs + Γd (local) - Σ_j 2 m_j (2S) Γd_j (non-local), flow_su2_self.jl:22-27
+inline double sigma_loop(const orc_system &y, const Dims &d, const orc_action &a, const SigmaBufs &b, Mask ml, Mask mn) {
+    double val = 3.0 * gamma_comp(y, d, a, 0, 0, b.b2s, b.b2t, b.b2u, ml) + gamma_comp(y, d, a, 1, 0, b.b2s, b.b2t, b.b2u, ml);
+    for (int j = 0; j < d.ns; ++j) val -= 2.0 * y.mult[j] * (2.0 * y.S) * gamma_comp(y, d, a, 1, j, b.b1s, b.b1t, b.b1u, mn);
+    return val;
+}
+
+const double SIGMA_INNER_BP_SCALE[3] = {-2.0, 0.0, 2.0};
+
+}  // namespace
+
+// ======================================================================== C API
+extern "C" {
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+// get_param (param.jl:65-86): returns lo, hi (0-based) and weights
+void orc_get_param(double val, const double *list, int n, int32_t *lo, int32_t *hi, double *wl, double *wh) {
+    Param p = get_param(val, list, n);
+    *lo = p.lo; *hi = p.hi; *wl = p.wl; *wh = p.wh;
+}
+
+// get_buffer_{s,t,u} (buffer.jl:88-199): out_i = [exch, map, sgn_mu, sgn_nu, kernel, lo1, hi1, lo2, hi2, lo3, hi3]
+void orc_get_buffer(const orc_system *y, int ch, double w, double v, double vp, int32_t *out_i, double *out_w) {
+    Buffer b = (ch == 0) ? buffer_s(*y, w, v, vp) : (ch == 1) ? buffer_t(*y, w, v, vp) : buffer_u(*y, w, v, vp);
+    out_i[0] = b.exch; out_i[1] = b.map; out_i[2] = b.sgn_mu; out_i[3] = b.sgn_nu; out_i[4] = b.kernel;
+    out_i[5] = b.p1.lo; out_i[6] = b.p1.hi; out_i[7] = b.p2.lo; out_i[8] = b.p2.hi; out_i[9] = b.p3.lo; out_i[10] = b.p3.hi;
+    out_w[0] = b.p1.wl; out_w[1] = b.p1.wh; out_w[2] = b.p2.wl; out_w[3] = b.p2.wh; out_w[4] = b.p3.wl; out_w[5] = b.p3.wh;
+}
+
+// vertex component on all sites for a buffer triple built from (ch, w, v, vp) frequencies of the
+// three channels: get_Γ_avx! with every channel enabled. freqs = [ws, vs, vps, wt, vt, vpt, wu, vu, vpu]
+void orc_get_gamma_avx(const orc_system *y, const orc_action *a, const double *freqs, int32_t mask, double *out) {
+    Dims d = dims_of(*y);
+    Buffer bs = buffer_s(*y, freqs[0], freqs[1], freqs[2]);
+    Buffer bt = buffer_t(*y, freqs[3], freqs[4], freqs[5]);
+    Buffer bu = buffer_u(*y, freqs[6], freqs[7], freqs[8]);
+    gamma_vec(*y, d, *a, bs, bt, bu, Mask{(mask & 1) != 0, (mask & 2) != 0, (mask & 4) != 0}, out);
+}
+
+// vertex value of one channel (vertex.jl get_vertex) at one site, no flags applied
+double orc_get_vertex(const orc_system *y, const orc_action *a, int comp, int ch, double w, double v, double vp, int site) {
+    Dims d = dims_of(*y);
+    Buffer b = (ch == 0) ? buffer_s(*y, w, v, vp) : (ch == 1) ? buffer_t(*y, w, v, vp) : buffer_u(*y, w, v, vp);
+    return get_channel(d, chan(d, a->gamma, comp, ch), b, site);
+}
+
+double orc_get_sigma(const orc_system *y, const double *Sig, double w) { return get_sigma(*y, Sig, w); }
+double orc_G_bare(double L, double w) { return G_bare(L, w); }
+double orc_G(const orc_system *y, double L, double w, const double *Sig) { return G_full(*y, L, w, Sig); }
+double orc_S(const orc_system *y, double L, double w, const double *Sig) { return S_single(*y, L, w, Sig); }
+double orc_prop_kat(const orc_system *y, double L, double w1, double w2, const double *Sig, const double *dSig) {
+    return prop_kat(*y, L, w1, w2, Sig, dSig);
+}
+double orc_prop(const orc_system *y, double L, double w1, double w2, const double *Sig) { return prop(*y, L, w1, w2, Sig); }
+
+// Flow/test.jl test kernel: b[i] += dx * x^i exp(-x^2) sin(2πx), i = 1..n, integrated by
+// integrate_lin! (mode 0) or integrate_log! (mode 1) with the given eval and tolerances.
+void orc_test_integrate(int mode, int n, double a, double b, int eval, double atol, double rtol, double *out) {
+    std::vector<double> t2(n), t3(n);
+    for (int i = 0; i < n; ++i) out[i] = 0.0;
+    auto f = [&](double *buf, double x, double dx) {
+        for (int i = 0; i < n; ++i) buf[i] += dx * std::pow(x, i + 1) * std::exp(-x * x) * std::sin(2.0 * PI * x);
+    };
+    long nodes = 0;
+    if (mode == 0) integrate_lin(f, out, t2.data(), t3.data(), n, a, b, eval, atol, rtol, &nodes);
+    else integrate_log(f, out, t2.data(), t3.data(), n, a, b, eval, atol, rtol, 1.0, &nodes);
+}
+
+// quadgk restatement on closed-form test integrands: kind 0 = exp(-x^2) over (-Inf, bp.., Inf);
+// kind 1 = 1/(1+x^2)^2 over (s0, Inf); kind 2 = exp(x) over (-Inf, s0)
+double orc_test_quadgk(int kind, double s0, double atol, double rtol) {
+    if (kind == 0) {
+        double inner[3] = {-2.0 * s0, 0.0, 2.0 * s0};
+        return quadgk_inf_inf([](double x) { return std::exp(-x * x); }, inner, 3, atol, rtol);
+    } else if (kind == 1) {
+        return quadgk_to_inf([](double x) { return 1.0 / ((1.0 + x * x) * (1.0 + x * x)); }, s0, atol, rtol);
+    }
+    return quadgk_from_inf([](double x) { return std::exp(x); }, s0, atol, rtol);
+}
+
+void orc_compute_corrs(const orc_system *y, double L, const orc_action *a, const orc_action *da, int kat, double *corrs,
+                       const double *tol) {
+    compute_corrs(*y, L, *a, da, kat != 0, corrs, tol[0], tol[1]);
+}
+
+// one output element of any pass (for fine-grained parity checks); returns the node count
+long orc_compute_element(const orc_system *y, int pass, double L, const orc_action *a, const orc_action *da,
+                         const double *corrs, int ch, int kernel, int w1, int w2, int w3, int eval, const double *tol,
+                         orc_action *out) {
+    PassCtx C{y, dims_of(*y), L, pass, a, da, corrs};
+    std::vector<double> t1(2 * y->n_sites), t2(2 * y->n_sites), t3(2 * y->n_sites);
+    return compute_element(C, ch, kernel, w1, w2, w3, *out, t1.data(), t2.data(), t3.data(), eval, tol[0], tol[1]);
+}
+
+// one element with per-sub-interval node counts and spin results of site 0 recorded (4*eval each)
+long orc_compute_element_rec(const orc_system *y, int pass, double L, const orc_action *a, const orc_action *da,
+                             const double *corrs, int ch, int kernel, int w1, int w2, int w3, int eval,
+                             const double *tol, orc_action *out, long *rec_nodes, double *rec_res) {
+    g_rec_nodes = rec_nodes;
+    g_rec_res = rec_res;
+    long n = orc_compute_element(y, pass, L, a, da, corrs, ch, kernel, w1, w2, w3, eval, tol, out);
+    g_rec_nodes = nullptr;
+    g_rec_res = nullptr;
+    return n;
+}
+
+// integrand of one element at loop frequency v (dv = 1): out[comp*ns + i]
+void orc_integrand(const orc_system *y, int pass, double L, const orc_action *a, const orc_action *da, int ch,
+                   int kernel, int w1, int w2, int w3, double v, double *out) {
+    PassCtx C{y, dims_of(*y), L, pass, a, da, nullptr};
+    double W, vw, vwp;
+    if (pass == P_REDUCED) {
+        W = y->Os[w1]; vw = 0.0; vwp = (kernel == 1) ? INFINITY : y->ns[w3];
+    } else {
+        kernel_args(*y, ch, kernel, w1, w2, w3, W, vw, vwp);
+    }
+    const int n = 2 * y->n_sites;
+    Integrand f{&C, ch, W, vw, vwp, std::vector<double>(n), std::vector<double>(n)};
+    std::vector<double> b(n, 0.0);
+    f(b.data(), v, 1.0);
+    for (int i = 0; i < y->n_sites; ++i) {
+        out[i] = b[2 * i];
+        out[y->n_sites + i] = b[2 * i + 1];
+    }
+}
+
+// a whole pass without symmetrisation: pass in {KAT, LEFT, CENTRAL, BSE, REDUCED}; full = all (w2,w3)
+long orc_run_pass(const orc_system *y, int pass, double L, const orc_action *a, const orc_action *da, const double *corrs,
+                  int full, int eval, const double *tol, orc_action *out) {
+    Dims d = dims_of(*y);
+    PassCtx C{y, d, L, pass, a, da, corrs};
+    std::vector<Elem> E;
+    if (pass == P_REDUCED) {
+        for (int w1 = 0; w1 < d.nO; ++w1) {
+            E.push_back({0, 1, w1, -1, -1});
+            for (int w3 = 0; w3 < d.nn; ++w3) E.push_back({0, 3, w1, -1, w3});
+        }
+    } else {
+        E = elements(d, full != 0);
+    }
+    return run_pass(C, E, *out, eval, tol[0], tol[1]);
+}
+
+void orc_symmetrize(const orc_system *y, orc_action *a) { symmetrize(*y, dims_of(*y), *a); }
+void orc_symmetrize_add_to(const orc_system *y, const orc_action *al, orc_action *a) {
+    symmetrize_add_to(*y, dims_of(*y), *al, *a);
+}
+
+// compute_dΣ!, Flow/flow_self.jl:2-20 with compute_dΣ_kernel (flow_su2_self.jl:2-33)
+void orc_compute_dsigma(const orc_system *y, double L, const orc_action *a, orc_action *da, const double *tol) {
+    Dims d = dims_of(*y);
+    double inner[3] = {-2.0 * L, 0.0, 2.0 * L};
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 1; i < y->n_sigma; ++i) {
+        const double w = y->sigma[i];
+        auto f = [&](double v) {
+            SigmaBufs b = sigma_bufs(*y, w, v);
+            double val = sigma_loop(*y, d, *a, b, ALL, ALL);
+            val *= S_single(*y, L, v, a->Sigma) / (2.0 * PI);
+            return val;
+        };
+        da->Sigma[i] = quadgk_inf_inf(f, inner, 3, tol[0], tol[1]);
+    }
+}
+
+// compute_dΣ_corr!, Flow/flow_self.jl:23-50 with the kernels at flow_su2_self.jl:36-101
+void orc_compute_dsigma_corr(const orc_system *y, double L, const orc_action *a, orc_action *da, orc_action *daS,
+                             const double *tol) {
+    Dims d = dims_of(*y);
+    double inner[3] = {-2.0 * L, 0.0, 2.0 * L};
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 1; i < y->n_sigma; ++i) {
+        const double w = y->sigma[i];
+        auto f = [&](double v) {
+            SigmaBufs b = sigma_bufs(*y, w, v);
+            double val = sigma_loop(*y, d, *daS, b, Mask{true, true, false}, Mask{true, false, true});
+            val *= -G_full(*y, L, v, a->Sigma) / (2.0 * PI);
+            return val;
+        };
+        daS->Sigma[i] = quadgk_inf_inf(f, inner, 3, tol[0], tol[1]);
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 1; i < y->n_sigma; ++i) {
+        const double w = y->sigma[i];
+        auto f = [&](double v) {
+            SigmaBufs b = sigma_bufs(*y, w, v);
+            double val = sigma_loop(*y, d, *a, b, ALL, ALL);
+            double G = G_full(*y, L, v, a->Sigma);
+            val *= G * G * get_sigma(*y, daS->Sigma, v) / (2.0 * PI);
+            return val;
+        };
+        da->Sigma[i] += daS->Sigma[i];
+        da->Sigma[i] += quadgk_inf_inf(f, inner, 3, tol[0], tol[1]);
+    }
+}
+
+// compute_dΓ_1l!, Flow/compute.jl:57-112; returns node count
+long orc_compute_dgamma_1l(const orc_system *y, double L, const orc_action *a, orc_action *da, int eval,
+                           const double *tol, double *corrs) {
+    compute_corrs(*y, L, *a, da, true, corrs, tol[0], tol[1]);
+    long n = orc_run_pass(y, P_KAT, L, a, da, corrs, 0, eval, tol, da);
+    symmetrize(*y, dims_of(*y), *da);
+    return n;
+}
+
+// compute_dΓ_2l!, Flow/compute.jl:115-174
+long orc_compute_dgamma_2l(const orc_system *y, double L, const orc_action *a, orc_action *da, orc_action *dal, int eval,
+                           const double *tol, double *corrs) {
+    long n = orc_compute_dgamma_1l(y, L, a, da, eval, tol, corrs);
+    compute_corrs(*y, L, *a, nullptr, false, corrs, tol[0], tol[1]);
+    n += orc_run_pass(y, P_LEFT, L, a, da, corrs, 1, eval, tol, dal);
+    symmetrize_add_to(*y, dims_of(*y), *dal, *da);
+    return n;
+}
+
+// compute_dΓ_ml!, Flow/compute.jl:177-286
+long orc_compute_dgamma_ml(const orc_system *y, double L, int loops, const orc_action *a, orc_action *da,
+                           orc_action *dal, orc_action *dac, orc_action *datemp, orc_action *daS, int eval,
+                           const double *tol, double *corrs) {
+    Dims d = dims_of(*y);
+    const long G = gamma_size(d);
+    long n = orc_compute_dgamma_2l(y, L, a, da, dal, eval, tol, corrs);
+    for (long k = 0; k < G; ++k) { datemp->gamma[k] *= 0.0; daS->gamma[k] *= 0.0; }
+    symmetrize_add_to(*y, d, *dal, *datemp);
+    for (int loop = 3; loop <= loops; ++loop) {
+        n += orc_run_pass(y, P_CENTRAL, L, a, dal, corrs, 0, eval, tol, dac);
+        n += orc_run_pass(y, P_LEFT, L, a, datemp, corrs, 1, eval, tol, dal);
+        symmetrize(*y, d, *dac);
+        for (long k = 0; k < G; ++k) datemp->gamma[k] = dac->gamma[k];
+        symmetrize_add_to(*y, d, *dal, *datemp);
+        for (long k = 0; k < G; ++k) daS->gamma[k] += 1.0 * dac->gamma[k];
+        for (long k = 0; k < G; ++k) da->gamma[k] += 1.0 * datemp->gamma[k];
+    }
+    return n;
+}
+
+// compute_Σ!, Flow/SDE.jl:58-96 with compute_Σ_kernel (SDE_su2.jl:45-77)
+long orc_compute_sigma_sde(const orc_system *y, double L, const orc_action *a1, orc_action *a2, int eval,
+                           const double *gtol, const double *stol, double *corrs) {
+    Dims d = dims_of(*y);
+    compute_corrs(*y, L, *a1, nullptr, false, corrs, gtol[0], gtol[1]);
+    long n = orc_run_pass(y, P_REDUCED, L, a1, a1, corrs, 0, eval, gtol, a2);
+    double inner[3] = {-2.0 * L, 0.0, 2.0 * L};
+    const Mask SONLY{true, false, false};
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 1; i < y->n_sigma; ++i) {
+        const double w = y->sigma[i];
+        auto f = [&](double v) {
+            Buffer b1s = buffer_s(*y, v + w, INFINITY, 0.5 * (v - w));
+            Buffer b2s = buffer_s(*y, v + w, INFINITY, 0.5 * (-v + w));
+            Buffer e = buffer_empty();
+            double val = 3.0 * gamma_comp(*y, d, *a2, 0, 0, b2s, e, e, SONLY) + gamma_comp(*y, d, *a2, 1, 0, b2s, e, e, SONLY);
+            for (int j = 0; j < d.ns; ++j) val -= 2.0 * y->mult[j] * (2.0 * y->S) * gamma_comp(*y, d, *a2, 1, j, b1s, e, e, SONLY);
+            val *= -G_full(*y, L, v, a1->Sigma) / (2.0 * PI);
+            return val;
+        };
+        a2->Sigma[i] = quadgk_inf_inf(f, inner, 3, stol[0], stol[1]);
+    }
+    return n;
+}
+
+// compute_Γ!, Flow/compute.jl:2-54 (uses the corrs left by the preceding compute_Σ!)
+long orc_compute_gamma_bse(const orc_system *y, double L, const orc_action *a1, orc_action *a2, int eval,
+                           const double *tol, const double *corrs) {
+    long n = orc_run_pass(y, P_BSE, L, a1, a1, corrs, 0, eval, tol, a2);
+    symmetrize(*y, dims_of(*y), *a2);
+    return n;
+}
+
+}  // extern "C"

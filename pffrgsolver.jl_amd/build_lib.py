@@ -1,0 +1,57 @@
+"""Compile the gfx950 HIP library libpffrg.so in-tree (pffrg/_lib/).
+
+The vertex-pass kernel is instantiated once per pass type in csrc/pass_*.hip; the
+translation units compile in parallel and are linked into one shared library.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+SOURCES = ["pffrg.hip", "pass_kat.hip", "pass_left.hip", "pass_central.hip", "pass_bse.hip", "pass_reduced.hip"]
+OUT = PKG / "pffrg" / "_lib" / "libpffrg.so"
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    # keep hipcc from emitting packed FP32 instructions
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> Path:
+    OUT.parent.mkdir(parents=True, exist_ok=True)
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    inc = [f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+    with tempfile.TemporaryDirectory(prefix="pffrg_build_") as tmpd:
+        def compile_one(src: str) -> str:
+            obj = str(Path(tmpd) / (Path(src).stem + ".o"))
+            r = subprocess.run([hipcc, *HIPCC_FLAGS, *inc, "-c", "-o", obj, str(CSRC / src)],
+                               capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}{r.stderr}")
+            if verbose:
+                sys.stderr.write(r.stderr)
+            return obj
+
+        n = jobs or min(len(SOURCES), os.cpu_count() or 1, 8)
+        with ThreadPoolExecutor(max_workers=n) as ex:
+            objs = list(ex.map(compile_one, SOURCES))
+        tmp = OUT.with_suffix(".so.tmp")
+        r = subprocess.run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed:\n{r.stdout}{r.stderr}")
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,1055 @@
+// pffrg.hip -- gfx950 kernels and C ABI of the pf-FRG flow right-hand side.
+//
+// One launch per pass evaluates every output element (channel, kernel, Ω, ν, ν')
+// of the vertex flow: one 64-lane wavefront (= one workgroup) owns an element and
+// splits its lanes into groups of n_sites lanes; each group runs the reference's
+// adaptive Simpson + Richardson rule on one loop-frequency sub-interval at a time
+// (lane = irreducible site), so the sub-intervals of an element are integrated
+// concurrently and summed in the reference's order.  Meshes, Σ, the site exchange
+// map and the t-channel overlap rows are staged in LDS; the vertex is gathered
+// from a site-fastest structure-of-arrays buffer (FP64, or an FP32 copy in the
+// mixed mode), so every trilinear corner is one coalesced n_sites-wide segment.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "pffrg.h"
+#include "pffrg_device.h"
+#include "pffrg_pass.h"
+
+using namespace pffrg;
+
+namespace {
+
+// ======================================================================= tail corrections
+struct CorrArgs {
+    Sys y;
+    const double *mesh;
+    const double *sigA, *sigD;
+    double *corrs;
+    double L, atol, rtol;
+    int kat;
+};
+
+// compute_corrs_kat! / compute_corrs! (Flow/bubbles.jl:99-220): one workgroup per (Ω, channel, side)
+__global__ __launch_bounds__(64) void k_corrs(CorrArgs C) {
+    __shared__ double s_mesh[5 * MAXN];
+    __shared__ double s_sA[MAXN], s_sD[MAXN];
+    __shared__ Seg segs[GK_MAX_SEGS];
+    __shared__ double fv[64];
+    __shared__ int ctl[4];
+    __shared__ double ctld[2];
+    const Sys &y = C.y;
+    const int lane = threadIdx.x;
+    for (int k = lane; k < 5 * MAXN; k += 64) s_mesh[k] = C.mesh[k];
+    for (int k = lane; k < y.nsig; k += 64) {
+        s_sA[k] = C.sigA[k];
+        s_sD[k] = C.kat ? C.sigD[k] : 0.0;
+    }
+    __syncthreads();
+    const int side = blockIdx.x & 1;
+    const int ch = (blockIdx.x >> 1) % 3;
+    const int wi = (blockIdx.x >> 1) / 3;
+    const double W = (ch == 1) ? s_mesh[3 * MAXN + wi] : s_mesh[MAXN + wi];
+    const double val = (ch == 1) ? y.valT : y.valS;
+    const double *sm = s_mesh;
+    const int nsig = y.nsig;
+    const double L = C.L;
+    const int kat = C.kat;
+    auto f = [&](double v) -> double {
+        double x1, x2;
+        bubble_args(ch, W, v, x1, x2);
+        return kat ? bubble_kat(L, x1, x2, sm, nsig, s_sA, s_sD) : bubble_plain(L, x1, x2, sm, nsig, s_sA);
+    };
+    const double bnd = f(side ? val : -val);
+    double out = 0.0;
+    if (fabs(bnd) > 1e-8) {
+        const double bp[2] = {0.0, 1.0};
+        const double I = gk_block(f, side ? 1 : 2, side ? val : -val, bp, 2, C.atol, C.rtol, segs, fv, ctl, ctld);
+        out = I / bnd;
+    }
+    if (lane == 0) C.corrs[side + 2 * (ch + 3 * wi)] = out;
+}
+
+// ======================================================================= self energy
+enum SigmaKind { SK_DSIGMA = 0, SK_CORR1 = 1, SK_CORR2 = 2, SK_SDE = 3 };
+
+struct SigmaArgs {
+    Sys y;
+    const double *mesh;
+    const int *exch;
+    const double *mult;
+    ActV<double> A;   // flowing action (propagator; vertex for DSIGMA/CORR2)
+    ActV<double> V;   // vertex source: da_Σ (CORR1), a2 (SDE)
+    const double *sigV;  // Σ of da_Σ (CORR2)
+    double *out;      // Σ output
+    const double *add;   // CORR2: da_Σ.Σ added first
+    double L, atol, rtol;
+};
+
+// compute_dΣ! / compute_dΣ_corr! / compute_Σ! (Flow/flow_self.jl, Flow/SDE.jl:168-174):
+// one workgroup per σ point w_i, i >= 1; the integrand restates compute_dΣ_kernel(_corr1,
+// _corr2) (flow_su2_self.jl) and compute_Σ_kernel (SDE_su2.jl:45-77).
+template <int KIND>
+__global__ __launch_bounds__(64) void k_sigma(SigmaArgs Q) {
+    __shared__ double s_mesh[5 * MAXN];
+    __shared__ double s_sA[MAXN], s_sV[MAXN];
+    __shared__ int s_exch[64];
+    __shared__ double s_mult[64];
+    __shared__ Seg segs[GK_MAX_SEGS];
+    __shared__ double fv[64];
+    __shared__ int ctl[4];
+    __shared__ double ctld[2];
+    const Sys &y = Q.y;
+    const int lane = threadIdx.x;
+    for (int k = lane; k < 5 * MAXN; k += 64) s_mesh[k] = Q.mesh[k];
+    for (int k = lane; k < y.nsig; k += 64) {
+        s_sA[k] = Q.A.Sigma[k];
+        s_sV[k] = (KIND == SK_CORR2) ? Q.sigV[k] : 0.0;
+    }
+    if (lane < y.ns) {
+        s_exch[lane] = Q.exch[lane];
+        s_mult[lane] = Q.mult[lane];
+    }
+    __syncthreads();
+    const Meshes M{s_mesh, s_mesh + MAXN, s_mesh + 2 * MAXN, s_mesh + 3 * MAXN, s_mesh + 4 * MAXN};
+    const int si = blockIdx.x + 1;
+    const double w = s_mesh[si];
+    const double L = Q.L;
+    auto f = [&](double v) -> double {
+        double val;
+        if (KIND == SK_SDE) {
+            const double INF = __builtin_huge_val();
+            const Buf b1s = buf_s(M, y, v + w, INF, 0.5 * (v - w));
+            const Buf b2s = buf_s(M, y, v + w, INF, 0.5 * (-v + w));
+            const Buf e = buf_empty();
+            double s0, d0;
+            gamma_site<double, true, false, false>(Q.V, b2s, e, e, 0, s_exch, y, s0, d0);
+            val = 3.0 * s0 + d0;
+            for (int j = 0; j < y.ns; ++j) {
+                double sj, dj;
+                gamma_site<double, true, false, false>(Q.V, b1s, e, e, j, s_exch, y, sj, dj);
+                val -= 2.0 * s_mult[j] * y.S2 * dj;
+            }
+            val *= -G_full(L, v, s_mesh, y.nsig, s_sA) / (2.0 * PI_D);
+            return val;
+        }
+        const Buf b1s = buf_s(M, y, v + w, 0.5 * (-v + w), 0.5 * (v - w));
+        const Buf b1t = buf_t(M, y, 0.0, w, v);
+        const Buf b1u = buf_u(M, y, -v + w, 0.5 * (v + w), 0.5 * (v + w));
+        const Buf b2s = buf_s(M, y, v + w, 0.5 * (-v + w), 0.5 * (-v + w));
+        const Buf b2t = buf_t(M, y, v - w, 0.5 * (v + w), 0.5 * (v + w));
+        const Buf b2u = buf_u(M, y, 0.0, w, v);
+        double s0, d0;
+        if (KIND == SK_CORR1) gamma_site<double, true, true, false>(Q.V, b2s, b2t, b2u, 0, s_exch, y, s0, d0);
+        else gamma_site<double, true, true, true>(Q.A, b2s, b2t, b2u, 0, s_exch, y, s0, d0);
+        val = 3.0 * s0 + d0;
+        for (int j = 0; j < y.ns; ++j) {
+            double sj, dj;
+            if (KIND == SK_CORR1) gamma_site<double, true, false, true>(Q.V, b1s, b1t, b1u, j, s_exch, y, sj, dj);
+            else gamma_site<double, true, true, true>(Q.A, b1s, b1t, b1u, j, s_exch, y, sj, dj);
+            val -= 2.0 * s_mult[j] * y.S2 * dj;
+        }
+        if (KIND == SK_DSIGMA) {
+            val *= S_single(L, v, s_mesh, y.nsig, s_sA) / (2.0 * PI_D);
+        } else if (KIND == SK_CORR1) {
+            val *= -G_full(L, v, s_mesh, y.nsig, s_sA) / (2.0 * PI_D);
+        } else {
+            const double G = G_full(L, v, s_mesh, y.nsig, s_sA);
+            val *= G * G * sigma_at(v, s_mesh, y.nsig, s_sV) / (2.0 * PI_D);
+        }
+        return val;
+    };
+    // breakpoints (-Inf, -2Λ, 0, 2Λ, Inf) in the t/(1-t²) variable
+    double bp[5];
+    bp[0] = -1.0;
+    const double inner[3] = {-2.0 * L, 0.0, 2.0 * L};
+    for (int k = 0; k < 3; ++k) bp[k + 1] = 2.0 * inner[k] / (1.0 + hypot(1.0, 2.0 * inner[k]));
+    bp[4] = 1.0;
+    const double I = gk_block(f, 0, 0.0, bp, 5, Q.atol, Q.rtol, segs, fv, ctl, ctld);
+    if (lane == 0) {
+        if (KIND == SK_CORR2) {
+            double o = Q.out[si];
+            o += Q.add[si];
+            o += I;
+            Q.out[si] = o;
+        } else {
+            Q.out[si] = I;
+        }
+    }
+}
+
+// ======================================================================= elementwise action kernels
+// symmetrize! (action_su2.jl:141-173): q3[i,w,v,v'] = q3[x(i),w,v',v] for v < v'
+__global__ void k_symmetrize(double *g, Sys y, const int *exch) {
+    const long per = (long)y.ns * y.nO * y.nn * y.nn;
+    const long total = 6 * per;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        const int cc = (int)(t / per);  // comp*3 + ch
+        long r = t % per;
+        const int i = (int)(r % y.ns); r /= y.ns;
+        const int w = (int)(r % y.nO); r /= y.nO;
+        const int v = (int)(r % y.nn);
+        const int vp = (int)(r / y.nn);
+        if (v >= vp) continue;
+        const int ch = cc % 3;
+        double *q3 = g + (cc / 3) * y.complen + ch * y.chlen + y.q3;
+        const int src = (ch == 2) ? i : exch[i];
+        q3[i + (long)y.ns * (w + (long)y.nO * (v + (long)y.nn * vp))] =
+            q3[src + (long)y.ns * (w + (long)y.nO * (vp + (long)y.nn * v))];
+    }
+}
+
+// symmetrize_add_to! (action_su2.jl:176-250): a += a_l + image of a_l under ν <-> ν'
+__global__ void k_symmetrize_add(const double *__restrict__ l, double *__restrict__ a, Sys y, const int *exch) {
+    const long total = 2 * y.complen;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        const long cc = t / y.chlen;
+        const int ch = (int)(cc % 3);
+        long r = t % y.chlen;
+        const long cbase = t - r;
+        const int i = (int)(r % y.ns);
+        const int ix = (ch == 2) ? i : exch[i];
+        double img;
+        if (r < y.q21) {
+            img = l[cbase + ix + (r - i)];
+        } else if (r < y.q22) {
+            const long k = r - y.q21;
+            img = l[cbase + y.q22 + ix + (k - i)];
+        } else if (r < y.q3) {
+            const long k = r - y.q22;
+            img = l[cbase + y.q21 + ix + (k - i)];
+        } else {
+            long k = r - y.q3;
+            k /= y.ns;
+            const int w = (int)(k % y.nO); k /= y.nO;
+            const int v = (int)(k % y.nn);
+            const int vp = (int)(k / y.nn);
+            img = l[cbase + y.q3 + ix + (long)y.ns * (w + (long)y.nO * (vp + (long)y.nn * v))];
+        }
+        a[t] += l[t] + img;
+    }
+}
+
+__global__ void k_axpy(const double *__restrict__ x, double fac, double *__restrict__ y, long n) {
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        y[t] += fac * x[t];
+}
+
+__global__ void k_scale(double *x, double fac, long n) {
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) x[t] *= fac;
+}
+
+__global__ void k_to_f32(const double *__restrict__ x, float *__restrict__ y, long n) {
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        y[t] = (float)x[t];
+}
+
+// max |q3| over the six q3 blocks (channel.jl:406-413): block max then 64-bit atomic max
+// on the IEEE bit pattern, which orders like the value for non-negative doubles.
+__global__ void k_absmax_q3(const double *g, Sys y, unsigned long long *out) {
+    __shared__ double red[256];
+    const long per = (long)y.ns * y.nO * y.nn * y.nn;
+    double m = 0.0;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < 6 * per; t += (long)gridDim.x * blockDim.x) {
+        const long cc = t / per;
+        const double v = fabs(g[(cc / 3) * y.complen + (cc % 3) * y.chlen + y.q3 + (t % per)]);
+        m = (v > m || v != v) ? v : m;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const double o = red[threadIdx.x + s];
+            if (o > red[threadIdx.x] || o != o) red[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+}  // namespace
+
+// ======================================================================= host side
+struct Slot {
+    double *Sigma = nullptr, *bare = nullptr, *gamma = nullptr;
+    float *gamma32 = nullptr;
+    bool f32_valid = false;
+};
+
+struct pffrg_ctx {
+    pffrg_dims dims{};
+    Sys sys{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<Slot> slots;
+    double *d_mesh = nullptr;
+    bool have_mesh = false, have_lattice = false;
+    int *d_exch = nullptr, *d_ov_off = nullptr;
+    double *d_mult = nullptr;
+    uint32_t *d_ov_rows = nullptr;
+    int n_ov = 0;
+    double *d_corrs = nullptr;
+    unsigned long long *d_count = nullptr;
+    unsigned long long *d_max = nullptr;
+    uint32_t *d_el_half = nullptr, *d_el_full = nullptr, *d_el_red = nullptr;
+    int n_half = 0, n_full = 0, n_red = 0;
+    int precision = PFFRG_FP64;
+    long long stat_elems = 0, stat_launches = 0;
+    bool timing = false;
+    double time_ms = 0.0;
+    long long time_launches = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<double> h_mesh_sig;
+    // recording of per-sub-interval node counts of the next vertex pass (diagnostics)
+    int32_t *h_dbg_nodes = nullptr;
+    double *h_dbg_res = nullptr;
+    int *d_dbg_nodes = nullptr;
+    double *d_dbg_res = nullptr;
+    long dbg_cap = 0;
+};
+
+namespace {
+
+int fail(pffrg_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                          \
+    do {                                                                                         \
+        hipError_t _e = (expr);                                                                  \
+        if (_e != hipSuccess) return fail((c), PFFRG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+int check_slot(pffrg_ctx *c, int s) {
+    if (!c) return PFFRG_ERR_ARG;
+    if (s < 0 || s >= (int)c->slots.size()) return fail(c, PFFRG_ERR_ARG, "slot index out of range: " + std::to_string(s));
+    return 0;
+}
+
+int ready(pffrg_ctx *c) {
+    if (!c) return PFFRG_ERR_ARG;
+    if (!c->have_mesh) return fail(c, PFFRG_ERR_STATE, "meshes not set (pffrg_set_mesh)");
+    if (!c->have_lattice) return fail(c, PFFRG_ERR_STATE, "lattice not set (pffrg_set_lattice)");
+    return 0;
+}
+
+uint32_t enc(int ch, int kernel, int w1, int w2, int w3) {
+    return (uint32_t)ch | ((uint32_t)kernel << 2) | ((uint32_t)w1 << 5) | ((uint32_t)(w2 < 0 ? 0 : w2) << 12) |
+           ((uint32_t)(w3 < 0 ? 0 : w3) << 19);
+}
+
+// element lists of compute_Γ!/compute_dΓ_*! (compute.jl) and compute_Σ! (SDE.jl:156-166),
+// channel-major then Ω so that each XCD slice covers neighbouring bosonic frequencies
+std::vector<uint32_t> element_list(int nO, int nn, int mode) {
+    std::vector<uint32_t> e;
+    if (mode == 2) {
+        for (int w1 = 0; w1 < nO; ++w1) {
+            e.push_back(enc(0, 1, w1, -1, -1));
+            for (int w3 = 0; w3 < nn; ++w3) e.push_back(enc(0, 3, w1, -1, w3));
+        }
+        return e;
+    }
+    for (int ch = 0; ch < 3; ++ch)
+        for (int w1 = 0; w1 < nO; ++w1) {
+            e.push_back(enc(ch, 1, w1, -1, -1));
+            for (int w3 = 0; w3 < nn; ++w3) {
+                e.push_back(enc(ch, 2, w1, w3, -1));
+                e.push_back(enc(ch, 3, w1, -1, w3));
+            }
+            for (int w3 = 0; w3 < nn; ++w3)
+                for (int w2 = (mode == 1 ? 0 : w3); w2 < nn; ++w2) e.push_back(enc(ch, 4, w1, w2, w3));
+        }
+    return e;
+}
+
+int grid_for(long n) {
+    long b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+int ensure_f32(pffrg_ctx *c, int s) {
+    Slot &sl = c->slots[s];
+    const long n = 2 * c->sys.complen;
+    if (!sl.gamma32) HIPCHK(c, hipMalloc(&sl.gamma32, n * sizeof(float)));
+    if (!sl.f32_valid) {
+        hipLaunchKernelGGL(k_to_f32, dim3(grid_for(n)), dim3(256), 0, c->stream, sl.gamma, sl.gamma32, n);
+        HIPCHK(c, hipGetLastError());
+        sl.f32_valid = true;
+    }
+    return 0;
+}
+
+void invalidate(pffrg_ctx *c, int s) { c->slots[s].f32_valid = false; }
+
+template <int PASS, typename TV>
+int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const double *tol, int mode,
+                  const double *dbg_v = nullptr, double *dbg_out = nullptr, int dbg_n = 0, uint32_t dbg_elem = 0) {
+    PassArgs<TV> P;
+    P.dbg_v = dbg_v;
+    P.dbg_out = dbg_out;
+    P.dbg_elem = dbg_elem;
+    P.dbg_sub_nodes = c->d_dbg_nodes;
+    P.dbg_sub_res = c->d_dbg_res;
+    P.dbg_cap = c->dbg_cap;
+    P.y = c->sys;
+    P.mesh = c->d_mesh;
+    P.exch = c->d_exch;
+    P.ov_off = c->d_ov_off;
+    P.ov_rows = c->d_ov_rows;
+    const Slot &SA = c->slots[a], &SD = c->slots[d];
+    if (std::is_same<TV, float>::value) {
+        P.A = ActV<TV>{(const TV *)SA.gamma32, SA.bare, SA.Sigma};
+        P.D = ActV<TV>{(const TV *)SD.gamma32, SD.bare, SD.Sigma};
+    } else {
+        P.A = ActV<TV>{(const TV *)SA.gamma, SA.bare, SA.Sigma};
+        P.D = ActV<TV>{(const TV *)SD.gamma, SD.bare, SD.Sigma};
+    }
+    P.out = c->slots[out].gamma;
+    P.corrs = c->d_corrs;
+    P.elems = (mode == 0) ? c->d_el_half : (mode == 1) ? c->d_el_full : c->d_el_red;
+    P.n_elems = (mode == 0) ? c->n_half : (mode == 1) ? c->n_full : c->n_red;
+    P.per_xcd = (P.n_elems + 7) / 8;
+    P.L = L;
+    P.eval = eval;
+    P.atol = tol[0];
+    P.rtol = tol[1];
+    P.node_count = c->d_count;
+    const size_t lds = (size_t)(c->n_ov + c->sys.ns + 1) * sizeof(uint32_t);
+    const int grid = dbg_v ? dbg_n : 8 * P.per_xcd;
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, (launch_vertex_pass<PASS, TV>(P, grid, lds, c->stream)));
+    if (c->timing) {
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->time_ms += ms;
+        c->time_launches += 1;
+    }
+    c->stat_elems += P.n_elems;
+    c->stat_launches += 1;
+    invalidate(c, out);
+    if (c->d_dbg_nodes && !dbg_v) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(c->h_dbg_nodes, c->d_dbg_nodes, c->dbg_cap * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(c->h_dbg_res, c->d_dbg_res, c->dbg_cap * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipFree(c->d_dbg_nodes));
+        HIPCHK(c, hipFree(c->d_dbg_res));
+        c->d_dbg_nodes = nullptr;
+        c->d_dbg_res = nullptr;
+        c->dbg_cap = 0;
+    }
+    return 0;
+}
+
+template <int PASS>
+int launch_pass(pffrg_ctx *c, double L, int a, int d, int out, int eval, const double *tol, int mode) {
+    if (eval < 1) return fail(c, PFFRG_ERR_ARG, "eval must be >= 1");
+    if (c->precision == PFFRG_FP32_MIXED) {
+        int rc;
+        if ((rc = ensure_f32(c, a))) return rc;
+        if ((PASS == P_LEFT || PASS == P_CENTRAL) && (rc = ensure_f32(c, d))) return rc;
+        return launch_pass_t<PASS, float>(c, L, a, d, out, eval, tol, mode);
+    }
+    return launch_pass_t<PASS, double>(c, L, a, d, out, eval, tol, mode);
+}
+
+int launch_corrs(pffrg_ctx *c, double L, int a, int da, bool kat, const double *tol) {
+    CorrArgs C;
+    C.y = c->sys;
+    C.mesh = c->d_mesh;
+    C.sigA = c->slots[a].Sigma;
+    C.sigD = kat ? c->slots[da].Sigma : c->slots[a].Sigma;
+    C.corrs = c->d_corrs;
+    C.L = L;
+    C.atol = tol[0];
+    C.rtol = tol[1];
+    C.kat = kat ? 1 : 0;
+    hipLaunchKernelGGL(k_corrs, dim3(c->sys.nO * 3 * 2), dim3(64), 0, c->stream, C);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+template <int KIND>
+int launch_sigma(pffrg_ctx *c, double L, int a, int v, double *out, const double *sigV, const double *add,
+                 const double *tol) {
+    SigmaArgs Q;
+    Q.y = c->sys;
+    Q.mesh = c->d_mesh;
+    Q.exch = c->d_exch;
+    Q.mult = c->d_mult;
+    Q.A = ActV<double>{c->slots[a].gamma, c->slots[a].bare, c->slots[a].Sigma};
+    Q.V = ActV<double>{c->slots[v].gamma, c->slots[v].bare, c->slots[v].Sigma};
+    Q.sigV = sigV;
+    Q.out = out;
+    Q.add = add;
+    Q.L = L;
+    Q.atol = tol[0];
+    Q.rtol = tol[1];
+    if (c->sys.nsig > 1) {
+        hipLaunchKernelGGL((k_sigma<KIND>), dim3(c->sys.nsig - 1), dim3(64), 0, c->stream, Q);
+        HIPCHK(c, hipGetLastError());
+    }
+    return 0;
+}
+
+int do_symmetrize(pffrg_ctx *c, int s) {
+    const long n = 6L * c->sys.ns * c->sys.nO * c->sys.nn * c->sys.nn;
+    hipLaunchKernelGGL(k_symmetrize, dim3(grid_for(n)), dim3(256), 0, c->stream, c->slots[s].gamma, c->sys, c->d_exch);
+    HIPCHK(c, hipGetLastError());
+    invalidate(c, s);
+    return 0;
+}
+
+int do_symmetrize_add(pffrg_ctx *c, int l, int a) {
+    const long n = 2 * c->sys.complen;
+    hipLaunchKernelGGL(k_symmetrize_add, dim3(grid_for(n)), dim3(256), 0, c->stream, c->slots[l].gamma,
+                       c->slots[a].gamma, c->sys, c->d_exch);
+    HIPCHK(c, hipGetLastError());
+    invalidate(c, a);
+    return 0;
+}
+
+int do_axpy(pffrg_ctx *c, int src, double fac, int dst, bool vertex_only) {
+    const long n = 2 * c->sys.complen;
+    hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(256), 0, c->stream, c->slots[src].gamma, fac, c->slots[dst].gamma, n);
+    HIPCHK(c, hipGetLastError());
+    if (!vertex_only) {
+        hipLaunchKernelGGL(k_axpy, dim3(1), dim3(256), 0, c->stream, c->slots[src].Sigma, fac, c->slots[dst].Sigma,
+                           (long)c->sys.nsig);
+        HIPCHK(c, hipGetLastError());
+    }
+    invalidate(c, dst);
+    return 0;
+}
+
+int do_scale(pffrg_ctx *c, int s, double fac, bool vertex_only) {
+    const long n = 2 * c->sys.complen;
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(256), 0, c->stream, c->slots[s].gamma, fac, n);
+    HIPCHK(c, hipGetLastError());
+    if (!vertex_only) {
+        hipLaunchKernelGGL(k_scale, dim3(1), dim3(256), 0, c->stream, c->slots[s].Sigma, fac, (long)c->sys.nsig);
+        HIPCHK(c, hipGetLastError());
+    }
+    invalidate(c, s);
+    return 0;
+}
+
+int do_copy(pffrg_ctx *c, int dst, int src, bool vertex_only) {
+    HIPCHK(c, hipMemcpyAsync(c->slots[dst].gamma, c->slots[src].gamma, 2 * c->sys.complen * sizeof(double),
+                             hipMemcpyDeviceToDevice, c->stream));
+    if (!vertex_only)
+        HIPCHK(c, hipMemcpyAsync(c->slots[dst].Sigma, c->slots[src].Sigma, c->sys.nsig * sizeof(double),
+                                 hipMemcpyDeviceToDevice, c->stream));
+    invalidate(c, dst);
+    return 0;
+}
+
+}  // namespace
+
+// ======================================================================= C ABI
+extern "C" {
+
+int pffrg_create(const pffrg_dims *d, pffrg_ctx **out) {
+    if (!d || !out) return PFFRG_ERR_ARG;
+    *out = nullptr;
+    if (d->n_sites < 1 || d->n_sites > 64 || d->n_sigma < 2 || d->n_sigma > MAXN || d->n_Omega < 2 ||
+        d->n_Omega > MAXN || d->n_nu < 2 || d->n_nu > MAXN || d->n_slots < 1 || d->S <= 0.0)
+        return PFFRG_ERR_LIMIT;
+    pffrg_ctx *c = new pffrg_ctx();
+    c->dims = *d;
+    c->device = d->device;
+    if (hipSetDevice(d->device) != hipSuccess) { delete c; return PFFRG_ERR_HIP; }
+    Sys &y = c->sys;
+    y.ns = d->n_sites;
+    y.nO = d->n_Omega;
+    y.nn = d->n_nu;
+    y.nsig = d->n_sigma;
+    y.GS = y.ns;
+    y.G = (y.ns <= 32) ? WAVE / y.ns : 1;
+    y.S2 = 2.0 * d->S;
+    y.q21 = (long)y.ns * y.nO;
+    y.q22 = y.q21 + (long)y.ns * y.nO * y.nn;
+    y.q3 = y.q22 + (long)y.ns * y.nO * y.nn;
+    y.chlen = y.q3 + (long)y.ns * y.nO * y.nn * y.nn;
+    y.complen = 3 * y.chlen;
+    bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_mesh, 5 * MAXN * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_corrs, 2 * 3 * y.nO * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_count, 2 * sizeof(unsigned long long)) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_max, sizeof(unsigned long long)) == hipSuccess;
+    ok = ok && hipMemset(c->d_corrs, 0, 2 * 3 * y.nO * sizeof(double)) == hipSuccess;
+    ok = ok && hipMemset(c->d_count, 0, 2 * sizeof(unsigned long long)) == hipSuccess;
+    ok = ok && hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
+    c->slots.resize(d->n_slots);
+    for (auto &s : c->slots) {
+        ok = ok && hipMalloc(&s.Sigma, y.nsig * sizeof(double)) == hipSuccess;
+        ok = ok && hipMalloc(&s.bare, 2 * y.ns * sizeof(double)) == hipSuccess;
+        ok = ok && hipMalloc(&s.gamma, 2 * y.complen * sizeof(double)) == hipSuccess;
+        ok = ok && hipMemset(s.Sigma, 0, y.nsig * sizeof(double)) == hipSuccess;
+        ok = ok && hipMemset(s.bare, 0, 2 * y.ns * sizeof(double)) == hipSuccess;
+        ok = ok && hipMemset(s.gamma, 0, 2 * y.complen * sizeof(double)) == hipSuccess;
+    }
+    auto upload_list = [&](uint32_t **dst, int *n, int mode) {
+        std::vector<uint32_t> e = element_list(y.nO, y.nn, mode);
+        *n = (int)e.size();
+        ok = ok && hipMalloc(dst, e.size() * sizeof(uint32_t)) == hipSuccess;
+        ok = ok && hipMemcpy(*dst, e.data(), e.size() * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess;
+    };
+    upload_list(&c->d_el_half, &c->n_half, 0);
+    upload_list(&c->d_el_full, &c->n_full, 1);
+    upload_list(&c->d_el_red, &c->n_red, 2);
+    if (!ok) {
+        pffrg_destroy(c);
+        return PFFRG_ERR_HIP;
+    }
+    *out = c;
+    return 0;
+}
+
+void pffrg_destroy(pffrg_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &s : c->slots) {
+        (void)hipFree(s.Sigma);
+        (void)hipFree(s.bare);
+        (void)hipFree(s.gamma);
+        (void)hipFree(s.gamma32);
+    }
+    (void)hipFree(c->d_mesh);
+    (void)hipFree(c->d_exch);
+    (void)hipFree(c->d_ov_off);
+    (void)hipFree(c->d_mult);
+    (void)hipFree(c->d_ov_rows);
+    (void)hipFree(c->d_corrs);
+    (void)hipFree(c->d_count);
+    (void)hipFree(c->d_max);
+    (void)hipFree(c->d_el_half);
+    (void)hipFree(c->d_el_full);
+    (void)hipFree(c->d_el_red);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *pffrg_last_error(const pffrg_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int64_t pffrg_gamma_length(const pffrg_ctx *c) { return c ? 2 * c->sys.complen : -1; }
+
+int pffrg_set_mesh(pffrg_ctx *c, const double *sigma, const double *Os, const double *ns, const double *Ot,
+                   const double *nt) {
+    if (!c || !sigma || !Os || !ns || !Ot || !nt) return fail(c, PFFRG_ERR_ARG, "null mesh pointer");
+    HIPCHK(c, hipSetDevice(c->device));
+    Sys &y = c->sys;
+    std::vector<double> h(5 * MAXN, 0.0);
+    auto put = [&](int k, const double *m, int n) {
+        for (int j = 0; j < n; ++j) h[k * MAXN + j] = m[j];
+    };
+    put(0, sigma, y.nsig);
+    put(1, Os, y.nO);
+    put(2, ns, y.nn);
+    put(3, Ot, y.nO);
+    put(4, nt, y.nn);
+    for (int k = 0; k < 5; ++k) {
+        const int n = (k == 0) ? y.nsig : ((k == 1 || k == 3) ? y.nO : y.nn);
+        if (h[k * MAXN] != 0.0) return fail(c, PFFRG_ERR_ARG, "meshes must start at 0");
+        for (int j = 1; j < n; ++j)
+            if (!(h[k * MAXN + j] > h[k * MAXN + j - 1])) return fail(c, PFFRG_ERR_ARG, "meshes must be strictly increasing");
+    }
+    y.valS = Os[y.nO - 1] + ns[y.nn - 1];
+    y.valT = Ot[y.nO - 1] + nt[y.nn - 1];
+    HIPCHK(c, hipMemcpy(c->d_mesh, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->have_mesh = true;
+    return 0;
+}
+
+int pffrg_set_lattice(pffrg_ctx *c, const int64_t *exchange, const double *mult, const int64_t *ov_off,
+                      const int64_t *ov_rows, int32_t base) {
+    if (!c || !exchange || !mult || !ov_off || !ov_rows) return fail(c, PFFRG_ERR_ARG, "null lattice pointer");
+    if (base != 0 && base != 1) return fail(c, PFFRG_ERR_ARG, "index_base must be 0 or 1");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int ns = c->sys.ns;
+    std::vector<int> ex(ns), off(ns + 1);
+    for (int i = 0; i < ns; ++i) {
+        ex[i] = (int)(exchange[i] - base);
+        if (ex[i] < 0 || ex[i] >= ns) return fail(c, PFFRG_ERR_ARG, "exchange index out of range");
+    }
+    if (ov_off[0] != 0) return fail(c, PFFRG_ERR_ARG, "ov_off[0] must be 0");
+    for (int i = 0; i <= ns; ++i) off[i] = (int)ov_off[i];
+    const int R = off[ns];
+    for (int i = 0; i < ns; ++i)
+        if (off[i + 1] < off[i]) return fail(c, PFFRG_ERR_ARG, "ov_off must be non-decreasing");
+    if ((size_t)(R + ns + 1) * 4 > 48 * 1024) return fail(c, PFFRG_ERR_LIMIT, "overlap table exceeds the LDS budget");
+    std::vector<uint32_t> rows(R > 0 ? R : 1);
+    for (int r = 0; r < R; ++r) {
+        const long j1 = ov_rows[3 * r] - base, j2 = ov_rows[3 * r + 1] - base, m = ov_rows[3 * r + 2];
+        if (j1 < 0 || j1 >= ns || j2 < 0 || j2 >= ns || m < 0 || m > 65535)
+            return fail(c, PFFRG_ERR_ARG, "overlap row out of range");
+        rows[r] = (uint32_t)j1 | ((uint32_t)j2 << 8) | ((uint32_t)m << 16);
+    }
+    (void)hipFree(c->d_exch);
+    (void)hipFree(c->d_ov_off);
+    (void)hipFree(c->d_ov_rows);
+    (void)hipFree(c->d_mult);
+    c->d_exch = nullptr; c->d_ov_off = nullptr; c->d_ov_rows = nullptr; c->d_mult = nullptr;
+    HIPCHK(c, hipMalloc(&c->d_exch, ns * sizeof(int)));
+    HIPCHK(c, hipMalloc(&c->d_ov_off, (ns + 1) * sizeof(int)));
+    HIPCHK(c, hipMalloc(&c->d_ov_rows, rows.size() * sizeof(uint32_t)));
+    HIPCHK(c, hipMalloc(&c->d_mult, ns * sizeof(double)));
+    HIPCHK(c, hipMemcpy(c->d_exch, ex.data(), ns * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_ov_off, off.data(), (ns + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_ov_rows, rows.data(), rows.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_mult, mult, ns * sizeof(double), hipMemcpyHostToDevice));
+    c->n_ov = R;
+    c->have_lattice = true;
+    return 0;
+}
+
+int pffrg_upload_action(pffrg_ctx *c, int32_t s, const double *Sigma, const double *bare, const double *gamma) {
+    int rc = check_slot(c, s);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    Slot &sl = c->slots[s];
+    if (Sigma) HIPCHK(c, hipMemcpy(sl.Sigma, Sigma, c->sys.nsig * sizeof(double), hipMemcpyHostToDevice));
+    if (bare) HIPCHK(c, hipMemcpy(sl.bare, bare, 2 * c->sys.ns * sizeof(double), hipMemcpyHostToDevice));
+    if (gamma) HIPCHK(c, hipMemcpy(sl.gamma, gamma, 2 * c->sys.complen * sizeof(double), hipMemcpyHostToDevice));
+    invalidate(c, s);
+    return 0;
+}
+
+int pffrg_download_action(pffrg_ctx *c, int32_t s, double *Sigma, double *bare, double *gamma) {
+    int rc = check_slot(c, s);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    Slot &sl = c->slots[s];
+    if (Sigma) HIPCHK(c, hipMemcpy(Sigma, sl.Sigma, c->sys.nsig * sizeof(double), hipMemcpyDeviceToHost));
+    if (bare) HIPCHK(c, hipMemcpy(bare, sl.bare, 2 * c->sys.ns * sizeof(double), hipMemcpyDeviceToHost));
+    if (gamma) HIPCHK(c, hipMemcpy(gamma, sl.gamma, 2 * c->sys.complen * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pffrg_set_precision(pffrg_ctx *c, int32_t mode) {
+    if (!c) return PFFRG_ERR_ARG;
+    if (mode != PFFRG_FP64 && mode != PFFRG_FP32_MIXED) return fail(c, PFFRG_ERR_ARG, "unknown precision mode");
+    c->precision = mode;
+    return 0;
+}
+
+int pffrg_compute_dsigma(pffrg_ctx *c, double L, int32_t a, int32_t da, const double *tol) {
+    int rc;
+    if ((rc = ready(c)) || (rc = check_slot(c, a)) || (rc = check_slot(c, da))) return rc;
+    if (!tol || !(L > 0.0)) return fail(c, PFFRG_ERR_ARG, "invalid Lambda or tolerance");
+    HIPCHK(c, hipSetDevice(c->device));
+    rc = launch_sigma<SK_DSIGMA>(c, L, a, a, c->slots[da].Sigma, nullptr, nullptr, tol);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pffrg_compute_dsigma_corr(pffrg_ctx *c, double L, int32_t a, int32_t da, int32_t daS, const double *tol) {
+    int rc;
+    if ((rc = ready(c)) || (rc = check_slot(c, a)) || (rc = check_slot(c, da)) || (rc = check_slot(c, daS))) return rc;
+    if (!tol || !(L > 0.0)) return fail(c, PFFRG_ERR_ARG, "invalid Lambda or tolerance");
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((rc = launch_sigma<SK_CORR1>(c, L, a, daS, c->slots[daS].Sigma, nullptr, nullptr, tol))) return rc;
+    if ((rc = launch_sigma<SK_CORR2>(c, L, a, a, c->slots[da].Sigma, c->slots[daS].Sigma, c->slots[daS].Sigma, tol)))
+        return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int dgamma_1l(pffrg_ctx *c, double L, int a, int da, int eval, const double *tol) {
+    int rc;
+    if ((rc = launch_corrs(c, L, a, da, true, tol))) return rc;
+    if ((rc = launch_pass<P_KAT>(c, L, a, da, da, eval, tol, 0))) return rc;
+    return do_symmetrize(c, da);
+}
+
+static int dgamma_2l(pffrg_ctx *c, double L, int a, int da, int dal, int eval, const double *tol) {
+    int rc;
+    if ((rc = dgamma_1l(c, L, a, da, eval, tol))) return rc;
+    if ((rc = launch_corrs(c, L, a, a, false, tol))) return rc;
+    if ((rc = launch_pass<P_LEFT>(c, L, a, da, dal, eval, tol, 1))) return rc;
+    return do_symmetrize_add(c, dal, da);
+}
+
+int pffrg_compute_dgamma_1l(pffrg_ctx *c, double L, int32_t a, int32_t da, int32_t eval, const double *tol) {
+    int rc;
+    if ((rc = ready(c)) || (rc = check_slot(c, a)) || (rc = check_slot(c, da))) return rc;
+    if (!tol || !(L > 0.0) || a == da) return fail(c, PFFRG_ERR_ARG, "invalid arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((rc = dgamma_1l(c, L, a, da, eval, tol))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pffrg_compute_dgamma_2l(pffrg_ctx *c, double L, int32_t a, int32_t da, int32_t dal, int32_t eval,
+                            const double *tol) {
+    int rc;
+    if ((rc = ready(c)) || (rc = check_slot(c, a)) || (rc = check_slot(c, da)) || (rc = check_slot(c, dal))) return rc;
+    if (!tol || !(L > 0.0) || a == da || a == dal || da == dal) return fail(c, PFFRG_ERR_ARG, "invalid arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((rc = dgamma_2l(c, L, a, da, dal, eval, tol))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pffrg_compute_dgamma_ml(pffrg_ctx *c, double L, int32_t loops, int32_t a, int32_t da, int32_t dal, int32_t dac,
+                            int32_t datemp, int32_t daS, int32_t eval, const double *tol) {
+    int rc;
+    if ((rc = ready(c))) return rc;
+    const int sl[6] = {a, da, dal, dac, datemp, daS};
+    for (int k = 0; k < 6; ++k) {
+        if ((rc = check_slot(c, sl[k]))) return rc;
+        for (int j = 0; j < k; ++j)
+            if (sl[j] == sl[k]) return fail(c, PFFRG_ERR_ARG, "multiloop slots must be distinct");
+    }
+    if (!tol || !(L > 0.0) || loops < 1) return fail(c, PFFRG_ERR_ARG, "invalid arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((rc = dgamma_2l(c, L, a, da, dal, eval, tol))) return rc;
+    if ((rc = do_scale(c, datemp, 0.0, true))) return rc;
+    if ((rc = do_scale(c, daS, 0.0, true))) return rc;
+    if ((rc = do_symmetrize_add(c, dal, datemp))) return rc;
+    for (int loop = 3; loop <= loops; ++loop) {
+        if ((rc = launch_pass<P_CENTRAL>(c, L, a, dal, dac, eval, tol, 0))) return rc;
+        if ((rc = launch_pass<P_LEFT>(c, L, a, datemp, dal, eval, tol, 1))) return rc;
+        if ((rc = do_symmetrize(c, dac))) return rc;
+        if ((rc = do_copy(c, datemp, dac, true))) return rc;
+        if ((rc = do_symmetrize_add(c, dal, datemp))) return rc;
+        if ((rc = do_axpy(c, dac, 1.0, daS, true))) return rc;
+        if ((rc = do_axpy(c, datemp, 1.0, da, true))) return rc;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pffrg_compute_sigma(pffrg_ctx *c, double L, int32_t a1, int32_t a2, int32_t eval, const double *gtol,
+                        const double *stol) {
+    int rc;
+    if ((rc = ready(c)) || (rc = check_slot(c, a1)) || (rc = check_slot(c, a2))) return rc;
+    if (!gtol || !stol || !(L > 0.0) || a1 == a2) return fail(c, PFFRG_ERR_ARG, "invalid arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((rc = launch_corrs(c, L, a1, a1, false, gtol))) return rc;
+    if ((rc = launch_pass<P_REDUCED>(c, L, a1, a1, a2, eval, gtol, 2))) return rc;
+    if ((rc = launch_sigma<SK_SDE>(c, L, a1, a2, c->slots[a2].Sigma, nullptr, nullptr, stol))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pffrg_compute_gamma(pffrg_ctx *c, double L, int32_t a1, int32_t a2, int32_t eval, const double *tol) {
+    int rc;
+    if ((rc = ready(c)) || (rc = check_slot(c, a1)) || (rc = check_slot(c, a2))) return rc;
+    if (!tol || !(L > 0.0) || a1 == a2) return fail(c, PFFRG_ERR_ARG, "invalid arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((rc = launch_pass<P_BSE>(c, L, a1, a1, a2, eval, tol, 0))) return rc;
+    if ((rc = do_symmetrize(c, a2))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pffrg_symmetrize(pffrg_ctx *c, int32_t s) {
+    int rc;
+    if ((rc = ready(c)) || (rc = check_slot(c, s))) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((rc = do_symmetrize(c, s))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pffrg_symmetrize_add_to(pffrg_ctx *c, int32_t l, int32_t a) {
+    int rc;
+    if ((rc = ready(c)) || (rc = check_slot(c, l)) || (rc = check_slot(c, a))) return rc;
+    if (l == a) return fail(c, PFFRG_ERR_ARG, "source and destination must differ");
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((rc = do_symmetrize_add(c, l, a))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pffrg_replace_with(pffrg_ctx *c, int32_t dst, int32_t src, int32_t vertex_only) {
+    int rc;
+    if ((rc = check_slot(c, dst)) || (rc = check_slot(c, src))) return rc;
+    if (dst == src) return 0;
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((rc = do_copy(c, dst, src, vertex_only != 0))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pffrg_mult_with(pffrg_ctx *c, int32_t s, double fac, int32_t vertex_only) {
+    int rc;
+    if ((rc = check_slot(c, s))) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((rc = do_scale(c, s, fac, vertex_only != 0))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pffrg_mult_with_add_to(pffrg_ctx *c, int32_t src, double fac, int32_t dst, int32_t vertex_only) {
+    int rc;
+    if ((rc = check_slot(c, src)) || (rc = check_slot(c, dst))) return rc;
+    if (src == dst) return fail(c, PFFRG_ERR_ARG, "source and destination must differ");
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((rc = do_axpy(c, src, fac, dst, vertex_only != 0))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pffrg_abs_max(pffrg_ctx *c, int32_t s, double *out) {
+    int rc;
+    if ((rc = check_slot(c, s))) return rc;
+    if (!out) return fail(c, PFFRG_ERR_ARG, "null output");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemsetAsync(c->d_max, 0, sizeof(unsigned long long), c->stream));
+    const long n = 6L * c->sys.ns * c->sys.nO * c->sys.nn * c->sys.nn;
+    hipLaunchKernelGGL(k_absmax_q3, dim3(grid_for(n)), dim3(256), 0, c->stream, c->slots[s].gamma, c->sys, c->d_max);
+    HIPCHK(c, hipGetLastError());
+    unsigned long long bits = 0;
+    HIPCHK(c, hipMemcpyAsync(&bits, c->d_max, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    *out = v;
+    return 0;
+}
+
+int pffrg_get_sigma(pffrg_ctx *c, int32_t s, double *Sigma) {
+    int rc;
+    if ((rc = check_slot(c, s))) return rc;
+    if (!Sigma) return fail(c, PFFRG_ERR_ARG, "null output");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(Sigma, c->slots[s].Sigma, c->sys.nsig * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pffrg_set_sigma(pffrg_ctx *c, int32_t s, const double *Sigma) {
+    int rc;
+    if ((rc = check_slot(c, s))) return rc;
+    if (!Sigma) return fail(c, PFFRG_ERR_ARG, "null input");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->slots[s].Sigma, Sigma, c->sys.nsig * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int pffrg_synchronize(pffrg_ctx *c) {
+    if (!c) return PFFRG_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pffrg_reset_stats(pffrg_ctx *c) {
+    if (!c) return PFFRG_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemset(c->d_count, 0, 2 * sizeof(unsigned long long)));
+    c->stat_elems = 0;
+    c->stat_launches = 0;
+    c->time_ms = 0.0;
+    c->time_launches = 0;
+    return 0;
+}
+
+int pffrg_get_stats(pffrg_ctx *c, int64_t *stats) {
+    if (!c || !stats) return PFFRG_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unsigned long long n[2] = {0, 0};
+    HIPCHK(c, hipMemcpy(n, c->d_count, sizeof(n), hipMemcpyDeviceToHost));
+    stats[0] = (int64_t)n[0];
+    stats[1] = c->stat_elems;
+    stats[2] = c->stat_launches;
+    stats[3] = (int64_t)n[1];
+    return 0;
+}
+
+int pffrg_get_corrs(pffrg_ctx *c, double *out) {
+    if (!c || !out) return PFFRG_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, c->d_corrs, 2 * 3 * c->sys.nO * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pffrg_probe_integrand(pffrg_ctx *c, int32_t pass, double L, int32_t a, int32_t da, int32_t ch, int32_t kernel,
+                          int32_t w1, int32_t w2, int32_t w3, int32_t n, const double *v, double *out) {
+    int rc;
+    if ((rc = ready(c)) || (rc = check_slot(c, a)) || (rc = check_slot(c, da))) return rc;
+    if (n < 1 || !v || !out || pass < 0 || pass > 4 || ch < 0 || ch > 2 || kernel < 1 || kernel > 4)
+        return fail(c, PFFRG_ERR_ARG, "invalid probe arguments");
+    if (w1 < 0 || w1 >= c->sys.nO || w2 >= c->sys.nn || w3 >= c->sys.nn) return fail(c, PFFRG_ERR_ARG, "index");
+    HIPCHK(c, hipSetDevice(c->device));
+    double *dv = nullptr, *dout = nullptr;
+    HIPCHK(c, hipMalloc(&dv, n * sizeof(double)));
+    HIPCHK(c, hipMalloc(&dout, (size_t)n * 2 * c->sys.ns * sizeof(double)));
+    HIPCHK(c, hipMemcpy(dv, v, n * sizeof(double), hipMemcpyHostToDevice));
+    const uint32_t e = enc(ch, kernel, w1, w2, w3);
+    const double tol[2] = {1e-8, 1e-4};
+    const bool f32 = c->precision == PFFRG_FP32_MIXED;
+    if (f32) {
+        if ((rc = ensure_f32(c, a)) || (rc = ensure_f32(c, da))) return rc;
+    }
+#define PROBE(PS)                                                                                                  \
+    rc = f32 ? launch_pass_t<PS, float>(c, L, a, da, a, 1, tol, 0, dv, dout, n, e)                               \
+             : launch_pass_t<PS, double>(c, L, a, da, a, 1, tol, 0, dv, dout, n, e);
+    switch (pass) {
+        case P_KAT: PROBE(P_KAT) break;
+        case P_LEFT: PROBE(P_LEFT) break;
+        case P_CENTRAL: PROBE(P_CENTRAL) break;
+        case P_BSE: PROBE(P_BSE) break;
+        default: PROBE(P_REDUCED) break;
+    }
+#undef PROBE
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, dout, (size_t)n * 2 * c->sys.ns * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipFree(dv));
+    HIPCHK(c, hipFree(dout));
+    return 0;
+}
+
+int pffrg_debug_record(pffrg_ctx *c, int32_t *sub_nodes, double *sub_res, int64_t capacity) {
+    if (!c || !sub_nodes || !sub_res || capacity < 1) return PFFRG_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->d_dbg_nodes) return fail(c, PFFRG_ERR_STATE, "a recording is already armed");
+    HIPCHK(c, hipMalloc(&c->d_dbg_nodes, capacity * sizeof(int)));
+    HIPCHK(c, hipMalloc(&c->d_dbg_res, capacity * sizeof(double)));
+    HIPCHK(c, hipMemset(c->d_dbg_nodes, 0, capacity * sizeof(int)));
+    HIPCHK(c, hipMemset(c->d_dbg_res, 0, capacity * sizeof(double)));
+    c->h_dbg_nodes = sub_nodes;
+    c->h_dbg_res = sub_res;
+    c->dbg_cap = capacity;
+    return 0;
+}
+
+int pffrg_set_timing(pffrg_ctx *c, int32_t enable) {
+    if (!c) return PFFRG_ERR_ARG;
+    c->timing = enable != 0;
+    return 0;
+}
+
+int pffrg_get_timing(pffrg_ctx *c, double *total_ms, int64_t *launches) {
+    if (!c || !total_ms || !launches) return PFFRG_ERR_ARG;
+    *total_ms = c->time_ms;
+    *launches = c->time_launches;
+    return 0;
+}
+
+}  // extern "C"

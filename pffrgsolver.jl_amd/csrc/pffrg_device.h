@@ -1,0 +1,437 @@
+// pffrg_device.h -- device-side building blocks of the gfx950 flow right-hand side:
+// mesh lookup, frequency folding, vertex interpolation, propagators and the
+// Gauss-Kronrod block integrator.  Included by pffrg.hip only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pffrg {
+
+constexpr int MAXN = 64;      // mesh points per axis held in LDS
+constexpr int WAVE = 64;      // gfx950 wavefront
+constexpr double PI_D = 3.14159265358979323846;
+
+enum PassType { P_KAT = 0, P_LEFT = 1, P_CENTRAL = 2, P_BSE = 3, P_REDUCED = 4 };
+
+// Channel/array offsets of the packed vertex (site fastest, column-major).
+struct Sys {
+    int ns, nO, nn, nsig;
+    int G, GS;            // site groups per wavefront and lanes per group
+    double S2;            // 2S
+    long q21, q22, q3;    // offsets of q2_1, q2_2, q3 inside a channel block
+    long chlen, complen;  // channel block length, component block length
+    double valS, valT;    // Ω_max + ν_max of the s/u and t meshes
+};
+
+// Mesh views; in the pass kernels they point into LDS.
+struct Meshes {
+    const double *sig, *Os, *ns, *Ot, *nt;
+};
+
+struct Param {
+    int lo, hi;
+    double wl, wh;
+};
+
+// get_indices/get_param semantics (Frequency/param.jl:26-86) with a bisection over the
+// LDS-resident mesh instead of the reference's linear scan: upper = first index with
+// mesh[idx] >= val, lower = upper unless val < mesh[upper]; clamp to the last point.
+__device__ __forceinline__ Param find_param(double val, const double *m, int n) {
+    Param p;
+    if (val >= m[n - 1]) {
+        p.lo = p.hi = n - 1;
+        p.wl = 1.0;
+        p.wh = 0.0;
+        return p;
+    }
+    int l = -1, h = n - 1;
+    while (h - l > 1) {
+        int mid = (l + h) >> 1;
+        if (m[mid] >= val) h = mid;
+        else l = mid;
+    }
+    p.hi = h;
+    p.lo = (val < m[h]) ? h - 1 : h;
+    if (p.lo < p.hi) {
+        double d = 1.0 / (m[p.hi] - m[p.lo]);
+        p.wl = d * (m[p.hi] - val);
+        p.wh = d * (val - m[p.lo]);
+    } else {
+        p.wl = 1.0;
+        p.wh = 0.0;
+    }
+    return p;
+}
+
+// Access buffer of Frequency/buffer.jl:16-26: kernel 0 = nothing, 1 = q1, 2 = q2_1, 3 = q2_2, 4 = q3.
+struct Buf {
+    int kernel;
+    bool exch, map, smu, snu;
+    Param p1, p2, p3;
+};
+
+__device__ __forceinline__ Buf buf_empty() {
+    Buf b;
+    b.kernel = 0;
+    b.exch = b.map = b.smu = b.snu = false;
+    b.p1 = b.p2 = b.p3 = Param{0, 0, 0.0, 0.0};
+    return b;
+}
+
+// get_buffer, Frequency/buffer.jl:51-86
+__device__ __forceinline__ Buf make_buf(double w, double v, double vp, const double *Om, const double *nu, int nO,
+                                        int nn, bool ex, bool mp, bool smu, bool snu) {
+    Buf b = buf_empty();
+    if (Om[nO - 1] < fabs(w)) return b;
+    b.exch = ex;
+    b.map = mp;
+    b.smu = smu;
+    b.snu = snu;
+    const bool av = nu[nn - 1] < fabs(v);
+    const bool avp = nu[nn - 1] < fabs(vp);
+    b.p1 = find_param(w, Om, nO);
+    if (av) {
+        if (avp) {
+            b.kernel = 1;
+        } else {
+            b.kernel = 3;
+            b.p3 = find_param(vp, nu, nn);
+        }
+    } else {
+        b.p2 = find_param(v, nu, nn);
+        if (avp) {
+            b.kernel = 2;
+        } else {
+            b.kernel = 4;
+            b.p3 = find_param(vp, nu, nn);
+        }
+    }
+    return b;
+}
+
+// get_buffer_s (buffer.jl:88-124)
+__device__ __forceinline__ Buf buf_s(const Meshes &M, const Sys &y, double w, double v, double vp) {
+    bool ex = false, mp = false, smu = false, snu = false;
+    if (w < 0.0) { w = -w; ex = !ex; }
+    if (v < 0.0) { v = -v; ex = !ex; mp = !mp; smu = !smu; }
+    if (vp < 0.0) { vp = -vp; mp = !mp; snu = !snu; }
+    return make_buf(w, v, vp, M.Os, M.ns, y.nO, y.nn, ex, mp, smu, snu);
+}
+
+// get_buffer_t (buffer.jl:127-160)
+__device__ __forceinline__ Buf buf_t(const Meshes &M, const Sys &y, double w, double v, double vp) {
+    bool smu = false, snu = false;
+    if (w < 0.0) { w = -w; }
+    if (v < 0.0) { v = -v; smu = !smu; }
+    if (vp < 0.0) { vp = -vp; snu = !snu; }
+    return make_buf(w, v, vp, M.Ot, M.nt, y.nO, y.nn, false, false, smu, snu);
+}
+
+// get_buffer_u (buffer.jl:163-199)
+__device__ __forceinline__ Buf buf_u(const Meshes &M, const Sys &y, double w, double v, double vp) {
+    bool ex = false, mp = false, snu = false;
+    if (w < 0.0) { w = -w; ex = !ex; }
+    if (v < 0.0) { v = -v; ex = !ex; mp = !mp; snu = !snu; }
+    if (vp < 0.0) { vp = -vp; mp = !mp; snu = !snu; }
+    return make_buf(w, v, vp, M.Os, M.ns, y.nO, y.nn, ex, mp, false, snu);
+}
+
+// Interpolate one channel at one site for both SU(2) components (channel.jl:42-305):
+// c0/c1 point at the channel block of component 0/1.  Corner order and weight
+// products follow the reference so FP64 results agree to rounding.
+template <typename TV>
+__device__ __forceinline__ void interp2(const TV *__restrict__ c0, const TV *__restrict__ c1, const Buf &b, int site,
+                                        const Sys &y, TV &r0, TV &r1) {
+    const long ns = y.ns, nO = y.nO, nn = y.nn;
+    if (b.kernel == 1) {
+        const TV wl = (TV)b.p1.wl, wh = (TV)b.p1.wh;
+        const long o0 = site + ns * b.p1.lo, o1 = site + ns * b.p1.hi;
+        r0 = wl * c0[o0];
+        r0 += wh * c0[o1];
+        r1 = wl * c1[o0];
+        r1 += wh * c1[o1];
+    } else if (b.kernel == 4) {
+        const TV *q0 = c0 + y.q3, *q1 = c1 + y.q3;
+        const double a1 = b.p1.wl, b1 = b.p1.wh, a2 = b.p2.wl, b2 = b.p2.wh, a3 = b.p3.wl, b3 = b.p3.wh;
+        const TV w000 = (TV)(a1 * a2 * a3), w100 = (TV)(b1 * a2 * a3), w010 = (TV)(a1 * b2 * a3),
+                 w110 = (TV)(b1 * b2 * a3), w001 = (TV)(a1 * a2 * b3), w101 = (TV)(b1 * a2 * b3),
+                 w011 = (TV)(a1 * b2 * b3), w111 = (TV)(b1 * b2 * b3);
+        const long l1 = b.p1.lo, h1 = b.p1.hi;
+        const long l2 = nO * b.p2.lo, h2 = nO * b.p2.hi;
+        const long l3 = nO * nn * b.p3.lo, h3 = nO * nn * b.p3.hi;
+        const long o000 = site + ns * (l1 + l2 + l3), o100 = site + ns * (h1 + l2 + l3);
+        const long o010 = site + ns * (l1 + h2 + l3), o110 = site + ns * (h1 + h2 + l3);
+        const long o001 = site + ns * (l1 + l2 + h3), o101 = site + ns * (h1 + l2 + h3);
+        const long o011 = site + ns * (l1 + h2 + h3), o111 = site + ns * (h1 + h2 + h3);
+        TV v0 = w000 * q0[o000];
+        TV v1 = w000 * q1[o000];
+        v0 += w100 * q0[o100]; v1 += w100 * q1[o100];
+        v0 += w010 * q0[o010]; v1 += w010 * q1[o010];
+        v0 += w110 * q0[o110]; v1 += w110 * q1[o110];
+        v0 += w001 * q0[o001]; v1 += w001 * q1[o001];
+        v0 += w101 * q0[o101]; v1 += w101 * q1[o101];
+        v0 += w011 * q0[o011]; v1 += w011 * q1[o011];
+        v0 += w111 * q0[o111]; v1 += w111 * q1[o111];
+        r0 = v0;
+        r1 = v1;
+    } else if (b.kernel == 2 || b.kernel == 3) {
+        const long off = (b.kernel == 2) ? y.q21 : y.q22;
+        const Param &pv = (b.kernel == 2) ? b.p2 : b.p3;
+        const TV *q0 = c0 + off, *q1 = c1 + off;
+        const double a1 = b.p1.wl, b1 = b.p1.wh, a2 = pv.wl, b2 = pv.wh;
+        const TV w00 = (TV)(a1 * a2), w10 = (TV)(b1 * a2), w01 = (TV)(a1 * b2), w11 = (TV)(b1 * b2);
+        const long o00 = site + ns * (b.p1.lo + nO * pv.lo), o10 = site + ns * (b.p1.hi + nO * pv.lo);
+        const long o01 = site + ns * (b.p1.lo + nO * pv.hi), o11 = site + ns * (b.p1.hi + nO * pv.hi);
+        TV v0 = w00 * q0[o00];
+        TV v1 = w00 * q1[o00];
+        v0 += w10 * q0[o10]; v1 += w10 * q1[o10];
+        v0 += w01 * q0[o01]; v1 += w01 * q1[o01];
+        v0 += w11 * q0[o11]; v1 += w11 * q1[o11];
+        r0 = v0;
+        r1 = v1;
+    } else {
+        r0 = (TV)0;
+        r1 = (TV)0;
+    }
+}
+
+// apply_flags_su2 (action_su2.jl:73-92): density sign from the two spin flags
+__device__ __forceinline__ double dens_sign(const Buf &b) {
+    double s = 1.0;
+    if (b.smu) s *= -1.0;
+    if (b.snu) s *= -1.0;
+    return s;
+}
+
+// One action as seen by a kernel: vertex channels (TV), bare and Σ (FP64).
+template <typename TV>
+struct ActV {
+    const TV *gamma;
+    const double *bare;
+    const double *Sigma;
+};
+
+// get_Γ_comp / get_Γ_comp_avx! (Action/Action.jl:29-167) for one site and both
+// components, with channel masks MS/MT/MU (the reference's ch_s/ch_t/ch_u keywords).
+__device__ __forceinline__ int corners(int kernel) { return kernel == 4 ? 8 : (kernel == 1 ? 2 : (kernel ? 4 : 0)); }
+
+// Returns the number of interpolation corners gathered per component (for the
+// algorithmic-bytes counter).
+template <typename TV, bool MS, bool MT, bool MU>
+__device__ __forceinline__ int gamma_site(const ActV<TV> &A, const Buf &bs, const Buf &bt, const Buf &bu, int i,
+                                          const int *exch, const Sys &y, TV &vs, TV &vd) {
+    vs = (TV)A.bare[i];
+    vd = (TV)A.bare[y.ns + i];
+    if (MS && bs.kernel) {
+        const int s = bs.exch ? exch[i] : i;
+        const long cb = (bs.map ? 2 : 0) * y.chlen;
+        TV r0, r1;
+        interp2(A.gamma + cb, A.gamma + y.complen + cb, bs, s, y, r0, r1);
+        vs += r0;
+        vd += (TV)dens_sign(bs) * r1;
+    }
+    if (MT && bt.kernel) {
+        const int s = bt.exch ? exch[i] : i;
+        const long cb = y.chlen;
+        TV r0, r1;
+        interp2(A.gamma + cb, A.gamma + y.complen + cb, bt, s, y, r0, r1);
+        vs += r0;
+        vd += (TV)dens_sign(bt) * r1;
+    }
+    if (MU && bu.kernel) {
+        const int s = bu.exch ? exch[i] : i;
+        const long cb = (bu.map ? 0 : 2) * y.chlen;
+        TV r0, r1;
+        interp2(A.gamma + cb, A.gamma + y.complen + cb, bu, s, y, r0, r1);
+        vs += r0;
+        vd += (TV)dens_sign(bu) * r1;
+    }
+    return (MS ? corners(bs.kernel) : 0) + (MT ? corners(bt.kernel) : 0) + (MU ? corners(bu.kernel) : 0);
+}
+
+// ---------------------------------------------------------------- propagators (Flow/bubbles.jl)
+// get_Σ (Action.jl:8-26)
+__device__ __forceinline__ double sigma_at(double w, const double *sm, int n, const double *Sig) {
+    if (fabs(w) <= sm[n - 1]) {
+        const Param p = find_param(fabs(w), sm, n);
+        const double sg = (w > 0.0) ? 1.0 : ((w < 0.0) ? -1.0 : w);
+        return sg * (p.wl * Sig[p.lo] + p.wh * Sig[p.hi]);
+    }
+    return sm[n - 1] * Sig[n - 1] / w;
+}
+
+__device__ __forceinline__ double G_bare(double L, double w) {
+    return (fabs(w) > 1e-8) ? -expm1(-(w * w) / (L * L)) / w : 0.0;
+}
+
+__device__ __forceinline__ double G_full(double L, double w, const double *sm, int n, const double *Sig) {
+    if (!(fabs(w) > 1e-8)) return 0.0;
+    const double s = sigma_at(w, sm, n, Sig);
+    const double G0 = G_bare(L, w);
+    const double denom = (w * s >= 0.0) ? 1.0 / G0 + s : 1.0 / G0;
+    return 1.0 / denom;
+}
+
+__device__ __forceinline__ double S_single(double L, double w, const double *sm, int n, const double *Sig) {
+    if (!(fabs(w) > 1e-8)) return 0.0;
+    const double G = G_full(L, w, sm, n, Sig);
+    const double G0 = G_bare(L, w);
+    const double r = G / G0;
+    return r * r * exp(-(w * w) / (L * L)) * 2.0 * w / (L * L * L);
+}
+
+// symmetrised Katanin bubble K(x1,x2) + K(x2,x1), K = (S + G²dΣ) G' / 2π (bubbles.jl:64-80)
+__device__ __forceinline__ double bubble_kat(double L, double x1, double x2, const double *sm, int n,
+                                             const double *Sig, const double *dSig) {
+    const double G1 = G_full(L, x1, sm, n, Sig), G2 = G_full(L, x2, sm, n, Sig);
+    const double S1 = S_single(L, x1, sm, n, Sig), S2 = S_single(L, x2, sm, n, Sig);
+    const double d1 = sigma_at(x1, sm, n, dSig), d2 = sigma_at(x2, sm, n, dSig);
+    return (S1 + G1 * G1 * d1) * G2 / (2.0 * PI_D) + (S2 + G2 * G2 * d2) * G1 / (2.0 * PI_D);
+}
+
+// -G1 G2 / 2π (bubbles.jl:83-96, with the sign of the callers)
+__device__ __forceinline__ double bubble_plain(double L, double x1, double x2, const double *sm, int n,
+                                               const double *Sig) {
+    const double G1 = G_full(L, x1, sm, n, Sig), G2 = G_full(L, x2, sm, n, Sig);
+    return -(G1 * G2 / (2.0 * PI_D));
+}
+
+// fermionic arguments of the bubble per channel (flow_su2_{s,t,u}.jl:18)
+__device__ __forceinline__ void bubble_args(int ch, double W, double v, double &x1, double &x2) {
+    if (ch == 0) { x1 = v + 0.5 * W; x2 = 0.5 * W - v; }
+    else if (ch == 1) { x1 = v + 0.5 * W; x2 = v - 0.5 * W; }
+    else { x1 = v - 0.5 * W; x2 = v + 0.5 * W; }
+}
+
+// ---------------------------------------------------------------- Gauss-Kronrod (QuadGK.jl restated)
+__device__ static const double GKX[8] = {0.991455371120812639206854697526329, 0.949107912342758524526189684047851,
+                                         0.864864423359769072789712788640926, 0.741531185599394439863864773280788,
+                                         0.586087235467691130294144845693013, 0.405845151377397166906606412076961,
+                                         0.207784955007898467600689403773245, 0.000000000000000000000000000000000};
+__device__ static const double GKWK[8] = {0.022935322010529224963732008058970, 0.063092092629978553290700663189204,
+                                          0.104790010322250183839876322541518, 0.140653259715525918745189590510238,
+                                          0.169004726639267902826583426598550, 0.190350578064785409913256402421014,
+                                          0.204432940075298892414161999234649, 0.209482141084727828012999174891714};
+__device__ static const double GKWG[4] = {0.129484966168869693270611432679082, 0.279705391489276667901467771423780,
+                                          0.381830050505118944950369775488975, 0.417959183673469387755102040816327};
+
+constexpr int GK_MAX_SEGS = 512;
+
+struct Seg {
+    double a, b, I, E;
+};
+
+// node k (0..14) of the 15-point rule on [a, b]; order matches the oracle's evaluation order
+__device__ __forceinline__ double gk_node(double a, double b, int k) {
+    const double s = 0.5 * (b - a);
+    if (k < 12) {
+        const int i = k >> 2, r = k & 3;
+        const double x = (r < 2) ? GKX[2 * i + 1] : GKX[2 * i];
+        return (r & 1) ? a + (1.0 + x) * s : a + (1.0 - x) * s;
+    }
+    if (k == 12) return a + s;
+    if (k == 13) return a + (1.0 - GKX[6]) * s;
+    return a + (1.0 + GKX[6]) * s;
+}
+
+__device__ __forceinline__ Seg gk_combine(double a, double b, const double *F) {
+    const double s = 0.5 * (b - a);
+    double Ig = 0.0, Ik = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        const double fg = F[4 * i] + F[4 * i + 1];
+        const double fk = F[4 * i + 2] + F[4 * i + 3];
+        Ig += fg * GKWG[i];
+        Ik += fg * GKWK[2 * i + 1] + fk * GKWK[2 * i];
+    }
+    const double f0 = F[12];
+    Ig += f0 * GKWG[3];
+    Ik += f0 * GKWK[7] + (F[13] + F[14]) * GKWK[6];
+    const double Iks = Ik * s, Igs = Ig * s;
+    return Seg{a, b, Iks, fabs(Iks - Igs)};
+}
+
+// Variable maps of quadgk for infinite endpoints: mode 0 = (-Inf, Inf) with t/(1-t²),
+// mode 1 = (s0, Inf) with s0 + t/(1-t), mode 2 = (-Inf, s0) with s0 - t/(1-t).
+template <class F>
+__device__ __forceinline__ double gk_mapped(const F &f, int mode, double s0, double t) {
+    if (mode == 0) {
+        const double t2 = t * t;
+        const double den = 1.0 / (1.0 - t2);
+        return f(t * den) * (1.0 + t2) * den * den;
+    }
+    const double den = 1.0 / (1.0 - t);
+    return (mode == 1 ? f(s0 + t * den) : f(s0 - t * den)) * den * den;
+}
+
+// Adaptive Gauss-Kronrod over one interval family, evaluated by one 64-lane workgroup:
+// lanes evaluate the 15 (initial: up to 60, bisection: 30) rule nodes in parallel,
+// lane 0 keeps the segment list.  The segment with the largest error estimate is
+// bisected until E <= atol or E <= rtol |I| (QuadGK's criterion) or the cap is hit;
+// the total is re-summed in segment order.  Must be called by all 64 lanes.
+template <class F>
+__device__ double gk_block(const F &f, int mode, double s0, const double *bp, int nbp, double atol, double rtol,
+                           Seg *segs, double *fv, int *ctl, double *ctld) {
+    const int lane = threadIdx.x;
+    const int nseg0 = nbp - 1;
+    // initial segments
+    if (lane < 15 * nseg0) {
+        const int q = lane / 15, k = lane % 15;
+        fv[lane] = gk_mapped(f, mode, s0, gk_node(bp[q], bp[q + 1], k));
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double I = 0.0, E = 0.0;
+        for (int q = 0; q < nseg0; ++q) {
+            segs[q] = gk_combine(bp[q], bp[q + 1], fv + 15 * q);
+            if (q == 0) { I = segs[0].I; E = segs[0].E; }
+            else { I += segs[q].I; E += segs[q].E; }
+        }
+        ctl[0] = nseg0;
+        ctld[0] = I;
+        ctld[1] = E;
+        ctl[1] = (E <= atol || E <= rtol * fabs(I)) ? 0 : 1;
+    }
+    __syncthreads();
+    while (true) {
+        if (lane == 0) {
+            const double I = ctld[0], E = ctld[1];
+            int go = ctl[1] && (E > atol && E > rtol * fabs(I) && ctl[0] < GK_MAX_SEGS);
+            if (go) {
+                int k = 0;
+                for (int j = 1; j < ctl[0]; ++j)
+                    if (segs[j].E > segs[k].E) k = j;
+                ctl[2] = k;
+            }
+            ctl[1] = go;
+        }
+        __syncthreads();
+        if (!ctl[1]) break;
+        const int k = ctl[2];
+        const double a = segs[k].a, b = segs[k].b;
+        const double mid = 0.5 * (a + b);
+        if (lane < 30) {
+            const int q = lane / 15, kk = lane % 15;
+            fv[lane] = gk_mapped(f, mode, s0, q == 0 ? gk_node(a, mid, kk) : gk_node(mid, b, kk));
+        }
+        __syncthreads();
+        if (lane == 0) {
+            const Seg s = segs[k];
+            const Seg s1 = gk_combine(a, mid, fv), s2 = gk_combine(mid, b, fv + 15);
+            ctld[0] = (ctld[0] - s.I) + s1.I + s2.I;
+            ctld[1] = (ctld[1] - s.E) + s1.E + s2.E;
+            segs[k] = s1;
+            segs[ctl[0]] = s2;
+            ctl[0] += 1;
+        }
+        __syncthreads();
+    }
+    double I = 0.0;
+    if (lane == 0) {
+        I = segs[0].I;
+        for (int j = 1; j < ctl[0]; ++j) I += segs[j].I;
+    }
+    return I;  // valid on lane 0
+}
+
+}  // namespace pffrg

@@ -1,0 +1,217 @@
+"""Host-side drivers around the device right-hand side: one RHS evaluation, the
+Bogacki-Shampine flow loop (launcher_1l.jl / launcher_2l.jl / launcher_ml.jl) and
+the parquet iteration (parquet.jl), all driving the C ABI through native.Context.
+
+Step control, the embedded-error norm and the parquet residual stay in FP64
+(pffrg_abs_max reads FP64 device data).  The frequency meshes stay fixed during a
+flow: the reference's per-step mesh rescanning (Action.jl:369-557) is not part of
+this path.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import lattice as LT
+from . import mesh as M
+from .action import Action
+from .native import FP64, FP32_MIXED, Context
+
+# slot roles
+A, DA, A_STAGE, A_INTER, A_ERR, DA_L, DA_C, DA_TEMP, DA_SIGMA = range(9)
+
+
+@dataclass
+class Problem:
+    """Lattice + meshes + bare action of a launch! call (Launcher.jl:565-680)."""
+    name: str
+    size: int
+    J: list
+    S: float = 0.5
+    A_rep: float = 0.0
+    mesh: M.Mesh = None
+    l: LT.Lattice = None
+    r: LT.ReducedLattice = None
+    bare: np.ndarray = None
+
+    @classmethod
+    def build(cls, name, size, J, initial, num_sigma=25, num_Omega=15, num_nu=10, num_chi=10, S=0.5, A=0.0,
+              normalize=True):
+        if normalize:
+            J, A = LT.normalize_couplings(J, A)
+        l, r = LT.build(name, size, J)
+        m = M.launch_mesh(initial, num_sigma, num_Omega, num_nu, num_chi)
+        bare = LT.bare_couplings(l, r, S, A)
+        return cls(name, size, J, S, A, m, l, r, bare)
+
+    def bare_action(self) -> Action:
+        a = Action.empty(self.r.n_sites, self.mesh, self.S)
+        a.bare[:] = self.bare
+        return a
+
+    def context(self, n_slots: int = 9, precision: int = FP64, device: int = 0) -> Context:
+        c = Context(self.r.n_sites, self.mesh, n_slots, self.S, device)
+        c.set_lattice(self.r)
+        c.set_precision(precision)
+        return c
+
+
+def rhs(ctx: Context, L: float, loops: int, a: int, da: int, eval_: int, Gtol=(1e-8, 1e-4), Stol=(1e-8, 1e-4),
+        da_l: int = DA_L, da_c: int = DA_C, da_temp: int = DA_TEMP, da_sigma: int = DA_SIGMA, sigma_corr=True):
+    """One complete right-hand side as evaluated per BS3 stage (launcher_1l.jl:85-86,
+    launcher_ml.jl:94-95): compute_dΣ! then compute_dΓ_{1l,2l,ml}! (+ compute_dΣ_corr!)."""
+    ctx.compute_dsigma(L, a, da, Stol)
+    if loops == 1:
+        ctx.compute_dgamma_1l(L, a, da, eval_, Gtol)
+    elif loops == 2:
+        ctx.compute_dgamma_2l(L, a, da, da_l, eval_, Gtol)
+    else:
+        ctx.compute_dgamma_ml(L, loops, a, da, da_l, da_c, da_temp, da_sigma, eval_, Gtol)
+        if sigma_corr:
+            ctx.compute_dsigma_corr(L, a, da, da_sigma, Stol)
+
+
+@dataclass
+class FlowResult:
+    Lambdas: list = field(default_factory=list)
+    gamma_max: list = field(default_factory=list)
+    accepted: int = 0
+    rejected: int = 0
+    rhs_evals: int = 0
+    wall_s: float = 0.0
+    status: str = ""
+
+
+def launch_flow(ctx: Context, prob: Problem, a0: Action, Li: float, Lf: float, loops: int = 1, sigma_corr=True,
+                bmin=1e-4, bmax=0.1, min_eval=10, max_eval=100, Stol=(1e-8, 1e-4), Gtol=(1e-8, 1e-4),
+                ODE_tol=(1e-8, 1e-2), max_steps: int | None = None, verbose=False) -> FlowResult:
+    """BS3 flow from Li to Lf (launcher_1l.jl:72-198 and its 2l/ml twins).  Slot A holds
+    the flowing action on return."""
+    t0 = time.time()
+    res = FlowResult()
+    ctx.upload(A, a0)
+    bare = prob.bare_action()
+    # da, a_stage, a_err start empty; a_inter carries the bare couplings (launcher_1l.jl:39-44)
+    zero = Action.empty(prob.r.n_sites, prob.mesh, prob.S)
+    for s in (DA, A_STAGE, A_ERR, DA_L, DA_C, DA_TEMP, DA_SIGMA):
+        ctx.upload(s, zero)
+    ctx.upload(A_INTER, bare)
+    L, dL = Li, bmax * Li
+    cps = sorted({Li, Lf}, reverse=True)
+    for c in cps:
+        if c < L:
+            dL = min(dL, 0.85 * (L - c))
+            break
+    steps = 0
+    while L > Lf:
+        if max_steps is not None and steps >= max_steps:
+            res.status = "max_steps"
+            break
+        steps += 1
+        ev = M.get_eval(L, min_eval, max_eval)
+        ctx.replace_with(DA, A)
+        ctx.replace_with(A_ERR, A)
+        # k1
+        rhs(ctx, L, loops, A, A_STAGE, ev, Gtol, Stol, sigma_corr=sigma_corr)
+        ctx.mult_with_add_to(A_STAGE, -2.0 * dL / 9.0, DA)
+        ctx.mult_with_add_to(A_STAGE, -7.0 * dL / 24.0, A_ERR)
+        # k2
+        ctx.replace_with(A_INTER, A)
+        ctx.mult_with_add_to(A_STAGE, -0.5 * dL, A_INTER)
+        rhs(ctx, L - 0.5 * dL, loops, A_INTER, A_STAGE, ev, Gtol, Stol, sigma_corr=sigma_corr)
+        ctx.mult_with_add_to(A_STAGE, -1.0 * dL / 3.0, DA)
+        ctx.mult_with_add_to(A_STAGE, -1.0 * dL / 4.0, A_ERR)
+        # k3
+        ctx.replace_with(A_INTER, A)
+        ctx.mult_with_add_to(A_STAGE, -0.75 * dL, A_INTER)
+        rhs(ctx, L - 0.75 * dL, loops, A_INTER, A_STAGE, ev, Gtol, Stol, sigma_corr=sigma_corr)
+        ctx.mult_with_add_to(A_STAGE, -4.0 * dL / 9.0, DA)
+        ctx.mult_with_add_to(A_STAGE, -1.0 * dL / 3.0, A_ERR)
+        # k4
+        ctx.replace_with(A_INTER, DA)
+        rhs(ctx, L - dL, loops, A_INTER, A_STAGE, ev, Gtol, Stol, sigma_corr=sigma_corr)
+        ctx.mult_with_add_to(A_STAGE, -1.0 * dL / 8.0, A_ERR)
+        res.rhs_evals += 4
+        # embedded error (launcher_1l.jl:113-116)
+        ctx.mult_with_add_to(A_INTER, -1.0, A_ERR)
+        delta = ctx.abs_max(A_ERR)
+        gmax = ctx.abs_max(A_INTER)
+        scale = ODE_tol[0] + max(gmax, ctx.abs_max(A)) * ODE_tol[1]
+        err = delta / scale
+        if verbose:
+            print(f"L={L:.6g} dL={dL:.3g} eval={ev} err={err:.3g} Gmax={gmax:.4g}", flush=True)
+        if err >= 10.0:
+            res.status = "unstable"
+            break
+        if err <= 1.0 or dL <= bmin:
+            L -= dL
+            dL = max(bmin, min(bmax * L, 0.85 * (1.0 / err) ** (1.0 / 3.0) * dL))
+            lower = [c - L for c in cps if c - L < 0.0]
+            L_cp = L + lower[0] if lower else 0.0
+            dL = min(dL, L - L_cp)
+            res.accepted += 1
+            res.Lambdas.append(L)
+            res.gamma_max.append(gmax)
+            ctx.replace_with(A, A_INTER)
+            if gmax >= 1000.0:
+                res.status = "diverged"
+                break
+        else:
+            dL = max(bmin, min(bmax * L, 0.85 * (1.0 / err) ** (1.0 / 3.0) * dL))
+            lower = [c - L for c in cps if c - L < 0.0]
+            L_cp = L + lower[0] if lower else 0.0
+            dL = min(dL, L - L_cp)
+            res.rejected += 1
+    else:
+        res.status = "finished"
+    ctx.synchronize()
+    res.wall_s = time.time() - t0
+    return res
+
+
+@dataclass
+class ParquetResult:
+    iterations: int = 0
+    sde_iterations: int = 0
+    abs_err: float = float("inf")
+    rel_err: float = float("inf")
+    history: list = field(default_factory=list)
+
+
+def launch_parquet(ctx: Context, L: float, a0: Action, max_iter=50, min_eval=10, max_eval=100, Stol=(1e-8, 1e-4),
+                   Gtol=(1e-8, 1e-4), parquet_tol=(1e-8, 1e-4), slot_a=A, slot_ap=DA, slot_err=A_ERR) -> ParquetResult:
+    """Damped parquet fixed point (parquet.jl:1-100); slot_a holds the solution on return."""
+    res = ParquetResult()
+    ctx.upload(slot_a, a0)
+    zero = Action.empty(a0.ns, ctx.m, a0.S)
+    ctx.upload(slot_ap, zero)
+    ctx.upload(slot_err, zero)
+    beta = min(L, 1.0)
+    ev = M.get_eval(L, min_eval, max_eval)
+    abs_err = rel_err = np.inf
+    count = 1
+    while abs_err >= parquet_tol[0] and rel_err >= parquet_tol[1] and count <= max_iter:
+        s_abs = s_rel = np.inf
+        s_count = 1
+        while s_abs >= parquet_tol[0] and s_rel >= parquet_tol[1] and s_count <= max_iter:
+            ctx.compute_sigma(L, slot_a, slot_ap, ev, Gtol, Stol)
+            sa, sp = ctx.get_sigma(slot_a), ctx.get_sigma(slot_ap)
+            s_abs = float(np.max(np.abs(sp - sa)))
+            s_rel = s_abs / max(float(np.max(np.abs(sa))), float(np.max(np.abs(sp))))
+            ctx.set_sigma(slot_a, (1.0 - beta) * sa + beta * sp)
+            s_count += 1
+            res.sde_iterations += 1
+        ctx.compute_gamma(L, slot_a, slot_ap, ev, Gtol)
+        ctx.replace_with(slot_err, slot_ap, vertex_only=True)
+        ctx.mult_with_add_to(slot_a, -1.0, slot_err, vertex_only=True)
+        abs_err = ctx.abs_max(slot_err)
+        rel_err = abs_err / max(ctx.abs_max(slot_a), ctx.abs_max(slot_ap))
+        res.history.append((abs_err, rel_err))
+        ctx.mult_with(slot_a, 1.0 - beta, vertex_only=True)
+        ctx.mult_with_add_to(slot_ap, beta, slot_a, vertex_only=True)
+        count += 1
+    res.iterations = count - 1
+    res.abs_err, res.rel_err = abs_err, rel_err
+    return res

@@ -1,0 +1,264 @@
+"""ctypes binding of libpffrg.so (the C ABI declared in include/pffrg.h).
+
+The library is the only compute path: if it is missing, or no HIP device is
+visible, every call raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from .action import Action
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libpffrg.so"
+
+FP64 = 0
+FP32_MIXED = 1
+
+_lib = None
+
+
+class PffrgError(RuntimeError):
+    pass
+
+
+class Dims(C.Structure):
+    _fields_ = [("n_sites", C.c_int32), ("n_sigma", C.c_int32), ("n_Omega", C.c_int32), ("n_nu", C.c_int32),
+                ("S", C.c_double), ("n_slots", C.c_int32), ("device", C.c_int32)]
+
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+_I64 = C.POINTER(C.c_int64)
+_i = C.c_int32
+_d = C.c_double
+
+_SIGS = {
+    "pffrg_create": ([C.POINTER(Dims), C.POINTER(_P)], C.c_int),
+    "pffrg_destroy": ([_P], None),
+    "pffrg_last_error": ([_P], C.c_char_p),
+    "pffrg_set_mesh": ([_P, _D, _D, _D, _D, _D], C.c_int),
+    "pffrg_set_lattice": ([_P, _I64, _D, _I64, _I64, _i], C.c_int),
+    "pffrg_upload_action": ([_P, _i, _D, _D, _D], C.c_int),
+    "pffrg_download_action": ([_P, _i, _D, _D, _D], C.c_int),
+    "pffrg_gamma_length": ([_P], C.c_int64),
+    "pffrg_set_precision": ([_P, _i], C.c_int),
+    "pffrg_compute_dsigma": ([_P, _d, _i, _i, _D], C.c_int),
+    "pffrg_compute_dsigma_corr": ([_P, _d, _i, _i, _i, _D], C.c_int),
+    "pffrg_compute_dgamma_1l": ([_P, _d, _i, _i, _i, _D], C.c_int),
+    "pffrg_compute_dgamma_2l": ([_P, _d, _i, _i, _i, _i, _D], C.c_int),
+    "pffrg_compute_dgamma_ml": ([_P, _d, _i, _i, _i, _i, _i, _i, _i, _i, _D], C.c_int),
+    "pffrg_compute_sigma": ([_P, _d, _i, _i, _i, _D, _D], C.c_int),
+    "pffrg_compute_gamma": ([_P, _d, _i, _i, _i, _D], C.c_int),
+    "pffrg_symmetrize": ([_P, _i], C.c_int),
+    "pffrg_symmetrize_add_to": ([_P, _i, _i], C.c_int),
+    "pffrg_replace_with": ([_P, _i, _i, _i], C.c_int),
+    "pffrg_mult_with": ([_P, _i, _d, _i], C.c_int),
+    "pffrg_mult_with_add_to": ([_P, _i, _d, _i, _i], C.c_int),
+    "pffrg_abs_max": ([_P, _i, _D], C.c_int),
+    "pffrg_get_sigma": ([_P, _i, _D], C.c_int),
+    "pffrg_set_sigma": ([_P, _i, _D], C.c_int),
+    "pffrg_synchronize": ([_P], C.c_int),
+    "pffrg_reset_stats": ([_P], C.c_int),
+    "pffrg_get_stats": ([_P, _I64], C.c_int),
+    "pffrg_get_corrs": ([_P, _D], C.c_int),
+    "pffrg_probe_integrand": ([_P, _i, _d, _i, _i, _i, _i, _i, _i, _i, _i, _D, _D], C.c_int),
+    "pffrg_set_timing": ([_P, _i], C.c_int),
+    "pffrg_get_timing": ([_P, _D, _I64], C.c_int),
+}
+
+
+def load() -> C.CDLL:
+    """Load libpffrg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise PffrgError(f"{LIB_PATH} not found: build it with __graft_entry__.build()")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def _dp(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def _ip(a: np.ndarray):
+    assert a.dtype == np.int64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_I64)
+
+
+def _tol(t) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(t, dtype=np.float64).reshape(2))
+
+
+class Context:
+    """One device context: meshes, lattice tables and n_slots action slots on the GPU."""
+
+    def __init__(self, n_sites: int, m, n_slots: int, S: float = 0.5, device: int = 0):
+        self.lib = load()
+        self.m = m
+        self.ns = n_sites
+        self.S = S
+        d = Dims(n_sites, m.num_sigma, m.num_Omega, m.num_nu, S, n_slots, device)
+        h = _P()
+        rc = self.lib.pffrg_create(C.byref(d), C.byref(h))
+        if rc != 0 or not h:
+            raise PffrgError(f"pffrg_create failed with code {rc} (no HIP device, or dims out of range)")
+        self.h = h
+        self.n_slots = n_slots
+        self.set_mesh(m)
+
+    def _ck(self, rc: int, what: str):
+        if rc != 0:
+            msg = self.lib.pffrg_last_error(self.h)
+            raise PffrgError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pffrg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---------------------------------------------------------------- setup
+    def set_mesh(self, m):
+        arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (m.sigma, m.Os, m.ns, m.Ot, m.nt)]
+        self._ck(self.lib.pffrg_set_mesh(self.h, *[_dp(a) for a in arrs]), "set_mesh")
+        self.m = m
+
+    def set_lattice(self, r):
+        off, rows = r.overlap_csr()
+        ex = np.ascontiguousarray(r.exchange, dtype=np.int64)
+        mult = np.ascontiguousarray(r.mult, dtype=np.float64)
+        off = np.ascontiguousarray(off, dtype=np.int64)
+        rows = np.ascontiguousarray(rows, dtype=np.int64).reshape(-1)
+        if rows.size == 0:
+            rows = np.zeros(3, dtype=np.int64)
+        self._ck(self.lib.pffrg_set_lattice(self.h, _ip(ex), _dp(mult), _ip(off), _ip(rows), 0), "set_lattice")
+
+    def set_precision(self, mode: int):
+        self._ck(self.lib.pffrg_set_precision(self.h, mode), "set_precision")
+
+    # ---------------------------------------------------------------- transfer
+    def upload(self, slot: int, a: Action):
+        self._ck(self.lib.pffrg_upload_action(self.h, slot, _dp(np.ascontiguousarray(a.Sigma)),
+                                              _dp(np.ascontiguousarray(a.bare.reshape(-1))),
+                                              _dp(np.ascontiguousarray(a.gamma))), "upload_action")
+
+    def download(self, slot: int) -> Action:
+        a = Action.empty(self.ns, self.m, self.S)
+        bare = np.zeros(2 * self.ns)
+        self._ck(self.lib.pffrg_download_action(self.h, slot, _dp(a.Sigma), _dp(bare), _dp(a.gamma)), "download_action")
+        a.bare[:] = bare.reshape(2, self.ns)
+        return a
+
+    def get_sigma(self, slot: int) -> np.ndarray:
+        s = np.zeros(self.m.num_sigma)
+        self._ck(self.lib.pffrg_get_sigma(self.h, slot, _dp(s)), "get_sigma")
+        return s
+
+    def set_sigma(self, slot: int, s: np.ndarray):
+        self._ck(self.lib.pffrg_set_sigma(self.h, slot, _dp(np.ascontiguousarray(s, dtype=np.float64))), "set_sigma")
+
+    # ---------------------------------------------------------------- flow right-hand side
+    def compute_dsigma(self, L, a, da, tol=(1e-8, 1e-4)):
+        self._ck(self.lib.pffrg_compute_dsigma(self.h, L, a, da, _dp(_tol(tol))), "compute_dsigma")
+
+    def compute_dsigma_corr(self, L, a, da, da_sigma, tol=(1e-8, 1e-4)):
+        self._ck(self.lib.pffrg_compute_dsigma_corr(self.h, L, a, da, da_sigma, _dp(_tol(tol))), "compute_dsigma_corr")
+
+    def compute_dgamma_1l(self, L, a, da, eval_, tol=(1e-8, 1e-4)):
+        self._ck(self.lib.pffrg_compute_dgamma_1l(self.h, L, a, da, eval_, _dp(_tol(tol))), "compute_dgamma_1l")
+
+    def compute_dgamma_2l(self, L, a, da, da_l, eval_, tol=(1e-8, 1e-4)):
+        self._ck(self.lib.pffrg_compute_dgamma_2l(self.h, L, a, da, da_l, eval_, _dp(_tol(tol))), "compute_dgamma_2l")
+
+    def compute_dgamma_ml(self, L, loops, a, da, da_l, da_c, da_temp, da_sigma, eval_, tol=(1e-8, 1e-4)):
+        self._ck(self.lib.pffrg_compute_dgamma_ml(self.h, L, loops, a, da, da_l, da_c, da_temp, da_sigma, eval_,
+                                                  _dp(_tol(tol))), "compute_dgamma_ml")
+
+    def compute_sigma(self, L, a1, a2, eval_, gtol=(1e-8, 1e-4), stol=(1e-8, 1e-4)):
+        self._ck(self.lib.pffrg_compute_sigma(self.h, L, a1, a2, eval_, _dp(_tol(gtol)), _dp(_tol(stol))),
+                 "compute_sigma")
+
+    def compute_gamma(self, L, a1, a2, eval_, tol=(1e-8, 1e-4)):
+        self._ck(self.lib.pffrg_compute_gamma(self.h, L, a1, a2, eval_, _dp(_tol(tol))), "compute_gamma")
+
+    # ---------------------------------------------------------------- algebra
+    def symmetrize(self, s):
+        self._ck(self.lib.pffrg_symmetrize(self.h, s), "symmetrize")
+
+    def symmetrize_add_to(self, l, a):
+        self._ck(self.lib.pffrg_symmetrize_add_to(self.h, l, a), "symmetrize_add_to")
+
+    def replace_with(self, dst, src, vertex_only=False):
+        self._ck(self.lib.pffrg_replace_with(self.h, dst, src, int(vertex_only)), "replace_with")
+
+    def mult_with(self, s, fac, vertex_only=False):
+        self._ck(self.lib.pffrg_mult_with(self.h, s, fac, int(vertex_only)), "mult_with")
+
+    def mult_with_add_to(self, src, fac, dst, vertex_only=False):
+        self._ck(self.lib.pffrg_mult_with_add_to(self.h, src, fac, dst, int(vertex_only)), "mult_with_add_to")
+
+    def abs_max(self, s) -> float:
+        out = C.c_double()
+        self._ck(self.lib.pffrg_abs_max(self.h, s, C.byref(out)), "abs_max")
+        return out.value
+
+    # ---------------------------------------------------------------- measurement
+    def synchronize(self):
+        self._ck(self.lib.pffrg_synchronize(self.h), "synchronize")
+
+    def reset_stats(self):
+        self._ck(self.lib.pffrg_reset_stats(self.h), "reset_stats")
+
+    def stats(self) -> dict:
+        s = np.zeros(4, dtype=np.int64)
+        self._ck(self.lib.pffrg_get_stats(self.h, _ip(s)), "get_stats")
+        return {"nodes": int(s[0]), "elements": int(s[1]), "launches": int(s[2]), "gathered_bytes": int(s[3])}
+
+    def probe_integrand(self, pass_, L, a, da, ch, kernel, w1, w2, w3, v) -> np.ndarray:
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros((len(v), 2, self.ns))
+        self._ck(self.lib.pffrg_probe_integrand(self.h, pass_, L, a, da, ch, kernel, w1, w2, w3, len(v), _dp(v),
+                                                out.ctypes.data_as(_D)), "probe_integrand")
+        return out
+
+    def corrs(self) -> np.ndarray:
+        out = np.zeros(2 * 3 * self.m.num_Omega)
+        self._ck(self.lib.pffrg_get_corrs(self.h, _dp(out)), "get_corrs")
+        return out
+
+    def set_timing(self, enable: bool):
+        self._ck(self.lib.pffrg_set_timing(self.h, int(enable)), "set_timing")
+
+    def timing(self) -> tuple[float, int]:
+        ms = C.c_double()
+        n = C.c_int64()
+        self._ck(self.lib.pffrg_get_timing(self.h, C.byref(ms), C.byref(n)), "get_timing")
+        return ms.value, n.value
