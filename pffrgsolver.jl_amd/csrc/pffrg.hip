@@ -314,6 +314,14 @@ struct pffrg_ctx {
     int *d_dbg_nodes = nullptr;
     double *d_dbg_res = nullptr;
     long dbg_cap = 0;
+    // host copies of the meshes and the log-domain breakpoint table
+    std::vector<double> h_Os, h_Ot;
+    double *d_logpts = nullptr;
+    long logpts_cap = 0;
+    std::vector<double> h_logpts;
+    bool logpts_valid = false;
+    double logpts_L = 0.0;
+    int logpts_eval = 0;
 };
 
 namespace {
@@ -417,6 +425,40 @@ int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const
     }
     P.out = c->slots[out].gamma;
     P.corrs = c->d_corrs;
+    if (!(c->logpts_valid && c->logpts_L == L && c->logpts_eval == eval)) {
+        // breakpoints ξ^k·2ref of the log domains (quadrature.jl:113-139), both mesh families;
+        // rebuilt only when Λ, eval or the meshes change, after the stream has drained
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const int nO = c->sys.nO;
+        const long need = 2L * nO * (eval + 1);
+        if (need > c->logpts_cap) {
+            HIPCHK(c, hipFree(c->d_logpts));
+            c->d_logpts = nullptr;
+            HIPCHK(c, hipMalloc(&c->d_logpts, need * sizeof(double)));
+            c->logpts_cap = need;
+        }
+        c->h_logpts.assign(need, 0.0);
+        for (int fam = 0; fam < 2; ++fam) {
+            const std::vector<double> &Om = fam ? c->h_Ot : c->h_Os;
+            const double val = fam ? c->sys.valT : c->sys.valS;
+            for (int w1 = 0; w1 < nO; ++w1) {
+                double *pt = c->h_logpts.data() + ((long)fam * nO + w1) * (eval + 1);
+                const double lo = 2.0 * (L + 0.5 * Om[w1]), hi = 1.0 * val;
+                if (!(hi > lo)) {
+                    pt[0] = -1.0;
+                    continue;
+                }
+                const double xi = std::pow(hi / lo, 1.0 / eval);
+                for (int k = 0; k <= eval; ++k) pt[k] = std::pow(xi, k) * lo;
+            }
+        }
+        HIPCHK(c, hipMemcpyAsync(c->d_logpts, c->h_logpts.data(), need * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+        c->logpts_valid = true;
+        c->logpts_L = L;
+        c->logpts_eval = eval;
+    }
+    P.logpts = c->d_logpts;
     P.elems = (mode == 0) ? c->d_el_half : (mode == 1) ? c->d_el_full : c->d_el_red;
     P.n_elems = (mode == 0) ? c->n_half : (mode == 1) ? c->n_full : c->n_red;
     P.per_xcd = (P.n_elems + 7) / 8;
@@ -639,6 +681,7 @@ void pffrg_destroy(pffrg_ctx *c) {
     (void)hipFree(c->d_el_half);
     (void)hipFree(c->d_el_full);
     (void)hipFree(c->d_el_red);
+    (void)hipFree(c->d_logpts);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -671,6 +714,10 @@ int pffrg_set_mesh(pffrg_ctx *c, const double *sigma, const double *Os, const do
     }
     y.valS = Os[y.nO - 1] + ns[y.nn - 1];
     y.valT = Ot[y.nO - 1] + nt[y.nn - 1];
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->h_Os.assign(Os, Os + y.nO);
+    c->h_Ot.assign(Ot, Ot + y.nO);
+    c->logpts_valid = false;
     HIPCHK(c, hipMemcpy(c->d_mesh, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     c->have_mesh = true;
     return 0;

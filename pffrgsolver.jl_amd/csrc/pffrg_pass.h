@@ -22,6 +22,9 @@ struct PassArgs {
     const double *corrs;      // [nO][3][2]
     const uint32_t *elems;
     int n_elems;
+    // breakpoints of the logarithmic domains [2ref, val], (eval+1) per (mesh family, Ω index),
+    // computed on the host with libm pow; entry 0 < 0 marks an empty domain
+    const double *logpts;
     int per_xcd;              // elements handled by each XCD slice of the grid
     double L;
     int eval;
@@ -180,9 +183,6 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
     }
 }
 
-// node position a + c*h rounded as the reference computes it (product, then sum)
-__device__ __forceinline__ double node_at(double a, double c, double h) { return __dadd_rn(a, __dmul_rn(c, h)); }
-
 // element code: ch 2 bits | kernel 3 bits | w1 7 bits | w2 7 bits | w3 7 bits
 __device__ __forceinline__ void decode(uint32_t e, int &ch, int &kernel, int &w1, int &w2, int &w3) {
     ch = e & 3;
@@ -192,17 +192,24 @@ __device__ __forceinline__ void decode(uint32_t e, int &ch, int &kernel, int &w1
     w3 = (e >> 19) & 127;
 }
 
+// node position a + c*h rounded as the reference computes it (product, then sum)
+// (the pragma keeps hipcc from fusing it into an FMA; __dmul_rn/__dadd_rn are fused on gfx950)
+__device__ __forceinline__ double node_at(double a, double c, double h) {
+#pragma clang fp contract(off)
+    return a + c * h;
+}
+
 // sub-interval j of the four loop-frequency domains (flow_s.jl:31-34 with
 // quadrature.jl:84-139): [2ref,val] mirrored, [-2ref,0], [0,2ref], [2ref,val]
-__device__ __forceinline__ bool subinterval(int j, int eval, double ref, double val, double &a, double &b,
-                                            double &sgn) {
+__device__ __forceinline__ bool subinterval(int j, int eval, double ref, const double *logpts, double &a,
+                                            double &b, double &sgn) {
     const int d = j / eval, k = j % eval;
     if (d == 0 || d == 3) {
-        const double lo = 2.0 * ref, hi = 1.0 * val;
-        if (!(hi > lo)) return false;
-        const double xi = pow(hi / lo, 1.0 / eval);
-        a = pow(xi, (double)k) * lo;
-        b = pow(xi, (double)(k + 1)) * lo;
+        // the last breakpoint lands on the mesh edge Ω_max+ν_max where the integrand jumps, so
+        // the points come from the host's pow (the oracle's) rather than the device's
+        if (!(logpts[0] > 0.0)) return false;
+        a = logpts[k];
+        b = logpts[k + 1];
         sgn = (d == 0) ? -1.0 : 1.0;
         return true;
     }
@@ -212,8 +219,8 @@ __device__ __forceinline__ bool subinterval(int j, int eval, double ref, double 
     const double h = (hi - lo) / eval;
     // separately rounded like the reference (no FMA): the last piece of [-2ref, 0] must end
     // where the reference's does, since the integrand jumps at v = 0 (sign fold of ν')
-    a = __dadd_rn(lo, __dmul_rn((double)k, h));
-    b = __dadd_rn(lo, __dmul_rn((double)(k + 1), h));
+    a = node_at(lo, (double)k, h);
+    b = node_at(lo, (double)(k + 1), h);
     sgn = 1.0;
     return true;
 }
@@ -276,6 +283,7 @@ __global__ __launch_bounds__(64) void k_vertex_pass(PassArgs<TV> P) {
     }
     const double ref = P.L + 0.5 * W;
     const double val = (ch == 1) ? y.valT : y.valS;
+    const double *logpts = P.logpts + ((ch == 1 ? y.nO : 0) + w1) * (P.eval + 1);
 
     const int GS = y.GS, G = y.G;
     const int g = lane / GS;
@@ -302,7 +310,7 @@ __global__ __launch_bounds__(64) void k_vertex_pass(PassArgs<TV> P) {
     for (int r = 0; r < rounds; ++r) {
         const int j = r * G + g;
         double a = 0.0, b = 0.0, sgn = 1.0;
-        bool act = real_lane && j < nsub && subinterval(j, P.eval, ref, val, a, b, sgn);
+        bool act = real_lane && j < nsub && subinterval(j, P.eval, ref, logpts, a, b, sgn);
         if (__ballot(act) == 0) continue;  // wave-uniform skip of empty rounds
         double b10 = 0.0, b11 = 0.0, b20 = 0.0, b21 = 0.0;
         const unsigned long long nodes_before = nodes;
@@ -396,7 +404,7 @@ __global__ __launch_bounds__(64) void k_vertex_pass(PassArgs<TV> P) {
         for (int gg = 0; gg < G; ++gg) {
             const int jj = r * G + gg;
             double ta, tb, ts;
-            if (jj < nsub && subinterval(jj, P.eval, ref, val, ta, tb, ts)) {
+            if (jj < nsub && subinterval(jj, P.eval, ref, logpts, ta, tb, ts)) {
                 acc0 += s_scr[2 * (gg * GS + i) + 0];
                 acc1 += s_scr[2 * (gg * GS + i) + 1];
             }
