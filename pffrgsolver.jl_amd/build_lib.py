@@ -26,14 +26,16 @@ HIPCC_FLAGS = [
 ]
 
 
-def build(verbose: bool = False, jobs: int | None = None) -> Path:
-    OUT.parent.mkdir(parents=True, exist_ok=True)
+def build(verbose: bool = False, jobs: int | None = None, out: Path | None = None, defines=()) -> Path:
+    out = Path(out) if out else OUT
+    out.parent.mkdir(parents=True, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     inc = [f"-I{ROOT / 'include'}", f"-I{CSRC}"]
     with tempfile.TemporaryDirectory(prefix="pffrg_build_") as tmpd:
         def compile_one(src: str) -> str:
             obj = str(Path(tmpd) / (Path(src).stem + ".o"))
-            r = subprocess.run([hipcc, *HIPCC_FLAGS, *inc, "-c", "-o", obj, str(CSRC / src)],
+            r = subprocess.run([hipcc, *HIPCC_FLAGS, *[f"-D{d}" for d in defines], *inc, "-c", "-o", obj,
+                                str(CSRC / src)],
                                capture_output=True, text=True)
             if r.returncode != 0:
                 raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}{r.stderr}")
@@ -44,13 +46,13 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
         n = jobs or min(len(SOURCES), os.cpu_count() or 1, 8)
         with ThreadPoolExecutor(max_workers=n) as ex:
             objs = list(ex.map(compile_one, SOURCES))
-        tmp = OUT.with_suffix(".so.tmp")
+        tmp = out.with_suffix(".so.tmp")
         r = subprocess.run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs],
                            capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"link failed:\n{r.stdout}{r.stderr}")
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -467,7 +467,8 @@ int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const
     P.atol = tol[0];
     P.rtol = tol[1];
     P.node_count = c->d_count;
-    const size_t lds = (size_t)(c->n_ov + c->sys.ns + 1) * sizeof(uint32_t);
+    const size_t lds = (size_t)((c->n_ov + c->sys.ns + 1 + 3) & ~3) * sizeof(uint32_t) +
+                       (size_t)c->sys.G * MAX_DESC * sizeof(Desc);
     const int grid = dbg_v ? dbg_n : 8 * P.per_xcd;
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, (launch_vertex_pass<PASS, TV>(P, grid, lds, c->stream)));
@@ -620,6 +621,7 @@ int pffrg_create(const pffrg_dims *d, pffrg_ctx **out) {
     y.nsig = d->n_sigma;
     y.GS = y.ns;
     y.G = (y.ns <= 32) ? WAVE / y.ns : 1;
+    if (y.G > MAX_GROUPS) y.G = MAX_GROUPS;
     y.S2 = 2.0 * d->S;
     y.q21 = (long)y.ns * y.nO;
     y.q22 = y.q21 + (long)y.ns * y.nO * y.nn;
@@ -739,7 +741,8 @@ int pffrg_set_lattice(pffrg_ctx *c, const int64_t *exchange, const double *mult,
     const int R = off[ns];
     for (int i = 0; i < ns; ++i)
         if (off[i + 1] < off[i]) return fail(c, PFFRG_ERR_ARG, "ov_off must be non-decreasing");
-    if ((size_t)(R + ns + 1) * 4 > 48 * 1024) return fail(c, PFFRG_ERR_LIMIT, "overlap table exceeds the LDS budget");
+    if ((size_t)(R + ns + 4) * 4 + (size_t)c->sys.G * MAX_DESC * sizeof(Desc) > 48 * 1024)
+        return fail(c, PFFRG_ERR_LIMIT, "overlap table exceeds the LDS budget");
     std::vector<uint32_t> rows(R > 0 ? R : 1);
     for (int r = 0; r < R; ++r) {
         const long j1 = ov_rows[3 * r] - base, j2 = ov_rows[3 * r + 1] - base, m = ov_rows[3 * r + 2];

@@ -9,6 +9,10 @@
 
 namespace pffrg {
 
+#ifndef PFFRG_PASS_WAVES_PER_EU
+#define PFFRG_PASS_WAVES_PER_EU 3
+#endif
+
 template <typename TV>
 struct PassArgs {
     Sys y;
@@ -40,123 +44,280 @@ struct PassArgs {
     long dbg_cap;
 };
 
+// Interpolation descriptor of one buffer (one channel of one vertex slot) at one node, built
+// once per node by one lane and read by every site lane of the group.
+struct Desc {
+    int meta;            // kernel (bits 0-2) | exchange << 3 | map << 4 | negative density sign << 5
+    int off;             // offset of the first corner inside the channel block (site 0)
+    int d1, d2, d3;      // offset steps to the upper corner along Ω, ν, ν'
+    int pad[3];
+    double w[8];         // corner weights in the reference's corner order
+};
+
+constexpr int MAX_GROUPS = 16;  // site groups per wavefront
+constexpr int MAX_DESC = 12;    // 4 vertex slots x 3 channels
+
 struct Lds {
     const double *Os, *ns, *Ot, *nt, *sig, *sigA, *sigD;
     const int *exch, *ov_off;
     const uint32_t *ov;
-    double *scr;  // 64 x 4 doubles of lane exchange scratch
+    double *scr;   // 64 x 4 doubles of lane exchange scratch
+    Desc *desc;    // [group][MAX_DESC]
+    double *bub;   // [group][2][2]: (G, K) of the two bubble legs
 };
+
+// Vertex-slot configuration of an integrand: bits 0-2 = enabled channels (s, t, u; the
+// reference's ch_s/ch_t/ch_u keywords), bit 3 = the slot reads the second action D instead
+// of A.  Slots 0/1 are the left/right vertices on every site, slots 2/3 the local (site 0)
+// vertices of the t channel.  A slot with no channel is the bare vertex (reduced s bubble).
+template <int PASS>
+__device__ __forceinline__ int vertex_cfg(int ch, int slot) {
+    constexpr int D = 8;
+    if (ch == 1) {  // t: flow_su2_t.jl, BSE_su2_t.jl
+        if (PASS == P_LEFT) return slot == 0 ? (5 | D) : slot == 1 ? 7 : slot == 2 ? (3 | D) : 7;
+        if (PASS == P_CENTRAL) return slot == 0 ? 7 : slot == 1 ? (2 | D) : slot == 2 ? 7 : (4 | D);
+        if (PASS == P_BSE) return slot == 0 ? 5 : slot == 1 ? 7 : slot == 2 ? 3 : 7;
+        return 7;
+    }
+    if (ch == 0) {  // s: flow_su2_s.jl, BSE_su2_s.jl, SDE_su2.jl
+        if (PASS == P_LEFT) return slot == 0 ? (6 | D) : 7;
+        if (PASS == P_CENTRAL) return slot == 0 ? 7 : (1 | D);
+        if (PASS == P_BSE) return slot == 0 ? 6 : 7;
+        if (PASS == P_REDUCED) return slot == 0 ? 0 : 7;
+        return 7;
+    }
+    // u: flow_su2_u.jl, BSE_su2_u.jl
+    if (PASS == P_LEFT) return slot == 0 ? (3 | D) : 7;
+    if (PASS == P_CENTRAL) return slot == 0 ? 7 : (4 | D);
+    if (PASS == P_BSE) return slot == 0 ? 3 : 7;
+    return 7;
+}
+
+// Frequency arguments (w, v, v') of the buffer of channel c for vertex slot `slot`, written
+// with the reference's operation order (flow_su2_{s,t,u}.jl:21-39).
+__device__ __forceinline__ void slot_freqs(int ch, int slot, int c, double W, double a1, double a2, double v,
+                                           double &w, double &x, double &y) {
+    if (ch == 0) {  // s = W, vs = a1, vsp = a2
+        const double s = W;
+        if (slot == 0) {
+            if (c == 0) { w = s; x = a1; y = -v; }
+            else if (c == 1) { w = v - a1; x = 0.5 * (s + v + a1); y = 0.5 * (s - v - a1); }
+            else { w = v + a1; x = 0.5 * (s - v + a1); y = 0.5 * (s + v - a1); }
+        } else {
+            if (c == 0) { w = s; x = v; y = a2; }
+            else if (c == 1) { w = -v - a2; x = 0.5 * (s + v - a2); y = 0.5 * (s - v + a2); }
+            else { w = v - a2; x = 0.5 * (s + v + a2); y = 0.5 * (s - v - a2); }
+        }
+    } else if (ch == 2) {  // u = W, vu = a1, vup = a2
+        const double u = W;
+        if (slot == 0) {
+            if (c == 0) { w = v + a1; x = 0.5 * (u - v + a1); y = 0.5 * (-u - v + a1); }
+            else if (c == 1) { w = v - a1; x = 0.5 * (u + v + a1); y = 0.5 * (-u + v + a1); }
+            else { w = u; x = a1; y = v; }
+        } else {
+            if (c == 0) { w = v + a2; x = 0.5 * (u + v - a2); y = 0.5 * (-u + v - a2); }
+            else if (c == 1) { w = -v + a2; x = 0.5 * (u + v + a2); y = 0.5 * (-u + v + a2); }
+            else { w = u; x = v; y = a2; }
+        }
+    } else {  // t = W, vt = a1, vtp = a2
+        const double t = W;
+        if (slot == 0) {
+            if (c == 0) { w = v + a1; x = 0.5 * (-t - v + a1); y = 0.5 * (-t + v - a1); }
+            else if (c == 1) { w = t; x = a1; y = v; }
+            else { w = -v + a1; x = 0.5 * (-t + v + a1); y = 0.5 * (t + v + a1); }
+        } else if (slot == 1) {
+            if (c == 0) { w = v + a2; x = 0.5 * (-t + v - a2); y = 0.5 * (-t - v + a2); }
+            else if (c == 1) { w = t; x = v; y = a2; }
+            else { w = v - a2; x = 0.5 * (-t + v + a2); y = 0.5 * (t + v + a2); }
+        } else if (slot == 2) {
+            if (c == 0) { w = v + a1; x = 0.5 * (t + v - a1); y = 0.5 * (-t + v - a1); }
+            else if (c == 1) { w = -v + a1; x = 0.5 * (t + v + a1); y = 0.5 * (-t + v + a1); }
+            else { w = t; x = v; y = a1; }
+        } else {
+            if (c == 0) { w = v + a2; x = 0.5 * (t - v + a2); y = 0.5 * (-t - v + a2); }
+            else if (c == 1) { w = v - a2; x = 0.5 * (t + v + a2); y = 0.5 * (-t + v + a2); }
+            else { w = t; x = a2; y = v; }
+        }
+    }
+}
+
+// Build the descriptor of one buffer: fold flags, asymptotic kernel, bracketing indices
+// (get_buffer_{s,t,u}, buffer.jl:51-199) and the corner weights/offsets of the
+// interpolation (channel.jl:42-305) with the reference's weight products.
+__device__ __forceinline__ void build_desc(int c, double w, double x, double z, const Meshes &M, const Sys &y,
+                                           Desc &d) {
+    const Buf b = (c == 0) ? buf_s(M, y, w, x, z) : (c == 1) ? buf_t(M, y, w, x, z) : buf_u(M, y, w, x, z);
+    d.meta = b.kernel | (b.exch ? 8 : 0) | (b.map ? 16 : 0) | ((b.smu != b.snu) ? 32 : 0);
+    if (!b.kernel) return;
+    const int ns = y.ns, nO = y.nO, nn = y.nn;
+    const double a1 = b.p1.wl, b1 = b.p1.wh;
+    if (b.kernel == 1) {
+        d.off = ns * b.p1.lo;
+        d.d1 = ns * (b.p1.hi - b.p1.lo);
+        d.w[0] = a1;
+        d.w[1] = b1;
+    } else if (b.kernel == 4) {
+        const double a2 = b.p2.wl, b2 = b.p2.wh, a3 = b.p3.wl, b3 = b.p3.wh;
+        d.off = (int)y.q3 + ns * (b.p1.lo + nO * (b.p2.lo + nn * b.p3.lo));
+        d.d1 = ns * (b.p1.hi - b.p1.lo);
+        d.d2 = ns * nO * (b.p2.hi - b.p2.lo);
+        d.d3 = ns * nO * nn * (b.p3.hi - b.p3.lo);
+        d.w[0] = a1 * a2 * a3; d.w[1] = b1 * a2 * a3; d.w[2] = a1 * b2 * a3; d.w[3] = b1 * b2 * a3;
+        d.w[4] = a1 * a2 * b3; d.w[5] = b1 * a2 * b3; d.w[6] = a1 * b2 * b3; d.w[7] = b1 * b2 * b3;
+    } else {
+        const Param &pv = (b.kernel == 2) ? b.p2 : b.p3;
+        const double a2 = pv.wl, b2 = pv.wh;
+        d.off = (int)((b.kernel == 2) ? y.q21 : y.q22) + ns * (b.p1.lo + nO * pv.lo);
+        d.d1 = ns * (b.p1.hi - b.p1.lo);
+        d.d2 = ns * nO * (pv.hi - pv.lo);
+        d.w[0] = a1 * a2; d.w[1] = b1 * a2; d.w[2] = a1 * b2; d.w[3] = b1 * b2;
+    }
+}
+
+// Both SU(2) components of one channel at one site from a descriptor, corners in the
+// reference's order (first corner initialises, the rest accumulate).  q is the slot's vertex
+// base (wave-uniform); o the 32-bit element offset of the first corner of component 0 and
+// cstep the component stride, so the loads use a scalar base with 32-bit lane offsets.
+template <typename TV>
+__device__ __forceinline__ void gather(const Desc &d, const TV *__restrict__ q, uint32_t o, uint32_t cstep, TV &r0,
+                                       TV &r1) {
+    const int k = d.meta & 7;
+    const uint32_t c1 = (uint32_t)d.d1, c2 = (uint32_t)d.d2, c3 = (uint32_t)d.d3;
+    const uint32_t p = o + cstep;
+    if (k == 4) {
+        const TV x0 = q[o], y0 = q[p], x1 = q[o + c1], y1 = q[p + c1], x2 = q[o + c2], y2 = q[p + c2];
+        const TV x3 = q[o + c1 + c2], y3 = q[p + c1 + c2], x4 = q[o + c3], y4 = q[p + c3];
+        const TV x5 = q[o + c1 + c3], y5 = q[p + c1 + c3], x6 = q[o + c2 + c3], y6 = q[p + c2 + c3];
+        const TV x7 = q[o + c1 + c2 + c3], y7 = q[p + c1 + c2 + c3];
+        TV v0 = (TV)d.w[0] * x0, v1 = (TV)d.w[0] * y0;
+        v0 += (TV)d.w[1] * x1; v1 += (TV)d.w[1] * y1;
+        v0 += (TV)d.w[2] * x2; v1 += (TV)d.w[2] * y2;
+        v0 += (TV)d.w[3] * x3; v1 += (TV)d.w[3] * y3;
+        v0 += (TV)d.w[4] * x4; v1 += (TV)d.w[4] * y4;
+        v0 += (TV)d.w[5] * x5; v1 += (TV)d.w[5] * y5;
+        v0 += (TV)d.w[6] * x6; v1 += (TV)d.w[6] * y6;
+        v0 += (TV)d.w[7] * x7; v1 += (TV)d.w[7] * y7;
+        r0 = v0;
+        r1 = v1;
+    } else if (k == 1) {
+        const TV x0 = q[o], y0 = q[p], x1 = q[o + c1], y1 = q[p + c1];
+        r0 = (TV)d.w[0] * x0;
+        r0 += (TV)d.w[1] * x1;
+        r1 = (TV)d.w[0] * y0;
+        r1 += (TV)d.w[1] * y1;
+    } else {
+        const TV x0 = q[o], y0 = q[p], x1 = q[o + c1], y1 = q[p + c1];
+        const TV x2 = q[o + c2], y2 = q[p + c2], x3 = q[o + c1 + c2], y3 = q[p + c1 + c2];
+        TV v0 = (TV)d.w[0] * x0, v1 = (TV)d.w[0] * y0;
+        v0 += (TV)d.w[1] * x1; v1 += (TV)d.w[1] * y1;
+        v0 += (TV)d.w[2] * x2; v1 += (TV)d.w[2] * y2;
+        v0 += (TV)d.w[3] * x3; v1 += (TV)d.w[3] * y3;
+        r0 = v0;
+        r1 = v1;
+    }
+}
+
+// One leg x of the bubble: G(x) and, for the Katanin bubble, K(x) = S(x) + G(x)² dΣ(x)
+// (bubbles.jl:17-80; S reuses G as the reference's get_S recomputes the same value).
+template <bool KAT>
+__device__ __forceinline__ void bubble_leg(double L, double x, const double *sm, int n, const double *Sig,
+                                           const double *dSig, double &G, double &K) {
+    G = G_full(L, x, sm, n, Sig);
+    K = 0.0;
+    if (KAT) {
+        double S = 0.0;
+        if (fabs(x) > 1e-8) {
+            const double G0 = G_bare(L, x);
+            const double r = G / G0;
+            S = r * r * exp(-(x * x) / (L * L)) * 2.0 * x / (L * L * L);
+        }
+        K = S + G * G * sigma_at(x, sm, n, dSig);
+    }
+}
 
 // Integrand of one element at loop frequency v for site i: returns Γs, Γd (before the
 // dv weight).  Restates compute_{s,t,u}_{kat,left,central}! (flow_lib/flow_su2/*.jl),
 // compute_{s,t,u}_BSE! (parquet_lib/parquet_su2/BSE_su2_*.jl) and compute_s_reduced!
-// (SDE_su2.jl:2-43).  Must be executed by every lane of the wavefront (the t channel
-// exchanges the site vectors of a group through LDS).
+// (SDE_su2.jl:2-43).  The per-node scalar work (two bubble legs, one buffer per vertex
+// slot and channel) is split over the lanes of the group and published in LDS; every lane
+// then gathers its own site.  Must be executed by every lane of the wavefront.
 template <int PASS, typename TV>
 __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, int ch, double W, double vw,
-                                          double vwp, double v, int i, int slot, int gbase, TV &Gs, TV &Gd,
-                                          int &nc) {
+                                          double vwp, double v, int i, int slot, int g, int gbase, bool real_lane,
+                                          TV &Gs, TV &Gd, int &nc) {
     const Sys &y = P.y;
     const Meshes M{S.sig, S.Os, S.ns, S.Ot, S.nt};
-    double x1, x2;
-    bubble_args(ch, W, v, x1, x2);
-    double p;
-    if (PASS == P_KAT) p = bubble_kat(P.L, x1, x2, S.sig, y.nsig, S.sigA, S.sigD);
-    else p = bubble_plain(P.L, x1, x2, S.sig, y.nsig, S.sigA);
-    const TV tp = (TV)p;
-    TV v1s, v1d, v2s, v2d;
-    nc = 0;
-    int nc0 = 0;
-    if (ch == 0) {
-        const double s = W, vs = vw, vsp = vwp;
-        const Buf bs2 = buf_s(M, y, s, v, vsp);
-        const Buf bt2 = (PASS == P_CENTRAL) ? buf_empty() : buf_t(M, y, -v - vsp, 0.5 * (s + v - vsp), 0.5 * (s - v + vsp));
-        const Buf bu2 = (PASS == P_CENTRAL) ? buf_empty() : buf_u(M, y, v - vsp, 0.5 * (s + v + vsp), 0.5 * (s - v - vsp));
-        if (PASS == P_REDUCED) {
-            v1s = (TV)P.A.bare[i];
-            v1d = (TV)P.A.bare[y.ns + i];
-            nc += gamma_site<TV, true, true, true>(P.A, bs2, bt2, bu2, i, S.exch, y, v2s, v2d);
-        } else {
-            const Buf bs1 = (PASS == P_LEFT || PASS == P_BSE) ? buf_empty() : buf_s(M, y, s, vs, -v);
-            const Buf bt1 = buf_t(M, y, v - vs, 0.5 * (s + v + vs), 0.5 * (s - v - vs));
-            const Buf bu1 = buf_u(M, y, v + vs, 0.5 * (s - v + vs), 0.5 * (s + v - vs));
-            if (PASS == P_KAT) {
-                nc += gamma_site<TV, true, true, true>(P.A, bs1, bt1, bu1, i, S.exch, y, v1s, v1d);
-                nc += gamma_site<TV, true, true, true>(P.A, bs2, bt2, bu2, i, S.exch, y, v2s, v2d);
-            } else if (PASS == P_LEFT) {
-                nc += gamma_site<TV, false, true, true>(P.D, bs1, bt1, bu1, i, S.exch, y, v1s, v1d);
-                nc += gamma_site<TV, true, true, true>(P.A, bs2, bt2, bu2, i, S.exch, y, v2s, v2d);
-            } else if (PASS == P_CENTRAL) {
-                nc += gamma_site<TV, true, true, true>(P.A, bs1, bt1, bu1, i, S.exch, y, v1s, v1d);
-                nc += gamma_site<TV, true, false, false>(P.D, bs2, bt2, bu2, i, S.exch, y, v2s, v2d);
-            } else {  // BSE
-                nc += gamma_site<TV, false, true, true>(P.A, bs1, bt1, bu1, i, S.exch, y, v1s, v1d);
-                nc += gamma_site<TV, true, true, true>(P.A, bs2, bt2, bu2, i, S.exch, y, v2s, v2d);
+    const int nslot = (ch == 1) ? 4 : 2;
+    const int ntask = 2 + 3 * nslot;
+    const int gs = real_lane ? g : 0;
+    Desc *dg = S.desc + gs * MAX_DESC;
+    double *bg = S.bub + gs * 4;
+    if (real_lane) {
+#pragma clang loop unroll(disable)
+        for (int t = i; t < ntask; t += y.GS) {
+            if (t < 2) {
+                double x1, x2;
+                bubble_args(ch, W, v, x1, x2);
+                double G, K;
+                bubble_leg<PASS == P_KAT>(P.L, t == 0 ? x1 : x2, S.sig, y.nsig, S.sigA, S.sigD, G, K);
+                bg[2 * t] = G;
+                bg[2 * t + 1] = K;
+            } else {
+                const int k = (t - 2) / 3, c = (t - 2) % 3;
+                const int cfg = vertex_cfg<PASS>(ch, k);
+                Desc &d = dg[t - 2];
+                if ((cfg >> c) & 1) {
+                    double w, x, z;
+                    slot_freqs(ch, k, c, W, vw, vwp, v, w, x, z);
+                    build_desc(c, w, x, z, M, y, d);
+                } else {
+                    d.meta = 0;
+                }
             }
         }
+    }
+    __syncthreads();
+    const double G1 = bg[0], K1 = bg[1], G2 = bg[2], K2 = bg[3];
+    const double p = (PASS == P_KAT) ? K1 * G2 / (2.0 * PI_D) + K2 * G1 / (2.0 * PI_D) : -(G1 * G2 / (2.0 * PI_D));
+    const TV tp = (TV)p;
+    TV v1s = 0, v1d = 0, v2s = 0, v2d = 0, v3s = 0, v3d = 0, v4s = 0, v4d = 0;
+    nc = 0;
+    int nc0 = 0;
+#pragma clang loop unroll(disable)
+    for (int k = 0; k < nslot; ++k) {
+        const int cfg = vertex_cfg<PASS>(ch, k);
+        const ActV<TV> &V = (cfg & 8) ? P.D : P.A;
+        const int site = (k < 2) ? i : 0;
+        TV vs = (TV)V.bare[site], vd = (TV)V.bare[y.ns + site];
+        int cn = 0;
+#pragma clang loop unroll(disable)
+        for (int c = 0; c < 3; ++c) {
+            const Desc &d = dg[3 * k + c];
+            const int meta = d.meta;
+            if (!(meta & 7)) continue;
+            const int sx = (meta & 8) ? S.exch[site] : site;
+            const bool map = (meta & 16) != 0;
+            const int chn = (c == 1) ? 1 : ((c == 0) == map ? 2 : 0);
+            TV r0, r1;
+            gather<TV>(d, V.gamma, (uint32_t)(chn * y.chlen + d.off + sx), (uint32_t)y.complen, r0, r1);
+            vs += r0;
+            vd += (meta & 32) ? -r1 : r1;
+            cn += corners(meta & 7);
+        }
+        if (k == 0) { v1s = vs; v1d = vd; nc += cn; }
+        else if (k == 1) { v2s = vs; v2d = vd; nc += cn; }
+        else if (k == 2) { v3s = vs; v3d = vd; nc0 += cn; }
+        else { v4s = vs; v4d = vd; nc0 += cn; }
+    }
+    if (ch == 0) {
         Gs = -tp * ((TV)-2.0 * v1s * v2s + v1s * v2d + v1d * v2s);
         Gd = -tp * ((TV)3.0 * v1s * v2s + v1d * v2d);
     } else if (ch == 2) {
-        const double u = W, vu = vw, vup = vwp;
-        const Buf bs1 = buf_s(M, y, v + vu, 0.5 * (u - v + vu), 0.5 * (-u - v + vu));
-        const Buf bt1 = buf_t(M, y, v - vu, 0.5 * (u + v + vu), 0.5 * (-u + v + vu));
-        const Buf bu1 = (PASS == P_LEFT || PASS == P_BSE) ? buf_empty() : buf_u(M, y, u, vu, v);
-        const Buf bs2 = (PASS == P_CENTRAL) ? buf_empty() : buf_s(M, y, v + vup, 0.5 * (u + v - vup), 0.5 * (-u + v - vup));
-        const Buf bt2 = (PASS == P_CENTRAL) ? buf_empty() : buf_t(M, y, -v + vup, 0.5 * (u + v + vup), 0.5 * (-u + v + vup));
-        const Buf bu2 = buf_u(M, y, u, v, vup);
-        if (PASS == P_KAT) {
-            nc += gamma_site<TV, true, true, true>(P.A, bs1, bt1, bu1, i, S.exch, y, v1s, v1d);
-            nc += gamma_site<TV, true, true, true>(P.A, bs2, bt2, bu2, i, S.exch, y, v2s, v2d);
-        } else if (PASS == P_LEFT) {
-            nc += gamma_site<TV, true, true, false>(P.D, bs1, bt1, bu1, i, S.exch, y, v1s, v1d);
-            nc += gamma_site<TV, true, true, true>(P.A, bs2, bt2, bu2, i, S.exch, y, v2s, v2d);
-        } else if (PASS == P_CENTRAL) {
-            nc += gamma_site<TV, true, true, true>(P.A, bs1, bt1, bu1, i, S.exch, y, v1s, v1d);
-            nc += gamma_site<TV, false, false, true>(P.D, bs2, bt2, bu2, i, S.exch, y, v2s, v2d);
-        } else {
-            nc += gamma_site<TV, true, true, false>(P.A, bs1, bt1, bu1, i, S.exch, y, v1s, v1d);
-            nc += gamma_site<TV, true, true, true>(P.A, bs2, bt2, bu2, i, S.exch, y, v2s, v2d);
-        }
         Gs = -tp * ((TV)2.0 * v1s * v2s + v1s * v2d + v1d * v2s);
         Gd = -tp * ((TV)3.0 * v1s * v2s + v1d * v2d);
     } else {
-        const double t = W, vt = vw, vtp = vwp;
-        const Buf bs1 = buf_s(M, y, v + vt, 0.5 * (-t - v + vt), 0.5 * (-t + v - vt));
-        const Buf bt1 = (PASS == P_LEFT || PASS == P_BSE) ? buf_empty() : buf_t(M, y, t, vt, v);
-        const Buf bu1 = buf_u(M, y, -v + vt, 0.5 * (-t + v + vt), 0.5 * (t + v + vt));
-        const Buf bs2 = (PASS == P_CENTRAL) ? buf_empty() : buf_s(M, y, v + vtp, 0.5 * (-t + v - vtp), 0.5 * (-t - v + vtp));
-        const Buf bt2 = buf_t(M, y, t, v, vtp);
-        const Buf bu2 = (PASS == P_CENTRAL) ? buf_empty() : buf_u(M, y, v - vtp, 0.5 * (-t + v + vtp), 0.5 * (t + v + vtp));
-        const Buf bs3 = buf_s(M, y, v + vt, 0.5 * (t + v - vt), 0.5 * (-t + v - vt));
-        const Buf bt3 = buf_t(M, y, -v + vt, 0.5 * (t + v + vt), 0.5 * (-t + v + vt));
-        const Buf bu3 = (PASS == P_LEFT || PASS == P_BSE) ? buf_empty() : buf_u(M, y, t, v, vt);
-        const Buf bs4 = (PASS == P_CENTRAL) ? buf_empty() : buf_s(M, y, v + vtp, 0.5 * (t - v + vtp), 0.5 * (-t - v + vtp));
-        const Buf bt4 = (PASS == P_CENTRAL) ? buf_empty() : buf_t(M, y, v - vtp, 0.5 * (t + v + vtp), 0.5 * (-t + v + vtp));
-        const Buf bu4 = buf_u(M, y, t, vtp, v);
-        TV v3s, v3d, v4s, v4d;
-        if (PASS == P_KAT) {
-            nc0 += gamma_site<TV, true, true, true>(P.A, bs3, bt3, bu3, 0, S.exch, y, v3s, v3d);
-            nc0 += gamma_site<TV, true, true, true>(P.A, bs4, bt4, bu4, 0, S.exch, y, v4s, v4d);
-            nc += gamma_site<TV, true, true, true>(P.A, bs1, bt1, bu1, i, S.exch, y, v1s, v1d);
-            nc += gamma_site<TV, true, true, true>(P.A, bs2, bt2, bu2, i, S.exch, y, v2s, v2d);
-        } else if (PASS == P_LEFT) {
-            nc0 += gamma_site<TV, true, true, false>(P.D, bs3, bt3, bu3, 0, S.exch, y, v3s, v3d);
-            nc0 += gamma_site<TV, true, true, true>(P.A, bs4, bt4, bu4, 0, S.exch, y, v4s, v4d);
-            nc += gamma_site<TV, true, false, true>(P.D, bs1, bt1, bu1, i, S.exch, y, v1s, v1d);
-            nc += gamma_site<TV, true, true, true>(P.A, bs2, bt2, bu2, i, S.exch, y, v2s, v2d);
-        } else if (PASS == P_CENTRAL) {
-            nc0 += gamma_site<TV, true, true, true>(P.A, bs3, bt3, bu3, 0, S.exch, y, v3s, v3d);
-            nc0 += gamma_site<TV, false, false, true>(P.D, bs4, bt4, bu4, 0, S.exch, y, v4s, v4d);
-            nc += gamma_site<TV, true, true, true>(P.A, bs1, bt1, bu1, i, S.exch, y, v1s, v1d);
-            nc += gamma_site<TV, false, true, false>(P.D, bs2, bt2, bu2, i, S.exch, y, v2s, v2d);
-        } else {
-            nc0 += gamma_site<TV, true, true, false>(P.A, bs3, bt3, bu3, 0, S.exch, y, v3s, v3d);
-            nc0 += gamma_site<TV, true, true, true>(P.A, bs4, bt4, bu4, 0, S.exch, y, v4s, v4d);
-            nc += gamma_site<TV, true, false, true>(P.A, bs1, bt1, bu1, i, S.exch, y, v1s, v1d);
-            nc += gamma_site<TV, true, true, true>(P.A, bs2, bt2, bu2, i, S.exch, y, v2s, v2d);
-        }
-        TV gs = -tp * ((TV)-1.0 * v1s * v4s + v1s * v4d - (TV)1.0 * v3s * v2s + v3d * v2s);
-        TV gd = -tp * ((TV)3.0 * v1d * v4s + v1d * v4d + (TV)3.0 * v3s * v2d + v3d * v2d);
+        TV gsum = -tp * ((TV)-1.0 * v1s * v4s + v1s * v4d - (TV)1.0 * v3s * v2s + v3d * v2s);
+        TV gdsum = -tp * ((TV)3.0 * v1d * v4s + v1d * v4d + (TV)3.0 * v3s * v2d + v3d * v2d);
         // overlap gather Σ_(j1,j2,m) -p (-2) m (2S) v1[j1] v2[j2]  (flow_su2_t.jl:59-74)
         double *sc = S.scr;
         sc[4 * slot + 0] = (double)v1s;
@@ -164,7 +325,7 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
         sc[4 * slot + 2] = (double)v2s;
         sc[4 * slot + 3] = (double)v2d;
         __syncthreads();
-        const TV c = -tp * (TV)(-2.0);
+        const TV cc = -tp * (TV)(-2.0);
         const TV S2 = (TV)y.S2;
         const int r0 = S.ov_off[i], r1 = S.ov_off[i + 1];
         for (int r = r0; r < r1; ++r) {
@@ -173,14 +334,15 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
             const TV m = (TV)(double)(row >> 16);
             const double *e1 = sc + 4 * (gbase + j1);
             const double *e2 = sc + 4 * (gbase + j2);
-            gs += c * m * S2 * (TV)e1[0] * (TV)e2[2];
-            gd += c * m * S2 * (TV)e1[1] * (TV)e2[3];
+            gsum += cc * m * S2 * (TV)e1[0] * (TV)e2[2];
+            gdsum += cc * m * S2 * (TV)e1[1] * (TV)e2[3];
         }
-        __syncthreads();
-        Gs = gs;
-        Gd = gd;
+        Gs = gsum;
+        Gd = gdsum;
         if (i == 0) nc += nc0;  // site-0 (local) vertices are gathered once per node
     }
+    // the descriptors and bubble legs of this node are read; the next node may overwrite them
+    __syncthreads();
 }
 
 // element code: ch 2 bits | kernel 3 bits | w1 7 bits | w2 7 bits | w3 7 bits
@@ -192,8 +354,8 @@ __device__ __forceinline__ void decode(uint32_t e, int &ch, int &kernel, int &w1
     w3 = (e >> 19) & 127;
 }
 
-// node position a + c*h rounded as the reference computes it (product, then sum)
-// (the pragma keeps hipcc from fusing it into an FMA; __dmul_rn/__dadd_rn are fused on gfx950)
+// node position a + c*h rounded as the reference computes it (product, then sum); the
+// pragma keeps hipcc from fusing it into an FMA (__dmul_rn/__dadd_rn are fused on gfx950)
 __device__ __forceinline__ double node_at(double a, double c, double h) {
 #pragma clang fp contract(off)
     return a + c * h;
@@ -226,7 +388,7 @@ __device__ __forceinline__ bool subinterval(int j, int eval, double ref, const d
 }
 
 template <int PASS, typename TV>
-__global__ __launch_bounds__(64) void k_vertex_pass(PassArgs<TV> P) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_WAVES_PER_EU))) void k_vertex_pass(PassArgs<TV> P) {
     extern __shared__ uint32_t s_dyn[];  // overlap rows then ov_off
     __shared__ double s_mesh[5 * MAXN];
     __shared__ double s_sigA[MAXN], s_sigD[MAXN];
@@ -248,6 +410,7 @@ __global__ __launch_bounds__(64) void k_vertex_pass(PassArgs<TV> P) {
     }
     if (lane < y.ns) s_exch[lane] = P.exch[lane];
     const int n_ov = P.ov_off[y.ns];
+    __shared__ double s_bub[MAX_GROUPS * 4];
     for (int k = lane; k < n_ov; k += 64) s_dyn[k] = P.ov_rows[k];
     for (int k = lane; k <= y.ns; k += 64) s_dyn[n_ov + k] = (uint32_t)P.ov_off[k];
     if (lane == 0) { s_cnt = 0; s_bytes = 0; }
@@ -265,6 +428,9 @@ __global__ __launch_bounds__(64) void k_vertex_pass(PassArgs<TV> P) {
     S.ov = s_dyn;
     S.ov_off = reinterpret_cast<const int *>(s_dyn + n_ov);
     S.scr = s_scr;
+    // descriptors follow the overlap table in the dynamic LDS block, 16-byte aligned
+    S.desc = reinterpret_cast<Desc *>(s_dyn + ((n_ov + y.ns + 1 + 3) & ~3));
+    S.bub = s_bub;
 
     int ch, kernel, w1, w2, w3;
     decode(P.dbg_v ? P.dbg_elem : P.elems[e_idx], ch, kernel, w1, w2, w3);
@@ -292,67 +458,58 @@ __global__ __launch_bounds__(64) void k_vertex_pass(PassArgs<TV> P) {
     const int gbase = real_lane ? g * GS : 0;
     const int nsub = 4 * P.eval;
     const int rounds = (nsub + G - 1) / G;
+    const double c0 = P.corrs[0 + 2 * (ch + 3 * w1)], c1 = P.corrs[1 + 2 * (ch + 3 * w1)];
 
-    if (P.dbg_v) {
-        TV Gs, Gd;
-        int ncd;
-        node_eval<PASS, TV>(P, S, ch, W, vw, vwp, P.dbg_v[bid], i, lane, gbase, Gs, Gd, ncd);
-        if (g == 0) {
-            P.dbg_out[(2 * bid + 0) * y.ns + i] = (double)Gs;
-            P.dbg_out[(2 * bid + 1) * y.ns + i] = (double)Gd;
-        }
-        return;
-    }
+    // Wave-uniform state machine; the groups of a wavefront advance in lockstep (they start
+    // a round together and, while refining, all sit at the same level n), so a single
+    // node_eval call site serves the probe, the Simpson stages and the tail corrections.
+    enum { M_ROUND, M_NODES, M_CHECK, M_TAIL, M_PROBE };
+    int mode = P.dbg_v ? M_PROBE : M_ROUND;
+    int r = 0, k = 0, nk = 0, j = 0;
+    long n = 4;
+    bool init = true, act = false, done = true;
+    double a = 0.0, b = 0.0, sgn = 1.0, h = 0.0;
+    double b10 = 0.0, b11 = 0.0, b20 = 0.0, b21 = 0.0, res0 = 0.0, res1 = 0.0;
     double acc0 = 0.0, acc1 = 0.0;
-    unsigned long long nodes = 0, corner_sum = 0;
-    int nc = 0;
-
-    for (int r = 0; r < rounds; ++r) {
-        const int j = r * G + g;
-        double a = 0.0, b = 0.0, sgn = 1.0;
-        bool act = real_lane && j < nsub && subinterval(j, P.eval, ref, logpts, a, b, sgn);
-        if (__ballot(act) == 0) continue;  // wave-uniform skip of empty rounds
-        double b10 = 0.0, b11 = 0.0, b20 = 0.0, b21 = 0.0;
-        const unsigned long long nodes_before = nodes;
-        // initial Simpson S1 on (a, m, b) and S2 (simps!, quadrature.jl:57-68)
-        for (int k = 0; k < 6; ++k) {
-            double x, dx;
-            if (k == 0) { x = a; dx = 1.0 * (b - a) / 6.0; }
-            else if (k == 1) { x = 0.5 * (a + b); dx = 2.0 * (b - a) / 3.0; }
-            else if (k == 2) { x = b; dx = 1.0 * (b - a) / 6.0; }
-            else {
-                const double h = 0.5 * (b - a) / 2;
-                if (k == 3) { x = node_at(a, 2.0 * 1 - 1.0, h); dx = +4.0 * h / 3.0; }
-                else if (k == 4) { x = node_at(a, 2.0 * 2 - 1.0, h); dx = +4.0 * h / 3.0; }
-                else { x = node_at(a, 4.0 * 1 - 2.0, h); dx = -2.0 * h / 3.0; }
+    unsigned long long nodes = 0, corner_sum = 0, nodes_before = 0;
+    while (true) {
+        if (mode == M_ROUND) {
+            if (r >= rounds) {
+                mode = M_TAIL;
+                k = 0;
+                nk = 2;
+                continue;
             }
-            if (k == 3) { b20 = 0.5 * b10; b21 = 0.5 * b11; }
-            TV Gs, Gd;
-            node_eval<PASS, TV>(P, S, ch, W, vw, vwp, act ? sgn * x : 0.0, i, lane, gbase, Gs, Gd, nc);
-            if (act) {
-                if (k < 3) { b10 += dx * (double)Gs; b11 += dx * (double)Gd; }
-                else { b20 += dx * (double)Gs; b21 += dx * (double)Gd; }
-                ++nodes;
-                corner_sum += nc;
+            j = r * G + g;
+            act = real_lane && j < nsub && subinterval(j, P.eval, ref, logpts, a, b, sgn);
+            if (__ballot(act) == 0) {  // wave-uniform skip of empty rounds
+                ++r;
+                continue;
             }
+            b10 = b11 = b20 = b21 = res0 = res1 = 0.0;
+            nodes_before = nodes;
+            done = !act;
+            init = true;
+            n = 4;
+            k = 0;
+            nk = 6;
+            mode = M_NODES;
+            continue;
         }
-        bool done = !act;
-        double res0 = 0.0, res1 = 0.0;
-        long n = 4;
-        while (true) {
+        if (mode == M_CHECK) {
             // Richardson extrapolation and error estimate (quadrature.jl:72-88)
             double r10 = b10 * (-1.0 / 15.0), r11 = b11 * (-1.0 / 15.0);
             r10 += 16.0 / 15.0 * b20;
             r11 += 16.0 / 15.0 * b21;
-            s_scr[4 * lane + 0] = r10;
-            s_scr[4 * lane + 1] = r11;
-            s_scr[4 * lane + 2] = b20;
-            s_scr[4 * lane + 3] = b21;
+            S.scr[4 * lane + 0] = r10;
+            S.scr[4 * lane + 1] = r11;
+            S.scr[4 * lane + 2] = b20;
+            S.scr[4 * lane + 3] = b21;
             __syncthreads();
             double q1 = 0.0, q2 = 0.0, q3 = 0.0;
             if (real_lane) {
                 for (int jj = 0; jj < y.ns; ++jj) {
-                    const double *e = s_scr + 4 * (gbase + jj);
+                    const double *e = S.scr + 4 * (gbase + jj);
                     const double x0 = e[0], x1 = e[1], z0 = e[2], z1 = e[3];
                     q1 += x0 * x0; q1 += x1 * x1;
                     q2 += z0 * z0; q2 += z1 * z1;
@@ -374,58 +531,93 @@ __global__ __launch_bounds__(64) void k_vertex_pass(PassArgs<TV> P) {
                     b20 = 0.5 * b10; b21 = 0.5 * b11;
                 }
             }
-            if (__ballot(!done) == 0) break;
-            // residual! with n new and n/2 re-weighted nodes (quadrature.jl:2-24)
-            const double h = 0.5 * (b - a) / n;
-            const long nk = n + n / 2;
-            for (long k = 0; k < nk; ++k) {
-                double x, dx;
-                if (k < n) { x = node_at(a, 2.0 * (k + 1) - 1.0, h); dx = +4.0 * h / 3.0; }
-                else { x = node_at(a, 4.0 * (k - n + 1) - 2.0, h); dx = -2.0 * h / 3.0; }
-                TV Gs, Gd;
-                node_eval<PASS, TV>(P, S, ch, W, vw, vwp, done ? 0.0 : sgn * x, i, lane, gbase, Gs, Gd, nc);
-                if (!done) {
-                    b20 += dx * (double)Gs;
-                    b21 += dx * (double)Gd;
-                    ++nodes;
-                    corner_sum += nc;
+            if (__ballot(!done) != 0) {
+                // residual! with n new and n/2 re-weighted nodes (quadrature.jl:2-24)
+                h = 0.5 * (b - a) / n;
+                init = false;
+                k = 0;
+                nk = (int)(n + n / 2);
+                mode = M_NODES;
+                continue;
+            }
+            if (P.dbg_sub_nodes && act && i == 0 && (long)e_idx * nsub + j < P.dbg_cap) {
+                P.dbg_sub_nodes[(long)e_idx * nsub + j] = (int)(nodes - nodes_before);
+                P.dbg_sub_res[(long)e_idx * nsub + j] = res0;
+            }
+            // ordered accumulation of this round's sub-interval results
+            S.scr[2 * lane + 0] = res0;
+            S.scr[2 * lane + 1] = res1;
+            __syncthreads();
+            for (int gg = 0; gg < G; ++gg) {
+                const int jj = r * G + gg;
+                double ta, tb, ts;
+                if (jj < nsub && subinterval(jj, P.eval, ref, logpts, ta, tb, ts)) {
+                    acc0 += S.scr[2 * (gg * GS + i) + 0];
+                    acc1 += S.scr[2 * (gg * GS + i) + 1];
                 }
             }
-            n *= 2;
+            __syncthreads();
+            ++r;
+            mode = M_ROUND;
+            continue;
         }
-        if (P.dbg_sub_nodes && act && i == 0 && (long)e_idx * nsub + j < P.dbg_cap) {
-            P.dbg_sub_nodes[(long)e_idx * nsub + j] = (int)(nodes - nodes_before);
-            P.dbg_sub_res[(long)e_idx * nsub + j] = res0;
-        }
-        // ordered accumulation of this round's sub-interval results
-        s_scr[2 * lane + 0] = res0;
-        s_scr[2 * lane + 1] = res1;
-        __syncthreads();
-        for (int gg = 0; gg < G; ++gg) {
-            const int jj = r * G + gg;
-            double ta, tb, ts;
-            if (jj < nsub && subinterval(jj, P.eval, ref, logpts, ta, tb, ts)) {
-                acc0 += s_scr[2 * (gg * GS + i) + 0];
-                acc1 += s_scr[2 * (gg * GS + i) + 1];
+        // one node per group: Simpson stages, tail corrections or the probe
+        double x = 0.0, dx = 0.0;
+        bool use = false;
+        if (mode == M_PROBE) {
+            x = P.dbg_v[bid];
+            use = true;
+        } else if (mode == M_TAIL) {
+            x = (k == 0) ? -val : val;  // boundary corrections (flow_s.jl:37-38)
+            dx = (k == 0) ? c0 : c1;
+            use = true;
+        } else if (init) {
+            // initial Simpson S1 on (a, m, b) and S2 (simps!, quadrature.jl:57-68)
+            if (k == 0) { x = a; dx = 1.0 * (b - a) / 6.0; }
+            else if (k == 1) { x = 0.5 * (a + b); dx = 2.0 * (b - a) / 3.0; }
+            else if (k == 2) { x = b; dx = 1.0 * (b - a) / 6.0; }
+            else {
+                const double h2 = 0.5 * (b - a) / 2;
+                if (k == 3) { x = node_at(a, 2.0 * 1 - 1.0, h2); dx = +4.0 * h2 / 3.0; }
+                else if (k == 4) { x = node_at(a, 2.0 * 2 - 1.0, h2); dx = +4.0 * h2 / 3.0; }
+                else { x = node_at(a, 4.0 * 1 - 2.0, h2); dx = -2.0 * h2 / 3.0; }
             }
+            if (k == 3) { b20 = 0.5 * b10; b21 = 0.5 * b11; }
+            x *= sgn;
+            use = act;
+        } else {
+            if (k < n) { x = node_at(a, 2.0 * (k + 1) - 1.0, h); dx = +4.0 * h / 3.0; }
+            else { x = node_at(a, 4.0 * (k - n + 1) - 2.0, h); dx = -2.0 * h / 3.0; }
+            x *= sgn;
+            use = !done;
         }
-        __syncthreads();
-    }
-    // boundary corrections (flow_s.jl:37-38)
-    {
-        const double c0 = P.corrs[0 + 2 * (ch + 3 * w1)], c1 = P.corrs[1 + 2 * (ch + 3 * w1)];
         TV Gs, Gd;
-        int nc1;
-        node_eval<PASS, TV>(P, S, ch, W, vw, vwp, -val, i, lane, gbase, Gs, Gd, nc1);
-        acc0 += c0 * (double)Gs;
-        acc1 += c0 * (double)Gd;
-        int nc2;
-        node_eval<PASS, TV>(P, S, ch, W, vw, vwp, val, i, lane, gbase, Gs, Gd, nc2);
-        acc0 += c1 * (double)Gs;
-        acc1 += c1 * (double)Gd;
-        if (g == 0) {
-            nodes += 2;
-            corner_sum += nc1 + nc2;
+        int nc;
+        node_eval<PASS, TV>(P, S, ch, W, vw, vwp, use ? x : 0.0, i, lane, g, gbase, real_lane, Gs, Gd, nc);
+        if (mode == M_PROBE) {
+            if (g == 0) {
+                P.dbg_out[(2 * bid + 0) * y.ns + i] = (double)Gs;
+                P.dbg_out[(2 * bid + 1) * y.ns + i] = (double)Gd;
+            }
+            return;
+        }
+        if (mode == M_TAIL) {
+            acc0 += dx * (double)Gs;
+            acc1 += dx * (double)Gd;
+            if (g == 0) {
+                ++nodes;
+                corner_sum += nc;
+            }
+        } else if (use) {
+            if (init && k < 3) { b10 += dx * (double)Gs; b11 += dx * (double)Gd; }
+            else { b20 += dx * (double)Gs; b21 += dx * (double)Gd; }
+            ++nodes;
+            corner_sum += nc;
+        }
+        if (++k == nk) {
+            if (mode == M_TAIL) break;
+            if (!init) n *= 2;
+            mode = M_CHECK;
         }
     }
     // write the site vector (flow_s.jl:41-51)

@@ -13,7 +13,7 @@ import numpy as np
 
 from .action import Action
 
-LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libpffrg.so"
+LIB_PATH = Path(os.environ.get("PFFRG_LIB", Path(__file__).resolve().parent / "_lib" / "libpffrg.so"))
 
 FP64 = 0
 FP32_MIXED = 1
