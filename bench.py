@@ -140,12 +140,25 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (3 times), the known-size reference "
+                         "for scaling hardware byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is a design note, "
               f"not part of this build", file=sys.stderr)
         sys.exit(2)
     eval_ = 10
+    if args.copy_calibration > 0:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 x {args.copy_calibration} MiB device copies")
+        del src, dst
     t0 = time.time()
     prob = build_problem(args.L, args.N)
     ns, m = prob.r.n_sites, prob.mesh

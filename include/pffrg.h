@@ -131,6 +131,10 @@ int pffrg_get_corrs(pffrg_ctx *ctx, double *corrs);
 int pffrg_probe_integrand(pffrg_ctx *ctx, int32_t pass, double Lambda, int32_t a, int32_t da, int32_t ch,
                           int32_t kernel, int32_t w1, int32_t w2, int32_t w3, int32_t n, const double *v,
                           double *out);
+/* arm a one-shot recording for the next vertex pass: per (element, sub-interval) quadrature
+ * node counts and the site-0 spin result, written to the host arrays when that pass ends
+ * (capacity entries each, indexed element * 4*eval + sub-interval); diagnostics only */
+int pffrg_debug_record(pffrg_ctx *ctx, int32_t *sub_nodes, double *sub_res, int64_t capacity);
 /* enable = 1: time every vertex-pass launch with HIP events on the context stream */
 int pffrg_set_timing(pffrg_ctx *ctx, int32_t enable);
 /* total milliseconds and launch count of the vertex-pass kernel since the last reset */

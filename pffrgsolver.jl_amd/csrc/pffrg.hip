@@ -118,7 +118,7 @@ __global__ __launch_bounds__(64) void k_sigma(SigmaArgs Q) {
         s_mult[lane] = Q.mult[lane];
     }
     __syncthreads();
-    const Meshes M{s_mesh, s_mesh + MAXN, s_mesh + 2 * MAXN, s_mesh + 3 * MAXN, s_mesh + 4 * MAXN};
+    const Meshes<> M{s_mesh, s_mesh + MAXN, s_mesh + 2 * MAXN, s_mesh + 3 * MAXN, s_mesh + 4 * MAXN};
     const int si = blockIdx.x + 1;
     const double w = s_mesh[si];
     const double L = Q.L;
@@ -126,9 +126,9 @@ __global__ __launch_bounds__(64) void k_sigma(SigmaArgs Q) {
         double val;
         if (KIND == SK_SDE) {
             const double INF = __builtin_huge_val();
-            const Buf b1s = buf_s(M, y, v + w, INF, 0.5 * (v - w));
-            const Buf b2s = buf_s(M, y, v + w, INF, 0.5 * (-v + w));
-            const Buf e = buf_empty();
+            const Buf<> b1s = buf_s(M, y, v + w, INF, 0.5 * (v - w));
+            const Buf<> b2s = buf_s(M, y, v + w, INF, 0.5 * (-v + w));
+            const Buf<> e = buf_empty<double>();
             double s0, d0;
             gamma_site<double, true, false, false>(Q.V, b2s, e, e, 0, s_exch, y, s0, d0);
             val = 3.0 * s0 + d0;
@@ -140,12 +140,12 @@ __global__ __launch_bounds__(64) void k_sigma(SigmaArgs Q) {
             val *= -G_full(L, v, s_mesh, y.nsig, s_sA) / (2.0 * PI_D);
             return val;
         }
-        const Buf b1s = buf_s(M, y, v + w, 0.5 * (-v + w), 0.5 * (v - w));
-        const Buf b1t = buf_t(M, y, 0.0, w, v);
-        const Buf b1u = buf_u(M, y, -v + w, 0.5 * (v + w), 0.5 * (v + w));
-        const Buf b2s = buf_s(M, y, v + w, 0.5 * (-v + w), 0.5 * (-v + w));
-        const Buf b2t = buf_t(M, y, v - w, 0.5 * (v + w), 0.5 * (v + w));
-        const Buf b2u = buf_u(M, y, 0.0, w, v);
+        const Buf<> b1s = buf_s(M, y, v + w, 0.5 * (-v + w), 0.5 * (v - w));
+        const Buf<> b1t = buf_t(M, y, 0.0, w, v);
+        const Buf<> b1u = buf_u(M, y, -v + w, 0.5 * (v + w), 0.5 * (v + w));
+        const Buf<> b2s = buf_s(M, y, v + w, 0.5 * (-v + w), 0.5 * (-v + w));
+        const Buf<> b2t = buf_t(M, y, v - w, 0.5 * (v + w), 0.5 * (v + w));
+        const Buf<> b2u = buf_u(M, y, 0.0, w, v);
         double s0, d0;
         if (KIND == SK_CORR1) gamma_site<double, true, true, false>(Q.V, b2s, b2t, b2u, 0, s_exch, y, s0, d0);
         else gamma_site<double, true, true, true>(Q.A, b2s, b2t, b2u, 0, s_exch, y, s0, d0);

@@ -24,25 +24,29 @@ struct Sys {
     double valS, valT;    // Ω_max + ν_max of the s/u and t meshes
 };
 
-// Mesh views; in the pass kernels they point into LDS.
+// Mesh views; in the pass kernels they point into LDS.  ST is the scalar type of the
+// per-node work (double for the FP64 path, float for the FP32-mixed path).
+template <typename ST = double>
 struct Meshes {
-    const double *sig, *Os, *ns, *Ot, *nt;
+    const ST *sig, *Os, *ns, *Ot, *nt;
 };
 
+template <typename ST = double>
 struct Param {
     int lo, hi;
-    double wl, wh;
+    ST wl, wh;
 };
 
 // get_indices/get_param semantics (Frequency/param.jl:26-86) with a bisection over the
 // LDS-resident mesh instead of the reference's linear scan: upper = first index with
 // mesh[idx] >= val, lower = upper unless val < mesh[upper]; clamp to the last point.
-__device__ __forceinline__ Param find_param(double val, const double *m, int n) {
-    Param p;
+template <typename ST>
+__device__ __forceinline__ Param<ST> find_param(ST val, const ST *m, int n) {
+    Param<ST> p;
     if (val >= m[n - 1]) {
         p.lo = p.hi = n - 1;
-        p.wl = 1.0;
-        p.wh = 0.0;
+        p.wl = (ST)1;
+        p.wh = (ST)0;
         return p;
     }
     int l = -1, h = n - 1;
@@ -54,35 +58,38 @@ __device__ __forceinline__ Param find_param(double val, const double *m, int n) 
     p.hi = h;
     p.lo = (val < m[h]) ? h - 1 : h;
     if (p.lo < p.hi) {
-        double d = 1.0 / (m[p.hi] - m[p.lo]);
+        const ST d = (ST)1 / (m[p.hi] - m[p.lo]);
         p.wl = d * (m[p.hi] - val);
         p.wh = d * (val - m[p.lo]);
     } else {
-        p.wl = 1.0;
-        p.wh = 0.0;
+        p.wl = (ST)1;
+        p.wh = (ST)0;
     }
     return p;
 }
 
 // Access buffer of Frequency/buffer.jl:16-26: kernel 0 = nothing, 1 = q1, 2 = q2_1, 3 = q2_2, 4 = q3.
+template <typename ST = double>
 struct Buf {
     int kernel;
     bool exch, map, smu, snu;
-    Param p1, p2, p3;
+    Param<ST> p1, p2, p3;
 };
 
-__device__ __forceinline__ Buf buf_empty() {
-    Buf b;
+template <typename ST = double>
+__device__ __forceinline__ Buf<ST> buf_empty() {
+    Buf<ST> b;
     b.kernel = 0;
     b.exch = b.map = b.smu = b.snu = false;
-    b.p1 = b.p2 = b.p3 = Param{0, 0, 0.0, 0.0};
+    b.p1 = b.p2 = b.p3 = Param<ST>{0, 0, (ST)0, (ST)0};
     return b;
 }
 
 // get_buffer, Frequency/buffer.jl:51-86
-__device__ __forceinline__ Buf make_buf(double w, double v, double vp, const double *Om, const double *nu, int nO,
-                                        int nn, bool ex, bool mp, bool smu, bool snu) {
-    Buf b = buf_empty();
+template <typename ST>
+__device__ __forceinline__ Buf<ST> make_buf(ST w, ST v, ST vp, const ST *Om, const ST *nu, int nO, int nn, bool ex,
+                                            bool mp, bool smu, bool snu) {
+    Buf<ST> b = buf_empty<ST>();
     if (Om[nO - 1] < fabs(w)) return b;
     b.exch = ex;
     b.map = mp;
@@ -111,29 +118,32 @@ __device__ __forceinline__ Buf make_buf(double w, double v, double vp, const dou
 }
 
 // get_buffer_s (buffer.jl:88-124)
-__device__ __forceinline__ Buf buf_s(const Meshes &M, const Sys &y, double w, double v, double vp) {
+template <typename ST>
+__device__ __forceinline__ Buf<ST> buf_s(const Meshes<ST> &M, const Sys &y, ST w, ST v, ST vp) {
     bool ex = false, mp = false, smu = false, snu = false;
-    if (w < 0.0) { w = -w; ex = !ex; }
-    if (v < 0.0) { v = -v; ex = !ex; mp = !mp; smu = !smu; }
-    if (vp < 0.0) { vp = -vp; mp = !mp; snu = !snu; }
+    if (w < (ST)0) { w = -w; ex = !ex; }
+    if (v < (ST)0) { v = -v; ex = !ex; mp = !mp; smu = !smu; }
+    if (vp < (ST)0) { vp = -vp; mp = !mp; snu = !snu; }
     return make_buf(w, v, vp, M.Os, M.ns, y.nO, y.nn, ex, mp, smu, snu);
 }
 
 // get_buffer_t (buffer.jl:127-160)
-__device__ __forceinline__ Buf buf_t(const Meshes &M, const Sys &y, double w, double v, double vp) {
+template <typename ST>
+__device__ __forceinline__ Buf<ST> buf_t(const Meshes<ST> &M, const Sys &y, ST w, ST v, ST vp) {
     bool smu = false, snu = false;
-    if (w < 0.0) { w = -w; }
-    if (v < 0.0) { v = -v; smu = !smu; }
-    if (vp < 0.0) { vp = -vp; snu = !snu; }
+    if (w < (ST)0) { w = -w; }
+    if (v < (ST)0) { v = -v; smu = !smu; }
+    if (vp < (ST)0) { vp = -vp; snu = !snu; }
     return make_buf(w, v, vp, M.Ot, M.nt, y.nO, y.nn, false, false, smu, snu);
 }
 
 // get_buffer_u (buffer.jl:163-199)
-__device__ __forceinline__ Buf buf_u(const Meshes &M, const Sys &y, double w, double v, double vp) {
+template <typename ST>
+__device__ __forceinline__ Buf<ST> buf_u(const Meshes<ST> &M, const Sys &y, ST w, ST v, ST vp) {
     bool ex = false, mp = false, snu = false;
-    if (w < 0.0) { w = -w; ex = !ex; }
-    if (v < 0.0) { v = -v; ex = !ex; mp = !mp; snu = !snu; }
-    if (vp < 0.0) { vp = -vp; mp = !mp; snu = !snu; }
+    if (w < (ST)0) { w = -w; ex = !ex; }
+    if (v < (ST)0) { v = -v; ex = !ex; mp = !mp; snu = !snu; }
+    if (vp < (ST)0) { vp = -vp; mp = !mp; snu = !snu; }
     return make_buf(w, v, vp, M.Os, M.ns, y.nO, y.nn, ex, mp, false, snu);
 }
 
@@ -141,7 +151,7 @@ __device__ __forceinline__ Buf buf_u(const Meshes &M, const Sys &y, double w, do
 // c0/c1 point at the channel block of component 0/1.  Corner order and weight
 // products follow the reference so FP64 results agree to rounding.
 template <typename TV>
-__device__ __forceinline__ void interp2(const TV *__restrict__ c0, const TV *__restrict__ c1, const Buf &b, int site,
+__device__ __forceinline__ void interp2(const TV *__restrict__ c0, const TV *__restrict__ c1, const Buf<> &b, int site,
                                         const Sys &y, TV &r0, TV &r1) {
     const long ns = y.ns, nO = y.nO, nn = y.nn;
     if (b.kernel == 1) {
@@ -177,7 +187,7 @@ __device__ __forceinline__ void interp2(const TV *__restrict__ c0, const TV *__r
         r1 = v1;
     } else if (b.kernel == 2 || b.kernel == 3) {
         const long off = (b.kernel == 2) ? y.q21 : y.q22;
-        const Param &pv = (b.kernel == 2) ? b.p2 : b.p3;
+        const Param<> &pv = (b.kernel == 2) ? b.p2 : b.p3;
         const TV *q0 = c0 + off, *q1 = c1 + off;
         const double a1 = b.p1.wl, b1 = b.p1.wh, a2 = pv.wl, b2 = pv.wh;
         const TV w00 = (TV)(a1 * a2), w10 = (TV)(b1 * a2), w01 = (TV)(a1 * b2), w11 = (TV)(b1 * b2);
@@ -197,7 +207,7 @@ __device__ __forceinline__ void interp2(const TV *__restrict__ c0, const TV *__r
 }
 
 // apply_flags_su2 (action_su2.jl:73-92): density sign from the two spin flags
-__device__ __forceinline__ double dens_sign(const Buf &b) {
+__device__ __forceinline__ double dens_sign(const Buf<> &b) {
     double s = 1.0;
     if (b.smu) s *= -1.0;
     if (b.snu) s *= -1.0;
@@ -219,7 +229,7 @@ __device__ __forceinline__ int corners(int kernel) { return kernel == 4 ? 8 : (k
 // Returns the number of interpolation corners gathered per component (for the
 // algorithmic-bytes counter).
 template <typename TV, bool MS, bool MT, bool MU>
-__device__ __forceinline__ int gamma_site(const ActV<TV> &A, const Buf &bs, const Buf &bt, const Buf &bu, int i,
+__device__ __forceinline__ int gamma_site(const ActV<TV> &A, const Buf<> &bs, const Buf<> &bt, const Buf<> &bu, int i,
                                           const int *exch, const Sys &y, TV &vs, TV &vd) {
     vs = (TV)A.bare[i];
     vd = (TV)A.bare[y.ns + i];
@@ -252,31 +262,34 @@ __device__ __forceinline__ int gamma_site(const ActV<TV> &A, const Buf &bs, cons
 
 // ---------------------------------------------------------------- propagators (Flow/bubbles.jl)
 // get_Σ (Action.jl:8-26)
-__device__ __forceinline__ double sigma_at(double w, const double *sm, int n, const double *Sig) {
+template <typename ST>
+__device__ __forceinline__ ST sigma_at(ST w, const ST *sm, int n, const ST *Sig) {
     if (fabs(w) <= sm[n - 1]) {
-        const Param p = find_param(fabs(w), sm, n);
-        const double sg = (w > 0.0) ? 1.0 : ((w < 0.0) ? -1.0 : w);
+        const Param<ST> p = find_param<ST>(fabs(w), sm, n);
+        const ST sg = (w > (ST)0) ? (ST)1 : ((w < (ST)0) ? (ST)-1 : w);
         return sg * (p.wl * Sig[p.lo] + p.wh * Sig[p.hi]);
     }
     return sm[n - 1] * Sig[n - 1] / w;
 }
 
-__device__ __forceinline__ double G_bare(double L, double w) {
-    return (fabs(w) > 1e-8) ? -expm1(-(w * w) / (L * L)) / w : 0.0;
+template <typename ST>
+__device__ __forceinline__ ST G_bare(ST L, ST w) {
+    return (fabs(w) > (ST)1e-8) ? -expm1(-(w * w) / (L * L)) / w : (ST)0;
 }
 
-__device__ __forceinline__ double G_full(double L, double w, const double *sm, int n, const double *Sig) {
-    if (!(fabs(w) > 1e-8)) return 0.0;
-    const double s = sigma_at(w, sm, n, Sig);
-    const double G0 = G_bare(L, w);
-    const double denom = (w * s >= 0.0) ? 1.0 / G0 + s : 1.0 / G0;
-    return 1.0 / denom;
+template <typename ST>
+__device__ __forceinline__ ST G_full(ST L, ST w, const ST *sm, int n, const ST *Sig) {
+    if (!(fabs(w) > (ST)1e-8)) return (ST)0;
+    const ST s = sigma_at<ST>(w, sm, n, Sig);
+    const ST G0 = G_bare<ST>(L, w);
+    const ST denom = (w * s >= (ST)0) ? (ST)1 / G0 + s : (ST)1 / G0;
+    return (ST)1 / denom;
 }
 
 __device__ __forceinline__ double S_single(double L, double w, const double *sm, int n, const double *Sig) {
     if (!(fabs(w) > 1e-8)) return 0.0;
-    const double G = G_full(L, w, sm, n, Sig);
-    const double G0 = G_bare(L, w);
+    const double G = G_full<double>(L, w, sm, n, Sig);
+    const double G0 = G_bare<double>(L, w);
     const double r = G / G0;
     return r * r * exp(-(w * w) / (L * L)) * 2.0 * w / (L * L * L);
 }
@@ -284,24 +297,26 @@ __device__ __forceinline__ double S_single(double L, double w, const double *sm,
 // symmetrised Katanin bubble K(x1,x2) + K(x2,x1), K = (S + G²dΣ) G' / 2π (bubbles.jl:64-80)
 __device__ __forceinline__ double bubble_kat(double L, double x1, double x2, const double *sm, int n,
                                              const double *Sig, const double *dSig) {
-    const double G1 = G_full(L, x1, sm, n, Sig), G2 = G_full(L, x2, sm, n, Sig);
+    const double G1 = G_full<double>(L, x1, sm, n, Sig), G2 = G_full<double>(L, x2, sm, n, Sig);
     const double S1 = S_single(L, x1, sm, n, Sig), S2 = S_single(L, x2, sm, n, Sig);
-    const double d1 = sigma_at(x1, sm, n, dSig), d2 = sigma_at(x2, sm, n, dSig);
+    const double d1 = sigma_at<double>(x1, sm, n, dSig), d2 = sigma_at<double>(x2, sm, n, dSig);
     return (S1 + G1 * G1 * d1) * G2 / (2.0 * PI_D) + (S2 + G2 * G2 * d2) * G1 / (2.0 * PI_D);
 }
 
 // -G1 G2 / 2π (bubbles.jl:83-96, with the sign of the callers)
 __device__ __forceinline__ double bubble_plain(double L, double x1, double x2, const double *sm, int n,
                                                const double *Sig) {
-    const double G1 = G_full(L, x1, sm, n, Sig), G2 = G_full(L, x2, sm, n, Sig);
+    const double G1 = G_full<double>(L, x1, sm, n, Sig), G2 = G_full<double>(L, x2, sm, n, Sig);
     return -(G1 * G2 / (2.0 * PI_D));
 }
 
 // fermionic arguments of the bubble per channel (flow_su2_{s,t,u}.jl:18)
-__device__ __forceinline__ void bubble_args(int ch, double W, double v, double &x1, double &x2) {
-    if (ch == 0) { x1 = v + 0.5 * W; x2 = 0.5 * W - v; }
-    else if (ch == 1) { x1 = v + 0.5 * W; x2 = v - 0.5 * W; }
-    else { x1 = v - 0.5 * W; x2 = v + 0.5 * W; }
+template <typename ST>
+__device__ __forceinline__ void bubble_args(int ch, ST W, ST v, ST &x1, ST &x2) {
+    const ST h = (ST)0.5 * W;
+    if (ch == 0) { x1 = v + h; x2 = h - v; }
+    else if (ch == 1) { x1 = v + h; x2 = v - h; }
+    else { x1 = v - h; x2 = v + h; }
 }
 
 // ---------------------------------------------------------------- Gauss-Kronrod (QuadGK.jl restated)

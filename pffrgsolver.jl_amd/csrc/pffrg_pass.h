@@ -144,9 +144,9 @@ __device__ __forceinline__ void slot_freqs(int ch, int slot, int c, double W, do
 // Build the descriptor of one buffer: fold flags, asymptotic kernel, bracketing indices
 // (get_buffer_{s,t,u}, buffer.jl:51-199) and the corner weights/offsets of the
 // interpolation (channel.jl:42-305) with the reference's weight products.
-__device__ __forceinline__ void build_desc(int c, double w, double x, double z, const Meshes &M, const Sys &y,
+__device__ __forceinline__ void build_desc(int c, double w, double x, double z, const Meshes<> &M, const Sys &y,
                                            Desc &d) {
-    const Buf b = (c == 0) ? buf_s(M, y, w, x, z) : (c == 1) ? buf_t(M, y, w, x, z) : buf_u(M, y, w, x, z);
+    const Buf<> b = (c == 0) ? buf_s(M, y, w, x, z) : (c == 1) ? buf_t(M, y, w, x, z) : buf_u(M, y, w, x, z);
     d.meta = b.kernel | (b.exch ? 8 : 0) | (b.map ? 16 : 0) | ((b.smu != b.snu) ? 32 : 0);
     if (!b.kernel) return;
     const int ns = y.ns, nO = y.nO, nn = y.nn;
@@ -165,7 +165,7 @@ __device__ __forceinline__ void build_desc(int c, double w, double x, double z, 
         d.w[0] = a1 * a2 * a3; d.w[1] = b1 * a2 * a3; d.w[2] = a1 * b2 * a3; d.w[3] = b1 * b2 * a3;
         d.w[4] = a1 * a2 * b3; d.w[5] = b1 * a2 * b3; d.w[6] = a1 * b2 * b3; d.w[7] = b1 * b2 * b3;
     } else {
-        const Param &pv = (b.kernel == 2) ? b.p2 : b.p3;
+        const Param<> &pv = (b.kernel == 2) ? b.p2 : b.p3;
         const double a2 = pv.wl, b2 = pv.wh;
         d.off = (int)((b.kernel == 2) ? y.q21 : y.q22) + ns * (b.p1.lo + nO * pv.lo);
         d.d1 = ns * (b.p1.hi - b.p1.lo);
@@ -222,16 +222,16 @@ __device__ __forceinline__ void gather(const Desc &d, const TV *__restrict__ q, 
 template <bool KAT>
 __device__ __forceinline__ void bubble_leg(double L, double x, const double *sm, int n, const double *Sig,
                                            const double *dSig, double &G, double &K) {
-    G = G_full(L, x, sm, n, Sig);
+    G = G_full<double>(L, x, sm, n, Sig);
     K = 0.0;
     if (KAT) {
         double S = 0.0;
         if (fabs(x) > 1e-8) {
-            const double G0 = G_bare(L, x);
+            const double G0 = G_bare<double>(L, x);
             const double r = G / G0;
             S = r * r * exp(-(x * x) / (L * L)) * 2.0 * x / (L * L * L);
         }
-        K = S + G * G * sigma_at(x, sm, n, dSig);
+        K = S + G * G * sigma_at<double>(x, sm, n, dSig);
     }
 }
 
@@ -246,7 +246,7 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
                                           double vwp, double v, int i, int slot, int g, int gbase, bool real_lane,
                                           TV &Gs, TV &Gd, int &nc) {
     const Sys &y = P.y;
-    const Meshes M{S.sig, S.Os, S.ns, S.Ot, S.nt};
+    const Meshes<> M{S.sig, S.Os, S.ns, S.Ot, S.nt};
     const int nslot = (ch == 1) ? 4 : 2;
     const int ntask = 2 + 3 * nslot;
     const int gs = real_lane ? g : 0;

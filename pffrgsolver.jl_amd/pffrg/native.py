@@ -66,6 +66,7 @@ _SIGS = {
     "pffrg_get_stats": ([_P, _I64], C.c_int),
     "pffrg_get_corrs": ([_P, _D], C.c_int),
     "pffrg_probe_integrand": ([_P, _i, _d, _i, _i, _i, _i, _i, _i, _i, _i, _D, _D], C.c_int),
+    "pffrg_debug_record": ([_P, C.POINTER(C.c_int32), _D, C.c_int64], C.c_int),
     "pffrg_set_timing": ([_P, _i], C.c_int),
     "pffrg_get_timing": ([_P, _D, _I64], C.c_int),
 }

@@ -263,13 +263,21 @@ def test_bare_only_rhs_closed_form():
     b = a.bare[0]
     nb = [np.array(v) for v in ([1, 0], [-1, 0], [0, 1], [0, -1])]
     P = np.array([sum(1 for k in nb if np.abs(k - v[:2]).sum() == 1) for v in r.vecs])
+    # committed scipy values of ∫ p_kat dv at W = 0 (the first Ω mesh point) for three Λ
+    n_gold = 0
     for g in gold:
-        if g["L"] != L:
+        if g["W"] != 0.0:
             continue
-        ch, W = g["ch"], g["W"]
-        Om = m.Ot if ch == 1 else m.Os
-        if W not in Om:
-            continue
+        Lg, ch = g["L"], g["ch"]
+        orc.compute_corrs(Lg, a, da, True)
+        out = A.Action.empty(r.n_sites, m)
+        orc.compute_element(O.P_KAT, Lg, a, da, ch, 4, 0, 1, 2, 10, out)
+        qs = out.view(0, ch, "q3")[:, 0, 1, 2]
+        ref = {0: 2 * b * b, 1: 2 * (2 * 0.5) * P / 16.0, 2: -2 * b * b}[ch] * g["integral"]
+        np.testing.assert_allclose(qs, ref, rtol=1e-4, atol=1e-4 * np.max(np.abs(ref)))
+        n_gold += 1
+    assert n_gold == 9
+    orc.compute_corrs(L, a, da, True)
     # evaluate at mesh bosonic frequencies and compare with a direct scipy integral of p
     from scipy.integrate import quad
     sys_ok = 0
