@@ -57,10 +57,19 @@ struct Desc {
 constexpr int MAX_GROUPS = 16;  // site groups per wavefront
 constexpr int MAX_DESC = 12;    // 4 vertex slots x 3 channels
 
+// Published state of one sub-interval owner (one per group), read by the groups that
+// evaluate its pooled nodes.
+struct Ctl {
+    double a, b, h, sgn;
+    int n, init, issued, pend;
+};
+
 struct Lds {
     const double *Os, *ns, *Ot, *nt, *sig, *sigA, *sigD;
     const int *exch, *ov_off;
     const uint32_t *ov;
+    Ctl *ctl;      // [group]
+    int *own;      // [group]: owner whose node the group evaluated this step, -1 = none
     double *scr;   // 64 x 4 doubles of lane exchange scratch
     Desc *desc;    // [group][MAX_DESC]
     double *bub;   // [group][2][2]: (G, K) of the two bubble legs
@@ -411,6 +420,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
     if (lane < y.ns) s_exch[lane] = P.exch[lane];
     const int n_ov = P.ov_off[y.ns];
     __shared__ double s_bub[MAX_GROUPS * 4];
+    __shared__ Ctl s_ctl[MAX_GROUPS];
+    __shared__ int s_own[MAX_GROUPS];
     for (int k = lane; k < n_ov; k += 64) s_dyn[k] = P.ov_rows[k];
     for (int k = lane; k <= y.ns; k += 64) s_dyn[n_ov + k] = (uint32_t)P.ov_off[k];
     if (lane == 0) { s_cnt = 0; s_bytes = 0; }
@@ -431,6 +442,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
     // descriptors follow the overlap table in the dynamic LDS block, 16-byte aligned
     S.desc = reinterpret_cast<Desc *>(s_dyn + ((n_ov + y.ns + 1 + 3) & ~3));
     S.bub = s_bub;
+    S.ctl = s_ctl;
+    S.own = s_own;
 
     int ch, kernel, w1, w2, w3;
     decode(P.dbg_v ? P.dbg_elem : P.elems[e_idx], ch, kernel, w1, w2, w3);
@@ -459,25 +472,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
     const int nsub = 4 * P.eval;
     const int rounds = (nsub + G - 1) / G;
     const double c0 = P.corrs[0 + 2 * (ch + 3 * w1)], c1 = P.corrs[1 + 2 * (ch + 3 * w1)];
+    Ctl *ctl = S.ctl;
 
-    // Wave-uniform state machine; the groups of a wavefront advance in lockstep (they start
-    // a round together and, while refining, all sit at the same level n), so a single
-    // node_eval call site serves the probe, the Simpson stages and the tail corrections.
-    enum { M_ROUND, M_NODES, M_CHECK, M_TAIL, M_PROBE };
+    // Work layout.  A round hands sub-interval r*G+g to group g (its owner), whose lanes keep
+    // the Simpson state of their site.  The quadrature nodes pending on all owners of the round
+    // form one pool; every wave step gives each group the next pooled node (owners in order,
+    // nodes in ascending order), so groups whose sub-interval converged early keep working on
+    // the others.  Each owner adds the contributions of its nodes in node order, which keeps
+    // the reference's summation order exactly.  A single node_eval call site serves the
+    // probe, the pooled steps and the tail corrections.
+    enum { M_ROUND, M_STEP, M_TAIL, M_PROBE, M_END };
     int mode = P.dbg_v ? M_PROBE : M_ROUND;
-    int r = 0, k = 0, nk = 0, j = 0;
-    long n = 4;
-    bool init = true, act = false, done = true;
+    int r = 0, j = 0, tk = 0;
+    long n = 4;             // owner: current level's n (quadrature.jl:2-24)
+    int lvl = 0, cnt = 0;   // owner: nodes in the current level, contributions received
+    bool init = true, done = true, act = false;
     double a = 0.0, b = 0.0, sgn = 1.0, h = 0.0;
     double b10 = 0.0, b11 = 0.0, b20 = 0.0, b21 = 0.0, res0 = 0.0, res1 = 0.0;
     double acc0 = 0.0, acc1 = 0.0;
-    unsigned long long nodes = 0, corner_sum = 0, nodes_before = 0;
-    while (true) {
+    unsigned long long nodes = 0, corner_sum = 0;
+    int sub_nodes = 0;
+    while (mode != M_END) {
         if (mode == M_ROUND) {
             if (r >= rounds) {
                 mode = M_TAIL;
-                k = 0;
-                nk = 2;
                 continue;
             }
             j = r * G + g;
@@ -487,109 +505,73 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
                 continue;
             }
             b10 = b11 = b20 = b21 = res0 = res1 = 0.0;
-            nodes_before = nodes;
             done = !act;
             init = true;
             n = 4;
-            k = 0;
-            nk = 6;
-            mode = M_NODES;
+            lvl = act ? 6 : 0;
+            cnt = 0;
+            sub_nodes = 0;
+            mode = M_STEP;
             continue;
         }
-        if (mode == M_CHECK) {
-            // Richardson extrapolation and error estimate (quadrature.jl:72-88)
-            double r10 = b10 * (-1.0 / 15.0), r11 = b11 * (-1.0 / 15.0);
-            r10 += 16.0 / 15.0 * b20;
-            r11 += 16.0 / 15.0 * b21;
-            S.scr[4 * lane + 0] = r10;
-            S.scr[4 * lane + 1] = r11;
-            S.scr[4 * lane + 2] = b20;
-            S.scr[4 * lane + 3] = b21;
-            __syncthreads();
-            double q1 = 0.0, q2 = 0.0, q3 = 0.0;
-            if (real_lane) {
-                for (int jj = 0; jj < y.ns; ++jj) {
-                    const double *e = S.scr + 4 * (gbase + jj);
-                    const double x0 = e[0], x1 = e[1], z0 = e[2], z1 = e[3];
-                    q1 += x0 * x0; q1 += x1 * x1;
-                    q2 += z0 * z0; q2 += z1 * z1;
-                    const double d0 = x0 - z0, d1 = x1 - z1;
-                    q3 += d0 * d0; q3 += d1 * d1;
-                }
-            }
-            __syncthreads();
-            const double n1 = sqrt(q1), n2 = sqrt(q2), adiff = sqrt(q3);
-            const double rdiff = adiff / fmax(n1, n2);
-            if (!done) {
-                const double d0 = r10 - b20, d1 = r11 - b21;
-                if (adiff < P.atol || rdiff < P.rtol || n > (1L << 20)) {
-                    res0 = d0 + b20;
-                    res1 = d1 + b21;
-                    done = true;
-                } else {
-                    b10 = b20; b11 = b21;
-                    b20 = 0.5 * b10; b21 = 0.5 * b11;
-                }
-            }
-            if (__ballot(!done) != 0) {
-                // residual! with n new and n/2 re-weighted nodes (quadrature.jl:2-24)
-                h = 0.5 * (b - a) / n;
-                init = false;
-                k = 0;
-                nk = (int)(n + n / 2);
-                mode = M_NODES;
-                continue;
-            }
-            if (P.dbg_sub_nodes && act && i == 0 && (long)e_idx * nsub + j < P.dbg_cap) {
-                P.dbg_sub_nodes[(long)e_idx * nsub + j] = (int)(nodes - nodes_before);
-                P.dbg_sub_res[(long)e_idx * nsub + j] = res0;
-            }
-            // ordered accumulation of this round's sub-interval results
-            S.scr[2 * lane + 0] = res0;
-            S.scr[2 * lane + 1] = res1;
-            __syncthreads();
-            for (int gg = 0; gg < G; ++gg) {
-                const int jj = r * G + gg;
-                double ta, tb, ts;
-                if (jj < nsub && subinterval(jj, P.eval, ref, logpts, ta, tb, ts)) {
-                    acc0 += S.scr[2 * (gg * GS + i) + 0];
-                    acc1 += S.scr[2 * (gg * GS + i) + 1];
-                }
-            }
-            __syncthreads();
-            ++r;
-            mode = M_ROUND;
-            continue;
-        }
-        // one node per group: Simpson stages, tail corrections or the probe
+        // ---- one node per group
         double x = 0.0, dx = 0.0;
         bool use = false;
+        int own = -1, kk = 0;
         if (mode == M_PROBE) {
             x = P.dbg_v[bid];
             use = true;
         } else if (mode == M_TAIL) {
-            x = (k == 0) ? -val : val;  // boundary corrections (flow_s.jl:37-38)
-            dx = (k == 0) ? c0 : c1;
-            use = true;
-        } else if (init) {
-            // initial Simpson S1 on (a, m, b) and S2 (simps!, quadrature.jl:57-68)
-            if (k == 0) { x = a; dx = 1.0 * (b - a) / 6.0; }
-            else if (k == 1) { x = 0.5 * (a + b); dx = 2.0 * (b - a) / 3.0; }
-            else if (k == 2) { x = b; dx = 1.0 * (b - a) / 6.0; }
-            else {
-                const double h2 = 0.5 * (b - a) / 2;
-                if (k == 3) { x = node_at(a, 2.0 * 1 - 1.0, h2); dx = +4.0 * h2 / 3.0; }
-                else if (k == 4) { x = node_at(a, 2.0 * 2 - 1.0, h2); dx = +4.0 * h2 / 3.0; }
-                else { x = node_at(a, 4.0 * 1 - 2.0, h2); dx = -2.0 * h2 / 3.0; }
+            // boundary corrections (flow_s.jl:37-38), -val then +val; node tk+g for group g
+            if (real_lane && tk + g < 2) {
+                x = (tk + g == 0) ? -val : val;
+                dx = (tk + g == 0) ? c0 : c1;
+                use = true;
             }
-            if (k == 3) { b20 = 0.5 * b10; b21 = 0.5 * b11; }
-            x *= sgn;
-            use = act;
         } else {
-            if (k < n) { x = node_at(a, 2.0 * (k + 1) - 1.0, h); dx = +4.0 * h / 3.0; }
-            else { x = node_at(a, 4.0 * (k - n + 1) - 2.0, h); dx = -2.0 * h / 3.0; }
-            x *= sgn;
-            use = !done;
+            // publish each owner's state, then claim the g-th pooled node
+            if (real_lane && i == 0) {
+                Ctl &C = ctl[g];
+                C.a = a; C.b = b; C.h = h; C.sgn = sgn;
+                C.n = (int)n; C.init = init ? 1 : 0;
+                C.issued = cnt; C.pend = done ? 0 : lvl - cnt;
+            }
+            __syncthreads();
+            if (real_lane) {
+                int cum = 0;
+                for (int o = 0; o < G; ++o) {
+                    const int p = ctl[o].pend;
+                    if (g < cum + p) {
+                        own = o;
+                        kk = ctl[o].issued + (g - cum);
+                        break;
+                    }
+                    cum += p;
+                }
+            }
+            if (own >= 0) {
+                const Ctl &C = ctl[own];
+                const double ca = C.a, cb = C.b, ch_ = C.h;
+                if (C.init) {
+                    // initial Simpson S1 on (a, m, b) and S2 (simps!, quadrature.jl:57-68)
+                    if (kk == 0) { x = ca; dx = 1.0 * (cb - ca) / 6.0; }
+                    else if (kk == 1) { x = 0.5 * (ca + cb); dx = 2.0 * (cb - ca) / 3.0; }
+                    else if (kk == 2) { x = cb; dx = 1.0 * (cb - ca) / 6.0; }
+                    else {
+                        const double h2 = 0.5 * (cb - ca) / 2;
+                        if (kk == 3) { x = node_at(ca, 2.0 * 1 - 1.0, h2); dx = +4.0 * h2 / 3.0; }
+                        else if (kk == 4) { x = node_at(ca, 2.0 * 2 - 1.0, h2); dx = +4.0 * h2 / 3.0; }
+                        else { x = node_at(ca, 4.0 * 1 - 2.0, h2); dx = -2.0 * h2 / 3.0; }
+                    }
+                } else {
+                    // residual!: n new nodes, then n/2 re-weighted ones (quadrature.jl:2-24)
+                    const int cn = C.n;
+                    if (kk < cn) { x = node_at(ca, 2.0 * (kk + 1) - 1.0, ch_); dx = +4.0 * ch_ / 3.0; }
+                    else { x = node_at(ca, 4.0 * (kk - cn + 1) - 2.0, ch_); dx = -2.0 * ch_ / 3.0; }
+                }
+                x *= C.sgn;
+                use = true;
+            }
         }
         TV Gs, Gd;
         int nc;
@@ -601,24 +583,107 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
             }
             return;
         }
-        if (mode == M_TAIL) {
-            acc0 += dx * (double)Gs;
-            acc1 += dx * (double)Gd;
-            if (g == 0) {
-                ++nodes;
-                corner_sum += nc;
-            }
-        } else if (use) {
-            if (init && k < 3) { b10 += dx * (double)Gs; b11 += dx * (double)Gd; }
-            else { b20 += dx * (double)Gs; b21 += dx * (double)Gd; }
-            ++nodes;
+        if (use && real_lane) {
+            if (i == 0) ++nodes;
             corner_sum += nc;
         }
-        if (++k == nk) {
-            if (mode == M_TAIL) break;
-            if (!init) n *= 2;
-            mode = M_CHECK;
+        // contributions dv * Γ of every group, in group (= pooled node) order
+        double *sc = S.scr;
+        const double v0 = use ? dx * (double)Gs : 0.0, v1 = use ? dx * (double)Gd : 0.0;
+        sc[2 * lane + 0] = v0;
+        sc[2 * lane + 1] = v1;
+        if (real_lane && i == 0) S.own[g] = use ? own : -1;
+        __syncthreads();
+        if (mode == M_TAIL) {
+            for (int e = 0; e < G && tk + e < 2; ++e) {
+                acc0 += sc[2 * (e * GS + i) + 0];
+                acc1 += sc[2 * (e * GS + i) + 1];
+            }
+            __syncthreads();
+            tk += G;
+            if (tk >= 2) mode = M_END;
+            continue;
         }
+        bool need_check = false;
+        if (real_lane && !done) {
+            for (int e = 0; e < G; ++e) {
+                if (S.own[e] != g) continue;
+                const double u0 = sc[2 * (e * GS + i) + 0], u1 = sc[2 * (e * GS + i) + 1];
+                if (init && cnt < 3) {
+                    b10 += u0; b11 += u1;
+                } else {
+                    if (init && cnt == 3) { b20 = 0.5 * b10; b21 = 0.5 * b11; }
+                    b20 += u0; b21 += u1;
+                }
+                ++cnt;
+                ++sub_nodes;
+            }
+            if (cnt == lvl) {
+                if (!init) n *= 2;
+                need_check = true;
+            }
+        }
+        if (__ballot(need_check) != 0) {
+            // Richardson extrapolation and error estimate (quadrature.jl:72-88); the norms run
+            // over the owner's sites in site order, as the reference's do
+            double r10 = b10 * (-1.0 / 15.0), r11 = b11 * (-1.0 / 15.0);
+            r10 += 16.0 / 15.0 * b20;
+            r11 += 16.0 / 15.0 * b21;
+            __syncthreads();
+            sc[4 * lane + 0] = r10;
+            sc[4 * lane + 1] = r11;
+            sc[4 * lane + 2] = b20;
+            sc[4 * lane + 3] = b21;
+            __syncthreads();
+            if (need_check) {
+                double q1 = 0.0, q2 = 0.0, q3 = 0.0;
+                for (int jj = 0; jj < y.ns; ++jj) {
+                    const double *e = sc + 4 * (gbase + jj);
+                    const double x0 = e[0], x1 = e[1], z0 = e[2], z1 = e[3];
+                    q1 += x0 * x0; q1 += x1 * x1;
+                    q2 += z0 * z0; q2 += z1 * z1;
+                    const double d0 = x0 - z0, d1 = x1 - z1;
+                    q3 += d0 * d0; q3 += d1 * d1;
+                }
+                const double n1 = sqrt(q1), n2 = sqrt(q2), adiff = sqrt(q3);
+                const double rdiff = adiff / fmax(n1, n2);
+                const double d0 = r10 - b20, d1 = r11 - b21;
+                if (adiff < P.atol || rdiff < P.rtol || n > (1L << 20)) {
+                    res0 = d0 + b20;
+                    res1 = d1 + b21;
+                    done = true;
+                    lvl = 0;
+                    if (P.dbg_sub_nodes && i == 0 && (long)e_idx * nsub + j < P.dbg_cap) {
+                        P.dbg_sub_nodes[(long)e_idx * nsub + j] = sub_nodes;
+                        P.dbg_sub_res[(long)e_idx * nsub + j] = res0;
+                    }
+                } else {
+                    b10 = b20; b11 = b21;
+                    b20 = 0.5 * b10; b21 = 0.5 * b11;
+                    init = false;
+                    h = 0.5 * (b - a) / n;
+                    lvl = (int)(n + n / 2);
+                    cnt = 0;
+                }
+            }
+            __syncthreads();
+        }
+        if (__ballot(real_lane && !done) != 0) continue;
+        // round complete: ordered accumulation of its sub-interval results
+        sc[2 * lane + 0] = res0;
+        sc[2 * lane + 1] = res1;
+        __syncthreads();
+        for (int gg = 0; gg < G; ++gg) {
+            const int jj = r * G + gg;
+            double ta, tb, ts;
+            if (jj < nsub && subinterval(jj, P.eval, ref, logpts, ta, tb, ts)) {
+                acc0 += sc[2 * (gg * GS + i) + 0];
+                acc1 += sc[2 * (gg * GS + i) + 1];
+            }
+        }
+        __syncthreads();
+        ++r;
+        mode = M_ROUND;
     }
     // write the site vector (flow_s.jl:41-51)
     if (g == 0) {
