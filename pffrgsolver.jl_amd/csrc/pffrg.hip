@@ -629,7 +629,7 @@ int pffrg_create(const pffrg_dims *d, pffrg_ctx **out) {
     y.chlen = y.q3 + (long)y.ns * y.nO * y.nn * y.nn;
     y.complen = 3 * y.chlen;
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
-    ok = ok && hipMalloc(&c->d_mesh, 5 * MAXN * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_mesh, 10 * MAXN * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_corrs, 2 * 3 * y.nO * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_count, 2 * sizeof(unsigned long long)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_max, sizeof(unsigned long long)) == hipSuccess;
@@ -699,9 +699,12 @@ int pffrg_set_mesh(pffrg_ctx *c, const double *sigma, const double *Os, const do
     if (!c || !sigma || !Os || !ns || !Ot || !nt) return fail(c, PFFRG_ERR_ARG, "null mesh pointer");
     HIPCHK(c, hipSetDevice(c->device));
     Sys &y = c->sys;
-    std::vector<double> h(5 * MAXN, 0.0);
+    // five meshes, then the inverse spacings 1/(m[j+1]-m[j]) of each (the reciprocal get_param
+    // computes, param.jl:73-76, evaluated once with the same IEEE division)
+    std::vector<double> h(10 * MAXN, 0.0);
     auto put = [&](int k, const double *m, int n) {
         for (int j = 0; j < n; ++j) h[k * MAXN + j] = m[j];
+        for (int j = 0; j + 1 < n; ++j) h[(5 + k) * MAXN + j] = 1.0 / (m[j + 1] - m[j]);
     };
     put(0, sigma, y.nsig);
     put(1, Os, y.nO);

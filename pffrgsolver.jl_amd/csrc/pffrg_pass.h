@@ -68,6 +68,8 @@ struct Lds {
     const double *Os, *ns, *Ot, *nt, *sig, *sigA, *sigD;
     const int *exch, *ov_off;
     const uint32_t *ov;
+    const double *inv;      // inverse mesh spacings: σ | Ωs | νs | Ωt | νt, MAXN each
+    const uint32_t *code;   // [MAX_DESC] frequency-transfer codes of this element's vertex slots
     Ctl *ctl;      // [group]
     int *own;      // [group]: owner whose node the group evaluated this step, -1 = none
     double *scr;   // 64 x 4 doubles of lane exchange scratch
@@ -150,37 +152,126 @@ __device__ __forceinline__ void slot_freqs(int ch, int slot, int c, double W, do
     }
 }
 
-// Build the descriptor of one buffer: fold flags, asymptotic kernel, bracketing indices
-// (get_buffer_{s,t,u}, buffer.jl:51-199) and the corner weights/offsets of the
-// interpolation (channel.jl:42-305) with the reference's weight products.
-__device__ __forceinline__ void build_desc(int c, double w, double x, double z, const Meshes<> &M, const Sys &y,
-                                           Desc &d) {
-    const Buf<> b = (c == 0) ? buf_s(M, y, w, x, z) : (c == 1) ? buf_t(M, y, w, x, z) : buf_u(M, y, w, x, z);
-    d.meta = b.kernel | (b.exch ? 8 : 0) | (b.map ? 16 : 0) | ((b.smu != b.snu) ? 32 : 0);
-    if (!b.kernel) return;
-    const int ns = y.ns, nO = y.nO, nn = y.nn;
-    const double a1 = b.p1.wl, b1 = b.p1.wh;
-    if (b.kernel == 1) {
-        d.off = ns * b.p1.lo;
-        d.d1 = ns * (b.p1.hi - b.p1.lo);
-        d.w[0] = a1;
-        d.w[1] = b1;
-    } else if (b.kernel == 4) {
-        const double a2 = b.p2.wl, b2 = b.p2.wh, a3 = b.p3.wl, b3 = b.p3.wh;
-        d.off = (int)y.q3 + ns * (b.p1.lo + nO * (b.p2.lo + nn * b.p3.lo));
-        d.d1 = ns * (b.p1.hi - b.p1.lo);
-        d.d2 = ns * nO * (b.p2.hi - b.p2.lo);
-        d.d3 = ns * nO * nn * (b.p3.hi - b.p3.lo);
-        d.w[0] = a1 * a2 * a3; d.w[1] = b1 * a2 * a3; d.w[2] = a1 * b2 * a3; d.w[3] = b1 * b2 * a3;
-        d.w[4] = a1 * a2 * b3; d.w[5] = b1 * a2 * b3; d.w[6] = a1 * b2 * b3; d.w[7] = b1 * b2 * b3;
-    } else {
-        const Param<> &pv = (b.kernel == 2) ? b.p2 : b.p3;
-        const double a2 = pv.wl, b2 = pv.wh;
-        d.off = (int)((b.kernel == 2) ? y.q21 : y.q22) + ns * (b.p1.lo + nO * pv.lo);
-        d.d1 = ns * (b.p1.hi - b.p1.lo);
-        d.d2 = ns * nO * (pv.hi - pv.lo);
-        d.w[0] = a1 * a2; d.w[1] = b1 * a2; d.w[2] = a1 * b2; d.w[3] = b1 * b2;
+// Frequency-transfer code of (vertex slot k, channel c): bit 0 = channel enabled, bit 1 = the
+// slot reads action D; bytes 1-3 encode the buffer arguments (w, v, v') of slot_freqs, each
+// as ((sW·W + sv·v) + sa·A)·h with signs s ∈ {0, +, -}, A ∈ {a1, a2}, h ∈ {1, 1/2}
+// (bits 0-1 sW, 2-3 sv, 4-5 sa, 6 A = a2, 7 h = 1/2).  The code is read off slot_freqs itself
+// by probing it with unit arguments, so both forms are the same formulas; every formula there
+// has this shape, and the IEEE sum of the signed terms in this order equals the original
+// expression exactly.
+template <int PASS>
+__device__ uint32_t slot_code(int ch, int k, int c) {
+    const int cfg = vertex_cfg<PASS>(ch, k);
+    uint32_t code = (uint32_t)((cfg >> c) & 1) | ((cfg & 8) ? 2u : 0u);
+    double o[4][3];
+    slot_freqs(ch, k, c, 1.0, 0.0, 0.0, 0.0, o[0][0], o[0][1], o[0][2]);
+    slot_freqs(ch, k, c, 0.0, 0.0, 0.0, 1.0, o[1][0], o[1][1], o[1][2]);
+    slot_freqs(ch, k, c, 0.0, 1.0, 0.0, 0.0, o[2][0], o[2][1], o[2][2]);
+    slot_freqs(ch, k, c, 0.0, 0.0, 1.0, 0.0, o[3][0], o[3][1], o[3][2]);
+    for (int q = 0; q < 3; ++q) {
+        uint32_t e = 0;
+        bool half = false;
+        const double tw = o[0][q], tv = o[1][q], t1 = o[2][q], t2 = o[3][q];
+        if (tw != 0.0) { e |= (tw > 0.0) ? 1u : 2u; half = half || fabs(tw) == 0.5; }
+        if (tv != 0.0) { e |= (tv > 0.0) ? 4u : 8u; half = half || fabs(tv) == 0.5; }
+        if (t1 != 0.0) { e |= (t1 > 0.0) ? 16u : 32u; half = half || fabs(t1) == 0.5; }
+        if (t2 != 0.0) { e |= ((t2 > 0.0) ? 16u : 32u) | 64u; half = half || fabs(t2) == 0.5; }
+        if (half) e |= 128u;
+        code |= e << (8 + 8 * q);
     }
+    return code;
+}
+
+template <typename ST>
+__device__ __forceinline__ ST coord(uint32_t e, ST W, ST v, ST a1, ST a2) {
+    const ST A = (e & 64u) ? a2 : a1;
+    const ST tW = (e & 1u) ? W : ((e & 2u) ? -W : (ST)0);
+    const ST tv = (e & 4u) ? v : ((e & 8u) ? -v : (ST)0);
+    const ST ta = (e & 16u) ? A : ((e & 32u) ? -A : (ST)0);
+    const ST r = (tW + tv) + ta;
+    return (e & 128u) ? (ST)0.5 * r : r;
+}
+
+// get_indices + get_param (param.jl:26-86) for val >= 0 on a mesh of n <= 64 points starting
+// at 0: a fixed six-step search for the first point >= val, then the reference's bracket and
+// weights, with the reciprocal spacing read from a table built by the same IEEE division.
+template <typename ST>
+__device__ __forceinline__ void bracket(ST val, const ST *m, const ST *inv, int n, int &lo, int &hi, ST &wl,
+                                        ST &wh) {
+    int base = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+        const int t = base + step;
+        if (t <= n - 1 && m[t - 1] < val) base = t;
+    }
+    hi = base;
+    const ST mh = m[base];
+    lo = (val < mh) ? base - 1 : base;
+    if (lo < hi) {
+        const ST d = inv[lo];
+        wl = d * (mh - val);
+        wh = d * (val - m[lo]);
+    } else {
+        wl = (ST)1;
+        wh = (ST)0;
+    }
+}
+
+// get_Σ (Action.jl:8-26) with the table-driven bracket
+__device__ __forceinline__ double sigma_lookup(double w, const double *sm, const double *inv, int n,
+                                               const double *Sig) {
+    const double aw = fabs(w);
+    if (aw <= sm[n - 1]) {
+        int lo, hi;
+        double wl, wh;
+        bracket(aw, sm, inv, n, lo, hi, wl, wh);
+        const double sg = (w > 0.0) ? 1.0 : ((w < 0.0) ? -1.0 : w);
+        return sg * (wl * Sig[lo] + wh * Sig[hi]);
+    }
+    return sm[n - 1] * Sig[n - 1] / w;
+}
+
+// Descriptor of one buffer in straight-line form: frequency transfer from the slot code, the
+// sign folding of get_buffer_{s,t,u} (buffer.jl:88-199), the asymptotic-kernel choice of
+// get_buffer (buffer.jl:51-86), three brackets, and the corner weights and offsets of the
+// interpolation (channel.jl:42-305) in the reference's product order.  The unused dimensions of
+// q1/q2 carry the neutral bracket (index 0, weights 1 and 0), which reproduces the reference's
+// weights exactly and leaves the extra corners at weight zero.
+__device__ __forceinline__ void build_desc_u(const Lds &S, const Sys &y, uint32_t code, int c, double W, double a1,
+                                             double a2, double v, Desc &d) {
+    double w = coord(code >> 8 & 255u, W, v, a1, a2);
+    double x = coord(code >> 16 & 255u, W, v, a1, a2);
+    double z = coord(code >> 24, W, v, a1, a2);
+    const bool nw = w < 0.0, nx = x < 0.0, nz = z < 0.0;
+    w = nw ? -w : w;
+    x = nx ? -x : x;
+    z = nz ? -z : z;
+    const bool tch = (c == 1);
+    const bool ex = !tch && (nw != nx), mp = !tch && (nx != nz), negd = (nx != nz);
+    const int nO = y.nO, nn = y.nn, ns = y.ns;
+    const double *Om = tch ? S.Ot : S.Os;
+    const double *nu = tch ? S.nt : S.ns;
+    const double *iO = S.inv + (tch ? 3 : 1) * MAXN;
+    const double *inu = S.inv + (tch ? 4 : 2) * MAXN;
+    const bool out = Om[nO - 1] < w;
+    const bool av = nu[nn - 1] < x, avp = nu[nn - 1] < z;
+    const int kernel = out ? 0 : (av ? (avp ? 1 : 3) : (avp ? 2 : 4));
+    int l1, h1, l2, h2, l3, h3;
+    double a1w, b1w, a2w, b2w, a3w, b3w;
+    bracket(w, Om, iO, nO, l1, h1, a1w, b1w);
+    bracket(x, nu, inu, nn, l2, h2, a2w, b2w);
+    bracket(z, nu, inu, nn, l3, h3, a3w, b3w);
+    if (kernel == 3) { l2 = l3; h2 = h3; a2w = a3w; b2w = b3w; }
+    if (kernel != 4) { l3 = 0; h3 = 0; a3w = 1.0; b3w = 0.0; }
+    if (kernel == 1) { l2 = 0; h2 = 0; a2w = 1.0; b2w = 0.0; }
+    const int base = (kernel == 2) ? (int)y.q21 : (kernel == 3) ? (int)y.q22 : (kernel == 4) ? (int)y.q3 : 0;
+    d.meta = ((code & 1u) && kernel) ? (kernel | (ex ? 8 : 0) | (mp ? 16 : 0) | (negd ? 32 : 0)) : 0;
+    d.off = base + ns * (l1 + nO * (l2 + nn * l3));
+    d.d1 = ns * (h1 - l1);
+    d.d2 = ns * nO * (h2 - l2);
+    d.d3 = ns * nO * nn * (h3 - l3);
+    d.w[0] = a1w * a2w * a3w; d.w[1] = b1w * a2w * a3w; d.w[2] = a1w * b2w * a3w; d.w[3] = b1w * b2w * a3w;
+    d.w[4] = a1w * a2w * b3w; d.w[5] = b1w * a2w * b3w; d.w[6] = a1w * b2w * b3w; d.w[7] = b1w * b2w * b3w;
 }
 
 // Both SU(2) components of one channel at one site from a descriptor, corners in the
@@ -229,18 +320,25 @@ __device__ __forceinline__ void gather(const Desc &d, const TV *__restrict__ q, 
 // One leg x of the bubble: G(x) and, for the Katanin bubble, K(x) = S(x) + G(x)² dΣ(x)
 // (bubbles.jl:17-80; S reuses G as the reference's get_S recomputes the same value).
 template <bool KAT>
-__device__ __forceinline__ void bubble_leg(double L, double x, const double *sm, int n, const double *Sig,
-                                           const double *dSig, double &G, double &K) {
-    G = G_full<double>(L, x, sm, n, Sig);
+__device__ __forceinline__ void bubble_leg(double L, double x, const double *sm, const double *inv, int n,
+                                           const double *Sig, const double *dSig, double &G, double &K) {
+    G = 0.0;
+    double G0 = 0.0;
+    if (fabs(x) > 1e-8) {
+        // get_G (bubbles.jl:17-36)
+        const double s = sigma_lookup(x, sm, inv, n, Sig);
+        G0 = G_bare<double>(L, x);
+        const double denom = (x * s >= 0.0) ? 1.0 / G0 + s : 1.0 / G0;
+        G = 1.0 / denom;
+    }
     K = 0.0;
     if (KAT) {
         double S = 0.0;
         if (fabs(x) > 1e-8) {
-            const double G0 = G_bare<double>(L, x);
             const double r = G / G0;
             S = r * r * exp(-(x * x) / (L * L)) * 2.0 * x / (L * L * L);
         }
-        K = S + G * G * sigma_at<double>(x, sm, n, dSig);
+        K = S + G * G * sigma_lookup(x, sm, inv, n, dSig);
     }
 }
 
@@ -255,7 +353,6 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
                                           double vwp, double v, int i, int slot, int g, int gbase, bool real_lane,
                                           TV &Gs, TV &Gd, int &nc) {
     const Sys &y = P.y;
-    const Meshes<> M{S.sig, S.Os, S.ns, S.Ot, S.nt};
     const int nslot = (ch == 1) ? 4 : 2;
     const int ntask = 2 + 3 * nslot;
     const int gs = real_lane ? g : 0;
@@ -268,20 +365,11 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
                 double x1, x2;
                 bubble_args(ch, W, v, x1, x2);
                 double G, K;
-                bubble_leg<PASS == P_KAT>(P.L, t == 0 ? x1 : x2, S.sig, y.nsig, S.sigA, S.sigD, G, K);
+                bubble_leg<PASS == P_KAT>(P.L, t == 0 ? x1 : x2, S.sig, S.inv, y.nsig, S.sigA, S.sigD, G, K);
                 bg[2 * t] = G;
                 bg[2 * t + 1] = K;
             } else {
-                const int k = (t - 2) / 3, c = (t - 2) % 3;
-                const int cfg = vertex_cfg<PASS>(ch, k);
-                Desc &d = dg[t - 2];
-                if ((cfg >> c) & 1) {
-                    double w, x, z;
-                    slot_freqs(ch, k, c, W, vw, vwp, v, w, x, z);
-                    build_desc(c, w, x, z, M, y, d);
-                } else {
-                    d.meta = 0;
-                }
+                build_desc_u(S, y, S.code[t - 2], (t - 2) % 3, W, vw, vwp, v, dg[t - 2]);
             }
         }
     }
@@ -399,8 +487,9 @@ __device__ __forceinline__ bool subinterval(int j, int eval, double ref, const d
 template <int PASS, typename TV>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_WAVES_PER_EU))) void k_vertex_pass(PassArgs<TV> P) {
     extern __shared__ uint32_t s_dyn[];  // overlap rows then ov_off
-    __shared__ double s_mesh[5 * MAXN];
+    __shared__ double s_mesh[10 * MAXN];  // meshes, then their inverse spacings
     __shared__ double s_sigA[MAXN], s_sigD[MAXN];
+    __shared__ uint32_t s_code[MAX_DESC];
     __shared__ double s_scr[64 * 4];
     __shared__ int s_exch[64];
     __shared__ unsigned long long s_cnt, s_bytes;
@@ -412,7 +501,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
     const int lane = threadIdx.x;
 
     // ---- stage tables in LDS
-    for (int k = lane; k < 5 * MAXN; k += 64) s_mesh[k] = P.mesh[k];
+    for (int k = lane; k < 10 * MAXN; k += 64) s_mesh[k] = P.mesh[k];
     for (int k = lane; k < y.nsig; k += 64) {
         s_sigA[k] = P.A.Sigma[k];
         s_sigD[k] = (PASS == P_KAT) ? P.D.Sigma[k] : 0.0;
@@ -433,6 +522,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
     S.ns = s_mesh + 2 * MAXN;
     S.Ot = s_mesh + 3 * MAXN;
     S.nt = s_mesh + 4 * MAXN;
+    S.inv = s_mesh + 5 * MAXN;
+    S.code = s_code;
     S.sigA = s_sigA;
     S.sigD = s_sigD;
     S.exch = s_exch;
@@ -447,6 +538,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
 
     int ch, kernel, w1, w2, w3;
     decode(P.dbg_v ? P.dbg_elem : P.elems[e_idx], ch, kernel, w1, w2, w3);
+    if (lane < MAX_DESC) s_code[lane] = slot_code<PASS>(ch, lane / 3, lane % 3);
+    __syncthreads();
     const double INF = __builtin_huge_val();
     double W, vw, vwp;
     if (PASS == P_REDUCED) {
