@@ -461,7 +461,6 @@ int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const
     P.logpts = c->d_logpts;
     P.elems = (mode == 0) ? c->d_el_half : (mode == 1) ? c->d_el_full : c->d_el_red;
     P.n_elems = (mode == 0) ? c->n_half : (mode == 1) ? c->n_full : c->n_red;
-    P.per_xcd = (P.n_elems + 7) / 8;
     P.L = L;
     P.eval = eval;
     P.atol = tol[0];
@@ -469,7 +468,7 @@ int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const
     P.node_count = c->d_count;
     const size_t lds = (size_t)((c->n_ov + c->sys.ns + 1 + 3) & ~3) * sizeof(uint32_t) +
                        (size_t)c->sys.G * MAX_DESC * sizeof(Desc);
-    const int grid = dbg_v ? dbg_n : 8 * P.per_xcd;
+    const int grid = dbg_v ? dbg_n : P.n_elems;
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, (launch_vertex_pass<PASS, TV>(P, grid, lds, c->stream)));
     if (c->timing) {

@@ -29,7 +29,6 @@ struct PassArgs {
     // breakpoints of the logarithmic domains [2ref, val], (eval+1) per (mesh family, Ω index),
     // computed on the host with libm pow; entry 0 < 0 marks an empty domain
     const double *logpts;
-    int per_xcd;              // elements handled by each XCD slice of the grid
     double L;
     int eval;
     double atol, rtol;
@@ -72,7 +71,7 @@ struct Lds {
     const uint32_t *code;   // [MAX_DESC] frequency-transfer codes of this element's vertex slots
     Ctl *ctl;      // [group]
     int *own;      // [group]: owner whose node the group evaluated this step, -1 = none
-    double *scr;   // 64 x 4 doubles of lane exchange scratch
+    double *scr;   // lane exchange scratch, [4][64] doubles (one row per quantity)
     Desc *desc;    // [group][MAX_DESC]
     double *bub;   // [group][2][2]: (G, K) of the two bubble legs
 };
@@ -417,10 +416,10 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
         TV gdsum = -tp * ((TV)3.0 * v1d * v4s + v1d * v4d + (TV)3.0 * v3s * v2d + v3d * v2d);
         // overlap gather Σ_(j1,j2,m) -p (-2) m (2S) v1[j1] v2[j2]  (flow_su2_t.jl:59-74)
         double *sc = S.scr;
-        sc[4 * slot + 0] = (double)v1s;
-        sc[4 * slot + 1] = (double)v1d;
-        sc[4 * slot + 2] = (double)v2s;
-        sc[4 * slot + 3] = (double)v2d;
+        sc[0 * 64 + slot] = (double)v1s;
+        sc[1 * 64 + slot] = (double)v1d;
+        sc[2 * 64 + slot] = (double)v2s;
+        sc[3 * 64 + slot] = (double)v2d;
         __syncthreads();
         const TV cc = -tp * (TV)(-2.0);
         const TV S2 = (TV)y.S2;
@@ -429,10 +428,10 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
             const uint32_t row = S.ov[r];
             const int j1 = row & 255, j2 = (row >> 8) & 255;
             const TV m = (TV)(double)(row >> 16);
-            const double *e1 = sc + 4 * (gbase + j1);
-            const double *e2 = sc + 4 * (gbase + j2);
-            gsum += cc * m * S2 * (TV)e1[0] * (TV)e2[2];
-            gdsum += cc * m * S2 * (TV)e1[1] * (TV)e2[3];
+            const double *e1 = sc + gbase + j1;
+            const double *e2 = sc + gbase + j2;
+            gsum += cc * m * S2 * (TV)e1[0] * (TV)e2[2 * 64];
+            gdsum += cc * m * S2 * (TV)e1[64] * (TV)e2[3 * 64];
         }
         Gs = gsum;
         Gd = gdsum;
@@ -496,7 +495,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
 
     const Sys &y = P.y;
     const int bid = blockIdx.x;
-    const int e_idx = (bid & 7) * P.per_xcd + (bid >> 3);  // XCD-contiguous element ranges
+    // consecutive blocks go to different XCDs, so each XCD receives an even sample of the
+    // element list (channels and Ω differ widely in cost)
+    const int e_idx = bid;
     if (!P.dbg_v && e_idx >= P.n_elems) return;
     const int lane = threadIdx.x;
 
@@ -683,14 +684,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
         // contributions dv * Γ of every group, in group (= pooled node) order
         double *sc = S.scr;
         const double v0 = use ? dx * (double)Gs : 0.0, v1 = use ? dx * (double)Gd : 0.0;
-        sc[2 * lane + 0] = v0;
-        sc[2 * lane + 1] = v1;
+        sc[lane] = v0;
+        sc[64 + lane] = v1;
         if (real_lane && i == 0) S.own[g] = use ? own : -1;
         __syncthreads();
         if (mode == M_TAIL) {
             for (int e = 0; e < G && tk + e < 2; ++e) {
-                acc0 += sc[2 * (e * GS + i) + 0];
-                acc1 += sc[2 * (e * GS + i) + 1];
+                acc0 += sc[e * GS + i];
+                acc1 += sc[64 + e * GS + i];
             }
             __syncthreads();
             tk += G;
@@ -701,7 +702,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
         if (real_lane && !done) {
             for (int e = 0; e < G; ++e) {
                 if (S.own[e] != g) continue;
-                const double u0 = sc[2 * (e * GS + i) + 0], u1 = sc[2 * (e * GS + i) + 1];
+                const double u0 = sc[e * GS + i], u1 = sc[64 + e * GS + i];
                 if (init && cnt < 3) {
                     b10 += u0; b11 += u1;
                 } else {
@@ -723,16 +724,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
             r10 += 16.0 / 15.0 * b20;
             r11 += 16.0 / 15.0 * b21;
             __syncthreads();
-            sc[4 * lane + 0] = r10;
-            sc[4 * lane + 1] = r11;
-            sc[4 * lane + 2] = b20;
-            sc[4 * lane + 3] = b21;
+            sc[lane] = r10;
+            sc[64 + lane] = r11;
+            sc[128 + lane] = b20;
+            sc[192 + lane] = b21;
             __syncthreads();
             if (need_check) {
                 double q1 = 0.0, q2 = 0.0, q3 = 0.0;
                 for (int jj = 0; jj < y.ns; ++jj) {
-                    const double *e = sc + 4 * (gbase + jj);
-                    const double x0 = e[0], x1 = e[1], z0 = e[2], z1 = e[3];
+                    const double *e = sc + gbase + jj;
+                    const double x0 = e[0], x1 = e[64], z0 = e[128], z1 = e[192];
                     q1 += x0 * x0; q1 += x1 * x1;
                     q2 += z0 * z0; q2 += z1 * z1;
                     const double d0 = x0 - z0, d1 = x1 - z1;
@@ -763,15 +764,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
         }
         if (__ballot(real_lane && !done) != 0) continue;
         // round complete: ordered accumulation of its sub-interval results
-        sc[2 * lane + 0] = res0;
-        sc[2 * lane + 1] = res1;
+        sc[lane] = res0;
+        sc[64 + lane] = res1;
         __syncthreads();
         for (int gg = 0; gg < G; ++gg) {
             const int jj = r * G + gg;
             double ta, tb, ts;
             if (jj < nsub && subinterval(jj, P.eval, ref, logpts, ta, tb, ts)) {
-                acc0 += sc[2 * (gg * GS + i) + 0];
-                acc1 += sc[2 * (gg * GS + i) + 1];
+                acc0 += sc[gg * GS + i];
+                acc1 += sc[64 + gg * GS + i];
             }
         }
         __syncthreads();
