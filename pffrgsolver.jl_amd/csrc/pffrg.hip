@@ -246,9 +246,13 @@ __global__ void k_scale(double *x, double fac, long n) {
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) x[t] *= fac;
 }
 
-__global__ void k_to_f32(const double *__restrict__ x, float *__restrict__ y, long n) {
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
-        y[t] = (float)x[t];
+// gather copy: y[2t + comp] = x[comp * complen + t]
+template <typename T>
+__global__ void k_interleave(const double *__restrict__ x, T *__restrict__ y, long complen) {
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < complen; t += (long)gridDim.x * blockDim.x) {
+        y[2 * t] = (T)x[t];
+        y[2 * t + 1] = (T)x[complen + t];
+    }
 }
 
 // max |q3| over the six q3 blocks (channel.jl:406-413): block max then 64-bit atomic max
@@ -277,10 +281,14 @@ __global__ void k_absmax_q3(const double *g, Sys y, unsigned long long *out) {
 }  // namespace
 
 // ======================================================================= host side
+// One action on the device: Σ, bare and the vertex in the reference's layout, plus the
+// gather copies read by the vertex passes, built on demand: the same vertex with the two
+// SU(2) components of each entry stored next to each other, in FP64 and in FP32.
 struct Slot {
     double *Sigma = nullptr, *bare = nullptr, *gamma = nullptr;
-    float *gamma32 = nullptr;
-    bool f32_valid = false;
+    double *gat64 = nullptr;
+    float *gat32 = nullptr;
+    bool gat64_valid = false, gat32_valid = false;
 };
 
 struct pffrg_ctx {
@@ -386,19 +394,28 @@ int grid_for(long n) {
     return (int)b;
 }
 
-int ensure_f32(pffrg_ctx *c, int s) {
+int ensure_gather(pffrg_ctx *c, int s, bool f32) {
     Slot &sl = c->slots[s];
-    const long n = 2 * c->sys.complen;
-    if (!sl.gamma32) HIPCHK(c, hipMalloc(&sl.gamma32, n * sizeof(float)));
-    if (!sl.f32_valid) {
-        hipLaunchKernelGGL(k_to_f32, dim3(grid_for(n)), dim3(256), 0, c->stream, sl.gamma, sl.gamma32, n);
-        HIPCHK(c, hipGetLastError());
-        sl.f32_valid = true;
+    const long n = c->sys.complen;
+    if (f32) {
+        if (!sl.gat32) HIPCHK(c, hipMalloc(&sl.gat32, 2 * n * sizeof(float)));
+        if (!sl.gat32_valid) {
+            hipLaunchKernelGGL(k_interleave<float>, dim3(grid_for(n)), dim3(256), 0, c->stream, sl.gamma, sl.gat32, n);
+            HIPCHK(c, hipGetLastError());
+            sl.gat32_valid = true;
+        }
+    } else {
+        if (!sl.gat64) HIPCHK(c, hipMalloc(&sl.gat64, 2 * n * sizeof(double)));
+        if (!sl.gat64_valid) {
+            hipLaunchKernelGGL(k_interleave<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, sl.gamma, sl.gat64, n);
+            HIPCHK(c, hipGetLastError());
+            sl.gat64_valid = true;
+        }
     }
     return 0;
 }
 
-void invalidate(pffrg_ctx *c, int s) { c->slots[s].f32_valid = false; }
+void invalidate(pffrg_ctx *c, int s) { c->slots[s].gat64_valid = c->slots[s].gat32_valid = false; }
 
 template <int PASS, typename TV>
 int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const double *tol, int mode,
@@ -417,11 +434,11 @@ int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const
     P.ov_rows = c->d_ov_rows;
     const Slot &SA = c->slots[a], &SD = c->slots[d];
     if (std::is_same<TV, float>::value) {
-        P.A = ActV<TV>{(const TV *)SA.gamma32, SA.bare, SA.Sigma};
-        P.D = ActV<TV>{(const TV *)SD.gamma32, SD.bare, SD.Sigma};
+        P.A = ActV<TV>{(const TV *)SA.gat32, SA.bare, SA.Sigma};
+        P.D = ActV<TV>{(const TV *)SD.gat32, SD.bare, SD.Sigma};
     } else {
-        P.A = ActV<TV>{(const TV *)SA.gamma, SA.bare, SA.Sigma};
-        P.D = ActV<TV>{(const TV *)SD.gamma, SD.bare, SD.Sigma};
+        P.A = ActV<TV>{(const TV *)SA.gat64, SA.bare, SA.Sigma};
+        P.D = ActV<TV>{(const TV *)SD.gat64, SD.bare, SD.Sigma};
     }
     P.out = c->slots[out].gamma;
     P.corrs = c->d_corrs;
@@ -467,7 +484,7 @@ int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const
     P.rtol = tol[1];
     P.node_count = c->d_count;
     const size_t lds = (size_t)((c->n_ov + c->sys.ns + 1 + 3) & ~3) * sizeof(uint32_t) +
-                       (size_t)c->sys.G * MAX_DESC * sizeof(Desc);
+                       (size_t)c->sys.G * MAX_DESC * sizeof(DescT<TV>);
     const int grid = dbg_v ? dbg_n : P.n_elems;
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, (launch_vertex_pass<PASS, TV>(P, grid, lds, c->stream)));
@@ -498,12 +515,11 @@ int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const
 template <int PASS>
 int launch_pass(pffrg_ctx *c, double L, int a, int d, int out, int eval, const double *tol, int mode) {
     if (eval < 1) return fail(c, PFFRG_ERR_ARG, "eval must be >= 1");
-    if (c->precision == PFFRG_FP32_MIXED) {
-        int rc;
-        if ((rc = ensure_f32(c, a))) return rc;
-        if ((PASS == P_LEFT || PASS == P_CENTRAL) && (rc = ensure_f32(c, d))) return rc;
-        return launch_pass_t<PASS, float>(c, L, a, d, out, eval, tol, mode);
-    }
+    const bool f32 = c->precision == PFFRG_FP32_MIXED;
+    int rc;
+    if ((rc = ensure_gather(c, a, f32))) return rc;
+    if ((PASS == P_LEFT || PASS == P_CENTRAL) && (rc = ensure_gather(c, d, f32))) return rc;
+    if (f32) return launch_pass_t<PASS, float>(c, L, a, d, out, eval, tol, mode);
     return launch_pass_t<PASS, double>(c, L, a, d, out, eval, tol, mode);
 }
 
@@ -669,7 +685,8 @@ void pffrg_destroy(pffrg_ctx *c) {
         (void)hipFree(s.Sigma);
         (void)hipFree(s.bare);
         (void)hipFree(s.gamma);
-        (void)hipFree(s.gamma32);
+        (void)hipFree(s.gat32);
+        (void)hipFree(s.gat64);
     }
     (void)hipFree(c->d_mesh);
     (void)hipFree(c->d_exch);
@@ -743,7 +760,7 @@ int pffrg_set_lattice(pffrg_ctx *c, const int64_t *exchange, const double *mult,
     const int R = off[ns];
     for (int i = 0; i < ns; ++i)
         if (off[i + 1] < off[i]) return fail(c, PFFRG_ERR_ARG, "ov_off must be non-decreasing");
-    if ((size_t)(R + ns + 4) * 4 + (size_t)c->sys.G * MAX_DESC * sizeof(Desc) > 48 * 1024)
+    if ((size_t)(R + ns + 4) * 4 + (size_t)c->sys.G * MAX_DESC * sizeof(DescT<double>) > 48 * 1024)
         return fail(c, PFFRG_ERR_LIMIT, "overlap table exceeds the LDS budget");
     std::vector<uint32_t> rows(R > 0 ? R : 1);
     for (int r = 0; r < R; ++r) {
@@ -1055,9 +1072,7 @@ int pffrg_probe_integrand(pffrg_ctx *c, int32_t pass, double L, int32_t a, int32
     const uint32_t e = enc(ch, kernel, w1, w2, w3);
     const double tol[2] = {1e-8, 1e-4};
     const bool f32 = c->precision == PFFRG_FP32_MIXED;
-    if (f32) {
-        if ((rc = ensure_f32(c, a)) || (rc = ensure_f32(c, da))) return rc;
-    }
+    if ((rc = ensure_gather(c, a, f32)) || (rc = ensure_gather(c, da, f32))) return rc;
 #define PROBE(PS)                                                                                                  \
     rc = f32 ? launch_pass_t<PS, float>(c, L, a, da, a, 1, tol, 0, dv, dout, n, e)                               \
              : launch_pass_t<PS, double>(c, L, a, da, a, 1, tol, 0, dv, dout, n, e);

@@ -45,12 +45,24 @@ struct PassArgs {
 
 // Interpolation descriptor of one buffer (one channel of one vertex slot) at one node, built
 // once per node by one lane and read by every site lane of the group.
-struct Desc {
+template <typename TV>
+struct DescT {
     int meta;            // kernel (bits 0-2) | exchange << 3 | map << 4 | negative density sign << 5
     int off;             // offset of the first corner inside the channel block (site 0)
     int d1, d2, d3;      // offset steps to the upper corner along Ω, ν, ν'
     int pad[3];
-    double w[8];         // corner weights in the reference's corner order
+    TV w[8];             // corner weights in the reference's corner order
+};
+
+template <typename TV>
+struct Vec2;
+template <>
+struct Vec2<float> {
+    using T = float2;
+};
+template <>
+struct Vec2<double> {
+    using T = double2;
 };
 
 constexpr int MAX_GROUPS = 16;  // site groups per wavefront
@@ -72,7 +84,7 @@ struct Lds {
     Ctl *ctl;      // [group]
     int *own;      // [group]: owner whose node the group evaluated this step, -1 = none
     double *scr;   // lane exchange scratch, [4][64] doubles (one row per quantity)
-    Desc *desc;    // [group][MAX_DESC]
+    void *desc;    // DescT<TV>[group][MAX_DESC]
     double *bub;   // [group][2][2]: (G, K) of the two bubble legs
 };
 
@@ -236,8 +248,9 @@ __device__ __forceinline__ double sigma_lookup(double w, const double *sm, const
 // interpolation (channel.jl:42-305) in the reference's product order.  The unused dimensions of
 // q1/q2 carry the neutral bracket (index 0, weights 1 and 0), which reproduces the reference's
 // weights exactly and leaves the extra corners at weight zero.
+template <typename TV>
 __device__ __forceinline__ void build_desc_u(const Lds &S, const Sys &y, uint32_t code, int c, double W, double a1,
-                                             double a2, double v, Desc &d) {
+                                             double a2, double v, DescT<TV> &d) {
     double w = coord(code >> 8 & 255u, W, v, a1, a2);
     double x = coord(code >> 16 & 255u, W, v, a1, a2);
     double z = coord(code >> 24, W, v, a1, a2);
@@ -269,50 +282,113 @@ __device__ __forceinline__ void build_desc_u(const Lds &S, const Sys &y, uint32_
     d.d1 = ns * (h1 - l1);
     d.d2 = ns * nO * (h2 - l2);
     d.d3 = ns * nO * nn * (h3 - l3);
-    d.w[0] = a1w * a2w * a3w; d.w[1] = b1w * a2w * a3w; d.w[2] = a1w * b2w * a3w; d.w[3] = b1w * b2w * a3w;
-    d.w[4] = a1w * a2w * b3w; d.w[5] = b1w * a2w * b3w; d.w[6] = a1w * b2w * b3w; d.w[7] = b1w * b2w * b3w;
+    d.w[0] = (TV)(a1w * a2w * a3w); d.w[1] = (TV)(b1w * a2w * a3w);
+    d.w[2] = (TV)(a1w * b2w * a3w); d.w[3] = (TV)(b1w * b2w * a3w);
+    d.w[4] = (TV)(a1w * a2w * b3w); d.w[5] = (TV)(b1w * a2w * b3w);
+    d.w[6] = (TV)(a1w * b2w * b3w); d.w[7] = (TV)(b1w * b2w * b3w);
 }
 
-// Both SU(2) components of one channel at one site from a descriptor, corners in the
-// reference's order (first corner initialises, the rest accumulate).  q is the slot's vertex
-// base (wave-uniform); o the 32-bit element offset of the first corner of component 0 and
-// cstep the component stride, so the loads use a scalar base with 32-bit lane offsets.
+// Corners any group of the wavefront needs for one channel (wave-uniform): 8, 4, 2 or 0.
+__device__ __forceinline__ int wave_corners(int meta) {
+    const int k = meta & 7;
+    if (__ballot(k == 4) != 0ull) return 8;
+    if (__ballot(k == 2 || k == 3) != 0ull) return 4;
+    if (__ballot(k == 1) != 0ull) return 2;
+    return 0;
+}
+
+template <typename V2>
+__device__ __forceinline__ void load_corners(const V2 *__restrict__ q, uint32_t o, uint32_t c1, uint32_t c2, uint32_t c3,
+                                             int n, V2 *x) {
+    x[0] = q[o];
+    x[1] = q[o + c1];
+    if (n >= 4) {
+        x[2] = q[o + c2];
+        x[3] = q[o + c1 + c2];
+    }
+    if (n >= 8) {
+        x[4] = q[o + c3];
+        x[5] = q[o + c1 + c3];
+        x[6] = q[o + c2 + c3];
+        x[7] = q[o + c1 + c2 + c3];
+    }
+}
+
+// Interpolated value of both SU(2) components: the first corner initialises, the rest
+// accumulate in the reference's corner order (channel.jl:42-305).  Corners beyond the
+// buffer's own count carry weight zero.
+template <typename TV, typename V2>
+__device__ __forceinline__ void corner_sum(const DescT<TV> &d, int n, const V2 *x, TV &r0, TV &r1) {
+    r0 = d.w[0] * x[0].x;
+    r1 = d.w[0] * x[0].y;
+    r0 += d.w[1] * x[1].x; r1 += d.w[1] * x[1].y;
+    if (n >= 4) {
+        r0 += d.w[2] * x[2].x; r1 += d.w[2] * x[2].y;
+        r0 += d.w[3] * x[3].x; r1 += d.w[3] * x[3].y;
+    }
+    if (n >= 8) {
+        r0 += d.w[4] * x[4].x; r1 += d.w[4] * x[4].y;
+        r0 += d.w[5] * x[5].x; r1 += d.w[5] * x[5].y;
+        r0 += d.w[6] * x[6].x; r1 += d.w[6] * x[6].y;
+        r0 += d.w[7] * x[7].x; r1 += d.w[7] * x[7].y;
+    }
+}
+
+// Vertex of one slot at one site: bare + Σ_channels sign·interpolation (get_Γ_avx!,
+// action_su2.jl:95-134).  The gather copy stores (spin, density) pairs, so one load fetches
+// both components of a corner.  In FP32 the loads of all three channels are issued before
+// any is consumed; in FP64 one channel at a time to bound registers.
 template <typename TV>
-__device__ __forceinline__ void gather(const Desc &d, const TV *__restrict__ q, uint32_t o, uint32_t cstep, TV &r0,
-                                       TV &r1) {
-    const int k = d.meta & 7;
-    const uint32_t c1 = (uint32_t)d.d1, c2 = (uint32_t)d.d2, c3 = (uint32_t)d.d3;
-    const uint32_t p = o + cstep;
-    if (k == 4) {
-        const TV x0 = q[o], y0 = q[p], x1 = q[o + c1], y1 = q[p + c1], x2 = q[o + c2], y2 = q[p + c2];
-        const TV x3 = q[o + c1 + c2], y3 = q[p + c1 + c2], x4 = q[o + c3], y4 = q[p + c3];
-        const TV x5 = q[o + c1 + c3], y5 = q[p + c1 + c3], x6 = q[o + c2 + c3], y6 = q[p + c2 + c3];
-        const TV x7 = q[o + c1 + c2 + c3], y7 = q[p + c1 + c2 + c3];
-        TV v0 = (TV)d.w[0] * x0, v1 = (TV)d.w[0] * y0;
-        v0 += (TV)d.w[1] * x1; v1 += (TV)d.w[1] * y1;
-        v0 += (TV)d.w[2] * x2; v1 += (TV)d.w[2] * y2;
-        v0 += (TV)d.w[3] * x3; v1 += (TV)d.w[3] * y3;
-        v0 += (TV)d.w[4] * x4; v1 += (TV)d.w[4] * y4;
-        v0 += (TV)d.w[5] * x5; v1 += (TV)d.w[5] * y5;
-        v0 += (TV)d.w[6] * x6; v1 += (TV)d.w[6] * y6;
-        v0 += (TV)d.w[7] * x7; v1 += (TV)d.w[7] * y7;
-        r0 = v0;
-        r1 = v1;
-    } else if (k == 1) {
-        const TV x0 = q[o], y0 = q[p], x1 = q[o + c1], y1 = q[p + c1];
-        r0 = (TV)d.w[0] * x0;
-        r0 += (TV)d.w[1] * x1;
-        r1 = (TV)d.w[0] * y0;
-        r1 += (TV)d.w[1] * y1;
+__device__ __forceinline__ void gather_slot(const DescT<TV> *ds, const TV *gamma, const int *exch, int site,
+                                            const Sys &y, TV &vs, TV &vd, int &cn) {
+    using V2 = typename Vec2<TV>::T;
+    const V2 *q = reinterpret_cast<const V2 *>(gamma);
+    int meta[3], n[3];
+    uint32_t o[3], c1[3], c2[3], c3[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const DescT<TV> &d = ds[c];
+        meta[c] = d.meta;
+        n[c] = wave_corners(meta[c]);
+        const bool on = (meta[c] & 7) != 0;
+        const int sx = (meta[c] & 8) ? exch[site] : site;
+        const bool map = (meta[c] & 16) != 0;
+        const int chn = (c == 1) ? 1 : ((c == 0) == map ? 2 : 0);
+        o[c] = on ? (uint32_t)(chn * y.chlen + d.off + sx) : 0u;
+        c1[c] = on ? (uint32_t)d.d1 : 0u;
+        c2[c] = on ? (uint32_t)d.d2 : 0u;
+        c3[c] = on ? (uint32_t)d.d3 : 0u;
+    }
+    cn = 0;
+    if (sizeof(TV) == 4) {
+        V2 x[3][8];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (n[c]) load_corners(q, o[c], c1[c], c2[c], c3[c], n[c], x[c]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (meta[c] & 7) {
+                TV r0, r1;
+                corner_sum(ds[c], n[c], x[c], r0, r1);
+                vs += r0;
+                vd += (meta[c] & 32) ? -r1 : r1;
+                cn += corners(meta[c] & 7);
+            }
+        }
     } else {
-        const TV x0 = q[o], y0 = q[p], x1 = q[o + c1], y1 = q[p + c1];
-        const TV x2 = q[o + c2], y2 = q[p + c2], x3 = q[o + c1 + c2], y3 = q[p + c1 + c2];
-        TV v0 = (TV)d.w[0] * x0, v1 = (TV)d.w[0] * y0;
-        v0 += (TV)d.w[1] * x1; v1 += (TV)d.w[1] * y1;
-        v0 += (TV)d.w[2] * x2; v1 += (TV)d.w[2] * y2;
-        v0 += (TV)d.w[3] * x3; v1 += (TV)d.w[3] * y3;
-        r0 = v0;
-        r1 = v1;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (!n[c]) continue;
+            V2 x[8];
+            load_corners(q, o[c], c1[c], c2[c], c3[c], n[c], x);
+            if (meta[c] & 7) {
+                TV r0, r1;
+                corner_sum(ds[c], n[c], x, r0, r1);
+                vs += r0;
+                vd += (meta[c] & 32) ? -r1 : r1;
+                cn += corners(meta[c] & 7);
+            }
+        }
     }
 }
 
@@ -355,7 +431,7 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
     const int nslot = (ch == 1) ? 4 : 2;
     const int ntask = 2 + 3 * nslot;
     const int gs = real_lane ? g : 0;
-    Desc *dg = S.desc + gs * MAX_DESC;
+    DescT<TV> *dg = reinterpret_cast<DescT<TV> *>(S.desc) + gs * MAX_DESC;
     double *bg = S.bub + gs * 4;
     if (real_lane) {
 #pragma clang loop unroll(disable)
@@ -385,21 +461,8 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
         const ActV<TV> &V = (cfg & 8) ? P.D : P.A;
         const int site = (k < 2) ? i : 0;
         TV vs = (TV)V.bare[site], vd = (TV)V.bare[y.ns + site];
-        int cn = 0;
-#pragma clang loop unroll(disable)
-        for (int c = 0; c < 3; ++c) {
-            const Desc &d = dg[3 * k + c];
-            const int meta = d.meta;
-            if (!(meta & 7)) continue;
-            const int sx = (meta & 8) ? S.exch[site] : site;
-            const bool map = (meta & 16) != 0;
-            const int chn = (c == 1) ? 1 : ((c == 0) == map ? 2 : 0);
-            TV r0, r1;
-            gather<TV>(d, V.gamma, (uint32_t)(chn * y.chlen + d.off + sx), (uint32_t)y.complen, r0, r1);
-            vs += r0;
-            vd += (meta & 32) ? -r1 : r1;
-            cn += corners(meta & 7);
-        }
+        int cn;
+        gather_slot<TV>(dg + 3 * k, V.gamma, S.exch, site, y, vs, vd, cn);
         if (k == 0) { v1s = vs; v1d = vd; nc += cn; }
         else if (k == 1) { v2s = vs; v2d = vd; nc += cn; }
         else if (k == 2) { v3s = vs; v3d = vd; nc0 += cn; }
@@ -532,7 +595,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
     S.ov_off = reinterpret_cast<const int *>(s_dyn + n_ov);
     S.scr = s_scr;
     // descriptors follow the overlap table in the dynamic LDS block, 16-byte aligned
-    S.desc = reinterpret_cast<Desc *>(s_dyn + ((n_ov + y.ns + 1 + 3) & ~3));
+    S.desc = reinterpret_cast<void *>(s_dyn + ((n_ov + y.ns + 1 + 3) & ~3));
     S.bub = s_bub;
     S.ctl = s_ctl;
     S.own = s_own;
