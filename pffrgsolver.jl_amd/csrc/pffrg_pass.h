@@ -79,7 +79,8 @@ struct Lds {
     const double *Os, *ns, *Ot, *nt, *sig, *sigA, *sigD;
     const int *exch, *ov_off;
     const uint32_t *ov;
-    const double *inv;      // inverse mesh spacings: σ | Ωs | νs | Ωt | νt, MAXN each
+    const double *inv;      // inverse mesh spacings: σ | Ωs | νs | Ωt | νt, mp each
+    int mp;                 // stride of the mesh tables
     const uint32_t *code;   // [MAX_DESC] frequency-transfer codes of this element's vertex slots
     Ctl *ctl;      // [group]
     int *own;      // [group]: owner whose node the group evaluated this step, -1 = none
@@ -87,6 +88,31 @@ struct Lds {
     void *desc;    // DescT<TV>[group][MAX_DESC]
     double *bub;   // [group][2][2]: (G, K) of the two bubble legs
 };
+
+// Byte offsets of the dynamic LDS block of k_vertex_pass, sized to the problem: the five
+// meshes and their inverse-spacing tables (stride mp doubles), Σ of both actions, the lane
+// exchange rows, bubble legs, owner states, descriptors, overlap rows and row offsets.
+struct PassLds {
+    int mp, o_sig, o_scr, o_bub, o_ctl, o_desc, o_ov, o_ovoff, total;
+};
+
+__host__ __device__ inline PassLds pass_lds(const Sys &y, int n_ov, int desc_size) {
+    PassLds L;
+    int mp = y.nsig;
+    if (y.nO > mp) mp = y.nO;
+    if (y.nn > mp) mp = y.nn;
+    L.mp = (mp + 1) & ~1;
+    int o = 10 * L.mp * 8;
+    L.o_sig = o; o += 2 * L.mp * 8;
+    L.o_scr = o; o += 4 * 64 * 8;
+    L.o_bub = o; o += y.G * 4 * 8;
+    L.o_ctl = o; o += y.G * (int)sizeof(Ctl);
+    L.o_desc = o; o += y.G * MAX_DESC * desc_size;
+    L.o_ov = o; o += n_ov * 4;
+    L.o_ovoff = o; o += (y.ns + 1) * 4;
+    L.total = o;
+    return L;
+}
 
 // Vertex-slot configuration of an integrand: bits 0-2 = enabled channels (s, t, u; the
 // reference's ch_s/ch_t/ch_u keywords), bit 3 = the slot reads the second action D instead
@@ -213,7 +239,8 @@ __device__ __forceinline__ void bracket(ST val, const ST *m, const ST *inv, int 
 #pragma unroll
     for (int step = 32; step > 0; step >>= 1) {
         const int t = base + step;
-        if (t <= n - 1 && m[t - 1] < val) base = t;
+        const int tc = (t <= n - 1) ? t : n - 1;
+        if (t <= n - 1 && m[tc - 1] < val) base = t;
     }
     hi = base;
     const ST mh = m[base];
@@ -263,8 +290,8 @@ __device__ __forceinline__ void build_desc_u(const Lds &S, const Sys &y, uint32_
     const int nO = y.nO, nn = y.nn, ns = y.ns;
     const double *Om = tch ? S.Ot : S.Os;
     const double *nu = tch ? S.nt : S.ns;
-    const double *iO = S.inv + (tch ? 3 : 1) * MAXN;
-    const double *inu = S.inv + (tch ? 4 : 2) * MAXN;
+    const double *iO = S.inv + (tch ? 3 : 1) * S.mp;
+    const double *inu = S.inv + (tch ? 4 : 2) * S.mp;
     const bool out = Om[nO - 1] < w;
     const bool av = nu[nn - 1] < x, avp = nu[nn - 1] < z;
     const int kernel = out ? 0 : (av ? (avp ? 1 : 3) : (avp ? 2 : 4));
@@ -548,12 +575,10 @@ __device__ __forceinline__ bool subinterval(int j, int eval, double ref, const d
 
 template <int PASS, typename TV>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_WAVES_PER_EU))) void k_vertex_pass(PassArgs<TV> P) {
-    extern __shared__ uint32_t s_dyn[];  // overlap rows then ov_off
-    __shared__ double s_mesh[10 * MAXN];  // meshes, then their inverse spacings
-    __shared__ double s_sigA[MAXN], s_sigD[MAXN];
+    extern __shared__ __align__(16) unsigned char s_dyn[];
     __shared__ uint32_t s_code[MAX_DESC];
-    __shared__ double s_scr[64 * 4];
     __shared__ int s_exch[64];
+    __shared__ int s_own[MAX_GROUPS];
     __shared__ unsigned long long s_cnt, s_bytes;
 
     const Sys &y = P.y;
@@ -563,41 +588,44 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
     const int e_idx = bid;
     if (!P.dbg_v && e_idx >= P.n_elems) return;
     const int lane = threadIdx.x;
+    const int n_ov = P.ov_off[y.ns];
+    const PassLds Lay = pass_lds(y, n_ov, (int)sizeof(DescT<TV>));
+    const int mp = Lay.mp;
+    double *s_mesh = reinterpret_cast<double *>(s_dyn);
+    double *s_sigA = reinterpret_cast<double *>(s_dyn + Lay.o_sig), *s_sigD = s_sigA + mp;
+    uint32_t *s_ov = reinterpret_cast<uint32_t *>(s_dyn + Lay.o_ov);
+    int *s_ovoff = reinterpret_cast<int *>(s_dyn + Lay.o_ovoff);
 
     // ---- stage tables in LDS
-    for (int k = lane; k < 10 * MAXN; k += 64) s_mesh[k] = P.mesh[k];
+    for (int k = lane; k < 10 * mp; k += 64) s_mesh[k] = P.mesh[(k / mp) * MAXN + (k % mp)];
     for (int k = lane; k < y.nsig; k += 64) {
         s_sigA[k] = P.A.Sigma[k];
         s_sigD[k] = (PASS == P_KAT) ? P.D.Sigma[k] : 0.0;
     }
     if (lane < y.ns) s_exch[lane] = P.exch[lane];
-    const int n_ov = P.ov_off[y.ns];
-    __shared__ double s_bub[MAX_GROUPS * 4];
-    __shared__ Ctl s_ctl[MAX_GROUPS];
-    __shared__ int s_own[MAX_GROUPS];
-    for (int k = lane; k < n_ov; k += 64) s_dyn[k] = P.ov_rows[k];
-    for (int k = lane; k <= y.ns; k += 64) s_dyn[n_ov + k] = (uint32_t)P.ov_off[k];
+    for (int k = lane; k < n_ov; k += 64) s_ov[k] = P.ov_rows[k];
+    for (int k = lane; k <= y.ns; k += 64) s_ovoff[k] = P.ov_off[k];
     if (lane == 0) { s_cnt = 0; s_bytes = 0; }
     __syncthreads();
 
     Lds S;
     S.sig = s_mesh;
-    S.Os = s_mesh + MAXN;
-    S.ns = s_mesh + 2 * MAXN;
-    S.Ot = s_mesh + 3 * MAXN;
-    S.nt = s_mesh + 4 * MAXN;
-    S.inv = s_mesh + 5 * MAXN;
+    S.Os = s_mesh + mp;
+    S.ns = s_mesh + 2 * mp;
+    S.Ot = s_mesh + 3 * mp;
+    S.nt = s_mesh + 4 * mp;
+    S.inv = s_mesh + 5 * mp;
+    S.mp = mp;
     S.code = s_code;
     S.sigA = s_sigA;
     S.sigD = s_sigD;
     S.exch = s_exch;
-    S.ov = s_dyn;
-    S.ov_off = reinterpret_cast<const int *>(s_dyn + n_ov);
-    S.scr = s_scr;
-    // descriptors follow the overlap table in the dynamic LDS block, 16-byte aligned
-    S.desc = reinterpret_cast<void *>(s_dyn + ((n_ov + y.ns + 1 + 3) & ~3));
-    S.bub = s_bub;
-    S.ctl = s_ctl;
+    S.ov = s_ov;
+    S.ov_off = s_ovoff;
+    S.scr = reinterpret_cast<double *>(s_dyn + Lay.o_scr);
+    S.desc = reinterpret_cast<void *>(s_dyn + Lay.o_desc);
+    S.bub = reinterpret_cast<double *>(s_dyn + Lay.o_bub);
+    S.ctl = reinterpret_cast<Ctl *>(s_dyn + Lay.o_ctl);
     S.own = s_own;
 
     int ch, kernel, w1, w2, w3;
