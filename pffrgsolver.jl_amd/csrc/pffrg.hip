@@ -483,7 +483,7 @@ int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const
     P.atol = tol[0];
     P.rtol = tol[1];
     P.node_count = c->d_count;
-    const size_t lds = (size_t)pass_lds(c->sys, c->n_ov, (int)sizeof(DescT<TV>)).total;
+    const size_t lds = (size_t)pass_lds(c->sys, c->n_ov, (int)sizeof(DescT<TV>), std::is_same<typename PassScalar<TV>::T, float>::value).total;
     const int grid = dbg_v ? dbg_n : P.n_elems;
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, (launch_vertex_pass<PASS, TV>(P, grid, lds, c->stream)));
@@ -759,7 +759,7 @@ int pffrg_set_lattice(pffrg_ctx *c, const int64_t *exchange, const double *mult,
     const int R = off[ns];
     for (int i = 0; i < ns; ++i)
         if (off[i + 1] < off[i]) return fail(c, PFFRG_ERR_ARG, "ov_off must be non-decreasing");
-    if ((size_t)pass_lds(c->sys, R, (int)sizeof(DescT<double>)).total > 48 * 1024)
+    if ((size_t)pass_lds(c->sys, R, (int)sizeof(DescT<double>), false).total > 48 * 1024)
         return fail(c, PFFRG_ERR_LIMIT, "overlap table exceeds the LDS budget");
     std::vector<uint32_t> rows(R > 0 ? R : 1);
     for (int r = 0; r < R; ++r) {

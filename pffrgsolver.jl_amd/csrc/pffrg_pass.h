@@ -50,8 +50,16 @@ struct DescT {
     int meta;            // kernel (bits 0-2) | exchange << 3 | map << 4 | negative density sign << 5
     int off;             // offset of the first corner inside the channel block (site 0)
     int d1, d2, d3;      // offset steps to the upper corner along Ω, ν, ν'
-    int pad[3];
+    int pad[sizeof(TV) == 4 ? 7 : 3];  // 80- or 96-byte stride: the groups' descriptors sit in distinct LDS banks
     TV w[8];             // corner weights in the reference's corner order
+};
+
+// Scalar type of the per-node work (bubble legs, frequency transfer, brackets, weights).  FP64
+// in both modes: an FP32 variant measured no faster, the pass being latency-bound, and FP64
+// keeps the FP32-mixed mode's node placement and adaptive decisions those of the FP64 path.
+template <typename TV>
+struct PassScalar {
+    using T = double;
 };
 
 template <typename TV>
@@ -67,6 +75,15 @@ struct Vec2<double> {
 
 constexpr int MAX_GROUPS = 16;  // site groups per wavefront
 constexpr int MAX_DESC = 12;    // 4 vertex slots x 3 channels
+constexpr int SCR_ROW = 80;     // padded lane rows: (G-1)(GS+1)+GS-1 <= 78, 79 = idle lanes
+
+// Meshes, inverse-spacing tables and the two Σ arrays in LDS, in the scalar type of the
+// per-node work (FP64 path: the FP64 tables; FP32-mixed path: FP32 copies of them).
+template <typename ST>
+struct MeshV {
+    const ST *sig, *Os, *ns, *Ot, *nt, *inv, *sigA, *sigD;
+    int mp;
+};
 
 // Published state of one sub-interval owner (one per group), read by the groups that
 // evaluate its pooled nodes.
@@ -81,10 +98,12 @@ struct Lds {
     const uint32_t *ov;
     const double *inv;      // inverse mesh spacings: σ | Ωs | νs | Ωt | νt, mp each
     int mp;                 // stride of the mesh tables
+    MeshV<float> mf;        // FP32 copies (FP32-mixed passes only)
     const uint32_t *code;   // [MAX_DESC] frequency-transfer codes of this element's vertex slots
     Ctl *ctl;      // [group]
     int *own;      // [group]: owner whose node the group evaluated this step, -1 = none
-    double *scr;   // lane exchange scratch, [4][64] doubles (one row per quantity)
+    double *scr;   // lane exchange scratch, [4][SCR_ROW] doubles (one row per quantity); lane (g, i)
+                   // at g*(GS+1)+i so that equal sites of different groups fall in different banks
     void *desc;    // DescT<TV>[group][MAX_DESC]
     double *bub;   // [group][2][2]: (G, K) of the two bubble legs
 };
@@ -93,10 +112,10 @@ struct Lds {
 // meshes and their inverse-spacing tables (stride mp doubles), Σ of both actions, the lane
 // exchange rows, bubble legs, owner states, descriptors, overlap rows and row offsets.
 struct PassLds {
-    int mp, o_sig, o_scr, o_bub, o_ctl, o_desc, o_ov, o_ovoff, total;
+    int mp, o_sig, o_scr, o_bub, o_ctl, o_desc, o_mf, o_ov, o_ovoff, total;
 };
 
-__host__ __device__ inline PassLds pass_lds(const Sys &y, int n_ov, int desc_size) {
+__host__ __device__ inline PassLds pass_lds(const Sys &y, int n_ov, int desc_size, bool f32) {
     PassLds L;
     int mp = y.nsig;
     if (y.nO > mp) mp = y.nO;
@@ -104,10 +123,11 @@ __host__ __device__ inline PassLds pass_lds(const Sys &y, int n_ov, int desc_siz
     L.mp = (mp + 1) & ~1;
     int o = 10 * L.mp * 8;
     L.o_sig = o; o += 2 * L.mp * 8;
-    L.o_scr = o; o += 4 * 64 * 8;
+    L.o_scr = o; o += 4 * SCR_ROW * 8;
     L.o_bub = o; o += y.G * 4 * 8;
     L.o_ctl = o; o += y.G * (int)sizeof(Ctl);
     L.o_desc = o; o += y.G * MAX_DESC * desc_size;
+    L.o_mf = o; o += f32 ? 12 * L.mp * 4 : 0;
     L.o_ov = o; o += n_ov * 4;
     L.o_ovoff = o; o += (y.ns + 1) * 4;
     L.total = o;
@@ -189,6 +209,17 @@ __device__ __forceinline__ void slot_freqs(int ch, int slot, int c, double W, do
     }
 }
 
+template <typename ST>
+__device__ __forceinline__ MeshV<ST> meshv(const Lds &S);
+template <>
+__device__ __forceinline__ MeshV<double> meshv<double>(const Lds &S) {
+    return MeshV<double>{S.sig, S.Os, S.ns, S.Ot, S.nt, S.inv, S.sigA, S.sigD, S.mp};
+}
+template <>
+__device__ __forceinline__ MeshV<float> meshv<float>(const Lds &S) {
+    return S.mf;
+}
+
 // Frequency-transfer code of (vertex slot k, channel c): bit 0 = channel enabled, bit 1 = the
 // slot reads action D; bytes 1-3 encode the buffer arguments (w, v, v') of slot_freqs, each
 // as ((sW·W + sv·v) + sa·A)·h with signs s ∈ {0, +, -}, A ∈ {a1, a2}, h ∈ {1, 1/2}
@@ -255,54 +286,41 @@ __device__ __forceinline__ void bracket(ST val, const ST *m, const ST *inv, int 
     }
 }
 
-// get_Σ (Action.jl:8-26) with the table-driven bracket
-__device__ __forceinline__ double sigma_lookup(double w, const double *sm, const double *inv, int n,
-                                               const double *Sig) {
-    const double aw = fabs(w);
-    if (aw <= sm[n - 1]) {
-        int lo, hi;
-        double wl, wh;
-        bracket(aw, sm, inv, n, lo, hi, wl, wh);
-        const double sg = (w > 0.0) ? 1.0 : ((w < 0.0) ? -1.0 : w);
-        return sg * (wl * Sig[lo] + wh * Sig[hi]);
-    }
-    return sm[n - 1] * Sig[n - 1] / w;
-}
-
 // Descriptor of one buffer in straight-line form: frequency transfer from the slot code, the
 // sign folding of get_buffer_{s,t,u} (buffer.jl:88-199), the asymptotic-kernel choice of
 // get_buffer (buffer.jl:51-86), three brackets, and the corner weights and offsets of the
 // interpolation (channel.jl:42-305) in the reference's product order.  The unused dimensions of
 // q1/q2 carry the neutral bracket (index 0, weights 1 and 0), which reproduces the reference's
 // weights exactly and leaves the extra corners at weight zero.
-template <typename TV>
-__device__ __forceinline__ void build_desc_u(const Lds &S, const Sys &y, uint32_t code, int c, double W, double a1,
-                                             double a2, double v, DescT<TV> &d) {
-    double w = coord(code >> 8 & 255u, W, v, a1, a2);
-    double x = coord(code >> 16 & 255u, W, v, a1, a2);
-    double z = coord(code >> 24, W, v, a1, a2);
-    const bool nw = w < 0.0, nx = x < 0.0, nz = z < 0.0;
+template <typename TV, typename ST>
+__device__ __forceinline__ void build_desc_u(const Lds &S, const Sys &y, uint32_t code, int c, ST W, ST a1, ST a2, ST v,
+                                             DescT<TV> &d) {
+    const MeshV<ST> M = meshv<ST>(S);
+    ST w = coord<ST>(code >> 8 & 255u, W, v, a1, a2);
+    ST x = coord<ST>(code >> 16 & 255u, W, v, a1, a2);
+    ST z = coord<ST>(code >> 24, W, v, a1, a2);
+    const bool nw = w < (ST)0, nx = x < (ST)0, nz = z < (ST)0;
     w = nw ? -w : w;
     x = nx ? -x : x;
     z = nz ? -z : z;
     const bool tch = (c == 1);
     const bool ex = !tch && (nw != nx), mp = !tch && (nx != nz), negd = (nx != nz);
     const int nO = y.nO, nn = y.nn, ns = y.ns;
-    const double *Om = tch ? S.Ot : S.Os;
-    const double *nu = tch ? S.nt : S.ns;
-    const double *iO = S.inv + (tch ? 3 : 1) * S.mp;
-    const double *inu = S.inv + (tch ? 4 : 2) * S.mp;
+    const ST *Om = tch ? M.Ot : M.Os;
+    const ST *nu = tch ? M.nt : M.ns;
+    const ST *iO = M.inv + (tch ? 3 : 1) * M.mp;
+    const ST *inu = M.inv + (tch ? 4 : 2) * M.mp;
     const bool out = Om[nO - 1] < w;
     const bool av = nu[nn - 1] < x, avp = nu[nn - 1] < z;
     const int kernel = out ? 0 : (av ? (avp ? 1 : 3) : (avp ? 2 : 4));
     int l1, h1, l2, h2, l3, h3;
-    double a1w, b1w, a2w, b2w, a3w, b3w;
-    bracket(w, Om, iO, nO, l1, h1, a1w, b1w);
-    bracket(x, nu, inu, nn, l2, h2, a2w, b2w);
-    bracket(z, nu, inu, nn, l3, h3, a3w, b3w);
+    ST a1w, b1w, a2w, b2w, a3w, b3w;
+    bracket<ST>(w, Om, iO, nO, l1, h1, a1w, b1w);
+    bracket<ST>(x, nu, inu, nn, l2, h2, a2w, b2w);
+    bracket<ST>(z, nu, inu, nn, l3, h3, a3w, b3w);
     if (kernel == 3) { l2 = l3; h2 = h3; a2w = a3w; b2w = b3w; }
-    if (kernel != 4) { l3 = 0; h3 = 0; a3w = 1.0; b3w = 0.0; }
-    if (kernel == 1) { l2 = 0; h2 = 0; a2w = 1.0; b2w = 0.0; }
+    if (kernel != 4) { l3 = 0; h3 = 0; a3w = (ST)1; b3w = (ST)0; }
+    if (kernel == 1) { l2 = 0; h2 = 0; a2w = (ST)1; b2w = (ST)0; }
     const int base = (kernel == 2) ? (int)y.q21 : (kernel == 3) ? (int)y.q22 : (kernel == 4) ? (int)y.q3 : 0;
     d.meta = ((code & 1u) && kernel) ? (kernel | (ex ? 8 : 0) | (mp ? 16 : 0) | (negd ? 32 : 0)) : 0;
     d.off = base + ns * (l1 + nO * (l2 + nn * l3));
@@ -420,27 +438,39 @@ __device__ __forceinline__ void gather_slot(const DescT<TV> *ds, const TV *gamma
 }
 
 // One leg x of the bubble: G(x) and, for the Katanin bubble, K(x) = S(x) + G(x)² dΣ(x)
-// (bubbles.jl:17-80; S reuses G as the reference's get_S recomputes the same value).
-template <bool KAT>
-__device__ __forceinline__ void bubble_leg(double L, double x, const double *sm, const double *inv, int n,
-                                           const double *Sig, const double *dSig, double &G, double &K) {
-    G = 0.0;
-    double G0 = 0.0;
-    if (fabs(x) > 1e-8) {
-        // get_G (bubbles.jl:17-36)
-        const double s = sigma_lookup(x, sm, inv, n, Sig);
-        G0 = G_bare<double>(L, x);
-        const double denom = (x * s >= 0.0) ? 1.0 / G0 + s : 1.0 / G0;
-        G = 1.0 / denom;
+// (bubbles.jl:17-80; S reuses G as the reference's get_S recomputes the same value).  Σ and
+// dΣ are interpolated with one shared bracket of |x| (get_Σ, Action.jl:8-26).
+template <bool KAT, typename ST>
+__device__ __forceinline__ void bubble_leg(ST L, ST x, const MeshV<ST> &M, int n, ST &G, ST &K) {
+    const ST ax = fabs(x);
+    ST s, ds = (ST)0;
+    if (ax <= M.sig[n - 1]) {
+        int lo, hi;
+        ST wl, wh;
+        bracket<ST>(ax, M.sig, M.inv, n, lo, hi, wl, wh);
+        const ST sg = (x > (ST)0) ? (ST)1 : ((x < (ST)0) ? (ST)-1 : x);
+        s = sg * (wl * M.sigA[lo] + wh * M.sigA[hi]);
+        if (KAT) ds = sg * (wl * M.sigD[lo] + wh * M.sigD[hi]);
+    } else {
+        s = M.sig[n - 1] * M.sigA[n - 1] / x;
+        if (KAT) ds = M.sig[n - 1] * M.sigD[n - 1] / x;
     }
-    K = 0.0;
+    G = (ST)0;
+    ST G0 = (ST)0;
+    if (ax > (ST)1e-8) {
+        // get_G (bubbles.jl:17-36)
+        G0 = G_bare<ST>(L, x);
+        const ST denom = (x * s >= (ST)0) ? (ST)1 / G0 + s : (ST)1 / G0;
+        G = (ST)1 / denom;
+    }
+    K = (ST)0;
     if (KAT) {
-        double S = 0.0;
-        if (fabs(x) > 1e-8) {
-            const double r = G / G0;
-            S = r * r * exp(-(x * x) / (L * L)) * 2.0 * x / (L * L * L);
+        ST Sx = (ST)0;
+        if (ax > (ST)1e-8) {
+            const ST r = G / G0;
+            Sx = r * r * exp(-(x * x) / (L * L)) * (ST)2 * x / (L * L * L);
         }
-        K = S + G * G * sigma_lookup(x, sm, inv, n, dSig);
+        K = Sx + G * G * ds;
     }
 }
 
@@ -455,6 +485,7 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
                                           double vwp, double v, int i, int slot, int g, int gbase, bool real_lane,
                                           TV &Gs, TV &Gd, int &nc) {
     const Sys &y = P.y;
+    using ST = typename PassScalar<TV>::T;
     const int nslot = (ch == 1) ? 4 : 2;
     const int ntask = 2 + 3 * nslot;
     const int gs = real_lane ? g : 0;
@@ -464,21 +495,21 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
 #pragma clang loop unroll(disable)
         for (int t = i; t < ntask; t += y.GS) {
             if (t < 2) {
-                double x1, x2;
-                bubble_args(ch, W, v, x1, x2);
-                double G, K;
-                bubble_leg<PASS == P_KAT>(P.L, t == 0 ? x1 : x2, S.sig, S.inv, y.nsig, S.sigA, S.sigD, G, K);
-                bg[2 * t] = G;
-                bg[2 * t + 1] = K;
+                ST x1, x2;
+                bubble_args<ST>(ch, (ST)W, (ST)v, x1, x2);
+                ST G, K;
+                bubble_leg<PASS == P_KAT, ST>((ST)P.L, t == 0 ? x1 : x2, meshv<ST>(S), y.nsig, G, K);
+                bg[2 * t] = (double)G;
+                bg[2 * t + 1] = (double)K;
             } else {
-                build_desc_u(S, y, S.code[t - 2], (t - 2) % 3, W, vw, vwp, v, dg[t - 2]);
+                build_desc_u<TV, ST>(S, y, S.code[t - 2], (t - 2) % 3, (ST)W, (ST)vw, (ST)vwp, (ST)v, dg[t - 2]);
             }
         }
     }
     __syncthreads();
-    const double G1 = bg[0], K1 = bg[1], G2 = bg[2], K2 = bg[3];
-    const double p = (PASS == P_KAT) ? K1 * G2 / (2.0 * PI_D) + K2 * G1 / (2.0 * PI_D) : -(G1 * G2 / (2.0 * PI_D));
-    const TV tp = (TV)p;
+    const ST G1 = (ST)bg[0], K1 = (ST)bg[1], G2 = (ST)bg[2], K2 = (ST)bg[3];
+    const ST twopi = (ST)2 * (ST)PI_D;
+    const TV tp = (TV)((PASS == P_KAT) ? K1 * G2 / twopi + K2 * G1 / twopi : -(G1 * G2 / twopi));
     TV v1s = 0, v1d = 0, v2s = 0, v2d = 0, v3s = 0, v3d = 0, v4s = 0, v4d = 0;
     nc = 0;
     int nc0 = 0;
@@ -506,10 +537,10 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
         TV gdsum = -tp * ((TV)3.0 * v1d * v4s + v1d * v4d + (TV)3.0 * v3s * v2d + v3d * v2d);
         // overlap gather Σ_(j1,j2,m) -p (-2) m (2S) v1[j1] v2[j2]  (flow_su2_t.jl:59-74)
         double *sc = S.scr;
-        sc[0 * 64 + slot] = (double)v1s;
-        sc[1 * 64 + slot] = (double)v1d;
-        sc[2 * 64 + slot] = (double)v2s;
-        sc[3 * 64 + slot] = (double)v2d;
+        sc[0 * SCR_ROW + slot] = (double)v1s;
+        sc[1 * SCR_ROW + slot] = (double)v1d;
+        sc[2 * SCR_ROW + slot] = (double)v2s;
+        sc[3 * SCR_ROW + slot] = (double)v2d;
         __syncthreads();
         const TV cc = -tp * (TV)(-2.0);
         const TV S2 = (TV)y.S2;
@@ -520,8 +551,8 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
             const TV m = (TV)(double)(row >> 16);
             const double *e1 = sc + gbase + j1;
             const double *e2 = sc + gbase + j2;
-            gsum += cc * m * S2 * (TV)e1[0] * (TV)e2[2 * 64];
-            gdsum += cc * m * S2 * (TV)e1[64] * (TV)e2[3 * 64];
+            gsum += cc * m * S2 * (TV)e1[0] * (TV)e2[2 * SCR_ROW];
+            gdsum += cc * m * S2 * (TV)e1[SCR_ROW] * (TV)e2[3 * SCR_ROW];
         }
         Gs = gsum;
         Gd = gdsum;
@@ -589,7 +620,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
     if (!P.dbg_v && e_idx >= P.n_elems) return;
     const int lane = threadIdx.x;
     const int n_ov = P.ov_off[y.ns];
-    const PassLds Lay = pass_lds(y, n_ov, (int)sizeof(DescT<TV>));
+    constexpr bool ST32 = sizeof(typename PassScalar<TV>::T) == 4;
+    const PassLds Lay = pass_lds(y, n_ov, (int)sizeof(DescT<TV>), ST32);
     const int mp = Lay.mp;
     double *s_mesh = reinterpret_cast<double *>(s_dyn);
     double *s_sigA = reinterpret_cast<double *>(s_dyn + Lay.o_sig), *s_sigD = s_sigA + mp;
@@ -601,6 +633,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
     for (int k = lane; k < y.nsig; k += 64) {
         s_sigA[k] = P.A.Sigma[k];
         s_sigD[k] = (PASS == P_KAT) ? P.D.Sigma[k] : 0.0;
+    }
+    if (ST32) {
+        float *mf = reinterpret_cast<float *>(s_dyn + Lay.o_mf);
+        for (int k = lane; k < 10 * mp; k += 64) mf[k] = (float)P.mesh[(k / mp) * MAXN + (k % mp)];
+        for (int k = lane; k < y.nsig; k += 64) {
+            mf[10 * mp + k] = (float)P.A.Sigma[k];
+            mf[11 * mp + k] = (PASS == P_KAT) ? (float)P.D.Sigma[k] : 0.0f;
+        }
     }
     if (lane < y.ns) s_exch[lane] = P.exch[lane];
     for (int k = lane; k < n_ov; k += 64) s_ov[k] = P.ov_rows[k];
@@ -627,6 +667,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
     S.bub = reinterpret_cast<double *>(s_dyn + Lay.o_bub);
     S.ctl = reinterpret_cast<Ctl *>(s_dyn + Lay.o_ctl);
     S.own = s_own;
+    {
+        const float *mf = reinterpret_cast<const float *>(s_dyn + Lay.o_mf);
+        S.mf = MeshV<float>{mf, mf + mp, mf + 2 * mp, mf + 3 * mp, mf + 4 * mp, mf + 5 * mp, mf + 10 * mp,
+                            mf + 11 * mp, mp};
+    }
 
     int ch, kernel, w1, w2, w3;
     decode(P.dbg_v ? P.dbg_elem : P.elems[e_idx], ch, kernel, w1, w2, w3);
@@ -653,7 +698,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
     const int g = lane / GS;
     const bool real_lane = g < G;
     const int i = lane % GS;
-    const int gbase = real_lane ? g * GS : 0;
+    const int gbase = real_lane ? g * (GS + 1) : 0;  // padded row of the group in the exchange scratch
+    const int ri = real_lane ? gbase + i : SCR_ROW - 1;
     const int nsub = 4 * P.eval;
     const int rounds = (nsub + G - 1) / G;
     const double c0 = P.corrs[0 + 2 * (ch + 3 * w1)], c1 = P.corrs[1 + 2 * (ch + 3 * w1)];
@@ -760,7 +806,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
         }
         TV Gs, Gd;
         int nc;
-        node_eval<PASS, TV>(P, S, ch, W, vw, vwp, use ? x : 0.0, i, lane, g, gbase, real_lane, Gs, Gd, nc);
+        node_eval<PASS, TV>(P, S, ch, W, vw, vwp, use ? x : 0.0, i, ri, g, gbase, real_lane, Gs, Gd, nc);
         if (mode == M_PROBE) {
             if (g == 0) {
                 P.dbg_out[(2 * bid + 0) * y.ns + i] = (double)Gs;
@@ -775,14 +821,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
         // contributions dv * Γ of every group, in group (= pooled node) order
         double *sc = S.scr;
         const double v0 = use ? dx * (double)Gs : 0.0, v1 = use ? dx * (double)Gd : 0.0;
-        sc[lane] = v0;
-        sc[64 + lane] = v1;
+        sc[ri] = v0;
+        sc[SCR_ROW + ri] = v1;
         if (real_lane && i == 0) S.own[g] = use ? own : -1;
         __syncthreads();
         if (mode == M_TAIL) {
             for (int e = 0; e < G && tk + e < 2; ++e) {
-                acc0 += sc[e * GS + i];
-                acc1 += sc[64 + e * GS + i];
+                acc0 += sc[e * (GS + 1) + i];
+                acc1 += sc[SCR_ROW + e * (GS + 1) + i];
             }
             __syncthreads();
             tk += G;
@@ -793,7 +839,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
         if (real_lane && !done) {
             for (int e = 0; e < G; ++e) {
                 if (S.own[e] != g) continue;
-                const double u0 = sc[e * GS + i], u1 = sc[64 + e * GS + i];
+                const double u0 = sc[e * (GS + 1) + i], u1 = sc[SCR_ROW + e * (GS + 1) + i];
                 if (init && cnt < 3) {
                     b10 += u0; b11 += u1;
                 } else {
@@ -815,16 +861,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
             r10 += 16.0 / 15.0 * b20;
             r11 += 16.0 / 15.0 * b21;
             __syncthreads();
-            sc[lane] = r10;
-            sc[64 + lane] = r11;
-            sc[128 + lane] = b20;
-            sc[192 + lane] = b21;
+            sc[ri] = r10;
+            sc[SCR_ROW + ri] = r11;
+            sc[2 * SCR_ROW + ri] = b20;
+            sc[3 * SCR_ROW + ri] = b21;
             __syncthreads();
             if (need_check) {
                 double q1 = 0.0, q2 = 0.0, q3 = 0.0;
                 for (int jj = 0; jj < y.ns; ++jj) {
                     const double *e = sc + gbase + jj;
-                    const double x0 = e[0], x1 = e[64], z0 = e[128], z1 = e[192];
+                    const double x0 = e[0], x1 = e[SCR_ROW], z0 = e[2 * SCR_ROW], z1 = e[3 * SCR_ROW];
                     q1 += x0 * x0; q1 += x1 * x1;
                     q2 += z0 * z0; q2 += z1 * z1;
                     const double d0 = x0 - z0, d1 = x1 - z1;
@@ -855,15 +901,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
         }
         if (__ballot(real_lane && !done) != 0) continue;
         // round complete: ordered accumulation of its sub-interval results
-        sc[lane] = res0;
-        sc[64 + lane] = res1;
+        sc[ri] = res0;
+        sc[SCR_ROW + ri] = res1;
         __syncthreads();
         for (int gg = 0; gg < G; ++gg) {
             const int jj = r * G + gg;
             double ta, tb, ts;
             if (jj < nsub && subinterval(jj, P.eval, ref, logpts, ta, tb, ts)) {
-                acc0 += sc[gg * GS + i];
-                acc1 += sc[64 + gg * GS + i];
+                acc0 += sc[gg * (GS + 1) + i];
+                acc1 += sc[SCR_ROW + gg * (GS + 1) + i];
             }
         }
         __syncthreads();
