@@ -6,13 +6,15 @@ cpu_baseline leg of bench.py as the checker.  It is never used by the product pa
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB = HERE / "build" / "liboracle.so"
+# PFFRG_ORACLE_LIB points the checker at another build of this file (the planted-mistake tests)
+LIB = Path(os.environ.get("PFFRG_ORACLE_LIB", HERE / "build" / "liboracle.so"))
 
 P_KAT, P_LEFT, P_CENTRAL, P_BSE, P_REDUCED = range(5)
 

@@ -340,3 +340,50 @@ def test_synthetic_generator_deterministic():
     np.testing.assert_array_equal(a1.gamma, a2.gamma)
     z = A.normals(1, 200000)
     assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+
+
+def _symmetric_action(r, m, bare, seed):
+    """A vertex with the stored ν <-> ν' symmetry the reference's symmetrize!/symmetrize_add_to!
+    assume (action_su2.jl:141-250): q3[i,w,v,v'] = q3[x(i),w,v',v], q2_2[i] = q2_1[x(i)],
+    q1[i] = q1[x(i)] in the s and t channels (x = exchange), the same without x in u."""
+    ns = r.n_sites
+    x = np.asarray(r.exchange)
+    a = A.synthetic_action(ns, m, bare, seed=seed)
+    for comp in range(2):
+        for ch in range(3):
+            ex = x if ch != 2 else np.arange(ns)
+            q1 = a.view(comp, ch, "q1")
+            q1[:] = 0.5 * (q1 + q1[ex])
+            a.view(comp, ch, "q2_2")[:] = a.view(comp, ch, "q2_1")[ex]
+            q3 = a.view(comp, ch, "q3")
+            for v in range(m.num_nu):
+                q3[:, :, v, v] = 0.5 * (q3[:, :, v, v] + q3[ex][:, :, v, v])
+    return a
+
+
+@pytest.mark.parametrize("name,size", [("square", 2), ("pyrochlore", 3)])
+def test_katanin_pass_preserves_vertex_symmetry(name, size):
+    """Invariant of the one-loop flow: for an input vertex with the stored ν <-> ν' symmetry,
+    the Katanin derivative computed on the full (v, v') grid has the same symmetry, up to the
+    quadrature tolerance (the reference computes half the grid and copies the rest).  A sign
+    or argument error in any frequency-transfer formula of flow_su2_{s,t,u}.jl breaks it at the
+    1e-2..1 level; the intact oracle meets it to ~1e-4."""
+    m = M.launch_mesh(1.0, 6, 6, 6, 5)
+    l, r = LT.build(name, size, [[1.0]])
+    bare = LT.bare_couplings(l, r)
+    a = _symmetric_action(r, m, bare, seed=7)
+    orc = O.Oracle(r, m)
+    orc.symmetrize(a)
+    da = A.Action.empty(r.n_sites, m)
+    orc.compute_dsigma(1.0, a, da)
+    orc.compute_corrs(1.0, a, da, True)
+    out = A.Action.empty(r.n_sites, m)
+    orc.run_pass(O.P_KAT, 1.0, a, da, out, True, 10)
+    x = np.asarray(r.exchange)
+    for comp in range(2):
+        for ch in range(3):
+            ex = x if ch != 2 else np.arange(r.n_sites)
+            q3 = out.view(comp, ch, "q3")
+            assert np.abs(q3 - q3[ex].transpose(0, 1, 3, 2)).max() <= 2e-3 * np.abs(q3).max()
+            q21, q22 = out.view(comp, ch, "q2_1"), out.view(comp, ch, "q2_2")
+            assert np.abs(q22 - q21[ex]).max() <= 2e-3 * np.abs(q21).max()
