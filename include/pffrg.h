@@ -140,6 +140,22 @@ int pffrg_set_timing(pffrg_ctx *ctx, int32_t enable);
 /* total milliseconds and launch count of the vertex-pass kernel since the last reset */
 int pffrg_get_timing(pffrg_ctx *ctx, double *total_ms, int64_t *launches);
 
+/* ---- Batched problems.  A batch groups contexts of one device (up to 64: different lattices,
+ * truncations, couplings, meshes and Λ) whose 1-loop right-hand sides run together: the
+ * self-energy and boundary-correction kernels per problem, then ONE vertex-pass launch over the
+ * elements of all problems, then the per-problem symmetrisation.  It replaces a sequence of
+ * compute_dΣ! + compute_dΓ_1l! calls (flow_self.jl:2-20, compute.jl:57-112), one per problem,
+ * for callers that advance many flows at once (coupling sweeps; launch_1l!, launcher_1l.jl:85-109,
+ * run once per problem).  The contexts stay owned by the caller and must outlive the batch. */
+typedef struct pffrg_batch pffrg_batch;
+int pffrg_batch_create(pffrg_ctx *const *ctxs, int32_t n, pffrg_batch **out);
+void pffrg_batch_destroy(pffrg_batch *batch);
+const char *pffrg_batch_last_error(const pffrg_batch *batch);
+/* per problem k (caller's order): dΣ and dΓ_1l of slot a[k] into slot da[k] at Lambda[k] with
+ * eval[k]; all contexts must use the same precision mode */
+int pffrg_batch_rhs_1l(pffrg_batch *batch, const double *Lambda, const int32_t *a, const int32_t *da,
+                       const int32_t *eval, const double *sigma_tol, const double *gamma_tol);
+
 #ifdef __cplusplus
 }
 #endif

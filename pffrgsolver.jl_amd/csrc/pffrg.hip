@@ -11,6 +11,7 @@
 // mixed mode), so every trilinear corner is one coalesced n_sites-wide segment.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -332,6 +333,19 @@ struct pffrg_ctx {
     int logpts_eval = 0;
 };
 
+// Several contexts on one device whose 1-loop right-hand sides run as one vertex-pass launch.
+struct pffrg_batch {
+    std::vector<pffrg_ctx *> ctx;  // in launch order: larger problems first
+    std::vector<int> order;        // order[k] = caller's index of ctx[k]
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    uint32_t *d_codes = nullptr;   // half-pass elements of all problems, problem index in bits 26-31
+    int n_codes = 0;
+    void *d_tab = nullptr;         // PassArgs of every problem
+    size_t tab_bytes = 0;
+};
+
 namespace {
 
 int fail(pffrg_ctx *c, int code, const std::string &msg) {
@@ -417,10 +431,11 @@ int ensure_gather(pffrg_ctx *c, int s, bool f32) {
 
 void invalidate(pffrg_ctx *c, int s) { c->slots[s].gat64_valid = c->slots[s].gat32_valid = false; }
 
+// Kernel arguments of one pass of one context (also rebuilds the log-domain breakpoints when
+// Λ, eval or the meshes changed).
 template <int PASS, typename TV>
-int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const double *tol, int mode,
-                  const double *dbg_v = nullptr, double *dbg_out = nullptr, int dbg_n = 0, uint32_t dbg_elem = 0) {
-    PassArgs<TV> P;
+int pass_args(pffrg_ctx *c, double L, int a, int d, int out, int eval, const double *tol, int mode, PassArgs<TV> &P,
+              const double *dbg_v = nullptr, double *dbg_out = nullptr, uint32_t dbg_elem = 0) {
     P.dbg_v = dbg_v;
     P.dbg_out = dbg_out;
     P.dbg_elem = dbg_elem;
@@ -483,7 +498,22 @@ int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const
     P.atol = tol[0];
     P.rtol = tol[1];
     P.node_count = c->d_count;
-    const size_t lds = (size_t)pass_lds(c->sys, c->n_ov, (int)sizeof(DescT<TV>), std::is_same<typename PassScalar<TV>::T, float>::value).total;
+    return 0;
+}
+
+template <typename TV>
+size_t pass_lds_bytes(const pffrg_ctx *c) {
+    return (size_t)pass_lds(c->sys, c->n_ov, (int)sizeof(DescT<TV>),
+                            std::is_same<typename PassScalar<TV>::T, float>::value).total;
+}
+
+template <int PASS, typename TV>
+int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const double *tol, int mode,
+                  const double *dbg_v = nullptr, double *dbg_out = nullptr, int dbg_n = 0, uint32_t dbg_elem = 0) {
+    PassArgs<TV> P;
+    int rc;
+    if ((rc = pass_args<PASS, TV>(c, L, a, d, out, eval, tol, mode, P, dbg_v, dbg_out, dbg_elem))) return rc;
+    const size_t lds = pass_lds_bytes<TV>(c);
     const int grid = dbg_v ? dbg_n : P.n_elems;
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, (launch_vertex_pass<PASS, TV>(P, grid, lds, c->stream)));
@@ -1116,6 +1146,136 @@ int pffrg_get_timing(pffrg_ctx *c, double *total_ms, int64_t *launches) {
     *total_ms = c->time_ms;
     *launches = c->time_launches;
     return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------- batched problems
+namespace {
+int bfail(pffrg_batch *b, int code, const std::string &msg) {
+    if (b) b->err = msg;
+    return code;
+}
+
+template <typename TV>
+int batch_rhs_1l_t(pffrg_batch *b, const double *Lam, const int32_t *a, const int32_t *da, const int32_t *eval,
+                   const double *stol, const double *gtol) {
+    const int n = (int)b->ctx.size();
+    const bool f32 = std::is_same<TV, float>::value;
+    std::vector<PassArgs<TV>> tab(n);
+    size_t lds = 0;
+    for (int k = 0; k < n; ++k) {
+        pffrg_ctx *c = b->ctx[k];
+        const int q = b->order[k];
+        int rc;
+        if ((rc = launch_sigma<SK_DSIGMA>(c, Lam[q], a[q], a[q], c->slots[da[q]].Sigma, nullptr, nullptr, stol)))
+            return bfail(b, rc, c->err);
+        if ((rc = launch_corrs(c, Lam[q], a[q], da[q], true, gtol))) return bfail(b, rc, c->err);
+        if ((rc = ensure_gather(c, a[q], f32))) return bfail(b, rc, c->err);
+        if ((rc = pass_args<P_KAT, TV>(c, Lam[q], a[q], da[q], da[q], eval[q], gtol, 0, tab[k])))
+            return bfail(b, rc, c->err);
+        lds = std::max(lds, pass_lds_bytes<TV>(c));
+    }
+    for (pffrg_ctx *c : b->ctx)
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "stream synchronize failed");
+    const size_t bytes = n * sizeof(PassArgs<TV>);
+    if (bytes > b->tab_bytes) {
+        if (b->d_tab) (void)hipFree(b->d_tab);
+        b->d_tab = nullptr;
+        if (hipMalloc(&b->d_tab, bytes) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "hipMalloc failed");
+        b->tab_bytes = bytes;
+    }
+    if (hipMemcpy(b->d_tab, tab.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+        return bfail(b, PFFRG_ERR_HIP, "argument table upload failed");
+    hipError_t e = launch_vertex_pass_batch<P_KAT, TV>(reinterpret_cast<const PassArgs<TV> *>(b->d_tab), b->d_codes,
+                                                     b->n_codes, lds, b->stream);
+    if (e != hipSuccess) return bfail(b, PFFRG_ERR_HIP, std::string("batched vertex pass: ") + hipGetErrorString(e));
+    if ((e = hipStreamSynchronize(b->stream)) != hipSuccess)
+        return bfail(b, PFFRG_ERR_HIP, std::string("batched vertex pass: ") + hipGetErrorString(e));
+    for (int k = 0; k < n; ++k) {
+        pffrg_ctx *c = b->ctx[k];
+        const int q = b->order[k];
+        c->stat_elems += c->n_half;
+        c->stat_launches += 1;
+        invalidate(c, da[q]);
+        int rc;
+        if ((rc = do_symmetrize(c, da[q]))) return bfail(b, rc, c->err);
+    }
+    for (pffrg_ctx *c : b->ctx)
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "stream synchronize failed");
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int pffrg_batch_create(pffrg_ctx *const *ctxs, int32_t n, pffrg_batch **out) {
+    if (!ctxs || !out || n < 1 || n > 64) return PFFRG_ERR_ARG;
+    *out = nullptr;
+    pffrg_batch *b = new pffrg_batch();
+    b->order.resize(n);
+    for (int k = 0; k < n; ++k) {
+        if (!ctxs[k] || ready(ctxs[k])) {
+            delete b;
+            return PFFRG_ERR_STATE;
+        }
+        b->order[k] = k;
+    }
+    b->device = ctxs[0]->device;
+    for (int k = 1; k < n; ++k)
+        if (ctxs[k]->device != b->device) {
+            delete b;
+            return PFFRG_ERR_ARG;
+        }
+    // larger problems first, so the most expensive elements start early
+    std::stable_sort(b->order.begin(), b->order.end(),
+                     [&](int x, int y) { return ctxs[x]->sys.ns * ctxs[x]->n_half > ctxs[y]->sys.ns * ctxs[y]->n_half; });
+    std::vector<uint32_t> codes;
+    for (int k = 0; k < n; ++k) {
+        pffrg_ctx *c = ctxs[b->order[k]];
+        b->ctx.push_back(c);
+        for (uint32_t e : element_list(c->sys.nO, c->sys.nn, 0)) codes.push_back(e | ((uint32_t)k << 26));
+    }
+    b->n_codes = (int)codes.size();
+    bool ok = hipSetDevice(b->device) == hipSuccess;
+    ok = ok && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipMalloc(&b->d_codes, codes.size() * sizeof(uint32_t)) == hipSuccess;
+    ok = ok && hipMemcpy(b->d_codes, codes.data(), codes.size() * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        pffrg_batch_destroy(b);
+        return PFFRG_ERR_HIP;
+    }
+    *out = b;
+    return 0;
+}
+
+void pffrg_batch_destroy(pffrg_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->d_codes) (void)hipFree(b->d_codes);
+    if (b->d_tab) (void)hipFree(b->d_tab);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char *pffrg_batch_last_error(const pffrg_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int pffrg_batch_rhs_1l(pffrg_batch *b, const double *Lambda, const int32_t *a, const int32_t *da, const int32_t *eval,
+                       const double *sigma_tol, const double *gamma_tol) {
+    if (!b || !Lambda || !a || !da || !eval || !sigma_tol || !gamma_tol) return PFFRG_ERR_ARG;
+    const int n = (int)b->ctx.size();
+    const int prec = b->ctx[0]->precision;
+    for (int k = 0; k < n; ++k) {
+        pffrg_ctx *c = b->ctx[k];
+        const int q = b->order[k];
+        if (c->precision != prec) return bfail(b, PFFRG_ERR_STATE, "all problems of a batch need the same precision");
+        if (check_slot(c, a[q]) || check_slot(c, da[q]) || a[q] == da[q])
+            return bfail(b, PFFRG_ERR_ARG, "invalid slot of problem " + std::to_string(q));
+        if (!(Lambda[q] > 0.0) || eval[q] < 1) return bfail(b, PFFRG_ERR_ARG, "invalid Lambda or eval of problem " + std::to_string(q));
+    }
+    if (hipSetDevice(b->device) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "hipSetDevice failed");
+    return prec == PFFRG_FP32_MIXED ? batch_rhs_1l_t<float>(b, Lambda, a, da, eval, sigma_tol, gamma_tol)
+                                    : batch_rhs_1l_t<double>(b, Lambda, a, da, eval, sigma_tol, gamma_tol);
 }
 
 }  // extern "C"

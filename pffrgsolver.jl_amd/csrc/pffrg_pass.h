@@ -604,8 +604,10 @@ __device__ __forceinline__ bool subinterval(int j, int eval, double ref, const d
     return true;
 }
 
+// One output element of one pass (block = one wavefront).  e_idx is the element's position in
+// its pass (and the probe index in probe mode), code its encoded (ch, kernel, w1, w2, w3).
 template <int PASS, typename TV>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_WAVES_PER_EU))) void k_vertex_pass(PassArgs<TV> P) {
+__device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_idx, uint32_t code) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     __shared__ uint32_t s_code[MAX_DESC];
     __shared__ int s_exch[64];
@@ -613,11 +615,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
     __shared__ unsigned long long s_cnt, s_bytes;
 
     const Sys &y = P.y;
-    const int bid = blockIdx.x;
-    // consecutive blocks go to different XCDs, so each XCD receives an even sample of the
-    // element list (channels and Ω differ widely in cost)
-    const int e_idx = bid;
-    if (!P.dbg_v && e_idx >= P.n_elems) return;
+    const int bid = e_idx;
     const int lane = threadIdx.x;
     const int n_ov = P.ov_off[y.ns];
     constexpr bool ST32 = sizeof(typename PassScalar<TV>::T) == 4;
@@ -674,7 +672,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
     }
 
     int ch, kernel, w1, w2, w3;
-    decode(P.dbg_v ? P.dbg_elem : P.elems[e_idx], ch, kernel, w1, w2, w3);
+    decode(code, ch, kernel, w1, w2, w3);
     if (lane < MAX_DESC) s_code[lane] = slot_code<PASS>(ch, lane / 3, lane % 3);
     __syncthreads();
     const double INF = __builtin_huge_val();
@@ -940,16 +938,48 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
 }
 
 
+// One pass of one problem.  Consecutive blocks go to different XCDs, so each XCD receives an
+// even sample of the element list (channels and Ω differ widely in cost).
+template <int PASS, typename TV>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_WAVES_PER_EU))) void k_vertex_pass(
+    PassArgs<TV> P) {
+    const int e_idx = blockIdx.x;
+    if (!P.dbg_v && e_idx >= P.n_elems) return;
+    vertex_pass_body<PASS, TV>(P, e_idx, P.dbg_v ? P.dbg_elem : P.elems[e_idx]);
+}
+
+// The same pass of several independent problems in one launch: element code bits 26-31 select
+// the problem's arguments in tab (batched coupling sweeps and truncations).
+template <int PASS, typename TV>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_WAVES_PER_EU))) void k_vertex_pass_batch(
+    const PassArgs<TV> *__restrict__ tab, const uint32_t *__restrict__ codes, int n) {
+    const int bid = blockIdx.x;
+    if (bid >= n) return;
+    const uint32_t c = codes[bid];
+    vertex_pass_body<PASS, TV>(tab[c >> 26], bid, c & 0x03ffffffu);
+}
+
 template <int PASS, typename TV>
 hipError_t launch_vertex_pass(const PassArgs<TV> &P, int grid, size_t lds, hipStream_t stream) {
     hipLaunchKernelGGL((k_vertex_pass<PASS, TV>), dim3(grid), dim3(64), lds, stream, P);
     return hipGetLastError();
 }
 
+template <int PASS, typename TV>
+hipError_t launch_vertex_pass_batch(const PassArgs<TV> *tab, const uint32_t *codes, int n, size_t lds,
+                                    hipStream_t stream) {
+    hipLaunchKernelGGL((k_vertex_pass_batch<PASS, TV>), dim3(n), dim3(64), lds, stream, tab, codes, n);
+    return hipGetLastError();
+}
+
 // explicit instantiations live in pass_*.hip; other translation units only link to them
 #define PFFRG_PASS_INSTANCES(KW, PASS)                                                                      \
     KW template hipError_t launch_vertex_pass<PASS, double>(const PassArgs<double> &, int, size_t, hipStream_t); \
-    KW template hipError_t launch_vertex_pass<PASS, float>(const PassArgs<float> &, int, size_t, hipStream_t);
+    KW template hipError_t launch_vertex_pass<PASS, float>(const PassArgs<float> &, int, size_t, hipStream_t);   \
+    KW template hipError_t launch_vertex_pass_batch<PASS, double>(const PassArgs<double> *, const uint32_t *, int, \
+                                                                  size_t, hipStream_t);                        \
+    KW template hipError_t launch_vertex_pass_batch<PASS, float>(const PassArgs<float> *, const uint32_t *, int,   \
+                                                                 size_t, hipStream_t);
 
 PFFRG_PASS_INSTANCES(extern, P_KAT)
 PFFRG_PASS_INSTANCES(extern, P_LEFT)

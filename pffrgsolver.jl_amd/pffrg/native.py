@@ -69,6 +69,11 @@ _SIGS = {
     "pffrg_debug_record": ([_P, C.POINTER(C.c_int32), _D, C.c_int64], C.c_int),
     "pffrg_set_timing": ([_P, _i], C.c_int),
     "pffrg_get_timing": ([_P, _D, _I64], C.c_int),
+    "pffrg_batch_create": ([C.POINTER(_P), _i, C.POINTER(_P)], C.c_int),
+    "pffrg_batch_destroy": ([_P], None),
+    "pffrg_batch_last_error": ([_P], C.c_char_p),
+    "pffrg_batch_rhs_1l": ([_P, _D, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32), _D, _D],
+                           C.c_int),
 }
 
 
@@ -263,3 +268,49 @@ class Context:
         n = C.c_int64()
         self._ck(self.lib.pffrg_get_timing(self.h, C.byref(ms), C.byref(n)), "get_timing")
         return ms.value, n.value
+
+
+class Batch:
+    """Several contexts of one device whose 1-loop right-hand sides run as one vertex-pass
+    launch (pffrg_batch_*).  The contexts must stay open while the batch is used."""
+
+    def __init__(self, ctxs):
+        self.lib = load()
+        self.ctxs = list(ctxs)
+        arr = (_P * len(self.ctxs))(*[c.h for c in self.ctxs])
+        h = _P()
+        rc = self.lib.pffrg_batch_create(arr, len(self.ctxs), C.byref(h))
+        if rc != 0 or not h:
+            raise PffrgError(f"pffrg_batch_create failed with code {rc}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pffrg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def rhs_1l(self, Lambdas, a, da, evals, sigma_tol=(1e-8, 1e-4), gamma_tol=(1e-8, 1e-4)):
+        """compute_dΣ! + compute_dΓ_1l! of every problem: slot a[k] -> slot da[k] at Lambdas[k]."""
+        n = len(self.ctxs)
+        L = np.ascontiguousarray(np.broadcast_to(np.asarray(Lambdas, dtype=np.float64), (n,)))
+        ia = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.int32), (n,)))
+        ida = np.ascontiguousarray(np.broadcast_to(np.asarray(da, dtype=np.int32), (n,)))
+        iev = np.ascontiguousarray(np.broadcast_to(np.asarray(evals, dtype=np.int32), (n,)))
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int32))
+        rc = self.lib.pffrg_batch_rhs_1l(self.h, _dp(L), ip(ia), ip(ida), ip(iev), _dp(_tol(sigma_tol)),
+                                         _dp(_tol(gamma_tol)))
+        if rc != 0:
+            msg = self.lib.pffrg_batch_last_error(self.h)
+            raise PffrgError(f"batch_rhs_1l failed ({rc}): {msg.decode() if msg else ''}")
