@@ -17,7 +17,7 @@ import numpy as np
 from . import lattice as LT
 from . import mesh as M
 from .action import Action
-from .native import FP64, FP32_MIXED, Context
+from .native import FP64, FP32_MIXED, Batch, Context
 
 # slot roles
 A, DA, A_STAGE, A_INTER, A_ERR, DA_L, DA_C, DA_TEMP, DA_SIGMA = range(9)
@@ -84,91 +84,177 @@ class FlowResult:
     status: str = ""
 
 
-def launch_flow(ctx: Context, prob: Problem, a0: Action, Li: float, Lf: float, loops: int = 1, sigma_corr=True,
-                bmin=1e-4, bmax=0.1, min_eval=10, max_eval=100, Stol=(1e-8, 1e-4), Gtol=(1e-8, 1e-4),
-                ODE_tol=(1e-8, 1e-2), max_steps: int | None = None, verbose=False) -> FlowResult:
-    """BS3 flow from Li to Lf (launcher_1l.jl:72-198 and its 2l/ml twins).  Slot A holds
-    the flowing action on return."""
-    t0 = time.time()
-    res = FlowResult()
-    ctx.upload(A, a0)
-    bare = prob.bare_action()
-    # da, a_stage, a_err start empty; a_inter carries the bare couplings (launcher_1l.jl:39-44)
-    zero = Action.empty(prob.r.n_sites, prob.mesh, prob.S)
-    for s in (DA, A_STAGE, A_ERR, DA_L, DA_C, DA_TEMP, DA_SIGMA):
-        ctx.upload(s, zero)
-    ctx.upload(A_INTER, bare)
-    L, dL = Li, bmax * Li
-    cps = sorted({Li, Lf}, reverse=True)
-    for c in cps:
-        if c < L:
-            dL = min(dL, 0.85 * (L - c))
-            break
-    steps = 0
-    while L > Lf:
-        if max_steps is not None and steps >= max_steps:
-            res.status = "max_steps"
-            break
-        steps += 1
-        ev = M.get_eval(L, min_eval, max_eval)
-        ctx.replace_with(DA, A)
-        ctx.replace_with(A_ERR, A)
-        # k1
-        rhs(ctx, L, loops, A, A_STAGE, ev, Gtol, Stol, sigma_corr=sigma_corr)
-        ctx.mult_with_add_to(A_STAGE, -2.0 * dL / 9.0, DA)
-        ctx.mult_with_add_to(A_STAGE, -7.0 * dL / 24.0, A_ERR)
-        # k2
-        ctx.replace_with(A_INTER, A)
-        ctx.mult_with_add_to(A_STAGE, -0.5 * dL, A_INTER)
-        rhs(ctx, L - 0.5 * dL, loops, A_INTER, A_STAGE, ev, Gtol, Stol, sigma_corr=sigma_corr)
-        ctx.mult_with_add_to(A_STAGE, -1.0 * dL / 3.0, DA)
-        ctx.mult_with_add_to(A_STAGE, -1.0 * dL / 4.0, A_ERR)
-        # k3
-        ctx.replace_with(A_INTER, A)
-        ctx.mult_with_add_to(A_STAGE, -0.75 * dL, A_INTER)
-        rhs(ctx, L - 0.75 * dL, loops, A_INTER, A_STAGE, ev, Gtol, Stol, sigma_corr=sigma_corr)
-        ctx.mult_with_add_to(A_STAGE, -4.0 * dL / 9.0, DA)
-        ctx.mult_with_add_to(A_STAGE, -1.0 * dL / 3.0, A_ERR)
-        # k4
-        ctx.replace_with(A_INTER, DA)
-        rhs(ctx, L - dL, loops, A_INTER, A_STAGE, ev, Gtol, Stol, sigma_corr=sigma_corr)
-        ctx.mult_with_add_to(A_STAGE, -1.0 * dL / 8.0, A_ERR)
+class BS3Flow:
+    """One Bogacki-Shampine flow from Li to Lf (launcher_1l.jl:72-198 and its 2l/ml twins) kept
+    as a state machine, so that a caller can evaluate the right-hand sides of many flows
+    together.  A step has four stages; before(k) prepares the input of stage k, the caller
+    evaluates that right-hand side (stage_input(k) -> slot A_STAGE), after(k) folds it into the
+    update and the error estimate, and finish_step() accepts or rejects the step."""
+
+    def __init__(self, ctx: Context, prob: Problem, a0: Action, Li: float, Lf: float, loops: int = 1, sigma_corr=True,
+                 bmin=1e-4, bmax=0.1, min_eval=10, max_eval=100, Stol=(1e-8, 1e-4), Gtol=(1e-8, 1e-4),
+                 ODE_tol=(1e-8, 1e-2), max_steps: int | None = None, verbose=False):
+        self.ctx, self.prob, self.Lf, self.loops, self.sigma_corr = ctx, prob, Lf, loops, sigma_corr
+        self.bmin, self.bmax, self.min_eval, self.max_eval = bmin, bmax, min_eval, max_eval
+        self.Stol, self.Gtol, self.ODE_tol, self.max_steps, self.verbose = Stol, Gtol, ODE_tol, max_steps, verbose
+        self.res = FlowResult()
+        self.t0 = time.time()
+        ctx.upload(A, a0)
+        # da, a_stage, a_err start empty; a_inter carries the bare couplings (launcher_1l.jl:39-44)
+        zero = Action.empty(prob.r.n_sites, prob.mesh, prob.S)
+        for s in (DA, A_STAGE, A_ERR, DA_L, DA_C, DA_TEMP, DA_SIGMA):
+            ctx.upload(s, zero)
+        ctx.upload(A_INTER, prob.bare_action())
+        self.L, self.dL = Li, bmax * Li
+        self.cps = sorted({Li, Lf}, reverse=True)
+        for c in self.cps:
+            if c < self.L:
+                self.dL = min(self.dL, 0.85 * (self.L - c))
+                break
+        self.steps = 0
+        self.done = not (self.L > Lf)
+        if self.done:
+            self.res.status = "finished"
+        self.ev = 0
+
+    def begin_step(self) -> bool:
+        """False when the flow has ended (Λ_f reached, step limit, divergence)."""
+        if self.done:
+            return False
+        if self.max_steps is not None and self.steps >= self.max_steps:
+            self.res.status = "max_steps"
+            self.done = True
+            return False
+        self.steps += 1
+        self.ev = M.get_eval(self.L, self.min_eval, self.max_eval)
+        return True
+
+    def before(self, k: int):
+        ctx, L, dL = self.ctx, self.L, self.dL
+        if k == 0:
+            ctx.replace_with(DA, A)
+            ctx.replace_with(A_ERR, A)
+        elif k in (1, 2):
+            ctx.replace_with(A_INTER, A)
+            ctx.mult_with_add_to(A_STAGE, (-0.5 if k == 1 else -0.75) * dL, A_INTER)
+        else:
+            ctx.replace_with(A_INTER, DA)
+
+    def stage_input(self, k: int):
+        """(Λ, input slot, eval) of stage k; its right-hand side goes to slot A_STAGE."""
+        L, dL = self.L, self.dL
+        return ((L, A), (L - 0.5 * dL, A_INTER), (L - 0.75 * dL, A_INTER), (L - dL, A_INTER))[k] + (self.ev,)
+
+    def after(self, k: int):
+        ctx, dL = self.ctx, self.dL
+        if k == 0:
+            ctx.mult_with_add_to(A_STAGE, -2.0 * dL / 9.0, DA)
+            ctx.mult_with_add_to(A_STAGE, -7.0 * dL / 24.0, A_ERR)
+        elif k == 1:
+            ctx.mult_with_add_to(A_STAGE, -1.0 * dL / 3.0, DA)
+            ctx.mult_with_add_to(A_STAGE, -1.0 * dL / 4.0, A_ERR)
+        elif k == 2:
+            ctx.mult_with_add_to(A_STAGE, -4.0 * dL / 9.0, DA)
+            ctx.mult_with_add_to(A_STAGE, -1.0 * dL / 3.0, A_ERR)
+        else:
+            ctx.mult_with_add_to(A_STAGE, -1.0 * dL / 8.0, A_ERR)
+
+    def rhs(self, k: int):
+        L, a, ev = self.stage_input(k)
+        rhs(self.ctx, L, self.loops, a, A_STAGE, ev, self.Gtol, self.Stol, sigma_corr=self.sigma_corr)
+
+    def finish_step(self):
+        ctx, res = self.ctx, self.res
         res.rhs_evals += 4
         # embedded error (launcher_1l.jl:113-116)
         ctx.mult_with_add_to(A_INTER, -1.0, A_ERR)
         delta = ctx.abs_max(A_ERR)
         gmax = ctx.abs_max(A_INTER)
-        scale = ODE_tol[0] + max(gmax, ctx.abs_max(A)) * ODE_tol[1]
+        scale = self.ODE_tol[0] + max(gmax, ctx.abs_max(A)) * self.ODE_tol[1]
         err = delta / scale
-        if verbose:
-            print(f"L={L:.6g} dL={dL:.3g} eval={ev} err={err:.3g} Gmax={gmax:.4g}", flush=True)
+        if self.verbose:
+            print(f"L={self.L:.6g} dL={self.dL:.3g} eval={self.ev} err={err:.3g} Gmax={gmax:.4g}", flush=True)
         if err >= 10.0:
             res.status = "unstable"
-            break
-        if err <= 1.0 or dL <= bmin:
-            L -= dL
-            dL = max(bmin, min(bmax * L, 0.85 * (1.0 / err) ** (1.0 / 3.0) * dL))
-            lower = [c - L for c in cps if c - L < 0.0]
-            L_cp = L + lower[0] if lower else 0.0
-            dL = min(dL, L - L_cp)
+            self.done = True
+            return
+        accept = err <= 1.0 or self.dL <= self.bmin
+        if accept:
+            self.L -= self.dL
+        self.dL = max(self.bmin, min(self.bmax * self.L, 0.85 * (1.0 / err) ** (1.0 / 3.0) * self.dL))
+        lower = [c - self.L for c in self.cps if c - self.L < 0.0]
+        L_cp = self.L + lower[0] if lower else 0.0
+        self.dL = min(self.dL, self.L - L_cp)
+        if accept:
             res.accepted += 1
-            res.Lambdas.append(L)
+            res.Lambdas.append(self.L)
             res.gamma_max.append(gmax)
             ctx.replace_with(A, A_INTER)
             if gmax >= 1000.0:
                 res.status = "diverged"
-                break
+                self.done = True
+                return
         else:
-            dL = max(bmin, min(bmax * L, 0.85 * (1.0 / err) ** (1.0 / 3.0) * dL))
-            lower = [c - L for c in cps if c - L < 0.0]
-            L_cp = L + lower[0] if lower else 0.0
-            dL = min(dL, L - L_cp)
             res.rejected += 1
-    else:
-        res.status = "finished"
-    ctx.synchronize()
-    res.wall_s = time.time() - t0
-    return res
+        if not (self.L > self.Lf):
+            res.status = "finished"
+            self.done = True
+
+    def result(self) -> FlowResult:
+        self.ctx.synchronize()
+        self.res.wall_s = time.time() - self.t0
+        return self.res
+
+
+def launch_flow(ctx: Context, prob: Problem, a0: Action, Li: float, Lf: float, loops: int = 1, sigma_corr=True,
+                bmin=1e-4, bmax=0.1, min_eval=10, max_eval=100, Stol=(1e-8, 1e-4), Gtol=(1e-8, 1e-4),
+                ODE_tol=(1e-8, 1e-2), max_steps: int | None = None, verbose=False) -> FlowResult:
+    """BS3 flow from Li to Lf (launcher_1l.jl:72-198 and its 2l/ml twins).  Slot A holds
+    the flowing action on return."""
+    f = BS3Flow(ctx, prob, a0, Li, Lf, loops, sigma_corr, bmin, bmax, min_eval, max_eval, Stol, Gtol, ODE_tol,
+                max_steps, verbose)
+    while f.begin_step():
+        for k in range(4):
+            f.before(k)
+            f.rhs(k)
+            f.after(k)
+        f.finish_step()
+    return f.result()
+
+
+def launch_flow_batch(ctxs, probs, a0s, Li, Lf, max_steps: int | None = None, **kw) -> list[FlowResult]:
+    """1-loop BS3 flows of several problems advanced together: every stage evaluates the
+    right-hand sides of all flows still running in one batched launch (pffrg_batch_rhs_1l).
+    Each flow keeps its own Λ, step size and acceptance, exactly as launch_flow would."""
+    n = len(ctxs)
+    Lis = Li if np.ndim(Li) else [Li] * n
+    Lfs = Lf if np.ndim(Lf) else [Lf] * n
+    flows = [BS3Flow(ctxs[k], probs[k], a0s[k], Lis[k], Lfs[k], 1, max_steps=max_steps, **kw) for k in range(n)]
+    batch, members = None, None
+    try:
+        while True:
+            live = [k for k in range(n) if flows[k].begin_step()]
+            if not live:
+                break
+            if members != live:
+                if batch is not None:
+                    batch.close()
+                batch, members = Batch([ctxs[k] for k in live]), live
+            for k in range(4):
+                ins = []
+                for q in live:
+                    flows[q].before(k)
+                    ins.append(flows[q].stage_input(k))
+                batch.rhs_1l([x[0] for x in ins], [x[1] for x in ins], A_STAGE, [x[2] for x in ins],
+                             flows[live[0]].Stol, flows[live[0]].Gtol)
+                for q in live:
+                    flows[q].after(k)
+            for q in live:
+                flows[q].finish_step()
+    finally:
+        if batch is not None:
+            batch.close()
+    return [f.result() for f in flows]
 
 
 @dataclass

@@ -1,0 +1,47 @@
+"""An oracle-backed stand-in for pffrg.native.Context: the same slot API (upload, algebra,
+compute_*) on host actions, evaluated by the FP64 CPU oracle.  The host flow drivers of
+pffrg.flow run unchanged on it, which makes the GPU flow testable against the oracle's."""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle as O
+from pffrg.action import Action
+
+
+class OracleContext:
+    def __init__(self, prob, n_slots: int = 9):
+        self.orc = O.Oracle(prob.r, prob.mesh)
+        self.m = prob.mesh
+        self.ns = prob.r.n_sites
+        self.S = prob.S
+        self.slots = [Action.empty(self.ns, self.m, self.S) for _ in range(n_slots)]
+
+    def upload(self, s, a):
+        self.slots[s] = a.copy()
+
+    def download(self, s):
+        return self.slots[s].copy()
+
+    def replace_with(self, dst, src, vertex_only=False):
+        self.slots[dst].gamma[:] = self.slots[src].gamma
+        if not vertex_only:
+            self.slots[dst].Sigma[:] = self.slots[src].Sigma
+
+    def mult_with_add_to(self, src, fac, dst, vertex_only=False):
+        self.slots[dst].gamma[:] += fac * self.slots[src].gamma
+        if not vertex_only:
+            self.slots[dst].Sigma[:] += fac * self.slots[src].Sigma
+
+    def abs_max(self, s) -> float:
+        a = self.slots[s]
+        return max(float(np.max(np.abs(a.view(c, ch, "q3")))) for c in range(2) for ch in range(3))
+
+    def synchronize(self):
+        pass
+
+    def compute_dsigma(self, L, a, da, tol=(1e-8, 1e-4)):
+        self.orc.compute_dsigma(L, self.slots[a], self.slots[da], tol)
+
+    def compute_dgamma_1l(self, L, a, da, eval_, tol=(1e-8, 1e-4)):
+        self.orc.compute_dgamma_1l(L, self.slots[a], self.slots[da], eval_, tol)

@@ -1,0 +1,58 @@
+"""Host flow drivers on the GPU: a BS3 flow (launch_flow, launcher_1l.jl:72-198) against the
+same driver run on the FP64 oracle, and batched flows (launch_flow_batch) against single ones.
+
+Tolerance: the GPU right-hand sides agree with the oracle's to ~1e-15 and the stage algebra
+differs only by FMA rounding, so the accepted Λ sequence is identical and the final actions
+agree to <= 1e-10 relative (max-norm)."""
+import numpy as np
+import pytest
+
+from pffrg import action as A
+from pffrg import flow as F
+
+from helpers import rel_err, small_problem
+from oracle_ctx import OracleContext
+
+pytestmark = pytest.mark.gpu
+
+
+def _flow_setup(name, size, N, J=((1.0,),)):
+    prob = small_problem(name, size, N, 5.0, J)
+    return prob, prob.bare_action()
+
+
+def test_flow_matches_oracle_flow():
+    prob, a0 = _flow_setup("square", 2, 6)
+    with prob.context(n_slots=9) as ctx:
+        res = F.launch_flow(ctx, prob, a0, 5.0, 1.0, max_steps=6)
+        got = ctx.download(F.A)
+    octx = OracleContext(prob)
+    ref_res = F.launch_flow(octx, prob, a0, 5.0, 1.0, max_steps=6)
+    ref = octx.download(F.A)
+    print(f"GPU Λ {res.Lambdas} ({res.accepted}/{res.rejected}); oracle Λ {ref_res.Lambdas}", flush=True)
+    assert res.accepted == ref_res.accepted and res.rejected == ref_res.rejected
+    np.testing.assert_allclose(res.Lambdas, ref_res.Lambdas, rtol=1e-12)
+    e = max(rel_err(got.gamma, ref.gamma), rel_err(got.Sigma, ref.Sigma))
+    print(f"final action rel err {e:.3e}", flush=True)
+    assert e < 1e-10
+
+
+def test_batched_flows_equal_single_flows():
+    setups = [_flow_setup("square", 2, 6), _flow_setup("kagome", 3, 6), _flow_setup("square", 3, 6, ((1.0,), (0.4,)))]
+    single = []
+    for prob, a0 in setups:
+        with prob.context(n_slots=9) as ctx:
+            res = F.launch_flow(ctx, prob, a0, 5.0, 2.0, max_steps=4)
+            single.append((res, ctx.download(F.A)))
+    ctxs = [prob.context(n_slots=9) for prob, _ in setups]
+    try:
+        results = F.launch_flow_batch(ctxs, [p for p, _ in setups], [a for _, a in setups], 5.0, 2.0, max_steps=4)
+        for k, ctx in enumerate(ctxs):
+            got = ctx.download(F.A)
+            print(f"problem {k}: Λ {results[k].Lambdas} status {results[k].status}", flush=True)
+            assert results[k].Lambdas == single[k][0].Lambdas
+            assert results[k].accepted == single[k][0].accepted and results[k].rejected == single[k][0].rejected
+            np.testing.assert_array_equal(got.gamma, single[k][1].gamma)
+    finally:
+        for ctx in ctxs:
+            ctx.close()
