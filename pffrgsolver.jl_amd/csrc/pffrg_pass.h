@@ -50,7 +50,8 @@ struct DescT {
     int meta;            // kernel (bits 0-2) | exchange << 3 | map << 4 | negative density sign << 5
     int off;             // offset of the first corner inside the channel block (site 0)
     int d1, d2, d3;      // offset steps to the upper corner along Ω, ν, ν'
-    int pad[sizeof(TV) == 4 ? 7 : 3];  // 80- or 96-byte stride: the groups' descriptors sit in distinct LDS banks
+    int pad[sizeof(TV) == 4 ? 1 : 5];  // 56- or 104-byte stride (14 or 26 dwords): the descriptors of up to
+                                       // 16 groups start in distinct LDS banks
     TV w[8];             // corner weights in the reference's corner order
 };
 
