@@ -140,6 +140,15 @@ int pffrg_set_timing(pffrg_ctx *ctx, int32_t enable);
 /* total milliseconds and launch count of the vertex-pass kernel since the last reset */
 int pffrg_get_timing(pffrg_ctx *ctx, double *total_ms, int64_t *launches);
 
+/* compute_χ!  (Observable/correlation_lib/correlation_su2.jl:58-86): static real-space
+ * correlations chi[i + n_sites*k] of the irreducible sites i at the n_chi frequencies
+ * chi_mesh[k] (Mesh.χ), from slot a at Lambda, to the tolerance (atol, rtol) of χ_tol.  The
+ * nine boxes of integrate_χ_boxes (correlation_lib.jl:2-30) are integrated by adaptive 2-D
+ * cubature (degree-7 Genz-Malik rule), at most 8192 regions per box; nreg (optional,
+ * 9*n_sites*n_chi entries) receives the final region count of every box. */
+int pffrg_compute_chi(pffrg_ctx *ctx, double Lambda, int32_t a, const double *chi_mesh, int32_t n_chi,
+                      const double *chi_tol, double *chi, int32_t *nreg);
+
 /* ---- Batched problems.  A batch groups contexts of one device (up to 64: different lattices,
  * truncations, couplings, meshes and Λ) whose 1-loop right-hand sides run together: the
  * self-energy and boundary-correction kernels per problem, then ONE vertex-pass launch over the

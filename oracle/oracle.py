@@ -19,6 +19,15 @@ LIB = Path(os.environ.get("PFFRG_ORACLE_LIB", HERE / "build" / "liboracle.so"))
 P_KAT, P_LEFT, P_CENTRAL, P_BSE, P_REDUCED = range(5)
 
 
+def test_hcubature(kind, p, q, lo, hi, atol, rtol):
+    """2-D adaptive cubature of x^p y^q (kind 0) or exp(-(x^2+y^2)) (kind 1); returns (I, regions)."""
+    nr = C.c_int32(0)
+    lo = np.ascontiguousarray(lo, dtype=np.float64)
+    hi = np.ascontiguousarray(hi, dtype=np.float64)
+    I = lib().orc_test_hcubature(kind, p, q, _dp(lo), _dp(hi), atol, rtol, C.byref(nr))
+    return I, nr.value
+
+
 def build() -> Path:
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
     return LIB
@@ -93,6 +102,11 @@ def lib() -> C.CDLL:
         L.orc_compute_sigma_sde.restype = C.c_long
         L.orc_compute_gamma_bse.argtypes = [_S, C.c_double, _A, _A, C.c_int, _D, _D]
         L.orc_compute_gamma_bse.restype = C.c_long
+        L.orc_compute_chi.argtypes = [_S, C.c_double, _A, _D, C.c_int, _D, _D, C.POINTER(C.c_int32)]
+        L.orc_compute_chi.restype = None
+        L.orc_test_hcubature.argtypes = [C.c_int, C.c_int, C.c_int, _D, _D, C.c_double, C.c_double,
+                                         C.POINTER(C.c_int32)]
+        L.orc_test_hcubature.restype = C.c_double
         _lib = L
     return _lib
 
@@ -183,6 +197,16 @@ class Oracle:
         self.L.orc_integrand(C.byref(self.sys), pass_, Lam, C.byref(self._act(a)), C.byref(self._act(da)), ch, kernel,
                              w1, w2, w3, v, _dp(out))
         return out
+
+    def compute_chi(self, Lam, a, tol=(1e-8, 1e-4), chi_mesh=None, with_regions=False):
+        """compute_χ! (correlation_su2.jl:58-86): chi[site, w] over the irreducible sites."""
+        cm = np.ascontiguousarray(self.m.chi if chi_mesh is None else chi_mesh, dtype=np.float64)
+        out = np.zeros(self.ns * len(cm))
+        nreg = np.zeros(9 * self.ns * len(cm), dtype=np.int32)
+        self.L.orc_compute_chi(C.byref(self.sys), Lam, C.byref(self._act(a)), _dp(cm), len(cm), _dp(_tol(tol)),
+                               _dp(out), nreg.ctypes.data_as(C.POINTER(C.c_int32)))
+        chi = out.reshape(len(cm), self.ns).T.copy()
+        return (chi, nreg) if with_regions else chi
 
     def symmetrize(self, a):
         self.L.orc_symmetrize(C.byref(self.sys), C.byref(self._act(a)))

@@ -816,6 +816,168 @@ void compute_corrs(const orc_system &y, double L, const orc_action &a, const orc
 }
 
 // vertex buffers shared by the self-energy kernels, Flow/flow_lib/flow_su2/flow_su2_self.jl:11-19
+
+// ---------------------------------------------------------------- adaptive 2-D h-cubature
+// Restatement of the h-adaptive cubature of Cubature.jl 1.5 (a wrapper of S. G. Johnson's C
+// library "cubature", not part of the reference sources) as called by integrate_χ_boxes
+// (correlation_lib.jl:2-30): the degree-7 Genz-Malik rule with its embedded degree-5 rule for
+// the error estimate (A. C. Genz, A. A. Malik, J. Comput. Appl. Math. 6 (1980) 295-302), and
+// repeated bisection of the region of largest error along its axis of largest fourth
+// difference until the total error meets max(atol, rtol |I|).  The reference calls the
+// vectorised entry hcubature_v, whose schedule subdivides several regions per round; this
+// restatement splits one region per step (the scalar hcubature schedule), which converges to
+// the same tolerance with fewer evaluations.  Region order (split region in place, upper half
+// appended), ties (lowest index) and the summation order of the totals (64 strided partial
+// sums combined by an xor butterfly) are fixed here and shared with the GPU kernel, so both
+// produce the same regions and sums.
+struct Region {
+    double c[2], h[2], val, err;
+    int split;
+};
+
+const double GM_L2 = 0.35856858280031806;  // sqrt(9/70)
+const double GM_L4 = 0.9486832980505138;   // sqrt(9/10)
+const double GM_L5 = 0.6882472016116853;   // sqrt(9/19)
+const double GM_W1 = -3816.0 / 19683.0, GM_W2 = 980.0 / 6561.0, GM_W3 = 1020.0 / 19683.0, GM_W4 = 200.0 / 19683.0,
+             GM_W5 = 6859.0 / 19683.0 / 4.0;
+const double GM_E1 = -971.0 / 729.0, GM_E2 = 245.0 / 486.0, GM_E3 = 65.0 / 1458.0, GM_E4 = 25.0 / 729.0;
+
+// the 17 rule points of a region in the fixed order f0 | ±λ2 x, ±λ2 y | ±λ4 x, ±λ4 y |
+// (±λ4, ±λ4) | (±λ5, ±λ5)
+inline void gm_point(const Region &R, int k, double &x, double &y) {
+    static const signed char sx[17] = {0, -1, 1, 0, 0, -1, 1, 0, 0, -1, 1, -1, 1, -1, 1, -1, 1};
+    static const signed char sy[17] = {0, 0, 0, -1, 1, 0, 0, -1, 1, -1, -1, 1, 1, -1, -1, 1, 1};
+    const double l = (k < 5) ? GM_L2 : (k < 13) ? GM_L4 : GM_L5;
+    x = R.c[0] + sx[k] * (l * R.h[0]);
+    y = R.c[1] + sy[k] * (l * R.h[1]);
+}
+
+// rule values from the 17 function values (same arithmetic on the GPU)
+inline void gm_combine(Region &R, const double *f) {
+    const double f0 = f[0];
+    const double s2 = ((f[1] + f[2]) + f[3]) + f[4];
+    const double s3 = ((f[5] + f[6]) + f[7]) + f[8];
+    const double s4 = ((f[9] + f[10]) + f[11]) + f[12];
+    const double s5 = ((f[13] + f[14]) + f[15]) + f[16];
+    const double vol = (2.0 * R.h[0]) * (2.0 * R.h[1]);
+    const double r7 = vol * ((((GM_W1 * f0 + GM_W2 * s2) + GM_W3 * s3) + GM_W4 * s4) + GM_W5 * s5);
+    const double r5 = vol * (((GM_E1 * f0 + GM_E2 * s2) + GM_E3 * s3) + GM_E4 * s4);
+    R.val = r7;
+    R.err = std::fabs(r5 - r7);
+    const double d0 = std::fabs(((f[1] + f[2]) - 2.0 * f0) - (1.0 / 7.0) * ((f[5] + f[6]) - 2.0 * f0));
+    const double d1 = std::fabs(((f[3] + f[4]) - 2.0 * f0) - (1.0 / 7.0) * ((f[7] + f[8]) - 2.0 * f0));
+    R.split = (d1 > d0 || (d1 == d0 && R.h[1] > R.h[0])) ? 1 : 0;
+}
+
+inline bool cub_converged(double val, double err, double atol, double rtol) {
+    return err <= atol || err <= std::fabs(val) * rtol;
+}
+
+const int CUB_MAX_REGIONS = 8192;
+
+// Σ_j x[j] over j < n: lane l of 64 adds x[l], x[l+64], ... in order, then the 64 partials are
+// combined by the butterfly p[l] += p[l ^ m] for m = 32, 16, ..., 1 (the GPU reduction order)
+template <class G>
+double lane_sum(int n, const G &x) {
+    double p[64];
+    for (int l = 0; l < 64; ++l) {
+        double a = 0.0;
+        for (int j = l; j < n; j += 64) a += x(j);
+        p[l] = a;
+    }
+    for (int m = 32; m >= 1; m >>= 1) {
+        double q[64];
+        for (int l = 0; l < 64; ++l) q[l] = p[l] + p[l ^ m];
+        for (int l = 0; l < 64; ++l) p[l] = q[l];
+    }
+    return p[0];
+}
+
+// returns the integral over [lo, hi]; *nreg receives the final region count
+template <class F>
+double hcubature2(const F &f, const double *lo, const double *hi, double atol, double rtol, int *nreg = nullptr) {
+    std::vector<Region> R;
+    R.reserve(CUB_MAX_REGIONS);
+    auto eval = [&](Region &r) {
+        double fv[17];
+        for (int k = 0; k < 17; ++k) {
+            double x, y;
+            gm_point(r, k, x, y);
+            fv[k] = f(x, y);
+        }
+        gm_combine(r, fv);
+    };
+    Region r0;
+    r0.c[0] = 0.5 * (lo[0] + hi[0]);
+    r0.c[1] = 0.5 * (lo[1] + hi[1]);
+    r0.h[0] = 0.5 * (hi[0] - lo[0]);
+    r0.h[1] = 0.5 * (hi[1] - lo[1]);
+    eval(r0);
+    R.push_back(r0);
+    double tv = r0.val;
+    while (true) {
+        const int n = (int)R.size();
+        tv = lane_sum(n, [&](int j) { return R[j].val; });
+        const double te = lane_sum(n, [&](int j) { return R[j].err; });
+        if (cub_converged(tv, te, atol, rtol) || n >= CUB_MAX_REGIONS) break;
+        int k = 0;
+        for (int j = 1; j < n; ++j)
+            if (R[j].err > R[k].err) k = j;
+        Region a = R[k], b = R[k];
+        const int d = R[k].split;
+        a.h[d] = 0.5 * R[k].h[d];
+        b.h[d] = a.h[d];
+        a.c[d] = R[k].c[d] - a.h[d];
+        b.c[d] = R[k].c[d] + a.h[d];
+        eval(a);
+        eval(b);
+        R[k] = a;
+        R.push_back(b);
+    }
+    if (nreg) *nreg = (int)R.size();
+    return tv;
+}
+
+// ---------------------------------------------------------------- correlations
+// compute_χ_kernel! (correlation_su2.jl:14-55) at one point (x, y) of the unit square
+inline double chi_kernel(const orc_system &y, const Dims &d, double L, int site, double w, const orc_action &a,
+                         double xx, double yy) {
+    const double v = (2.0 * xx - 1.0) / ((1.0 - xx) * xx);
+    const double vp = (2.0 * yy - 1.0) / ((1.0 - yy) * yy);
+    const double dv = (2.0 * xx * xx - 2.0 * xx + 1) / ((1.0 - xx) * (1.0 - xx) * xx * xx);
+    const double dvp = (2.0 * yy * yy - 2.0 * yy + 1) / ((1.0 - yy) * (1.0 - yy) * yy * yy);
+    const Buffer bs1 = buffer_s(y, v + vp, 0.5 * (v - w - vp), 0.5 * (-v - w + vp));
+    const Buffer bt1 = buffer_t(y, w, v, vp);
+    const Buffer bu1 = buffer_u(y, v - vp, 0.5 * (v - w + vp), 0.5 * (v + w + vp));
+    const double S2 = 2.0 * y.S;
+    const double tp2 = (2.0 * PI) * (2.0 * PI);
+    double val = -(S2 * S2) * gamma_comp(y, d, a, 0, site, bs1, bt1, bu1, ALL) / tp2;
+    if (site == 0) {
+        const Buffer bs2 = buffer_s(y, v + vp, 0.5 * (v - w - vp), 0.5 * (v + w - vp));
+        const Buffer bt2 = buffer_t(y, -v + vp, 0.5 * (v - w + vp), 0.5 * (v + w + vp));
+        const Buffer bu2 = buffer_u(y, -w, v, vp);
+        const double vs = gamma_comp(y, d, a, 0, site, bs2, bt2, bu2, ALL);
+        const double vd = gamma_comp(y, d, a, 1, site, bs2, bt2, bu2, ALL);
+        val -= S2 * (vs - vd) / (2.0 * tp2);
+    }
+    val *= G_full(y, L, v - 0.5 * w, a.Sigma) * G_full(y, L, v + 0.5 * w, a.Sigma) * dv;
+    val *= G_full(y, L, vp - 0.5 * w, a.Sigma) * G_full(y, L, vp + 0.5 * w, a.Sigma) * dvp;
+    return val;
+}
+
+// the nine boxes of integrate_χ_boxes (correlation_lib.jl:2-30), box k = [lo, hi]
+inline void chi_box(double width, int k, double *lo, double *hi) {
+    const double widthp = (width - 2.0 + std::sqrt(width * width + 4.0)) / (2.0 * width);
+    const double size = std::fabs(widthp - 0.5);
+    const double a = 0.5 - size, b = 0.5 + size;
+    const double xl[3] = {0.0, a, b}, xh[3] = {a, b, 1.0};
+    const double yl[3] = {b, a, 0.0}, yh[3] = {1.0, b, a};
+    lo[0] = xl[k % 3];
+    hi[0] = xh[k % 3];
+    lo[1] = yl[k / 3];
+    hi[1] = yh[k / 3];
+}
+
 struct SigmaBufs {
     Buffer b1s, b1t, b1u, b2s, b2t, b2u;
 };
@@ -1112,6 +1274,50 @@ long orc_compute_gamma_bse(const orc_system *y, double L, const orc_action *a1, 
     long n = orc_run_pass(y, P_BSE, L, a1, a1, corrs, 0, eval, tol, a2);
     symmetrize(*y, dims_of(*y), *a2);
     return n;
+}
+
+// compute_χ! (correlation_su2.jl:58-86): chi[i + ns*w] for the irreducible sites and the χ mesh
+void orc_compute_chi(const orc_system *y, double L, const orc_action *a, const double *chi_mesh, int nchi,
+                     const double *tol, double *chi, int32_t *nreg) {
+    Dims d = dims_of(*y);
+    const int ns = d.ns;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int t = 0; t < nchi * ns; ++t) {
+        const int w = t / ns, i = t % ns;
+        const double W = chi_mesh[w];
+        const double ref = L + 0.5 * W;
+        double I = 0.0;
+        for (int k = 0; k < 9; ++k) {
+            double lo[2], hi[2];
+            chi_box(4.0 * ref, k, lo, hi);
+            int nr = 0;
+            I += hcubature2([&](double xx, double yy) { return chi_kernel(*y, d, L, i, W, *a, xx, yy); }, lo, hi,
+                            tol[0], tol[1], &nr);
+            if (nreg) nreg[9 * t + k] = nr;
+        }
+        if (i == 0) {
+            const double inner[3] = {-2.0 * ref, 0.0, 2.0 * ref};
+            auto f = [&](double v) {
+                return (2.0 * y->S) * G_full(*y, L, v - 0.5 * W, a->Sigma) * G_full(*y, L, v + 0.5 * W, a->Sigma) /
+                       (4.0 * PI);
+            };
+            I += quadgk_inf_inf(f, inner, 3, tol[0], tol[1]);
+        }
+        chi[i + ns * w] = I;
+    }
+}
+
+// 2-D cubature of a test integrand over [lo, hi]: kind 0 = x^p y^q, kind 1 = exp(-(x^2+y^2))
+double orc_test_hcubature(int kind, int p, int q, const double *lo, const double *hi, double atol, double rtol,
+                          int32_t *nreg) {
+    int nr = 0;
+    double I;
+    if (kind == 0)
+        I = hcubature2([&](double x, double y) { return std::pow(x, p) * std::pow(y, q); }, lo, hi, atol, rtol, &nr);
+    else
+        I = hcubature2([&](double x, double y) { return std::exp(-(x * x + y * y)); }, lo, hi, atol, rtol, &nr);
+    if (nreg) *nreg = nr;
+    return I;
 }
 
 }  // extern "C"

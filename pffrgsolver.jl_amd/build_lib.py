@@ -15,7 +15,11 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
-SOURCES = ["pffrg.hip", "pass_kat.hip", "pass_left.hip", "pass_central.hip", "pass_bse.hip", "pass_reduced.hip"]
+SOURCES = ["pffrg.hip", "pass_kat.hip", "pass_left.hip", "pass_central.hip", "pass_bse.hip", "pass_reduced.hip",
+           "chi.hip"]
+# per-file flags: the correlation cubature restates the oracle's arithmetic without FMA contraction,
+# so that its adaptive region choices are the oracle's
+FILE_FLAGS = {"chi.hip": ["-ffp-contract=off"]}
 OUT = PKG / "pffrg" / "_lib" / "libpffrg.so"
 
 HIPCC_FLAGS = [
@@ -34,7 +38,8 @@ def build(verbose: bool = False, jobs: int | None = None, out: Path | None = Non
     with tempfile.TemporaryDirectory(prefix="pffrg_build_") as tmpd:
         def compile_one(src: str) -> str:
             obj = str(Path(tmpd) / (Path(src).stem + ".o"))
-            r = subprocess.run([hipcc, *HIPCC_FLAGS, *[f"-D{d}" for d in defines], *inc, "-c", "-o", obj,
+            r = subprocess.run([hipcc, *HIPCC_FLAGS, *FILE_FLAGS.get(src, []), *[f"-D{d}" for d in defines], *inc,
+                                "-c", "-o", obj,
                                 str(CSRC / src)],
                                capture_output=True, text=True)
             if r.returncode != 0:

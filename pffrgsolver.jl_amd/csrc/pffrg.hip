@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "pffrg.h"
+#include "pffrg_chi.h"
 #include "pffrg_device.h"
 #include "pffrg_pass.h"
 
@@ -323,6 +324,9 @@ struct pffrg_ctx {
     int *d_dbg_nodes = nullptr;
     double *d_dbg_res = nullptr;
     long dbg_cap = 0;
+    // correlation cubature scratch (regions of every task), grown on demand
+    void *chi_mem = nullptr;
+    size_t chi_bytes = 0;
     // host copies of the meshes and the log-domain breakpoint table
     std::vector<double> h_Os, h_Ot;
     double *d_logpts = nullptr;
@@ -717,6 +721,7 @@ void pffrg_destroy(pffrg_ctx *c) {
         (void)hipFree(s.gat32);
         (void)hipFree(s.gat64);
     }
+    (void)hipFree(c->chi_mem);
     (void)hipFree(c->d_mesh);
     (void)hipFree(c->d_exch);
     (void)hipFree(c->d_ov_off);
@@ -1276,6 +1281,81 @@ int pffrg_batch_rhs_1l(pffrg_batch *b, const double *Lambda, const int32_t *a, c
     if (hipSetDevice(b->device) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "hipSetDevice failed");
     return prec == PFFRG_FP32_MIXED ? batch_rhs_1l_t<float>(b, Lambda, a, da, eval, sigma_tol, gamma_tol)
                                     : batch_rhs_1l_t<double>(b, Lambda, a, da, eval, sigma_tol, gamma_tol);
+}
+
+// compute_χ! (correlation_su2.jl:58-86)
+int pffrg_compute_chi(pffrg_ctx *c, double L, int32_t a, const double *chi_mesh, int32_t nchi, const double *tol,
+                      double *chi, int32_t *nreg) {
+    int rc;
+    if ((rc = ready(c)) || (rc = check_slot(c, a))) return rc;
+    if (!chi_mesh || !chi || !tol || nchi < 1 || !(L > 0.0)) return fail(c, PFFRG_ERR_ARG, "invalid chi arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int ns = c->sys.ns;
+    const long tasks = (long)nchi * ns * 9;
+    const long cap = CHI_MAX_REGIONS;
+    // regions: 6 double arrays + 1 int array of tasks*cap; then box[tasks], nreg[tasks], prop[nchi], mesh[nchi]
+    auto r256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t need = 6 * r256((size_t)tasks * cap * 8) + r256((size_t)tasks * cap * 4) + r256(tasks * 8) +
+                        r256(tasks * 4) + 2 * r256((size_t)nchi * 8);
+    if (need > c->chi_bytes) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipFree(c->chi_mem));
+        c->chi_mem = nullptr;
+        c->chi_bytes = 0;
+        HIPCHK(c, hipMalloc(&c->chi_mem, need));
+        c->chi_bytes = need;
+    }
+    char *p = (char *)c->chi_mem;
+    auto take = [&](size_t bytes) {
+        char *q = p;
+        p += (bytes + 255) & ~(size_t)255;
+        return (void *)q;
+    };
+    ChiArgs Q;
+    Q.y = c->sys;
+    Q.mesh = c->d_mesh;
+    const Slot &S = c->slots[a];
+    Q.A = ActV<double>{S.gamma, S.bare, S.Sigma};
+    Q.exch = c->d_exch;
+    Q.L = L;
+    Q.atol = tol[0];
+    Q.rtol = tol[1];
+    Q.nchi = nchi;
+    Q.cap = (int)cap;
+    const size_t rb = (size_t)tasks * cap;
+    p = (char *)c->chi_mem;
+    Q.rc0 = (double *)take(rb * 8);
+    Q.rc1 = (double *)take(rb * 8);
+    Q.rh0 = (double *)take(rb * 8);
+    Q.rh1 = (double *)take(rb * 8);
+    Q.rval = (double *)take(rb * 8);
+    Q.rerr = (double *)take(rb * 8);
+    Q.rsplit = (int *)take(rb * 4);
+    Q.box = (double *)take(tasks * 8);
+    Q.nreg = (int *)take(tasks * 4);
+    Q.prop = (double *)take(nchi * 8);
+    double *dchi = (double *)take(nchi * 8);
+    if ((size_t)(p - (char *)c->chi_mem) > c->chi_bytes) return fail(c, PFFRG_ERR_LIMIT, "chi scratch layout overflow");
+    Q.chi = dchi;
+    HIPCHK(c, hipMemcpyAsync(dchi, chi_mesh, nchi * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_chi(Q, c->stream));
+    std::vector<double> box(tasks), prop(nchi);
+    std::vector<int> nr(tasks);
+    HIPCHK(c, hipMemcpyAsync(box.data(), Q.box, tasks * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(nr.data(), Q.nreg, tasks * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(prop.data(), Q.prop, nchi * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int w = 0; w < nchi; ++w)
+        for (int i = 0; i < ns; ++i) {
+            const long t0 = ((long)w * ns + i) * 9;
+            double I = 0.0;
+            for (int k = 0; k < 9; ++k) I += box[t0 + k];
+            if (i == 0) I += prop[w];
+            chi[i + (long)ns * w] = I;
+            if (nreg)
+                for (int k = 0; k < 9; ++k) nreg[t0 + k] = nr[t0 + k];
+        }
+    return 0;
 }
 
 }  // extern "C"

@@ -69,6 +69,7 @@ _SIGS = {
     "pffrg_debug_record": ([_P, C.POINTER(C.c_int32), _D, C.c_int64], C.c_int),
     "pffrg_set_timing": ([_P, _i], C.c_int),
     "pffrg_get_timing": ([_P, _D, _I64], C.c_int),
+    "pffrg_compute_chi": ([_P, _d, _i, _D, _i, _D, _D, C.POINTER(C.c_int32)], C.c_int),
     "pffrg_batch_create": ([C.POINTER(_P), _i, C.POINTER(_P)], C.c_int),
     "pffrg_batch_destroy": ([_P], None),
     "pffrg_batch_last_error": ([_P], C.c_char_p),
@@ -216,6 +217,17 @@ class Context:
         self._ck(self.lib.pffrg_compute_gamma(self.h, L, a1, a2, eval_, _dp(_tol(tol))), "compute_gamma")
 
     # ---------------------------------------------------------------- algebra
+    def compute_chi(self, L, a, tol=(1e-8, 1e-4), chi_mesh=None, with_regions=False):
+        """compute_χ! (correlation_su2.jl:58-86): chi[site, k] over the irreducible sites and
+        the χ frequencies (the mesh's χ by default)."""
+        cm = np.ascontiguousarray(self.m.chi if chi_mesh is None else chi_mesh, dtype=np.float64)
+        out = np.zeros(self.ns * len(cm))
+        nreg = np.zeros(9 * self.ns * len(cm), dtype=np.int32)
+        self._ck(self.lib.pffrg_compute_chi(self.h, L, a, _dp(cm), len(cm), _dp(_tol(tol)), _dp(out),
+                                            nreg.ctypes.data_as(C.POINTER(C.c_int32))), "compute_chi")
+        chi = out.reshape(len(cm), self.ns).T.copy()
+        return (chi, nreg) if with_regions else chi
+
     def symmetrize(self, s):
         self._ck(self.lib.pffrg_symmetrize(self.h, s), "symmetrize")
 

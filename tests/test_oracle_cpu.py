@@ -387,3 +387,56 @@ def test_katanin_pass_preserves_vertex_symmetry(name, size):
             assert np.abs(q3 - q3[ex].transpose(0, 1, 3, 2)).max() <= 2e-3 * np.abs(q3).max()
             q21, q22 = out.view(comp, ch, "q2_1"), out.view(comp, ch, "q2_2")
             assert np.abs(q22 - q21[ex]).max() <= 2e-3 * np.abs(q21).max()
+
+
+# ---------------------------------------------------------------- correlations (compute_χ!)
+def test_hcubature_polynomial_exactness():
+    """The degree-7 Genz-Malik rule integrates every x^p y^q with p + q <= 7 exactly (and the
+    embedded degree-5 rule agrees below degree 6, so those converge on one region)."""
+    lo, hi = [-0.3, 0.2], [1.1, 0.9]
+    for p in range(8):
+        for q in range(8 - p):
+            I, nr = O.test_hcubature(0, p, q, lo, hi, 1e-14, 1e-14)
+            ex = (hi[0] ** (p + 1) - lo[0] ** (p + 1)) / (p + 1) * (hi[1] ** (q + 1) - lo[1] ** (q + 1)) / (q + 1)
+            assert abs(I - ex) <= 1e-13 * max(1.0, abs(ex)), (p, q, I, ex)
+            if p + q <= 5:
+                assert nr == 1
+
+
+def test_hcubature_gaussian():
+    import math
+    I, _ = O.test_hcubature(1, 0, 0, [-1.0, 0.0], [2.0, 1.0], 1e-12, 1e-10)
+    ex = math.pi / 4 * (math.erf(2.0) + math.erf(1.0)) * math.erf(1.0)
+    assert abs(I - ex) <= 1e-10 * ex
+
+
+def _chi_bare_expected(prob, L, W):
+    """χ of a frequency-independent (bare) vertex with Σ = 0: the double integral factorises
+    into (∫ G0(v-W/2) G0(v+W/2) dv)^2 (correlation_su2.jl:14-55), plus the site-0 propagator
+    term (2S)/(4π) ∫ G0 G0 dv (correlation_su2.jl:78-82)."""
+    import math
+    from scipy.integrate import quad
+    G0 = lambda w: 0.0 if abs(w) <= 1e-8 else -math.expm1(-w * w / L ** 2) / w
+    P = lambda v: G0(v - 0.5 * W) * G0(v + 0.5 * W)
+    IP = sum(quad(P, lo, hi, epsabs=1e-14, epsrel=1e-12, limit=400)[0]
+             for lo, hi in ((-np.inf, -W / 2 - 1), (-W / 2 - 1, W / 2 + 1), (W / 2 + 1, np.inf)))
+    S2 = 2.0 * prob.S
+    tp2 = (2.0 * math.pi) ** 2
+    b = prob.bare
+    out = -(S2 ** 2) * b[0] / tp2 * IP * IP
+    out[0] += -S2 * (b[0, 0] - b[1, 0]) / (2.0 * tp2) * IP * IP + S2 * IP / (4.0 * math.pi)
+    return out
+
+
+@pytest.mark.parametrize("tol,bound", [((1e-8, 1e-4), 2e-5), ((1e-12, 1e-6), 1e-7)])
+def test_chi_bare_vertex_closed_form(tol, bound):
+    from pffrg import flow as F
+    prob = F.Problem.build("square", 2, [[1.0]], initial=1.0, num_sigma=8, num_Omega=6, num_nu=6, num_chi=5,
+                           normalize=False)
+    a = A.Action.empty(prob.r.n_sites, prob.mesh)
+    a.bare[:] = prob.bare
+    orc = O.Oracle(prob.r, prob.mesh)
+    chi = orc.compute_chi(1.0, a, tol=tol)
+    for k, W in enumerate(prob.mesh.chi):
+        exp = _chi_bare_expected(prob, 1.0, W)
+        assert np.abs(chi[:, k] - exp).max() <= bound * np.abs(exp).max(), (W, chi[:, k], exp)
