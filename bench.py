@@ -140,7 +140,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["single", "batch"], default="single",
+    ap.add_argument("--workload", choices=["single", "batch", "flow"], default="single",
                     help="single: the 2-loop square L=6 RHS (headline); batch: the J1-J2 sweep + uneven "
                          "truncations as one batched 1-loop RHS")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -154,6 +154,8 @@ def main():
     eval_ = 10
     if args.workload == "batch":
         return batch_workload(args)
+    if args.workload == "flow":
+        return flow_workload(args)
     if args.copy_calibration > 0:
         import torch
         n = args.copy_calibration * (1 << 20) // 4
@@ -231,6 +233,77 @@ def main():
         "cpu_baseline": cpu,
     }
     line.update(res)
+    print(json.dumps(line), flush=True)
+
+
+def flow_workload(args):
+    """Full 1-loop flow of square L=3 Heisenberg, N=24 (25 points/axis), Λ 50 -> 0.3 with BS3
+    step control and χ measured after every accepted step, in FP64 and in FP32-mixed on the GPU;
+    the FP64 oracle's cost is estimated from a sample of its right-hand sides and χ."""
+    prob = F.Problem.build("square", 3, [[1.0]], initial=50.0, num_sigma=24, num_Omega=24, num_nu=24, num_chi=10)
+    a0 = prob.bare_action()
+    chi_tol = (1e-8, 1e-4)
+    runs = {}
+    for mode, tag in ((FP64, "fp64"), (FP32_MIXED, "fp32_mixed")):
+        ctx = prob.context(n_slots=9, precision=mode)
+        t0 = time.perf_counter()
+        res = F.launch_flow(ctx, prob, a0, 50.0, 0.3, loops=1, chi_tol=chi_tol)
+        dt = time.perf_counter() - t0
+        ctx.close()
+        runs[tag] = (res, dt)
+        log(f"  {tag}: {dt:.2f} s, {res.rhs_evals} RHS, {res.accepted} accepted / {res.rejected} rejected, "
+            f"chi {res.chi_s:.2f} s, status {res.status}, Lambda_end {res.Lambdas[-1] if res.Lambdas else None}")
+    r64, t64 = runs["fp64"]
+    r32, t32 = runs["fp32_mixed"]
+    common = [(i, j) for i, L in enumerate(r64.Lambdas) for j, L2 in enumerate(r32.Lambdas) if abs(L - L2) <= 1e-12 * L]
+    chi_err = [float(np.max(np.abs(r32.chi[j] - r64.chi[i])) / np.max(np.abs(r64.chi[i]))) for i, j in common]
+    last = common[-1] if common else None
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O  # checker-only import, cpu_baseline leg
+        orc = O.Oracle(prob.r, prob.mesh)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        orc.set_threads(threads)
+        # a right-hand side and a χ at the flow's start, middle and end, on the FP64 GPU actions of those steps
+        samples = []
+        Ls = [50.0, float(np.sqrt(50.0 * 0.3)), r64.Lambdas[-1]]
+        from pffrg.mesh import get_eval
+        for L in Ls:
+            da = A.Action.empty(prob.r.n_sites, prob.mesh)
+            t0 = time.perf_counter()
+            orc.compute_dsigma(L, a0, da)
+            orc.compute_dgamma_1l(L, a0, da, get_eval(L))
+            t_rhs = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            orc.compute_chi(L, a0, chi_tol)
+            t_chi = time.perf_counter() - t0
+            samples.append((L, t_rhs, t_chi))
+        t_rhs = float(np.mean([x[1] for x in samples]))
+        t_chi = float(np.mean([x[2] for x in samples]))
+        est = r64.rhs_evals * t_rhs + r64.accepted * t_chi
+        cpu = {"value": 1.0 / est, "unit": "flows/s", "cores": threads, "kind": "port",
+               "sample": f"FP64 oracle RHS and chi of the bare action at Lambda {[round(x[0], 3) for x in samples]} "
+                         f"(mean {t_rhs:.3f} s/RHS, {t_chi:.3f} s/chi) x the FP64 GPU flow's {r64.rhs_evals} RHS and "
+                         f"{r64.accepted} chi; estimated {est:.1f} s per flow"}
+    line = {
+        "metric": "full-flow wall time (FP32-mixed vs plain FP64 GPU) at chi rel err <= 1e-4",
+        "value": 1.0 / t32, "unit": "flows/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": t32 * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32-mixed (fp32 interpolation/products, fp64 quadrature and step control)",
+        "data": "bare couplings (deterministic)",
+        "config": {"workload": "1-loop flow square L=3 Heisenberg SU(2), N=24 (+1 points/axis), Lambda 50 -> 0.3, "
+                               "BS3, chi after every accepted step, fixed meshes"},
+        "fp64_wall_s": t64, "fp32_mixed_wall_s": t32, "speedup_vs_fp64": t64 / t32,
+        "rhs_evals": {"fp64": r64.rhs_evals, "fp32_mixed": r32.rhs_evals},
+        "accepted": {"fp64": r64.accepted, "fp32_mixed": r32.accepted},
+        "status": {"fp64": r64.status, "fp32_mixed": r32.status},
+        "chi_rel_err_fp32_vs_fp64": {"max_over_common_steps": max(chi_err) if chi_err else None,
+                                     "at_last_common_Lambda": chi_err[-1] if chi_err else None,
+                                     "last_common_Lambda": r64.Lambdas[last[0]] if last else None,
+                                     "common_steps": len(common)},
+        "cpu_baseline": cpu,
+    }
     print(json.dumps(line), flush=True)
 
 

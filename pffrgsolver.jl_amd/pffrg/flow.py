@@ -82,6 +82,8 @@ class FlowResult:
     rhs_evals: int = 0
     wall_s: float = 0.0
     status: str = ""
+    chi: list = field(default_factory=list)   # χ[site, ω] after every accepted step (if measured)
+    chi_s: float = 0.0                        # wall time spent on χ
 
 
 class BS3Flow:
@@ -93,8 +95,9 @@ class BS3Flow:
 
     def __init__(self, ctx: Context, prob: Problem, a0: Action, Li: float, Lf: float, loops: int = 1, sigma_corr=True,
                  bmin=1e-4, bmax=0.1, min_eval=10, max_eval=100, Stol=(1e-8, 1e-4), Gtol=(1e-8, 1e-4),
-                 ODE_tol=(1e-8, 1e-2), max_steps: int | None = None, verbose=False):
+                 ODE_tol=(1e-8, 1e-2), max_steps: int | None = None, verbose=False, chi_tol=None):
         self.ctx, self.prob, self.Lf, self.loops, self.sigma_corr = ctx, prob, Lf, loops, sigma_corr
+        self.chi_tol = chi_tol
         self.bmin, self.bmax, self.min_eval, self.max_eval = bmin, bmax, min_eval, max_eval
         self.Stol, self.Gtol, self.ODE_tol, self.max_steps, self.verbose = Stol, Gtol, ODE_tol, max_steps, verbose
         self.res = FlowResult()
@@ -194,11 +197,27 @@ class BS3Flow:
                 res.status = "diverged"
                 self.done = True
                 return
+            if self.chi_tol is not None and self.measure():
+                res.status = "non_monotone"
+                self.done = True
+                return
         else:
             res.rejected += 1
         if not (self.L > self.Lf):
             res.status = "finished"
             self.done = True
+
+    def measure(self) -> bool:
+        """compute_χ! of the accepted action (measure, Launcher.jl:2-40); True when the static
+        on-site correlation stopped growing (χ/χ_prev < 0.995, the reference's stop rule)."""
+        t = time.time()
+        chi = self.ctx.compute_chi(self.L, A, self.chi_tol)
+        self.res.chi_s += time.time() - t
+        prev = self.res.chi[-1] if self.res.chi else None
+        self.res.chi.append(chi)
+        if prev is None or prev[0, 0] == 0.0:
+            return False
+        return not (chi[0, 0] / prev[0, 0] >= 0.995)
 
     def result(self) -> FlowResult:
         self.ctx.synchronize()
@@ -208,11 +227,12 @@ class BS3Flow:
 
 def launch_flow(ctx: Context, prob: Problem, a0: Action, Li: float, Lf: float, loops: int = 1, sigma_corr=True,
                 bmin=1e-4, bmax=0.1, min_eval=10, max_eval=100, Stol=(1e-8, 1e-4), Gtol=(1e-8, 1e-4),
-                ODE_tol=(1e-8, 1e-2), max_steps: int | None = None, verbose=False) -> FlowResult:
+                ODE_tol=(1e-8, 1e-2), max_steps: int | None = None, verbose=False, chi_tol=None) -> FlowResult:
     """BS3 flow from Li to Lf (launcher_1l.jl:72-198 and its 2l/ml twins).  Slot A holds
-    the flowing action on return."""
+    the flowing action on return.  With chi_tol, χ is measured after every accepted step and
+    the flow stops when the static on-site correlation turns (launcher_1l.jl:160-178)."""
     f = BS3Flow(ctx, prob, a0, Li, Lf, loops, sigma_corr, bmin, bmax, min_eval, max_eval, Stol, Gtol, ODE_tol,
-                max_steps, verbose)
+                max_steps, verbose, chi_tol)
     while f.begin_step():
         for k in range(4):
             f.before(k)

@@ -45,3 +45,6 @@ class OracleContext:
 
     def compute_dgamma_1l(self, L, a, da, eval_, tol=(1e-8, 1e-4)):
         self.orc.compute_dgamma_1l(L, self.slots[a], self.slots[da], eval_, tol)
+
+    def compute_chi(self, L, a, tol=(1e-8, 1e-4)):
+        return self.orc.compute_chi(L, self.slots[a], tol)
