@@ -165,6 +165,36 @@ const char *pffrg_batch_last_error(const pffrg_batch *batch);
 int pffrg_batch_rhs_1l(pffrg_batch *batch, const double *Lambda, const int32_t *a, const int32_t *da,
                        const int32_t *eval, const double *sigma_tol, const double *gamma_tol);
 
+/* ---- Learned warm start of the parquet iteration (no reference counterpart: the reference
+ * starts launch_parquet! from the bare action, Launcher.jl:676-678 and parquet.jl:54).  A
+ * multilayer perceptron maps (couplings, Λ) features to coefficients of a fixed basis of
+ * parquet solutions; pffrg_decoder expands coefficients into a vertex.  Products run on MFMA
+ * with BF16 operands and FP32 accumulation; weights, moments and the decoder are FP32.
+ * Layers: n_in -> width (n_layers - 1 times, SiLU) -> n_out (linear). */
+typedef struct pffrg_mlp pffrg_mlp;
+int pffrg_mlp_create(int32_t n_in, int32_t width, int32_t n_layers, int32_t n_out, uint64_t seed, int32_t device,
+                     pffrg_mlp **out);
+void pffrg_mlp_destroy(pffrg_mlp *mlp);
+const char *pffrg_mlp_last_error(const pffrg_mlp *mlp);
+int64_t pffrg_mlp_num_params(const pffrg_mlp *mlp);
+/* parameters: per layer W (out x in, row-major) then b (out), FP32 */
+int pffrg_mlp_get_params(pffrg_mlp *mlp, float *params);
+int pffrg_mlp_set_params(pffrg_mlp *mlp, const float *params);
+/* y[batch][n_out] = mlp(x[batch][n_in]) */
+int pffrg_mlp_forward(pffrg_mlp *mlp, const float *x, int32_t batch, float *y);
+/* loss = mean (y - target)^2 and its gradient w.r.t. every parameter (kept on the device; also
+ * copied to grads when it is not NULL) */
+int pffrg_mlp_backward(pffrg_mlp *mlp, const float *x, const float *target, int32_t batch, float *loss, float *grads);
+/* one Adam update with the last gradient (step counts from 1) */
+int pffrg_mlp_adam(pffrg_mlp *mlp, float lr, float beta1, float beta2, float eps, int32_t step);
+
+typedef struct pffrg_decoder pffrg_decoder;
+/* out[j] = mean[j] + Σ_k coef[k] basis[k*length + j] */
+int pffrg_decoder_create(int64_t length, int32_t n_coef, const float *mean, const float *basis, int32_t device,
+                         pffrg_decoder **out);
+void pffrg_decoder_destroy(pffrg_decoder *dec);
+int pffrg_decoder_apply(pffrg_decoder *dec, const float *coef, double *out);
+
 #ifdef __cplusplus
 }
 #endif

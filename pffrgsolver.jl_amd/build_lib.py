@@ -16,7 +16,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 SOURCES = ["pffrg.hip", "pass_kat.hip", "pass_left.hip", "pass_central.hip", "pass_bse.hip", "pass_reduced.hip",
-           "chi.hip"]
+           "chi.hip", "mlp.hip"]
 # per-file flags: the correlation cubature restates the oracle's arithmetic without FMA contraction,
 # so that its adaptive region choices are the oracle's
 FILE_FLAGS = {"chi.hip": ["-ffp-contract=off"]}

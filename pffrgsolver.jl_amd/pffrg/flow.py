@@ -284,11 +284,17 @@ class ParquetResult:
     abs_err: float = float("inf")
     rel_err: float = float("inf")
     history: list = field(default_factory=list)
+    status: str = ""
 
 
 def launch_parquet(ctx: Context, L: float, a0: Action, max_iter=50, min_eval=10, max_eval=100, Stol=(1e-8, 1e-4),
-                   Gtol=(1e-8, 1e-4), parquet_tol=(1e-8, 1e-4), slot_a=A, slot_ap=DA, slot_err=A_ERR) -> ParquetResult:
-    """Damped parquet fixed point (parquet.jl:1-100); slot_a holds the solution on return."""
+                   Gtol=(1e-8, 1e-4), parquet_tol=(1e-8, 1e-4), slot_a=A, slot_ap=DA, slot_err=A_ERR,
+                   max_seconds: float | None = None, verbose=False) -> ParquetResult:
+    """Damped parquet fixed point (parquet.jl:1-100); slot_a holds the solution on return.
+    Besides the reference's stops (tolerance, max_iter), the iteration ends when the vertex
+    maximum reaches 1000 (the flow's divergence bound, launcher_1l.jl:145) and, optionally,
+    after max_seconds of wall time (checked between iterations)."""
+    t0 = time.time()
     res = ParquetResult()
     ctx.upload(slot_a, a0)
     zero = Action.empty(a0.ns, ctx.m, a0.S)
@@ -309,6 +315,10 @@ def launch_parquet(ctx: Context, L: float, a0: Action, max_iter=50, min_eval=10,
             ctx.set_sigma(slot_a, (1.0 - beta) * sa + beta * sp)
             s_count += 1
             res.sde_iterations += 1
+            if verbose and s_count % 10 == 0:
+                print(f"      SDE iteration {s_count - 1}: abs err {s_abs:.2e} ({time.time() - t0:.1f} s)", flush=True)
+            if max_seconds is not None and time.time() - t0 > max_seconds:
+                break
         ctx.compute_gamma(L, slot_a, slot_ap, ev, Gtol)
         ctx.replace_with(slot_err, slot_ap, vertex_only=True)
         ctx.mult_with_add_to(slot_a, -1.0, slot_err, vertex_only=True)
@@ -317,7 +327,18 @@ def launch_parquet(ctx: Context, L: float, a0: Action, max_iter=50, min_eval=10,
         res.history.append((abs_err, rel_err))
         ctx.mult_with(slot_a, 1.0 - beta, vertex_only=True)
         ctx.mult_with_add_to(slot_ap, beta, slot_a, vertex_only=True)
+        if verbose:
+            print(f"    parquet iteration {count}: abs err {abs_err:.2e} rel err {rel_err:.2e} "
+                  f"(SDE total {res.sde_iterations}, {time.time() - t0:.1f} s)", flush=True)
         count += 1
+        if ctx.abs_max(slot_a) >= 1000.0:
+            res.status = "diverged"
+            break
+        if max_seconds is not None and time.time() - t0 > max_seconds:
+            res.status = "time"
+            break
+    else:
+        res.status = "converged" if (abs_err < parquet_tol[0] or rel_err < parquet_tol[1]) else "max_iter"
     res.iterations = count - 1
     res.abs_err, res.rel_err = abs_err, rel_err
     return res

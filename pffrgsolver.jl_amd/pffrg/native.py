@@ -70,6 +70,19 @@ _SIGS = {
     "pffrg_set_timing": ([_P, _i], C.c_int),
     "pffrg_get_timing": ([_P, _D, _I64], C.c_int),
     "pffrg_compute_chi": ([_P, _d, _i, _D, _i, _D, _D, C.POINTER(C.c_int32)], C.c_int),
+    "pffrg_mlp_create": ([_i, _i, _i, _i, C.c_uint64, _i, C.POINTER(_P)], C.c_int),
+    "pffrg_mlp_destroy": ([_P], None),
+    "pffrg_mlp_last_error": ([_P], C.c_char_p),
+    "pffrg_mlp_num_params": ([_P], C.c_int64),
+    "pffrg_mlp_get_params": ([_P, C.POINTER(C.c_float)], C.c_int),
+    "pffrg_mlp_set_params": ([_P, C.POINTER(C.c_float)], C.c_int),
+    "pffrg_mlp_forward": ([_P, C.POINTER(C.c_float), _i, C.POINTER(C.c_float)], C.c_int),
+    "pffrg_mlp_backward": ([_P, C.POINTER(C.c_float), C.POINTER(C.c_float), _i, C.POINTER(C.c_float),
+                            C.POINTER(C.c_float)], C.c_int),
+    "pffrg_mlp_adam": ([_P, C.c_float, C.c_float, C.c_float, C.c_float, _i], C.c_int),
+    "pffrg_decoder_create": ([C.c_int64, _i, C.POINTER(C.c_float), C.POINTER(C.c_float), _i, C.POINTER(_P)], C.c_int),
+    "pffrg_decoder_destroy": ([_P], None),
+    "pffrg_decoder_apply": ([_P, C.POINTER(C.c_float), _D], C.c_int),
     "pffrg_batch_create": ([C.POINTER(_P), _i, C.POINTER(_P)], C.c_int),
     "pffrg_batch_destroy": ([_P], None),
     "pffrg_batch_last_error": ([_P], C.c_char_p),

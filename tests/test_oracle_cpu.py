@@ -440,3 +440,30 @@ def test_chi_bare_vertex_closed_form(tol, bound):
     for k, W in enumerate(prob.mesh.chi):
         exp = _chi_bare_expected(prob, 1.0, W)
         assert np.abs(chi[:, k] - exp).max() <= bound * np.abs(exp).max(), (W, chi[:, k], exp)
+
+
+def test_mlp_reference_backprop_matches_finite_differences():
+    """The numpy restatement of the warm-start network's back-propagation (FP64, no rounding)
+    against central finite differences of its loss."""
+    import mlp_ref as R
+    rng = np.random.default_rng(3)
+    dims = R.dims_of(2, 7, 4, 5)
+    n = sum(dims[l] * dims[l + 1] + dims[l + 1] for l in range(len(dims) - 1))
+    p = rng.standard_normal(n) * 0.5
+    x = rng.standard_normal((6, 2))
+    t = rng.standard_normal((6, 5))
+    _, g = R.loss_grad(p, dims, x, t, bf=False)
+    fd = np.zeros(n)
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = 1e-6
+        fd[i] = (R.loss_grad(p + e, dims, x, t, bf=False)[0] - R.loss_grad(p - e, dims, x, t, bf=False)[0]) / 2e-6
+    assert np.max(np.abs(g - fd)) <= 1e-7 * max(1.0, np.max(np.abs(fd)))
+
+
+def test_coupling_samples_deterministic():
+    from pffrg import learn
+    s = learn.coupling_samples(80, seed=5)
+    assert s.shape == (80, 2)
+    assert np.all((s[:, 0] >= -0.5) & (s[:, 0] <= 0.5)) and np.all((s[:, 1] >= 0.5) & (s[:, 1] <= 5.0))
+    np.testing.assert_array_equal(s, learn.coupling_samples(80, seed=5))
