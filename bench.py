@@ -145,6 +145,12 @@ def main():
     ap.add_argument("--ws-test", type=int, default=16, help="warmstart: held-out couplings")
     ap.add_argument("--ws-steps", type=int, default=20000, help="warmstart: Adam steps")
     ap.add_argument("--ws-verbose", action="store_true", help="warmstart: print every parquet iteration")
+    ap.add_argument("--flow-config", choices=["square", "pyrochlore"], default="square",
+                    help="flow: square L=3 1-loop N=24 50->0.3, or pyrochlore L=4 3-loop with self-energy "
+                         "corrections N=32 50->0.1")
+    ap.add_argument("--flow-modes", default="fp64,fp32", help="flow: precisions to run, in order")
+    ap.add_argument("--flow-max-seconds", type=float, default=None, help="flow: wall-time budget per flow")
+    ap.add_argument("--flow-verbose", action="store_true", help="flow: print every BS3 step")
     ap.add_argument("--workload", choices=["single", "batch", "flow", "warmstart"], default="single",
                     help="single: the 2-loop square L=6 RHS (headline); batch: the J1-J2 sweep + uneven "
                          "truncations as one batched 1-loop RHS")
@@ -243,72 +249,120 @@ def main():
     print(json.dumps(line), flush=True)
 
 
+FLOW_CONFIGS = {
+    # launch!(f, "square", 3, "heisenberg", "su2", [1.0]; loops=1, parquet=false, initial=50.0, final=0.3,
+    #         num_σ=num_Ω=num_ν=24, num_χ=10)
+    "square": dict(name="square", size=3, N=24, loops=1, Li=50.0, Lf=0.3,
+                   workload="1-loop flow square L=3 Heisenberg SU(2), N=24 (+1 points/axis), Lambda 50 -> 0.3"),
+    # launch!(f, "pyrochlore", 4, "heisenberg", "su2", [1.0]; loops=3, Σ_corr=true, parquet=false,
+    #         initial=50.0, final=0.1, num_σ=num_Ω=num_ν=32)
+    "pyrochlore": dict(name="pyrochlore", size=4, N=32, loops=3, Li=50.0, Lf=0.1,
+                       workload="3-loop flow with self-energy corrections, pyrochlore L=4 Heisenberg SU(2), "
+                                "N=32 (+1 points/axis), Lambda 50 -> 0.1"),
+}
+
+
 def flow_workload(args):
-    """Full 1-loop flow of square L=3 Heisenberg, N=24 (25 points/axis), Λ 50 -> 0.3 with BS3
-    step control and χ measured after every accepted step, in FP64 and in FP32-mixed on the GPU;
-    the FP64 oracle's cost is estimated from a sample of its right-hand sides and χ."""
-    prob = F.Problem.build("square", 3, [[1.0]], initial=50.0, num_sigma=24, num_Omega=24, num_nu=24, num_chi=10)
+    """Full flow (BS3 step control, χ measured and meshes rescanned after every accepted step) of
+    the square L=3 1-loop or the pyrochlore L=4 3-loop configuration, in FP64 and in FP32-mixed
+    on the GPU; the FP64 oracle's cost is estimated from a sample of its right-hand sides and χ
+    at the flow's start and end."""
+    cfg = FLOW_CONFIGS[args.flow_config]
+    N = cfg["N"]
+    prob = F.Problem.build(cfg["name"], cfg["size"], [[1.0]], initial=cfg["Li"], num_sigma=N, num_Omega=N, num_nu=N,
+                           num_chi=10)
     a0 = prob.bare_action()
     chi_tol = (1e-8, 1e-4)
     runs = {}
-    for mode, tag in ((FP64, "fp64"), (FP32_MIXED, "fp32_mixed")):
+    end_state = None
+    modes = [m.strip() for m in args.flow_modes.split(",") if m.strip()]
+    for tag in modes:
+        mode = {"fp64": FP64, "fp32": FP32_MIXED}[tag]
         ctx = prob.context(n_slots=9, precision=mode)
+        log(f"flow {args.flow_config} {tag}: Lambda {cfg['Li']} -> {cfg['Lf']}, {cfg['loops']}-loop, "
+            f"{prob.r.n_sites} sites, N={N}")
         t0 = time.perf_counter()
-        res = F.launch_flow(ctx, prob, a0, 50.0, 0.3, loops=1, chi_tol=chi_tol)
+        res = F.launch_flow(ctx, prob, a0, cfg["Li"], cfg["Lf"], loops=cfg["loops"], chi_tol=chi_tol,
+                            verbose=args.flow_verbose, max_seconds=args.flow_max_seconds)
         dt = time.perf_counter() - t0
+        if end_state is None:
+            end_state = (ctx.download(F.A), ctx.m)
         ctx.close()
         runs[tag] = (res, dt)
         log(f"  {tag}: {dt:.2f} s, {res.rhs_evals} RHS, {res.accepted} accepted / {res.rejected} rejected, "
             f"chi {res.chi_s:.2f} s, status {res.status}, Lambda_end {res.Lambdas[-1] if res.Lambdas else None}")
-    r64, t64 = runs["fp64"]
-    r32, t32 = runs["fp32_mixed"]
-    common = [(i, j) for i, L in enumerate(r64.Lambdas) for j, L2 in enumerate(r32.Lambdas) if abs(L - L2) <= 1e-12 * L]
-    chi_err = [float(np.max(np.abs(r32.chi[j] - r64.chi[i])) / np.max(np.abs(r64.chi[i]))) for i, j in common]
+    r64, t64 = runs.get("fp64", (None, None))
+    r32, t32 = runs.get("fp32", (None, None))
+    common, chi_err, mesh_diff = [], [], []
+    if r64 is not None and r32 is not None:
+        common = [(i, j) for i, L in enumerate(r64.Lambdas) for j, L2 in enumerate(r32.Lambdas)
+                  if abs(L - L2) <= 1e-12 * L]
+        chi_err = [float(np.max(np.abs(r32.chi[j] - r64.chi[i])) / np.max(np.abs(r64.chi[i]))) for i, j in common]
+        mesh_diff = [max(float(np.max(np.abs(getattr(r32.meshes[j], f) - getattr(r64.meshes[i], f)) /
+                                   np.max(getattr(r64.meshes[i], f)))) for f in ("sigma", "Os", "ns", "Ot", "nt", "chi"))
+                     for i, j in common if i < len(r64.meshes) and j < len(r32.meshes)]
     last = common[-1] if common else None
+    rref = r64 if r64 is not None else r32
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "oracle"))
         import oracle as O  # checker-only import, cpu_baseline leg
-        orc = O.Oracle(prob.r, prob.mesh)
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-        orc.set_threads(threads)
-        # a right-hand side and a χ at the flow's start, middle and end, on the FP64 GPU actions of those steps
+        # a right-hand side and a χ at the flow's start (bare action, initial meshes) and at its end
+        # (the first run's final action on its final meshes)
         samples = []
-        Ls = [50.0, float(np.sqrt(50.0 * 0.3)), r64.Lambdas[-1]]
         from pffrg.mesh import get_eval
-        for L in Ls:
-            da = A.Action.empty(prob.r.n_sites, prob.mesh)
+        a_end, m_end = end_state
+        for L, act, m in ((cfg["Li"], a0, prob.mesh), (rref.Lambdas[-1], a_end, m_end)):
+            o = O.Oracle(prob.r, m)
+            o.set_threads(threads)
+            slots = [A.Action.empty(prob.r.n_sites, m) for _ in range(5)]
+            log(f"  oracle sample at Lambda {L:.4g}")
             t0 = time.perf_counter()
-            orc.compute_dsigma(L, a0, da)
-            orc.compute_dgamma_1l(L, a0, da, get_eval(L))
+            o.compute_dsigma(L, act, slots[0])
+            ev = get_eval(L)
+            if cfg["loops"] == 1:
+                o.compute_dgamma_1l(L, act, slots[0], ev)
+            else:
+                o.compute_dgamma_ml(L, cfg["loops"], act, slots[0], slots[1], slots[2], slots[3], slots[4], ev)
+                o.compute_dsigma_corr(L, act, slots[0], slots[4])
             t_rhs = time.perf_counter() - t0
             t0 = time.perf_counter()
-            orc.compute_chi(L, a0, chi_tol)
+            o.compute_chi(L, act, chi_tol)
             t_chi = time.perf_counter() - t0
             samples.append((L, t_rhs, t_chi))
+            log(f"    {t_rhs:.1f} s RHS, {t_chi:.1f} s chi")
         t_rhs = float(np.mean([x[1] for x in samples]))
         t_chi = float(np.mean([x[2] for x in samples]))
-        est = r64.rhs_evals * t_rhs + r64.accepted * t_chi
+        est = rref.rhs_evals * t_rhs + rref.accepted * t_chi
         cpu = {"value": 1.0 / est, "unit": "flows/s", "cores": threads, "kind": "port",
-               "sample": f"FP64 oracle RHS and chi of the bare action at Lambda {[round(x[0], 3) for x in samples]} "
-                         f"(mean {t_rhs:.3f} s/RHS, {t_chi:.3f} s/chi) x the FP64 GPU flow's {r64.rhs_evals} RHS and "
-                         f"{r64.accepted} chi; estimated {est:.1f} s per flow"}
+               "sample": f"FP64 oracle RHS and chi at the flow's first and last Lambda {[round(x[0], 3) for x in samples]} "
+                         f"(mean {t_rhs:.3f} s/RHS, {t_chi:.3f} s/chi) x the GPU flow's {rref.rhs_evals} RHS and "
+                         f"{rref.accepted} chi; estimated {est:.1f} s per flow"}
+    tv = t32 if t32 is not None else t64
     line = {
         "metric": "full-flow wall time (FP32-mixed vs plain FP64 GPU) at chi rel err <= 1e-4",
-        "value": 1.0 / t32, "unit": "flows/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": t32 * 1e3,
+        "value": 1.0 / tv, "unit": "flows/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": tv * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32-mixed (fp32 interpolation/products, fp64 quadrature and step control)",
+        "dtype": ("fp32-mixed (fp32 interpolation/products, fp64 quadrature and step control)" if t32 is not None
+                  else "f64"),
         "data": "bare couplings (deterministic)",
-        "config": {"workload": "1-loop flow square L=3 Heisenberg SU(2), N=24 (+1 points/axis), Lambda 50 -> 0.3, "
-                               "BS3, chi after every accepted step, fixed meshes"},
-        "fp64_wall_s": t64, "fp32_mixed_wall_s": t32, "speedup_vs_fp64": t64 / t32,
-        "rhs_evals": {"fp64": r64.rhs_evals, "fp32_mixed": r32.rhs_evals},
-        "accepted": {"fp64": r64.accepted, "fp32_mixed": r32.accepted},
-        "status": {"fp64": r64.status, "fp32_mixed": r32.status},
+        "config": {"workload": cfg["workload"] + ", BS3, chi and mesh rescanning after every accepted step",
+                   "modes": modes, "max_seconds": args.flow_max_seconds},
+        "wall_s": {k: v[1] for k, v in runs.items()},
+        "speedup_vs_fp64": (t64 / t32) if (t64 and t32) else None,
+        "rhs_evals": {k: v[0].rhs_evals for k, v in runs.items()},
+        "accepted": {k: v[0].accepted for k, v in runs.items()},
+        "rejected": {k: v[0].rejected for k, v in runs.items()},
+        "status": {k: v[0].status for k, v in runs.items()},
+        "Lambda_end": {k: (v[0].Lambdas[-1] if v[0].Lambdas else None) for k, v in runs.items()},
+        "chi_s": {k: v[0].chi_s for k, v in runs.items()},
         "chi_rel_err_fp32_vs_fp64": {"max_over_common_steps": max(chi_err) if chi_err else None,
                                      "at_last_common_Lambda": chi_err[-1] if chi_err else None,
                                      "last_common_Lambda": r64.Lambdas[last[0]] if last else None,
                                      "common_steps": len(common)},
+        "mesh_rel_diff_fp32_vs_fp64": max(mesh_diff) if mesh_diff else None,
+        "chi_final": {k: v[0].chi[-1][:, 0].tolist() for k, v in runs.items() if v[0].chi},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -360,17 +414,22 @@ def warmstart_workload(args):
                      "bare_iterations": rb.iterations,
                      "warm_iterations": rw.iterations, "bare_residual": rb.abs_err, "warm_residual": rw.abs_err,
                      "solution_rel_diff": d})
-    ib = np.array([r["bare_iterations"] for r in rows], dtype=float)
-    iw = np.array([r["warm_iterations"] for r in rows], dtype=float)
+    # the iteration count is compared where both starts reach the residual; a solve that stalls
+    # above it (quadrature noise of the right-hand side) runs to max_iter from either start
+    conv = [r for r in rows if r["status"] == ["converged", "converged"]]
+    ib = np.array([r["bare_iterations"] for r in conv], dtype=float)
+    iw = np.array([r["warm_iterations"] for r in conv], dtype=float)
     line = {
         "metric": "parquet iterations to residual 1e-8: bare start / learned warm start",
-        "value": float(ib.sum() / iw.sum()), "unit": "x fewer iterations", "n_gpus": 1, "steps": len(rows),
+        "value": float(ib.sum() / iw.sum()) if conv else None, "unit": "x fewer iterations", "n_gpus": 1,
+        "steps": len(rows),
         "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "bf16 MFMA (fp32 accumulation) network, fp32 decoder, fp64 parquet",
         "data": f"parquet solutions generated here ({len(train)} training, {len(test)} held-out, seed 5)",
         "config": {"workload": f"kagome L=5 J1-J2 Heisenberg, N={N} (+1 points/axis), J2/J1 in [-0.5,0.5], "
                                f"Lambda in [0.5,5], MLP 4 x 512 SiLU, SVD coefficients per channel {rep.n_coef}"},
-        "mean_iterations": {"bare": float(ib.mean()), "warm": float(iw.mean())},
+        "mean_iterations": {"bare": float(ib.mean()) if conv else None, "warm": float(iw.mean()) if conv else None},
+        "converged_both": len(conv), "stalled": len(rows) - len(conv),
         "held_out": rows, "train_iterations_mean": float(np.mean(it_train)),
         "times_s": {"data": t_data, "fit": t_fit, "total": time.perf_counter() - t0},
         "fit": {"loss_first": rep.losses[0], "loss_last": rep.losses[-1], "train_rel_err": rep.train_rel_err},

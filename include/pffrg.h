@@ -149,6 +149,35 @@ int pffrg_get_timing(pffrg_ctx *ctx, double *total_ms, int64_t *launches);
 int pffrg_compute_chi(pffrg_ctx *ctx, double Lambda, int32_t a, const double *chi_mesh, int32_t n_chi,
                       const double *chi_tol, double *chi, int32_t *nreg);
 
+/* ---- Mesh rescanning (resample_from_to, Action/Action.jl:483-557), in three parts:
+ * pffrg_scan_cuts gathers what scan_channel reads, pffrg_scan is the scan heuristic, and
+ * pffrg_resample moves an action onto the new meshes. */
+
+/* scan_channel (Action.jl:449-480), device part: for slot a and every (component c, channel
+ * ch), k = 3*c + ch, the first position of max |q3| in column-major order (Julia's argmax),
+ * 0-based (site, w, v, v') into idx[4k .. 4k+3], and the four cuts through it into
+ * cuts[k*(n_Omega + 3*n_nu) ...]:
+ *   q3[i, :, v, v'] (n_Omega values),
+ *   q3[i, w, x, x]  - q3[i, w, end, end],
+ *   q3[i, w, x, v'] - q3[i, w, end, v'],
+ *   q3[i, w, v, x]  - q3[i, w, v, end]   (n_nu values each, x = 0 .. n_nu-1). */
+int pffrg_scan_cuts(pffrg_ctx *ctx, int32_t a, int32_t *idx, double *cuts);
+
+/* scan (Action.jl:369-446), host only: the new linear extent of a mesh x[0..n-1] (built by
+ * get_mesh with linear fraction p[0]) for the cut y[0..n-1], from the peak and origin
+ * heuristics with bounds p[1] < Δ < p[2] and p[3] <= δ <= p[4] (p[3], p[4] already
+ * scaled by Λ).  Needs no device. */
+int pffrg_scan(const double *x, const double *y, int32_t n, const double *p, double *delta);
+
+/* resample_from_to (Action.jl:531-556; channel.jl:416-470; vertex.jl:148-160): slot src,
+ * held on the context's meshes, is interpolated onto the new meshes (sigma, Os, ns, Ot, nt;
+ * same lengths) into slot dst: Σ by get_Σ (Action.jl:8-26), every q1/q2/q3 entry by the
+ * reference's multilinear rule (get_param clamps beyond the old mesh), the bare couplings
+ * copied.  The new meshes then replace the context's, as pffrg_set_mesh; the other slots
+ * keep their data.  src != dst. */
+int pffrg_resample(pffrg_ctx *ctx, int32_t src, int32_t dst, const double *sigma, const double *Os,
+                   const double *ns, const double *Ot, const double *nt);
+
 /* ---- Batched problems.  A batch groups contexts of one device (up to 64: different lattices,
  * truncations, couplings, meshes and Λ) whose 1-loop right-hand sides run together: the
  * self-energy and boundary-correction kernels per problem, then ONE vertex-pass launch over the

@@ -107,6 +107,12 @@ def lib() -> C.CDLL:
         L.orc_test_hcubature.argtypes = [C.c_int, C.c_int, C.c_int, _D, _D, C.c_double, C.c_double,
                                          C.POINTER(C.c_int32)]
         L.orc_test_hcubature.restype = C.c_double
+        L.orc_scan.argtypes = [_D, _D, C.c_int, _D, _D]
+        L.orc_scan.restype = C.c_int
+        L.orc_scan_cuts.argtypes = [_S, _A, C.POINTER(C.c_int32), _D]
+        L.orc_scan_cuts.restype = None
+        L.orc_resample.argtypes = [_S, _D, _D, _D, _D, _D, _A, _A]
+        L.orc_resample.restype = None
         _lib = L
     return _lib
 
@@ -208,6 +214,21 @@ class Oracle:
         chi = out.reshape(len(cm), self.ns).T.copy()
         return (chi, nreg) if with_regions else chi
 
+    def scan_cuts(self, a):
+        """scan_channel's argmax positions idx[6, 4] and cuts[6, nO + 3 nn] (Action.jl:449-476)."""
+        idx = np.zeros(24, dtype=np.int32)
+        cuts = np.zeros(6 * (self.m.num_Omega + 3 * self.m.num_nu))
+        self.L.orc_scan_cuts(C.byref(self.sys), C.byref(self._act(a)), idx.ctypes.data_as(C.POINTER(C.c_int32)),
+                             _dp(cuts))
+        return idx.reshape(6, 4), cuts.reshape(6, -1)
+
+    def resample(self, a_old, m_new, a_new):
+        """resample_from_to's interpolation of a_old (on this oracle's meshes) onto m_new."""
+        arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (m_new.sigma, m_new.Os, m_new.ns, m_new.Ot,
+                                                                    m_new.nt)]
+        self.L.orc_resample(C.byref(self.sys), *[_dp(x) for x in arrs], C.byref(self._act(a_old)),
+                            C.byref(self._act(a_new)))
+
     def symmetrize(self, a):
         self.L.orc_symmetrize(C.byref(self.sys), C.byref(self._act(a)))
 
@@ -238,6 +259,17 @@ class Oracle:
 
     def S(self, Lam, w, Sig) -> float:
         return self.L.orc_S(C.byref(self.sys), Lam, w, _dp(np.ascontiguousarray(Sig)))
+
+
+def scan(x, y, p) -> float:
+    """scan (Action.jl:369-446): new linear extent for the cut y on mesh x."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    pp = np.ascontiguousarray(p, dtype=np.float64)
+    d = np.zeros(1)
+    if lib().orc_scan(_dp(x), _dp(y), len(x), _dp(pp), _dp(d)) != 0:
+        raise ValueError("scan: invalid mesh")
+    return float(d[0])
 
 
 def get_param(val: float, mesh: np.ndarray):

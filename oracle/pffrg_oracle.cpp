@@ -1320,4 +1320,171 @@ double orc_test_hcubature(int kind, int p, int q, const double *lo, const double
     return I;
 }
 
+
+// ======================================================= mesh rescanning (Action.jl:369-557)
+// get_mesh, Frequency/mesh.jl:86-121 (returns false where the reference asserts)
+static bool orc_mesh(double linear, double upper, int num, double p, std::vector<double> &m) {
+    if (!(linear > 0.0) || !(upper > 0.0) || !(linear < upper)) return false;
+    const int num_lin = (int)std::ceil(p * num);
+    const int num_log = num - num_lin;
+    if (num_log <= 0) return false;
+    const double h = linear / num_lin;
+    const double xi = std::pow(upper / linear, 1.0 / num_log);
+    m.assign(num + 1, 0.0);
+    for (int i = 1; i <= num_lin; ++i) m[i] = i * h;
+    for (int i = 1; i <= num_log; ++i) m[num_lin + i] = std::pow(xi, (double)i) * linear;
+    return true;
+}
+
+static double jsign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : x); }
+static double jmax(double a, double b) {
+    if (std::isnan(a) || std::isnan(b)) return NAN;
+    if (a == b) return std::signbit(a) ? b : a;
+    return a > b ? a : b;
+}
+static double jmin(double a, double b) {
+    if (std::isnan(a) || std::isnan(b)) return NAN;
+    if (a == b) return std::signbit(a) ? a : b;
+    return a < b ? a : b;
+}
+
+// scan, Action/Action.jl:369-446.  Returns 0, or -1 where the reference asserts.
+int orc_scan(const double *x, const double *y, int n, const double *prm, double *delta) {
+    const double p0 = prm[0], p1 = prm[1], p2 = prm[2], p3 = prm[3], p4 = prm[4];
+    const int num_lin = (int)std::ceil(p0 * (n - 1));
+    // a) (1-based idx of the reference = i + 1 here)
+    double da = INFINITY;
+    int idx = 0;
+    double slope = (y[1] - y[0]) / (x[1] - x[0]);
+    for (int i = 1; i <= n - 2; ++i) {
+        if (jsign((y[i + 1] - y[i]) / (x[i + 1] - x[i])) != jsign(slope)) {
+            idx = i;
+            break;
+        } else {
+            slope = (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
+        }
+    }
+    if (idx > 0) {
+        for (int i = idx + 1; i <= n - 1; ++i) {
+            if (std::fabs(y[i] / y[idx]) <= 0.75) {
+                da = jmax(x[i], num_lin * x[idx] / std::ceil(0.5 * num_lin));
+                break;
+            }
+        }
+    }
+    // b)
+    double db = num_lin * x[1];
+    double ymax = std::fabs(y[0]);
+    for (int i = 1; i < n; ++i) ymax = jmax(ymax, std::fabs(y[i]));
+    if (std::fabs(y[0]) / ymax > 1e-2) {
+        double D = std::fabs(y[1] - y[0]) / jmax(std::fabs(y[1]), std::fabs(y[0]));
+        std::vector<double> xp, yp(n);
+        while ((p1 < D && D < p2) == false) {
+            if (D >= p2) {
+                db *= 0.98;
+            } else if (D <= p1) {
+                db *= 1.02;
+            }
+            if (db >= x[n - 1]) break;
+            if (!orc_mesh(db, x[n - 1], n - 1, p0, xp)) return -1;
+            for (int i = 0; i < n; ++i) {
+                Param p = get_param(xp[i], x, n);
+                yp[i] = p.wl * y[p.lo] + p.wh * y[p.hi];
+            }
+            D = std::fabs(yp[1] - yp[0]) / jmax(std::fabs(yp[1]), std::fabs(yp[0]));
+        }
+    }
+    // c)
+    *delta = jmin(jmax(num_lin * p3, jmin(da, db)), p4);
+    return 0;
+}
+
+// scan_channel's argmax and cuts, Action/Action.jl:449-476: k = 3*comp + channel
+void orc_scan_cuts(const orc_system *y, const orc_action *a, int32_t *idx, double *cuts) {
+    const Dims d = dims_of(*y);
+    const int ns = d.ns, nO = d.nO, nn = d.nn;
+    const long n3 = (long)ns * nO * nn * nn;
+    for (int k = 0; k < 6; ++k) {
+        const double *q3 = chan(d, a->gamma, k / 3, k % 3) + d.q3;
+        long best = 0;
+        for (long j = 1; j < n3; ++j)
+            if (std::fabs(q3[j]) > std::fabs(q3[best])) best = j;
+        const int i = (int)(best % ns), w = (int)((best / ns) % nO), v = (int)((best / ((long)ns * nO)) % nn),
+                  vp = (int)(best / ((long)ns * nO * nn));
+        idx[4 * k] = i; idx[4 * k + 1] = w; idx[4 * k + 2] = v; idx[4 * k + 3] = vp;
+        auto at = [&](int ww, int x1, int x2) { return q3[i + (long)ns * (ww + (long)nO * (x1 + (long)nn * x2))]; };
+        double *o = cuts + (long)k * (nO + 3 * nn);
+        for (int x = 0; x < nO; ++x) o[x] = at(x, v, vp);
+        for (int x = 0; x < nn; ++x) o[nO + x] = at(w, x, x) - at(w, nn - 1, nn - 1);
+        for (int x = 0; x < nn; ++x) o[nO + nn + x] = at(w, x, vp) - at(w, nn - 1, vp);
+        for (int x = 0; x < nn; ++x) o[nO + 2 * nn + x] = at(w, v, x) - at(w, v, nn - 1);
+    }
+}
+
+// resample_from_to!, Action/channel.jl:416-470, for one channel
+static void resample_channel(const Dims &d, const double *Oo, const double *no, const double *c_old,
+                             const double *On, const double *nnw, double *c_new) {
+    const int ns = d.ns, nO = d.nO, nn = d.nn;
+    for (int w = 0; w < nO; ++w) {
+        Param p1 = get_param(On[w], Oo, nO);
+        for (int s = 0; s < ns; ++s) {
+            const double *q = c_old + d.q1;
+            double val = p1.wl * q[s + ns * p1.lo];
+            val += p1.wh * q[s + ns * p1.hi];
+            c_new[d.q1 + s + ns * w] = val;
+        }
+    }
+    for (int v = 0; v < nn; ++v)
+        for (int w = 0; w < nO; ++w) {
+            Param p1 = get_param(On[w], Oo, nO);
+            Param p2 = get_param(nnw[v], no, nn);
+            for (int s = 0; s < ns; ++s)
+                for (int which = 0; which < 2; ++which) {
+                    const long off = which == 0 ? d.q21 : d.q22;
+                    const double *q = c_old + off;
+                    auto at = [&](int a, int b) { return q[s + (long)ns * (a + (long)nO * b)]; };
+                    double val = p1.wl * p2.wl * at(p1.lo, p2.lo);
+                    val += p1.wh * p2.wl * at(p1.hi, p2.lo);
+                    val += p1.wl * p2.wh * at(p1.lo, p2.hi);
+                    val += p1.wh * p2.wh * at(p1.hi, p2.hi);
+                    c_new[off + s + (long)ns * (w + (long)nO * v)] = val;
+                }
+        }
+    for (int vp = 0; vp < nn; ++vp)
+        for (int v = 0; v < nn; ++v)
+            for (int w = 0; w < nO; ++w) {
+                Param p1 = get_param(On[w], Oo, nO);
+                Param p2 = get_param(nnw[v], no, nn);
+                Param p3 = get_param(nnw[vp], no, nn);
+                const double *q = c_old + d.q3;
+                for (int s = 0; s < ns; ++s) {
+                    const double *qs = q + s;
+                    auto g = [&](int a, int b, int c) { return qs[(long)ns * (a + (long)nO * (b + (long)nn * c))]; };
+                    double val = p1.wl * p2.wl * p3.wl * g(p1.lo, p2.lo, p3.lo);
+                    val += p1.wh * p2.wl * p3.wl * g(p1.hi, p2.lo, p3.lo);
+                    val += p1.wl * p2.wh * p3.wl * g(p1.lo, p2.hi, p3.lo);
+                    val += p1.wh * p2.wh * p3.wl * g(p1.hi, p2.hi, p3.lo);
+                    val += p1.wl * p2.wl * p3.wh * g(p1.lo, p2.lo, p3.hi);
+                    val += p1.wh * p2.wl * p3.wh * g(p1.hi, p2.lo, p3.hi);
+                    val += p1.wl * p2.wh * p3.wh * g(p1.lo, p2.hi, p3.hi);
+                    val += p1.wh * p2.wh * p3.wh * g(p1.hi, p2.hi, p3.hi);
+                    c_new[d.q3 + s + (long)ns * (w + (long)nO * (v + (long)nn * vp))] = val;
+                }
+            }
+}
+
+// resample_from_to, Action/Action.jl:543-553 (Σ by get_Σ) and vertex.jl:148-160 (channels):
+// a_old on the meshes of y_old onto the new meshes into a_new; bare copied
+void orc_resample(const orc_system *y_old, const double *sigma, const double *Os, const double *ns_,
+                  const double *Ot, const double *nt, const orc_action *a_old, orc_action *a_new) {
+    const Dims d = dims_of(*y_old);
+    for (int w = 0; w < y_old->n_sigma; ++w) a_new->Sigma[w] = get_sigma(*y_old, a_old->Sigma, sigma[w]);
+    for (int c = 0; c < 2; ++c) {
+        resample_channel(d, y_old->Os, y_old->ns, chan(d, a_old->gamma, c, 0), Os, ns_, chan(d, a_new->gamma, c, 0));
+        resample_channel(d, y_old->Ot, y_old->nt, chan(d, a_old->gamma, c, 1), Ot, nt, chan(d, a_new->gamma, c, 1));
+        resample_channel(d, y_old->Os, y_old->ns, chan(d, a_old->gamma, c, 2), Os, ns_, chan(d, a_new->gamma, c, 2));
+    }
+    for (int k = 0; k < 2 * d.ns; ++k) a_new->bare[k] = a_old->bare[k];
+}
+
 }  // extern "C"

@@ -24,6 +24,7 @@
 #include "pffrg_chi.h"
 #include "pffrg_device.h"
 #include "pffrg_pass.h"
+#include "pffrg_rescan.h"
 
 using namespace pffrg;
 
@@ -293,6 +294,14 @@ struct Slot {
     bool gat64_valid = false, gat32_valid = false;
 };
 
+// mesh-rescanning scratch: argmax partials (values, positions), 6 x 4 positions, 6 cut sets of
+// n_Omega + 3 n_nu <= 4 MAXN values, the five new meshes
+constexpr size_t SCAN_OFF_PIDX = 6 * SCAN_BLOCKS * 8;
+constexpr size_t SCAN_OFF_IDX = SCAN_OFF_PIDX + 6 * SCAN_BLOCKS * 8;
+constexpr size_t SCAN_OFF_CUTS = SCAN_OFF_IDX + 256;
+constexpr size_t SCAN_OFF_MESH = SCAN_OFF_CUTS + 6 * 4 * MAXN * 8;
+constexpr size_t SCAN_MEM_BYTES = SCAN_OFF_MESH + 5 * MAXN * 8;
+
 struct pffrg_ctx {
     pffrg_dims dims{};
     Sys sys{};
@@ -317,7 +326,6 @@ struct pffrg_ctx {
     double time_ms = 0.0;
     long long time_launches = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::vector<double> h_mesh_sig;
     // recording of per-sub-interval node counts of the next vertex pass (diagnostics)
     int32_t *h_dbg_nodes = nullptr;
     double *h_dbg_res = nullptr;
@@ -326,6 +334,8 @@ struct pffrg_ctx {
     long dbg_cap = 0;
     // correlation cubature scratch (regions of every task), grown on demand
     void *chi_mem = nullptr;
+    // mesh-rescanning scratch (argmax partials, cuts, new meshes), SCAN_MEM_BYTES
+    char *scan_mem = nullptr;
     size_t chi_bytes = 0;
     // host copies of the meshes and the log-domain breakpoint table
     std::vector<double> h_Os, h_Ot;
@@ -637,6 +647,51 @@ int do_scale(pffrg_ctx *c, int s, double fac, bool vertex_only) {
     return 0;
 }
 
+// the device mesh table: five meshes, then the inverse spacings 1/(m[j+1]-m[j]) of each (the
+// reciprocal get_param computes, param.jl:73-76, evaluated once with the same IEEE division)
+int mesh_table(pffrg_ctx *c, const double *sigma, const double *Os, const double *ns, const double *Ot,
+               const double *nt, std::vector<double> &h) {
+    const Sys &y = c->sys;
+    h.assign(10 * MAXN, 0.0);
+    auto put = [&](int k, const double *m, int n) {
+        for (int j = 0; j < n; ++j) h[k * MAXN + j] = m[j];
+        for (int j = 0; j + 1 < n; ++j) h[(5 + k) * MAXN + j] = 1.0 / (m[j + 1] - m[j]);
+    };
+    put(0, sigma, y.nsig);
+    put(1, Os, y.nO);
+    put(2, ns, y.nn);
+    put(3, Ot, y.nO);
+    put(4, nt, y.nn);
+    for (int k = 0; k < 5; ++k) {
+        const int n = (k == 0) ? y.nsig : ((k == 1 || k == 3) ? y.nO : y.nn);
+        if (h[k * MAXN] != 0.0) return fail(c, PFFRG_ERR_ARG, "meshes must start at 0");
+        for (int j = 1; j < n; ++j)
+            if (!(h[k * MAXN + j] > h[k * MAXN + j - 1])) return fail(c, PFFRG_ERR_ARG, "meshes must be strictly increasing");
+    }
+    return 0;
+}
+
+// make a checked mesh table the context's (waits for the work queued on the old meshes)
+int apply_mesh(pffrg_ctx *c, const std::vector<double> &h) {
+    Sys &y = c->sys;
+    y.valS = h[1 * MAXN + y.nO - 1] + h[2 * MAXN + y.nn - 1];
+    y.valT = h[3 * MAXN + y.nO - 1] + h[4 * MAXN + y.nn - 1];
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->h_Os.assign(h.begin() + 1 * MAXN, h.begin() + 1 * MAXN + y.nO);
+    c->h_Ot.assign(h.begin() + 3 * MAXN, h.begin() + 3 * MAXN + y.nO);
+    c->logpts_valid = false;
+    HIPCHK(c, hipMemcpy(c->d_mesh, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->have_mesh = true;
+    return 0;
+}
+
+// small scratch of the rescanning kernels, allocated on first use
+int ensure_scan_mem(pffrg_ctx *c) {
+    if (c->scan_mem) return 0;
+    HIPCHK(c, hipMalloc(&c->scan_mem, SCAN_MEM_BYTES));
+    return 0;
+}
+
 int do_copy(pffrg_ctx *c, int dst, int src, bool vertex_only) {
     HIPCHK(c, hipMemcpyAsync(c->slots[dst].gamma, c->slots[src].gamma, 2 * c->sys.complen * sizeof(double),
                              hipMemcpyDeviceToDevice, c->stream));
@@ -722,6 +777,7 @@ void pffrg_destroy(pffrg_ctx *c) {
         (void)hipFree(s.gat64);
     }
     (void)hipFree(c->chi_mem);
+    (void)hipFree(c->scan_mem);
     (void)hipFree(c->d_mesh);
     (void)hipFree(c->d_exch);
     (void)hipFree(c->d_ov_off);
@@ -748,34 +804,10 @@ int pffrg_set_mesh(pffrg_ctx *c, const double *sigma, const double *Os, const do
                    const double *nt) {
     if (!c || !sigma || !Os || !ns || !Ot || !nt) return fail(c, PFFRG_ERR_ARG, "null mesh pointer");
     HIPCHK(c, hipSetDevice(c->device));
-    Sys &y = c->sys;
-    // five meshes, then the inverse spacings 1/(m[j+1]-m[j]) of each (the reciprocal get_param
-    // computes, param.jl:73-76, evaluated once with the same IEEE division)
-    std::vector<double> h(10 * MAXN, 0.0);
-    auto put = [&](int k, const double *m, int n) {
-        for (int j = 0; j < n; ++j) h[k * MAXN + j] = m[j];
-        for (int j = 0; j + 1 < n; ++j) h[(5 + k) * MAXN + j] = 1.0 / (m[j + 1] - m[j]);
-    };
-    put(0, sigma, y.nsig);
-    put(1, Os, y.nO);
-    put(2, ns, y.nn);
-    put(3, Ot, y.nO);
-    put(4, nt, y.nn);
-    for (int k = 0; k < 5; ++k) {
-        const int n = (k == 0) ? y.nsig : ((k == 1 || k == 3) ? y.nO : y.nn);
-        if (h[k * MAXN] != 0.0) return fail(c, PFFRG_ERR_ARG, "meshes must start at 0");
-        for (int j = 1; j < n; ++j)
-            if (!(h[k * MAXN + j] > h[k * MAXN + j - 1])) return fail(c, PFFRG_ERR_ARG, "meshes must be strictly increasing");
-    }
-    y.valS = Os[y.nO - 1] + ns[y.nn - 1];
-    y.valT = Ot[y.nO - 1] + nt[y.nn - 1];
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->h_Os.assign(Os, Os + y.nO);
-    c->h_Ot.assign(Ot, Ot + y.nO);
-    c->logpts_valid = false;
-    HIPCHK(c, hipMemcpy(c->d_mesh, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
-    c->have_mesh = true;
-    return 0;
+    std::vector<double> h;
+    int rc = mesh_table(c, sigma, Os, ns, Ot, nt, h);
+    if (rc) return rc;
+    return apply_mesh(c, h);
 }
 
 int pffrg_set_lattice(pffrg_ctx *c, const int64_t *exchange, const double *mult, const int64_t *ov_off,
@@ -1356,6 +1388,58 @@ int pffrg_compute_chi(pffrg_ctx *c, double L, int32_t a, const double *chi_mesh,
                 for (int k = 0; k < 9; ++k) nreg[t0 + k] = nr[t0 + k];
         }
     return 0;
+}
+
+
+// scan_channel's argmax and cuts (Action.jl:449-480)
+int pffrg_scan_cuts(pffrg_ctx *c, int32_t a, int32_t *idx, double *cuts) {
+    int rc;
+    if ((rc = ready(c)) || (rc = check_slot(c, a))) return rc;
+    if (!idx || !cuts) return fail(c, PFFRG_ERR_ARG, "null output pointer");
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((rc = ensure_scan_mem(c))) return rc;
+    ScanArgs A;
+    A.y = c->sys;
+    A.gamma = c->slots[a].gamma;
+    A.pval = (double *)c->scan_mem;
+    A.pidx = (long long *)(c->scan_mem + SCAN_OFF_PIDX);
+    A.idx = (int *)(c->scan_mem + SCAN_OFF_IDX);
+    A.cuts = (double *)(c->scan_mem + SCAN_OFF_CUTS);
+    HIPCHK(c, launch_scan_cuts(A, c->stream));
+    const int nc = c->sys.nO + 3 * c->sys.nn;
+    HIPCHK(c, hipMemcpyAsync(idx, A.idx, 24 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cuts, A.cuts, 6 * (size_t)nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// resample_from_to (Action.jl:531-556): slot src onto new meshes into slot dst
+int pffrg_resample(pffrg_ctx *c, int32_t src, int32_t dst, const double *sigma, const double *Os, const double *ns,
+                   const double *Ot, const double *nt) {
+    int rc;
+    if ((rc = ready(c)) || (rc = check_slot(c, src)) || (rc = check_slot(c, dst))) return rc;
+    if (src == dst) return fail(c, PFFRG_ERR_ARG, "resample needs two different slots");
+    if (!sigma || !Os || !ns || !Ot || !nt) return fail(c, PFFRG_ERR_ARG, "null mesh pointer");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<double> h;
+    if ((rc = mesh_table(c, sigma, Os, ns, Ot, nt, h))) return rc;
+    if ((rc = ensure_scan_mem(c))) return rc;
+    double *mnew = (double *)(c->scan_mem + SCAN_OFF_MESH);
+    HIPCHK(c, hipMemcpyAsync(mnew, h.data(), 5 * MAXN * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    ResampleArgs A;
+    A.y = c->sys;
+    A.mold = c->d_mesh;
+    A.mnew = mnew;
+    A.src_gamma = c->slots[src].gamma;
+    A.src_sigma = c->slots[src].Sigma;
+    A.dst_gamma = c->slots[dst].gamma;
+    A.dst_sigma = c->slots[dst].Sigma;
+    HIPCHK(c, launch_resample(A, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->slots[dst].bare, c->slots[src].bare, 2 * c->sys.ns * sizeof(double),
+                             hipMemcpyDeviceToDevice, c->stream));
+    invalidate(c, dst);
+    // the host table is released only after the copy from it has completed (apply_mesh waits)
+    return apply_mesh(c, h);
 }
 
 }  // extern "C"

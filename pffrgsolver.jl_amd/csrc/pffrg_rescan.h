@@ -1,0 +1,34 @@
+// pffrg_rescan.h -- mesh rescanning kernels (resample_from_to, Action.jl:483-557): the
+// arguments shared by the host and rescan.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pffrg_device.h"
+
+namespace pffrg {
+
+constexpr int SCAN_BLOCKS = 64;  // partial-argmax blocks per (component, channel)
+
+struct ScanArgs {
+    Sys y;
+    const double *gamma;  // the slot's vertex (reference layout)
+    double *pval;         // [6][SCAN_BLOCKS] partial maxima of |q3|
+    long long *pidx;      // [6][SCAN_BLOCKS] their column-major positions
+    int *idx;             // [6][4] (site, w, v, v') of the first maximum
+    double *cuts;         // [6][nO + 3 nn]
+};
+
+struct ResampleArgs {
+    Sys y;
+    const double *mold;  // old meshes σ | Ωs | νs | Ωt | νt, then their inverse spacings, MAXN each
+    const double *mnew;  // new meshes σ | Ωs | νs | Ωt | νt, MAXN each
+    const double *src_gamma, *src_sigma;
+    double *dst_gamma, *dst_sigma;
+};
+
+hipError_t launch_scan_cuts(const ScanArgs &A, hipStream_t stream);
+hipError_t launch_resample(const ResampleArgs &A, hipStream_t stream);
+
+}  // namespace pffrg

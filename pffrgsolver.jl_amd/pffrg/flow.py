@@ -3,12 +3,13 @@ Bogacki-Shampine flow loop (launcher_1l.jl / launcher_2l.jl / launcher_ml.jl) an
 the parquet iteration (parquet.jl), all driving the C ABI through native.Context.
 
 Step control, the embedded-error norm and the parquet residual stay in FP64
-(pffrg_abs_max reads FP64 device data).  The frequency meshes stay fixed during a
-flow: the reference's per-step mesh rescanning (Action.jl:369-557) is not part of
-this path.
+(pffrg_abs_max reads FP64 device data).  After every accepted step the flow measures χ
+and moves the action onto rescanned frequency meshes (resample_from_to, Action.jl:483-557),
+as the reference's launchers do.
 """
 from __future__ import annotations
 
+import sys
 import time
 from dataclasses import dataclass, field
 
@@ -17,7 +18,7 @@ import numpy as np
 from . import lattice as LT
 from . import mesh as M
 from .action import Action
-from .native import FP64, FP32_MIXED, Batch, Context
+from .native import FP64, FP32_MIXED, Batch, Context, scan
 
 # slot roles
 A, DA, A_STAGE, A_INTER, A_ERR, DA_L, DA_C, DA_TEMP, DA_SIGMA = range(9)
@@ -73,6 +74,86 @@ def rhs(ctx: Context, L: float, loops: int, a: int, da: int, eval_: int, Gtol=(1
             ctx.compute_dsigma_corr(L, a, da, da_sigma, Stol)
 
 
+@dataclass(frozen=True)
+class MeshParams:
+    """The mesh keywords of launch! (Launcher.jl:580-587)."""
+    p_sigma: tuple = M.P_SIGMA
+    p_Os: tuple = M.P_OMEGA_S
+    p_ns: tuple = M.P_NU_S
+    p_Ot: tuple = M.P_OMEGA_T
+    p_nt: tuple = M.P_NU_T
+    p_chi: tuple = M.P_CHI
+    lins: tuple = M.LINS
+    bounds: tuple = M.BOUNDS
+
+
+def _jl_min(*xs):
+    """Julia's min over floats: NaN wins."""
+    out = xs[0]
+    for x in xs[1:]:
+        if x != x or out != out:
+            return float("nan")
+        out = x if x < out else out
+    return out
+
+
+def rescanned_mesh(ctx, L: float, a: int, chi=None, mp: MeshParams = MeshParams()) -> M.Mesh:
+    """The meshes resample_from_to builds at Λ (Action.jl:483-541): linear extents proportional
+    to Λ above lins[0], below it from scans of the self-energy of slot a, of the cuts through
+    the maxima of its channels (scan_channel, Action.jl:449-480) and of χ[site, :] (the site of
+    the largest |χ|).  χ is needed below lins[0]."""
+    m_old = ctx.m
+    lins, bounds = mp.lins, mp.bounds
+    sig_lin, Os_lin, ns_lin = lins[1] * L, lins[2] * L, lins[3] * L
+    Ot_lin, nt_lin, chi_lin = lins[2] * L, lins[3] * L, lins[4] * L
+    if L < lins[0]:
+        sig = ctx.get_sigma(a)
+        sig_lin = mp.p_sigma[1] * m_old.sigma[int(np.argmax(np.abs(sig)))]
+        _, cuts = ctx.scan_cuts(a)
+        nO, nn = m_old.num_Omega, m_old.num_nu
+
+        def scan_channel(k, pO, pn, Om, nu):
+            c = cuts[k]
+            O_lin = scan(Om, c[:nO], (pO[0], pO[1], pO[2], pO[3] * L, pO[4] * L))
+            pv = (pn[0], pn[1], pn[2], pn[3] * L, pn[4] * L)
+            v_lin = _jl_min(*[scan(nu, c[nO + j * nn: nO + (j + 1) * nn], pv) for j in range(3)])
+            return O_lin, v_lin
+
+        Os_l, ns_l, Ot_l, nt_l = [], [], [], []
+        for comp in range(2):
+            Os_s, ns_s = scan_channel(3 * comp + 0, mp.p_Os, mp.p_ns, m_old.Os, m_old.ns)
+            Os_u, ns_u = scan_channel(3 * comp + 2, mp.p_Os, mp.p_ns, m_old.Os, m_old.ns)
+            Os_l.append(_jl_min(Os_s, Os_u))
+            ns_l.append(_jl_min(ns_s, ns_u))
+            Ot_c, nt_c = scan_channel(3 * comp + 1, mp.p_Ot, mp.p_nt, m_old.Ot, m_old.nt)
+            Ot_l.append(Ot_c)
+            nt_l.append(nt_c)
+        Os_lin, ns_lin = _jl_min(*Os_l), _jl_min(*ns_l)
+        Ot_lin, nt_lin = _jl_min(*Ot_l), _jl_min(*nt_l)
+        if chi is None:
+            raise ValueError("rescanning below lins[0] needs the correlations χ")
+        chi = np.asarray(chi)
+        site = int(np.argmax(np.abs(chi).T)) % chi.shape[0]  # first maximum in column-major order
+        pc = mp.p_chi
+        chi_lin = scan(m_old.chi, chi[site, :], (pc[0], pc[1], pc[2], pc[3] * L, pc[4] * L))
+    up = max(L, bounds[0])
+    return M.Mesh(
+        M.get_mesh(sig_lin, bounds[1] * up, m_old.num_sigma - 1, mp.p_sigma[0]) * mp.p_sigma[2],
+        M.get_mesh(Os_lin, bounds[2] * up, m_old.num_Omega - 1, mp.p_Os[0]) * mp.p_Os[5],
+        M.get_mesh(ns_lin, bounds[3] * up, m_old.num_nu - 1, mp.p_ns[0]) * mp.p_ns[5],
+        M.get_mesh(Ot_lin, bounds[2] * up, m_old.num_Omega - 1, mp.p_Ot[0]) * mp.p_Ot[5],
+        M.get_mesh(nt_lin, bounds[3] * up, m_old.num_nu - 1, mp.p_nt[0]) * mp.p_nt[5],
+        M.get_mesh(chi_lin, bounds[4] * up, m_old.num_chi - 1, mp.p_chi[0]) * mp.p_chi[5])
+
+
+def resample_from_to(ctx, L: float, src: int, dst: int, chi=None, mp: MeshParams = MeshParams()) -> M.Mesh:
+    """resample_from_to (Action.jl:483-557): new meshes at Λ (rescanned_mesh), then slot src
+    interpolated onto them into slot dst; the context continues on the new meshes."""
+    m_new = rescanned_mesh(ctx, L, src, chi, mp)
+    ctx.resample(src, dst, m_new)
+    return m_new
+
+
 @dataclass
 class FlowResult:
     Lambdas: list = field(default_factory=list)
@@ -84,6 +165,7 @@ class FlowResult:
     status: str = ""
     chi: list = field(default_factory=list)   # χ[site, ω] after every accepted step (if measured)
     chi_s: float = 0.0                        # wall time spent on χ
+    meshes: list = field(default_factory=list)  # meshes after every rescanned step
 
 
 class BS3Flow:
@@ -95,13 +177,19 @@ class BS3Flow:
 
     def __init__(self, ctx: Context, prob: Problem, a0: Action, Li: float, Lf: float, loops: int = 1, sigma_corr=True,
                  bmin=1e-4, bmax=0.1, min_eval=10, max_eval=100, Stol=(1e-8, 1e-4), Gtol=(1e-8, 1e-4),
-                 ODE_tol=(1e-8, 1e-2), max_steps: int | None = None, verbose=False, chi_tol=None):
+                 ODE_tol=(1e-8, 1e-2), max_steps: int | None = None, verbose=False, chi_tol=(1e-8, 1e-4),
+                 rescan: bool = True, mesh_params: MeshParams = MeshParams(), max_seconds: float | None = None):
         self.ctx, self.prob, self.Lf, self.loops, self.sigma_corr = ctx, prob, Lf, loops, sigma_corr
-        self.chi_tol = chi_tol
+        self.max_seconds = max_seconds
+        if rescan and chi_tol is None:
+            raise ValueError("mesh rescanning needs the χ measurement (chi_tol)")
+        self.chi_tol, self.rescan, self.mesh_params = chi_tol, rescan, mesh_params
         self.bmin, self.bmax, self.min_eval, self.max_eval = bmin, bmax, min_eval, max_eval
         self.Stol, self.Gtol, self.ODE_tol, self.max_steps, self.verbose = Stol, Gtol, ODE_tol, max_steps, verbose
         self.res = FlowResult()
         self.t0 = time.time()
+        if ctx.m is not prob.mesh:
+            ctx.set_mesh(prob.mesh)  # a0 lives on the problem's initial meshes
         ctx.upload(A, a0)
         # da, a_stage, a_err start empty; a_inter carries the bare couplings (launcher_1l.jl:39-44)
         zero = Action.empty(prob.r.n_sites, prob.mesh, prob.S)
@@ -126,6 +214,10 @@ class BS3Flow:
             return False
         if self.max_steps is not None and self.steps >= self.max_steps:
             self.res.status = "max_steps"
+            self.done = True
+            return False
+        if self.max_seconds is not None and time.time() - self.t0 > self.max_seconds:
+            self.res.status = "time_budget"
             self.done = True
             return False
         self.steps += 1
@@ -176,7 +268,8 @@ class BS3Flow:
         scale = self.ODE_tol[0] + max(gmax, ctx.abs_max(A)) * self.ODE_tol[1]
         err = delta / scale
         if self.verbose:
-            print(f"L={self.L:.6g} dL={self.dL:.3g} eval={self.ev} err={err:.3g} Gmax={gmax:.4g}", flush=True)
+            print(f"L={self.L:.6g} dL={self.dL:.3g} eval={self.ev} err={err:.3g} Gmax={gmax:.4g} "
+                  f"t={time.time() - self.t0:.1f}s", file=sys.stderr, flush=True)
         if err >= 10.0:
             res.status = "unstable"
             self.done = True
@@ -192,26 +285,34 @@ class BS3Flow:
             res.accepted += 1
             res.Lambdas.append(self.L)
             res.gamma_max.append(gmax)
-            ctx.replace_with(A, A_INTER)
+            # divergence, measurement, monotonicity, then new meshes (launcher_1l.jl:140-178)
+            stop = None
             if gmax >= 1000.0:
-                res.status = "diverged"
+                stop = "diverged"
+            if self.chi_tol is not None and self.measure(A_INTER) and stop is None:
+                stop = "non_monotone"
+            if stop is not None:
+                ctx.replace_with(A, A_INTER)
+                res.status = stop
                 self.done = True
                 return
-            if self.chi_tol is not None and self.measure():
-                res.status = "non_monotone"
-                self.done = True
-                return
+            if self.rescan:
+                resample_from_to(ctx, self.L, A_INTER, A, res.chi[-1], self.mesh_params)
+                res.meshes.append(ctx.m)
+            else:
+                ctx.replace_with(A, A_INTER)
         else:
             res.rejected += 1
         if not (self.L > self.Lf):
             res.status = "finished"
             self.done = True
 
-    def measure(self) -> bool:
-        """compute_χ! of the accepted action (measure, Launcher.jl:2-40); True when the static
-        on-site correlation stopped growing (χ/χ_prev < 0.995, the reference's stop rule)."""
+    def measure(self, slot: int = A) -> bool:
+        """compute_χ! of the accepted action (measure, Launcher.jl:2-40) on the current χ mesh; True
+        when the static on-site correlation stopped growing (χ/χ_prev < 0.995, the reference's
+        stop rule)."""
         t = time.time()
-        chi = self.ctx.compute_chi(self.L, A, self.chi_tol)
+        chi = self.ctx.compute_chi(self.L, slot, self.chi_tol)
         self.res.chi_s += time.time() - t
         prev = self.res.chi[-1] if self.res.chi else None
         self.res.chi.append(chi)
@@ -227,12 +328,15 @@ class BS3Flow:
 
 def launch_flow(ctx: Context, prob: Problem, a0: Action, Li: float, Lf: float, loops: int = 1, sigma_corr=True,
                 bmin=1e-4, bmax=0.1, min_eval=10, max_eval=100, Stol=(1e-8, 1e-4), Gtol=(1e-8, 1e-4),
-                ODE_tol=(1e-8, 1e-2), max_steps: int | None = None, verbose=False, chi_tol=None) -> FlowResult:
-    """BS3 flow from Li to Lf (launcher_1l.jl:72-198 and its 2l/ml twins).  Slot A holds
-    the flowing action on return.  With chi_tol, χ is measured after every accepted step and
-    the flow stops when the static on-site correlation turns (launcher_1l.jl:160-178)."""
+                ODE_tol=(1e-8, 1e-2), max_steps: int | None = None, verbose=False, chi_tol=(1e-8, 1e-4),
+                rescan: bool = True, mesh_params: MeshParams = MeshParams(),
+                max_seconds: float | None = None) -> FlowResult:
+    """BS3 flow from Li to Lf (launcher_1l.jl:72-198 and its 2l/ml twins).  Slot A holds the
+    flowing action on return, on the context's current meshes.  After every accepted step χ is
+    measured (chi_tol; None skips it), the flow stops when the static on-site correlation turns
+    (launcher_1l.jl:160-172), and the action moves onto rescanned meshes (rescan)."""
     f = BS3Flow(ctx, prob, a0, Li, Lf, loops, sigma_corr, bmin, bmax, min_eval, max_eval, Stol, Gtol, ODE_tol,
-                max_steps, verbose, chi_tol)
+                max_steps, verbose, chi_tol, rescan, mesh_params, max_seconds)
     while f.begin_step():
         for k in range(4):
             f.before(k)

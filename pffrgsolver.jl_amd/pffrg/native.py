@@ -70,6 +70,9 @@ _SIGS = {
     "pffrg_set_timing": ([_P, _i], C.c_int),
     "pffrg_get_timing": ([_P, _D, _I64], C.c_int),
     "pffrg_compute_chi": ([_P, _d, _i, _D, _i, _D, _D, C.POINTER(C.c_int32)], C.c_int),
+    "pffrg_scan_cuts": ([_P, _i, C.POINTER(C.c_int32), _D], C.c_int),
+    "pffrg_scan": ([_D, _D, _i, _D, _D], C.c_int),
+    "pffrg_resample": ([_P, _i, _i, _D, _D, _D, _D, _D], C.c_int),
     "pffrg_mlp_create": ([_i, _i, _i, _i, C.c_uint64, _i, C.POINTER(_P)], C.c_int),
     "pffrg_mlp_destroy": ([_P], None),
     "pffrg_mlp_last_error": ([_P], C.c_char_p),
@@ -104,6 +107,21 @@ def load() -> C.CDLL:
             fn.restype = res
         _lib = lib
     return _lib
+
+
+def scan(x, y, p) -> float:
+    """scan (Action.jl:369-446) on the host: the new linear extent of mesh x for the cut y, with
+    p = (p0, p1, p2, p3*Λ, p4*Λ)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    pp = np.ascontiguousarray(p, dtype=np.float64)
+    if len(x) != len(y) or len(pp) != 5:
+        raise PffrgError("scan: x, y must have equal length and p five entries")
+    d = np.zeros(1)
+    rc = load().pffrg_scan(_dp(x), _dp(y), len(x), _dp(pp), _dp(d))
+    if rc != 0:
+        raise PffrgError(f"pffrg_scan failed ({rc})")
+    return float(d[0])
 
 
 def exported_symbols() -> list[str]:
@@ -240,6 +258,26 @@ class Context:
                                             nreg.ctypes.data_as(C.POINTER(C.c_int32))), "compute_chi")
         chi = out.reshape(len(cm), self.ns).T.copy()
         return (chi, nreg) if with_regions else chi
+
+    def scan_cuts(self, a):
+        """scan_channel's argmax positions idx[3*comp + ch] = (site, w, v, v') and the cuts through
+        them, cuts[3*comp + ch] = (Ω cut, three ν cuts) (Action.jl:449-476)."""
+        idx = np.zeros(24, dtype=np.int32)
+        cuts = np.zeros(6 * (self.m.num_Omega + 3 * self.m.num_nu))
+        self._ck(self.lib.pffrg_scan_cuts(self.h, a, idx.ctypes.data_as(C.POINTER(C.c_int32)), _dp(cuts)),
+                 "scan_cuts")
+        return idx.reshape(6, 4), cuts.reshape(6, -1)
+
+    def resample(self, src, dst, m_new):
+        """resample_from_to's interpolation (Action.jl:543-553): slot src onto the meshes m_new into
+        slot dst; m_new becomes the context's mesh (its χ mesh included)."""
+        arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (m_new.sigma, m_new.Os, m_new.ns, m_new.Ot,
+                                                                    m_new.nt)]
+        if [len(x) for x in arrs] != [self.m.num_sigma, self.m.num_Omega, self.m.num_nu, self.m.num_Omega,
+                                      self.m.num_nu]:
+            raise PffrgError("resample: new meshes must keep the point counts")
+        self._ck(self.lib.pffrg_resample(self.h, src, dst, *[_dp(x) for x in arrs]), "resample")
+        self.m = m_new
 
     def symmetrize(self, s):
         self._ck(self.lib.pffrg_symmetrize(self.h, s), "symmetrize")

@@ -11,11 +11,28 @@ from pffrg.action import Action
 
 class OracleContext:
     def __init__(self, prob, n_slots: int = 9):
+        self.r = prob.r
         self.orc = O.Oracle(prob.r, prob.mesh)
         self.m = prob.mesh
         self.ns = prob.r.n_sites
         self.S = prob.S
         self.slots = [Action.empty(self.ns, self.m, self.S) for _ in range(n_slots)]
+
+    def set_mesh(self, m):
+        self.orc = O.Oracle(self.r, m, self.S)
+        self.m = m
+
+    def get_sigma(self, s):
+        return self.slots[s].Sigma.copy()
+
+    def scan_cuts(self, a):
+        return self.orc.scan_cuts(self.slots[a])
+
+    def resample(self, src, dst, m_new):
+        out = Action.empty(self.ns, m_new, self.S)
+        self.orc.resample(self.slots[src], m_new, out)
+        self.slots[dst] = out
+        self.set_mesh(m_new)
 
     def upload(self, s, a):
         self.slots[s] = a.copy()

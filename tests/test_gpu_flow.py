@@ -1,5 +1,6 @@
-"""Host flow drivers on the GPU: a BS3 flow (launch_flow, launcher_1l.jl:72-198) against the
-same driver run on the FP64 oracle, and batched flows (launch_flow_batch) against single ones.
+"""Host flow drivers on the GPU: a BS3 flow (launch_flow, launcher_1l.jl:72-198) with χ
+measurement and mesh rescanning after every accepted step against the same driver run on the
+FP64 oracle, and batched flows (launch_flow_batch) against single ones.
 
 Tolerance: the GPU right-hand sides agree with the oracle's to ~1e-15 and the stage algebra
 differs only by FMA rounding, so the accepted Λ sequence is identical and the final actions
@@ -36,6 +37,12 @@ def test_flow_matches_oracle_flow():
     ec = max(rel_err(c, r) for c, r in zip(res.chi, ref_res.chi))
     print(f"final action rel err {e:.3e}; chi along the flow {ec:.3e}", flush=True)
     assert e < 1e-10 and ec < 1e-10 and len(res.chi) == res.accepted
+    # rescanned meshes after every accepted step (resample_from_to, Action.jl:483-557)
+    assert len(res.meshes) == len(ref_res.meshes) == res.accepted
+    for m, rm in zip(res.meshes, ref_res.meshes):
+        for f in ("sigma", "Os", "ns", "Ot", "nt", "chi"):
+            np.testing.assert_allclose(getattr(m, f), getattr(rm, f), rtol=1e-12)
+    assert not np.array_equal(res.meshes[-1].Os, prob.mesh.Os)
 
 
 def test_batched_flows_equal_single_flows():
