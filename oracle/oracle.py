@@ -107,6 +107,10 @@ def lib() -> C.CDLL:
         L.orc_test_hcubature.argtypes = [C.c_int, C.c_int, C.c_int, _D, _D, C.c_double, C.c_double,
                                          C.POINTER(C.c_int32)]
         L.orc_test_hcubature.restype = C.c_double
+        L.orc_pf_expm1.argtypes = [C.c_double]
+        L.orc_pf_expm1.restype = C.c_double
+        L.orc_pf_exp.argtypes = [C.c_double]
+        L.orc_pf_exp.restype = C.c_double
         L.orc_scan.argtypes = [_D, _D, C.c_int, _D, _D]
         L.orc_scan.restype = C.c_int
         L.orc_scan_cuts.argtypes = [_S, _A, C.POINTER(C.c_int32), _D]

@@ -22,6 +22,8 @@
 #include <cstdint>
 #include <cstring>
 #include <vector>
+
+#include "pffrg_math.h"  // exp/expm1 with the device's exact bits (pffrgsolver.jl_amd/csrc)
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -265,7 +267,7 @@ inline double get_sigma(const orc_system &y, const double *Sig, double w) {
 // get_G_bare, Flow/bubbles.jl:2-14
 inline double G_bare(double L, double w) {
     double val = 0.0;
-    if (std::fabs(w) > 1e-8) val = -std::expm1(-(w * w) / (L * L)) / w;
+    if (std::fabs(w) > 1e-8) val = -pffrg::pf_expm1(-(w * w) / (L * L)) / w;
     return val;
 }
 
@@ -290,7 +292,7 @@ inline double S_single(const orc_system &y, double L, double w, const double *Si
         double G = G_full(y, L, w, Sig);
         double G0 = G_bare(L, w);
         double r = G / G0;
-        val = r * r * std::exp(-(w * w) / (L * L)) * 2.0 * w / (L * L * L);
+        val = r * r * pffrg::pf_exp(-(w * w) / (L * L)) * 2.0 * w / (L * L * L);
     }
     return val;
 }
@@ -820,16 +822,19 @@ void compute_corrs(const orc_system &y, double L, const orc_action &a, const orc
 // ---------------------------------------------------------------- adaptive 2-D h-cubature
 // Restatement of the h-adaptive cubature of Cubature.jl 1.5 (a wrapper of S. G. Johnson's C
 // library "cubature", not part of the reference sources) as called by integrate_χ_boxes
-// (correlation_lib.jl:2-30): the degree-7 Genz-Malik rule with its embedded degree-5 rule for
-// the error estimate (A. C. Genz, A. A. Malik, J. Comput. Appl. Math. 6 (1980) 295-302), and
-// repeated bisection of the region of largest error along its axis of largest fourth
-// difference until the total error meets max(atol, rtol |I|).  The reference calls the
-// vectorised entry hcubature_v, whose schedule subdivides several regions per round; this
-// restatement splits one region per step (the scalar hcubature schedule), which converges to
-// the same tolerance with fewer evaluations.  Region order (split region in place, upper half
-// appended), ties (lowest index) and the summation order of the totals (64 strided partial
-// sums combined by an xor butterfly) are fixed here and shared with the GPU kernel, so both
-// produce the same regions and sums.
+// (correlation_lib.jl:2-30): hcubature_v with maxevals = 10^7.  Each region is integrated by
+// the degree-7 Genz-Malik rule with its embedded degree-5 rule as the error estimate (A. C.
+// Genz, A. A. Malik, J. Comput. Appl. Math. 6 (1980) 295-302) and bisected along its axis of
+// largest fourth difference.  hcubature_v runs the library's vectorised schedule (I. Gladwell's,
+// as described by J. M. Bull and T. L. Freeman, 1994): every round splits at once the
+// largest-error regions that must be refined for the error of the others to meet
+// max(atol, rtol |I|), until the total error meets it or the evaluation budget is spent.
+// Fixed here, and shared with the GPU kernel so both produce the same regions and sums:
+//   - the selection threshold is rounded down to 1/16 octave (the top 16 bits of the IEEE
+//     double of the error), so a round splits every region within that bucket;
+//   - split regions keep their index, the upper halves are appended in index order;
+//   - totals are summed in the order of a 256-thread block (thread t adds indices t, t+256, ...,
+//     then p[t] += p[t+s] for s = 128, 64, ..., 1).
 struct Region {
     double c[2], h[2], val, err;
     int split;
@@ -873,23 +878,28 @@ inline bool cub_converged(double val, double err, double atol, double rtol) {
     return err <= atol || err <= std::fabs(val) * rtol;
 }
 
-const int CUB_MAX_REGIONS = 8192;
+const long CUB_MAX_EVALS = 10000000;                          // maxevals of integrate_χ_boxes
+const long CUB_MAX_REGIONS = 1 + (CUB_MAX_EVALS - 17) / 34;   // regions that budget allows
+const int CUB_T = 256;
 
-// Σ_j x[j] over j < n: lane l of 64 adds x[l], x[l+64], ... in order, then the 64 partials are
-// combined by the butterfly p[l] += p[l ^ m] for m = 32, 16, ..., 1 (the GPU reduction order)
+// the selection key: the top 16 bits of a non-negative double (sign, exponent, 4 mantissa bits)
+inline unsigned err_key(double e) {
+    uint64_t b;
+    std::memcpy(&b, &e, 8);
+    return (unsigned)(b >> 48);
+}
+
+// Σ_j x[j] over j < n in the GPU block's order
 template <class G>
-double lane_sum(int n, const G &x) {
-    double p[64];
-    for (int l = 0; l < 64; ++l) {
+double block_sum(long n, const G &x) {
+    double p[CUB_T];
+    for (int t = 0; t < CUB_T; ++t) {
         double a = 0.0;
-        for (int j = l; j < n; j += 64) a += x(j);
-        p[l] = a;
+        for (long j = t; j < n; j += CUB_T) a += x(j);
+        p[t] = a;
     }
-    for (int m = 32; m >= 1; m >>= 1) {
-        double q[64];
-        for (int l = 0; l < 64; ++l) q[l] = p[l] + p[l ^ m];
-        for (int l = 0; l < 64; ++l) p[l] = q[l];
-    }
+    for (int s = CUB_T / 2; s >= 1; s >>= 1)
+        for (int t = 0; t < s; ++t) p[t] += p[t + s];
     return p[0];
 }
 
@@ -897,7 +907,6 @@ double lane_sum(int n, const G &x) {
 template <class F>
 double hcubature2(const F &f, const double *lo, const double *hi, double atol, double rtol, int *nreg = nullptr) {
     std::vector<Region> R;
-    R.reserve(CUB_MAX_REGIONS);
     auto eval = [&](Region &r) {
         double fv[17];
         for (int k = 0; k < 17; ++k) {
@@ -915,24 +924,40 @@ double hcubature2(const F &f, const double *lo, const double *hi, double atol, d
     eval(r0);
     R.push_back(r0);
     double tv = r0.val;
+    std::vector<long> sel;
     while (true) {
-        const int n = (int)R.size();
-        tv = lane_sum(n, [&](int j) { return R[j].val; });
-        const double te = lane_sum(n, [&](int j) { return R[j].err; });
+        const long n = (long)R.size();
+        tv = block_sum(n, [&](long j) { return R[j].val; });
+        const double te = block_sum(n, [&](long j) { return R[j].err; });
         if (cub_converged(tv, te, atol, rtol) || n >= CUB_MAX_REGIONS) break;
-        int k = 0;
-        for (int j = 1; j < n; ++j)
-            if (R[j].err > R[k].err) k = j;
-        Region a = R[k], b = R[k];
-        const int d = R[k].split;
-        a.h[d] = 0.5 * R[k].h[d];
-        b.h[d] = a.h[d];
-        a.c[d] = R[k].c[d] - a.h[d];
-        b.c[d] = R[k].c[d] + a.h[d];
-        eval(a);
-        eval(b);
-        R[k] = a;
-        R.push_back(b);
+        // the largest key K whose regions {err_key >= K} carry enough error (K = 0 takes all)
+        unsigned klo = 0, khi = 65536;
+        while (khi - klo > 1) {
+            const unsigned mid = (klo + khi) / 2;
+            const double S = block_sum(n, [&](long j) { return err_key(R[j].err) >= mid ? R[j].err : 0.0; });
+            if (cub_converged(tv, te - S, atol, rtol))
+                klo = mid;
+            else
+                khi = mid;
+        }
+        sel.clear();
+        const long room = CUB_MAX_REGIONS - n;
+        for (long j = 0; j < n && (long)sel.size() < room; ++j)
+            if (err_key(R[j].err) >= klo) sel.push_back(j);
+        R.resize(n + sel.size());
+        for (size_t q = 0; q < sel.size(); ++q) {
+            const Region r = R[sel[q]];
+            Region a = r, b = r;
+            const int d = r.split;
+            a.h[d] = 0.5 * r.h[d];
+            b.h[d] = a.h[d];
+            a.c[d] = r.c[d] - a.h[d];
+            b.c[d] = r.c[d] + a.h[d];
+            eval(a);
+            eval(b);
+            R[sel[q]] = a;
+            R[n + q] = b;
+        }
     }
     if (nreg) *nreg = (int)R.size();
     return tv;
@@ -1486,5 +1511,9 @@ void orc_resample(const orc_system *y_old, const double *sigma, const double *Os
     }
     for (int k = 0; k < 2 * d.ns; ++k) a_new->bare[k] = a_old->bare[k];
 }
+
+// the shared written-out exp / expm1 (pffrg_math.h), for their accuracy test
+double orc_pf_expm1(double x) { return pffrg::pf_expm1(x); }
+double orc_pf_exp(double x) { return pffrg::pf_exp(x); }
 
 }  // extern "C"

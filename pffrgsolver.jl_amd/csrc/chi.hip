@@ -1,11 +1,13 @@
 // chi.hip -- real-space correlations χ_ij(ω) (compute_χ!, correlation_su2.jl:58-86).
 //
-// One wavefront per (χ frequency, site, box) task runs the adaptive 2-D cubature of
+// One 256-thread block per (χ frequency, site, box) task runs the adaptive 2-D cubature of
 // integrate_χ_boxes (correlation_lib.jl:2-30) on the kernel of compute_χ_kernel!
-// (correlation_su2.jl:14-55): each step evaluates the 2 x 17 Genz-Malik points of the two
-// halves of the worst region on 34 lanes.  Regions live in global memory (up to
-// CHI_MAX_REGIONS per task); the totals and the worst region are found with strided per-lane
-// scans and an xor butterfly, in the order the oracle's restatement uses.  This file is
+// (correlation_su2.jl:14-55) with the vectorised schedule of Cubature's hcubature_v: every
+// round splits at once the largest-error regions that must be refined for the error of the
+// others to meet the tolerance, each thread evaluating the 2 x 17 Genz-Malik points of the
+// halves of its regions.  Regions live in global memory; the totals are block sums in a fixed
+// order, the selection threshold is found by bisection over the top 16 bits of the error, and
+// new regions are appended in index order, all as in the oracle's restatement.  This file is
 // compiled without FMA contraction so that its arithmetic, and with it every adaptive
 // decision, is the oracle's.
 #include "pffrg_chi.h"
@@ -94,7 +96,7 @@ __device__ __forceinline__ void chi_box(double width, int k, double *lo, double 
     hi[1] = (ky == 0) ? 1.0 : (ky == 1) ? b : a;
 }
 
-__device__ __forceinline__ void store_reg(const ChiArgs &Q, long base, int j, const Reg &R) {
+__device__ __forceinline__ void store_reg(const ChiArgs &Q, long base, long j, const Reg &R) {
     Q.rc0[base + j] = R.c0;
     Q.rc1[base + j] = R.c1;
     Q.rh0[base + j] = R.h0;
@@ -104,107 +106,157 @@ __device__ __forceinline__ void store_reg(const ChiArgs &Q, long base, int j, co
     Q.rsplit[base + j] = R.split;
 }
 
-__global__ __launch_bounds__(64) void k_chi(ChiArgs Q) {
+// the selection key: the top 16 bits of a non-negative double
+__device__ __forceinline__ unsigned err_key(double e) {
+    return (unsigned)((unsigned long long)__double_as_longlong(e) >> 48);
+}
+
+// Σ_j x(j) over j < n: thread t adds j = t, t + CHI_T, ..., then a tree over the threads
+template <class G>
+__device__ double block_sum(long n, const G &x, double *s_red) {
+    const int t = threadIdx.x;
+    double a = 0.0;
+    for (long j = t; j < n; j += CHI_T) a += x(j);
+    s_red[t] = a;
+    __syncthreads();
+    for (int s = CHI_T / 2; s >= 1; s >>= 1) {
+        if (t < s) s_red[t] += s_red[t + s];
+        __syncthreads();
+    }
+    const double r = s_red[0];
+    __syncthreads();
+    return r;
+}
+
+// exclusive prefix of v over the block's threads (and the total in *total)
+__device__ long block_scan(long v, long *s_scan, long *total) {
+    const int t = threadIdx.x;
+    s_scan[t] = v;
+    __syncthreads();
+    for (int s = 1; s < CHI_T; s <<= 1) {
+        const long o = t >= s ? s_scan[t - s] : 0;
+        __syncthreads();
+        s_scan[t] += o;
+        __syncthreads();
+    }
+    const long incl = s_scan[t];
+    *total = s_scan[CHI_T - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__device__ __forceinline__ void eval_region(const ChiArgs &Q, const Meshes<> &M, const double *sA, const int *exch,
+                                            int site, double W, Reg &R) {
+    double f[17];
+    for (int k = 0; k < 17; ++k) {
+        double xx, yy;
+        gm_point(R, k, xx, yy);
+        f[k] = chi_point(Q, M, sA, exch, site, W, xx, yy);
+    }
+    gm_combine(R, f);
+}
+
+// one block per task slot; Q.tasks[slot] names the (frequency, site, box) task
+__global__ __launch_bounds__(CHI_T) void k_chi(ChiArgs Q) {
     __shared__ double s_mesh[5 * MAXN];
     __shared__ double s_sA[MAXN];
     __shared__ int s_exch[64];
-    __shared__ double s_f[34];
+    __shared__ double s_red[CHI_T];
+    __shared__ long s_scan[CHI_T];
     const Sys &y = Q.y;
-    const int lane = threadIdx.x;
-    const int t = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int slot = blockIdx.x;
+    const int t = Q.tasks[slot];
     const int box = t % 9, q = t / 9;
     const int w = q / y.ns, site = q % y.ns;
-    for (int k = lane; k < 5 * MAXN; k += 64) s_mesh[k] = Q.mesh[k];
-    for (int k = lane; k < y.nsig; k += 64) s_sA[k] = Q.A.Sigma[k];
-    if (lane < y.ns) s_exch[lane] = Q.exch[lane];
+    for (int k = tid; k < 5 * MAXN; k += CHI_T) s_mesh[k] = Q.mesh[k];
+    for (int k = tid; k < y.nsig; k += CHI_T) s_sA[k] = Q.A.Sigma[k];
+    if (tid < y.ns) s_exch[tid] = Q.exch[tid];
     __syncthreads();
     const Meshes<> M{s_mesh, s_mesh + MAXN, s_mesh + 2 * MAXN, s_mesh + 3 * MAXN, s_mesh + 4 * MAXN};
     const double W = Q.chi[w];
     const double ref = Q.L + 0.5 * W;
     double lo[2], hi[2];
     chi_box(4.0 * ref, box, lo, hi);
-    const long base = (long)t * Q.cap;
-    Reg r0;
-    r0.c0 = 0.5 * (lo[0] + hi[0]);
-    r0.c1 = 0.5 * (lo[1] + hi[1]);
-    r0.h0 = 0.5 * (hi[0] - lo[0]);
-    r0.h1 = 0.5 * (hi[1] - lo[1]);
-    if (lane < 17) {
-        double xx, yy;
-        gm_point(r0, lane, xx, yy);
-        s_f[lane] = chi_point(Q, M, s_sA, s_exch, site, W, xx, yy);
-    }
-    __syncthreads();
-    if (lane == 0) {
-        gm_combine(r0, s_f);
+    const long base = (long)slot * Q.cap;
+    if (tid == 0) {
+        Reg r0;
+        r0.c0 = 0.5 * (lo[0] + hi[0]);
+        r0.c1 = 0.5 * (lo[1] + hi[1]);
+        r0.h0 = 0.5 * (hi[0] - lo[0]);
+        r0.h1 = 0.5 * (hi[1] - lo[1]);
+        eval_region(Q, M, s_sA, s_exch, site, W, r0);
         store_reg(Q, base, 0, r0);
     }
     __syncthreads();
-    int n = 1;
+    long n = 1;
     double tv;
+    int status = 0;
     while (true) {
-        // totals in strided per-lane order + xor butterfly, and the first region of largest error
-        double pv = 0.0, pe = 0.0, be = -1.0;
-        int bj = -1;
-        for (int j = lane; j < n; j += 64) {
-            const double e = Q.rerr[base + j];
-            pv += Q.rval[base + j];
-            pe += e;
-            if (e > be) {
-                be = e;
-                bj = j;
+        tv = block_sum(n, [&](long j) { return Q.rval[base + j]; }, s_red);
+        const double te = block_sum(n, [&](long j) { return Q.rerr[base + j]; }, s_red);
+        if (te <= Q.atol || te <= fabs(tv) * Q.rtol || n >= CHI_MAX_REGIONS) break;
+        // the largest key K whose regions {err_key >= K} carry enough error (K = 0 takes all)
+        unsigned klo = 0, khi = 65536;
+        while (khi - klo > 1) {
+            const unsigned mid = (klo + khi) / 2;
+            const double S = block_sum(
+                n, [&](long j) { const double e = Q.rerr[base + j]; return err_key(e) >= mid ? e : 0.0; }, s_red);
+            const double rest = te - S;
+            if (rest <= Q.atol || rest <= fabs(tv) * Q.rtol)
+                klo = mid;
+            else
+                khi = mid;
+        }
+        // selected regions in index order: thread t scans the contiguous range [j0, j1)
+        const long per = (n + CHI_T - 1) / CHI_T;
+        const long j0 = (long)tid * per < n ? (long)tid * per : n;
+        const long j1 = j0 + per < n ? j0 + per : n;
+        long cnt = 0;
+        for (long j = j0; j < j1; ++j) cnt += err_key(Q.rerr[base + j]) >= klo ? 1 : 0;
+        long total;
+        long pos = block_scan(cnt, s_scan, &total);
+        const long room = CHI_MAX_REGIONS - n;
+        const long k = total < room ? total : room;
+        if (n + k > Q.cap) {
+            status = 1;  // this slot is too small: the host runs the task again with more room
+            break;
+        }
+        for (long j = j0; j < j1 && pos < k; ++j)
+            if (err_key(Q.rerr[base + j]) >= klo) Q.rsel[base + pos++] = (int)j;
+        __syncthreads();
+        for (long r = tid; r < k; r += CHI_T) {
+            const long j = Q.rsel[base + r];
+            Reg R;
+            R.c0 = Q.rc0[base + j];
+            R.c1 = Q.rc1[base + j];
+            R.h0 = Q.rh0[base + j];
+            R.h1 = Q.rh1[base + j];
+            const int d = Q.rsplit[base + j];
+            Reg a = R, b = R;
+            if (d == 0) {
+                a.h0 = 0.5 * R.h0;
+                b.h0 = a.h0;
+                a.c0 = R.c0 - a.h0;
+                b.c0 = R.c0 + a.h0;
+            } else {
+                a.h1 = 0.5 * R.h1;
+                b.h1 = a.h1;
+                a.c1 = R.c1 - a.h1;
+                b.c1 = R.c1 + a.h1;
             }
-        }
-        for (int m = 32; m >= 1; m >>= 1) {
-            pv = pv + __shfl_xor(pv, m);
-            pe = pe + __shfl_xor(pe, m);
-            const double oe = __shfl_xor(be, m);
-            const int oj = __shfl_xor(bj, m);
-            if (oe > be || (oe == be && oj >= 0 && (bj < 0 || oj < bj))) {
-                be = oe;
-                bj = oj;
-            }
-        }
-        tv = pv;
-        if (pe <= Q.atol || pe <= fabs(pv) * Q.rtol || n >= Q.cap) break;
-        const int k = bj;
-        Reg R;
-        R.c0 = Q.rc0[base + k];
-        R.c1 = Q.rc1[base + k];
-        R.h0 = Q.rh0[base + k];
-        R.h1 = Q.rh1[base + k];
-        const int d = Q.rsplit[base + k];
-        Reg a = R, b = R;
-        if (d == 0) {
-            a.h0 = 0.5 * R.h0;
-            b.h0 = a.h0;
-            a.c0 = R.c0 - a.h0;
-            b.c0 = R.c0 + a.h0;
-        } else {
-            a.h1 = 0.5 * R.h1;
-            b.h1 = a.h1;
-            a.c1 = R.c1 - a.h1;
-            b.c1 = R.c1 + a.h1;
-        }
-        if (lane < 34) {
-            double xx, yy;
-            gm_point(lane < 17 ? a : b, lane % 17, xx, yy);
-            s_f[lane] = chi_point(Q, M, s_sA, s_exch, site, W, xx, yy);
+            eval_region(Q, M, s_sA, s_exch, site, W, a);
+            eval_region(Q, M, s_sA, s_exch, site, W, b);
+            store_reg(Q, base, j, a);
+            store_reg(Q, base, n + r, b);
         }
         __syncthreads();
-        if (lane == 0) {
-            gm_combine(a, s_f);
-            store_reg(Q, base, k, a);
-        } else if (lane == 17) {
-            gm_combine(b, s_f + 17);
-            store_reg(Q, base, n, b);
-        }
-        __syncthreads();
-        ++n;
+        n += k;
     }
-    if (lane == 0) {
+    if (tid == 0) {
         Q.box[t] = tv;
-        Q.nreg[t] = n;
+        Q.nreg[t] = (int)n;
+        Q.status[t] = status;
     }
 }
 
@@ -240,9 +292,9 @@ __global__ __launch_bounds__(64) void k_chi_prop(ChiArgs Q) {
 }  // namespace
 
 hipError_t launch_chi(const ChiArgs &Q, hipStream_t stream) {
-    hipLaunchKernelGGL(k_chi, dim3(Q.nchi * Q.y.ns * 9), dim3(64), 0, stream, Q);
+    hipLaunchKernelGGL(k_chi, dim3(Q.nslots), dim3(CHI_T), 0, stream, Q);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess || !Q.with_prop) return e;
     hipLaunchKernelGGL(k_chi_prop, dim3(Q.nchi), dim3(64), 0, stream, Q);
     return hipGetLastError();
 }

@@ -1324,11 +1324,14 @@ int pffrg_compute_chi(pffrg_ctx *c, double L, int32_t a, const double *chi_mesh,
     HIPCHK(c, hipSetDevice(c->device));
     const int ns = c->sys.ns;
     const long tasks = (long)nchi * ns * 9;
-    const long cap = CHI_MAX_REGIONS;
-    // regions: 6 double arrays + 1 int array of tasks*cap; then box[tasks], nreg[tasks], prop[nchi], mesh[nchi]
     auto r256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t need = 6 * r256((size_t)tasks * cap * 8) + r256((size_t)tasks * cap * 4) + r256(tasks * 8) +
-                        r256(tasks * 4) + 2 * r256((size_t)nchi * 8);
+    // per call: box[tasks], nreg[tasks], status[tasks], task list[tasks], prop[nchi], χ mesh[nchi]
+    const size_t fixed = r256(tasks * 8) + 3 * r256(tasks * 4) + 2 * r256((size_t)nchi * 8);
+    // every task first runs in a slot of CHI_CAP0 regions; the few that need more run again in
+    // slots of the full budget, CHI_POOL_SLOTS at a time (the result does not depend on the slot size)
+    const long CHI_CAP0 = 16384, CHI_POOL_SLOTS = 64;
+    const size_t pool = std::max((size_t)tasks * CHI_CAP0, (size_t)CHI_POOL_SLOTS * CHI_MAX_REGIONS);
+    const size_t need = fixed + 6 * r256(pool * 8) + 2 * r256(pool * 4);
     if (need > c->chi_bytes) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         HIPCHK(c, hipFree(c->chi_mem));
@@ -1340,7 +1343,7 @@ int pffrg_compute_chi(pffrg_ctx *c, double L, int32_t a, const double *chi_mesh,
     char *p = (char *)c->chi_mem;
     auto take = [&](size_t bytes) {
         char *q = p;
-        p += (bytes + 255) & ~(size_t)255;
+        p += r256(bytes);
         return (void *)q;
     };
     ChiArgs Q;
@@ -1353,30 +1356,55 @@ int pffrg_compute_chi(pffrg_ctx *c, double L, int32_t a, const double *chi_mesh,
     Q.atol = tol[0];
     Q.rtol = tol[1];
     Q.nchi = nchi;
-    Q.cap = (int)cap;
-    const size_t rb = (size_t)tasks * cap;
-    p = (char *)c->chi_mem;
-    Q.rc0 = (double *)take(rb * 8);
-    Q.rc1 = (double *)take(rb * 8);
-    Q.rh0 = (double *)take(rb * 8);
-    Q.rh1 = (double *)take(rb * 8);
-    Q.rval = (double *)take(rb * 8);
-    Q.rerr = (double *)take(rb * 8);
-    Q.rsplit = (int *)take(rb * 4);
     Q.box = (double *)take(tasks * 8);
     Q.nreg = (int *)take(tasks * 4);
+    Q.status = (int *)take(tasks * 4);
+    int *d_tasks = (int *)take(tasks * 4);
+    Q.tasks = d_tasks;
     Q.prop = (double *)take(nchi * 8);
     double *dchi = (double *)take(nchi * 8);
-    if ((size_t)(p - (char *)c->chi_mem) > c->chi_bytes) return fail(c, PFFRG_ERR_LIMIT, "chi scratch layout overflow");
     Q.chi = dchi;
+    Q.rc0 = (double *)take(pool * 8);
+    Q.rc1 = (double *)take(pool * 8);
+    Q.rh0 = (double *)take(pool * 8);
+    Q.rh1 = (double *)take(pool * 8);
+    Q.rval = (double *)take(pool * 8);
+    Q.rerr = (double *)take(pool * 8);
+    Q.rsplit = (int *)take(pool * 4);
+    Q.rsel = (int *)take(pool * 4);
+    if ((size_t)(p - (char *)c->chi_mem) > c->chi_bytes) return fail(c, PFFRG_ERR_LIMIT, "chi scratch layout overflow");
+    std::vector<int> list(tasks);
+    for (long t = 0; t < tasks; ++t) list[t] = (int)t;
     HIPCHK(c, hipMemcpyAsync(dchi, chi_mesh, nchi * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_tasks, list.data(), tasks * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    Q.cap = CHI_CAP0;
+    Q.nslots = (int)tasks;
+    Q.with_prop = true;
     HIPCHK(c, launch_chi(Q, c->stream));
     std::vector<double> box(tasks), prop(nchi);
-    std::vector<int> nr(tasks);
+    std::vector<int> nr(tasks), st(tasks);
+    HIPCHK(c, hipMemcpyAsync(st.data(), Q.status, tasks * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // tasks whose regions outgrew the first slot size: again with the full budget
+    std::vector<int> again;
+    for (long t = 0; t < tasks; ++t)
+        if (st[t] != 0) again.push_back((int)t);
+    Q.cap = CHI_MAX_REGIONS;
+    Q.with_prop = false;
+    for (size_t b0 = 0; b0 < again.size(); b0 += CHI_POOL_SLOTS) {
+        const int nb = (int)std::min((size_t)CHI_POOL_SLOTS, again.size() - b0);
+        HIPCHK(c, hipMemcpy(d_tasks, again.data() + b0, nb * sizeof(int), hipMemcpyHostToDevice));
+        Q.nslots = nb;
+        HIPCHK(c, launch_chi(Q, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     HIPCHK(c, hipMemcpyAsync(box.data(), Q.box, tasks * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(nr.data(), Q.nreg, tasks * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(st.data(), Q.status, tasks * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(prop.data(), Q.prop, nchi * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (long t = 0; t < tasks; ++t)
+        if (st[t] != 0) return fail(c, PFFRG_ERR_LIMIT, "chi cubature slot overflow");
     for (int w = 0; w < nchi; ++w)
         for (int i = 0; i < ns; ++i) {
             const long t0 = ((long)w * ns + i) * 9;

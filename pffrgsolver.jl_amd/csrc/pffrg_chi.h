@@ -8,7 +8,9 @@
 
 namespace pffrg {
 
-constexpr int CHI_MAX_REGIONS = 8192;  // cubature regions per (site, frequency, box) task
+constexpr long CHI_MAX_EVALS = 10000000;                       // maxevals of integrate_χ_boxes
+constexpr long CHI_MAX_REGIONS = 1 + (CHI_MAX_EVALS - 17) / 34;  // regions that budget allows
+constexpr int CHI_T = 256;                                       // threads per cubature task
 
 struct ChiArgs {
     Sys y;
@@ -17,16 +19,21 @@ struct ChiArgs {
     const int *exch;
     const double *chi;   // χ frequencies [nchi]
     double L, atol, rtol;
-    int nchi, cap;
-    // cubature regions of every task, [task][cap]
+    int nchi;
+    long cap;            // region capacity of one task slot
+    int nslots;          // task slots of this launch
+    const int *tasks;    // [nslots] task of each slot, task = (w * ns + site) * 9 + box
+    bool with_prop;      // also run the site-0 propagator term
+    // cubature regions of every slot, [slot][cap]
     double *rc0, *rc1, *rh0, *rh1, *rval, *rerr;
-    int *rsplit;
+    int *rsplit, *rsel;
     double *box;         // [task] integral of the task's box
     int *nreg;           // [task] final region count
+    int *status;         // [task] 0 done, 1 the slot's capacity was too small
     double *prop;        // [nchi] propagator term of site 0
 };
 
-// tasks = nchi * ns * 9 (task = (w * ns + site) * 9 + box)
+// k_chi over the nslots task slots, then (with_prop) the propagator term
 hipError_t launch_chi(const ChiArgs &Q, hipStream_t stream);
 
 }  // namespace pffrg

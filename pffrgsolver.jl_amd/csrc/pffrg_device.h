@@ -6,6 +6,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
+#include "pffrg_math.h"
+
 namespace pffrg {
 
 constexpr int MAXN = 64;      // mesh points per axis held in LDS
@@ -272,9 +276,23 @@ __device__ __forceinline__ ST sigma_at(ST w, const ST *sm, int n, const ST *Sig)
     return sm[n - 1] * Sig[n - 1] / w;
 }
 
+// exp and expm1: the written-out double versions shared with the oracle, the device library's
+// in single precision
+template <typename ST>
+__device__ __forceinline__ ST exp_st(ST x) {
+    if constexpr (std::is_same<ST, double>::value) return pf_exp(x);
+    else return expf(x);
+}
+
+template <typename ST>
+__device__ __forceinline__ ST expm1_st(ST x) {
+    if constexpr (std::is_same<ST, double>::value) return pf_expm1(x);
+    else return expm1f(x);
+}
+
 template <typename ST>
 __device__ __forceinline__ ST G_bare(ST L, ST w) {
-    return (fabs(w) > (ST)1e-8) ? -expm1(-(w * w) / (L * L)) / w : (ST)0;
+    return (fabs(w) > (ST)1e-8) ? -expm1_st<ST>(-(w * w) / (L * L)) / w : (ST)0;
 }
 
 template <typename ST>
@@ -291,7 +309,7 @@ __device__ __forceinline__ double S_single(double L, double w, const double *sm,
     const double G = G_full<double>(L, w, sm, n, Sig);
     const double G0 = G_bare<double>(L, w);
     const double r = G / G0;
-    return r * r * exp(-(w * w) / (L * L)) * 2.0 * w / (L * L * L);
+    return r * r * pf_exp(-(w * w) / (L * L)) * 2.0 * w / (L * L * L);
 }
 
 // symmetrised Katanin bubble K(x1,x2) + K(x2,x1), K = (S + G²dΣ) G' / 2π (bubbles.jl:64-80)

@@ -469,7 +469,7 @@ __device__ __forceinline__ void bubble_leg(ST L, ST x, const MeshV<ST> &M, int n
         ST Sx = (ST)0;
         if (ax > (ST)1e-8) {
             const ST r = G / G0;
-            Sx = r * r * exp(-(x * x) / (L * L)) * (ST)2 * x / (L * L * L);
+            Sx = r * r * exp_st<ST>(-(x * x) / (L * L)) * (ST)2 * x / (L * L * L);
         }
         K = Sx + G * G * ds;
     }

@@ -467,3 +467,24 @@ def test_coupling_samples_deterministic():
     assert s.shape == (80, 2)
     assert np.all((s[:, 0] >= -0.5) & (s[:, 0] <= 0.5)) and np.all((s[:, 1] >= 0.5) & (s[:, 1] <= 5.0))
     np.testing.assert_array_equal(s, learn.coupling_samples(80, seed=5))
+
+
+def test_shared_exp_expm1_accuracy():
+    """pffrg_math.h's exp/expm1 (used by the device code and the oracle alike, so that both
+    produce the same bits) against the 80-bit long-double library: within 3 ulp."""
+    lib = O.lib()
+    rng = np.random.default_rng(17)
+    xs = np.concatenate([rng.uniform(-45, 5, 20000), rng.uniform(-1e-3, 1e-3, 5000), rng.uniform(-1, 1, 5000),
+                         rng.uniform(-700, 700, 5000), [0.0, 1e-300, -1e-20, -40.0, -40.5, 0.34657359, -0.34657359]])
+    worst = 0.0
+    for x in xs:
+        for f, ref in ((lib.orc_pf_expm1, np.expm1), (lib.orc_pf_exp, np.exp)):
+            got = f(float(x))
+            r = ref(np.longdouble(x))
+            if r == 0:
+                assert got == 0.0
+                continue
+            ulp = np.spacing(np.abs(np.float64(r)))
+            worst = max(worst, float(abs(np.longdouble(got) - r) / np.longdouble(ulp)))
+    assert worst <= 3.0, worst
+    assert lib.orc_pf_expm1(-50.0) == -1.0 and lib.orc_pf_exp(-800.0) == 0.0

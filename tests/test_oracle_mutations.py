@@ -63,8 +63,9 @@ def test_invariant_detects_planted_mistake(orig, bad, tmp_path):
     assert text.count(orig) == 1, orig
     (tmp_path / "m.cpp").write_text(text.replace(orig, bad))
     lib = tmp_path / "liboracle_m.so"
-    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-shared", "-o", str(lib),
-                    str(tmp_path / "m.cpp")], check=True, timeout=300)
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off",
+                    f"-I{ROOT / 'pffrgsolver.jl_amd' / 'csrc'}", "-shared", "-o", str(lib), str(tmp_path / "m.cpp")],
+                   check=True, timeout=300)
     assert _worst(lib, tmp_path) > 1e-2
 
 
