@@ -830,8 +830,8 @@ void compute_corrs(const orc_system &y, double L, const orc_action &a, const orc
 // largest-error regions that must be refined for the error of the others to meet
 // max(atol, rtol |I|), until the total error meets it or the evaluation budget is spent.
 // Fixed here, and shared with the GPU kernel so both produce the same regions and sums:
-//   - the selection threshold is rounded down to 1/16 octave (the top 16 bits of the IEEE
-//     double of the error), so a round splits every region within that bucket;
+//   - the selection works on quarter-octave bins of the error below the largest one, with
+//     the bin sums in fixed point, so a round splits every region of the bins it needs;
 //   - split regions keep their index, the upper halves are appended in index order;
 //   - totals are summed in the order of a 256-thread block (thread t adds indices t, t+256, ...,
 //     then p[t] += p[t+s] for s = 128, 64, ..., 1).
@@ -882,12 +882,25 @@ const long CUB_MAX_EVALS = 10000000;                          // maxevals of int
 const long CUB_MAX_REGIONS = 1 + (CUB_MAX_EVALS - 17) / 34;   // regions that budget allows
 const int CUB_T = 256;
 
-// the selection key: the top 16 bits of a non-negative double (sign, exponent, 4 mantissa bits)
-inline unsigned err_key(double e) {
+// selection bins of a round: errors are binned by quarter octaves below the largest one (bin 0
+// holds the largest, bin 255 everything from 2^-63 of it down) and summed per bin in 44-bit fixed
+// point relative to the largest (integer sums, so the order of accumulation does not matter)
+const int CUB_BINS = 256;
+
+inline uint64_t dbits(double e) {
     uint64_t b;
     std::memcpy(&b, &e, 8);
-    return (unsigned)(b >> 48);
+    return b;
 }
+
+inline int err_exp(double e) { return (int)((dbits(e) >> 52) & 0x7FF); }
+
+inline int err_bin(double e, int E) {
+    const int d = (4 * E + 3) - (int)((dbits(e) >> 50) & 0x1FFF);
+    return d < 0 ? 0 : (d > CUB_BINS - 1 ? CUB_BINS - 1 : d);
+}
+
+inline uint64_t err_fixed(double e, int E) { return (uint64_t)std::ldexp(e, 1066 - (E > 1 ? E : 1)); }
 
 // Σ_j x[j] over j < n in the GPU block's order
 template <class G>
@@ -930,20 +943,35 @@ double hcubature2(const F &f, const double *lo, const double *hi, double atol, d
         tv = block_sum(n, [&](long j) { return R[j].val; });
         const double te = block_sum(n, [&](long j) { return R[j].err; });
         if (cub_converged(tv, te, atol, rtol) || n >= CUB_MAX_REGIONS) break;
-        // the largest key K whose regions {err_key >= K} carry enough error (K = 0 takes all)
-        unsigned klo = 0, khi = 65536;
-        while (khi - klo > 1) {
-            const unsigned mid = (klo + khi) / 2;
-            const double S = block_sum(n, [&](long j) { return err_key(R[j].err) >= mid ? R[j].err : 0.0; });
-            if (cub_converged(tv, te - S, atol, rtol))
-                klo = mid;
-            else
-                khi = mid;
+        // the fewest leading bins whose regions carry enough error for the rest to converge
+        int E = 0;
+        for (long j = 0; j < n; ++j) E = std::max(E, err_exp(R[j].err));
+        uint64_t H[CUB_BINS] = {0};
+        long N[CUB_BINS] = {0};
+        for (long j = 0; j < n; ++j) {
+            H[err_bin(R[j].err, E)] += err_fixed(R[j].err, E);
+            N[err_bin(R[j].err, E)] += 1;
+        }
+        uint64_t Q = 0;
+        for (int b = 0; b < CUB_BINS; ++b) Q += H[b];
+        // from the first occupied bin on, so that a round splits at least one region
+        int b0 = 0;
+        while (b0 < CUB_BINS - 1 && N[b0] == 0) ++b0;
+        int bstar = CUB_BINS - 1;
+        uint64_t Cs = 0;
+        for (int b = 0; b < b0; ++b) Cs += H[b];
+        for (int b = b0; b < CUB_BINS; ++b) {
+            Cs += H[b];
+            const double rest = std::ldexp((double)(Q - Cs), (E > 1 ? E : 1) - 1066);
+            if (cub_converged(tv, rest, atol, rtol)) {
+                bstar = b;
+                break;
+            }
         }
         sel.clear();
         const long room = CUB_MAX_REGIONS - n;
         for (long j = 0; j < n && (long)sel.size() < room; ++j)
-            if (err_key(R[j].err) >= klo) sel.push_back(j);
+            if (err_bin(R[j].err, E) <= bstar) sel.push_back(j);
         R.resize(n + sel.size());
         for (size_t q = 0; q < sel.size(); ++q) {
             const Region r = R[sel[q]];

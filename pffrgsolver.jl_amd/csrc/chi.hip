@@ -1,15 +1,17 @@
 // chi.hip -- real-space correlations χ_ij(ω) (compute_χ!, correlation_su2.jl:58-86).
 //
-// One 256-thread block per (χ frequency, site, box) task runs the adaptive 2-D cubature of
-// integrate_χ_boxes (correlation_lib.jl:2-30) on the kernel of compute_χ_kernel!
-// (correlation_su2.jl:14-55) with the vectorised schedule of Cubature's hcubature_v: every
-// round splits at once the largest-error regions that must be refined for the error of the
-// others to meet the tolerance, each thread evaluating the 2 x 17 Genz-Malik points of the
-// halves of its regions.  Regions live in global memory; the totals are block sums in a fixed
-// order, the selection threshold is found by bisection over the top 16 bits of the error, and
-// new regions are appended in index order, all as in the oracle's restatement.  This file is
-// compiled without FMA contraction so that its arithmetic, and with it every adaptive
-// decision, is the oracle's.
+// Every (χ frequency, site, box) task runs the adaptive 2-D cubature of integrate_χ_boxes
+// (correlation_lib.jl:2-30) on the kernel of compute_χ_kernel! (correlation_su2.jl:14-55) with
+// the vectorised schedule of Cubature's hcubature_v: every round splits at once the
+// largest-error regions that must be refined for the error of the others to meet the
+// tolerance.  A round is two launches over all unfinished tasks: k_chi_round (one 256-thread
+// block per task: totals, convergence, selection) and k_chi_split (one thread per selected
+// region, over as many blocks as the round has splits, evaluating the 2 x 17 Genz-Malik points
+// of its halves), so that a task with many regions spreads over the whole GPU.  Regions live in
+// global memory; the totals are block sums in a fixed order, the selection takes whole
+// quarter-octave error bins (fixed-point bin sums in LDS), and new regions are appended in
+// index order, all as in the oracle's restatement.  This file is compiled without FMA
+// contraction so that its arithmetic, and with it every adaptive decision, is the oracle's.
 #include "pffrg_chi.h"
 
 namespace pffrg {
@@ -106,26 +108,46 @@ __device__ __forceinline__ void store_reg(const ChiArgs &Q, long base, long j, c
     Q.rsplit[base + j] = R.split;
 }
 
-// the selection key: the top 16 bits of a non-negative double
-__device__ __forceinline__ unsigned err_key(double e) {
-    return (unsigned)((unsigned long long)__double_as_longlong(e) >> 48);
+// selection bins: quarter octaves below the largest error (bin 0 holds the largest, the last
+// bin everything from 2^-63 of it down), summed in 44-bit fixed point relative to the largest
+constexpr int CHI_BINS = 256;
+
+__device__ __forceinline__ unsigned long long dbits(double e) { return (unsigned long long)__double_as_longlong(e); }
+
+__device__ __forceinline__ int err_exp(double e) { return (int)((dbits(e) >> 52) & 0x7FF); }
+
+__device__ __forceinline__ int err_bin(double e, int E) {
+    const int d = (4 * E + 3) - (int)((dbits(e) >> 50) & 0x1FFF);
+    return d < 0 ? 0 : (d > CHI_BINS - 1 ? CHI_BINS - 1 : d);
 }
 
-// Σ_j x(j) over j < n: thread t adds j = t, t + CHI_T, ..., then a tree over the threads
-template <class G>
-__device__ double block_sum(long n, const G &x, double *s_red) {
+__device__ __forceinline__ unsigned long long err_fixed(double e, int E) {
+    return (unsigned long long)ldexp(e, 1066 - (E > 1 ? E : 1));
+}
+
+// the region totals Σ val and Σ err in one pass: thread t adds j = t, t + CHI_T, ..., then a
+// tree over the threads (p[t] += p[t + s] for s = CHI_T/2, ..., 1)
+__device__ void block_sum2(long n, const double *val, const double *err, double *s_a, double *s_b, double &sv,
+                           double &se) {
     const int t = threadIdx.x;
-    double a = 0.0;
-    for (long j = t; j < n; j += CHI_T) a += x(j);
-    s_red[t] = a;
+    double a = 0.0, b = 0.0;
+    for (long j = t; j < n; j += CHI_T) {
+        a += val[j];
+        b += err[j];
+    }
+    s_a[t] = a;
+    s_b[t] = b;
     __syncthreads();
     for (int s = CHI_T / 2; s >= 1; s >>= 1) {
-        if (t < s) s_red[t] += s_red[t + s];
+        if (t < s) {
+            s_a[t] += s_a[t + s];
+            s_b[t] += s_b[t + s];
+        }
         __syncthreads();
     }
-    const double r = s_red[0];
+    sv = s_a[0];
+    se = s_b[0];
     __syncthreads();
-    return r;
 }
 
 // exclusive prefix of v over the block's threads (and the total in *total)
@@ -156,108 +178,191 @@ __device__ __forceinline__ void eval_region(const ChiArgs &Q, const Meshes<> &M,
     gm_combine(R, f);
 }
 
-// one block per task slot; Q.tasks[slot] names the (frequency, site, box) task
-__global__ __launch_bounds__(CHI_T) void k_chi(ChiArgs Q) {
-    __shared__ double s_mesh[5 * MAXN];
-    __shared__ double s_sA[MAXN];
-    __shared__ int s_exch[64];
-    __shared__ double s_red[CHI_T];
-    __shared__ long s_scan[CHI_T];
-    const Sys &y = Q.y;
-    const int tid = threadIdx.x;
-    const int slot = blockIdx.x;
-    const int t = Q.tasks[slot];
+// the LDS copies every χ kernel reads: meshes, Σ of the action, the exchange table
+struct ChiLds {
+    double mesh[5 * MAXN];
+    double sA[MAXN];
+    int exch[64];
+};
+
+__device__ __forceinline__ void load_lds(const ChiArgs &Q, ChiLds &S) {
+    for (int k = threadIdx.x; k < 5 * MAXN; k += blockDim.x) S.mesh[k] = Q.mesh[k];
+    for (int k = threadIdx.x; k < Q.y.nsig; k += blockDim.x) S.sA[k] = Q.A.Sigma[k];
+    if ((int)threadIdx.x < Q.y.ns) S.exch[threadIdx.x] = Q.exch[threadIdx.x];
+    __syncthreads();
+}
+
+struct TaskGeo {
+    int site;
+    double W, lo[2], hi[2];
+};
+
+__device__ __forceinline__ TaskGeo task_geo(const ChiArgs &Q, int t) {
+    TaskGeo g;
     const int box = t % 9, q = t / 9;
-    const int w = q / y.ns, site = q % y.ns;
-    for (int k = tid; k < 5 * MAXN; k += CHI_T) s_mesh[k] = Q.mesh[k];
-    for (int k = tid; k < y.nsig; k += CHI_T) s_sA[k] = Q.A.Sigma[k];
-    if (tid < y.ns) s_exch[tid] = Q.exch[tid];
+    const int w = q / Q.y.ns;
+    g.site = q % Q.y.ns;
+    g.W = Q.chi[w];
+    chi_box(4.0 * (Q.L + 0.5 * g.W), box, g.lo, g.hi);
+    return g;
+}
+
+// the first region (the whole box) of every slot: one thread per rule point
+__global__ __launch_bounds__(64) void k_chi_init(ChiArgs Q) {
+    __shared__ ChiLds S;
+    __shared__ double s_f[17];
+    load_lds(Q, S);
+    const Meshes<> M{S.mesh, S.mesh + MAXN, S.mesh + 2 * MAXN, S.mesh + 3 * MAXN, S.mesh + 4 * MAXN};
+    const int slot = blockIdx.x;
+    const TaskGeo g = task_geo(Q, Q.tasks[slot]);
+    Reg r0;
+    r0.c0 = 0.5 * (g.lo[0] + g.hi[0]);
+    r0.c1 = 0.5 * (g.lo[1] + g.hi[1]);
+    r0.h0 = 0.5 * (g.hi[0] - g.lo[0]);
+    r0.h1 = 0.5 * (g.hi[1] - g.lo[1]);
+    if (threadIdx.x < 17) {
+        double xx, yy;
+        gm_point(r0, threadIdx.x, xx, yy);
+        s_f[threadIdx.x] = chi_point(Q, M, S.sA, S.exch, g.site, g.W, xx, yy);
+    }
     __syncthreads();
-    const Meshes<> M{s_mesh, s_mesh + MAXN, s_mesh + 2 * MAXN, s_mesh + 3 * MAXN, s_mesh + 4 * MAXN};
-    const double W = Q.chi[w];
-    const double ref = Q.L + 0.5 * W;
-    double lo[2], hi[2];
-    chi_box(4.0 * ref, box, lo, hi);
+    if (threadIdx.x == 0) {
+        gm_combine(r0, s_f);
+        store_reg(Q, (long)slot * Q.cap, 0, r0);
+        Q.rn[slot] = 1;
+        Q.rk[slot] = 0;
+        Q.rdone[slot] = 0;
+    }
+}
+
+// one round of every active slot (Q.active[blockIdx.x]): append the previous round's halves,
+// totals and convergence, else the selection of the regions to split next
+__global__ __launch_bounds__(CHI_T) void k_chi_round(ChiArgs Q) {
+    __shared__ double s_red[CHI_T], s_red2[CHI_T];
+    __shared__ long s_scan[CHI_T];
+    __shared__ unsigned long long s_hist[CHI_BINS];
+    __shared__ int s_cnt[CHI_BINS];
+    __shared__ int s_ctl[2];
+    const int tid = threadIdx.x;
+    const int slot = Q.active[blockIdx.x];
+    const int t = Q.tasks[slot];
     const long base = (long)slot * Q.cap;
-    if (tid == 0) {
-        Reg r0;
-        r0.c0 = 0.5 * (lo[0] + hi[0]);
-        r0.c1 = 0.5 * (lo[1] + hi[1]);
-        r0.h0 = 0.5 * (hi[0] - lo[0]);
-        r0.h1 = 0.5 * (hi[1] - lo[1]);
-        eval_region(Q, M, s_sA, s_exch, site, W, r0);
-        store_reg(Q, base, 0, r0);
+    const long n = Q.rn[slot] + Q.rk[slot];
+    double tv, te;
+    block_sum2(n, Q.rval + base, Q.rerr + base, s_red, s_red2, tv, te);
+    if (te <= Q.atol || te <= fabs(tv) * Q.rtol || n >= CHI_MAX_REGIONS) {
+        if (tid == 0) {
+            Q.box[t] = tv;
+            Q.nreg[t] = (int)n;
+            Q.status[t] = 0;
+            Q.rn[slot] = n;
+            Q.rk[slot] = 0;
+            Q.rdone[slot] = 1;
+        }
+        return;
+    }
+    // the fewest leading bins whose regions carry enough error for the rest to converge
+    if (tid == 0) s_ctl[0] = 0;
+    for (int b = tid; b < CHI_BINS; b += CHI_T) {
+        s_hist[b] = 0ull;
+        s_cnt[b] = 0;
     }
     __syncthreads();
-    long n = 1;
-    double tv;
-    int status = 0;
-    while (true) {
-        tv = block_sum(n, [&](long j) { return Q.rval[base + j]; }, s_red);
-        const double te = block_sum(n, [&](long j) { return Q.rerr[base + j]; }, s_red);
-        if (te <= Q.atol || te <= fabs(tv) * Q.rtol || n >= CHI_MAX_REGIONS) break;
-        // the largest key K whose regions {err_key >= K} carry enough error (K = 0 takes all)
-        unsigned klo = 0, khi = 65536;
-        while (khi - klo > 1) {
-            const unsigned mid = (klo + khi) / 2;
-            const double S = block_sum(
-                n, [&](long j) { const double e = Q.rerr[base + j]; return err_key(e) >= mid ? e : 0.0; }, s_red);
-            const double rest = te - S;
-            if (rest <= Q.atol || rest <= fabs(tv) * Q.rtol)
-                klo = mid;
-            else
-                khi = mid;
-        }
-        // selected regions in index order: thread t scans the contiguous range [j0, j1)
-        const long per = (n + CHI_T - 1) / CHI_T;
-        const long j0 = (long)tid * per < n ? (long)tid * per : n;
-        const long j1 = j0 + per < n ? j0 + per : n;
-        long cnt = 0;
-        for (long j = j0; j < j1; ++j) cnt += err_key(Q.rerr[base + j]) >= klo ? 1 : 0;
-        long total;
-        long pos = block_scan(cnt, s_scan, &total);
-        const long room = CHI_MAX_REGIONS - n;
-        const long k = total < room ? total : room;
-        if (n + k > Q.cap) {
-            status = 1;  // this slot is too small: the host runs the task again with more room
-            break;
-        }
-        for (long j = j0; j < j1 && pos < k; ++j)
-            if (err_key(Q.rerr[base + j]) >= klo) Q.rsel[base + pos++] = (int)j;
-        __syncthreads();
-        for (long r = tid; r < k; r += CHI_T) {
-            const long j = Q.rsel[base + r];
-            Reg R;
-            R.c0 = Q.rc0[base + j];
-            R.c1 = Q.rc1[base + j];
-            R.h0 = Q.rh0[base + j];
-            R.h1 = Q.rh1[base + j];
-            const int d = Q.rsplit[base + j];
-            Reg a = R, b = R;
-            if (d == 0) {
-                a.h0 = 0.5 * R.h0;
-                b.h0 = a.h0;
-                a.c0 = R.c0 - a.h0;
-                b.c0 = R.c0 + a.h0;
-            } else {
-                a.h1 = 0.5 * R.h1;
-                b.h1 = a.h1;
-                a.c1 = R.c1 - a.h1;
-                b.c1 = R.c1 + a.h1;
-            }
-            eval_region(Q, M, s_sA, s_exch, site, W, a);
-            eval_region(Q, M, s_sA, s_exch, site, W, b);
-            store_reg(Q, base, j, a);
-            store_reg(Q, base, n + r, b);
-        }
-        __syncthreads();
-        n += k;
+    int em = 0;
+    for (long j = tid; j < n; j += CHI_T) em = max(em, err_exp(Q.rerr[base + j]));
+    atomicMax(&s_ctl[0], em);
+    __syncthreads();
+    const int E = s_ctl[0];
+    for (long j = tid; j < n; j += CHI_T) {
+        const double e = Q.rerr[base + j];
+        const int b = err_bin(e, E);
+        atomicAdd(&s_hist[b], err_fixed(e, E));
+        atomicAdd(&s_cnt[b], 1);
     }
+    __syncthreads();
     if (tid == 0) {
-        Q.box[t] = tv;
-        Q.nreg[t] = (int)n;
-        Q.status[t] = status;
+        unsigned long long Qs = 0ull, Cs = 0ull;
+        for (int b = 0; b < CHI_BINS; ++b) Qs += s_hist[b];
+        // from the first occupied bin on, so that a round splits at least one region
+        int b0 = 0;
+        while (b0 < CHI_BINS - 1 && s_cnt[b0] == 0) ++b0;
+        for (int b = 0; b < b0; ++b) Cs += s_hist[b];
+        int bstar = CHI_BINS - 1;
+        for (int b = b0; b < CHI_BINS; ++b) {
+            Cs += s_hist[b];
+            const double rest = ldexp((double)(Qs - Cs), (E > 1 ? E : 1) - 1066);
+            if (rest <= Q.atol || rest <= fabs(tv) * Q.rtol) {
+                bstar = b;
+                break;
+            }
+        }
+        s_ctl[1] = bstar;
     }
+    __syncthreads();
+    const int bstar = s_ctl[1];
+    // selected regions in index order: thread t scans the contiguous range [j0, j1)
+    const long per = (n + CHI_T - 1) / CHI_T;
+    const long j0 = (long)tid * per < n ? (long)tid * per : n;
+    const long j1 = j0 + per < n ? j0 + per : n;
+    long cnt = 0;
+    for (long j = j0; j < j1; ++j) cnt += err_bin(Q.rerr[base + j], E) <= bstar ? 1 : 0;
+    long total;
+    long pos = block_scan(cnt, s_scan, &total);
+    const long room = CHI_MAX_REGIONS - n;
+    const long k = total < room ? total : room;
+    if (n + k > Q.cap) {
+        // this slot is too small: the host runs the task again with more room
+        if (tid == 0) {
+            Q.status[t] = 1;
+            Q.rn[slot] = n;
+            Q.rk[slot] = 0;
+            Q.rdone[slot] = 1;
+        }
+        return;
+    }
+    for (long j = j0; j < j1 && pos < k; ++j)
+        if (err_bin(Q.rerr[base + j], E) <= bstar) Q.rsel[base + pos++] = (int)j;
+    if (tid == 0) {
+        Q.rn[slot] = n;
+        Q.rk[slot] = k;
+    }
+}
+
+// the splits of a round: block b handles selected regions first .. first+63 of slot
+// Q.split_tab[b]; the lower half replaces the region, the upper half is appended at n + r
+__global__ __launch_bounds__(64) void k_chi_split(ChiArgs Q) {
+    __shared__ ChiLds S;
+    load_lds(Q, S);
+    const Meshes<> M{S.mesh, S.mesh + MAXN, S.mesh + 2 * MAXN, S.mesh + 3 * MAXN, S.mesh + 4 * MAXN};
+    const int slot = Q.split_tab[2 * blockIdx.x];
+    const long r = (long)Q.split_tab[2 * blockIdx.x + 1] + threadIdx.x;
+    if (r >= Q.rk[slot]) return;
+    const TaskGeo g = task_geo(Q, Q.tasks[slot]);
+    const long base = (long)slot * Q.cap;
+    const long n = Q.rn[slot];
+    const long j = Q.rsel[base + r];
+    Reg R;
+    R.c0 = Q.rc0[base + j];
+    R.c1 = Q.rc1[base + j];
+    R.h0 = Q.rh0[base + j];
+    R.h1 = Q.rh1[base + j];
+    const int d = Q.rsplit[base + j];
+    Reg a = R, b = R;
+    if (d == 0) {
+        a.h0 = 0.5 * R.h0;
+        b.h0 = a.h0;
+        a.c0 = R.c0 - a.h0;
+        b.c0 = R.c0 + a.h0;
+    } else {
+        a.h1 = 0.5 * R.h1;
+        b.h1 = a.h1;
+        a.c1 = R.c1 - a.h1;
+        b.c1 = R.c1 + a.h1;
+    }
+    eval_region(Q, M, S.sA, S.exch, g.site, g.W, a);
+    eval_region(Q, M, S.sA, S.exch, g.site, g.W, b);
+    store_reg(Q, base, j, a);
+    store_reg(Q, base, n + r, b);
 }
 
 // propagator term of site 0 (correlation_su2.jl:78-82): QuadGK over (-Inf, -2ref, 0, 2ref, Inf)
@@ -291,10 +396,22 @@ __global__ __launch_bounds__(64) void k_chi_prop(ChiArgs Q) {
 
 }  // namespace
 
-hipError_t launch_chi(const ChiArgs &Q, hipStream_t stream) {
-    hipLaunchKernelGGL(k_chi, dim3(Q.nslots), dim3(CHI_T), 0, stream, Q);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || !Q.with_prop) return e;
+hipError_t launch_chi_init(const ChiArgs &Q, hipStream_t stream) {
+    hipLaunchKernelGGL(k_chi_init, dim3(Q.nslots), dim3(64), 0, stream, Q);
+    return hipGetLastError();
+}
+
+hipError_t launch_chi_round(const ChiArgs &Q, int n_active, hipStream_t stream) {
+    hipLaunchKernelGGL(k_chi_round, dim3(n_active), dim3(CHI_T), 0, stream, Q);
+    return hipGetLastError();
+}
+
+hipError_t launch_chi_split(const ChiArgs &Q, int n_blocks, hipStream_t stream) {
+    hipLaunchKernelGGL(k_chi_split, dim3(n_blocks), dim3(64), 0, stream, Q);
+    return hipGetLastError();
+}
+
+hipError_t launch_chi_prop(const ChiArgs &Q, hipStream_t stream) {
     hipLaunchKernelGGL(k_chi_prop, dim3(Q.nchi), dim3(64), 0, stream, Q);
     return hipGetLastError();
 }

@@ -1325,12 +1325,17 @@ int pffrg_compute_chi(pffrg_ctx *c, double L, int32_t a, const double *chi_mesh,
     const int ns = c->sys.ns;
     const long tasks = (long)nchi * ns * 9;
     auto r256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    // per call: box[tasks], nreg[tasks], status[tasks], task list[tasks], prop[nchi], χ mesh[nchi]
-    const size_t fixed = r256(tasks * 8) + 3 * r256(tasks * 4) + 2 * r256((size_t)nchi * 8);
     // every task first runs in a slot of CHI_CAP0 regions; the few that need more run again in
     // slots of the full budget, CHI_POOL_SLOTS at a time (the result does not depend on the slot size)
     const long CHI_CAP0 = 16384, CHI_POOL_SLOTS = 64;
+    const long max_slots = std::max(tasks, CHI_POOL_SLOTS);
     const size_t pool = std::max((size_t)tasks * CHI_CAP0, (size_t)CHI_POOL_SLOTS * CHI_MAX_REGIONS);
+    // split-table entries: at most one per slot plus one per 64 regions
+    const size_t tab_cap = std::max((size_t)tasks * (CHI_CAP0 / 64 + 1), (size_t)CHI_POOL_SLOTS * (CHI_MAX_REGIONS / 64 + 1));
+    // per call: box[tasks], nreg[tasks], status[tasks], prop[nchi], χ mesh[nchi]; per slot: task,
+    // active list, rn, rk, done; the split table; the region pool
+    const size_t fixed = r256(tasks * 8) + 2 * r256(tasks * 4) + 2 * r256((size_t)nchi * 8) +
+                         3 * r256(max_slots * 4) + 2 * r256(max_slots * 8) + r256(tab_cap * 8);
     const size_t need = fixed + 6 * r256(pool * 8) + 2 * r256(pool * 4);
     if (need > c->chi_bytes) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1359,11 +1364,18 @@ int pffrg_compute_chi(pffrg_ctx *c, double L, int32_t a, const double *chi_mesh,
     Q.box = (double *)take(tasks * 8);
     Q.nreg = (int *)take(tasks * 4);
     Q.status = (int *)take(tasks * 4);
-    int *d_tasks = (int *)take(tasks * 4);
-    Q.tasks = d_tasks;
     Q.prop = (double *)take(nchi * 8);
     double *dchi = (double *)take(nchi * 8);
     Q.chi = dchi;
+    int *d_tasks = (int *)take(max_slots * 4);
+    int *d_active = (int *)take(max_slots * 4);
+    Q.rdone = (int *)take(max_slots * 4);
+    Q.rn = (long *)take(max_slots * 8);
+    Q.rk = (long *)take(max_slots * 8);
+    int *d_tab = (int *)take(tab_cap * 8);
+    Q.tasks = d_tasks;
+    Q.active = d_active;
+    Q.split_tab = d_tab;
     Q.rc0 = (double *)take(pool * 8);
     Q.rc1 = (double *)take(pool * 8);
     Q.rh0 = (double *)take(pool * 8);
@@ -1373,30 +1385,61 @@ int pffrg_compute_chi(pffrg_ctx *c, double L, int32_t a, const double *chi_mesh,
     Q.rsplit = (int *)take(pool * 4);
     Q.rsel = (int *)take(pool * 4);
     if ((size_t)(p - (char *)c->chi_mem) > c->chi_bytes) return fail(c, PFFRG_ERR_LIMIT, "chi scratch layout overflow");
+    HIPCHK(c, hipMemcpyAsync(dchi, chi_mesh, nchi * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_chi_prop(Q, c->stream));
+    // the rounds of one pass over a list of tasks with slots of cap regions
+    std::vector<int> act, next, tab, done;
+    std::vector<long> rk;
+    auto run_pass = [&](const int *list, int nslots, long cap) -> int {
+        Q.cap = cap;
+        Q.nslots = nslots;
+        HIPCHK(c, hipMemcpyAsync(d_tasks, list, nslots * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, launch_chi_init(Q, c->stream));
+        act.resize(nslots);
+        for (int k = 0; k < nslots; ++k) act[k] = k;
+        rk.assign(nslots, 0);
+        done.assign(nslots, 0);
+        long rounds = 0;
+        while (!act.empty()) {
+            // every round adds a region to each unfinished slot, so cap + 1 rounds end any pass
+            if (++rounds > cap + 1) return fail(c, PFFRG_ERR_STATE, "chi cubature made no progress");
+            HIPCHK(c, hipMemcpyAsync(d_active, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, launch_chi_round(Q, (int)act.size(), c->stream));
+            HIPCHK(c, hipMemcpyAsync(rk.data(), Q.rk, nslots * sizeof(long), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipMemcpyAsync(done.data(), Q.rdone, nslots * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            next.clear();
+            tab.clear();
+            for (int sl : act) {
+                if (done[sl]) continue;
+                next.push_back(sl);
+                for (long f = 0; f < rk[sl]; f += 64) {
+                    tab.push_back(sl);
+                    tab.push_back((int)f);
+                }
+            }
+            if (tab.size() / 2 > tab_cap) return fail(c, PFFRG_ERR_LIMIT, "chi split table overflow");
+            if (!tab.empty()) {
+                HIPCHK(c, hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+                HIPCHK(c, launch_chi_split(Q, (int)(tab.size() / 2), c->stream));
+            }
+            act.swap(next);
+        }
+        return 0;
+    };
     std::vector<int> list(tasks);
     for (long t = 0; t < tasks; ++t) list[t] = (int)t;
-    HIPCHK(c, hipMemcpyAsync(dchi, chi_mesh, nchi * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(d_tasks, list.data(), tasks * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    Q.cap = CHI_CAP0;
-    Q.nslots = (int)tasks;
-    Q.with_prop = true;
-    HIPCHK(c, launch_chi(Q, c->stream));
+    if ((rc = run_pass(list.data(), (int)tasks, CHI_CAP0))) return rc;
     std::vector<double> box(tasks), prop(nchi);
     std::vector<int> nr(tasks), st(tasks);
-    HIPCHK(c, hipMemcpyAsync(st.data(), Q.status, tasks * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(st.data(), Q.status, tasks * sizeof(int), hipMemcpyDeviceToHost));
     // tasks whose regions outgrew the first slot size: again with the full budget
     std::vector<int> again;
     for (long t = 0; t < tasks; ++t)
         if (st[t] != 0) again.push_back((int)t);
-    Q.cap = CHI_MAX_REGIONS;
-    Q.with_prop = false;
     for (size_t b0 = 0; b0 < again.size(); b0 += CHI_POOL_SLOTS) {
         const int nb = (int)std::min((size_t)CHI_POOL_SLOTS, again.size() - b0);
-        HIPCHK(c, hipMemcpy(d_tasks, again.data() + b0, nb * sizeof(int), hipMemcpyHostToDevice));
-        Q.nslots = nb;
-        HIPCHK(c, launch_chi(Q, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if ((rc = run_pass(again.data() + b0, nb, CHI_MAX_REGIONS))) return rc;
     }
     HIPCHK(c, hipMemcpyAsync(box.data(), Q.box, tasks * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(nr.data(), Q.nreg, tasks * sizeof(int), hipMemcpyDeviceToHost, c->stream));

@@ -21,9 +21,13 @@ struct ChiArgs {
     double L, atol, rtol;
     int nchi;
     long cap;            // region capacity of one task slot
-    int nslots;          // task slots of this launch
+    int nslots;          // task slots of this pass
     const int *tasks;    // [nslots] task of each slot, task = (w * ns + site) * 9 + box
-    bool with_prop;      // also run the site-0 propagator term
+    const int *active;   // [unfinished slots] slot of each k_chi_round block
+    const int *split_tab;  // [2 * blocks] (slot, first selected region) of each k_chi_split block
+    // per slot: regions so far, regions selected for splitting this round, finished flag
+    long *rn, *rk;
+    int *rdone;
     // cubature regions of every slot, [slot][cap]
     double *rc0, *rc1, *rh0, *rh1, *rval, *rerr;
     int *rsplit, *rsel;
@@ -33,7 +37,11 @@ struct ChiArgs {
     double *prop;        // [nchi] propagator term of site 0
 };
 
-// k_chi over the nslots task slots, then (with_prop) the propagator term
-hipError_t launch_chi(const ChiArgs &Q, hipStream_t stream);
+// first regions of the nslots slots; one round of n_active unfinished slots; the splits of a
+// round over n_blocks blocks of 64 selected regions; the site-0 propagator term
+hipError_t launch_chi_init(const ChiArgs &Q, hipStream_t stream);
+hipError_t launch_chi_round(const ChiArgs &Q, int n_active, hipStream_t stream);
+hipError_t launch_chi_split(const ChiArgs &Q, int n_blocks, hipStream_t stream);
+hipError_t launch_chi_prop(const ChiArgs &Q, hipStream_t stream);
 
 }  // namespace pffrg
