@@ -5,9 +5,9 @@
 // the vectorised schedule of Cubature's hcubature_v: every round splits at once the
 // largest-error regions that must be refined for the error of the others to meet the
 // tolerance.  A round is two launches over all unfinished tasks: k_chi_round (one 256-thread
-// block per task: totals, convergence, selection) and k_chi_split (one thread per selected
-// region, over as many blocks as the round has splits, evaluating the 2 x 17 Genz-Malik points
-// of its halves), so that a task with many regions spreads over the whole GPU.  Regions live in
+// block per task: totals, convergence, selection) and k_chi_split (one thread per Genz-Malik
+// point of each half of each selected region, over as many blocks as the round has splits), so
+// that a task with many regions spreads over the whole GPU.  Regions live in
 // global memory; the totals are block sums in a fixed order, the selection takes whole
 // quarter-octave error bins (fixed-point bin sums in LDS), and new regions are appended in
 // index order, all as in the oracle's restatement.  This file is compiled without FMA
@@ -167,17 +167,6 @@ __device__ long block_scan(long v, long *s_scan, long *total) {
     return incl - v;
 }
 
-__device__ __forceinline__ void eval_region(const ChiArgs &Q, const Meshes<> &M, const double *sA, const int *exch,
-                                            int site, double W, Reg &R) {
-    double f[17];
-    for (int k = 0; k < 17; ++k) {
-        double xx, yy;
-        gm_point(R, k, xx, yy);
-        f[k] = chi_point(Q, M, sA, exch, site, W, xx, yy);
-    }
-    gm_combine(R, f);
-}
-
 // the LDS copies every χ kernel reads: meshes, Σ of the action, the exchange table
 struct ChiLds {
     double mesh[5 * MAXN];
@@ -328,41 +317,65 @@ __global__ __launch_bounds__(CHI_T) void k_chi_round(ChiArgs Q) {
     }
 }
 
-// the splits of a round: block b handles selected regions first .. first+63 of slot
-// Q.split_tab[b]; the lower half replaces the region, the upper half is appended at n + r
-__global__ __launch_bounds__(64) void k_chi_split(ChiArgs Q) {
+// the splits of a round: block b takes CHI_RPB selected regions of slot Q.split_tab[2b] from
+// Q.split_tab[2b+1] on; one thread per rule point of each half (CHI_RPB x 34 threads), then one
+// thread per half combines its 17 values; the lower half replaces the region, the upper half is
+// appended at n + r
+__global__ __launch_bounds__(CHI_SPLIT_T) void k_chi_split(ChiArgs Q) {
     __shared__ ChiLds S;
+    __shared__ double s_f[CHI_RPB * 34];
+    __shared__ Reg s_par[CHI_RPB];
+    __shared__ long s_j[CHI_RPB];
     load_lds(Q, S);
     const Meshes<> M{S.mesh, S.mesh + MAXN, S.mesh + 2 * MAXN, S.mesh + 3 * MAXN, S.mesh + 4 * MAXN};
     const int slot = Q.split_tab[2 * blockIdx.x];
-    const long r = (long)Q.split_tab[2 * blockIdx.x + 1] + threadIdx.x;
-    if (r >= Q.rk[slot]) return;
-    const TaskGeo g = task_geo(Q, Q.tasks[slot]);
+    const long first = Q.split_tab[2 * blockIdx.x + 1];
+    const long k = Q.rk[slot];
     const long base = (long)slot * Q.cap;
-    const long n = Q.rn[slot];
-    const long j = Q.rsel[base + r];
-    Reg R;
-    R.c0 = Q.rc0[base + j];
-    R.c1 = Q.rc1[base + j];
-    R.h0 = Q.rh0[base + j];
-    R.h1 = Q.rh1[base + j];
-    const int d = Q.rsplit[base + j];
-    Reg a = R, b = R;
-    if (d == 0) {
-        a.h0 = 0.5 * R.h0;
-        b.h0 = a.h0;
-        a.c0 = R.c0 - a.h0;
-        b.c0 = R.c0 + a.h0;
-    } else {
-        a.h1 = 0.5 * R.h1;
-        b.h1 = a.h1;
-        a.c1 = R.c1 - a.h1;
-        b.c1 = R.c1 + a.h1;
+    const int tid = threadIdx.x;
+    if (tid < CHI_RPB && first + tid < k) {
+        const long j = Q.rsel[base + first + tid];
+        Reg R;
+        R.c0 = Q.rc0[base + j];
+        R.c1 = Q.rc1[base + j];
+        R.h0 = Q.rh0[base + j];
+        R.h1 = Q.rh1[base + j];
+        R.split = Q.rsplit[base + j];
+        s_par[tid] = R;
+        s_j[tid] = j;
     }
-    eval_region(Q, M, S.sA, S.exch, g.site, g.W, a);
-    eval_region(Q, M, S.sA, S.exch, g.site, g.W, b);
-    store_reg(Q, base, j, a);
-    store_reg(Q, base, n + r, b);
+    __syncthreads();
+    const TaskGeo g = task_geo(Q, Q.tasks[slot]);
+    auto half = [&](int q, int h) {
+        const Reg &R = s_par[q];
+        Reg X = R;
+        if (R.split == 0) {
+            X.h0 = 0.5 * R.h0;
+            X.c0 = h ? R.c0 + X.h0 : R.c0 - X.h0;
+        } else {
+            X.h1 = 0.5 * R.h1;
+            X.c1 = h ? R.c1 + X.h1 : R.c1 - X.h1;
+        }
+        return X;
+    };
+    {
+        const int q = tid / 34, rem = tid % 34;
+        if (q < CHI_RPB && first + q < k) {
+            const Reg X = half(q, rem / 17);
+            double xx, yy;
+            gm_point(X, rem % 17, xx, yy);
+            s_f[tid] = chi_point(Q, M, S.sA, S.exch, g.site, g.W, xx, yy);
+        }
+    }
+    __syncthreads();
+    if (tid < 2 * CHI_RPB) {
+        const int q = tid / 2, h = tid % 2;
+        if (first + q < k) {
+            Reg X = half(q, h);
+            gm_combine(X, s_f + q * 34 + h * 17);
+            store_reg(Q, base, h ? Q.rn[slot] + first + q : s_j[q], X);
+        }
+    }
 }
 
 // propagator term of site 0 (correlation_su2.jl:78-82): QuadGK over (-Inf, -2ref, 0, 2ref, Inf)
@@ -407,7 +420,7 @@ hipError_t launch_chi_round(const ChiArgs &Q, int n_active, hipStream_t stream) 
 }
 
 hipError_t launch_chi_split(const ChiArgs &Q, int n_blocks, hipStream_t stream) {
-    hipLaunchKernelGGL(k_chi_split, dim3(n_blocks), dim3(64), 0, stream, Q);
+    hipLaunchKernelGGL(k_chi_split, dim3(n_blocks), dim3(CHI_SPLIT_T), 0, stream, Q);
     return hipGetLastError();
 }
 

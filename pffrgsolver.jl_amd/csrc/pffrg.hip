@@ -1330,8 +1330,9 @@ int pffrg_compute_chi(pffrg_ctx *c, double L, int32_t a, const double *chi_mesh,
     const long CHI_CAP0 = 16384, CHI_POOL_SLOTS = 64;
     const long max_slots = std::max(tasks, CHI_POOL_SLOTS);
     const size_t pool = std::max((size_t)tasks * CHI_CAP0, (size_t)CHI_POOL_SLOTS * CHI_MAX_REGIONS);
-    // split-table entries: at most one per slot plus one per 64 regions
-    const size_t tab_cap = std::max((size_t)tasks * (CHI_CAP0 / 64 + 1), (size_t)CHI_POOL_SLOTS * (CHI_MAX_REGIONS / 64 + 1));
+    // split-table entries: at most one per slot plus one per CHI_RPB regions
+    const size_t tab_cap = std::max((size_t)tasks * (CHI_CAP0 / CHI_RPB + 1),
+                                    (size_t)CHI_POOL_SLOTS * (CHI_MAX_REGIONS / CHI_RPB + 1));
     // per call: box[tasks], nreg[tasks], status[tasks], prop[nchi], χ mesh[nchi]; per slot: task,
     // active list, rn, rk, done; the split table; the region pool
     const size_t fixed = r256(tasks * 8) + 2 * r256(tasks * 4) + 2 * r256((size_t)nchi * 8) +
@@ -1413,7 +1414,7 @@ int pffrg_compute_chi(pffrg_ctx *c, double L, int32_t a, const double *chi_mesh,
             for (int sl : act) {
                 if (done[sl]) continue;
                 next.push_back(sl);
-                for (long f = 0; f < rk[sl]; f += 64) {
+                for (long f = 0; f < rk[sl]; f += CHI_RPB) {
                     tab.push_back(sl);
                     tab.push_back((int)f);
                 }

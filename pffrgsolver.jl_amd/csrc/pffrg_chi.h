@@ -11,6 +11,8 @@ namespace pffrg {
 constexpr long CHI_MAX_EVALS = 10000000;                       // maxevals of integrate_χ_boxes
 constexpr long CHI_MAX_REGIONS = 1 + (CHI_MAX_EVALS - 17) / 34;  // regions that budget allows
 constexpr int CHI_T = 256;                                       // threads per cubature task
+constexpr int CHI_RPB = 7;                                       // selected regions per split block
+constexpr int CHI_SPLIT_T = 256;                                 // >= 34 CHI_RPB threads
 
 struct ChiArgs {
     Sys y;
@@ -24,7 +26,7 @@ struct ChiArgs {
     int nslots;          // task slots of this pass
     const int *tasks;    // [nslots] task of each slot, task = (w * ns + site) * 9 + box
     const int *active;   // [unfinished slots] slot of each k_chi_round block
-    const int *split_tab;  // [2 * blocks] (slot, first selected region) of each k_chi_split block
+    const int *split_tab;  // [2 * blocks] (slot, first of CHI_RPB selected regions) of each k_chi_split block
     // per slot: regions so far, regions selected for splitting this round, finished flag
     long *rn, *rk;
     int *rdone;
@@ -38,7 +40,7 @@ struct ChiArgs {
 };
 
 // first regions of the nslots slots; one round of n_active unfinished slots; the splits of a
-// round over n_blocks blocks of 64 selected regions; the site-0 propagator term
+// round over n_blocks blocks of CHI_RPB selected regions; the site-0 propagator term
 hipError_t launch_chi_init(const ChiArgs &Q, hipStream_t stream);
 hipError_t launch_chi_round(const ChiArgs &Q, int n_active, hipStream_t stream);
 hipError_t launch_chi_split(const ChiArgs &Q, int n_blocks, hipStream_t stream);
