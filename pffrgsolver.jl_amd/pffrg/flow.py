@@ -346,20 +346,29 @@ def launch_flow(ctx: Context, prob: Problem, a0: Action, Li: float, Lf: float, l
     return f.result()
 
 
-def launch_flow_batch(ctxs, probs, a0s, Li, Lf, max_steps: int | None = None, **kw) -> list[FlowResult]:
+def launch_flow_batch(ctxs, probs, a0s, Li, Lf, max_steps: int | None = None, progress: int = 0,
+                      **kw) -> list[FlowResult]:
     """1-loop BS3 flows of several problems advanced together: every stage evaluates the
     right-hand sides of all flows still running in one batched launch (pffrg_batch_rhs_1l).
-    Each flow keeps its own Λ, step size and acceptance, exactly as launch_flow would."""
+    Each flow keeps its own Λ, step size and acceptance, exactly as launch_flow would.  With
+    progress = k > 0, a line goes to stderr every k batched steps."""
     n = len(ctxs)
     Lis = Li if np.ndim(Li) else [Li] * n
     Lfs = Lf if np.ndim(Lf) else [Lf] * n
     flows = [BS3Flow(ctxs[k], probs[k], a0s[k], Lis[k], Lfs[k], 1, max_steps=max_steps, **kw) for k in range(n)]
     batch, members = None, None
+    step = 0
+    t0 = time.time()
     try:
         while True:
             live = [k for k in range(n) if flows[k].begin_step()]
             if not live:
                 break
+            step += 1
+            if progress and step % progress == 0:
+                Ls = [flows[k].L for k in live]
+                print(f"batched step {step}: {len(live)} flows running, Lambda {min(Ls):.4g}..{max(Ls):.4g}, "
+                      f"{time.time() - t0:.1f} s", file=sys.stderr, flush=True)
             if members != live:
                 if batch is not None:
                     batch.close()
