@@ -63,5 +63,15 @@ class OracleContext:
     def compute_dgamma_1l(self, L, a, da, eval_, tol=(1e-8, 1e-4)):
         self.orc.compute_dgamma_1l(L, self.slots[a], self.slots[da], eval_, tol)
 
+    def compute_dgamma_2l(self, L, a, da, dal, eval_, tol=(1e-8, 1e-4)):
+        self.orc.compute_dgamma_2l(L, self.slots[a], self.slots[da], self.slots[dal], eval_, tol)
+
+    def compute_dgamma_ml(self, L, loops, a, da, dal, dac, datemp, daS, eval_, tol=(1e-8, 1e-4)):
+        sl = self.slots
+        self.orc.compute_dgamma_ml(L, loops, sl[a], sl[da], sl[dal], sl[dac], sl[datemp], sl[daS], eval_, tol)
+
+    def compute_dsigma_corr(self, L, a, da, daS, tol=(1e-8, 1e-4)):
+        self.orc.compute_dsigma_corr(L, self.slots[a], self.slots[da], self.slots[daS], tol)
+
     def compute_chi(self, L, a, tol=(1e-8, 1e-4)):
         return self.orc.compute_chi(L, self.slots[a], tol)

@@ -45,6 +45,23 @@ def test_flow_matches_oracle_flow():
     assert not np.array_equal(res.meshes[-1].Os, prob.mesh.Os)
 
 
+def test_multiloop_flow_matches_oracle_flow():
+    # 3-loop flow with self-energy corrections (launcher_ml.jl:78-198) for a few steps
+    prob, a0 = _flow_setup("square", 2, 6)
+    with prob.context(n_slots=9) as ctx:
+        res = F.launch_flow(ctx, prob, a0, 5.0, 1.0, loops=3, max_steps=3)
+        got = ctx.download(F.A)
+    octx = OracleContext(prob)
+    ref_res = F.launch_flow(octx, prob, a0, 5.0, 1.0, loops=3, max_steps=3)
+    ref = octx.download(F.A)
+    print(f"3-loop: GPU Λ {res.Lambdas}; oracle Λ {ref_res.Lambdas}", flush=True)
+    assert res.accepted == ref_res.accepted and res.rejected == ref_res.rejected
+    np.testing.assert_allclose(res.Lambdas, ref_res.Lambdas, rtol=1e-12)
+    e = max(rel_err(got.gamma, ref.gamma), rel_err(got.Sigma, ref.Sigma))
+    print(f"3-loop final action rel err {e:.3e}", flush=True)
+    assert e < 1e-10
+
+
 def test_batched_flows_equal_single_flows():
     setups = [_flow_setup("square", 2, 6), _flow_setup("kagome", 3, 6), _flow_setup("square", 3, 6, ((1.0,), (0.4,)))]
     single = []
