@@ -10,6 +10,7 @@ import pytest
 
 from pffrg import action as A
 from pffrg import flow as F
+from pffrg import native
 
 from helpers import rel_err, small_problem
 from oracle_ctx import OracleContext
@@ -60,6 +61,23 @@ def test_multiloop_flow_matches_oracle_flow():
     e = max(rel_err(got.gamma, ref.gamma), rel_err(got.Sigma, ref.Sigma))
     print(f"3-loop final action rel err {e:.3e}", flush=True)
     assert e < 1e-10
+
+
+def test_fp32_mixed_flow_chi_within_tolerance():
+    # the headline tolerance: χ of the FP32-mixed flow against the FP64 flow, rel err <= 1e-4
+    prob, a0 = _flow_setup("square", 3, 8)
+    runs = {}
+    for mode in (native.FP64, native.FP32_MIXED):
+        with prob.context(n_slots=9, precision=mode) as ctx:
+            runs[mode] = F.launch_flow(ctx, prob, a0, 5.0, 0.5, max_steps=12)
+    r64, r32 = runs[native.FP64], runs[native.FP32_MIXED]
+    print(f"FP64 Λ {r64.Lambdas}\nFP32-mixed Λ {r32.Lambdas}", flush=True)
+    common = [(i, j) for i, L in enumerate(r64.Lambdas) for j, L2 in enumerate(r32.Lambdas)
+              if abs(L - L2) <= 1e-12 * L]
+    assert len(common) >= min(len(r64.Lambdas), len(r32.Lambdas)) - 1 >= 3
+    err = max(rel_err(r32.chi[j], r64.chi[i]) for i, j in common)
+    print(f"χ FP32-mixed vs FP64 over {len(common)} steps: {err:.2e}", flush=True)
+    assert err <= 1e-4
 
 
 def test_batched_flows_equal_single_flows():
