@@ -144,7 +144,8 @@ int pffrg_get_timing(pffrg_ctx *ctx, double *total_ms, int64_t *launches);
  * correlations chi[i + n_sites*k] of the irreducible sites i at the n_chi frequencies
  * chi_mesh[k] (Mesh.χ), from slot a at Lambda, to the tolerance (atol, rtol) of χ_tol.  The
  * nine boxes of integrate_χ_boxes (correlation_lib.jl:2-30) are integrated by adaptive 2-D
- * cubature (degree-7 Genz-Malik rule), at most 8192 regions per box; nreg (optional,
+ * cubature with hcubature_v's schedule (degree-7 Genz-Malik rule, several regions split per
+ * round, at most 10^7 evaluations per box, as the reference's call); nreg (optional,
  * 9*n_sites*n_chi entries) receives the final region count of every box. */
 int pffrg_compute_chi(pffrg_ctx *ctx, double Lambda, int32_t a, const double *chi_mesh, int32_t n_chi,
                       const double *chi_tol, double *chi, int32_t *nreg);
