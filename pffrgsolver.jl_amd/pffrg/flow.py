@@ -39,10 +39,10 @@ class Problem:
 
     @classmethod
     def build(cls, name, size, J, initial, num_sigma=25, num_Omega=15, num_nu=10, num_chi=10, S=0.5, A=0.0,
-              normalize=True):
+              normalize=True, model="heisenberg"):
         if normalize:
             J, A = LT.normalize_couplings(J, A)
-        l, r = LT.build(name, size, J)
+        l, r = LT.build(name, size, J, model)
         m = M.launch_mesh(initial, num_sigma, num_Omega, num_nu, num_chi)
         bare = LT.bare_couplings(l, r, S, A)
         return cls(name, size, J, S, A, m, l, r, bare)

@@ -3,7 +3,7 @@
 Produces the tables the flow kernels consume: the irreducible sites, their
 multiplicities, the site-exchange images and the overlap rows of the
 t-channel site sum.  Follows src/Lattice/{unitcell,site,bond,build,reduced}.jl
-and src/Lattice/model_lib/model_heisenberg.jl of PFFRGSolver.jl; tables are
+and src/Lattice/model_lib/model_{heisenberg,breathing}.jl of PFFRGSolver.jl; tables are
 returned 0-based (the reference's are 1-based).
 
 Site coordinates are matched through a hash of coordinates rounded to 1e-6
@@ -275,25 +275,78 @@ def get_nbs(n: int, i: int, vecs: np.ndarray) -> np.ndarray:
     return np.nonzero(np.abs(dn - d) <= 1e-8)[0]
 
 
+def _add_heisenberg(l: Lattice, i: int, j: int, c: float) -> None:
+    for k in range(3):
+        l.bonds[i, j, k, k] += c
+
+
+def _add_shell(l: Lattice, i: int, n: int, Jn, nbs) -> None:
+    """n-th neighbour shell of site i: uniform if Jn has one value, else one value per bond
+    distance class in ascending order (model_heisenberg.jl:31-52, model_breathing.jl:50-71)."""
+    if len(Jn) == 1:
+        for j in nbs:
+            _add_heisenberg(l, i, j, Jn[0])
+    else:
+        dist = [get_metric(l.ints[j], l.ints[i], l.uc) for j in nbs]
+        kinds = sorted(set(dist))
+        assert len(Jn) == len(kinds), (
+            f"{l.name} has {len(kinds)} inequivalent {n}-th nearest neighbors, but {len(Jn)} couplings were supplied.")
+        for nk, kind in enumerate(kinds):
+            for j, dj in zip(nbs, dist):
+                if dj == kind:
+                    _add_heisenberg(l, i, j, Jn[nk])
+
+
 def init_model_heisenberg(J: list[list[float]], l: Lattice) -> None:
     """Heisenberg couplings J[n] on n-th neighbour shells (model_heisenberg.jl:13-56)."""
     for i in range(len(l.ints)):
         for n, Jn in enumerate(J, start=1):
+            _add_shell(l, i, n, Jn, get_nbs(n, i, l.vecs))
+
+
+def init_model_breathing(J: list[list[float]], l: Lattice) -> None:
+    """Breathing kagome / pyrochlore Heisenberg model (model_breathing.jl:14-76): J[0] =
+    [J_a, J_b], J_a on nearest-neighbour bonds inside the unit cell (up triangles/tetrahedra),
+    J_b on those that cross it; further shells as in the Heisenberg model."""
+    assert l.name in ("kagome", "pyrochlore"), "Breathing model requires Pyrochlore or Kagome lattice."
+    assert len(J[0]) == 2, "Breathing model needs two nearest neighbor couplings."
+    for i in range(len(l.ints)):
+        for n, Jn in enumerate(J, start=1):
             nbs = get_nbs(n, i, l.vecs)
-            if len(Jn) == 1:
+            if n == 1:
                 for j in nbs:
-                    for k in range(3):
-                        l.bonds[i, j, k, k] += Jn[0]
+                    inside = tuple(l.ints[j][:3]) == tuple(l.ints[i][:3])
+                    _add_heisenberg(l, i, j, Jn[0] if inside else Jn[1])
             else:
-                dist = [get_metric(l.ints[j], l.ints[i], l.uc) for j in nbs]
-                kinds = sorted(set(dist))
-                assert len(Jn) == len(kinds), (
-                    f"{l.name} has {len(kinds)} inequivalent {n}-th nearest neighbors, but {len(Jn)} couplings were supplied.")
-                for nk, kind in enumerate(kinds):
-                    for j, dj in zip(nbs, dist):
-                        if dj == kind:
-                            for k in range(3):
-                                l.bonds[i, j, k, k] += Jn[nk]
+                _add_shell(l, i, n, Jn, nbs)
+
+
+def init_model_pyrochlore_breathing_c3(J: list[list[float]], l: Lattice) -> None:
+    """Breathing pyrochlore with broken C3 (model_breathing.jl:91-182): J[0] = [J_a, J_b, δ_a,
+    δ_b]; a nearest-neighbour bond inside (across) the unit cell carries J_a ± δ_a (J_b ± δ_b),
+    + on bonds joining basis sites 1-2 or 3-4, − on the others."""
+    assert l.name == "pyrochlore", "Model requires Pyrochlore lattice."
+    assert len(J[0]) == 4, "Model requires two nearest neighbor couplings and two C3 breaking perturbations."
+    for i in range(len(l.ints)):
+        for n, Jn in enumerate(J, start=1):
+            nbs = get_nbs(n, i, l.vecs)
+            if n == 1:
+                for j in nbs:
+                    inside = tuple(l.ints[j][:3]) == tuple(l.ints[i][:3])
+                    pair = {l.ints[j][3], l.ints[i][3]}
+                    plus = pair == {0, 1} or pair == {2, 3}  # basis sites 1-2 or 3-4 (1-based)
+                    Jc, dc = (Jn[0], Jn[2]) if inside else (Jn[1], Jn[3])
+                    _add_heisenberg(l, i, j, Jc + dc if plus else Jc - dc)
+            else:
+                _add_shell(l, i, n, Jn, nbs)
+
+
+# SU(2) models of init_model! (build.jl:183-202); triangular-dm-c3 needs the U(1)-DM action, not built
+MODELS = {
+    "heisenberg": init_model_heisenberg,
+    "breathing": init_model_breathing,
+    "pyrochlore-breathing-c3": init_model_pyrochlore_breathing_c3,
+}
 
 
 def _are_equal(b1, b2) -> bool:
@@ -520,9 +573,9 @@ class ReducedLattice:
 
 def get_reduced_lattice(model: str, J: list[list[float]], l: Lattice) -> ReducedLattice:
     """reduced.jl:716-759."""
-    if model != "heisenberg":
-        raise NotImplementedError(f"Model {model} unknown or not provided.")
-    init_model_heisenberg(J, l)
+    if model not in MODELS:
+        raise NotImplementedError(f"Model {model} unknown or not provided (SU(2) models: {', '.join(MODELS)}).")
+    MODELS[model](J, l)
     reduced = get_reduced(l)
     irreducible = np.array(list(dict.fromkeys(reduced.tolist())), dtype=np.int64)
     pos = {int(v): k for k, v in enumerate(irreducible)}

@@ -79,6 +79,29 @@ def test_rhs_1l(name, size, L):
     assert st["nodes"] == nodes_ref
 
 
+@pytest.mark.parametrize("model,J", [("breathing", [[1.0, 0.6]]), ("pyrochlore-breathing-c3", [[1.0, 0.6, 0.1, 0.05]])])
+def test_rhs_1l_breathing_models(model, J):
+    # lattice tables of the lower-symmetry SU(2) models (model_breathing.jl) through the same passes
+    name = "kagome" if model == "breathing" else "pyrochlore"
+    prob = F.Problem.build(name, 2, J, initial=1.0, num_sigma=8, num_Omega=6, num_nu=6, normalize=False, model=model)
+    ns, m = prob.r.n_sites, prob.mesh
+    a = A.synthetic_action(ns, m, prob.bare, seed=9)
+    orc = O.Oracle(prob.r, m)
+    ref = _empty(ns, m)
+    orc.compute_dsigma(1.0, a, ref)
+    nodes_ref = orc.compute_dgamma_1l(1.0, a, ref, 10)
+    with prob.context(n_slots=2) as ctx:
+        ctx.upload(0, a)
+        ctx.reset_stats()
+        F.rhs(ctx, 1.0, 1, 0, 1, 10)
+        st = ctx.stats()
+        got = ctx.download(1)
+    errs = per_array_err(got, ref)
+    _print(f"1l {model} {name} L=2 ({ns} sites)", errs)
+    assert rel_err(got.Sigma, ref.Sigma) < TOL64 and max(errs.values()) < TOL64
+    assert st["nodes"] == nodes_ref
+
+
 @pytest.mark.parametrize("name,size", [("square", 2), ("kagome", 3)])
 def test_rhs_2l(name, size):
     prob, ns, m, a = _setup(name, size, N=6, seed=5)
