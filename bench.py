@@ -151,6 +151,7 @@ def main():
     ap.add_argument("--flow-modes", default="fp64,fp32", help="flow: precisions to run, in order")
     ap.add_argument("--flow-max-seconds", type=float, default=None, help="flow: wall-time budget per flow")
     ap.add_argument("--flow-verbose", action="store_true", help="flow: print every BS3 step")
+    ap.add_argument("--batch-N", type=int, default=32, help="batch / batchflow: mesh size of every problem")
     ap.add_argument("--workload", choices=["single", "batch", "batchflow", "flow", "warmstart"], default="single",
                     help="single: the 2-loop square L=6 RHS (headline); batch: the J1-J2 sweep + uneven "
                          "truncations as one batched 1-loop RHS")
@@ -441,13 +442,14 @@ def warmstart_workload(args):
 
 def batch_workload(args):
     """16 square L=5 J1-J2 problems (J2/J1 = k/15) and square L=3..7 at J2/J1 = 0.5, 1-loop,
-    N=32 (33 points/axis), couplings normalised per problem, one RHS each at Λ=1 with a seeded
+    N=32 by default (--batch-N), couplings normalised per problem, one RHS each at Λ=1 with a seeded
     synthetic vertex: all 21 right-hand sides as one batched launch, against the same 21
     evaluated one problem at a time."""
     from pffrg.native import Batch
     specs = [(5, k / 15.0) for k in range(16)] + [(L, 0.5) for L in (3, 4, 5, 6, 7)]
     t0 = time.time()
-    probs = [F.Problem.build("square", L, [[1.0], [j2]], initial=1.0, num_sigma=32, num_Omega=32, num_nu=32,
+    N = args.batch_N
+    probs = [F.Problem.build("square", L, [[1.0], [j2]], initial=1.0, num_sigma=N, num_Omega=N, num_nu=N,
                              num_chi=10) for L, j2 in specs]
     acts = [A.synthetic_action(p.r.n_sites, p.mesh, p.bare, seed=SEED + k) for k, p in enumerate(probs)]
     log(f"batch workload: {len(probs)} problems, sites {[p.r.n_sites for p in probs]} (setup {time.time() - t0:.1f}s)")
@@ -484,7 +486,7 @@ def batch_workload(args):
         "vs_baseline": None, "dtype": "fp32-mixed (fp32 interpolation/products, fp64 quadrature)",
         "data": "synthetic (seeded N(0,1) vertex with 1/(1+|W|+|v|+|v'|) decay)",
         "config": {"workload": "21 x 1-loop RHS: square L=5 J2/J1=k/15 (k=0..15) + square L=3..7 J2/J1=0.5, "
-                               "N=32 (+1 points/axis), Lambda=1, eval=10, one batched launch",
+                               f"N={N} (+1 points/axis), Lambda=1, eval=10, one batched launch",
                    "problems": n, "sites": [p.r.n_sites for p in probs]},
         "sequential_rhs_per_s": n * args.steps / dt_seq, "batch_speedup": dt_seq / dt_b,
         "batched_equals_sequential": bool(same),
@@ -496,11 +498,13 @@ def batch_workload(args):
 
 def batchflow_workload(args):
     """The batched configuration as full flows: 16 square L=5 J1-J2 problems (J2/J1 = k/15) and
-    square L=3..7 at J2/J1 = 0.5, 1-loop, N=32, Λ 50 -> 0.3 (couplings normalised per problem),
+    square L=3..7 at J2/J1 = 0.5, 1-loop, N=32 by default (--batch-N), Λ 50 -> 0.3 (couplings
+    normalised per problem),
     BS3 with χ and mesh rescanning after every accepted step, FP32-mixed: all 21 flows advanced
     together (one batched launch per BS3 stage) against the same flows run one at a time."""
     specs = [(5, k / 15.0) for k in range(16)] + [(L, 0.5) for L in (3, 4, 5, 6, 7)]
-    probs = [F.Problem.build("square", L, [[1.0], [j2]], initial=50.0, num_sigma=32, num_Omega=32, num_nu=32,
+    N = args.batch_N
+    probs = [F.Problem.build("square", L, [[1.0], [j2]], initial=50.0, num_sigma=N, num_Omega=N, num_nu=N,
                              num_chi=10) for L, j2 in specs]
     n = len(probs)
     log(f"batched flows: {n} problems, sites {[p.r.n_sites for p in probs]}")
@@ -533,7 +537,7 @@ def batchflow_workload(args):
         "dtype": "fp32-mixed (fp32 interpolation/products, fp64 quadrature and step control)",
         "data": "bare couplings (deterministic)",
         "config": {"workload": "21 x 1-loop flow: square L=5 J2/J1=k/15 (k=0..15) + square L=3..7 J2/J1=0.5, "
-                               "N=32 (+1 points/axis), Lambda 50 -> 0.3, chi and mesh rescanning every accepted step",
+                               f"N={N} (+1 points/axis), Lambda 50 -> 0.3, chi and mesh rescanning every accepted step",
                    "problems": n, "sites": [p.r.n_sites for p in probs], "max_seconds": args.flow_max_seconds},
         "batched_s": dt_b, "one_at_a_time_s": dt_seq, "batch_speedup": dt_seq / dt_b,
         "rhs_evals": rhs, "aggregate_rhs_per_s": rhs / dt_b, "batched_equals_one_at_a_time": bool(same),
