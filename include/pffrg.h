@@ -195,6 +195,14 @@ const char *pffrg_batch_last_error(const pffrg_batch *batch);
 int pffrg_batch_rhs_1l(pffrg_batch *batch, const double *Lambda, const int32_t *a, const int32_t *da,
                        const int32_t *eval, const double *sigma_tol, const double *gamma_tol);
 
+/* compute_χ! (correlation_su2.jl:58-86) of every problem k of the batch (caller's order): slot
+ * a[k] at Lambda[k], χ frequencies chi_mesh[k][0 .. n_chi[k]-1], result chi[k][i + n_sites*w] as
+ * pffrg_compute_chi; all problems' cubature tasks run through one set of rounds.  Each
+ * problem's result equals its pffrg_compute_chi bitwise.  n_chi[k] = 0 leaves problem k out
+ * (its chi_mesh[k] and chi[k] may then be NULL). */
+int pffrg_batch_compute_chi(pffrg_batch *batch, const double *Lambda, const int32_t *a, const double *const *chi_mesh,
+                            const int32_t *n_chi, const double *chi_tol, double *const *chi);
+
 /* ---- Learned warm start of the parquet iteration (no reference counterpart: the reference
  * starts launch_parquet! from the bare action, Launcher.jl:676-678 and parquet.jl:54).  A
  * multilayer perceptron maps (couplings, Λ) features to coefficients of a fixed basis of

@@ -57,7 +57,7 @@ __device__ __forceinline__ void gm_combine(Reg &R, const double *f) {
 }
 
 // compute_χ_kernel! at (x, y) of the unit square
-__device__ double chi_point(const ChiArgs &Q, const Meshes<> &M, const double *sA, const int *exch, int site,
+__device__ double chi_point(const ChiProb &Q, const Meshes<> &M, const double *sA, const int *exch, int site,
                             double w, double xx, double yy) {
     const Sys &y = Q.y;
     const double L = Q.L;
@@ -174,7 +174,7 @@ struct ChiLds {
     int exch[64];
 };
 
-__device__ __forceinline__ void load_lds(const ChiArgs &Q, ChiLds &S) {
+__device__ __forceinline__ void load_lds(const ChiProb &Q, ChiLds &S) {
     for (int k = threadIdx.x; k < 5 * MAXN; k += blockDim.x) S.mesh[k] = Q.mesh[k];
     for (int k = threadIdx.x; k < Q.y.nsig; k += blockDim.x) S.sA[k] = Q.A.Sigma[k];
     if ((int)threadIdx.x < Q.y.ns) S.exch[threadIdx.x] = Q.exch[threadIdx.x];
@@ -186,7 +186,8 @@ struct TaskGeo {
     double W, lo[2], hi[2];
 };
 
-__device__ __forceinline__ TaskGeo task_geo(const ChiArgs &Q, int t) {
+// geometry of the problem's local task t = (w * ns + site) * 9 + box
+__device__ __forceinline__ TaskGeo task_geo(const ChiProb &Q, int t) {
     TaskGeo g;
     const int box = t % 9, q = t / 9;
     const int w = q / Q.y.ns;
@@ -200,10 +201,12 @@ __device__ __forceinline__ TaskGeo task_geo(const ChiArgs &Q, int t) {
 __global__ __launch_bounds__(64) void k_chi_init(ChiArgs Q) {
     __shared__ ChiLds S;
     __shared__ double s_f[17];
-    load_lds(Q, S);
-    const Meshes<> M{S.mesh, S.mesh + MAXN, S.mesh + 2 * MAXN, S.mesh + 3 * MAXN, S.mesh + 4 * MAXN};
     const int slot = blockIdx.x;
-    const TaskGeo g = task_geo(Q, Q.tasks[slot]);
+    const int t = Q.tasks[slot];
+    const ChiProb &P = Q.probs[Q.task_prob[t]];
+    load_lds(P, S);
+    const Meshes<> M{S.mesh, S.mesh + MAXN, S.mesh + 2 * MAXN, S.mesh + 3 * MAXN, S.mesh + 4 * MAXN};
+    const TaskGeo g = task_geo(P, t - P.task0);
     Reg r0;
     r0.c0 = 0.5 * (g.lo[0] + g.hi[0]);
     r0.c1 = 0.5 * (g.lo[1] + g.hi[1]);
@@ -212,7 +215,7 @@ __global__ __launch_bounds__(64) void k_chi_init(ChiArgs Q) {
     if (threadIdx.x < 17) {
         double xx, yy;
         gm_point(r0, threadIdx.x, xx, yy);
-        s_f[threadIdx.x] = chi_point(Q, M, S.sA, S.exch, g.site, g.W, xx, yy);
+        s_f[threadIdx.x] = chi_point(P, M, S.sA, S.exch, g.site, g.W, xx, yy);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -326,9 +329,11 @@ __global__ __launch_bounds__(CHI_SPLIT_T) void k_chi_split(ChiArgs Q) {
     __shared__ double s_f[CHI_RPB * 34];
     __shared__ Reg s_par[CHI_RPB];
     __shared__ long s_j[CHI_RPB];
-    load_lds(Q, S);
-    const Meshes<> M{S.mesh, S.mesh + MAXN, S.mesh + 2 * MAXN, S.mesh + 3 * MAXN, S.mesh + 4 * MAXN};
     const int slot = Q.split_tab[2 * blockIdx.x];
+    const int tsk = Q.tasks[slot];
+    const ChiProb &P = Q.probs[Q.task_prob[tsk]];
+    load_lds(P, S);
+    const Meshes<> M{S.mesh, S.mesh + MAXN, S.mesh + 2 * MAXN, S.mesh + 3 * MAXN, S.mesh + 4 * MAXN};
     const long first = Q.split_tab[2 * blockIdx.x + 1];
     const long k = Q.rk[slot];
     const long base = (long)slot * Q.cap;
@@ -345,7 +350,7 @@ __global__ __launch_bounds__(CHI_SPLIT_T) void k_chi_split(ChiArgs Q) {
         s_j[tid] = j;
     }
     __syncthreads();
-    const TaskGeo g = task_geo(Q, Q.tasks[slot]);
+    const TaskGeo g = task_geo(P, tsk - P.task0);
     auto half = [&](int q, int h) {
         const Reg &R = s_par[q];
         Reg X = R;
@@ -364,7 +369,7 @@ __global__ __launch_bounds__(CHI_SPLIT_T) void k_chi_split(ChiArgs Q) {
             const Reg X = half(q, rem / 17);
             double xx, yy;
             gm_point(X, rem % 17, xx, yy);
-            s_f[tid] = chi_point(Q, M, S.sA, S.exch, g.site, g.W, xx, yy);
+            s_f[tid] = chi_point(P, M, S.sA, S.exch, g.site, g.W, xx, yy);
         }
     }
     __syncthreads();
@@ -378,7 +383,8 @@ __global__ __launch_bounds__(CHI_SPLIT_T) void k_chi_split(ChiArgs Q) {
     }
 }
 
-// propagator term of site 0 (correlation_su2.jl:78-82): QuadGK over (-Inf, -2ref, 0, 2ref, Inf)
+// propagator term of site 0 (correlation_su2.jl:78-82): QuadGK over (-Inf, -2ref, 0, 2ref, Inf);
+// block b handles χ frequency b of all problems' frequencies (Q.prop_prob names the problem)
 __global__ __launch_bounds__(64) void k_chi_prop(ChiArgs Q) {
     __shared__ double s_mesh[5 * MAXN];
     __shared__ double s_sA[MAXN];
@@ -386,13 +392,14 @@ __global__ __launch_bounds__(64) void k_chi_prop(ChiArgs Q) {
     __shared__ double fv[64];
     __shared__ int ctl[4];
     __shared__ double ctld[2];
-    const Sys &y = Q.y;
+    const ChiProb &P = Q.probs[Q.prop_prob[blockIdx.x]];
+    const Sys &y = P.y;
     const int lane = threadIdx.x;
-    for (int k = lane; k < 5 * MAXN; k += 64) s_mesh[k] = Q.mesh[k];
-    for (int k = lane; k < y.nsig; k += 64) s_sA[k] = Q.A.Sigma[k];
+    for (int k = lane; k < 5 * MAXN; k += 64) s_mesh[k] = P.mesh[k];
+    for (int k = lane; k < y.nsig; k += 64) s_sA[k] = P.A.Sigma[k];
     __syncthreads();
-    const double W = Q.chi[blockIdx.x];
-    const double L = Q.L;
+    const double W = P.chi[blockIdx.x - P.prop0];
+    const double L = P.L;
     const double ref = L + 0.5 * W;
     auto f = [&](double v) -> double {
         return y.S2 * G_full<double>(L, v - 0.5 * W, s_mesh, y.nsig, s_sA) *
@@ -425,7 +432,7 @@ hipError_t launch_chi_split(const ChiArgs &Q, int n_blocks, hipStream_t stream) 
 }
 
 hipError_t launch_chi_prop(const ChiArgs &Q, hipStream_t stream) {
-    hipLaunchKernelGGL(k_chi_prop, dim3(Q.nchi), dim3(64), 0, stream, Q);
+    hipLaunchKernelGGL(k_chi_prop, dim3(Q.nfreq), dim3(64), 0, stream, Q);
     return hipGetLastError();
 }
 

@@ -358,6 +358,8 @@ struct pffrg_batch {
     int n_codes = 0;
     void *d_tab = nullptr;         // PassArgs of every problem
     size_t tab_bytes = 0;
+    void *chi_mem = nullptr;       // correlation cubature scratch of batched χ
+    size_t chi_bytes = 0;
 };
 
 namespace {
@@ -1242,6 +1244,217 @@ int batch_rhs_1l_t(pffrg_batch *b, const double *Lam, const int32_t *a, const in
         if (hipStreamSynchronize(c->stream) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "stream synchronize failed");
     return 0;
 }
+
+// ---- χ of one or several problems (compute_χ!, correlation_su2.jl:58-86)
+struct ChiJob {
+    pffrg_ctx *c;
+    double L;
+    int slot;
+    const double *chi_mesh;
+    int nchi;
+    double *chi;
+    int32_t *nreg;
+};
+
+#define CHICHK(expr)                                                      \
+    do {                                                                  \
+        hipError_t e_ = (expr);                                           \
+        if (e_ != hipSuccess) {                                           \
+            msg = std::string(#expr) + ": " + hipGetErrorString(e_);      \
+            return PFFRG_ERR_HIP;                                         \
+        }                                                                 \
+    } while (0)
+
+// every (problem, χ frequency, site, box) task of the jobs through one set of cubature rounds
+int chi_run(void *&mem, size_t &bytes, hipStream_t stream, const ChiJob *jobs, int njobs, const double *tol,
+            std::string &msg) {
+    auto r256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    std::vector<ChiProb> probs(njobs);
+    std::vector<int> task_prob, prop_prob;
+    long tasks = 0;
+    int nfreq = 0;
+    for (int k = 0; k < njobs; ++k) {
+        const pffrg_ctx *c = jobs[k].c;
+        const long tk = (long)jobs[k].nchi * c->sys.ns * 9;
+        probs[k].task0 = (int)tasks;
+        probs[k].prop0 = nfreq;
+        task_prob.insert(task_prob.end(), tk, k);
+        prop_prob.insert(prop_prob.end(), jobs[k].nchi, k);
+        tasks += tk;
+        nfreq += jobs[k].nchi;
+    }
+    // every task first runs in a slot of CHI_CAP0 regions; the few that need more run again in
+    // slots of the full budget, CHI_POOL_SLOTS at a time (the result does not depend on the slot size)
+    const long CHI_CAP0 = 16384, CHI_POOL_SLOTS = 64;
+    const long max_slots = std::max(tasks, CHI_POOL_SLOTS);
+    const size_t pool = std::max((size_t)tasks * CHI_CAP0, (size_t)CHI_POOL_SLOTS * CHI_MAX_REGIONS);
+    // split-table entries: at most one per slot plus one per CHI_RPB regions
+    const size_t tab_cap = std::max((size_t)tasks * (CHI_CAP0 / CHI_RPB + 1),
+                                    (size_t)CHI_POOL_SLOTS * (CHI_MAX_REGIONS / CHI_RPB + 1));
+    // per call: problems, box/nreg/status/task_prob per task, prop/prop_prob/χ mesh per frequency;
+    // per slot: task, active list, rn, rk, done; the split table; the region pool
+    const size_t fixed = r256(njobs * sizeof(ChiProb)) + r256(tasks * 8) + 3 * r256(tasks * 4) +
+                         2 * r256((size_t)nfreq * 8) + r256((size_t)nfreq * 4) + 3 * r256(max_slots * 4) +
+                         2 * r256(max_slots * 8) + r256(tab_cap * 8);
+    const size_t need = fixed + 6 * r256(pool * 8) + 2 * r256(pool * 4);
+    if (need > bytes) {
+        CHICHK(hipStreamSynchronize(stream));
+        CHICHK(hipFree(mem));
+        mem = nullptr;
+        bytes = 0;
+        CHICHK(hipMalloc(&mem, need));
+        bytes = need;
+    }
+    char *p = (char *)mem;
+    auto take = [&](size_t n) {
+        char *q = p;
+        p += r256(n);
+        return (void *)q;
+    };
+    ChiArgs Q;
+    ChiProb *d_probs = (ChiProb *)take(njobs * sizeof(ChiProb));
+    Q.probs = d_probs;
+    Q.box = (double *)take(tasks * 8);
+    Q.nreg = (int *)take(tasks * 4);
+    Q.status = (int *)take(tasks * 4);
+    int *d_task_prob = (int *)take(tasks * 4);
+    Q.task_prob = d_task_prob;
+    Q.prop = (double *)take((size_t)nfreq * 8);
+    double *d_chi = (double *)take((size_t)nfreq * 8);
+    int *d_prop_prob = (int *)take((size_t)nfreq * 4);
+    Q.prop_prob = d_prop_prob;
+    Q.nfreq = nfreq;
+    int *d_tasks = (int *)take(max_slots * 4);
+    int *d_active = (int *)take(max_slots * 4);
+    Q.rdone = (int *)take(max_slots * 4);
+    Q.rn = (long *)take(max_slots * 8);
+    Q.rk = (long *)take(max_slots * 8);
+    int *d_tab = (int *)take(tab_cap * 8);
+    Q.tasks = d_tasks;
+    Q.active = d_active;
+    Q.split_tab = d_tab;
+    Q.rc0 = (double *)take(pool * 8);
+    Q.rc1 = (double *)take(pool * 8);
+    Q.rh0 = (double *)take(pool * 8);
+    Q.rh1 = (double *)take(pool * 8);
+    Q.rval = (double *)take(pool * 8);
+    Q.rerr = (double *)take(pool * 8);
+    Q.rsplit = (int *)take(pool * 4);
+    Q.rsel = (int *)take(pool * 4);
+    if ((size_t)(p - (char *)mem) > bytes) {
+        msg = "chi scratch layout overflow";
+        return PFFRG_ERR_LIMIT;
+    }
+    Q.atol = tol[0];
+    Q.rtol = tol[1];
+    std::vector<double> hchi(nfreq);
+    for (int k = 0; k < njobs; ++k) {
+        const pffrg_ctx *c = jobs[k].c;
+        ChiProb &P = probs[k];
+        P.y = c->sys;
+        P.mesh = c->d_mesh;
+        const Slot &S = c->slots[jobs[k].slot];
+        P.A = ActV<double>{S.gamma, S.bare, S.Sigma};
+        P.exch = c->d_exch;
+        P.chi = d_chi + P.prop0;
+        P.L = jobs[k].L;
+        P.nchi = jobs[k].nchi;
+        for (int w = 0; w < jobs[k].nchi; ++w) hchi[P.prop0 + w] = jobs[k].chi_mesh[w];
+    }
+    CHICHK(hipMemcpyAsync(d_probs, probs.data(), njobs * sizeof(ChiProb), hipMemcpyHostToDevice, stream));
+    CHICHK(hipMemcpyAsync(d_task_prob, task_prob.data(), tasks * sizeof(int), hipMemcpyHostToDevice, stream));
+    CHICHK(hipMemcpyAsync(d_prop_prob, prop_prob.data(), nfreq * sizeof(int), hipMemcpyHostToDevice, stream));
+    CHICHK(hipMemcpyAsync(d_chi, hchi.data(), nfreq * sizeof(double), hipMemcpyHostToDevice, stream));
+    CHICHK(launch_chi_prop(Q, stream));
+    // the rounds of one pass over a list of tasks with slots of cap regions
+    std::vector<int> act, next, tab, done;
+    std::vector<long> rk;
+    auto run_pass = [&](const int *list, int nslots, long cap) -> int {
+        Q.cap = cap;
+        Q.nslots = nslots;
+        CHICHK(hipMemcpyAsync(d_tasks, list, nslots * sizeof(int), hipMemcpyHostToDevice, stream));
+        CHICHK(launch_chi_init(Q, stream));
+        act.resize(nslots);
+        for (int k = 0; k < nslots; ++k) act[k] = k;
+        rk.assign(nslots, 0);
+        done.assign(nslots, 0);
+        long rounds = 0;
+        while (!act.empty()) {
+            // every round adds a region to each unfinished slot, so cap + 1 rounds end any pass
+            if (++rounds > cap + 1) {
+                msg = "chi cubature made no progress";
+                return PFFRG_ERR_STATE;
+            }
+            CHICHK(hipMemcpyAsync(d_active, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+            CHICHK(launch_chi_round(Q, (int)act.size(), stream));
+            CHICHK(hipMemcpyAsync(rk.data(), Q.rk, nslots * sizeof(long), hipMemcpyDeviceToHost, stream));
+            CHICHK(hipMemcpyAsync(done.data(), Q.rdone, nslots * sizeof(int), hipMemcpyDeviceToHost, stream));
+            CHICHK(hipStreamSynchronize(stream));
+            next.clear();
+            tab.clear();
+            for (int sl : act) {
+                if (done[sl]) continue;
+                next.push_back(sl);
+                for (long f = 0; f < rk[sl]; f += CHI_RPB) {
+                    tab.push_back(sl);
+                    tab.push_back((int)f);
+                }
+            }
+            if (tab.size() / 2 > tab_cap) {
+                msg = "chi split table overflow";
+                return PFFRG_ERR_LIMIT;
+            }
+            if (!tab.empty()) {
+                CHICHK(hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+                CHICHK(launch_chi_split(Q, (int)(tab.size() / 2), stream));
+            }
+            act.swap(next);
+        }
+        return 0;
+    };
+    int rc;
+    std::vector<int> list(tasks);
+    for (long t = 0; t < tasks; ++t) list[t] = (int)t;
+    if ((rc = run_pass(list.data(), (int)tasks, CHI_CAP0))) return rc;
+    std::vector<double> box(tasks), prop(nfreq);
+    std::vector<int> nr(tasks), st(tasks);
+    CHICHK(hipMemcpy(st.data(), Q.status, tasks * sizeof(int), hipMemcpyDeviceToHost));
+    // tasks whose regions outgrew the first slot size: again with the full budget
+    std::vector<int> again;
+    for (long t = 0; t < tasks; ++t)
+        if (st[t] != 0) again.push_back((int)t);
+    for (size_t b0 = 0; b0 < again.size(); b0 += CHI_POOL_SLOTS) {
+        const int nb = (int)std::min((size_t)CHI_POOL_SLOTS, again.size() - b0);
+        if ((rc = run_pass(again.data() + b0, nb, CHI_MAX_REGIONS))) return rc;
+    }
+    CHICHK(hipMemcpyAsync(box.data(), Q.box, tasks * sizeof(double), hipMemcpyDeviceToHost, stream));
+    CHICHK(hipMemcpyAsync(nr.data(), Q.nreg, tasks * sizeof(int), hipMemcpyDeviceToHost, stream));
+    CHICHK(hipMemcpyAsync(st.data(), Q.status, tasks * sizeof(int), hipMemcpyDeviceToHost, stream));
+    CHICHK(hipMemcpyAsync(prop.data(), Q.prop, nfreq * sizeof(double), hipMemcpyDeviceToHost, stream));
+    CHICHK(hipStreamSynchronize(stream));
+    for (long t = 0; t < tasks; ++t)
+        if (st[t] != 0) {
+            msg = "chi cubature slot overflow";
+            return PFFRG_ERR_LIMIT;
+        }
+    for (int k = 0; k < njobs; ++k) {
+        const int ns = jobs[k].c->sys.ns, t0 = probs[k].task0, f0 = probs[k].prop0;
+        for (int w = 0; w < jobs[k].nchi; ++w)
+            for (int i = 0; i < ns; ++i) {
+                const long tb = t0 + ((long)w * ns + i) * 9;
+                double I = 0.0;
+                for (int b = 0; b < 9; ++b) I += box[tb + b];
+                if (i == 0) I += prop[f0 + w];
+                jobs[k].chi[i + (long)ns * w] = I;
+                if (jobs[k].nreg)
+                    for (int b = 0; b < 9; ++b) jobs[k].nreg[((long)w * ns + i) * 9 + b] = nr[tb + b];
+            }
+    }
+    return 0;
+}
+
+#undef CHICHK
+
 }  // namespace
 
 extern "C" {
@@ -1291,6 +1504,8 @@ void pffrg_batch_destroy(pffrg_batch *b) {
     (void)hipSetDevice(b->device);
     if (b->d_codes) (void)hipFree(b->d_codes);
     if (b->d_tab) (void)hipFree(b->d_tab);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->chi_mem) (void)hipFree(b->chi_mem);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -1315,6 +1530,31 @@ int pffrg_batch_rhs_1l(pffrg_batch *b, const double *Lambda, const int32_t *a, c
                                     : batch_rhs_1l_t<double>(b, Lambda, a, da, eval, sigma_tol, gamma_tol);
 }
 
+// compute_χ! of every problem of a batch in one set of cubature rounds
+int pffrg_batch_compute_chi(pffrg_batch *b, const double *Lambda, const int32_t *a, const double *const *chi_mesh,
+                            const int32_t *n_chi, const double *tol, double *const *chi) {
+    if (!b) return PFFRG_ERR_ARG;
+    if (!Lambda || !a || !chi_mesh || !n_chi || !tol || !chi) return bfail(b, PFFRG_ERR_ARG, "null argument");
+    const int n = (int)b->ctx.size();
+    std::vector<ChiJob> jobs;
+    for (int k = 0; k < n; ++k) {
+        pffrg_ctx *c = b->ctx[k];
+        const int q = b->order[k];
+        if (n_chi[q] == 0) continue;  // this problem sits out
+        int rc;
+        if ((rc = ready(c)) || (rc = check_slot(c, a[q]))) return bfail(b, rc, c->err);
+        if (!chi_mesh[q] || !chi[q] || n_chi[q] < 1 || !(Lambda[q] > 0.0))
+            return bfail(b, PFFRG_ERR_ARG, "invalid chi arguments");
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "stream synchronize failed");
+        jobs.push_back(ChiJob{c, Lambda[q], a[q], chi_mesh[q], n_chi[q], chi[q], nullptr});
+    }
+    if (jobs.empty()) return 0;
+    if (hipSetDevice(b->device) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "hipSetDevice failed");
+    std::string msg;
+    const int rc = chi_run(b->chi_mem, b->chi_bytes, b->stream, jobs.data(), (int)jobs.size(), tol, msg);
+    return rc ? bfail(b, rc, msg) : 0;
+}
+
 // compute_χ! (correlation_su2.jl:58-86)
 int pffrg_compute_chi(pffrg_ctx *c, double L, int32_t a, const double *chi_mesh, int32_t nchi, const double *tol,
                       double *chi, int32_t *nreg) {
@@ -1322,146 +1562,11 @@ int pffrg_compute_chi(pffrg_ctx *c, double L, int32_t a, const double *chi_mesh,
     if ((rc = ready(c)) || (rc = check_slot(c, a))) return rc;
     if (!chi_mesh || !chi || !tol || nchi < 1 || !(L > 0.0)) return fail(c, PFFRG_ERR_ARG, "invalid chi arguments");
     HIPCHK(c, hipSetDevice(c->device));
-    const int ns = c->sys.ns;
-    const long tasks = (long)nchi * ns * 9;
-    auto r256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    // every task first runs in a slot of CHI_CAP0 regions; the few that need more run again in
-    // slots of the full budget, CHI_POOL_SLOTS at a time (the result does not depend on the slot size)
-    const long CHI_CAP0 = 16384, CHI_POOL_SLOTS = 64;
-    const long max_slots = std::max(tasks, CHI_POOL_SLOTS);
-    const size_t pool = std::max((size_t)tasks * CHI_CAP0, (size_t)CHI_POOL_SLOTS * CHI_MAX_REGIONS);
-    // split-table entries: at most one per slot plus one per CHI_RPB regions
-    const size_t tab_cap = std::max((size_t)tasks * (CHI_CAP0 / CHI_RPB + 1),
-                                    (size_t)CHI_POOL_SLOTS * (CHI_MAX_REGIONS / CHI_RPB + 1));
-    // per call: box[tasks], nreg[tasks], status[tasks], prop[nchi], χ mesh[nchi]; per slot: task,
-    // active list, rn, rk, done; the split table; the region pool
-    const size_t fixed = r256(tasks * 8) + 2 * r256(tasks * 4) + 2 * r256((size_t)nchi * 8) +
-                         3 * r256(max_slots * 4) + 2 * r256(max_slots * 8) + r256(tab_cap * 8);
-    const size_t need = fixed + 6 * r256(pool * 8) + 2 * r256(pool * 4);
-    if (need > c->chi_bytes) {
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        HIPCHK(c, hipFree(c->chi_mem));
-        c->chi_mem = nullptr;
-        c->chi_bytes = 0;
-        HIPCHK(c, hipMalloc(&c->chi_mem, need));
-        c->chi_bytes = need;
-    }
-    char *p = (char *)c->chi_mem;
-    auto take = [&](size_t bytes) {
-        char *q = p;
-        p += r256(bytes);
-        return (void *)q;
-    };
-    ChiArgs Q;
-    Q.y = c->sys;
-    Q.mesh = c->d_mesh;
-    const Slot &S = c->slots[a];
-    Q.A = ActV<double>{S.gamma, S.bare, S.Sigma};
-    Q.exch = c->d_exch;
-    Q.L = L;
-    Q.atol = tol[0];
-    Q.rtol = tol[1];
-    Q.nchi = nchi;
-    Q.box = (double *)take(tasks * 8);
-    Q.nreg = (int *)take(tasks * 4);
-    Q.status = (int *)take(tasks * 4);
-    Q.prop = (double *)take(nchi * 8);
-    double *dchi = (double *)take(nchi * 8);
-    Q.chi = dchi;
-    int *d_tasks = (int *)take(max_slots * 4);
-    int *d_active = (int *)take(max_slots * 4);
-    Q.rdone = (int *)take(max_slots * 4);
-    Q.rn = (long *)take(max_slots * 8);
-    Q.rk = (long *)take(max_slots * 8);
-    int *d_tab = (int *)take(tab_cap * 8);
-    Q.tasks = d_tasks;
-    Q.active = d_active;
-    Q.split_tab = d_tab;
-    Q.rc0 = (double *)take(pool * 8);
-    Q.rc1 = (double *)take(pool * 8);
-    Q.rh0 = (double *)take(pool * 8);
-    Q.rh1 = (double *)take(pool * 8);
-    Q.rval = (double *)take(pool * 8);
-    Q.rerr = (double *)take(pool * 8);
-    Q.rsplit = (int *)take(pool * 4);
-    Q.rsel = (int *)take(pool * 4);
-    if ((size_t)(p - (char *)c->chi_mem) > c->chi_bytes) return fail(c, PFFRG_ERR_LIMIT, "chi scratch layout overflow");
-    HIPCHK(c, hipMemcpyAsync(dchi, chi_mesh, nchi * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, launch_chi_prop(Q, c->stream));
-    // the rounds of one pass over a list of tasks with slots of cap regions
-    std::vector<int> act, next, tab, done;
-    std::vector<long> rk;
-    auto run_pass = [&](const int *list, int nslots, long cap) -> int {
-        Q.cap = cap;
-        Q.nslots = nslots;
-        HIPCHK(c, hipMemcpyAsync(d_tasks, list, nslots * sizeof(int), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, launch_chi_init(Q, c->stream));
-        act.resize(nslots);
-        for (int k = 0; k < nslots; ++k) act[k] = k;
-        rk.assign(nslots, 0);
-        done.assign(nslots, 0);
-        long rounds = 0;
-        while (!act.empty()) {
-            // every round adds a region to each unfinished slot, so cap + 1 rounds end any pass
-            if (++rounds > cap + 1) return fail(c, PFFRG_ERR_STATE, "chi cubature made no progress");
-            HIPCHK(c, hipMemcpyAsync(d_active, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-            HIPCHK(c, launch_chi_round(Q, (int)act.size(), c->stream));
-            HIPCHK(c, hipMemcpyAsync(rk.data(), Q.rk, nslots * sizeof(long), hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipMemcpyAsync(done.data(), Q.rdone, nslots * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            next.clear();
-            tab.clear();
-            for (int sl : act) {
-                if (done[sl]) continue;
-                next.push_back(sl);
-                for (long f = 0; f < rk[sl]; f += CHI_RPB) {
-                    tab.push_back(sl);
-                    tab.push_back((int)f);
-                }
-            }
-            if (tab.size() / 2 > tab_cap) return fail(c, PFFRG_ERR_LIMIT, "chi split table overflow");
-            if (!tab.empty()) {
-                HIPCHK(c, hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-                HIPCHK(c, launch_chi_split(Q, (int)(tab.size() / 2), c->stream));
-            }
-            act.swap(next);
-        }
-        return 0;
-    };
-    std::vector<int> list(tasks);
-    for (long t = 0; t < tasks; ++t) list[t] = (int)t;
-    if ((rc = run_pass(list.data(), (int)tasks, CHI_CAP0))) return rc;
-    std::vector<double> box(tasks), prop(nchi);
-    std::vector<int> nr(tasks), st(tasks);
-    HIPCHK(c, hipMemcpy(st.data(), Q.status, tasks * sizeof(int), hipMemcpyDeviceToHost));
-    // tasks whose regions outgrew the first slot size: again with the full budget
-    std::vector<int> again;
-    for (long t = 0; t < tasks; ++t)
-        if (st[t] != 0) again.push_back((int)t);
-    for (size_t b0 = 0; b0 < again.size(); b0 += CHI_POOL_SLOTS) {
-        const int nb = (int)std::min((size_t)CHI_POOL_SLOTS, again.size() - b0);
-        if ((rc = run_pass(again.data() + b0, nb, CHI_MAX_REGIONS))) return rc;
-    }
-    HIPCHK(c, hipMemcpyAsync(box.data(), Q.box, tasks * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(nr.data(), Q.nreg, tasks * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(st.data(), Q.status, tasks * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(prop.data(), Q.prop, nchi * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (long t = 0; t < tasks; ++t)
-        if (st[t] != 0) return fail(c, PFFRG_ERR_LIMIT, "chi cubature slot overflow");
-    for (int w = 0; w < nchi; ++w)
-        for (int i = 0; i < ns; ++i) {
-            const long t0 = ((long)w * ns + i) * 9;
-            double I = 0.0;
-            for (int k = 0; k < 9; ++k) I += box[t0 + k];
-            if (i == 0) I += prop[w];
-            chi[i + (long)ns * w] = I;
-            if (nreg)
-                for (int k = 0; k < 9; ++k) nreg[t0 + k] = nr[t0 + k];
-        }
-    return 0;
+    ChiJob job{c, L, a, chi_mesh, nchi, chi, nreg};
+    std::string msg;
+    rc = chi_run(c->chi_mem, c->chi_bytes, c->stream, &job, 1, tol, msg);
+    return rc ? fail(c, rc, msg) : 0;
 }
-
 
 // scan_channel's argmax and cuts (Action.jl:449-480)
 int pffrg_scan_cuts(pffrg_ctx *c, int32_t a, int32_t *idx, double *cuts) {

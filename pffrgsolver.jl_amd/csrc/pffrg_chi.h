@@ -14,17 +14,28 @@ constexpr int CHI_T = 256;                                       // threads per 
 constexpr int CHI_RPB = 7;                                       // selected regions per split block
 constexpr int CHI_SPLIT_T = 256;                                 // >= 34 CHI_RPB threads
 
-struct ChiArgs {
+// one problem of a χ computation (a context's action, meshes and χ frequencies)
+struct ChiProb {
     Sys y;
     const double *mesh;  // σ | Ωs | νs | Ωt | νt, MAXN each
     ActV<double> A;      // the action (reference layout)
     const int *exch;
     const double *chi;   // χ frequencies [nchi]
-    double L, atol, rtol;
+    double L;
     int nchi;
+    int task0;           // first global task: local task (w * ns + site) * 9 + box
+    int prop0;           // first global χ frequency
+};
+
+struct ChiArgs {
+    const ChiProb *probs;   // [problems]
+    const int *task_prob;   // [global task] problem
+    const int *prop_prob;   // [global frequency] problem
+    int nfreq;              // global χ frequencies (k_chi_prop blocks)
+    double atol, rtol;
     long cap;            // region capacity of one task slot
     int nslots;          // task slots of this pass
-    const int *tasks;    // [nslots] task of each slot, task = (w * ns + site) * 9 + box
+    const int *tasks;    // [nslots] global task of each slot
     const int *active;   // [unfinished slots] slot of each k_chi_round block
     const int *split_tab;  // [2 * blocks] (slot, first of CHI_RPB selected regions) of each k_chi_split block
     // per slot: regions so far, regions selected for splitting this round, finished flag
@@ -33,10 +44,10 @@ struct ChiArgs {
     // cubature regions of every slot, [slot][cap]
     double *rc0, *rc1, *rh0, *rh1, *rval, *rerr;
     int *rsplit, *rsel;
-    double *box;         // [task] integral of the task's box
-    int *nreg;           // [task] final region count
-    int *status;         // [task] 0 done, 1 the slot's capacity was too small
-    double *prop;        // [nchi] propagator term of site 0
+    double *box;         // [global task] integral of the task's box
+    int *nreg;           // [global task] final region count
+    int *status;         // [global task] 0 done, 1 the slot's capacity was too small
+    double *prop;        // [global frequency] propagator term of site 0
 };
 
 // first regions of the nslots slots; one round of n_active unfinished slots; the splits of a

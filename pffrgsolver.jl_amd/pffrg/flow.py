@@ -259,8 +259,19 @@ class BS3Flow:
         rhs(self.ctx, L, self.loops, a, A_STAGE, ev, self.Gtol, self.Stol, sigma_corr=self.sigma_corr)
 
     def finish_step(self):
+        """Accept or reject the step (launcher_1l.jl:113-198): error estimate, step size, χ of an
+        accepted step, the stops, new meshes."""
+        if self.finish_step_a():
+            self.finish_step_b(self.measure_chi())
+        else:
+            self.finish_step_b(None)
+
+    def finish_step_a(self) -> bool:
+        """First half of finish_step: the embedded error, acceptance and the next step size.
+        True when an accepted step now needs χ (measure_chi, or a batched compute_chi)."""
         ctx, res = self.ctx, self.res
         res.rhs_evals += 4
+        self._accepted = False
         # embedded error (launcher_1l.jl:113-116)
         ctx.mult_with_add_to(A_INTER, -1.0, A_ERR)
         delta = ctx.abs_max(A_ERR)
@@ -273,7 +284,7 @@ class BS3Flow:
         if err >= 10.0:
             res.status = "unstable"
             self.done = True
-            return
+            return False
         accept = err <= 1.0 or self.dL <= self.bmin
         if accept:
             self.L -= self.dL
@@ -285,11 +296,22 @@ class BS3Flow:
             res.accepted += 1
             res.Lambdas.append(self.L)
             res.gamma_max.append(gmax)
+            self._accepted = True
+            self._gmax = gmax
+        else:
+            res.rejected += 1
+        return accept and self.chi_tol is not None
+
+    def finish_step_b(self, chi):
+        """Second half of finish_step, given χ of the accepted step (None when not measured):
+        divergence and monotonicity stops (launcher_1l.jl:140-172), new meshes, the end."""
+        ctx, res = self.ctx, self.res
+        if self.done:
+            return
+        if self._accepted:
             # divergence, measurement, monotonicity, then new meshes (launcher_1l.jl:140-178)
-            stop = None
-            if gmax >= 1000.0:
-                stop = "diverged"
-            if self.chi_tol is not None and self.measure(A_INTER) and stop is None:
+            stop = "diverged" if self._gmax >= 1000.0 else None
+            if chi is not None and self.record_chi(chi) and stop is None:
                 stop = "non_monotone"
             if stop is not None:
                 ctx.replace_with(A, A_INTER)
@@ -301,19 +323,21 @@ class BS3Flow:
                 res.meshes.append(ctx.m)
             else:
                 ctx.replace_with(A, A_INTER)
-        else:
-            res.rejected += 1
         if not (self.L > self.Lf):
             res.status = "finished"
             self.done = True
 
-    def measure(self, slot: int = A) -> bool:
-        """compute_χ! of the accepted action (measure, Launcher.jl:2-40) on the current χ mesh; True
-        when the static on-site correlation stopped growing (χ/χ_prev < 0.995, the reference's
-        stop rule)."""
+    def measure_chi(self):
+        """compute_χ! of the accepted action (slot A_INTER) on the current χ mesh (measure,
+        Launcher.jl:2-40)."""
         t = time.time()
-        chi = self.ctx.compute_chi(self.L, slot, self.chi_tol)
+        chi = self.ctx.compute_chi(self.L, A_INTER, self.chi_tol)
         self.res.chi_s += time.time() - t
+        return chi
+
+    def record_chi(self, chi) -> bool:
+        """Store χ of the step; True when the static on-site correlation stopped growing
+        (χ/χ_prev < 0.995, the reference's stop rule, Launcher.jl:33-35)."""
         prev = self.res.chi[-1] if self.res.chi else None
         self.res.chi.append(chi)
         if prev is None or prev[0, 0] == 0.0:
@@ -382,8 +406,17 @@ def launch_flow_batch(ctxs, probs, a0s, Li, Lf, max_steps: int | None = None, pr
                              flows[live[0]].Stol, flows[live[0]].Gtol)
                 for q in live:
                     flows[q].after(k)
-            for q in live:
-                flows[q].finish_step()
+            need = [flows[q].finish_step_a() for q in live]
+            chis = [None] * len(live)
+            if any(need):
+                t = time.time()
+                chis = batch.compute_chi([flows[q].L for q in live], A_INTER, flows[live[0]].chi_tol, active=need)
+                dt = time.time() - t
+                for q, nd in zip(live, need):
+                    if nd:
+                        flows[q].res.chi_s += dt / sum(need)
+            for q, chi in zip(live, chis):
+                flows[q].finish_step_b(chi)
     finally:
         if batch is not None:
             batch.close()

@@ -91,6 +91,8 @@ _SIGS = {
     "pffrg_batch_last_error": ([_P], C.c_char_p),
     "pffrg_batch_rhs_1l": ([_P, _D, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32), _D, _D],
                            C.c_int),
+    "pffrg_batch_compute_chi": ([_P, _D, C.POINTER(C.c_int32), C.POINTER(_D), C.POINTER(C.c_int32), _D,
+                                 C.POINTER(_D)], C.c_int),
 }
 
 
@@ -377,3 +379,25 @@ class Batch:
         if rc != 0:
             msg = self.lib.pffrg_batch_last_error(self.h)
             raise PffrgError(f"batch_rhs_1l failed ({rc}): {msg.decode() if msg else ''}")
+
+    def compute_chi(self, Lambdas, a, tol=(1e-8, 1e-4), chi_meshes=None, active=None):
+        """compute_χ! of every problem (slot a[k] at Lambdas[k], the context's χ mesh by default)
+        in one set of cubature rounds; returns chi[k][site, w] as Context.compute_chi does, or
+        None for a problem that active[k] leaves out."""
+        n = len(self.ctxs)
+        L = np.ascontiguousarray(np.broadcast_to(np.asarray(Lambdas, dtype=np.float64), (n,)))
+        ia = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.int32), (n,)))
+        on = [True] * n if active is None else [bool(x) for x in active]
+        meshes = [np.ascontiguousarray(c.m.chi if chi_meshes is None else chi_meshes[k], dtype=np.float64)
+                  for k, c in enumerate(self.ctxs)]
+        outs = [np.zeros(c.ns * len(m)) for c, m in zip(self.ctxs, meshes)]
+        pm = (_D * n)(*[_dp(m) for m in meshes])
+        po = (_D * n)(*[_dp(o) for o in outs])
+        nchi = np.ascontiguousarray([len(m) if on[k] else 0 for k, m in enumerate(meshes)], dtype=np.int32)
+        rc = self.lib.pffrg_batch_compute_chi(self.h, _dp(L), ia.ctypes.data_as(C.POINTER(C.c_int32)), pm,
+                                              nchi.ctypes.data_as(C.POINTER(C.c_int32)), _dp(_tol(tol)), po)
+        if rc != 0:
+            msg = self.lib.pffrg_batch_last_error(self.h)
+            raise PffrgError(f"batch_compute_chi failed ({rc}): {msg.decode() if msg else ''}")
+        return [o.reshape(len(m), c.ns).T.copy() if on[k] else None
+                for k, (o, m, c) in enumerate(zip(outs, meshes, self.ctxs))]

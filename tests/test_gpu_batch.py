@@ -74,3 +74,27 @@ def test_batch_matches_oracle():
     finally:
         for ctx in ctxs:
             ctx.close()
+
+
+def test_batch_chi_equals_single():
+    # pffrg_batch_compute_chi: every problem's χ equals its own compute_chi bitwise, region counts
+    # included; a problem left out returns nothing
+    probs = _problems()
+    ctxs = [p.context(n_slots=2) for p, _, _, _ in probs]
+    try:
+        single = []
+        for ctx, (p, a, L, ev) in zip(ctxs, probs):
+            ctx.upload(0, a)
+            single.append(ctx.compute_chi(L, 0))
+        active = [k != 1 for k in range(len(probs))]
+        with Batch(ctxs) as b:
+            got = b.compute_chi([L for _, _, L, _ in probs], 0, active=active)
+        for k, (g, s) in enumerate(zip(got, single)):
+            if not active[k]:
+                assert g is None
+                continue
+            print(f"problem {k}: chi {g.shape}, max |batch - single| = {np.max(np.abs(g - s)):.3e}", flush=True)
+            np.testing.assert_array_equal(g, s)
+    finally:
+        for ctx in ctxs:
+            ctx.close()

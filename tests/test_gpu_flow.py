@@ -96,6 +96,10 @@ def test_batched_flows_equal_single_flows():
             assert results[k].Lambdas == single[k][0].Lambdas
             assert results[k].accepted == single[k][0].accepted and results[k].rejected == single[k][0].rejected
             np.testing.assert_array_equal(got.gamma, single[k][1].gamma)
+            # χ measured in one batched call per step equals the single flow's
+            assert len(results[k].chi) == len(single[k][0].chi)
+            for x, y in zip(results[k].chi, single[k][0].chi):
+                np.testing.assert_array_equal(x, y)
     finally:
         for ctx in ctxs:
             ctx.close()
