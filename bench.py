@@ -66,6 +66,23 @@ def time_mode(prob, a, mode: int, loops: int, steps: int, warmup: int, eval_: in
     return dt, st, kms, klaunch, out
 
 
+def gather_ceilings(prob, a) -> dict:
+    """Practical ceiling of the vertex pass's gathers on this GPU: random trilinear q3 lookups
+    (8 corners x 3 channels, the pass's lane layout) on the same vertex's gather copies, with no
+    other work (pffrg_gather_ceiling), at the occupancy the hardware allows and capped at the
+    vertex pass's 3 waves per SIMD.  GB/s of gathered bytes, counted as the pass counts them."""
+    ctx = prob.context(n_slots=1, precision=FP32_MIXED)
+    ctx.upload(0, a)
+    out = {}
+    for name, mode in (("fp32", FP32_MIXED), ("fp64", FP64)):
+        for w in (0, 3):
+            gbps, ms = ctx.gather_ceiling(0, mode, w, n_iter=256, n_rep=5)
+            out[f"{name}_{'max_occupancy' if w == 0 else f'{w}_waves'}_GBps"] = gbps
+            log(f"  gather ceiling {name} waves/SIMD {'max' if w == 0 else w}: {gbps:.0f} GB/s ({ms:.2f} ms/launch)")
+    ctx.close()
+    return out
+
+
 def cpu_baseline(prob, a, da_gpu, loops: int, eval_: int, budget_s: float):
     """The FP64 oracle (C++/OpenMP restatement) on a strided sample of the same RHS's
     elements; extrapolated to one full RHS.  Returns (RHS/s, threads, sample text)."""
@@ -220,6 +237,8 @@ def main():
             traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
+    log("gather ceiling")
+    ceil = gather_ceilings(prob, a)
     cpu = None
     if not args.no_cpu:
         log("CPU baseline leg (FP64 oracle, bounded sample)")
@@ -245,7 +264,9 @@ def main():
         "nodes_per_rhs": st32["nodes"] / args.steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_vertex_pass", "bytes_per_launch": bytes_per_launch, "ms_per_launch": avg_ms},
+                     "kernel": "k_vertex_pass", "bytes_per_launch": bytes_per_launch, "ms_per_launch": avg_ms,
+                     "gather_ceiling": ceil,
+                     "frac_of_gather_ceiling_3_waves": achieved / ceil["fp32_3_waves_GBps"]},
         "cpu_baseline": cpu,
     }
     line.update(res)

@@ -16,7 +16,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 SOURCES = ["pffrg.hip", "pass_kat.hip", "pass_left.hip", "pass_central.hip", "pass_bse.hip", "pass_reduced.hip",
-           "chi.hip", "mlp.hip", "rescan.hip"]
+           "chi.hip", "mlp.hip", "rescan.hip", "gather_probe.hip"]
 # per-file flags: the correlation cubature and the mesh rescanning restate the oracle's arithmetic
 # without FMA contraction, so that adaptive region choices and resampled values are the oracle's
 FILE_FLAGS = {"chi.hip": ["-ffp-contract=off"], "rescan.hip": ["-ffp-contract=off"]}

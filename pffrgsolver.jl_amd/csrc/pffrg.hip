@@ -24,6 +24,7 @@
 #include "pffrg_chi.h"
 #include "pffrg_device.h"
 #include "pffrg_pass.h"
+#include "pffrg_probe.h"
 #include "pffrg_rescan.h"
 
 using namespace pffrg;
@@ -1184,6 +1185,43 @@ int pffrg_get_timing(pffrg_ctx *c, double *total_ms, int64_t *launches) {
     if (!c || !total_ms || !launches) return PFFRG_ERR_ARG;
     *total_ms = c->time_ms;
     *launches = c->time_launches;
+    return 0;
+}
+
+int pffrg_gather_ceiling(pffrg_ctx *c, int32_t slot, int32_t precision, int32_t waves_per_simd, int32_t n_iter,
+                         int32_t n_rep, double *gbps, double *ms_per_launch) {
+    if (int e = check_slot(c, slot)) return e;
+    if (!gbps || !ms_per_launch) return fail(c, PFFRG_ERR_ARG, "null output");
+    if (precision != PFFRG_FP64 && precision != PFFRG_FP32_MIXED) return fail(c, PFFRG_ERR_ARG, "unknown precision");
+    if (waves_per_simd < 0 || waves_per_simd > 8 || n_iter < 1 || n_iter > (1 << 20) || n_rep < 1 || n_rep > 1000)
+        return fail(c, PFFRG_ERR_ARG, "gather ceiling: waves_per_simd in 0..8, n_iter in 1..2^20, n_rep in 1..1000");
+    HIPCHK(c, hipSetDevice(c->device));
+    const bool f32 = precision == PFFRG_FP32_MIXED;
+    if (int e = ensure_gather(c, slot, f32)) return e;
+    int cus = 0, lds_cu = 0;
+    HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    HIPCHK(c, hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device));
+    // two rounds of resident waves; a nonzero cap reserves LDS so that at most 4 * cap one-wave
+    // blocks fit on a CU, i.e. cap waves per SIMD
+    const int wps = waves_per_simd > 0 ? waves_per_simd : 8;
+    const int n_blocks = 2 * cus * 4 * wps;
+    const size_t lds = waves_per_simd > 0 ? (size_t)(lds_cu / (4 * wps)) & ~(size_t)255 : 0;
+    float *out = nullptr;
+    HIPCHK(c, hipMalloc(&out, (size_t)n_blocks * 64 * sizeof(float)));
+    const void *g = f32 ? (const void *)c->slots[slot].gat32 : (const void *)c->slots[slot].gat64;
+    hipError_t e = launch_gather_probe(g, f32, c->sys, n_blocks, lds, n_iter, 1u, out, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    for (int r = 0; r < n_rep && e == hipSuccess; ++r)
+        e = launch_gather_probe(g, f32, c->sys, n_blocks, lds, n_iter, 2u + (uint32_t)r, out, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    (void)hipFree(out);
+    if (e != hipSuccess) return fail(c, PFFRG_ERR_HIP, std::string("gather ceiling: ") + hipGetErrorString(e));
+    const double bytes = (double)n_blocks * c->sys.G * c->sys.ns * (double)n_iter * 48.0 * (f32 ? 4.0 : 8.0);
+    *ms_per_launch = ms / n_rep;
+    *gbps = bytes / (*ms_per_launch * 1e-3) / 1e9;
     return 0;
 }
 
