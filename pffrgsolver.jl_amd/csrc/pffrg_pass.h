@@ -536,24 +536,43 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
     } else {
         TV gsum = -tp * ((TV)-1.0 * v1s * v4s + v1s * v4d - (TV)1.0 * v3s * v2s + v3d * v2s);
         TV gdsum = -tp * ((TV)3.0 * v1d * v4s + v1d * v4d + (TV)3.0 * v3s * v2d + v3d * v2d);
-        // overlap gather Σ_(j1,j2,m) -p (-2) m (2S) v1[j1] v2[j2]  (flow_su2_t.jl:59-74)
-        double *sc = S.scr;
-        sc[0 * SCR_ROW + slot] = (double)v1s;
-        sc[1 * SCR_ROW + slot] = (double)v1d;
-        sc[2 * SCR_ROW + slot] = (double)v2s;
-        sc[3 * SCR_ROW + slot] = (double)v2d;
+        // overlap gather Σ_(j1,j2,m) -p (-2) m (2S) v1[j1] v2[j2]  (flow_su2_t.jl:59-74).  The
+        // scratch holds (v1s, v1d, v2s, v2d) of each lane next to each other, so one LDS read
+        // fetches both components of a site; rows go four at a time, their reads issued
+        // together, and are added in row order.
+        using V2 = typename Vec2<TV>::T;
+        V2 *sx = reinterpret_cast<V2 *>(S.scr);
+        sx[2 * slot] = V2{v1s, v1d};
+        sx[2 * slot + 1] = V2{v2s, v2d};
         __syncthreads();
         const TV cc = -tp * (TV)(-2.0);
         const TV S2 = (TV)y.S2;
         const int r0 = S.ov_off[i], r1 = S.ov_off[i + 1];
-        for (int r = r0; r < r1; ++r) {
+        const V2 *xg = sx + 2 * gbase;
+        int r = r0;
+        for (; r + 4 <= r1; r += 4) {
+            uint32_t row[4];
+            V2 e1[4], e2[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) row[k] = S.ov[r + k];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                e1[k] = xg[2 * (int)(row[k] & 255)];
+                e2[k] = xg[2 * (int)((row[k] >> 8) & 255) + 1];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const TV m = (TV)(double)(row[k] >> 16);
+                gsum += cc * m * S2 * e1[k].x * e2[k].x;
+                gdsum += cc * m * S2 * e1[k].y * e2[k].y;
+            }
+        }
+        for (; r < r1; ++r) {
             const uint32_t row = S.ov[r];
-            const int j1 = row & 255, j2 = (row >> 8) & 255;
+            const V2 e1 = xg[2 * (int)(row & 255)], e2 = xg[2 * (int)((row >> 8) & 255) + 1];
             const TV m = (TV)(double)(row >> 16);
-            const double *e1 = sc + gbase + j1;
-            const double *e2 = sc + gbase + j2;
-            gsum += cc * m * S2 * (TV)e1[0] * (TV)e2[2 * SCR_ROW];
-            gdsum += cc * m * S2 * (TV)e1[SCR_ROW] * (TV)e2[3 * SCR_ROW];
+            gsum += cc * m * S2 * e1.x * e2.x;
+            gdsum += cc * m * S2 * e1.y * e2.y;
         }
         Gs = gsum;
         Gd = gdsum;
