@@ -650,12 +650,36 @@ int do_scale(pffrg_ctx *c, int s, double fac, bool vertex_only) {
     return 0;
 }
 
+// Bracket hints of one mesh of n >= 2 strictly increasing points from 0 (layout next to
+// HINT_CELLS): c0 puts m[1] in cell 1 and every smaller value in cell 0; the last cell starts
+// above m[n-1] unless that takes more than HINT_CELLS cells.  first[c] = first index whose
+// point is >= the cell's lower bound, so the answers of cell c lie in [first[c], first[c+1]].
+void mesh_hint(const double *m, int n, int &c0, int &nc, uint8_t *tab) {
+    c0 = hint_key(m[1]) - 1;
+    const int top = hint_key(m[n - 1]) - c0 + 1;
+    nc = std::min(top + 1, HINT_CELLS);
+    std::vector<int> first(nc + 1);
+    first[0] = 0;
+    for (int cc = 1; cc < nc; ++cc) {
+        const long long bits = (long long)(cc + c0) << 49;
+        double lb;
+        std::memcpy(&lb, &bits, sizeof lb);
+        int a = 0;
+        while (a < n && m[a] < lb) ++a;
+        first[cc] = std::min(a, n - 1);
+    }
+    first[nc] = n - 1;  // the last cell also holds every larger value, whose answer is n - 1
+    for (int cc = 0; cc < nc; ++cc)
+        tab[cc] = (uint8_t)(first[cc] | (first[cc + 1] - first[cc] > 3 ? 128 : 0));
+}
+
 // the device mesh table: five meshes, then the inverse spacings 1/(m[j+1]-m[j]) of each (the
-// reciprocal get_param computes, param.jl:73-76, evaluated once with the same IEEE division)
+// reciprocal get_param computes, param.jl:73-76, evaluated once with the same IEEE division),
+// then the bracket hints of the five meshes
 int mesh_table(pffrg_ctx *c, const double *sigma, const double *Os, const double *ns, const double *Ot,
                const double *nt, std::vector<double> &h) {
     const Sys &y = c->sys;
-    h.assign(10 * MAXN, 0.0);
+    h.assign(MESH_TABLE, 0.0);
     auto put = [&](int k, const double *m, int n) {
         for (int j = 0; j < n; ++j) h[k * MAXN + j] = m[j];
         for (int j = 0; j + 1 < n; ++j) h[(5 + k) * MAXN + j] = 1.0 / (m[j + 1] - m[j]);
@@ -671,6 +695,18 @@ int mesh_table(pffrg_ctx *c, const double *sigma, const double *Os, const double
         for (int j = 1; j < n; ++j)
             if (!(h[k * MAXN + j] > h[k * MAXN + j - 1])) return fail(c, PFFRG_ERR_ARG, "meshes must be strictly increasing");
     }
+    int16_t hdr[10];
+    std::vector<uint8_t> tab(5 * HINT_CELLS, 0);
+    for (int k = 0; k < 5; ++k) {
+        const int n = (k == 0) ? y.nsig : ((k == 1 || k == 3) ? y.nO : y.nn);
+        int c0, nc;
+        mesh_hint(&h[k * MAXN], n, c0, nc, &tab[k * HINT_CELLS]);
+        hdr[k] = (int16_t)c0;  // keys of positive doubles are < 2^14
+        hdr[5 + k] = (int16_t)nc;
+    }
+    unsigned char *dst = reinterpret_cast<unsigned char *>(&h[10 * MAXN]);
+    std::memcpy(dst, hdr, sizeof hdr);
+    std::memcpy(dst + sizeof hdr, tab.data(), tab.size());
     return 0;
 }
 
@@ -735,7 +771,7 @@ int pffrg_create(const pffrg_dims *d, pffrg_ctx **out) {
     y.chlen = y.q3 + (long)y.ns * y.nO * y.nn * y.nn;
     y.complen = 3 * y.chlen;
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
-    ok = ok && hipMalloc(&c->d_mesh, 10 * MAXN * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_mesh, MESH_TABLE * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_corrs, 2 * 3 * y.nO * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_count, 2 * sizeof(unsigned long long)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_max, sizeof(unsigned long long)) == hipSuccess;

@@ -14,6 +14,24 @@ namespace pffrg {
 
 constexpr int MAXN = 64;      // mesh points per axis held in LDS
 constexpr int WAVE = 64;      // gfx950 wavefront
+
+// Bracket hints of the five meshes (σ, Ωs, νs, Ωt, νt), stored after the meshes and their
+// inverse spacings in the device mesh table.  A positive double's bits shifted right by 49 are
+// its exponent and three leading mantissa bits: a cell of 1/8 octave with exact binary bounds.
+// Cell c of mesh k holds every value whose key minus hint_c0[k] is c, clamped to
+// [0, hint_nc[k]); tab[k][c] is the first point index >= the cell's lower bound (clamped to
+// n-1), with bit 7 set when the cell's answers span more than three indices.  96 cells cover
+// 12 octaves at full resolution; the table and its header take 500 bytes of LDS.
+constexpr int HINT_CELLS = 96;
+constexpr int HINT_WORDS = (20 + 5 * HINT_CELLS) / 4;  // int16 c0[5], nc[5], then uint8 tab[5][HINT_CELLS]
+constexpr int HINT_DOUBLES = (HINT_WORDS + 1) / 2;
+constexpr int MESH_TABLE = 10 * MAXN + HINT_DOUBLES;  // doubles of the device mesh table
+
+__host__ __device__ inline int hint_key(double v) {
+    long long b;
+    __builtin_memcpy(&b, &v, sizeof b);
+    return (int)(b >> 49);
+}
 constexpr double PI_D = 3.14159265358979323846;
 
 enum PassType { P_KAT = 0, P_LEFT = 1, P_CENTRAL = 2, P_BSE = 3, P_REDUCED = 4 };

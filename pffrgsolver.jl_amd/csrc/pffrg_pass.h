@@ -84,6 +84,7 @@ template <typename ST>
 struct MeshV {
     const ST *sig, *Os, *ns, *Ot, *nt, *inv, *sigA, *sigD;
     int mp;
+    const int16_t *hint;  // c0[5], nc[5], then uint8 tab[5][HINT_CELLS] (FP64 tables only; else null)
 };
 
 // Published state of one sub-interval owner (one per group), read by the groups that
@@ -99,6 +100,7 @@ struct Lds {
     const uint32_t *ov;
     const double *inv;      // inverse mesh spacings: σ | Ωs | νs | Ωt | νt, mp each
     int mp;                 // stride of the mesh tables
+    const int16_t *hint;    // bracket hints of the five meshes
     MeshV<float> mf;        // FP32 copies (FP32-mixed passes only)
     const uint32_t *code;   // [MAX_DESC] frequency-transfer codes of this element's vertex slots
     Ctl *ctl;      // [group]
@@ -111,9 +113,12 @@ struct Lds {
 
 // Byte offsets of the dynamic LDS block of k_vertex_pass, sized to the problem: the five
 // meshes and their inverse-spacing tables (stride mp doubles), Σ of both actions, the lane
-// exchange rows, bubble legs, owner states, descriptors, overlap rows and row offsets.
+// exchange rows, bubble legs, owner states, descriptors, overlap rows and row offsets, the
+// site-exchange table and the bracket hints.  At 3 waves per SIMD a block may take 1/12 of the
+// CU's LDS in 1280-byte granules (12800 bytes): the square L=6, N=40 FP32-mixed pass takes
+// 12524.
 struct PassLds {
-    int mp, o_sig, o_scr, o_bub, o_ctl, o_desc, o_mf, o_ov, o_ovoff, total;
+    int mp, o_sig, o_scr, o_bub, o_ctl, o_desc, o_mf, o_ov, o_ovoff, o_exch, o_hint, total;
 };
 
 __host__ __device__ inline PassLds pass_lds(const Sys &y, int n_ov, int desc_size, bool f32) {
@@ -131,6 +136,8 @@ __host__ __device__ inline PassLds pass_lds(const Sys &y, int n_ov, int desc_siz
     L.o_mf = o; o += f32 ? 12 * L.mp * 4 : 0;
     L.o_ov = o; o += n_ov * 4;
     L.o_ovoff = o; o += (y.ns + 1) * 4;
+    L.o_exch = o; o += y.ns * 4;
+    L.o_hint = o; o += HINT_WORDS * 4;
     L.total = o;
     return L;
 }
@@ -214,7 +221,7 @@ template <typename ST>
 __device__ __forceinline__ MeshV<ST> meshv(const Lds &S);
 template <>
 __device__ __forceinline__ MeshV<double> meshv<double>(const Lds &S) {
-    return MeshV<double>{S.sig, S.Os, S.ns, S.Ot, S.nt, S.inv, S.sigA, S.sigD, S.mp};
+    return MeshV<double>{S.sig, S.Os, S.ns, S.Ot, S.nt, S.inv, S.sigA, S.sigD, S.mp, S.hint};
 }
 template <>
 __device__ __forceinline__ MeshV<float> meshv<float>(const Lds &S) {
@@ -287,6 +294,51 @@ __device__ __forceinline__ void bracket(ST val, const ST *m, const ST *inv, int 
     }
 }
 
+// bracket() with the mesh's hint table (k = mesh family σ, Ωs, νs, Ωt, νt): the value's cell
+// gives the first candidate index A and the answer lies in [A, A+3], so three independent LDS
+// reads replace the six dependent steps of the search.  The bracket, the weights and their
+// operation order are bracket()'s; cells whose answers span more indices use bracket().
+__device__ __forceinline__ void bracket_hint(double val, const double *m, const double *inv, int n,
+                                             const int16_t *hint, int k, int &lo, int &hi, double &wl, double &wh) {
+    const int nc = hint[5 + k];
+    int cell = hint_key(val) - hint[k];
+    cell = cell < 0 ? 0 : (cell > nc - 1 ? nc - 1 : cell);
+    const int hb = reinterpret_cast<const uint8_t *>(hint + 10)[k * HINT_CELLS + cell];
+    if (hb & 128) {
+        bracket<double>(val, m, inv, n, lo, hi, wl, wh);
+        return;
+    }
+    // points of the window below val: m[A], m[A+1], m[A+2]; the answer is A + their count
+    const int n1 = n - 1;
+    const int j2 = hb + 1 < n1 ? hb + 1 : n1, j3 = hb + 2 < n1 ? hb + 2 : n1;
+    const int cnt = (m[hb] < val ? 1 : 0) + (m[j2] < val ? 1 : 0) + (m[j3] < val ? 1 : 0);
+    const int base = (hb + cnt < n1) ? hb + cnt : n1;
+    const double mh = m[base];
+    hi = base;
+    lo = (val < mh) ? base - 1 : base;
+    const double ml = m[lo], dl = inv[lo];
+    if (lo < hi) {
+        wl = dl * (mh - val);
+        wh = dl * (val - ml);
+    } else {
+        wl = 1.0;
+        wh = 0.0;
+    }
+}
+
+// bracket of mesh family k through its hint table where one exists (FP64 tables)
+template <typename ST>
+__device__ __forceinline__ void bracket_k(ST val, const MeshV<ST> &M, const ST *m, const ST *inv, int n, int k, int &lo,
+                                          int &hi, ST &wl, ST &wh) {
+    if constexpr (std::is_same<ST, double>::value) {
+        if (M.hint) {
+            bracket_hint(val, m, inv, n, M.hint, k, lo, hi, wl, wh);
+            return;
+        }
+    }
+    bracket<ST>(val, m, inv, n, lo, hi, wl, wh);
+}
+
 // Descriptor of one buffer in straight-line form: frequency transfer from the slot code, the
 // sign folding of get_buffer_{s,t,u} (buffer.jl:88-199), the asymptotic-kernel choice of
 // get_buffer (buffer.jl:51-86), three brackets, and the corner weights and offsets of the
@@ -316,9 +368,9 @@ __device__ __forceinline__ void build_desc_u(const Lds &S, const Sys &y, uint32_
     const int kernel = out ? 0 : (av ? (avp ? 1 : 3) : (avp ? 2 : 4));
     int l1, h1, l2, h2, l3, h3;
     ST a1w, b1w, a2w, b2w, a3w, b3w;
-    bracket<ST>(w, Om, iO, nO, l1, h1, a1w, b1w);
-    bracket<ST>(x, nu, inu, nn, l2, h2, a2w, b2w);
-    bracket<ST>(z, nu, inu, nn, l3, h3, a3w, b3w);
+    bracket_k<ST>(w, M, Om, iO, nO, tch ? 3 : 1, l1, h1, a1w, b1w);
+    bracket_k<ST>(x, M, nu, inu, nn, tch ? 4 : 2, l2, h2, a2w, b2w);
+    bracket_k<ST>(z, M, nu, inu, nn, tch ? 4 : 2, l3, h3, a3w, b3w);
     if (kernel == 3) { l2 = l3; h2 = h3; a2w = a3w; b2w = b3w; }
     if (kernel != 4) { l3 = 0; h3 = 0; a3w = (ST)1; b3w = (ST)0; }
     if (kernel == 1) { l2 = 0; h2 = 0; a2w = (ST)1; b2w = (ST)0; }
@@ -448,7 +500,7 @@ __device__ __forceinline__ void bubble_leg(ST L, ST x, const MeshV<ST> &M, int n
     if (ax <= M.sig[n - 1]) {
         int lo, hi;
         ST wl, wh;
-        bracket<ST>(ax, M.sig, M.inv, n, lo, hi, wl, wh);
+        bracket_k<ST>(ax, M, M.sig, M.inv, n, 0, lo, hi, wl, wh);
         const ST sg = (x > (ST)0) ? (ST)1 : ((x < (ST)0) ? (ST)-1 : x);
         s = sg * (wl * M.sigA[lo] + wh * M.sigA[hi]);
         if (KAT) ds = sg * (wl * M.sigD[lo] + wh * M.sigD[hi]);
@@ -630,7 +682,6 @@ template <int PASS, typename TV>
 __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_idx, uint32_t code) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     __shared__ uint32_t s_code[MAX_DESC];
-    __shared__ int s_exch[64];
     __shared__ int s_own[MAX_GROUPS];
     __shared__ unsigned long long s_cnt, s_bytes;
 
@@ -660,9 +711,15 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
             mf[11 * mp + k] = (PASS == P_KAT) ? (float)P.D.Sigma[k] : 0.0f;
         }
     }
+    int *s_exch = reinterpret_cast<int *>(s_dyn + Lay.o_exch);
     if (lane < y.ns) s_exch[lane] = P.exch[lane];
     for (int k = lane; k < n_ov; k += 64) s_ov[k] = P.ov_rows[k];
     for (int k = lane; k <= y.ns; k += 64) s_ovoff[k] = P.ov_off[k];
+    int32_t *s_hint = reinterpret_cast<int32_t *>(s_dyn + Lay.o_hint);  // copied as words
+    {
+        const int32_t *gh = reinterpret_cast<const int32_t *>(P.mesh + 10 * MAXN);
+        for (int k = lane; k < HINT_WORDS; k += 64) s_hint[k] = gh[k];
+    }
     if (lane == 0) { s_cnt = 0; s_bytes = 0; }
     __syncthreads();
 
@@ -680,6 +737,7 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
     S.exch = s_exch;
     S.ov = s_ov;
     S.ov_off = s_ovoff;
+    S.hint = reinterpret_cast<const int16_t *>(s_hint);
     S.scr = reinterpret_cast<double *>(s_dyn + Lay.o_scr);
     S.desc = reinterpret_cast<void *>(s_dyn + Lay.o_desc);
     S.bub = reinterpret_cast<double *>(s_dyn + Lay.o_bub);
@@ -688,7 +746,7 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
     {
         const float *mf = reinterpret_cast<const float *>(s_dyn + Lay.o_mf);
         S.mf = MeshV<float>{mf, mf + mp, mf + 2 * mp, mf + 3 * mp, mf + 4 * mp, mf + 5 * mp, mf + 10 * mp,
-                            mf + 11 * mp, mp};
+                            mf + 11 * mp, mp, nullptr};
     }
 
     int ch, kernel, w1, w2, w3;
