@@ -142,10 +142,12 @@ int pffrg_get_timing(pffrg_ctx *ctx, double *total_ms, int64_t *launches);
 /* Practical ceiling of the vertex pass's gathers on this device: random trilinear q3 lookups
  * (8 corners x 3 channels per lookup, the vertex pass's lane layout) on the slot's gather copy
  * in the given precision, with no other work.  waves_per_simd = 0 leaves occupancy to the
- * hardware, 1..8 caps it with reserved LDS; n_iter lookups per site group; n_rep timed
- * launches after one untimed one.  Returns the gathered GB/s and the mean launch time. */
-int pffrg_gather_ceiling(pffrg_ctx *ctx, int32_t slot, int32_t precision, int32_t waves_per_simd, int32_t n_iter,
-                         int32_t n_rep, double *gbps, double *ms_per_launch);
+ * hardware, 1..8 caps it with reserved LDS; footprint = 0 draws from the whole q3 block,
+ * footprint > 0 from as few ν' planes as fit that many bytes (at least 2 planes); n_iter
+ * lookups per site group; n_rep timed launches after one untimed one.  Returns the gathered
+ * GB/s and the mean launch time. */
+int pffrg_gather_ceiling(pffrg_ctx *ctx, int32_t slot, int32_t precision, int32_t waves_per_simd, int64_t footprint,
+                         int32_t n_iter, int32_t n_rep, double *gbps, double *ms_per_launch);
 
 /* compute_χ!  (Observable/correlation_lib/correlation_su2.jl:58-86): static real-space
  * correlations chi[i + n_sites*k] of the irreducible sites i at the n_chi frequencies

@@ -3,7 +3,8 @@
 //
 // Every wave has the vertex pass's lane layout: G groups of GS = n_sites lanes, one site per
 // lane.  Per iteration a group draws one random (Ω, ν, ν′) cell of the q3 block, away from the
-// upper mesh edge so that all 8 corners exist, and every lane loads the 8 corners of its site in
+// upper mesh edge so that all 8 corners exist, with ν′ below `planes` (all of them, or a few
+// to confine the lookups to a footprint that stays in L2), and every lane loads the 8 corners of its site in
 // all three channels (24 loads of one (spin, density) pair each, issued before any is used), as
 // a vertex pass does for one vertex slot.  The loads are independent of each other and of the
 // sum they feed, so with enough waves in flight the rate is what the L2 / Infinity Cache and
@@ -42,7 +43,7 @@ struct Pair<double> {
 
 // at most 4 waves per SIMD: room for all 24 loads of a lookup in flight at once
 template <typename TV>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_gather_probe(const TV *__restrict__ g, Sys y, int n_iter, uint32_t seed,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_gather_probe(const TV *__restrict__ g, Sys y, int planes, int n_iter, uint32_t seed,
                                                      float *__restrict__ out) {
     using P2 = typename Pair<TV>::T;
     extern __shared__ char occupancy_cap[];  // sized by the launch only to limit waves per CU
@@ -59,7 +60,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void
             const uint32_t h0 = mix32(stream + 2u * (uint32_t)it), h1 = mix32(stream + 2u * (uint32_t)it + 1u);
             const int w = (int)((h0 & 0xffffu) % (uint32_t)(y.nO - 1));
             const int v = (int)((h0 >> 16) % (uint32_t)(y.nn - 1));
-            const int vp = (int)((h1 & 0xffffu) % (uint32_t)(y.nn - 1));
+            const int vp = (int)((h1 & 0xffffu) % (uint32_t)planes);
             const long cell = y.q3 + site + s1 * w + s2 * v + s3 * vp;
             P2 c[3][8];
 #pragma unroll
@@ -86,14 +87,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void
 
 }  // namespace
 
-hipError_t launch_gather_probe(const void *g, bool f32, const Sys &y, int n_blocks, size_t lds_bytes, int n_iter,
-                               uint32_t seed, float *out, hipStream_t stream) {
+hipError_t launch_gather_probe(const void *g, bool f32, const Sys &y, int planes, int n_blocks, size_t lds_bytes,
+                               int n_iter, uint32_t seed, float *out, hipStream_t stream) {
+    if (planes < 1 || planes > y.nn - 1) return hipErrorInvalidValue;
     if (f32)
         hipLaunchKernelGGL(k_gather_probe<float>, dim3(n_blocks), dim3(64), lds_bytes, stream, (const float *)g, y,
-                           n_iter, seed, out);
+                           planes, n_iter, seed, out);
     else
         hipLaunchKernelGGL(k_gather_probe<double>, dim3(n_blocks), dim3(64), lds_bytes, stream, (const double *)g, y,
-                           n_iter, seed, out);
+                           planes, n_iter, seed, out);
     return hipGetLastError();
 }
 

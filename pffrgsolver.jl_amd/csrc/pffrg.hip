@@ -1224,13 +1224,15 @@ int pffrg_get_timing(pffrg_ctx *c, double *total_ms, int64_t *launches) {
     return 0;
 }
 
-int pffrg_gather_ceiling(pffrg_ctx *c, int32_t slot, int32_t precision, int32_t waves_per_simd, int32_t n_iter,
-                         int32_t n_rep, double *gbps, double *ms_per_launch) {
+int pffrg_gather_ceiling(pffrg_ctx *c, int32_t slot, int32_t precision, int32_t waves_per_simd, int64_t footprint,
+                         int32_t n_iter, int32_t n_rep, double *gbps, double *ms_per_launch) {
     if (int e = check_slot(c, slot)) return e;
     if (!gbps || !ms_per_launch) return fail(c, PFFRG_ERR_ARG, "null output");
     if (precision != PFFRG_FP64 && precision != PFFRG_FP32_MIXED) return fail(c, PFFRG_ERR_ARG, "unknown precision");
-    if (waves_per_simd < 0 || waves_per_simd > 8 || n_iter < 1 || n_iter > (1 << 20) || n_rep < 1 || n_rep > 1000)
-        return fail(c, PFFRG_ERR_ARG, "gather ceiling: waves_per_simd in 0..8, n_iter in 1..2^20, n_rep in 1..1000");
+    if (waves_per_simd < 0 || waves_per_simd > 8 || footprint < 0 || n_iter < 1 || n_iter > (1 << 20) || n_rep < 1 ||
+        n_rep > 1000)
+        return fail(c, PFFRG_ERR_ARG,
+                    "gather ceiling: waves_per_simd in 0..8, footprint >= 0, n_iter in 1..2^20, n_rep in 1..1000");
     HIPCHK(c, hipSetDevice(c->device));
     const bool f32 = precision == PFFRG_FP32_MIXED;
     if (int e = ensure_gather(c, slot, f32)) return e;
@@ -1245,10 +1247,15 @@ int pffrg_gather_ceiling(pffrg_ctx *c, int32_t slot, int32_t precision, int32_t 
     float *out = nullptr;
     HIPCHK(c, hipMalloc(&out, (size_t)n_blocks * 64 * sizeof(float)));
     const void *g = f32 ? (const void *)c->slots[slot].gat32 : (const void *)c->slots[slot].gat64;
-    hipError_t e = launch_gather_probe(g, f32, c->sys, n_blocks, lds, n_iter, 1u, out, c->stream);
+    // ν′ planes the lookups may touch: all, or as many as fit the footprint (corners included)
+    const Sys &y = c->sys;
+    const double plane = 3.0 * y.ns * y.nO * y.nn * 2.0 * (f32 ? 4.0 : 8.0);  // one ν′ plane, 3 channels
+    int planes = y.nn - 1;
+    if (footprint > 0) planes = std::max(1, std::min(planes, (int)(footprint / plane) - 1));
+    hipError_t e = launch_gather_probe(g, f32, y, planes, n_blocks, lds, n_iter, 1u, out, c->stream);
     if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
     for (int r = 0; r < n_rep && e == hipSuccess; ++r)
-        e = launch_gather_probe(g, f32, c->sys, n_blocks, lds, n_iter, 2u + (uint32_t)r, out, c->stream);
+        e = launch_gather_probe(g, f32, y, planes, n_blocks, lds, n_iter, 2u + (uint32_t)r, out, c->stream);
     if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
     if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
     float ms = 0.f;

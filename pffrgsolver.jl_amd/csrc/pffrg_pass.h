@@ -490,6 +490,36 @@ __device__ __forceinline__ void gather_slot(const DescT<TV> *ds, const TV *gamma
     }
 }
 
+// One channel of one vertex slot at one site: the part of gather_slot for channel c, for lanes
+// that each take a different (slot, channel).  The corner count is uniform over the lanes that
+// take part; (r0, r1) is the interpolation with the corner weights, before sign and density
+// flags; cn the corners it read (0 for a disabled channel).
+template <typename TV>
+__device__ __forceinline__ void gather_channel(const DescT<TV> &d, const TV *gamma, const int *exch, int site,
+                                               const Sys &y, int c, TV &r0, TV &r1, int &cn) {
+    using V2 = typename Vec2<TV>::T;
+    const V2 *q = reinterpret_cast<const V2 *>(gamma);
+    const int meta = d.meta;
+    const int n = wave_corners(meta);
+    const bool on = (meta & 7) != 0;
+    const int sx = (meta & 8) ? exch[site] : site;
+    const bool map = (meta & 16) != 0;
+    const int chn = (c == 1) ? 1 : ((c == 0) == map ? 2 : 0);
+    const uint32_t o = on ? (uint32_t)(chn * y.chlen + d.off + sx) : 0u;
+    const uint32_t c1 = on ? (uint32_t)d.d1 : 0u, c2 = on ? (uint32_t)d.d2 : 0u, c3 = on ? (uint32_t)d.d3 : 0u;
+    r0 = (TV)0;
+    r1 = (TV)0;
+    cn = 0;
+    if (n) {
+        V2 x[8];
+        load_corners(q, o, c1, c2, c3, n, x);
+        if (on) {
+            corner_sum(d, n, x, r0, r1);
+            cn = corners(meta & 7);
+        }
+    }
+}
+
 // One leg x of the bubble: G(x) and, for the Katanin bubble, K(x) = S(x) + G(x)² dΣ(x)
 // (bubbles.jl:17-80; S reuses G as the reference's get_S recomputes the same value).  Σ and
 // dΣ are interpolated with one shared bracket of |x| (get_Σ, Action.jl:8-26).
@@ -565,19 +595,54 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
     const TV tp = (TV)((PASS == P_KAT) ? K1 * G2 / twopi + K2 * G1 / twopi : -(G1 * G2 / twopi));
     TV v1s = 0, v1d = 0, v2s = 0, v2d = 0, v3s = 0, v3d = 0, v4s = 0, v4d = 0;
     nc = 0;
-    int nc0 = 0;
 #pragma clang loop unroll(disable)
-    for (int k = 0; k < nslot; ++k) {
+    for (int k = 0; k < 2; ++k) {
         const int cfg = vertex_cfg<PASS>(ch, k);
         const ActV<TV> &V = (cfg & 8) ? P.D : P.A;
-        const int site = (k < 2) ? i : 0;
-        TV vs = (TV)V.bare[site], vd = (TV)V.bare[y.ns + site];
+        TV vs = (TV)V.bare[i], vd = (TV)V.bare[y.ns + i];
         int cn;
-        gather_slot<TV>(dg + 3 * k, V.gamma, S.exch, site, y, vs, vd, cn);
-        if (k == 0) { v1s = vs; v1d = vd; nc += cn; }
-        else if (k == 1) { v2s = vs; v2d = vd; nc += cn; }
-        else if (k == 2) { v3s = vs; v3d = vd; nc0 += cn; }
-        else { v4s = vs; v4d = vd; nc0 += cn; }
+        gather_slot<TV>(dg + 3 * k, V.gamma, S.exch, i, y, vs, vd, cn);
+        if (k == 0) { v1s = vs; v1d = vd; }
+        else { v2s = vs; v2d = vd; }
+        nc += cn;
+    }
+    if (ch == 1) {
+        // local (site-0) vertices of slots 2 and 3, the same for every lane of a group: task
+        // q < 6 (channel q % 3 of slot 2 + q / 3) goes to lane q mod GS of the group, which
+        // publishes its interpolation; every lane then adds the three channels to bare[0] in
+        // gather_slot's order
+        double *loc = S.scr + 12 * gs;
+        if (real_lane) {
+#pragma clang loop unroll(disable)
+            for (int q = i; q < 6; q += y.GS) {
+                const int k = 2 + q / 3, c = q - 3 * (q / 3);
+                const ActV<TV> &V = (vertex_cfg<PASS>(ch, k) & 8) ? P.D : P.A;
+                TV r0, r1;
+                int cn;
+                gather_channel<TV>(dg[3 * k + c], V.gamma, S.exch, 0, y, c, r0, r1, cn);
+                loc[2 * q] = (double)r0;
+                loc[2 * q + 1] = (double)r1;
+                nc += cn;  // each local corner counted once per node
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 2; k < 4; ++k) {
+            const ActV<TV> &V = (vertex_cfg<PASS>(ch, k) & 8) ? P.D : P.A;
+            TV vs = (TV)V.bare[0], vd = (TV)V.bare[y.ns];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int meta = dg[3 * k + c].meta;
+                if (meta & 7) {
+                    const TV r0 = (TV)loc[2 * (3 * (k - 2) + c)], r1 = (TV)loc[2 * (3 * (k - 2) + c) + 1];
+                    vs += r0;
+                    vd += (meta & 32) ? -r1 : r1;
+                }
+            }
+            if (k == 2) { v3s = vs; v3d = vd; }
+            else { v4s = vs; v4d = vd; }
+        }
+        __syncthreads();  // the scratch is rewritten by the overlap exchange below
     }
     if (ch == 0) {
         Gs = -tp * ((TV)-2.0 * v1s * v2s + v1s * v2d + v1d * v2s);
@@ -628,7 +693,6 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
         }
         Gs = gsum;
         Gd = gdsum;
-        if (i == 0) nc += nc0;  // site-0 (local) vertices are gathered once per node
     }
     // the descriptors and bubble legs of this node are read; the next node may overwrite them
     __syncthreads();

@@ -69,7 +69,7 @@ _SIGS = {
     "pffrg_debug_record": ([_P, C.POINTER(C.c_int32), _D, C.c_int64], C.c_int),
     "pffrg_set_timing": ([_P, _i], C.c_int),
     "pffrg_get_timing": ([_P, _D, _I64], C.c_int),
-    "pffrg_gather_ceiling": ([_P, _i, _i, _i, _i, _i, _D, _D], C.c_int),
+    "pffrg_gather_ceiling": ([_P, _i, _i, _i, C.c_int64, _i, _i, _D, _D], C.c_int),
     "pffrg_compute_chi": ([_P, _d, _i, _D, _i, _D, _D, C.POINTER(C.c_int32)], C.c_int),
     "pffrg_scan_cuts": ([_P, _i, C.POINTER(C.c_int32), _D], C.c_int),
     "pffrg_scan": ([_D, _D, _i, _D, _D], C.c_int),
@@ -335,13 +335,14 @@ class Context:
         self._ck(self.lib.pffrg_get_timing(self.h, C.byref(ms), C.byref(n)), "get_timing")
         return ms.value, n.value
 
-    def gather_ceiling(self, slot: int, precision: int, waves_per_simd: int = 0, n_iter: int = 256,
-                       n_rep: int = 5) -> tuple[float, float]:
+    def gather_ceiling(self, slot: int, precision: int, waves_per_simd: int = 0, footprint: int = 0,
+                       n_iter: int = 256, n_rep: int = 5) -> tuple[float, float]:
         """GB/s and ms per launch of random trilinear q3 lookups on the slot's gather copy
-        (pffrg_gather_ceiling): the practical ceiling of the vertex pass's gathers."""
+        (pffrg_gather_ceiling): the practical ceiling of the vertex pass's gathers, over the
+        whole q3 block (footprint 0) or confined to about `footprint` bytes."""
         gbps = C.c_double()
         ms = C.c_double()
-        self._ck(self.lib.pffrg_gather_ceiling(self.h, slot, precision, waves_per_simd, n_iter, n_rep,
+        self._ck(self.lib.pffrg_gather_ceiling(self.h, slot, precision, waves_per_simd, footprint, n_iter, n_rep,
                                                C.byref(gbps), C.byref(ms)), "gather_ceiling")
         return gbps.value, ms.value
 
