@@ -357,6 +357,8 @@ struct pffrg_batch {
     std::string err;
     uint32_t *d_codes = nullptr;   // half-pass elements of all problems, problem index in bits 26-31
     int n_codes = 0;
+    std::vector<size_t> layout_lds;  // per-problem LDS (launch order) the launch layout was made for
+    int n_codes_full = 0;            // leading codes of the problems whose blocks fit 12 per CU
     void *d_tab = nullptr;         // PassArgs of every problem
     size_t tab_bytes = 0;
     void *chi_mem = nullptr;       // correlation cubature scratch of batched χ
@@ -1277,13 +1279,53 @@ int bfail(pffrg_batch *b, int code, const std::string &msg) {
     return code;
 }
 
+// LDS of a one-wave vertex-pass block that still lets 12 blocks share a CU, i.e. keeps the pass
+// at its register-bound 3 waves per SIMD (measured on MI355X: 12216 bytes give 12 blocks per
+// CU, 12896 bytes 11).
+constexpr size_t LDS_FULL_OCCUPANCY = 12800;
+
+// Launch layout of a batch for one precision: the problems whose blocks fit 12 per CU first,
+// then the others, each class in creation order (larger problems first).  The two classes run
+// as two launches, each with the LDS of its own largest problem, so one problem that needs
+// more LDS does not lower the occupancy of all the others.
+template <typename TV>
+int batch_layout(pffrg_batch *b) {
+    const int n = (int)b->ctx.size();
+    std::vector<size_t> cur(n);
+    for (int k = 0; k < n; ++k) cur[k] = pass_lds_bytes<TV>(b->ctx[k]);
+    if (cur == b->layout_lds) return 0;
+    std::vector<int> idx(n);
+    for (int k = 0; k < n; ++k) idx[k] = k;
+    std::stable_partition(idx.begin(), idx.end(), [&](int k) { return cur[k] <= LDS_FULL_OCCUPANCY; });
+    std::vector<pffrg_ctx *> ctx(n);
+    std::vector<int> order(n);
+    std::vector<uint32_t> codes;
+    int n_full = 0;
+    for (int k = 0; k < n; ++k) {
+        ctx[k] = b->ctx[idx[k]];
+        order[k] = b->order[idx[k]];
+        for (uint32_t e : element_list(ctx[k]->sys.nO, ctx[k]->sys.nn, 0)) codes.push_back(e | ((uint32_t)k << 26));
+        if (cur[idx[k]] <= LDS_FULL_OCCUPANCY) n_full = (int)codes.size();
+    }
+    if ((int)codes.size() != b->n_codes) return bfail(b, PFFRG_ERR_STATE, "batch element count changed");
+    if (hipMemcpy(b->d_codes, codes.data(), codes.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+        return bfail(b, PFFRG_ERR_HIP, "element list upload failed");
+    b->ctx = ctx;
+    b->order = order;
+    b->n_codes_full = n_full;
+    b->layout_lds.resize(n);
+    for (int k = 0; k < n; ++k) b->layout_lds[k] = cur[idx[k]];
+    return 0;
+}
+
 template <typename TV>
 int batch_rhs_1l_t(pffrg_batch *b, const double *Lam, const int32_t *a, const int32_t *da, const int32_t *eval,
                    const double *stol, const double *gtol) {
+    if (int rc = batch_layout<TV>(b)) return rc;
     const int n = (int)b->ctx.size();
     const bool f32 = std::is_same<TV, float>::value;
     std::vector<PassArgs<TV>> tab(n);
-    size_t lds = 0;
+    size_t lds_full = 0, lds_rest = 0;
     for (int k = 0; k < n; ++k) {
         pffrg_ctx *c = b->ctx[k];
         const int q = b->order[k];
@@ -1294,7 +1336,9 @@ int batch_rhs_1l_t(pffrg_batch *b, const double *Lam, const int32_t *a, const in
         if ((rc = ensure_gather(c, a[q], f32))) return bfail(b, rc, c->err);
         if ((rc = pass_args<P_KAT, TV>(c, Lam[q], a[q], da[q], da[q], eval[q], gtol, 0, tab[k])))
             return bfail(b, rc, c->err);
-        lds = std::max(lds, pass_lds_bytes<TV>(c));
+        const size_t l = pass_lds_bytes<TV>(c);
+        if (l <= LDS_FULL_OCCUPANCY) lds_full = std::max(lds_full, l);
+        else lds_rest = std::max(lds_rest, l);
     }
     for (pffrg_ctx *c : b->ctx)
         if (hipStreamSynchronize(c->stream) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "stream synchronize failed");
@@ -1307,8 +1351,13 @@ int batch_rhs_1l_t(pffrg_batch *b, const double *Lam, const int32_t *a, const in
     }
     if (hipMemcpy(b->d_tab, tab.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
         return bfail(b, PFFRG_ERR_HIP, "argument table upload failed");
-    hipError_t e = launch_vertex_pass_batch<P_KAT, TV>(reinterpret_cast<const PassArgs<TV> *>(b->d_tab), b->d_codes,
-                                                     b->n_codes, lds, b->stream);
+    const PassArgs<TV> *dtab = reinterpret_cast<const PassArgs<TV> *>(b->d_tab);
+    hipError_t e = hipSuccess;
+    if (b->n_codes_full > 0)
+        e = launch_vertex_pass_batch<P_KAT, TV>(dtab, b->d_codes, b->n_codes_full, lds_full, b->stream);
+    if (e == hipSuccess && b->n_codes > b->n_codes_full)
+        e = launch_vertex_pass_batch<P_KAT, TV>(dtab, b->d_codes + b->n_codes_full, b->n_codes - b->n_codes_full,
+                                                lds_rest, b->stream);
     if (e != hipSuccess) return bfail(b, PFFRG_ERR_HIP, std::string("batched vertex pass: ") + hipGetErrorString(e));
     if ((e = hipStreamSynchronize(b->stream)) != hipSuccess)
         return bfail(b, PFFRG_ERR_HIP, std::string("batched vertex pass: ") + hipGetErrorString(e));
