@@ -47,13 +47,21 @@ struct PassArgs {
 // once per node by one lane and read by every site lane of the group.
 template <typename TV>
 struct DescT {
-    int meta;            // kernel (bits 0-2) | exchange << 3 | map << 4 | negative density sign << 5
+    int meta;            // kernel (bits 0-2) | exchange << 3 | map << 4 | negative density sign << 5 |
+                         // upper corner one step up along Ω << 6, ν << 7, ν' << 8 (else the same point)
     int off;             // offset of the first corner inside the channel block (site 0)
-    int d1, d2, d3;      // offset steps to the upper corner along Ω, ν, ν'
-    int pad[sizeof(TV) == 4 ? 1 : 5];  // 56- or 104-byte stride (14 or 26 dwords): the descriptors of up to
-                                       // 16 groups start in distinct LDS banks
+    int pad[sizeof(TV) == 4 ? 1 : 0];  // 44- or 72-byte stride (11 or 18 dwords): the descriptors of up to
+                                       // 16 (FP32) or 8 (FP64) groups start in distinct LDS banks
     TV w[8];             // corner weights in the reference's corner order
 };
+
+// Offset steps to the upper corner along Ω, ν, ν' (0 where the bracket is a single point).
+__device__ __forceinline__ void corner_steps(int meta, const Sys &y, uint32_t &c1, uint32_t &c2, uint32_t &c3) {
+    const uint32_t s1 = (uint32_t)y.ns, s2 = s1 * (uint32_t)y.nO, s3 = s2 * (uint32_t)y.nn;
+    c1 = (meta & 64) ? s1 : 0u;
+    c2 = (meta & 128) ? s2 : 0u;
+    c3 = (meta & 256) ? s3 : 0u;
+}
 
 // Scalar type of the per-node work (bubble legs, frequency transfer, brackets, weights).  FP64
 // in both modes: an FP32 variant measured no faster, the pass being latency-bound, and FP64
@@ -76,7 +84,6 @@ struct Vec2<double> {
 
 constexpr int MAX_GROUPS = 16;  // site groups per wavefront
 constexpr int MAX_DESC = 12;    // 4 vertex slots x 3 channels
-constexpr int SCR_ROW = 80;     // padded lane rows: (G-1)(GS+1)+GS-1 <= 78, 79 = idle lanes
 
 // Meshes, inverse-spacing tables and the two Σ arrays in LDS, in the scalar type of the
 // per-node work (FP64 path: the FP64 tables; FP32-mixed path: FP32 copies of them).
@@ -105,7 +112,7 @@ struct Lds {
     const uint32_t *code;   // [MAX_DESC] frequency-transfer codes of this element's vertex slots
     Ctl *ctl;      // [group]
     int *own;      // [group]: owner whose node the group evaluated this step, -1 = none
-    double *scr;   // lane exchange scratch, [4][SCR_ROW] doubles (one row per quantity); lane (g, i)
+    double *scr;   // lane exchange scratch, [4][sr] doubles (one row per quantity); lane (g, i)
                    // at g*(GS+1)+i so that equal sites of different groups fall in different banks
     void *desc;    // DescT<TV>[group][MAX_DESC]
     double *bub;   // [group][2][2]: (G, K) of the two bubble legs
@@ -116,9 +123,9 @@ struct Lds {
 // exchange rows, bubble legs, owner states, descriptors, overlap rows and row offsets, the
 // site-exchange table and the bracket hints.  At 3 waves per SIMD a block may take 1/12 of the
 // CU's LDS in 1280-byte granules (12800 bytes): the square L=6, N=40 FP32-mixed pass takes
-// 12524.
+// 11660.
 struct PassLds {
-    int mp, o_sig, o_scr, o_bub, o_ctl, o_desc, o_mf, o_ov, o_ovoff, o_exch, o_hint, total;
+    int mp, sr, o_sig, o_scr, o_bub, o_ctl, o_desc, o_mf, o_ov, o_ovoff, o_exch, o_hint, total;
 };
 
 __host__ __device__ inline PassLds pass_lds(const Sys &y, int n_ov, int desc_size, bool f32) {
@@ -129,7 +136,10 @@ __host__ __device__ inline PassLds pass_lds(const Sys &y, int n_ov, int desc_siz
     L.mp = (mp + 1) & ~1;
     int o = 10 * L.mp * 8;
     L.o_sig = o; o += 2 * L.mp * 8;
-    L.o_scr = o; o += 4 * SCR_ROW * 8;
+    // exchange rows: lane (g, i) at g*(GS+1)+i, then one row for the idle lanes; the block also
+    // holds the 12 local-vertex values per group of the t channel
+    L.sr = (y.G * (y.GS + 1) + 2) & ~1;
+    L.o_scr = o; o += (4 * L.sr > 12 * y.G ? 4 * L.sr : 12 * y.G) * 8;
     L.o_bub = o; o += y.G * 4 * 8;
     L.o_ctl = o; o += y.G * (int)sizeof(Ctl);
     L.o_desc = o; o += y.G * MAX_DESC * desc_size;
@@ -375,11 +385,10 @@ __device__ __forceinline__ void build_desc_u(const Lds &S, const Sys &y, uint32_
     if (kernel != 4) { l3 = 0; h3 = 0; a3w = (ST)1; b3w = (ST)0; }
     if (kernel == 1) { l2 = 0; h2 = 0; a2w = (ST)1; b2w = (ST)0; }
     const int base = (kernel == 2) ? (int)y.q21 : (kernel == 3) ? (int)y.q22 : (kernel == 4) ? (int)y.q3 : 0;
-    d.meta = ((code & 1u) && kernel) ? (kernel | (ex ? 8 : 0) | (mp ? 16 : 0) | (negd ? 32 : 0)) : 0;
+    d.meta = ((code & 1u) && kernel) ? (kernel | (ex ? 8 : 0) | (mp ? 16 : 0) | (negd ? 32 : 0) |
+                                        (h1 > l1 ? 64 : 0) | (h2 > l2 ? 128 : 0) | (h3 > l3 ? 256 : 0))
+                                     : 0;
     d.off = base + ns * (l1 + nO * (l2 + nn * l3));
-    d.d1 = ns * (h1 - l1);
-    d.d2 = ns * nO * (h2 - l2);
-    d.d3 = ns * nO * nn * (h3 - l3);
     d.w[0] = (TV)(a1w * a2w * a3w); d.w[1] = (TV)(b1w * a2w * a3w);
     d.w[2] = (TV)(a1w * b2w * a3w); d.w[3] = (TV)(b1w * b2w * a3w);
     d.w[4] = (TV)(a1w * a2w * b3w); d.w[5] = (TV)(b1w * a2w * b3w);
@@ -453,9 +462,7 @@ __device__ __forceinline__ void gather_slot(const DescT<TV> *ds, const TV *gamma
         const bool map = (meta[c] & 16) != 0;
         const int chn = (c == 1) ? 1 : ((c == 0) == map ? 2 : 0);
         o[c] = on ? (uint32_t)(chn * y.chlen + d.off + sx) : 0u;
-        c1[c] = on ? (uint32_t)d.d1 : 0u;
-        c2[c] = on ? (uint32_t)d.d2 : 0u;
-        c3[c] = on ? (uint32_t)d.d3 : 0u;
+        corner_steps(meta[c], y, c1[c], c2[c], c3[c]);
     }
     cn = 0;
     if (sizeof(TV) == 4) {
@@ -506,7 +513,8 @@ __device__ __forceinline__ void gather_channel(const DescT<TV> &d, const TV *gam
     const bool map = (meta & 16) != 0;
     const int chn = (c == 1) ? 1 : ((c == 0) == map ? 2 : 0);
     const uint32_t o = on ? (uint32_t)(chn * y.chlen + d.off + sx) : 0u;
-    const uint32_t c1 = on ? (uint32_t)d.d1 : 0u, c2 = on ? (uint32_t)d.d2 : 0u, c3 = on ? (uint32_t)d.d3 : 0u;
+    uint32_t c1, c2, c3;
+    corner_steps(meta, y, c1, c2, c3);
     r0 = (TV)0;
     r1 = (TV)0;
     cn = 0;
@@ -838,8 +846,9 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
     const int g = lane / GS;
     const bool real_lane = g < G;
     const int i = lane % GS;
+    const int SR = Lay.sr;  // exchange row length
     const int gbase = real_lane ? g * (GS + 1) : 0;  // padded row of the group in the exchange scratch
-    const int ri = real_lane ? gbase + i : SCR_ROW - 1;
+    const int ri = real_lane ? gbase + i : SR - 1;
     const int nsub = 4 * P.eval;
     const int rounds = (nsub + G - 1) / G;
     const double c0 = P.corrs[0 + 2 * (ch + 3 * w1)], c1 = P.corrs[1 + 2 * (ch + 3 * w1)];
@@ -962,13 +971,13 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
         double *sc = S.scr;
         const double v0 = use ? dx * (double)Gs : 0.0, v1 = use ? dx * (double)Gd : 0.0;
         sc[ri] = v0;
-        sc[SCR_ROW + ri] = v1;
+        sc[SR + ri] = v1;
         if (real_lane && i == 0) S.own[g] = use ? own : -1;
         __syncthreads();
         if (mode == M_TAIL) {
             for (int e = 0; e < G && tk + e < 2; ++e) {
                 acc0 += sc[e * (GS + 1) + i];
-                acc1 += sc[SCR_ROW + e * (GS + 1) + i];
+                acc1 += sc[SR + e * (GS + 1) + i];
             }
             __syncthreads();
             tk += G;
@@ -979,7 +988,7 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
         if (real_lane && !done) {
             for (int e = 0; e < G; ++e) {
                 if (S.own[e] != g) continue;
-                const double u0 = sc[e * (GS + 1) + i], u1 = sc[SCR_ROW + e * (GS + 1) + i];
+                const double u0 = sc[e * (GS + 1) + i], u1 = sc[SR + e * (GS + 1) + i];
                 if (init && cnt < 3) {
                     b10 += u0; b11 += u1;
                 } else {
@@ -1002,15 +1011,15 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
             r11 += 16.0 / 15.0 * b21;
             __syncthreads();
             sc[ri] = r10;
-            sc[SCR_ROW + ri] = r11;
-            sc[2 * SCR_ROW + ri] = b20;
-            sc[3 * SCR_ROW + ri] = b21;
+            sc[SR + ri] = r11;
+            sc[2 * SR + ri] = b20;
+            sc[3 * SR + ri] = b21;
             __syncthreads();
             if (need_check) {
                 double q1 = 0.0, q2 = 0.0, q3 = 0.0;
                 for (int jj = 0; jj < y.ns; ++jj) {
                     const double *e = sc + gbase + jj;
-                    const double x0 = e[0], x1 = e[SCR_ROW], z0 = e[2 * SCR_ROW], z1 = e[3 * SCR_ROW];
+                    const double x0 = e[0], x1 = e[SR], z0 = e[2 * SR], z1 = e[3 * SR];
                     q1 += x0 * x0; q1 += x1 * x1;
                     q2 += z0 * z0; q2 += z1 * z1;
                     const double d0 = x0 - z0, d1 = x1 - z1;
@@ -1042,14 +1051,14 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
         if (__ballot(real_lane && !done) != 0) continue;
         // round complete: ordered accumulation of its sub-interval results
         sc[ri] = res0;
-        sc[SCR_ROW + ri] = res1;
+        sc[SR + ri] = res1;
         __syncthreads();
         for (int gg = 0; gg < G; ++gg) {
             const int jj = r * G + gg;
             double ta, tb, ts;
             if (jj < nsub && subinterval(jj, P.eval, ref, logpts, ta, tb, ts)) {
                 acc0 += sc[gg * (GS + 1) + i];
-                acc1 += sc[SCR_ROW + gg * (GS + 1) + i];
+                acc1 += sc[SR + gg * (GS + 1) + i];
             }
         }
         __syncthreads();
