@@ -60,5 +60,29 @@ def build(verbose: bool = False, jobs: int | None = None, out: Path | None = Non
     return out
 
 
+# HDF5 observable/checkpoint files (host code, no GPU): built against the HDF5 C library of the
+# image when its headers are present; pffrg.h5io fails loudly when the library is missing
+H5_PREFIX = Path(os.environ.get("PFFRG_HDF5_PREFIX", "/opt/conda"))
+H5_OUT = PKG / "pffrg" / "_lib" / "libpffrg_h5.so"
+
+
+def build_h5(out: Path | None = None) -> Path | None:
+    out = Path(out) if out else H5_OUT
+    if not (H5_PREFIX / "include" / "hdf5.h").exists():
+        sys.stderr.write(f"libpffrg_h5.so not built: no HDF5 headers under {H5_PREFIX}\n")
+        return None
+    out.parent.mkdir(parents=True, exist_ok=True)
+    tmp = out.with_suffix(".so.tmp")
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+           f"-I{ROOT / 'include'}", f"-I{H5_PREFIX / 'include'}", str(CSRC / "h5io.cpp"),
+           f"-L{H5_PREFIX / 'lib'}", "-lhdf5", f"-Wl,-rpath,{H5_PREFIX / 'lib'}", "-o", str(tmp)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for h5io.cpp:\n{r.stdout}{r.stderr}")
+    os.replace(tmp, out)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_h5())

@@ -178,8 +178,12 @@ class BS3Flow:
     def __init__(self, ctx: Context, prob: Problem, a0: Action, Li: float, Lf: float, loops: int = 1, sigma_corr=True,
                  bmin=1e-4, bmax=0.1, min_eval=10, max_eval=100, Stol=(1e-8, 1e-4), Gtol=(1e-8, 1e-4),
                  ODE_tol=(1e-8, 1e-2), max_steps: int | None = None, verbose=False, chi_tol=(1e-8, 1e-4),
-                 rescan: bool = True, mesh_params: MeshParams = MeshParams(), max_seconds: float | None = None):
+                 rescan: bool = True, mesh_params: MeshParams = MeshParams(), max_seconds: float | None = None,
+                 measurement=None):
         self.ctx, self.prob, self.Lf, self.loops, self.sigma_corr = ctx, prob, Lf, loops, sigma_corr
+        # measurement(Λ, dΛ, mesh, χ, Σ, get_action) after every accepted step with χ, e.g. an
+        # h5io.Measurement writing the observable and checkpoint files (measure, Launcher.jl:2-75)
+        self.measurement = measurement
         self.max_seconds = max_seconds
         if rescan and chi_tol is None:
             raise ValueError("mesh rescanning needs the χ measurement (chi_tol)")
@@ -311,6 +315,8 @@ class BS3Flow:
         if self._accepted:
             # divergence, measurement, monotonicity, then new meshes (launcher_1l.jl:140-178)
             stop = "diverged" if self._gmax >= 1000.0 else None
+            if chi is not None and self.measurement is not None:
+                self.measurement(self.L, self.dL, ctx.m, chi, ctx.get_sigma(A_INTER), lambda: ctx.download(A_INTER))
             if chi is not None and self.record_chi(chi) and stop is None:
                 stop = "non_monotone"
             if stop is not None:
@@ -354,13 +360,14 @@ def launch_flow(ctx: Context, prob: Problem, a0: Action, Li: float, Lf: float, l
                 bmin=1e-4, bmax=0.1, min_eval=10, max_eval=100, Stol=(1e-8, 1e-4), Gtol=(1e-8, 1e-4),
                 ODE_tol=(1e-8, 1e-2), max_steps: int | None = None, verbose=False, chi_tol=(1e-8, 1e-4),
                 rescan: bool = True, mesh_params: MeshParams = MeshParams(),
-                max_seconds: float | None = None) -> FlowResult:
+                max_seconds: float | None = None, measurement=None) -> FlowResult:
     """BS3 flow from Li to Lf (launcher_1l.jl:72-198 and its 2l/ml twins).  Slot A holds the
     flowing action on return, on the context's current meshes.  After every accepted step χ is
     measured (chi_tol; None skips it), the flow stops when the static on-site correlation turns
-    (launcher_1l.jl:160-172), and the action moves onto rescanned meshes (rescan)."""
+    (launcher_1l.jl:160-172), and the action moves onto rescanned meshes (rescan).  A
+    measurement (h5io.Measurement) writes the observable and checkpoint files as launch! does."""
     f = BS3Flow(ctx, prob, a0, Li, Lf, loops, sigma_corr, bmin, bmax, min_eval, max_eval, Stol, Gtol, ODE_tol,
-                max_steps, verbose, chi_tol, rescan, mesh_params, max_seconds)
+                max_steps, verbose, chi_tol, rescan, mesh_params, max_seconds, measurement)
     while f.begin_step():
         for k in range(4):
             f.before(k)

@@ -103,3 +103,30 @@ def test_batched_flows_equal_single_flows():
     finally:
         for ctx in ctxs:
             ctx.close()
+
+
+@pytest.mark.skipif(not __import__("pffrg.h5io", fromlist=["LIB_PATH"]).LIB_PATH.exists(),
+                    reason="libpffrg_h5.so not built (no HDF5 headers)")
+def test_flow_writes_observable_and_checkpoint_files(tmp_path):
+    # measure() of launch! (Launcher.jl:2-75): Σ and χ of every accepted step in the observable
+    # file, a checkpoint per step with ct = 0; a checkpoint read back reproduces its χ exactly
+    from pffrg import h5io as H
+    prob, a0 = _flow_setup("square", 2, 6)
+    obs, cp = tmp_path / "obs.h5", tmp_path / "cp.h5"
+    meas = H.Measurement(obs, cp, ct=0.0, lattice=(prob.name, prob.size, "heisenberg", False, prob.J))
+    with prob.context(n_slots=9) as ctx:
+        res = F.launch_flow(ctx, prob, a0, 5.0, 1.0, max_steps=5, measurement=meas)
+        with H.H5File(obs, "r") as f:
+            assert len(f.keys("χ")) == len(res.chi) == res.accepted
+            for L, c in zip(res.Lambdas, res.chi):
+                np.testing.assert_array_equal(f.read(f"χ/{H.julia_float(L)}/diag"), c)
+            assert H.read_lattice(f)[0] == prob.name
+        assert meas.checkpoints == res.Lambdas
+        with H.H5File(cp, "r") as f:
+            Lc, dL, m, a = H.read_checkpoint(f, res.Lambdas[-2])
+        assert Lc == res.Lambdas[-2] and dL > 0
+        ctx.set_mesh(m)
+        ctx.upload(F.A_INTER, a)
+        chi = ctx.compute_chi(Lc, F.A_INTER, (1e-8, 1e-4))
+    print(f"{res.accepted} steps written; checkpoint at {Lc}: chi max {np.max(np.abs(chi)):.4g}", flush=True)
+    np.testing.assert_array_equal(chi, res.chi[-2])
