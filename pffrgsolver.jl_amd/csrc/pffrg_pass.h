@@ -603,7 +603,7 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
     const TV tp = (TV)((PASS == P_KAT) ? K1 * G2 / twopi + K2 * G1 / twopi : -(G1 * G2 / twopi));
     TV v1s = 0, v1d = 0, v2s = 0, v2d = 0, v3s = 0, v3d = 0, v4s = 0, v4d = 0;
     nc = 0;
-#pragma clang loop unroll(disable)
+#pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int cfg = vertex_cfg<PASS>(ch, k);
         const ActV<TV> &V = (cfg & 8) ? P.D : P.A;
