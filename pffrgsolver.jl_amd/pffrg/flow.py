@@ -469,7 +469,8 @@ def launch_parquet(ctx: Context, L: float, a0: Action, max_iter=50, min_eval=10,
             s_count += 1
             res.sde_iterations += 1
             if verbose and s_count % 10 == 0:
-                print(f"      SDE iteration {s_count - 1}: abs err {s_abs:.2e} ({time.time() - t0:.1f} s)", flush=True)
+                print(f"      SDE iteration {s_count - 1}: abs err {s_abs:.2e} ({time.time() - t0:.1f} s)", file=sys.stderr,
+                      flush=True)
             if max_seconds is not None and time.time() - t0 > max_seconds:
                 break
         ctx.compute_gamma(L, slot_a, slot_ap, ev, Gtol)
@@ -482,7 +483,7 @@ def launch_parquet(ctx: Context, L: float, a0: Action, max_iter=50, min_eval=10,
         ctx.mult_with_add_to(slot_ap, beta, slot_a, vertex_only=True)
         if verbose:
             print(f"    parquet iteration {count}: abs err {abs_err:.2e} rel err {rel_err:.2e} "
-                  f"(SDE total {res.sde_iterations}, {time.time() - t0:.1f} s)", flush=True)
+                  f"(SDE total {res.sde_iterations}, {time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
         count += 1
         if ctx.abs_max(slot_a) >= 1000.0:
             res.status = "diverged"
