@@ -1172,7 +1172,15 @@ int pffrg_probe_integrand(pffrg_ctx *c, int32_t pass, double L, int32_t a, int32
         return fail(c, PFFRG_ERR_ARG, "invalid probe arguments");
     if (w1 < 0 || w1 >= c->sys.nO || w2 >= c->sys.nn || w3 >= c->sys.nn) return fail(c, PFFRG_ERR_ARG, "index");
     HIPCHK(c, hipSetDevice(c->device));
-    double *dv = nullptr, *dout = nullptr;
+    // the probe's buffers, released on every return
+    struct Bufs {
+        double *dv = nullptr, *dout = nullptr;
+        ~Bufs() {
+            (void)hipFree(dv);
+            (void)hipFree(dout);
+        }
+    } B;
+    double *&dv = B.dv, *&dout = B.dout;
     HIPCHK(c, hipMalloc(&dv, n * sizeof(double)));
     HIPCHK(c, hipMalloc(&dout, (size_t)n * 2 * c->sys.ns * sizeof(double)));
     HIPCHK(c, hipMemcpy(dv, v, n * sizeof(double), hipMemcpyHostToDevice));
@@ -1194,8 +1202,6 @@ int pffrg_probe_integrand(pffrg_ctx *c, int32_t pass, double L, int32_t a, int32
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(out, dout, (size_t)n * 2 * c->sys.ns * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(c, hipFree(dv));
-    HIPCHK(c, hipFree(dout));
     return 0;
 }
 
