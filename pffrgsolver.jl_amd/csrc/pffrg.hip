@@ -520,10 +520,10 @@ int pass_args(pffrg_ctx *c, double L, int a, int d, int out, int eval, const dou
     return 0;
 }
 
-template <typename TV>
+template <int PASS, typename TV>
 size_t pass_lds_bytes(const pffrg_ctx *c) {
     return (size_t)pass_lds(c->sys, c->n_ov, (int)sizeof(DescT<TV>),
-                            std::is_same<typename PassScalar<TV>::T, float>::value).total;
+                            std::is_same<typename PassScalar<TV>::T, float>::value, PASS == P_KAT).total;
 }
 
 template <int PASS, typename TV>
@@ -532,7 +532,7 @@ int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const
     PassArgs<TV> P;
     int rc;
     if ((rc = pass_args<PASS, TV>(c, L, a, d, out, eval, tol, mode, P, dbg_v, dbg_out, dbg_elem))) return rc;
-    const size_t lds = pass_lds_bytes<TV>(c);
+    const size_t lds = pass_lds_bytes<PASS, TV>(c);
     const int grid = dbg_v ? dbg_n : P.n_elems;
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, (launch_vertex_pass<PASS, TV>(P, grid, lds, c->stream)));
@@ -1298,7 +1298,7 @@ template <typename TV>
 int batch_layout(pffrg_batch *b) {
     const int n = (int)b->ctx.size();
     std::vector<size_t> cur(n);
-    for (int k = 0; k < n; ++k) cur[k] = pass_lds_bytes<TV>(b->ctx[k]);
+    for (int k = 0; k < n; ++k) cur[k] = pass_lds_bytes<P_KAT, TV>(b->ctx[k]);
     if (cur == b->layout_lds) return 0;
     std::vector<int> idx(n);
     for (int k = 0; k < n; ++k) idx[k] = k;
@@ -1342,7 +1342,7 @@ int batch_rhs_1l_t(pffrg_batch *b, const double *Lam, const int32_t *a, const in
         if ((rc = ensure_gather(c, a[q], f32))) return bfail(b, rc, c->err);
         if ((rc = pass_args<P_KAT, TV>(c, Lam[q], a[q], da[q], da[q], eval[q], gtol, 0, tab[k])))
             return bfail(b, rc, c->err);
-        const size_t l = pass_lds_bytes<TV>(c);
+        const size_t l = pass_lds_bytes<P_KAT, TV>(c);
         if (l <= LDS_FULL_OCCUPANCY) lds_full = std::max(lds_full, l);
         else lds_rest = std::max(lds_rest, l);
     }

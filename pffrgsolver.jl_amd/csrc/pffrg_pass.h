@@ -128,14 +128,15 @@ struct PassLds {
     int mp, sr, o_sig, o_scr, o_bub, o_ctl, o_desc, o_mf, o_ov, o_ovoff, o_exch, o_hint, total;
 };
 
-__host__ __device__ inline PassLds pass_lds(const Sys &y, int n_ov, int desc_size, bool f32) {
+// sig_d: the pass reads dΣ as well as Σ (the Katanin pass); the others keep only Σ in LDS.
+__host__ __device__ inline PassLds pass_lds(const Sys &y, int n_ov, int desc_size, bool f32, bool sig_d = true) {
     PassLds L;
     int mp = y.nsig;
     if (y.nO > mp) mp = y.nO;
     if (y.nn > mp) mp = y.nn;
     L.mp = (mp + 1) & ~1;
     int o = 10 * L.mp * 8;
-    L.o_sig = o; o += 2 * L.mp * 8;
+    L.o_sig = o; o += (sig_d ? 2 : 1) * L.mp * 8;
     // exchange rows: lane (g, i) at g*(GS+1)+i, then one row for the idle lanes; the block also
     // holds the 12 local-vertex values per group of the t channel
     L.sr = (y.G * (y.GS + 1) + 2) & ~1;
@@ -762,10 +763,11 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
     const int lane = threadIdx.x;
     const int n_ov = P.ov_off[y.ns];
     constexpr bool ST32 = sizeof(typename PassScalar<TV>::T) == 4;
-    const PassLds Lay = pass_lds(y, n_ov, (int)sizeof(DescT<TV>), ST32);
+    const PassLds Lay = pass_lds(y, n_ov, (int)sizeof(DescT<TV>), ST32, PASS == P_KAT);
     const int mp = Lay.mp;
     double *s_mesh = reinterpret_cast<double *>(s_dyn);
-    double *s_sigA = reinterpret_cast<double *>(s_dyn + Lay.o_sig), *s_sigD = s_sigA + mp;
+    double *s_sigA = reinterpret_cast<double *>(s_dyn + Lay.o_sig);
+    double *s_sigD = (PASS == P_KAT) ? s_sigA + mp : s_sigA;  // read by the Katanin bubble only
     uint32_t *s_ov = reinterpret_cast<uint32_t *>(s_dyn + Lay.o_ov);
     int *s_ovoff = reinterpret_cast<int *>(s_dyn + Lay.o_ovoff);
 
@@ -773,7 +775,7 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
     for (int k = lane; k < 10 * mp; k += 64) s_mesh[k] = P.mesh[(k / mp) * MAXN + (k % mp)];
     for (int k = lane; k < y.nsig; k += 64) {
         s_sigA[k] = P.A.Sigma[k];
-        s_sigD[k] = (PASS == P_KAT) ? P.D.Sigma[k] : 0.0;
+        if (PASS == P_KAT) s_sigD[k] = P.D.Sigma[k];
     }
     if (ST32) {
         float *mf = reinterpret_cast<float *>(s_dyn + Lay.o_mf);
