@@ -123,7 +123,7 @@ struct Lds {
 // exchange rows, bubble legs, owner states, descriptors, overlap rows and row offsets, the
 // site-exchange table and the bracket hints.  At 3 waves per SIMD a block may take 1/12 of the
 // CU's LDS in 1280-byte granules (12800 bytes): the square L=6, N=40 FP32-mixed pass takes
-// 11628.
+// 11628 (the Katanin pass; the others keep no dΣ).
 struct PassLds {
     int mp, sr, o_sig, o_scr, o_bub, o_ctl, o_desc, o_mf, o_ov, o_ovoff, o_exch, o_hint, total;
 };
