@@ -119,7 +119,7 @@ struct Lds {
 };
 
 // Byte offsets of the dynamic LDS block of k_vertex_pass, sized to the problem: the five
-// meshes and their inverse-spacing tables (stride mp doubles), Σ of both actions, the lane
+// meshes and their inverse-spacing tables (stride mp doubles), Σ (and dΣ in the Katanin pass), the lane
 // exchange rows, bubble legs, owner states, descriptors, overlap rows and row offsets, the
 // site-exchange table and the bracket hints.  At 3 waves per SIMD a block may take 1/12 of the
 // CU's LDS in 1280-byte granules (12800 bytes): the square L=6, N=40 FP32-mixed pass takes
