@@ -141,14 +141,37 @@ def cpu_baseline(prob, a, da_gpu, loops: int, eval_: int, budget_s: float):
         stride = max(1, stride // 4)
     sample = (f"1 in {stride} elements of both vertex passes ({sampled} elements) + full dSigma, "
               f"FP64 oracle, {threads} OpenMP threads, extrapolated to one RHS ({total_est:.1f} s)")
-    return 1.0 / total_est, threads, sample
+    # the same on one thread, over a sparser sample: the per-core rate
+    orc.set_threads(1)
+    t0 = time.perf_counter()
+    orc.compute_dsigma(1.0, a, ref)
+    est1 = time.perf_counter() - t0
+    stride1 = max(256, stride * 16)
+    for (ps, elems, aa, dd, cc) in passes:
+        orc.corrs[:] = cc
+        idx = list(range(0, len(elems), stride1))
+        t1 = time.perf_counter()
+        _sample_elements(orc, ps, aa, dd, elems, idx, scratch, eval_, workers=1)
+        est1 += (time.perf_counter() - t1) * len(elems) / max(1, len(idx))
+    orc.set_threads(threads)
+    model = "unknown"
+    try:
+        model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    extra = {"single_thread_value": 1.0 / est1,
+             "single_thread_sample": f"1 in {stride1} elements of both vertex passes + full dSigma on 1 thread, "
+                                     f"extrapolated to one RHS ({est1:.1f} s)",
+             "cpu_model": model, "host_cpus": os.cpu_count(),
+             "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    return 1.0 / total_est, threads, sample, extra
 
 
-def _sample_elements(orc, ps, aa, dd, elems, idx, out, eval_):
+def _sample_elements(orc, ps, aa, dd, elems, idx, out, eval_, workers=None):
     from concurrent.futures import ThreadPoolExecutor
     # the oracle parallelises a pass internally; for a sample run the elements as separate calls
     # on a thread pool (ctypes releases the GIL)
-    n = len(os.sched_getaffinity(0))
+    n = workers or len(os.sched_getaffinity(0))
     with ThreadPoolExecutor(max_workers=n) as ex:
         list(ex.map(lambda k: orc.compute_element(ps, 1.0, aa, dd, *elems[k], eval_, out), idx))
 
@@ -249,8 +272,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("CPU baseline leg (FP64 oracle, bounded sample)")
-        v, cores, sample = cpu_baseline(prob, a, ref_out, args.loops, eval_, args.cpu_budget)
-        cpu = {"value": v, "unit": "RHS evals/s", "cores": cores, "kind": "port", "sample": sample}
+        v, cores, sample, extra = cpu_baseline(prob, a, ref_out, args.loops, eval_, args.cpu_budget)
+        cpu = {"value": v, "unit": "RHS evals/s", "cores": cores, "kind": "port", "sample": sample, **extra}
         log(f"  cpu {v:.4g} RHS/s on {cores} threads")
     line = {
         "metric": METRIC,
