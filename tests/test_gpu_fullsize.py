@@ -1,0 +1,65 @@
+"""Parity at the single-RHS bench size (square L=6, 16 sites, N=40, Λ=1, eval=10, the seeded
+synthetic vertex): the GPU's compute_dΣ! and Katanin vertex pass against the FP64 oracle on a
+strided sample of the output elements, and FP32-mixed against FP64 on every element.
+
+Tolerances: FP64 ≤1e-12 relative to the largest entry (the pass restates the oracle's arithmetic
+and takes the same adaptive decisions; the oracle's elements read the GPU's dΣ so that they test
+the vertex pass alone); FP32-mixed ≤1e-4 per q3 array, the path's stated tolerance."""
+import numpy as np
+import pytest
+
+import oracle as O
+from pffrg import action as A
+from pffrg import flow as F
+from pffrg.native import FP32_MIXED
+
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+STRIDE = 997  # every 997th element of the half pass: 117 elements over all channels and kernels
+
+
+def _half_elements(nO, nn):
+    return [(ch, k, w1, w2, w3) for ch in range(3) for w1 in range(nO)
+            for (k, w2, w3) in ([(1, -1, -1)] + [(2, w, -1) for w in range(nn)] + [(3, -1, w) for w in range(nn)]
+                                + [(4, w2, w3) for w3 in range(nn) for w2 in range(w3, nn)])]
+
+
+def _entries(a: A.Action, ch, k, w1, w2, w3):
+    q = {1: "q1", 2: "q2_1", 3: "q2_2", 4: "q3"}[k]
+    idx = {1: (w1,), 2: (w1, w2), 3: (w1, w3), 4: (w1, w2, w3)}[k]
+    return np.concatenate([a.view(c, ch, q)[(slice(None),) + idx] for c in range(2)])
+
+
+def test_bench_size_katanin_rhs_matches_oracle_sample():
+    prob = F.Problem.build("square", 6, [[1.0]], initial=1.0, num_sigma=40, num_Omega=40, num_nu=40, num_chi=10)
+    ns, m = prob.r.n_sites, prob.mesh
+    a = A.synthetic_action(ns, m, prob.bare, seed=20261203)
+    outs = {}
+    for prec in (0, FP32_MIXED):
+        with prob.context(n_slots=2, precision=prec) as ctx:
+            ctx.upload(0, a)
+            ctx.compute_dsigma(1.0, 0, 1)
+            ctx.compute_dgamma_1l(1.0, 0, 1, 10)
+            outs[prec] = ctx.download(1)
+    got = outs[0]
+    orc = O.Oracle(prob.r, m)
+    ref_s = A.Action.empty(ns, m)
+    orc.compute_dsigma(1.0, a, ref_s)
+    es = rel_err(got.Sigma, ref_s.Sigma)
+    dd = A.Action.empty(ns, m)
+    dd.Sigma[:] = got.Sigma
+    orc.compute_corrs(1.0, a, dd, True)
+    out = A.Action.empty(ns, m)
+    elems = _half_elements(m.num_Omega, m.num_nu)
+    sample = elems[::STRIDE]
+    for e in sample:
+        orc.compute_element(O.P_KAT, 1.0, a, dd, *e, 10, out)
+    scale = float(np.max(np.abs(got.gamma)))
+    ev = max(float(np.max(np.abs(_entries(got, *e) - _entries(out, *e)))) for e in sample) / scale
+    e32 = max(rel_err(outs[FP32_MIXED].view(c, ch, "q3"), got.view(c, ch, "q3")) for c in range(2) for ch in range(3))
+    print(f"square L=6 N=40: dSigma {es:.2e}; {len(sample)} sampled elements {ev:.2e}; FP32-mixed q3 {e32:.2e}",
+          flush=True)
+    assert es < 1e-12 and ev < 1e-12
+    assert e32 < 1e-4
