@@ -1,6 +1,7 @@
 """Parity at the single-RHS bench size (square L=6, 16 sites, N=40, Λ=1, eval=10, the seeded
-synthetic vertex): the GPU's compute_dΣ! and Katanin vertex pass against the FP64 oracle on a
-strided sample of the output elements, and FP32-mixed against FP64 on every element.
+synthetic vertex): the GPU's compute_dΣ!, its Katanin pass and the left pass of the 2-loop
+right-hand side against the FP64 oracle on strided samples of the output elements, and
+FP32-mixed against FP64 on every element.
 
 Tolerances: FP64 ≤1e-12 relative to the largest entry (the pass restates the oracle's arithmetic
 and takes the same adaptive decisions; the oracle's elements read the GPU's dΣ so that they test
@@ -63,3 +64,35 @@ def test_bench_size_katanin_rhs_matches_oracle_sample():
           flush=True)
     assert es < 1e-12 and ev < 1e-12
     assert e32 < 1e-4
+
+
+def _full_elements(nO, nn):
+    return [(ch, k, w1, w2, w3) for ch in range(3) for w1 in range(nO)
+            for (k, w2, w3) in ([(1, -1, -1)] + [(2, w, -1) for w in range(nn)] + [(3, -1, w) for w in range(nn)]
+                                + [(4, w2, w3) for w3 in range(nn) for w2 in range(nn)])]
+
+
+def test_bench_size_left_pass_matches_oracle_sample():
+    # compute_dΓ_2l! (compute.jl:115-174): the left pass reads the action and its 1-loop
+    # derivative and writes every element of slot da_l, which the oracle's elements match
+    prob = F.Problem.build("square", 6, [[1.0]], initial=1.0, num_sigma=40, num_Omega=40, num_nu=40, num_chi=10)
+    ns, m = prob.r.n_sites, prob.mesh
+    a = A.synthetic_action(ns, m, prob.bare, seed=20261203)
+    with prob.context(n_slots=4) as ctx:
+        ctx.upload(0, a)
+        ctx.compute_dsigma(1.0, 0, 1)
+        ctx.compute_dgamma_1l(1.0, 0, 1, 10)
+        da1 = ctx.download(1)
+        ctx.compute_dsigma(1.0, 0, 2)
+        ctx.compute_dgamma_2l(1.0, 0, 2, 3, 10)
+        left = ctx.download(3)
+    orc = O.Oracle(prob.r, m)
+    orc.compute_corrs(1.0, a, a, False)
+    out = A.Action.empty(ns, m)
+    sample = _full_elements(m.num_Omega, m.num_nu)[::STRIDE]
+    for e in sample:
+        orc.compute_element(O.P_LEFT, 1.0, a, da1, *e, 10, out)
+    scale = float(np.max(np.abs(left.gamma)))
+    ev = max(float(np.max(np.abs(_entries(left, *e) - _entries(out, *e)))) for e in sample) / scale
+    print(f"square L=6 N=40 left pass: {len(sample)} sampled elements {ev:.2e}", flush=True)
+    assert ev < 1e-12
