@@ -1,7 +1,8 @@
 """Parity at the single-RHS bench size (square L=6, 16 sites, N=40, Λ=1, eval=10, the seeded
 synthetic vertex): the GPU's compute_dΣ!, its Katanin pass and the left pass of the 2-loop
 right-hand side against the FP64 oracle on strided samples of the output elements, and
-FP32-mixed against FP64 on every element.
+FP32-mixed against FP64 on every element; at the pyrochlore 3-loop flow's shape (L=4, N=32),
+the central pass of the multiloop right-hand side.
 
 Tolerances: FP64 ≤1e-12 relative to the largest entry (the pass restates the oracle's arithmetic
 and takes the same adaptive decisions; the oracle's elements read the GPU's dΣ so that they test
@@ -95,4 +96,31 @@ def test_bench_size_left_pass_matches_oracle_sample():
     scale = float(np.max(np.abs(left.gamma)))
     ev = max(float(np.max(np.abs(_entries(left, *e) - _entries(out, *e)))) for e in sample) / scale
     print(f"square L=6 N=40 left pass: {len(sample)} sampled elements {ev:.2e}", flush=True)
+    assert ev < 1e-12
+
+
+def test_pyrochlore_flow_size_central_pass_matches_oracle_sample():
+    # compute_dΓ_ml! (compute.jl:177-286) at the 3-loop flow's shape, pyrochlore L=4 (18 sites),
+    # N=32: the central pass of loop 3 reads the 2-loop left pass and fills slot da_c (its half
+    # of the elements, then symmetrized), which the oracle's elements match
+    prob = F.Problem.build("pyrochlore", 4, [[1.0]], initial=1.0, num_sigma=32, num_Omega=32, num_nu=32, num_chi=10)
+    ns, m = prob.r.n_sites, prob.mesh
+    a = A.synthetic_action(ns, m, prob.bare, seed=20261203)
+    with prob.context(n_slots=9) as ctx:
+        ctx.upload(0, a)
+        ctx.compute_dsigma(1.0, 0, 1)
+        ctx.compute_dgamma_2l(1.0, 0, 1, 2, 10)
+        left2 = ctx.download(2)
+        ctx.compute_dsigma(1.0, 0, 3)
+        ctx.compute_dgamma_ml(1.0, 3, 0, 3, 4, 5, 6, 7, 10)
+        central = ctx.download(5)
+    orc = O.Oracle(prob.r, m)
+    orc.compute_corrs(1.0, a, a, False)
+    out = A.Action.empty(ns, m)
+    sample = _half_elements(m.num_Omega, m.num_nu)[::499]
+    for e in sample:
+        orc.compute_element(O.P_CENTRAL, 1.0, a, left2, *e, 10, out)
+    scale = float(np.max(np.abs(central.gamma)))
+    ev = max(float(np.max(np.abs(_entries(central, *e) - _entries(out, *e)))) for e in sample) / scale
+    print(f"pyrochlore L=4 N=32 central pass: {len(sample)} sampled elements {ev:.2e}", flush=True)
     assert ev < 1e-12
