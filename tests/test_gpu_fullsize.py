@@ -2,7 +2,8 @@
 synthetic vertex): the GPU's compute_dΣ!, its Katanin pass and the left pass of the 2-loop
 right-hand side against the FP64 oracle on strided samples of the output elements, and
 FP32-mixed against FP64 on every element; at the pyrochlore 3-loop flow's shape (L=4, N=32),
-the central pass of the multiloop right-hand side.
+the central pass of the multiloop right-hand side; at the warm-start shape (kagome L=5, N=32),
+the parquet SDE and BSE passes.
 
 Tolerances: FP64 ≤1e-12 relative to the largest entry (the pass restates the oracle's arithmetic
 and takes the same adaptive decisions; the oracle's elements read the GPU's dΣ so that they test
@@ -124,3 +125,32 @@ def test_pyrochlore_flow_size_central_pass_matches_oracle_sample():
     ev = max(float(np.max(np.abs(_entries(central, *e) - _entries(out, *e)))) for e in sample) / scale
     print(f"pyrochlore L=4 N=32 central pass: {len(sample)} sampled elements {ev:.2e}", flush=True)
     assert ev < 1e-12
+
+
+def test_warmstart_size_parquet_passes_match_oracle():
+    # compute_Σ! (SDE.jl:58-96) and compute_Γ! (compute.jl:2-54) at the warm-start shape,
+    # kagome L=5 (19 sites), N=32: the reduced s bubble and Σ in full, the BSE pass on a sample
+    prob = F.Problem.build("kagome", 5, [[1.0], [0.2]], initial=1.0, num_sigma=32, num_Omega=32, num_nu=32,
+                           num_chi=10)
+    ns, m = prob.r.n_sites, prob.mesh
+    a = A.synthetic_action(ns, m, prob.bare, seed=20261203)
+    with prob.context(n_slots=3) as ctx:
+        ctx.upload(0, a)
+        ctx.compute_sigma(1.0, 0, 1, 10)
+        sde = ctx.download(1)
+        ctx.compute_gamma(1.0, 0, 2, 10)
+        bse = ctx.download(2)
+    orc = O.Oracle(prob.r, m)
+    ref = A.Action.empty(ns, m)
+    orc.compute_sigma(1.0, a, ref, 10)  # also leaves the tail factors the BSE pass uses
+    e_sde = max(rel_err(sde.view(c, 0, q), ref.view(c, 0, q)) for c in range(2) for q in ("q1", "q2_2"))
+    e_sig = rel_err(sde.Sigma, ref.Sigma)
+    out = A.Action.empty(ns, m)
+    sample = _half_elements(m.num_Omega, m.num_nu)[::499]
+    for e in sample:
+        orc.compute_element(O.P_BSE, 1.0, a, a, *e, 10, out)
+    scale = float(np.max(np.abs(bse.gamma)))
+    e_bse = max(float(np.max(np.abs(_entries(bse, *e) - _entries(out, *e)))) for e in sample) / scale
+    print(f"kagome L=5 N=32: reduced bubble {e_sde:.2e}, SDE sigma {e_sig:.2e}, {len(sample)} BSE elements "
+          f"{e_bse:.2e}", flush=True)
+    assert e_sde < 1e-12 and e_sig < 1e-12 and e_bse < 1e-12
