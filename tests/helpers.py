@@ -28,3 +28,24 @@ def per_array_err(got: A.Action, ref: A.Action) -> dict:
             for q in ("q1", "q2_1", "q2_2", "q3"):
                 out[f"{'sd'[comp]}{cn}{q}"] = rel_err(got.view(comp, ch, q), ref.view(comp, ch, q))
     return out
+
+
+def half_elements(nO, nn):
+    """Output elements (channel, kernel, w1, w2, w3) of a half pass: q3 with w2 >= w3 (compute.jl:57-112)."""
+    return [(ch, k, w1, w2, w3) for ch in range(3) for w1 in range(nO)
+            for (k, w2, w3) in ([(1, -1, -1)] + [(2, w, -1) for w in range(nn)] + [(3, -1, w) for w in range(nn)]
+                                + [(4, w2, w3) for w3 in range(nn) for w2 in range(w3, nn)])]
+
+
+def entries(a: A.Action, ch, k, w1, w2, w3):
+    """Both components of one output element of action a, concatenated."""
+    q = {1: "q1", 2: "q2_1", 3: "q2_2", 4: "q3"}[k]
+    idx = {1: (w1,), 2: (w1, w2), 3: (w1, w3), 4: (w1, w2, w3)}[k]
+    return np.concatenate([a.view(c, ch, q)[(slice(None),) + idx] for c in range(2)])
+
+
+def full_elements(nO, nn):
+    """Output elements of a full pass: every q3 entry (compute.jl:115-174)."""
+    return [(ch, k, w1, w2, w3) for ch in range(3) for w1 in range(nO)
+            for (k, w2, w3) in ([(1, -1, -1)] + [(2, w, -1) for w in range(nn)] + [(3, -1, w) for w in range(nn)]
+                                + [(4, w2, w3) for w3 in range(nn) for w2 in range(nn)])]

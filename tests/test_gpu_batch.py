@@ -7,10 +7,11 @@ import pytest
 
 import oracle as O
 from pffrg import action as A
+from pffrg import flow as F
 from pffrg.mesh import get_eval
 from pffrg.native import FP32_MIXED, FP64, Batch
 
-from helpers import rel_err, small_problem
+from helpers import entries, half_elements, rel_err, small_problem
 
 pytestmark = pytest.mark.gpu
 
@@ -98,3 +99,51 @@ def test_batch_chi_equals_single():
     finally:
         for ctx in ctxs:
             ctx.close()
+
+
+def test_batch_workload_size_equals_single_and_oracle_sample():
+    # the batched sweep at its real shape: 16 square L=5 J1-J2 problems (J2/J1 = k/15) and square
+    # L=3..7 at J2/J1 = 0.5, N=32, Λ=1, eval=10.  The problems fall into both LDS launch classes;
+    # each problem's batched RHS equals its own in both precisions, and the largest truncation
+    # (L=7, 20 sites) matches the oracle on a strided sample of Katanin elements
+    specs = [(5, k / 15.0) for k in range(16)] + [(L, 0.5) for L in (3, 4, 5, 6, 7)]
+    N = 32
+    probs = [F.Problem.build("square", L, [[1.0], [j2]], initial=1.0, num_sigma=N, num_Omega=N, num_nu=N,
+                             num_chi=10) for L, j2 in specs]
+    acts = [A.synthetic_action(p.r.n_sites, p.mesh, p.bare, seed=20261203 + k) for k, p in enumerate(probs)]
+    n = len(probs)
+    for precision in (FP64, FP32_MIXED):
+        ctxs = [p.context(n_slots=3, precision=precision) for p in probs]
+        try:
+            for c, a in zip(ctxs, acts):
+                c.upload(0, a)
+                F.rhs(c, 1.0, 1, 0, 1, 10)
+            with Batch(ctxs) as b:
+                b.rhs_1l([1.0] * n, 0, 2, [10] * n)
+            for k, c in enumerate(ctxs):
+                single, got = c.download(1), c.download(2)
+                np.testing.assert_array_equal(got.gamma, single.gamma, err_msg=f"problem {k} {specs[k]}")
+                np.testing.assert_array_equal(got.Sigma, single.Sigma, err_msg=f"problem {k} {specs[k]}")
+            print(f"precision {precision}: {n} batched RHS equal their single runs bitwise", flush=True)
+            if precision != FP64:
+                continue
+            p, a, got = probs[-1], acts[-1], ctxs[-1].download(2)
+            orc = O.Oracle(p.r, p.mesh)
+            dd = A.Action.empty(p.r.n_sites, p.mesh)
+            dd.Sigma[:] = got.Sigma
+            orc.compute_corrs(1.0, a, dd, True)
+            out = A.Action.empty(p.r.n_sites, p.mesh)
+            sample = half_elements(N, N)[::613]
+            for e in sample:
+                orc.compute_element(O.P_KAT, 1.0, a, dd, *e, 10, out)
+            scale = float(np.max(np.abs(got.gamma)))
+            ev = max(float(np.max(np.abs(entries(got, *e) - entries(out, *e)))) for e in sample) / scale
+            ref_s = A.Action.empty(p.r.n_sites, p.mesh)
+            orc.compute_dsigma(1.0, a, ref_s)
+            es = rel_err(got.Sigma, ref_s.Sigma)
+            print(f"square L=7 (20 sites) in the batch: dSigma {es:.2e}, {len(sample)} sampled elements {ev:.2e}",
+                  flush=True)
+            assert es < 1e-12 and ev < 1e-12
+        finally:
+            for c in ctxs:
+                c.close()

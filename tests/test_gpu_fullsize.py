@@ -16,23 +16,11 @@ from pffrg import action as A
 from pffrg import flow as F
 from pffrg.native import FP32_MIXED
 
-from helpers import rel_err
+from helpers import entries, full_elements, half_elements, rel_err
 
 pytestmark = pytest.mark.gpu
 
 STRIDE = 997  # every 997th element of the half pass: 117 elements over all channels and kernels
-
-
-def _half_elements(nO, nn):
-    return [(ch, k, w1, w2, w3) for ch in range(3) for w1 in range(nO)
-            for (k, w2, w3) in ([(1, -1, -1)] + [(2, w, -1) for w in range(nn)] + [(3, -1, w) for w in range(nn)]
-                                + [(4, w2, w3) for w3 in range(nn) for w2 in range(w3, nn)])]
-
-
-def _entries(a: A.Action, ch, k, w1, w2, w3):
-    q = {1: "q1", 2: "q2_1", 3: "q2_2", 4: "q3"}[k]
-    idx = {1: (w1,), 2: (w1, w2), 3: (w1, w3), 4: (w1, w2, w3)}[k]
-    return np.concatenate([a.view(c, ch, q)[(slice(None),) + idx] for c in range(2)])
 
 
 def test_bench_size_katanin_rhs_matches_oracle_sample():
@@ -55,23 +43,17 @@ def test_bench_size_katanin_rhs_matches_oracle_sample():
     dd.Sigma[:] = got.Sigma
     orc.compute_corrs(1.0, a, dd, True)
     out = A.Action.empty(ns, m)
-    elems = _half_elements(m.num_Omega, m.num_nu)
+    elems = half_elements(m.num_Omega, m.num_nu)
     sample = elems[::STRIDE]
     for e in sample:
         orc.compute_element(O.P_KAT, 1.0, a, dd, *e, 10, out)
     scale = float(np.max(np.abs(got.gamma)))
-    ev = max(float(np.max(np.abs(_entries(got, *e) - _entries(out, *e)))) for e in sample) / scale
+    ev = max(float(np.max(np.abs(entries(got, *e) - entries(out, *e)))) for e in sample) / scale
     e32 = max(rel_err(outs[FP32_MIXED].view(c, ch, "q3"), got.view(c, ch, "q3")) for c in range(2) for ch in range(3))
     print(f"square L=6 N=40: dSigma {es:.2e}; {len(sample)} sampled elements {ev:.2e}; FP32-mixed q3 {e32:.2e}",
           flush=True)
     assert es < 1e-12 and ev < 1e-12
     assert e32 < 1e-4
-
-
-def _full_elements(nO, nn):
-    return [(ch, k, w1, w2, w3) for ch in range(3) for w1 in range(nO)
-            for (k, w2, w3) in ([(1, -1, -1)] + [(2, w, -1) for w in range(nn)] + [(3, -1, w) for w in range(nn)]
-                                + [(4, w2, w3) for w3 in range(nn) for w2 in range(nn)])]
 
 
 def test_bench_size_left_pass_matches_oracle_sample():
@@ -91,11 +73,11 @@ def test_bench_size_left_pass_matches_oracle_sample():
     orc = O.Oracle(prob.r, m)
     orc.compute_corrs(1.0, a, a, False)
     out = A.Action.empty(ns, m)
-    sample = _full_elements(m.num_Omega, m.num_nu)[::STRIDE]
+    sample = full_elements(m.num_Omega, m.num_nu)[::STRIDE]
     for e in sample:
         orc.compute_element(O.P_LEFT, 1.0, a, da1, *e, 10, out)
     scale = float(np.max(np.abs(left.gamma)))
-    ev = max(float(np.max(np.abs(_entries(left, *e) - _entries(out, *e)))) for e in sample) / scale
+    ev = max(float(np.max(np.abs(entries(left, *e) - entries(out, *e)))) for e in sample) / scale
     print(f"square L=6 N=40 left pass: {len(sample)} sampled elements {ev:.2e}", flush=True)
     assert ev < 1e-12
 
@@ -118,11 +100,11 @@ def test_pyrochlore_flow_size_central_pass_matches_oracle_sample():
     orc = O.Oracle(prob.r, m)
     orc.compute_corrs(1.0, a, a, False)
     out = A.Action.empty(ns, m)
-    sample = _half_elements(m.num_Omega, m.num_nu)[::499]
+    sample = half_elements(m.num_Omega, m.num_nu)[::499]
     for e in sample:
         orc.compute_element(O.P_CENTRAL, 1.0, a, left2, *e, 10, out)
     scale = float(np.max(np.abs(central.gamma)))
-    ev = max(float(np.max(np.abs(_entries(central, *e) - _entries(out, *e)))) for e in sample) / scale
+    ev = max(float(np.max(np.abs(entries(central, *e) - entries(out, *e)))) for e in sample) / scale
     print(f"pyrochlore L=4 N=32 central pass: {len(sample)} sampled elements {ev:.2e}", flush=True)
     assert ev < 1e-12
 
@@ -146,11 +128,11 @@ def test_warmstart_size_parquet_passes_match_oracle():
     e_sde = max(rel_err(sde.view(c, 0, q), ref.view(c, 0, q)) for c in range(2) for q in ("q1", "q2_2"))
     e_sig = rel_err(sde.Sigma, ref.Sigma)
     out = A.Action.empty(ns, m)
-    sample = _half_elements(m.num_Omega, m.num_nu)[::499]
+    sample = half_elements(m.num_Omega, m.num_nu)[::499]
     for e in sample:
         orc.compute_element(O.P_BSE, 1.0, a, a, *e, 10, out)
     scale = float(np.max(np.abs(bse.gamma)))
-    e_bse = max(float(np.max(np.abs(_entries(bse, *e) - _entries(out, *e)))) for e in sample) / scale
+    e_bse = max(float(np.max(np.abs(entries(bse, *e) - entries(out, *e)))) for e in sample) / scale
     print(f"kagome L=5 N=32: reduced bubble {e_sde:.2e}, SDE sigma {e_sig:.2e}, {len(sample)} BSE elements "
           f"{e_bse:.2e}", flush=True)
     assert e_sde < 1e-12 and e_sig < 1e-12 and e_bse < 1e-12
