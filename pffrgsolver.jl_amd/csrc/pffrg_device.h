@@ -34,6 +34,19 @@ __host__ __device__ inline int hint_key(double v) {
 }
 constexpr double PI_D = 3.14159265358979323846;
 
+// A value equal on every lane of the wave, moved to scalar registers.  Values read from LDS
+// count as per-lane to the compiler, which then keeps them in vector registers for the whole
+// element; only for wave-uniform values (the result is lane 0's).
+__device__ __forceinline__ double wave_uniform(double v) {
+    unsigned long long b;
+    __builtin_memcpy(&b, &v, sizeof b);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    b = ((unsigned long long)hi << 32) | lo;
+    __builtin_memcpy(&v, &b, sizeof v);
+    return v;
+}
+
 enum PassType { P_KAT = 0, P_LEFT = 1, P_CENTRAL = 2, P_BSE = 3, P_REDUCED = 4 };
 
 // Channel/array offsets of the packed vertex (site fastest, column-major).

@@ -840,6 +840,9 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
         vw = (kernel == 2 || kernel == 4) ? nu[w2] : INF;
         vwp = (kernel == 3 || kernel == 4) ? nu[w3] : INF;
     }
+    W = wave_uniform(W);  // one element per wave
+    vw = wave_uniform(vw);
+    vwp = wave_uniform(vwp);
     const double ref = P.L + 0.5 * W;
     const double val = (ch == 1) ? y.valT : y.valS;
     const double *logpts = P.logpts + ((ch == 1 ? y.nO : 0) + w1) * (P.eval + 1);
@@ -1098,7 +1101,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
     PassArgs<TV> P) {
     const int e_idx = blockIdx.x;
     if (!P.dbg_v && e_idx >= P.n_elems) return;
-    vertex_pass_body<PASS, TV>(P, e_idx, P.dbg_v ? P.dbg_elem : P.elems[e_idx]);
+    vertex_pass_body<PASS, TV>(P, e_idx, __builtin_amdgcn_readfirstlane(P.dbg_v ? P.dbg_elem : P.elems[e_idx]));
 }
 
 // The same pass of several independent problems in one launch: element code bits 26-31 select
@@ -1108,7 +1111,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
     const PassArgs<TV> *__restrict__ tab, const uint32_t *__restrict__ codes, int n) {
     const int bid = blockIdx.x;
     if (bid >= n) return;
-    const uint32_t c = codes[bid];
+    const uint32_t c = __builtin_amdgcn_readfirstlane(codes[bid]);
     vertex_pass_body<PASS, TV>(tab[c >> 26], bid, c & 0x03ffffffu);
 }
 
