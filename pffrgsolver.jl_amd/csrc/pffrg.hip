@@ -968,6 +968,7 @@ int pffrg_compute_dgamma_1l(pffrg_ctx *c, double L, int32_t a, int32_t da, int32
     int rc;
     if ((rc = ready(c)) || (rc = check_slot(c, a)) || (rc = check_slot(c, da))) return rc;
     if (!tol || !(L > 0.0) || a == da) return fail(c, PFFRG_ERR_ARG, "invalid arguments");
+    if (eval < 1) return fail(c, PFFRG_ERR_ARG, "eval must be >= 1");
     HIPCHK(c, hipSetDevice(c->device));
     if ((rc = dgamma_1l(c, L, a, da, eval, tol))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -979,6 +980,7 @@ int pffrg_compute_dgamma_2l(pffrg_ctx *c, double L, int32_t a, int32_t da, int32
     int rc;
     if ((rc = ready(c)) || (rc = check_slot(c, a)) || (rc = check_slot(c, da)) || (rc = check_slot(c, dal))) return rc;
     if (!tol || !(L > 0.0) || a == da || a == dal || da == dal) return fail(c, PFFRG_ERR_ARG, "invalid arguments");
+    if (eval < 1) return fail(c, PFFRG_ERR_ARG, "eval must be >= 1");
     HIPCHK(c, hipSetDevice(c->device));
     if ((rc = dgamma_2l(c, L, a, da, dal, eval, tol))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -996,6 +998,7 @@ int pffrg_compute_dgamma_ml(pffrg_ctx *c, double L, int32_t loops, int32_t a, in
             if (sl[j] == sl[k]) return fail(c, PFFRG_ERR_ARG, "multiloop slots must be distinct");
     }
     if (!tol || !(L > 0.0) || loops < 1) return fail(c, PFFRG_ERR_ARG, "invalid arguments");
+    if (eval < 1) return fail(c, PFFRG_ERR_ARG, "eval must be >= 1");
     HIPCHK(c, hipSetDevice(c->device));
     if ((rc = dgamma_2l(c, L, a, da, dal, eval, tol))) return rc;
     if ((rc = do_scale(c, datemp, 0.0, true))) return rc;
@@ -1019,6 +1022,7 @@ int pffrg_compute_sigma(pffrg_ctx *c, double L, int32_t a1, int32_t a2, int32_t 
     int rc;
     if ((rc = ready(c)) || (rc = check_slot(c, a1)) || (rc = check_slot(c, a2))) return rc;
     if (!gtol || !stol || !(L > 0.0) || a1 == a2) return fail(c, PFFRG_ERR_ARG, "invalid arguments");
+    if (eval < 1) return fail(c, PFFRG_ERR_ARG, "eval must be >= 1");
     HIPCHK(c, hipSetDevice(c->device));
     if ((rc = launch_corrs(c, L, a1, a1, false, gtol))) return rc;
     if ((rc = launch_pass<P_REDUCED>(c, L, a1, a1, a2, eval, gtol, 2))) return rc;
@@ -1031,6 +1035,7 @@ int pffrg_compute_gamma(pffrg_ctx *c, double L, int32_t a1, int32_t a2, int32_t 
     int rc;
     if ((rc = ready(c)) || (rc = check_slot(c, a1)) || (rc = check_slot(c, a2))) return rc;
     if (!tol || !(L > 0.0) || a1 == a2) return fail(c, PFFRG_ERR_ARG, "invalid arguments");
+    if (eval < 1) return fail(c, PFFRG_ERR_ARG, "eval must be >= 1");
     HIPCHK(c, hipSetDevice(c->device));
     if ((rc = launch_pass<P_BSE>(c, L, a1, a1, a2, eval, tol, 0))) return rc;
     if ((rc = do_symmetrize(c, a2))) return rc;

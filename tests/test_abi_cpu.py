@@ -44,3 +44,22 @@ def test_create_without_device_fails_loudly():
     h = ctypes.c_void_p()
     rc = lib.pffrg_create(ctypes.byref(dims), ctypes.byref(h))
     assert rc != 0
+
+
+@pytest.mark.skipif(not native.LIB_PATH.exists(), reason="libpffrg.so not built")
+@pytest.mark.parametrize("dims", [
+    (0, 10, 10, 10, 0.5, 2, 0),    # no sites
+    (65, 10, 10, 10, 0.5, 2, 0),   # more sites than lanes of a wavefront
+    (4, 1, 10, 10, 0.5, 2, 0),     # a one-point Σ mesh
+    (4, 10, 65, 10, 0.5, 2, 0),    # more Ω points than the LDS tables hold
+    (4, 10, 10, 65, 0.5, 2, 0),    # more ν points than the LDS tables hold
+    (4, 10, 10, 10, 0.5, 0, 0),    # no slots
+    (4, 10, 10, 10, 0.0, 2, 0),    # S = 0
+])
+def test_create_rejects_dims_out_of_range(dims):
+    # the size limits are checked before any device call, so they hold without a GPU
+    lib = native.load()
+    h = ctypes.c_void_p()
+    assert lib.pffrg_create(ctypes.byref(native.Dims(*dims)), ctypes.byref(h)) == -4  # PFFRG_ERR_LIMIT
+    assert not h
+    assert lib.pffrg_create(None, ctypes.byref(h)) == -1  # PFFRG_ERR_ARG
