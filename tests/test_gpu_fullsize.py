@@ -3,7 +3,8 @@ synthetic vertex): the GPU's compute_dΣ!, its Katanin pass and the left pass of
 right-hand side against the FP64 oracle on strided samples of the output elements, and
 FP32-mixed against FP64 on every element; at the pyrochlore 3-loop flow's shape (L=4, N=32),
 the central pass of the multiloop right-hand side; at the warm-start shape (kagome L=5, N=32),
-the parquet SDE and BSE passes.
+the parquet SDE and BSE passes; at the largest meshes the library takes (64 points per axis), the
+Katanin pass.
 
 Tolerances: FP64 ≤1e-12 relative to the largest entry (the pass restates the oracle's arithmetic
 and takes the same adaptive decisions; the oracle's elements read the GPU's dΣ so that they test
@@ -136,3 +137,39 @@ def test_warmstart_size_parquet_passes_match_oracle():
     print(f"kagome L=5 N=32: reduced bubble {e_sde:.2e}, SDE sigma {e_sig:.2e}, {len(sample)} BSE elements "
           f"{e_bse:.2e}", flush=True)
     assert e_sde < 1e-12 and e_sig < 1e-12 and e_bse < 1e-12
+
+
+def test_largest_meshes_katanin_rhs_matches_oracle_sample():
+    # the largest meshes the library takes: 64 points per axis (N=63), the LDS tables full, on
+    # square L=3 (6 sites, 10 groups per wavefront): dΣ in full and a strided sample of the
+    # Katanin pass against the oracle, FP32-mixed q3 against FP64
+    prob = F.Problem.build("square", 3, [[1.0]], initial=1.0, num_sigma=63, num_Omega=63, num_nu=63, num_chi=10)
+    ns, m = prob.r.n_sites, prob.mesh
+    assert len(m.Os) == len(m.ns) == len(m.sigma) == 64
+    a = A.synthetic_action(ns, m, prob.bare, seed=20261204)
+    outs = {}
+    for prec in (0, FP32_MIXED):
+        with prob.context(n_slots=2, precision=prec) as ctx:
+            ctx.upload(0, a)
+            ctx.compute_dsigma(0.5, 0, 1)
+            ctx.compute_dgamma_1l(0.5, 0, 1, 10)
+            outs[prec] = ctx.download(1)
+    got = outs[0]
+    orc = O.Oracle(prob.r, m)
+    ref_s = A.Action.empty(ns, m)
+    orc.compute_dsigma(0.5, a, ref_s)
+    es = rel_err(got.Sigma, ref_s.Sigma)
+    dd = A.Action.empty(ns, m)
+    dd.Sigma[:] = got.Sigma
+    orc.compute_corrs(0.5, a, dd, True)
+    out = A.Action.empty(ns, m)
+    sample = half_elements(m.num_Omega, m.num_nu)[::2999]
+    for e in sample:
+        orc.compute_element(O.P_KAT, 0.5, a, dd, *e, 10, out)
+    scale = float(np.max(np.abs(got.gamma)))
+    ev = max(float(np.max(np.abs(entries(got, *e) - entries(out, *e)))) for e in sample) / scale
+    e32 = max(rel_err(outs[FP32_MIXED].view(c, ch, "q3"), got.view(c, ch, "q3")) for c in range(2) for ch in range(3))
+    print(f"square L=3 N=63 (64 points): dSigma {es:.2e}; {len(sample)} sampled elements {ev:.2e}; "
+          f"FP32-mixed q3 {e32:.2e}", flush=True)
+    assert es < 1e-12 and ev < 1e-12
+    assert e32 < 1e-4
