@@ -4,7 +4,7 @@ right-hand side against the FP64 oracle on strided samples of the output element
 FP32-mixed against FP64 on every element; at the pyrochlore 3-loop flow's shape (L=4, N=32),
 the central pass of the multiloop right-hand side; at the warm-start shape (kagome L=5, N=32),
 the parquet SDE and BSE passes; at the largest meshes the library takes (64 points per axis), the
-Katanin pass.
+Katanin pass; above 32 sites (one group per wavefront), the Katanin and left passes.
 
 Tolerances: FP64 ≤1e-12 relative to the largest entry (the pass restates the oracle's arithmetic
 and takes the same adaptive decisions; the oracle's elements read the GPU's dΣ so that they test
@@ -172,4 +172,53 @@ def test_largest_meshes_katanin_rhs_matches_oracle_sample():
     print(f"square L=3 N=63 (64 points): dSigma {es:.2e}; {len(sample)} sampled elements {ev:.2e}; "
           f"FP32-mixed q3 {e32:.2e}", flush=True)
     assert es < 1e-12 and ev < 1e-12
+    assert e32 < 1e-4
+
+
+@pytest.mark.parametrize("name,size", [("square", 12), ("pyrochlore", 6)])
+def test_more_than_32_sites_matches_oracle_sample(name, size):
+    # 49 and 48 sites: one group per wavefront with 15 or 16 idle lanes (the layout of every
+    # lattice above 32 sites), overlap tables of 7000-8000 rows in LDS.  dΣ in full, sampled
+    # Katanin elements and (square) sampled left-pass elements against the oracle, FP32-mixed
+    # q3 against FP64
+    prob = F.Problem.build(name, size, [[1.0]], initial=1.0, num_sigma=10, num_Omega=8, num_nu=8, num_chi=10)
+    ns, m = prob.r.n_sites, prob.mesh
+    assert ns > 32
+    a = A.synthetic_action(ns, m, prob.bare, seed=20261205)
+    outs = {}
+    for prec in (0, FP32_MIXED):
+        with prob.context(n_slots=4, precision=prec) as ctx:
+            ctx.upload(0, a)
+            ctx.compute_dsigma(1.0, 0, 1)
+            ctx.compute_dgamma_1l(1.0, 0, 1, 10)
+            kat = ctx.download(1)
+            ctx.compute_dsigma(1.0, 0, 2)
+            ctx.compute_dgamma_2l(1.0, 0, 2, 3, 10)
+            outs[prec] = (kat, ctx.download(3))
+    got, left = outs[0]
+    orc = O.Oracle(prob.r, m)
+    ref_s = A.Action.empty(ns, m)
+    orc.compute_dsigma(1.0, a, ref_s)
+    es = rel_err(got.Sigma, ref_s.Sigma)
+    dd = A.Action.empty(ns, m)
+    dd.Sigma[:] = got.Sigma
+    orc.compute_corrs(1.0, a, dd, True)
+    out = A.Action.empty(ns, m)
+    sample = half_elements(m.num_Omega, m.num_nu)[::11]
+    for e in sample:
+        orc.compute_element(O.P_KAT, 1.0, a, dd, *e, 10, out)
+    ev = max(float(np.max(np.abs(entries(got, *e) - entries(out, *e)))) for e in sample) / float(np.max(np.abs(got.gamma)))
+    el = 0.0
+    if name == "square":
+        orc.compute_corrs(1.0, a, a, False)
+        out_l = A.Action.empty(ns, m)
+        sample_l = full_elements(m.num_Omega, m.num_nu)[::23]
+        for e in sample_l:
+            orc.compute_element(O.P_LEFT, 1.0, a, got, *e, 10, out_l)
+        el = max(float(np.max(np.abs(entries(left, *e) - entries(out_l, *e)))) for e in sample_l) / \
+            float(np.max(np.abs(left.gamma)))
+    e32 = max(rel_err(outs[FP32_MIXED][0].view(c, ch, "q3"), got.view(c, ch, "q3")) for c in range(2) for ch in range(3))
+    print(f"{name} L={size} ({ns} sites): dSigma {es:.2e}; {len(sample)} Katanin elements {ev:.2e}; left {el:.2e}; "
+          f"FP32-mixed q3 {e32:.2e}", flush=True)
+    assert es < 1e-12 and ev < 1e-12 and el < 1e-12
     assert e32 < 1e-4
