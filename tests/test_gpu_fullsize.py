@@ -4,7 +4,7 @@ right-hand side against the FP64 oracle on strided samples of the output element
 FP32-mixed against FP64 on every element; at the pyrochlore 3-loop flow's shape (L=4, N=32),
 the central pass of the multiloop right-hand side; at the warm-start shape (kagome L=5, N=32),
 the parquet SDE and BSE passes; at the largest meshes the library takes (64 points per axis), the
-Katanin pass; above 32 sites (one group per wavefront), the Katanin and left passes.
+Katanin pass; at each wavefront layout by site count (2 to 49 sites), the Katanin and left passes.
 
 Tolerances: FP64 ≤1e-12 relative to the largest entry (the pass restates the oracle's arithmetic
 and takes the same adaptive decisions; the oracle's elements read the GPU's dΣ so that they test
@@ -175,15 +175,16 @@ def test_largest_meshes_katanin_rhs_matches_oracle_sample():
     assert e32 < 1e-4
 
 
-@pytest.mark.parametrize("name,size", [("square", 12), ("pyrochlore", 6)])
-def test_more_than_32_sites_matches_oracle_sample(name, size):
-    # 49 and 48 sites: one group per wavefront with 15 or 16 idle lanes (the layout of every
-    # lattice above 32 sites), overlap tables of 7000-8000 rows in LDS.  dΣ in full, sampled
-    # Katanin elements and (square) sampled left-pass elements against the oracle, FP32-mixed
-    # q3 against FP64
+@pytest.mark.parametrize("name,size,groups", [("square", 1, 16), ("square", 9, 2), ("pyrochlore", 5, 2),
+                                               ("square", 12, 1), ("pyrochlore", 6, 1)])
+def test_site_layouts_match_oracle_sample(name, size, groups):
+    # the wavefront layouts by site count: 2 sites (16 groups of 2 lanes, half the wavefront
+    # idle), 30 and 31 sites (2 groups), 49 and 48 sites (one group with 15 or 16 idle lanes,
+    # overlap tables of 7000-8000 rows in LDS).  dΣ in full, sampled Katanin elements and
+    # (square) sampled left-pass elements against the oracle, FP32-mixed q3 against FP64
     prob = F.Problem.build(name, size, [[1.0]], initial=1.0, num_sigma=10, num_Omega=8, num_nu=8, num_chi=10)
     ns, m = prob.r.n_sites, prob.mesh
-    assert ns > 32
+    assert min(64 // ns, 16) == groups
     a = A.synthetic_action(ns, m, prob.bare, seed=20261205)
     outs = {}
     for prec in (0, FP32_MIXED):
