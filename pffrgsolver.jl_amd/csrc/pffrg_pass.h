@@ -922,14 +922,16 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
                 C.issued = cnt; C.pend = done ? 0 : lvl - cnt;
             }
             __syncthreads();
-            if (real_lane) {
+            // group g takes pooled node g: the first owner o with g < (pending nodes of owners
+            // before o) + (pending of o).  The pending counts are the same on every lane, so the
+            // loop runs on scalar registers and ends once all G nodes are handed out.
+            {
                 int cum = 0;
-                for (int o = 0; o < G; ++o) {
-                    const int p = ctl[o].pend;
-                    if (g < cum + p) {
+                for (int o = 0; o < G && cum < G; ++o) {
+                    const int p = __builtin_amdgcn_readfirstlane(ctl[o].pend);
+                    if (real_lane && own < 0 && g < cum + p) {
                         own = o;
                         kk = ctl[o].issued + (g - cum);
-                        break;
                     }
                     cum += p;
                 }
@@ -992,7 +994,7 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
         bool need_check = false;
         if (real_lane && !done) {
             for (int e = 0; e < G; ++e) {
-                if (S.own[e] != g) continue;
+                if (__builtin_amdgcn_readfirstlane(S.own[e]) != g) continue;
                 const double u0 = sc[e * (GS + 1) + i], u1 = sc[SR + e * (GS + 1) + i];
                 if (init && cnt < 3) {
                     b10 += u0; b11 += u1;
