@@ -154,17 +154,36 @@ def cpu_baseline(prob, a, da_gpu, loops: int, eval_: int, budget_s: float):
         _sample_elements(orc, ps, aa, dd, elems, idx, scratch, eval_, workers=1)
         est1 += (time.perf_counter() - t1) * len(elems) / max(1, len(idx))
     orc.set_threads(threads)
-    model = "unknown"
-    try:
-        model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
-    except (OSError, StopIteration):
-        pass
     extra = {"single_thread_value": 1.0 / est1,
              "single_thread_sample": f"1 in {stride1} elements of both vertex passes + full dSigma on 1 thread, "
                                      f"extrapolated to one RHS ({est1:.1f} s)",
-             "cpu_model": model, "host_cpus": os.cpu_count(),
-             "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+             **host_cpu_info()}
     return 1.0 / total_est, threads, sample, extra
+
+
+def host_cpu_info() -> dict:
+    """The host CPU as lscpu names it, the CPUs this process may use and the thread settings.
+    The reference's own Julia solver is not run (no Julia on the machine), so its thread count
+    is recorded as not applicable."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if "Model name" not in info:
+        try:
+            info["Model name"] = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
+                                      if l.startswith("model name"))
+        except (OSError, StopIteration):
+            info["Model name"] = "unknown"
+    return {"cpu_model": info.get("Model name"), "lscpu": info, "host_cpus": os.cpu_count(),
+            "usable_cpus": len(os.sched_getaffinity(0)), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "JULIA_NUM_THREADS": "n/a (reference Julia not run: no Julia on this machine)"}
 
 
 def _sample_elements(orc, ps, aa, dd, elems, idx, out, eval_, workers=None):
@@ -392,7 +411,7 @@ def flow_workload(args):
         cpu = {"value": 1.0 / est, "unit": "flows/s", "cores": threads, "kind": "port",
                "sample": f"FP64 oracle RHS and chi at the flow's first and last Lambda {[round(x[0], 3) for x in samples]} "
                          f"(mean {t_rhs:.3f} s/RHS, {t_chi:.3f} s/chi) x the GPU flow's {rref.rhs_evals} RHS and "
-                         f"{rref.accepted} chi; estimated {est:.1f} s per flow"}
+                         f"{rref.accepted} chi; estimated {est:.1f} s per flow", **host_cpu_info()}
     tv = t32 if t32 is not None else t64
     line = {
         "metric": "full-flow wall time (FP32-mixed vs plain FP64 GPU) at chi rel err <= 1e-4",

@@ -57,10 +57,11 @@ const char *pffrg_last_error(const pffrg_ctx *ctx);
 int pffrg_set_mesh(pffrg_ctx *ctx, const double *sigma, const double *Os, const double *ns, const double *Ot,
                    const double *nt);
 
-/* Reduced_lattice tables (Lattice/reduced.jl:15-25): exchange[n_sites], mult[n_sites],
- * overlap as CSR: ov_off[n_sites+1] row offsets, ov_rows[3*R] = (j1, j2, multiplicity).
- * index_base = 1 for the reference's 1-based Julia tables, 0 for 0-based ones. */
-int pffrg_set_lattice(pffrg_ctx *ctx, const int64_t *exchange, const double *mult, const int64_t *ov_off,
+/* Reduced_lattice tables (Lattice/reduced.jl:15-25): exchange[n_sites], mult[n_sites] (the Int64
+ * orbit sizes, >= 1), overlap as CSR: ov_off[n_sites+1] row offsets, ov_rows[3*R] = (j1, j2,
+ * multiplicity).  index_base = 1 for the reference's 1-based Julia tables, 0 for 0-based ones.
+ * A rejected call (bad index, or a device error) keeps the previous lattice. */
+int pffrg_set_lattice(pffrg_ctx *ctx, const int64_t *exchange, const int64_t *mult, const int64_t *ov_off,
                       const int64_t *ov_rows, int32_t index_base);
 
 /* Host <-> device transfer of one action slot; NULL skips a part. */

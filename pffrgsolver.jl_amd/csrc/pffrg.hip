@@ -295,14 +295,6 @@ struct Slot {
     bool gat64_valid = false, gat32_valid = false;
 };
 
-// mesh-rescanning scratch: argmax partials (values, positions), 6 x 4 positions, 6 cut sets of
-// n_Omega + 3 n_nu <= 4 MAXN values, the five new meshes
-constexpr size_t SCAN_OFF_PIDX = 6 * SCAN_BLOCKS * 8;
-constexpr size_t SCAN_OFF_IDX = SCAN_OFF_PIDX + 6 * SCAN_BLOCKS * 8;
-constexpr size_t SCAN_OFF_CUTS = SCAN_OFF_IDX + 256;
-constexpr size_t SCAN_OFF_MESH = SCAN_OFF_CUTS + 6 * 4 * MAXN * 8;
-constexpr size_t SCAN_MEM_BYTES = SCAN_OFF_MESH + 5 * MAXN * 8;
-
 struct pffrg_ctx {
     pffrg_dims dims{};
     Sys sys{};
@@ -831,6 +823,8 @@ void pffrg_destroy(pffrg_ctx *c) {
     (void)hipFree(c->d_el_full);
     (void)hipFree(c->d_el_red);
     (void)hipFree(c->d_logpts);
+    (void)hipFree(c->d_dbg_nodes);
+    (void)hipFree(c->d_dbg_res);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -851,16 +845,20 @@ int pffrg_set_mesh(pffrg_ctx *c, const double *sigma, const double *Os, const do
     return apply_mesh(c, h);
 }
 
-int pffrg_set_lattice(pffrg_ctx *c, const int64_t *exchange, const double *mult, const int64_t *ov_off,
+int pffrg_set_lattice(pffrg_ctx *c, const int64_t *exchange, const int64_t *mult, const int64_t *ov_off,
                       const int64_t *ov_rows, int32_t base) {
     if (!c || !exchange || !mult || !ov_off || !ov_rows) return fail(c, PFFRG_ERR_ARG, "null lattice pointer");
     if (base != 0 && base != 1) return fail(c, PFFRG_ERR_ARG, "index_base must be 0 or 1");
     HIPCHK(c, hipSetDevice(c->device));
     const int ns = c->sys.ns;
     std::vector<int> ex(ns), off(ns + 1);
+    std::vector<double> mu(ns);
     for (int i = 0; i < ns; ++i) {
         ex[i] = (int)(exchange[i] - base);
         if (ex[i] < 0 || ex[i] >= ns) return fail(c, PFFRG_ERR_ARG, "exchange index out of range");
+        // orbit sizes (Reduced_lattice.mult, Int64); the self-energy kernel weights sites by them in FP64
+        if (mult[i] < 1 || mult[i] > (int64_t(1) << 40)) return fail(c, PFFRG_ERR_ARG, "site multiplicity out of range");
+        mu[i] = (double)mult[i];
     }
     if (ov_off[0] != 0) return fail(c, PFFRG_ERR_ARG, "ov_off[0] must be 0");
     for (int i = 0; i <= ns; ++i) off[i] = (int)ov_off[i];
@@ -876,19 +874,37 @@ int pffrg_set_lattice(pffrg_ctx *c, const int64_t *exchange, const double *mult,
             return fail(c, PFFRG_ERR_ARG, "overlap row out of range");
         rows[r] = (uint32_t)j1 | ((uint32_t)j2 << 8) | ((uint32_t)m << 16);
     }
+    // new tables are built aside and replace the old ones only when every allocation and copy
+    // succeeded, so a failed call keeps the previous lattice usable
+    int *n_exch = nullptr, *n_off = nullptr;
+    uint32_t *n_rows = nullptr;
+    double *n_mult = nullptr;
+    bool ok = hipMalloc(&n_exch, ns * sizeof(int)) == hipSuccess &&
+              hipMalloc(&n_off, (ns + 1) * sizeof(int)) == hipSuccess &&
+              hipMalloc(&n_rows, rows.size() * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&n_mult, ns * sizeof(double)) == hipSuccess &&
+              hipMemcpy(n_exch, ex.data(), ns * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(n_off, off.data(), (ns + 1) * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(n_rows, rows.data(), rows.size() * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(n_mult, mu.data(), ns * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        const hipError_t e = hipGetLastError();
+        (void)hipFree(n_exch);
+        (void)hipFree(n_off);
+        (void)hipFree(n_rows);
+        (void)hipFree(n_mult);
+        return fail(c, PFFRG_ERR_HIP, std::string("set_lattice: device tables not created: ") + hipGetErrorString(e));
+    }
+    // the old tables may still be read by queued work
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     (void)hipFree(c->d_exch);
     (void)hipFree(c->d_ov_off);
     (void)hipFree(c->d_ov_rows);
     (void)hipFree(c->d_mult);
-    c->d_exch = nullptr; c->d_ov_off = nullptr; c->d_ov_rows = nullptr; c->d_mult = nullptr;
-    HIPCHK(c, hipMalloc(&c->d_exch, ns * sizeof(int)));
-    HIPCHK(c, hipMalloc(&c->d_ov_off, (ns + 1) * sizeof(int)));
-    HIPCHK(c, hipMalloc(&c->d_ov_rows, rows.size() * sizeof(uint32_t)));
-    HIPCHK(c, hipMalloc(&c->d_mult, ns * sizeof(double)));
-    HIPCHK(c, hipMemcpy(c->d_exch, ex.data(), ns * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_ov_off, off.data(), (ns + 1) * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_ov_rows, rows.data(), rows.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_mult, mult, ns * sizeof(double), hipMemcpyHostToDevice));
+    c->d_exch = n_exch;
+    c->d_ov_off = n_off;
+    c->d_ov_rows = n_rows;
+    c->d_mult = n_mult;
     c->n_ov = R;
     c->have_lattice = true;
     return 0;
@@ -1709,6 +1725,23 @@ int pffrg_compute_chi(pffrg_ctx *c, double L, int32_t a, const double *chi_mesh,
     return rc ? fail(c, rc, msg) : 0;
 }
 
+// scan_channel's argmax and cuts of slot a into the scratch at mem (SCAN_CUTS_BYTES), then to the host
+static int scan_cuts_into(pffrg_ctx *c, int a, char *mem, int32_t *idx, double *cuts) {
+    ScanArgs A;
+    A.y = c->sys;
+    A.gamma = c->slots[a].gamma;
+    A.pval = (double *)mem;
+    A.pidx = (long long *)(mem + SCAN_OFF_PIDX);
+    A.idx = (int *)(mem + SCAN_OFF_IDX);
+    A.cuts = (double *)(mem + SCAN_OFF_CUTS);
+    HIPCHK(c, launch_scan_cuts(A, c->stream));
+    const int nc = c->sys.nO + 3 * c->sys.nn;
+    HIPCHK(c, hipMemcpyAsync(idx, A.idx, 24 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cuts, A.cuts, 6 * (size_t)nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 // scan_channel's argmax and cuts (Action.jl:449-480)
 int pffrg_scan_cuts(pffrg_ctx *c, int32_t a, int32_t *idx, double *cuts) {
     int rc;
@@ -1716,19 +1749,7 @@ int pffrg_scan_cuts(pffrg_ctx *c, int32_t a, int32_t *idx, double *cuts) {
     if (!idx || !cuts) return fail(c, PFFRG_ERR_ARG, "null output pointer");
     HIPCHK(c, hipSetDevice(c->device));
     if ((rc = ensure_scan_mem(c))) return rc;
-    ScanArgs A;
-    A.y = c->sys;
-    A.gamma = c->slots[a].gamma;
-    A.pval = (double *)c->scan_mem;
-    A.pidx = (long long *)(c->scan_mem + SCAN_OFF_PIDX);
-    A.idx = (int *)(c->scan_mem + SCAN_OFF_IDX);
-    A.cuts = (double *)(c->scan_mem + SCAN_OFF_CUTS);
-    HIPCHK(c, launch_scan_cuts(A, c->stream));
-    const int nc = c->sys.nO + 3 * c->sys.nn;
-    HIPCHK(c, hipMemcpyAsync(idx, A.idx, 24 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(cuts, A.cuts, 6 * (size_t)nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return 0;
+    return scan_cuts_into(c, a, c->scan_mem, idx, cuts);
 }
 
 // resample_from_to (Action.jl:531-556): slot src onto new meshes into slot dst

@@ -11,6 +11,16 @@ namespace pffrg {
 
 constexpr int SCAN_BLOCKS = 64;  // partial-argmax blocks per (component, channel)
 
+// mesh-rescanning scratch: argmax partials (values, positions), 6 x 4 positions, 6 cut sets of
+// n_Omega + 3 n_nu <= 4 MAXN values, the five new meshes
+constexpr size_t SCAN_OFF_PIDX = 6 * SCAN_BLOCKS * 8;
+constexpr size_t SCAN_OFF_IDX = SCAN_OFF_PIDX + 6 * SCAN_BLOCKS * 8;
+constexpr size_t SCAN_OFF_CUTS = SCAN_OFF_IDX + 256;
+constexpr size_t SCAN_OFF_MESH = SCAN_OFF_CUTS + 6 * 4 * MAXN * 8;
+constexpr size_t SCAN_MEM_BYTES = SCAN_OFF_MESH + 5 * MAXN * 8;
+// what one scan_cuts call uses of it (everything before the new meshes)
+constexpr size_t SCAN_CUTS_BYTES = SCAN_OFF_MESH;
+
 struct ScanArgs {
     Sys y;
     const double *gamma;  // the slot's vertex (reference layout)

@@ -555,7 +555,7 @@ class ReducedLattice:
     ints: list                # integer coordinates of irreducible sites
     vecs: np.ndarray
     overlap: list             # per irreducible site: (rows, 3) int64 of (j1, j2, mult), 0-based
-    mult: np.ndarray          # float64
+    mult: np.ndarray          # int64 orbit sizes (Reduced_lattice.mult)
     exchange: np.ndarray      # int64, 0-based
     project: np.ndarray       # (N, N) int64, -1 = out of range
 
@@ -588,7 +588,7 @@ def get_reduced_lattice(model: str, J: list[list[float]], l: Lattice) -> Reduced
         for p in temp:
             counts[p] += 1
         overlap.append(np.array([[pos[p[0]], pos[p[1]], counts[p]] for p in pairs], dtype=np.int64).reshape(-1, 3))
-    mult = np.array([float(np.sum(reduced == irr)) for irr in irreducible])
+    mult = np.array([int(np.sum(reduced == irr)) for irr in irreducible], dtype=np.int64)
     exchange = np.array([pos[int(mappings[irr, 0])] for irr in irreducible], dtype=np.int64)
     project = np.full(mappings.shape, -1, dtype=np.int64)
     ok = mappings >= 0

@@ -41,7 +41,7 @@ _SIGS = {
     "pffrg_destroy": ([_P], None),
     "pffrg_last_error": ([_P], C.c_char_p),
     "pffrg_set_mesh": ([_P, _D, _D, _D, _D, _D], C.c_int),
-    "pffrg_set_lattice": ([_P, _I64, _D, _I64, _I64, _i], C.c_int),
+    "pffrg_set_lattice": ([_P, _I64, _I64, _I64, _I64, _i], C.c_int),
     "pffrg_upload_action": ([_P, _i, _D, _D, _D], C.c_int),
     "pffrg_download_action": ([_P, _i, _D, _D, _D], C.c_int),
     "pffrg_gamma_length": ([_P], C.c_int64),
@@ -195,12 +195,12 @@ class Context:
     def set_lattice(self, r):
         off, rows = r.overlap_csr()
         ex = np.ascontiguousarray(r.exchange, dtype=np.int64)
-        mult = np.ascontiguousarray(r.mult, dtype=np.float64)
+        mult = np.ascontiguousarray(np.rint(r.mult), dtype=np.int64)
         off = np.ascontiguousarray(off, dtype=np.int64)
         rows = np.ascontiguousarray(rows, dtype=np.int64).reshape(-1)
         if rows.size == 0:
             rows = np.zeros(3, dtype=np.int64)
-        self._ck(self.lib.pffrg_set_lattice(self.h, _ip(ex), _dp(mult), _ip(off), _ip(rows), 0), "set_lattice")
+        self._ck(self.lib.pffrg_set_lattice(self.h, _ip(ex), _ip(mult), _ip(off), _ip(rows), 0), "set_lattice")
 
     def set_precision(self, mode: int):
         self._ck(self.lib.pffrg_set_precision(self.h, mode), "set_precision")

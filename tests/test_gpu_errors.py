@@ -53,18 +53,24 @@ def test_argument_and_state_errors_leave_the_context_usable():
         off, rows = r.overlap_csr()
         off = np.ascontiguousarray(off, dtype=np.int64)
         rows = np.ascontiguousarray(rows, dtype=np.int64).reshape(-1)
-        mult = np.ascontiguousarray(r.mult, dtype=np.float64)
+        mult = np.ascontiguousarray(np.rint(r.mult), dtype=np.int64)
         ex = np.array(r.exchange, dtype=np.int64)
         ex[0] = r.n_sites
-        rc = ctx.lib.pffrg_set_lattice(ctx.h, native._ip(ex), native._dp(mult), native._ip(off), native._ip(rows), 0)
+        rc = ctx.lib.pffrg_set_lattice(ctx.h, native._ip(ex), native._ip(mult), native._ip(off), native._ip(rows), 0)
         assert rc == -1 and b"exchange index out of range" in ctx.lib.pffrg_last_error(ctx.h)
         bad_rows = rows.copy()
         bad_rows[0] = r.n_sites
-        rc = ctx.lib.pffrg_set_lattice(ctx.h, native._ip(np.ascontiguousarray(r.exchange, dtype=np.int64)),
-                                       native._dp(mult), native._ip(off), native._ip(bad_rows), 0)
+        ex_ok = np.ascontiguousarray(r.exchange, dtype=np.int64)
+        rc = ctx.lib.pffrg_set_lattice(ctx.h, native._ip(ex_ok), native._ip(mult), native._ip(off),
+                                       native._ip(bad_rows), 0)
         assert rc == -1 and b"overlap row out of range" in ctx.lib.pffrg_last_error(ctx.h)
+        bad_mult = mult.copy()
+        bad_mult[0] = 0
+        rc = ctx.lib.pffrg_set_lattice(ctx.h, native._ip(ex_ok), native._ip(bad_mult), native._ip(off),
+                                       native._ip(rows), 0)
+        assert rc == -1 and b"multiplicity out of range" in ctx.lib.pffrg_last_error(ctx.h)
         got = _rhs(ctx, a)
-        print(f"{len(cases) + 3} rejected calls; RHS afterwards equals a fresh context's bitwise", flush=True)
+        print(f"{len(cases) + 4} rejected calls; RHS afterwards equals a fresh context's bitwise", flush=True)
         np.testing.assert_array_equal(got.gamma, ref.gamma)
         np.testing.assert_array_equal(got.Sigma, ref.Sigma)
 
