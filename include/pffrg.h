@@ -174,6 +174,16 @@ int pffrg_compute_chi(pffrg_ctx *ctx, double Lambda, int32_t a, const double *ch
  *   q3[i, w, v, x]  - q3[i, w, v, end]   (n_nu values each, x = 0 .. n_nu-1). */
 int pffrg_scan_cuts(pffrg_ctx *ctx, int32_t a, int32_t *idx, double *cuts);
 
+/* pffrg_scan_cuts without device allocation.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments and outputs as pffrg_scan_cuts; its argmax partials and
+ * cuts live in one static 18.7 KB scratch per GPU (held until the process ends) instead of the
+ * context's lazily allocated one.  Every context on a GPU shares that scratch, so calls that
+ * overlap on one GPU (several host threads or streams) race; each call overwrites what it reads.
+ * Rejects (PFFRG_ERR_LIMIT) meshes whose cuts exceed the scratch, which the n_Omega, n_nu <= 64
+ * limits of pffrg_create already exclude. */
+int pffrg_scan_cuts_noalloc(pffrg_ctx *ctx, int32_t a, int32_t *idx, double *cuts);
+
 /* scan (Action.jl:369-446), host only: the new linear extent of a mesh x[0..n-1] (built by
  * get_mesh with linear fraction p[0]) for the cut y[0..n-1], from the peak and origin
  * heuristics with bounds p[1] < Δ < p[2] and p[3] <= δ <= p[4] (p[3], p[4] already

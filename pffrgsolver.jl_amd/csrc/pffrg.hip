@@ -329,6 +329,8 @@ struct pffrg_ctx {
     void *chi_mem = nullptr;
     // mesh-rescanning scratch (argmax partials, cuts, new meshes), SCAN_MEM_BYTES
     char *scan_mem = nullptr;
+    // this device's static scan_cuts scratch (pffrg_scan_cuts_noalloc), looked up at creation
+    char *scan_scratch = nullptr;
     size_t chi_bytes = 0;
     // host copies of the meshes and the log-domain breakpoint table
     std::vector<double> h_Os, h_Ot;
@@ -764,7 +766,10 @@ int pffrg_create(const pffrg_dims *d, pffrg_ctx **out) {
     y.q3 = y.q22 + (long)y.ns * y.nO * y.nn;
     y.chlen = y.q3 + (long)y.ns * y.nO * y.nn * y.nn;
     y.complen = 3 * y.chlen;
-    bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+    void *scratch_ptr = nullptr;
+    bool ok = scan_scratch_address(d->device, &scratch_ptr) == hipSuccess;
+    c->scan_scratch = static_cast<char *>(scratch_ptr);
+    ok = ok && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipMalloc(&c->d_mesh, MESH_TABLE * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_corrs, 2 * 3 * y.nO * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_count, 2 * sizeof(unsigned long long)) == hipSuccess;
@@ -1750,6 +1755,20 @@ int pffrg_scan_cuts(pffrg_ctx *c, int32_t a, int32_t *idx, double *cuts) {
     HIPCHK(c, hipSetDevice(c->device));
     if ((rc = ensure_scan_mem(c))) return rc;
     return scan_cuts_into(c, a, c->scan_mem, idx, cuts);
+}
+
+// pffrg_scan_cuts on this device's static scratch: no device memory is allocated or freed
+int pffrg_scan_cuts_noalloc(pffrg_ctx *c, int32_t a, int32_t *idx, double *cuts) {
+    int rc;
+    if ((rc = ready(c)) || (rc = check_slot(c, a))) return rc;
+    if (!idx || !cuts) return fail(c, PFFRG_ERR_ARG, "null output pointer");
+    const size_t need = SCAN_OFF_CUTS + 6 * (size_t)(c->sys.nO + 3 * c->sys.nn) * sizeof(double);
+    if (need > SCAN_CUTS_BYTES)
+        return fail(c, PFFRG_ERR_LIMIT, "scan_cuts_noalloc: cuts of " + std::to_string(need) +
+                                            " bytes exceed the static scratch of " + std::to_string(SCAN_CUTS_BYTES));
+    if (!c->scan_scratch) return fail(c, PFFRG_ERR_STATE, "scan_cuts_noalloc: no static scratch on this device");
+    HIPCHK(c, hipSetDevice(c->device));
+    return scan_cuts_into(c, a, c->scan_scratch, idx, cuts);
 }
 
 // resample_from_to (Action.jl:531-556): slot src onto new meshes into slot dst

@@ -39,6 +39,10 @@ struct ResampleArgs {
 };
 
 hipError_t launch_scan_cuts(const ScanArgs &A, hipStream_t stream);
+
+// Device address of this GPU's static scan_cuts scratch (SCAN_CUTS_BYTES, one per device for the
+// life of the process), looked up once per device with that device current.
+hipError_t scan_scratch_address(int device, void **ptr);
 hipError_t launch_resample(const ResampleArgs &A, hipStream_t stream);
 
 }  // namespace pffrg

@@ -9,6 +9,7 @@
 // The file is compiled without FMA contraction so that every interpolated value is the one
 // the reference's sequence of products and sums gives.
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "pffrg.h"
@@ -218,6 +219,40 @@ hipError_t launch_scan_cuts(const ScanArgs &A, hipStream_t stream) {
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_scan_final, dim3(6), dim3(256), 0, stream, A);
     return hipGetLastError();
+}
+
+// Scratch of the allocation-free scan_cuts entry (pffrg_scan_cuts_noalloc): the argmax partials,
+// the maximum's position and the cuts of one call, SCAN_CUTS_BYTES (18.7 KB).  Each GPU holds its
+// own copy for the life of the process, shared by every context on that GPU: calls that overlap
+// on one GPU race.  Every byte a call reads is written by the same call first, so nothing needs
+// clearing between calls.  Host code must not pass `scratch` itself to a kernel (on the host the
+// name stands for a host-side shadow, not the device copy): it uses the address that
+// scan_scratch_address looks up on each device.
+static __device__ __align__(256) unsigned char scratch[SCAN_CUTS_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+}  // namespace
+
+hipError_t scan_scratch_address(int device, void **ptr) {
+    *ptr = nullptr;
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return hipSuccess;
 }
 
 hipError_t launch_resample(const ResampleArgs &A, hipStream_t stream) {

@@ -72,6 +72,7 @@ _SIGS = {
     "pffrg_gather_ceiling": ([_P, _i, _i, _i, C.c_int64, _i, _i, _D, _D], C.c_int),
     "pffrg_compute_chi": ([_P, _d, _i, _D, _i, _D, _D, C.POINTER(C.c_int32)], C.c_int),
     "pffrg_scan_cuts": ([_P, _i, C.POINTER(C.c_int32), _D], C.c_int),
+    "pffrg_scan_cuts_noalloc": ([_P, _i, C.POINTER(C.c_int32), _D], C.c_int),
     "pffrg_scan": ([_D, _D, _i, _D, _D], C.c_int),
     "pffrg_resample": ([_P, _i, _i, _D, _D, _D, _D, _D], C.c_int),
     "pffrg_mlp_create": ([_i, _i, _i, _i, C.c_uint64, _i, C.POINTER(_P)], C.c_int),
