@@ -34,7 +34,11 @@ extern "C" {
 
 /* arithmetic of the vertex passes */
 #define PFFRG_FP64 0        /* FP64 storage, interpolation, products and accumulation */
-#define PFFRG_FP32_MIXED 1  /* FP32 vertex copy, interpolation and products; FP64 quadrature */
+#define PFFRG_FP32_MIXED 1  /* FP32 vertex copy, interpolation and products; FP64 quadrature sums,
+                               Simpson sums and convergence tests; the elements of one (channel, Ω)
+                               share node evaluations (shared-node pass) */
+#define PFFRG_FP32_ELEMENT 2 /* the arithmetic of PFFRG_FP32_MIXED in the per-element pass
+                               (measurement and comparison) */
 
 typedef struct pffrg_ctx pffrg_ctx;
 
@@ -124,6 +128,10 @@ int pffrg_reset_stats(pffrg_ctx *ctx);
 /* stats[0] = quadrature nodes evaluated by vertex passes, stats[1] = vertex-pass elements,
  * stats[2] = vertex-pass launches, stats[3] = vertex bytes gathered by the interpolations */
 int pffrg_get_stats(pffrg_ctx *ctx, int64_t *stats);
+/* the context's n <= 16 raw device counters since pffrg_reset_stats: [0] quadrature nodes
+ * (element-nodes), [1] gathered bytes; [2..] cycle counts of the shared-node pass's phases in a
+ * build with PFFRG_SHARED_PROFILE defined (zero otherwise) */
+int pffrg_get_counters(pffrg_ctx *ctx, int64_t *out, int32_t n);
 /* boundary-correction factors corrs[Ω][channel][side] (2*3*n_Omega doubles) of the last pass */
 int pffrg_get_corrs(pffrg_ctx *ctx, double *corrs);
 /* integrand of one output element (pass 0..4 = Katanin, left, central, BSE, reduced SDE; channel

@@ -26,6 +26,7 @@
 #include "pffrg_pass.h"
 #include "pffrg_probe.h"
 #include "pffrg_rescan.h"
+#include "pffrg_shared.h"
 
 using namespace pffrg;
 
@@ -313,6 +314,12 @@ struct pffrg_ctx {
     unsigned long long *d_max = nullptr;
     uint32_t *d_el_half = nullptr, *d_el_full = nullptr, *d_el_red = nullptr;
     int n_half = 0, n_full = 0, n_red = 0;
+    // shared-node pass (FP32-mixed): work units per element-list mode, tile shape, partial sums
+    uint32_t *d_units[3] = {nullptr, nullptr, nullptr};
+    int n_units[3] = {0, 0, 0};
+    int sh_nw = 1, sh_spt = 1;
+    double *d_part = nullptr;
+    size_t part_bytes = 0;
     int precision = PFFRG_FP64;
     long long stat_elems = 0, stat_launches = 0;
     bool timing = false;
@@ -357,6 +364,10 @@ struct pffrg_batch {
     size_t tab_bytes = 0;
     void *chi_mem = nullptr;       // correlation cubature scratch of batched χ
     size_t chi_bytes = 0;
+    // shared-node pass (FP32-mixed): the work units of all problems, problem index in bits 26-31
+    uint32_t *d_sh_units = nullptr;
+    int n_sh_units = 0;
+    void *d_sh_tab = nullptr;
 };
 
 namespace {
@@ -554,13 +565,114 @@ int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const
     return 0;
 }
 
+// Launch shape of the shared-node pass: 8 waves, each holding the 16 x 16 tile on
+// ceil(n_sites / 8) <= SH_SPT sites (with fewer than 8 sites some waves hold none and only help
+// with the descriptors and interpolations).  Every problem has the same block, so a batched
+// launch runs each problem exactly as its own launch does.  Problems with more sites use the
+// per-element pass.
+bool shared_fits(const pffrg_ctx *c) { return c->sys.ns <= 8 * SH_SPT; }
+
+void shared_shape(pffrg_ctx *c) {
+    c->sh_nw = 8;
+    c->sh_spt = (c->sys.ns + 7) / 8;
+}
+
+// Work units of the shared-node pass for element-list mode m: every (channel, Ω, 16 x 16 tile)
+// holding an element of the pass, once per loop-frequency domain.  The two logarithmic domains
+// come first (they hold most nodes), and consecutive units differ in Ω and channel, so that the
+// XCDs, which take consecutive blocks in turn, receive even shares.
+std::vector<uint32_t> shared_units(const pffrg_ctx *c, int m) {
+    const int nO = c->sys.nO, nn = c->sys.nn;
+    const int NX2 = (m == 2) ? 1 : nn + 1, NX3 = nn + 1;
+    const int n2 = (NX2 + SH_T - 1) / SH_T, n3 = (NX3 + SH_T - 1) / SH_T;
+    const int nch = (m == 2) ? 1 : 3;
+    std::vector<uint32_t> u;
+    const int doms[4] = {0, 3, 1, 2};
+    for (int d : doms)
+        for (int t2 = 0; t2 < n2; ++t2)
+            for (int t3 = 0; t3 < n3; ++t3)
+                for (int w1 = 0; w1 < nO; ++w1)
+                    for (int ch = 0; ch < nch; ++ch) {
+                        bool any = false;
+                        for (int i2 = t2 * SH_T; i2 < std::min(NX2, t2 * SH_T + SH_T) && !any; ++i2)
+                            for (int i3 = t3 * SH_T; i3 < std::min(NX3, t3 * SH_T + SH_T) && !any; ++i3)
+                                any = shared_in_pass(m, i2, i3, nn);
+                        if (any) u.push_back(shared_unit(ch, w1, t2, t3, d));
+                    }
+    return u;
+}
+
+// Arguments of the shared-node pass of one context: the pass's tables, the work units of the
+// element-list mode (built on first use) and the partial-sum buffer (grown on demand).
+template <int PASS>
+int shared_args(pffrg_ctx *c, double L, int a, int d, int out, int eval, const double *tol, int mode, SharedArgs &A) {
+    int rc;
+    if ((rc = pass_args<PASS, float>(c, L, a, d, out, eval, tol, mode, A.P))) return rc;
+    if (!c->d_units[mode]) {
+        shared_shape(c);
+        std::vector<uint32_t> u = shared_units(c, mode);
+        HIPCHK(c, hipMalloc(&c->d_units[mode], std::max<size_t>(1, u.size()) * sizeof(uint32_t)));
+        if (!u.empty())
+            HIPCHK(c, hipMemcpy(c->d_units[mode], u.data(), u.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        c->n_units[mode] = (int)u.size();
+    }
+    const int nn = c->sys.nn;
+    A.units = c->d_units[mode];
+    A.n_units = c->n_units[mode];
+    A.mode = mode;
+    A.NX2 = (mode == 2) ? 1 : nn + 1;
+    A.NX3 = nn + 1;
+    A.nw = c->sh_nw;
+    A.spt = c->sh_spt;
+    A.count = c->d_count;
+    const size_t need = (size_t)SH_NPART * 3 * c->sys.nO * (size_t)A.NX2 * A.NX3 * 2 * c->sys.ns * sizeof(double);
+    if (need > c->part_bytes) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipFree(c->d_part));
+        c->d_part = nullptr;
+        c->part_bytes = 0;
+        HIPCHK(c, hipMalloc(&c->d_part, need));
+        c->part_bytes = need;
+    }
+    A.part = c->d_part;
+    return 0;
+}
+
+template <int PASS>
+size_t shared_lds_bytes(const pffrg_ctx *c, const SharedArgs &A) {
+    return (size_t)shared_lds(c->sys, c->n_ov, A.nw, PASS == P_KAT).total;
+}
+
+template <int PASS>
+int launch_shared_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const double *tol, int mode) {
+    SharedArgs A;
+    int rc;
+    if ((rc = shared_args<PASS>(c, L, a, d, out, eval, tol, mode, A))) return rc;
+    const size_t lds = shared_lds_bytes<PASS>(c, A);
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, launch_shared_pass<PASS>(A, lds, c->stream));
+    if (c->timing) {
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->time_ms += ms;
+        c->time_launches += 1;
+    }
+    c->stat_elems += A.P.n_elems;
+    c->stat_launches += 1;
+    invalidate(c, out);
+    return 0;
+}
+
 template <int PASS>
 int launch_pass(pffrg_ctx *c, double L, int a, int d, int out, int eval, const double *tol, int mode) {
     if (eval < 1) return fail(c, PFFRG_ERR_ARG, "eval must be >= 1");
-    const bool f32 = c->precision == PFFRG_FP32_MIXED;
+    const bool f32 = c->precision != PFFRG_FP64;
     int rc;
     if ((rc = ensure_gather(c, a, f32))) return rc;
     if ((PASS == P_LEFT || PASS == P_CENTRAL) && (rc = ensure_gather(c, d, f32))) return rc;
+    if (c->precision == PFFRG_FP32_MIXED && shared_fits(c)) return launch_shared_t<PASS>(c, L, a, d, out, eval, tol, mode);
     if (f32) return launch_pass_t<PASS, float>(c, L, a, d, out, eval, tol, mode);
     return launch_pass_t<PASS, double>(c, L, a, d, out, eval, tol, mode);
 }
@@ -772,10 +884,10 @@ int pffrg_create(const pffrg_dims *d, pffrg_ctx **out) {
     ok = ok && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipMalloc(&c->d_mesh, MESH_TABLE * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_corrs, 2 * 3 * y.nO * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&c->d_count, 2 * sizeof(unsigned long long)) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_count, N_COUNTERS * sizeof(unsigned long long)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_max, sizeof(unsigned long long)) == hipSuccess;
     ok = ok && hipMemset(c->d_corrs, 0, 2 * 3 * y.nO * sizeof(double)) == hipSuccess;
-    ok = ok && hipMemset(c->d_count, 0, 2 * sizeof(unsigned long long)) == hipSuccess;
+    ok = ok && hipMemset(c->d_count, 0, N_COUNTERS * sizeof(unsigned long long)) == hipSuccess;
     ok = ok && hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
     c->slots.resize(d->n_slots);
     for (auto &s : c->slots) {
@@ -827,6 +939,8 @@ void pffrg_destroy(pffrg_ctx *c) {
     (void)hipFree(c->d_el_half);
     (void)hipFree(c->d_el_full);
     (void)hipFree(c->d_el_red);
+    for (int m = 0; m < 3; ++m) (void)hipFree(c->d_units[m]);
+    (void)hipFree(c->d_part);
     (void)hipFree(c->d_logpts);
     (void)hipFree(c->d_dbg_nodes);
     (void)hipFree(c->d_dbg_res);
@@ -942,7 +1056,8 @@ int pffrg_download_action(pffrg_ctx *c, int32_t s, double *Sigma, double *bare, 
 
 int pffrg_set_precision(pffrg_ctx *c, int32_t mode) {
     if (!c) return PFFRG_ERR_ARG;
-    if (mode != PFFRG_FP64 && mode != PFFRG_FP32_MIXED) return fail(c, PFFRG_ERR_ARG, "unknown precision mode");
+    if (mode != PFFRG_FP64 && mode != PFFRG_FP32_MIXED && mode != PFFRG_FP32_ELEMENT)
+        return fail(c, PFFRG_ERR_ARG, "unknown precision mode");
     c->precision = mode;
     return 0;
 }
@@ -1161,7 +1276,7 @@ int pffrg_reset_stats(pffrg_ctx *c) {
     if (!c) return PFFRG_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemset(c->d_count, 0, 2 * sizeof(unsigned long long)));
+    HIPCHK(c, hipMemset(c->d_count, 0, N_COUNTERS * sizeof(unsigned long long)));
     c->stat_elems = 0;
     c->stat_launches = 0;
     c->time_ms = 0.0;
@@ -1179,6 +1294,16 @@ int pffrg_get_stats(pffrg_ctx *c, int64_t *stats) {
     stats[1] = c->stat_elems;
     stats[2] = c->stat_launches;
     stats[3] = (int64_t)n[1];
+    return 0;
+}
+
+int pffrg_get_counters(pffrg_ctx *c, int64_t *out, int32_t n) {
+    if (!c || !out || n < 1 || n > N_COUNTERS) return fail(c, PFFRG_ERR_ARG, "counters: 1..16 entries");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unsigned long long v[N_COUNTERS];
+    HIPCHK(c, hipMemcpy(v, c->d_count, sizeof(v), hipMemcpyDeviceToHost));
+    for (int k = 0; k < n; ++k) out[k] = (int64_t)v[k];
     return 0;
 }
 
@@ -1396,6 +1521,73 @@ int batch_rhs_1l_t(pffrg_batch *b, const double *Lam, const int32_t *a, const in
     for (int k = 0; k < n; ++k) {
         pffrg_ctx *c = b->ctx[k];
         const int q = b->order[k];
+        c->stat_elems += c->n_half;
+        c->stat_launches += 1;
+        invalidate(c, da[q]);
+        int rc;
+        if ((rc = do_symmetrize(c, da[q]))) return bfail(b, rc, c->err);
+    }
+    for (pffrg_ctx *c : b->ctx)
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "stream synchronize failed");
+    return 0;
+}
+
+// The FP32-mixed batch: dΣ and the tail factors per problem, then the shared-node Katanin pass of
+// every problem as one launch (the units of all problems, interleaved so that each part of the
+// grid mixes problems), then each problem's partial sums and symmetrisation on its own stream.
+int batch_rhs_1l_shared(pffrg_batch *b, const double *Lam, const int32_t *a, const int32_t *da, const int32_t *eval,
+                        const double *stol, const double *gtol) {
+    const int n = (int)b->ctx.size();
+    std::vector<SharedArgs> tab(n);
+    size_t lds = 0;
+    int nthreads = 64;
+    for (int k = 0; k < n; ++k) {
+        pffrg_ctx *c = b->ctx[k];
+        const int q = b->order[k];
+        int rc;
+        if ((rc = launch_sigma<SK_DSIGMA>(c, Lam[q], a[q], a[q], c->slots[da[q]].Sigma, nullptr, nullptr, stol)))
+            return bfail(b, rc, c->err);
+        if ((rc = launch_corrs(c, Lam[q], a[q], da[q], true, gtol))) return bfail(b, rc, c->err);
+        if ((rc = ensure_gather(c, a[q], true))) return bfail(b, rc, c->err);
+        if ((rc = shared_args<P_KAT>(c, Lam[q], a[q], da[q], da[q], eval[q], gtol, 0, tab[k]))) return bfail(b, rc, c->err);
+        lds = std::max(lds, shared_lds_bytes<P_KAT>(c, tab[k]));
+        nthreads = std::max(nthreads, shared_threads(tab[k]));
+    }
+    if (!b->d_sh_units) {
+        // round-robin over the problems' unit lists, each in its own order
+        std::vector<std::vector<uint32_t>> per(n);
+        size_t total = 0;
+        for (int k = 0; k < n; ++k) {
+            per[k] = shared_units(b->ctx[k], 0);
+            total += per[k].size();
+        }
+        std::vector<uint32_t> all;
+        all.reserve(total);
+        std::vector<size_t> pos(n, 0);
+        while (all.size() < total)
+            for (int k = 0; k < n; ++k)
+                if (pos[k] < per[k].size()) all.push_back(per[k][pos[k]++] | ((uint32_t)k << 26));
+        if (hipMalloc(&b->d_sh_units, std::max<size_t>(1, all.size()) * sizeof(uint32_t)) != hipSuccess ||
+            hipMalloc(&b->d_sh_tab, n * sizeof(SharedArgs)) != hipSuccess)
+            return bfail(b, PFFRG_ERR_HIP, "hipMalloc failed");
+        if (!all.empty() && hipMemcpy(b->d_sh_units, all.data(), all.size() * sizeof(uint32_t), hipMemcpyHostToDevice) !=
+                                hipSuccess)
+            return bfail(b, PFFRG_ERR_HIP, "unit list upload failed");
+        b->n_sh_units = (int)all.size();
+    }
+    for (pffrg_ctx *c : b->ctx)
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "stream synchronize failed");
+    if (hipMemcpy(b->d_sh_tab, tab.data(), n * sizeof(SharedArgs), hipMemcpyHostToDevice) != hipSuccess)
+        return bfail(b, PFFRG_ERR_HIP, "argument table upload failed");
+    hipError_t e = launch_shared_pass_batch<P_KAT>(reinterpret_cast<const SharedArgs *>(b->d_sh_tab), b->d_sh_units,
+                                                   b->n_sh_units, nthreads, lds, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return bfail(b, PFFRG_ERR_HIP, std::string("batched shared-node pass: ") + hipGetErrorString(e));
+    for (int k = 0; k < n; ++k) {
+        pffrg_ctx *c = b->ctx[k];
+        const int q = b->order[k];
+        if ((e = launch_shared_finish<P_KAT>(tab[k], c->stream)) != hipSuccess)
+            return bfail(b, PFFRG_ERR_HIP, std::string("partial sums: ") + hipGetErrorString(e));
         c->stat_elems += c->n_half;
         c->stat_launches += 1;
         invalidate(c, da[q]);
@@ -1666,6 +1858,8 @@ void pffrg_batch_destroy(pffrg_batch *b) {
     (void)hipSetDevice(b->device);
     if (b->d_codes) (void)hipFree(b->d_codes);
     if (b->d_tab) (void)hipFree(b->d_tab);
+    if (b->d_sh_units) (void)hipFree(b->d_sh_units);
+    if (b->d_sh_tab) (void)hipFree(b->d_sh_tab);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->chi_mem) (void)hipFree(b->chi_mem);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1688,8 +1882,12 @@ int pffrg_batch_rhs_1l(pffrg_batch *b, const double *Lambda, const int32_t *a, c
         if (!(Lambda[q] > 0.0) || eval[q] < 1) return bfail(b, PFFRG_ERR_ARG, "invalid Lambda or eval of problem " + std::to_string(q));
     }
     if (hipSetDevice(b->device) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "hipSetDevice failed");
-    return prec == PFFRG_FP32_MIXED ? batch_rhs_1l_t<float>(b, Lambda, a, da, eval, sigma_tol, gamma_tol)
-                                    : batch_rhs_1l_t<double>(b, Lambda, a, da, eval, sigma_tol, gamma_tol);
+    bool all_fit = true;
+    for (pffrg_ctx *c : b->ctx) all_fit = all_fit && shared_fits(c);
+    if (prec == PFFRG_FP32_MIXED && all_fit) return batch_rhs_1l_shared(b, Lambda, a, da, eval, sigma_tol, gamma_tol);
+    if (prec == PFFRG_FP32_MIXED) return batch_rhs_1l_t<float>(b, Lambda, a, da, eval, sigma_tol, gamma_tol);
+    return prec == PFFRG_FP32_ELEMENT ? batch_rhs_1l_t<float>(b, Lambda, a, da, eval, sigma_tol, gamma_tol)
+                                      : batch_rhs_1l_t<double>(b, Lambda, a, da, eval, sigma_tol, gamma_tol);
 }
 
 // compute_χ! of every problem of a batch in one set of cubature rounds

@@ -1,0 +1,822 @@
+// pffrg_shared.h -- the shared-node vertex pass of the FP32-mixed mode.
+//
+// All output elements of one (channel, Ω) block integrate over the same loop-frequency
+// sub-intervals (flow_s.jl:31-34) with the same Simpson nodes at every refinement level
+// (quadrature.jl:2-139): an element only decides how many levels it uses.  At a node, the
+// integrand of element (ν, ν') multiplies a vertex slot whose frequency arguments contain ν but
+// not ν' (the left vertex, e.g. flow_su2_s.jl:21-28) with one that contains ν' but not ν (the
+// right vertex).  This pass therefore integrates a tile of 16 x 16 elements together: per node
+// it interpolates the 32 site vectors once, where the per-element pass interpolates 512.  The
+// index ν = n_ν stands for the asymptotic argument (ν = ∞), so the q1, q2 and q3 elements of a
+// block share the tiles.
+//
+// At one node and site the 16 x 16 integrands of a tile are a sum of products of a ν-indexed and
+// a ν'-indexed factor: two terms (the spin and density components) in the s and u channels
+// (flow_su2_s.jl:41-42), two local terms plus one per overlap row in the t channel
+// (flow_su2_t.jl:56-74).  The pass forms them on the matrix cores: v_mfma_f32_16x16x4f32 with the
+// tile's ν index as rows, ν' as columns and (node, term) as the inner dimension, so that a batch
+// of nodes is summed inside the product.
+//
+// Every element keeps its own adaptive state: the Simpson sums of its last two levels, and the
+// Richardson convergence test of quadrature.jl:54-65 on its own sites.  It stops at its own
+// level, exactly when the reference would; a tile keeps refining while any of its elements is
+// unconverged, and only the rows and columns of unconverged elements are interpolated.  The
+// reweighted nodes of a level (the previous level's new nodes, quadrature.jl:16-21) are not
+// evaluated again: their raw sum is kept from the previous level.  Sums over levels and
+// sub-intervals, the Simpson sums and the convergence norms are FP64; the vertex copy, the
+// interpolation, the integrand products and the sum over one batch of nodes are FP32.  Results
+// differ from the per-element FP64 pass by FP32 rounding and summation order only.
+//
+// Lane l of wave w owns elements (ν row 4 (l/16) + r, ν' column l mod 16), r = 0..3, on the
+// sites w, w + nw, w + 2nw, ... (nw waves): the layout of the matrix cores' result.
+#pragma once
+
+#include "pffrg_pass.h"
+
+namespace pffrg {
+
+constexpr int SH_MAXT = 512;  // threads per block, at most (8 waves)
+constexpr int SH_K = 4;       // quadrature nodes evaluated per batch
+constexpr int SH_SPT = 3;     // sites per wave, at most (the pass takes up to 8 * SH_SPT sites)
+constexpr int SH_T = 16;      // tile edge (the matrix cores' 16 x 16 result)
+constexpr int SH_NPART = 5;   // partial sums per element: the four domains, then the two tail nodes
+constexpr int SH_NODES = 256; // nodes whose positions and bubbles are prepared together
+
+typedef float sh_v4f __attribute__((ext_vector_type(4)));
+
+// row stride (in float2) of the site vectors in LDS: odd, so that rows of neighbouring tile
+// positions start in different LDS banks
+__host__ __device__ inline int shared_vrow(int ns) { return ns | 1; }
+
+struct SharedArgs {
+    PassArgs<float> P;       // tables, FP32 gather copies, Λ, eval, tolerances, output slot
+    const uint32_t *units;   // work units (shared_unit codes)
+    int n_units;
+    int mode;                // 0 half pass (q3 with ν >= ν'), 1 full pass, 2 reduced s bubble
+    int NX2, NX3;            // extents of the extended ν and ν' indices (n_ν + 1; reduced pass: NX2 = 1)
+    int nw, spt;             // waves per block and sites per wave
+    double *part;            // [SH_NPART][3][nO][NX2][NX3][2][ns] partial sums
+    unsigned long long *count;  // optional: element-nodes, gathered bytes, phase cycles
+};
+
+// work unit: ch 2 bits | Ω index 7 | tile along ν 4 | tile along ν' 4 | domain 2
+__host__ __device__ inline uint32_t shared_unit(int ch, int w1, int t2, int t3, int d) {
+    return (uint32_t)ch | ((uint32_t)w1 << 2) | ((uint32_t)t2 << 9) | ((uint32_t)t3 << 13) | ((uint32_t)d << 17);
+}
+
+__host__ __device__ inline long shared_part_index(int d, int ch, int w1, int w2x, int w3x, int nO, int NX2, int NX3,
+                                                  int ns) {
+    return (((((long)d * 3 + ch) * nO + w1) * NX2 + w2x) * (long)NX3 + w3x) * 2L * ns;
+}
+
+// element (ν index w2x, ν' index w3x) of the extended grid belongs to the pass's element list
+// (compute.jl; SDE.jl:156-166): index n_ν is the asymptotic argument
+__host__ __device__ inline bool shared_in_pass(int mode, int w2x, int w3x, int nn) {
+    if (mode == 1 || mode == 2) return true;
+    return w2x == nn || w3x == nn || w2x >= w3x;
+}
+
+// dynamic LDS of k_shared_pass (byte offsets), sized to the problem.  The descriptors, site
+// vectors and local vectors of a batch share their space with the cross-wave norm sums of a
+// convergence test, which never overlaps a batch.
+struct SharedLds {
+    int mp, o_sig, o_mf, o_ov, o_ovoff, o_exch, o_hint, o_desc, o_V, o_loc, o_red, o_nodes, o_b1, total;
+};
+
+__host__ __device__ inline SharedLds shared_lds(const Sys &y, int n_ov, int nw, bool sig_d) {
+    SharedLds L;
+    int mp = y.nsig;
+    if (y.nO > mp) mp = y.nO;
+    if (y.nn > mp) mp = y.nn;
+    L.mp = (mp + 1) & ~1;
+    int o = 10 * L.mp * 8;
+    L.o_sig = o; o += (sig_d ? 2 : 1) * L.mp * 8;
+    L.o_mf = o; o += 12 * L.mp * 4;
+    L.o_ov = o; o += ((n_ov + 1) & ~1) * 4;
+    L.o_ovoff = o; o += ((y.ns + 2) & ~1) * 4;
+    L.o_exch = o; o += ((y.ns + 1) & ~1) * 4;
+    L.o_hint = o; o += ((HINT_WORDS + 1) & ~1) * 4;
+    o = (o + 15) & ~15;
+    const int base = o;
+    L.o_desc = o; o += SH_K * 4 * SH_T * 3 * (int)sizeof(DescT<float>);
+    o = (o + 15) & ~15;
+    L.o_V = o; o += SH_K * 2 * SH_T * shared_vrow(y.ns) * 8;
+    L.o_loc = o; o += SH_K * 2 * SH_T * 8;
+    L.o_red = base;
+    const int red_end = base + nw * SH_T * SH_T * 3 * 8;
+    if (red_end > o) o = red_end;
+    o = (o + 15) & ~15;
+    L.o_nodes = o; o += SH_NODES * (8 + 8 + 4 + 4);  // positions, weights, bubbles, reweight flags
+    L.o_b1 = o; o += nw * 64 * SH_SPT * 2 * 4 * 4;    // each lane's previous-level Simpson sums (FP32)
+    L.total = o;
+    return L;
+}
+
+// Vertex of one slot at one site from its three channel descriptors (get_Γ_avx!,
+// action_su2.jl:95-134): bare + Σ_channels sign·interpolation, one channel at a time to bound
+// registers; the corners of a channel are loaded together.  Returns the corners read.
+__device__ __forceinline__ int gather_site(const DescT<float> *ds, const float *gamma, const int *exch, int site,
+                                           const Sys &y, float &vs, float &vd) {
+    const float2 *q = reinterpret_cast<const float2 *>(gamma);
+    int cn = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const DescT<float> &d = ds[c];
+        const int meta = d.meta;
+        const int kern = meta & 7;
+        if (!kern) continue;
+        const int sx = (meta & 8) ? exch[site] : site;
+        const bool map = (meta & 16) != 0;
+        const int chn = (c == 1) ? 1 : ((c == 0) == map ? 2 : 0);
+        const uint32_t o = (uint32_t)(chn * y.chlen + d.off + sx);
+        uint32_t c1, c2, c3;
+        corner_steps(meta, y, c1, c2, c3);
+        const int n = corners(kern);
+        float2 x[8];
+        load_corners(q, o, c1, c2, c3, n, x);
+        float r0, r1;
+        corner_sum(d, n, x, r0, r1);
+        vs += r0;
+        vd += (meta & 32) ? -r1 : r1;
+        cn += n;
+    }
+    return cn;
+}
+
+// Phase timing of the shared-node pass (a measurement build with PFFRG_SHARED_PROFILE defined):
+// thread 0 of every block adds the shader-clock cycles of each phase to count[2 + phase].
+#ifdef PFFRG_SHARED_PROFILE
+#define SH_PROF_T0() const unsigned long long _t0 = (t == 0) ? clock64() : 0ull
+#define SH_PROF_ADD(k) do { if (t == 0 && A.count) s_prof[k] += clock64() - _t0; } while (0)
+#else
+#define SH_PROF_T0() do { } while (0)
+#define SH_PROF_ADD(k) do { } while (0)
+#endif
+enum { PR_ACTIVE, PR_PREP, PR_DESC, PR_GATHER, PR_CONTRACT, PR_CHECK, PR_BATCHES, PR_TOTAL, PR_N };
+
+// One work unit (code uc) of one problem.  Every thread of the block calls it.
+template <int PASS>
+__device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t uc) {
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    __shared__ uint32_t s_code[MAX_DESC];
+    __shared__ double s_a1[SH_T], s_a2[SH_T];
+    __shared__ int s_act2[SH_T], s_act3[SH_T], s_nact[2];
+    __shared__ int s_rowf[SH_T], s_colf[SH_T];
+    __shared__ unsigned long long s_cnt[4];
+    __shared__ unsigned long long s_prof[PR_N];
+
+    const PassArgs<float> &P = A.P;
+    const Sys &y = P.y;
+    const int t = threadIdx.x;
+    const int NT = blockDim.x;
+    const int lane = t & 63;
+    const int wv = t >> 6;  // this thread's wave
+    const int nw = A.nw;
+    const int n_ov = P.ov_off[y.ns];
+    const SharedLds Lay = shared_lds(y, n_ov, nw, PASS == P_KAT);
+    const int mp = Lay.mp;
+    double *s_mesh = reinterpret_cast<double *>(s_dyn);
+    double *s_sigA = reinterpret_cast<double *>(s_dyn + Lay.o_sig);
+    double *s_sigD = (PASS == P_KAT) ? s_sigA + mp : s_sigA;
+    uint32_t *s_ov = reinterpret_cast<uint32_t *>(s_dyn + Lay.o_ov);
+    int *s_ovoff = reinterpret_cast<int *>(s_dyn + Lay.o_ovoff);
+    int *s_exch = reinterpret_cast<int *>(s_dyn + Lay.o_exch);
+    int32_t *s_hint = reinterpret_cast<int32_t *>(s_dyn + Lay.o_hint);
+    float *mf = reinterpret_cast<float *>(s_dyn + Lay.o_mf);
+    DescT<float> *s_desc = reinterpret_cast<DescT<float> *>(s_dyn + Lay.o_desc);
+    float2 *s_V = reinterpret_cast<float2 *>(s_dyn + Lay.o_V);
+    float2 *s_loc = reinterpret_cast<float2 *>(s_dyn + Lay.o_loc);
+    double *s_red = reinterpret_cast<double *>(s_dyn + Lay.o_red);
+    double *s_xs = reinterpret_cast<double *>(s_dyn + Lay.o_nodes);  // node positions
+    double *s_wq = s_xs + SH_NODES;                                   // node weights
+    float *s_tp = reinterpret_cast<float *>(s_wq + SH_NODES);         // bubbles
+    int *s_rw = reinterpret_cast<int *>(s_tp + SH_NODES);             // node sums kept for the next level
+    const int VR = shared_vrow(y.ns);
+    // this lane's previous-level Simpson sums, output o at b1[o * NT] (consecutive lanes adjacent)
+    float *b1 = reinterpret_cast<float *>(s_dyn + Lay.o_b1) + t;
+
+    // ---- tables in LDS (as the per-element pass stages them)
+    for (int k = t; k < 10 * mp; k += NT) s_mesh[k] = P.mesh[(k / mp) * MAXN + (k % mp)];
+    for (int k = t; k < y.nsig; k += NT) {
+        s_sigA[k] = P.A.Sigma[k];
+        if (PASS == P_KAT) s_sigD[k] = P.D.Sigma[k];
+    }
+    for (int k = t; k < 10 * mp; k += NT) mf[k] = (float)P.mesh[(k / mp) * MAXN + (k % mp)];
+    for (int k = t; k < y.nsig; k += NT) {
+        mf[10 * mp + k] = (float)P.A.Sigma[k];
+        mf[11 * mp + k] = (PASS == P_KAT) ? (float)P.D.Sigma[k] : 0.0f;
+    }
+    for (int k = t; k < y.ns; k += NT) s_exch[k] = P.exch[k];
+    for (int k = t; k < n_ov; k += NT) s_ov[k] = P.ov_rows[k];
+    for (int k = t; k <= y.ns; k += NT) s_ovoff[k] = P.ov_off[k];
+    {
+        const int32_t *gh = reinterpret_cast<const int32_t *>(P.mesh + 10 * MAXN);
+        for (int k = t; k < HINT_WORDS; k += NT) s_hint[k] = gh[k];
+    }
+    if (t < 4) s_cnt[t] = 0;
+    if (t < PR_N) s_prof[t] = 0;
+#ifdef PFFRG_SHARED_PROFILE
+    const unsigned long long t_begin = clock64();
+#endif
+
+    Lds S;
+    S.sig = s_mesh;
+    S.Os = s_mesh + mp;
+    S.ns = s_mesh + 2 * mp;
+    S.Ot = s_mesh + 3 * mp;
+    S.nt = s_mesh + 4 * mp;
+    S.inv = s_mesh + 5 * mp;
+    S.mp = mp;
+    S.code = s_code;
+    S.sigA = s_sigA;
+    S.sigD = s_sigD;
+    S.exch = s_exch;
+    S.ov = s_ov;
+    S.ov_off = s_ovoff;
+    S.hint = reinterpret_cast<const int16_t *>(s_hint);
+    S.mf = MeshV<float>{mf, mf + mp, mf + 2 * mp, mf + 3 * mp, mf + 4 * mp, mf + 5 * mp, mf + 10 * mp, mf + 11 * mp,
+                        mp, nullptr};
+
+    // ---- the unit
+    const int ch = uc & 3, w1 = (uc >> 2) & 127, t2 = (uc >> 9) & 15, t3 = (uc >> 13) & 15, dom = (uc >> 17) & 3;
+    const int nn = y.nn;
+    if (t < MAX_DESC) s_code[t] = slot_code<PASS>(ch, t / 3, t % 3);
+    const double INF = __builtin_huge_val();
+    const double *Om = (PASS == P_REDUCED || ch != 1) ? S.Os : S.Ot;
+    const double *nu = (PASS == P_REDUCED || ch != 1) ? S.ns : S.nt;
+    __syncthreads();
+    const double W = wave_uniform(Om[w1]);
+    if (t < SH_T) {
+        const int w2x = t2 * SH_T + t, w3x = t3 * SH_T + t;
+        s_a1[t] = (PASS == P_REDUCED) ? 0.0 : (w2x >= nn ? INF : nu[w2x]);
+        s_a2[t] = (w3x >= nn) ? INF : nu[w3x];
+    }
+    const double ref = P.L + 0.5 * W;
+    const double val = (PASS != P_REDUCED && ch == 1) ? y.valT : y.valS;
+    const double *logpts = P.logpts + ((PASS != P_REDUCED && ch == 1 ? y.nO : 0) + w1) * (P.eval + 1);
+    const int nslot = (ch == 1) ? 4 : 2;
+
+    // ---- this lane's elements (4 rows, one column of the tile) and sites
+    const int e3 = lane & 15, r2 = 4 * (lane >> 4);
+    const int w3x = t3 * SH_T + e3;
+    bool valid[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int w2x = t2 * SH_T + r2 + r;
+        valid[r] = w2x < A.NX2 && w3x < A.NX3 && shared_in_pass(A.mode, w2x, w3x, nn);
+    }
+    auto part_at = [&](int d, int r) {
+        return A.part + shared_part_index(d, ch, w1, t2 * SH_T + r2 + r, w3x, y.nO, A.NX2, A.NX3, y.ns);
+    };
+    // sites of this wave: wv, wv + nw, ... (wave-uniform)
+    const int nsw = (y.ns - wv + nw - 1) / nw < A.spt ? (y.ns - wv + nw - 1) / nw : A.spt;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        if (valid[r]) {
+            double *pp = part_at(dom, r);
+            for (int s = 0; s < nsw; ++s) {
+                pp[wv + nw * s] = 0.0;
+                pp[y.ns + wv + nw * s] = 0.0;
+            }
+        }
+
+    // per-lane quadrature state: output o = (site slot s, component c, element r) at (s * 2 + c) * 4 + r
+    constexpr int NO = SH_SPT * 2 * 4;
+    double b2[NO];
+    float Pp[NO];
+#pragma unroll
+    for (int o = 0; o < NO; ++o) { b2[o] = 0.0; Pp[o] = 0.f; }
+    bool act[4] = {false, false, false, false};
+    unsigned long long my_nodes = 0;
+
+    // Positions, weights and bubbles of up to SH_NODES nodes, prepared together (one thread per
+    // node): node q of the chunk is gen(q); rw marks the nodes whose raw sum the next level
+    // reweights.  The bubble is the propagator product of the integrand (bubbles.jl:64-96; the
+    // symmetrised Katanin sum in the Katanin pass, flow_su2_s.jl:18).
+    auto prep_nodes = [&](int n, auto gen) {
+        SH_PROF_T0();
+        for (int q = t; q < n; q += NT) {
+            double x, w;
+            int rw;
+            gen(q, x, w, rw);
+            double x1, x2, G1, K1, G2, K2;
+            bubble_args<double>(ch, W, x, x1, x2);
+            bubble_leg<PASS == P_KAT, double>(P.L, x1, meshv<double>(S), y.nsig, G1, K1);
+            bubble_leg<PASS == P_KAT, double>(P.L, x2, meshv<double>(S), y.nsig, G2, K2);
+            const double twopi = 2.0 * PI_D;
+            s_xs[q] = x;
+            s_wq[q] = w;
+            s_rw[q] = rw;
+            s_tp[q] = (float)((PASS == P_KAT) ? K1 * G2 / twopi + K2 * G1 / twopi : -(G1 * G2 / twopi));
+        }
+        __syncthreads();
+        SH_PROF_ADD(PR_PREP);
+    };
+
+    // Evaluate the integrands of the tile at the prepared nodes q0 .. q0+nb-1, which share one
+    // weight w and reweight flag, and accumulate b2 += w Σ_q f_q (FP64) and, when flagged,
+    // Pp += Σ_q f_q.
+    auto eval_batch = [&](int q0, int nb) {
+#ifdef PFFRG_SHARED_PROFILE
+        unsigned long long _tb = (t == 0) ? clock64() : 0ull;
+        if (t == 0 && A.count) s_prof[PR_BATCHES] += 1;
+#define SH_PROF_LAP(k) do { if (t == 0 && A.count) { const unsigned long long _n = clock64(); s_prof[k] += _n - _tb; _tb = _n; } } while (0)
+#else
+#define SH_PROF_LAP(k) do { } while (0)
+#endif
+        const int nA2 = s_nact[0], nA3 = s_nact[1];
+        const int nAS = nA2 + nA3;
+        // descriptors (node, slot, row or column, channel)
+        {
+            const int per_k = 3 * (nslot / 2) * nAS;
+            const int ntask = nb * per_k;
+            for (int tau = t; tau < ntask; tau += NT) {
+                const int k = tau / per_k, r = tau - k * per_k;
+                const int c = r % 3, q = r / 3;
+                const int pr = q / nAS, u = q - pr * nAS;
+                const bool row = u < nA2;
+                const int s = 2 * pr + (row ? 0 : 1);
+                const int ee = row ? s_act2[u] : s_act3[u - nA2];
+                build_desc_u<float, double>(S, y, s_code[3 * s + c], c, W, row ? s_a1[ee] : 0.0, row ? 0.0 : s_a2[ee],
+                                            s_xs[q0 + k], s_desc[((k * 4 + s) * SH_T + ee) * 3 + c]);
+            }
+        }
+        __syncthreads();
+        SH_PROF_LAP(PR_DESC);
+        // site vectors of slots 0 (rows) and 1 (columns) on every site, and of the t channel's
+        // local slots 2 and 3 on site 0; rows and columns in separate loops, so that the vertex
+        // read is the same for the whole loop
+        {
+            unsigned long long cb = 0;
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int nA = s == 0 ? nA2 : nA3;
+                const int *act_l = s == 0 ? s_act2 : s_act3;
+                const ActV<float> &V = (vertex_cfg<PASS>(ch, s) & 8) ? P.D : P.A;
+                const int per_k = nA * y.ns;
+                const int ntask = nb * per_k;
+                for (int tau = t; tau < ntask; tau += NT) {
+                    const int k = tau / per_k, r = tau - k * per_k;
+                    const int u = r / y.ns, i = r - u * y.ns;
+                    const int ee = act_l[u];
+                    float vs = (float)V.bare[i], vd = (float)V.bare[y.ns + i];
+                    int cn;
+                    gather_slot<float>(s_desc + ((k * 4 + s) * SH_T + ee) * 3, V.gamma, s_exch, i, y, vs, vd, cn);
+                    cb += cn;
+                    s_V[((k * 2 + s) * SH_T + ee) * VR + i] = float2{vs, vd};
+                }
+            }
+            if (ch == 1) {
+#pragma unroll
+                for (int s = 2; s < 4; ++s) {
+                    const int nA = s == 2 ? nA2 : nA3;
+                    const int *act_l = s == 2 ? s_act2 : s_act3;
+                    const ActV<float> &V = (vertex_cfg<PASS>(ch, s) & 8) ? P.D : P.A;
+                    const int ntl = nb * nA;
+                    for (int tau = t; tau < ntl; tau += NT) {
+                        const int k = tau / nA, u = tau - k * nA;
+                        const int ee = act_l[u];
+                        float vs = (float)V.bare[0], vd = (float)V.bare[y.ns];
+                        cb += gather_site(s_desc + ((k * 4 + s) * SH_T + ee) * 3, V.gamma, s_exch, 0, y, vs, vd);
+                        s_loc[(k * 2 + (s - 2)) * SH_T + ee] = float2{vs, vd};
+                    }
+                }
+            }
+            if (A.count && cb) atomicAdd(&s_cnt[1], cb * 2ull * sizeof(float));
+        }
+        __syncthreads();
+        SH_PROF_LAP(PR_GATHER);
+        // the tile's integrands on this wave's sites, summed over the batch on the matrix cores:
+        // A rows = ν positions, B columns = ν' positions, inner index = (node, term)
+        {
+            const double wq = s_wq[q0];
+            const bool addp = s_rw[q0] != 0;
+            const int pos = lane & 15, kq = lane >> 4;
+            const float S2 = (float)y.S2;
+#pragma unroll
+            for (int s = 0; s < SH_SPT; ++s) {
+                if (s >= nsw) break;
+                const int i = wv + nw * s;
+                sh_v4f Ds = {0.f, 0.f, 0.f, 0.f}, Dd = {0.f, 0.f, 0.f, 0.f};
+                if (ch != 1) {
+                    // s and u (flow_su2_s.jl:41-42, flow_su2_u.jl:41-42): Γs = v1s (-tp (∓2 v2s + v2d))
+                    // + v1d (-tp v2s), Γd = v1s (-3 tp v2s) + v1d (-tp v2d); two nodes per product
+                    const float sgn2 = (ch == 0) ? -2.0f : 2.0f;
+                    for (int st = 0; st < (nb + 1) / 2; ++st) {
+                        const int kap = 4 * st + kq;
+                        const int n = kap >> 1, tau = kap & 1;
+                        float as = 0.f, bs = 0.f, bd = 0.f;
+                        if (n < nb) {
+                            const float2 v1 = s_V[((n * 2 + 0) * SH_T + pos) * VR + i];
+                            const float2 v2 = s_V[((n * 2 + 1) * SH_T + pos) * VR + i];
+                            const float tp = s_tp[q0 + n];
+                            as = tau ? v1.y : v1.x;
+                            bs = tau ? -tp * v2.x : -tp * (sgn2 * v2.x + v2.y);
+                            bd = tau ? -tp * v2.y : -tp * (3.0f * v2.x);
+                        }
+                        Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(as, bs, Ds, 0, 0, 0);
+                        Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(as, bd, Dd, 0, 0, 0);
+                    }
+                } else {
+                    // t (flow_su2_t.jl:56-74): two local terms with the site-0 vertices v3 (ν) and
+                    // v4 (ν'), then one term per overlap row (j1, j2, m) of site i
+                    const int r0 = s_ovoff[i], R = s_ovoff[i + 1] - r0, kk = 2 + R;
+                    const int nst = (kk + 3) >> 2;
+                    for (int n = 0; n < nb; ++n) {
+                        const float tp = s_tp[q0 + n];
+                        const float cc = -tp * -2.0f * S2;
+                        const float2 *V0 = s_V + (n * 2 + 0) * SH_T * VR + pos * VR;
+                        const float2 *V1 = s_V + (n * 2 + 1) * SH_T * VR + pos * VR;
+                        const float2 v1i = V0[i], v2i = V1[i];
+                        const float2 v3 = s_loc[(n * 2 + 0) * SH_T + pos], v4 = s_loc[(n * 2 + 1) * SH_T + pos];
+                        // operands of the inner index tau: 0 and 1 the local terms, 2 .. kk-1 the rows,
+                        // beyond kk zero; every lane reads a valid row and selects (no branches, so the
+                        // loads of consecutive steps overlap)
+#pragma unroll 2
+                        for (int st = 0; st < nst; ++st) {
+                            const int tau = 4 * st + kq;
+                            int rr = tau - 2;
+                            rr = rr < 0 ? 0 : (rr >= R ? (R > 0 ? R - 1 : 0) : rr);
+                            const uint32_t row = (R > 0) ? s_ov[r0 + rr] : 0u;  // a site without rows reads none
+                            const float2 e1 = V0[(int)(row & 255)];
+                            const float2 f2 = V1[(int)((row >> 8) & 255)];
+                            const float c = cc * (float)(row >> 16);
+                            const bool isrow = tau >= 2 && tau < kk;
+                            float as = isrow ? e1.x : 0.f, ad = isrow ? e1.y : 0.f;
+                            float bs = isrow ? c * f2.x : 0.f, bd = isrow ? c * f2.y : 0.f;
+                            if (tau == 0) {
+                                as = v1i.x;
+                                ad = v1i.y;
+                                bs = -tp * (-1.0f * v4.x + v4.y);
+                                bd = -tp * (3.0f * v4.x + v4.y);
+                            }
+                            if (tau == 1) {
+                                as = -1.0f * v3.x + v3.y;
+                                ad = 3.0f * v3.x + v3.y;
+                                bs = -tp * v2i.x;
+                                bd = -tp * v2i.y;
+                            }
+                            Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(as, bs, Ds, 0, 0, 0);
+                            Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(ad, bd, Dd, 0, 0, 0);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    b2[(s * 2 + 0) * 4 + r] += wq * (double)Ds[r];
+                    b2[(s * 2 + 1) * 4 + r] += wq * (double)Dd[r];
+                    if (addp) {
+                        Pp[(s * 2 + 0) * 4 + r] += Ds[r];
+                        Pp[(s * 2 + 1) * 4 + r] += Dd[r];
+                    }
+                }
+            }
+            if (wv == 0) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (act[r]) my_nodes += nb;
+            }
+        }
+        __syncthreads();  // the descriptors and site vectors are rewritten by the next batch
+        SH_PROF_LAP(PR_CONTRACT);
+#undef SH_PROF_LAP
+    };
+
+    // active rows and columns of the tile: those holding at least one unconverged element,
+    // compacted by the first wave with a ballot
+    auto set_active = [&]() {
+        SH_PROF_T0();
+        if (t < SH_T) { s_rowf[t] = 0; s_colf[t] = 0; }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (act[r]) {
+                s_rowf[r2 + r] = 1;
+                s_colf[e3] = 1;
+            }
+        __syncthreads();
+        if (t < 64) {
+            const int q = t & 31;
+            const bool isrow = t < 32;
+            const bool on = q < SH_T && (isrow ? s_rowf[q] : s_colf[q]) != 0;
+            const unsigned long long bal = __ballot(on);
+            const unsigned long long mine = isrow ? (bal & 0xffffffffull) : (bal >> 32);
+            const int p = __popcll(mine & ((1ull << q) - 1ull));
+            if (on) (isrow ? s_act2 : s_act3)[p] = q;
+            if (t == 0) s_nact[0] = __popcll(bal & 0xffffffffull);
+            if (t == 32) s_nact[1] = __popcll(bal >> 32);
+        }
+        __syncthreads();
+        SH_PROF_ADD(PR_ACTIVE);
+    };
+
+    // Convergence test of every active element over all its sites (quadrature.jl:54-65): each
+    // wave's partial sums of squares go through LDS and are added in wave order, the same on
+    // every wave; a converged element adds its Richardson result to its partial sum.
+    auto check_level = [&](long nn_) {
+        SH_PROF_T0();
+        double q[4][3];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) q[r][0] = q[r][1] = q[r][2] = 0.0;
+#pragma unroll
+        for (int o = 0; o < NO; ++o) {
+            const int s = o >> 3, r = o & 3;
+            if (s < nsw) {
+                double R = (double)b1[o * NT] * (-1.0 / 15.0);
+                R += 16.0 / 15.0 * b2[o];
+                q[r][0] += R * R;
+                q[r][1] += b2[o] * b2[o];
+                const double dd = R - b2[o];
+                q[r][2] += dd * dd;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int e = (r2 + r) * SH_T + e3;
+            double *d = s_red + ((long)wv * SH_T * SH_T + e) * 3;
+            d[0] = q[r][0];
+            d[1] = q[r][1];
+            d[2] = q[r][2];
+        }
+        __syncthreads();
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int e = (r2 + r) * SH_T + e3;
+            double q1 = 0.0, q2 = 0.0, q3 = 0.0;
+            for (int w = 0; w < nw; ++w) {
+                const double *d = s_red + ((long)w * SH_T * SH_T + e) * 3;
+                q1 += d[0];
+                q2 += d[1];
+                q3 += d[2];
+            }
+            const double n1 = sqrt(q1), n2 = sqrt(q2), adiff = sqrt(q3);
+            const double rdiff = adiff / fmax(n1, n2);
+            const bool conv = act[r] && (adiff < P.atol || rdiff < P.rtol || nn_ > (1L << 20));
+            if (conv) {
+                // Richardson-extrapolated result (quadrature.jl:72-88), added in sub-interval order
+                double *pp = part_at(dom, r);
+#pragma unroll
+                for (int s = 0; s < SH_SPT; ++s) {
+                    if (s >= nsw) break;
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        const int o = (s * 2 + c) * 4 + r;
+                        double R = (double)b1[o * NT] * (-1.0 / 15.0);
+                        R += 16.0 / 15.0 * b2[o];
+                        pp[c * y.ns + wv + nw * s] += (R - b2[o]) + b2[o];
+                        // one read-modify-write at a time (once per element and sub-interval): keeps
+                        // the compiler from holding every load of the loop in registers at once
+                        asm volatile("" ::: "memory");
+                    }
+                }
+                act[r] = false;
+            }
+            any = any || act[r];
+        }
+        // the next level: S1 <- S2, S2 <- S2 / 2 (quadrature.jl:74-76)
+#pragma unroll
+        for (int o = 0; o < NO; ++o) {
+            b1[o * NT] = (float)b2[o];
+            b2[o] = 0.5 * b2[o];
+        }
+        const bool more = __syncthreads_or(any ? 1 : 0);
+        SH_PROF_ADD(PR_CHECK);
+        return more;
+    };
+
+    // Quadrature of the unit's sub-intervals in order, then (last domain) the two tail nodes.
+    // One preparation and one evaluation call site serve every step: the three nodes of the
+    // initial Simpson rule (a and b, then the midpoint, whose weights differ), the new nodes of
+    // each level (SH_NODES prepared at a time, SH_K evaluated at a time), and the tail nodes
+    // (one at a time: their weights differ).
+    enum { ST_SUB, ST_INIT, ST_LEVEL, ST_TAIL };
+    int st = ST_SUB, kk = -1, n = 0, q0 = 0;
+    long nlev = 0, base = 0;
+    double a = 0.0, b = 0.0, sg = 1.0, h = 0.0;
+    bool need_prep = false;
+    auto start_level = [&]() {
+        // residual! (quadrature.jl:2-24): nlev new nodes a + (2i - 1) h with weight 4h/3; the
+        // previous level's new nodes get -2h/3, applied to their kept sum
+        h = 0.5 * (b - a) / nlev;
+#pragma unroll
+        for (int o = 0; o < NO; ++o) {
+            b2[o] += -2.0 * h / 3.0 * (double)Pp[o];
+            Pp[o] = 0.f;
+        }
+        base = 0;
+        n = (int)(nlev < SH_NODES ? nlev : SH_NODES);
+        need_prep = true;
+    };
+    while (true) {
+        if (st == ST_SUB) {
+            ++kk;
+            if (kk > P.eval || (kk == P.eval && dom != 3)) break;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) act[r] = valid[r];
+#pragma unroll
+            for (int o = 0; o < NO; ++o) { b2[o] = 0.0; Pp[o] = 0.f; }
+            if (kk == P.eval) {
+                // boundary corrections (flow_s.jl:37-38): -val and +val times the tail factors
+                st = ST_TAIL;
+                n = 2;
+            } else {
+                if (!subinterval(dom * P.eval + kk, P.eval, ref, logpts, a, b, sg)) continue;
+                a = wave_uniform(a);
+                b = wave_uniform(b);
+                sg = wave_uniform(sg);
+                // simps! (quadrature.jl:57-62): a and b with weight (b-a)/6, the midpoint with 2(b-a)/3
+                st = ST_INIT;
+                n = 3;
+            }
+            set_active();
+            need_prep = true;
+        }
+        if (need_prep) {
+            prep_nodes(n, [&](int q, double &x, double &w, int &rw) {
+                if (st == ST_INIT) {
+                    x = sg * (q == 0 ? a : (q == 1 ? b : 0.5 * (a + b)));
+                    w = (q == 2) ? 2.0 * (b - a) / 3.0 : 1.0 * (b - a) / 6.0;
+                    rw = (q == 2);  // the midpoint is the node the first level reweights
+                } else if (st == ST_TAIL) {
+                    x = (q == 0) ? -val : val;
+                    w = P.corrs[q + 2 * (ch + 3 * w1)];
+                    rw = 0;
+                } else {
+                    x = sg * node_at(a, 2.0 * (double)(base + q + 1) - 1.0, h);
+                    w = 4.0 * h / 3.0;
+                    rw = 1;
+                }
+            });
+            need_prep = false;
+            q0 = 0;
+        }
+        const int nb = (st == ST_INIT) ? (q0 == 0 ? 2 : 1) : (st == ST_TAIL ? 1 : ((n - q0) < SH_K ? n - q0 : SH_K));
+        eval_batch(q0, nb);
+        q0 += nb;
+        if (q0 < n) continue;
+        if (st == ST_INIT) {
+            // S1, then S2 starts from S1 / 2 (quadrature.jl:62-66)
+#pragma unroll
+            for (int o = 0; o < NO; ++o) {
+                b1[o * NT] = (float)b2[o];
+                b2[o] = 0.5 * b2[o];
+            }
+            nlev = 2;
+            st = ST_LEVEL;
+            start_level();
+            continue;
+        }
+        if (st == ST_TAIL) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (valid[r]) {
+                    double *pt = part_at(4, r);
+#pragma unroll
+                    for (int s = 0; s < SH_SPT; ++s) {
+                        if (s >= nsw) break;
+                        pt[wv + nw * s] = b2[(s * 2 + 0) * 4 + r];
+                        pt[y.ns + wv + nw * s] = b2[(s * 2 + 1) * 4 + r];
+                    }
+                }
+            break;
+        }
+        // ST_LEVEL: the next chunk of the level's nodes, or the level's convergence test
+        base += n;
+        if (base < nlev) {
+            n = (int)((nlev - base) < SH_NODES ? nlev - base : SH_NODES);
+            need_prep = true;
+            continue;
+        }
+        if (!check_level(2 * nlev)) {
+            st = ST_SUB;
+            continue;
+        }
+        set_active();
+        nlev *= 2;
+        start_level();
+    }
+    if (A.count) {
+        if (my_nodes) atomicAdd(&s_cnt[0], my_nodes);
+        __syncthreads();
+        if (t == 0) {
+            atomicAdd(A.count, s_cnt[0]);
+            atomicAdd(A.count + 1, s_cnt[1]);
+#ifdef PFFRG_SHARED_PROFILE
+            s_prof[PR_TOTAL] = clock64() - t_begin;
+            for (int k = 0; k < PR_N; ++k) atomicAdd(A.count + 2 + k, s_prof[k]);
+#endif
+        }
+    }
+}
+
+template <int PASS>
+__global__ __launch_bounds__(SH_MAXT) void k_shared_pass(SharedArgs A) {
+    shared_pass_body<PASS>(A, __builtin_amdgcn_readfirstlane(A.units[blockIdx.x]));
+}
+
+// The same pass of several problems in one launch: unit code bits 26-31 select the problem's
+// arguments in tab; the block has the threads of the problem that needs most, the others idle.
+template <int PASS>
+__global__ __launch_bounds__(SH_MAXT) void k_shared_pass_batch(const SharedArgs *__restrict__ tab,
+                                                                const uint32_t *__restrict__ units, int n) {
+    if ((int)blockIdx.x >= n) return;
+    const uint32_t uc = __builtin_amdgcn_readfirstlane(units[blockIdx.x]);
+    shared_pass_body<PASS>(tab[uc >> 26], uc & 0x03ffffffu);
+}
+
+// Sum of an element's partial results in the reference's order (the four domains, then the two
+// tail nodes) into the output vertex at the element's place (flow_s.jl:41-51).
+template <int PASS>
+__global__ __launch_bounds__(256) void k_shared_finish(SharedArgs A) {
+    const Sys &y = A.P.y;
+    const long total = (long)A.P.n_elems * 2 * y.ns;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(q % y.ns);
+        const int comp = (int)((q / y.ns) & 1);
+        const long el = q / (2L * y.ns);
+        const uint32_t code = A.P.elems[el];
+        int ch, kernel, w1, w2, w3;
+        decode(code, ch, kernel, w1, w2, w3);
+        const int nn = y.nn;
+        int w2x, w3x;
+        if (A.mode == 2) {
+            w2x = 0;
+            w3x = (kernel == 1) ? nn : w3;
+        } else {
+            w2x = (kernel == 2 || kernel == 4) ? w2 : nn;
+            w3x = (kernel == 3 || kernel == 4) ? w3 : nn;
+        }
+        double acc = 0.0;
+        for (int d = 0; d < SH_NPART; ++d)
+            acc += A.part[shared_part_index(d, ch, w1, w2x, w3x, y.nO, A.NX2, A.NX3, y.ns) + (long)comp * y.ns + i];
+        long base;
+        if (kernel == 1) base = (long)y.ns * w1;
+        else if (kernel == 2) base = y.q21 + (long)y.ns * (w1 + (long)y.nO * w2);
+        else if (kernel == 3) base = y.q22 + (long)y.ns * (w1 + (long)y.nO * w3);
+        else base = y.q3 + (long)y.ns * (w1 + (long)y.nO * (w2 + (long)y.nn * w3));
+        A.P.out[(long)comp * y.complen + (long)ch * y.chlen + base + i] = acc;
+    }
+}
+
+// threads of a block: nw waves, each holding the whole tile on its share of the sites
+__host__ __device__ inline int shared_threads(const SharedArgs &A) { return 64 * A.nw; }
+
+template <int PASS>
+hipError_t launch_shared_pass(const SharedArgs &A, size_t lds, hipStream_t stream) {
+    if (A.n_units > 0) {
+        if (lds > 64 * 1024) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_shared_pass<PASS>),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL((k_shared_pass<PASS>), dim3(A.n_units), dim3(shared_threads(A)), lds, stream, A);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    long total = (long)A.P.n_elems * 2 * A.P.y.ns;
+    long blocks = (total + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL((k_shared_finish<PASS>), dim3((unsigned)blocks), dim3(256), 0, stream, A);
+    return hipGetLastError();
+}
+
+// the units of all problems of a batch (problem index in bits 26-31), nthreads per block; the
+// partial sums of each problem are then summed by launch_shared_finish on the same stream
+template <int PASS>
+hipError_t launch_shared_pass_batch(const SharedArgs *tab, const uint32_t *units, int n, int nthreads, size_t lds,
+                                    hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_shared_pass_batch<PASS>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_shared_pass_batch<PASS>), dim3(n), dim3(nthreads), lds, stream, tab, units, n);
+    return hipGetLastError();
+}
+
+template <int PASS>
+hipError_t launch_shared_finish(const SharedArgs &A, hipStream_t stream) {
+    long total = (long)A.P.n_elems * 2 * A.P.y.ns;
+    long blocks = (total + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL((k_shared_finish<PASS>), dim3((unsigned)blocks), dim3(256), 0, stream, A);
+    return hipGetLastError();
+}
+
+#define PFFRG_SHARED_INSTANCES(KW, PASS)                                                                        \
+    KW template hipError_t launch_shared_pass<PASS>(const SharedArgs &, size_t, hipStream_t);                  \
+    KW template hipError_t launch_shared_pass_batch<PASS>(const SharedArgs *, const uint32_t *, int, int, size_t, \
+                                                          hipStream_t);                                        \
+    KW template hipError_t launch_shared_finish<PASS>(const SharedArgs &, hipStream_t);
+
+PFFRG_SHARED_INSTANCES(extern, P_KAT)
+PFFRG_SHARED_INSTANCES(extern, P_LEFT)
+PFFRG_SHARED_INSTANCES(extern, P_CENTRAL)
+PFFRG_SHARED_INSTANCES(extern, P_BSE)
+PFFRG_SHARED_INSTANCES(extern, P_REDUCED)
+
+}  // namespace pffrg

@@ -17,6 +17,7 @@ LIB_PATH = Path(os.environ.get("PFFRG_LIB", Path(__file__).resolve().parent / "_
 
 FP64 = 0
 FP32_MIXED = 1
+FP32_ELEMENT = 2
 
 _lib = None
 
@@ -73,6 +74,7 @@ _SIGS = {
     "pffrg_compute_chi": ([_P, _d, _i, _D, _i, _D, _D, C.POINTER(C.c_int32)], C.c_int),
     "pffrg_scan_cuts": ([_P, _i, C.POINTER(C.c_int32), _D], C.c_int),
     "pffrg_scan_cuts_noalloc": ([_P, _i, C.POINTER(C.c_int32), _D], C.c_int),
+    "pffrg_get_counters": ([_P, _I64, _i], C.c_int),
     "pffrg_scan": ([_D, _D, _i, _D, _D], C.c_int),
     "pffrg_resample": ([_P, _i, _i, _D, _D, _D, _D, _D], C.c_int),
     "pffrg_mlp_create": ([_i, _i, _i, _i, C.c_uint64, _i, C.POINTER(_P)], C.c_int),
@@ -314,6 +316,11 @@ class Context:
         s = np.zeros(4, dtype=np.int64)
         self._ck(self.lib.pffrg_get_stats(self.h, _ip(s)), "get_stats")
         return {"nodes": int(s[0]), "elements": int(s[1]), "launches": int(s[2]), "gathered_bytes": int(s[3])}
+
+    def counters(self, n: int = 16) -> np.ndarray:
+        s = np.zeros(n, dtype=np.int64)
+        self._ck(self.lib.pffrg_get_counters(self.h, _ip(s), n), "get_counters")
+        return s
 
     def probe_integrand(self, pass_, L, a, da, ch, kernel, w1, w2, w3, v) -> np.ndarray:
         v = np.ascontiguousarray(v, dtype=np.float64)
