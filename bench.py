@@ -215,6 +215,7 @@ def main():
                     help="flow: square L=3 1-loop N=24 50->0.3, or pyrochlore L=4 3-loop with self-energy "
                          "corrections N=32 50->0.1")
     ap.add_argument("--flow-modes", default="fp64,fp32", help="flow: precisions to run, in order")
+    ap.add_argument("--flow-N", type=int, default=0, help="flow: mesh size instead of the configuration's")
     ap.add_argument("--flow-max-seconds", type=float, default=None, help="flow: wall-time budget per flow")
     ap.add_argument("--flow-verbose", action="store_true", help="flow: print every BS3 step")
     ap.add_argument("--batch-N", type=int, default=32, help="batch / batchflow: mesh size of every problem")
@@ -340,7 +341,10 @@ def flow_workload(args):
     the square L=3 1-loop or the pyrochlore L=4 3-loop configuration, in FP64 and in FP32-mixed
     on the GPU; the FP64 oracle's cost is estimated from a sample of its right-hand sides and χ
     at the flow's start and end."""
-    cfg = FLOW_CONFIGS[args.flow_config]
+    cfg = dict(FLOW_CONFIGS[args.flow_config])
+    if args.flow_N:
+        cfg["N"] = args.flow_N
+        cfg["workload"] = cfg["workload"].replace(f"N={FLOW_CONFIGS[args.flow_config]['N']}", f"N={args.flow_N}")
     N = cfg["N"]
     prob = F.Problem.build(cfg["name"], cfg["size"], [[1.0]], initial=cfg["Li"], num_sigma=N, num_Omega=N, num_nu=N,
                            num_chi=10)
@@ -477,10 +481,12 @@ def warmstart_workload(args):
     rep = ws.fit(train, acts, steps=args.ws_steps, lr=1e-3, verbose=True)
     t_fit = time.perf_counter() - t1
     log(f"  fit: coefficients {rep.n_coef}, loss {rep.losses[0]:.2e} -> {rep.losses[-1]:.2e}, "
-        f"train rel err {rep.train_rel_err:.2e}, {t_fit:.1f} s")
+        f"train rel err per block (s, t, u, Sigma) {['%.1e' % e for e in rep.train_block_err]}, {t_fit:.1f} s")
     rows = []
+    held_acts = []
     for j2, lam in test:
         rb, ab, _ = solve(j2, lam)
+        held_acts.append(ab)
         rw, aw, _ = solve(j2, lam, lambda prob, bare: ws.initial_action((j2, lam), bare))
         d = float(np.max(np.abs(aw.gamma - ab.gamma)) / np.max(np.abs(ab.gamma)))
         rows.append({"J2": float(j2), "Lambda": float(lam), "status": [rb.status, rw.status],
@@ -505,7 +511,9 @@ def warmstart_workload(args):
         "converged_both": len(conv), "stalled": len(rows) - len(conv),
         "held_out": rows, "train_iterations_mean": float(np.mean(it_train)),
         "times_s": {"data": t_data, "fit": t_fit, "total": time.perf_counter() - t0},
-        "fit": {"loss_first": rep.losses[0], "loss_last": rep.losses[-1], "train_rel_err": rep.train_rel_err},
+        "fit": {"loss_first": rep.losses[0], "loss_last": rep.losses[-1], "train_rel_err": rep.train_rel_err,
+                "train_block_err": dict(zip(("s", "t", "u", "Sigma"), rep.train_block_err)),
+                "held_out_block_err": dict(zip(("s", "t", "u", "Sigma"), ws.held_out_errors(test, held_acts)))},
     }
     print(json.dumps(line), flush=True)
 

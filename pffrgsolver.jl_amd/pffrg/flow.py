@@ -179,7 +179,7 @@ class BS3Flow:
                  bmin=1e-4, bmax=0.1, min_eval=10, max_eval=100, Stol=(1e-8, 1e-4), Gtol=(1e-8, 1e-4),
                  ODE_tol=(1e-8, 1e-2), max_steps: int | None = None, verbose=False, chi_tol=(1e-8, 1e-4),
                  rescan: bool = True, mesh_params: MeshParams = MeshParams(), max_seconds: float | None = None,
-                 measurement=None):
+                 measurement=None, cps=()):
         self.ctx, self.prob, self.Lf, self.loops, self.sigma_corr = ctx, prob, Lf, loops, sigma_corr
         # measurement(Λ, dΛ, mesh, χ, Σ, get_action) after every accepted step with χ, e.g. an
         # h5io.Measurement writing the observable and checkpoint files (measure, Launcher.jl:2-75)
@@ -201,7 +201,8 @@ class BS3Flow:
             ctx.upload(s, zero)
         ctx.upload(A_INTER, prob.bare_action())
         self.L, self.dL = Li, bmax * Li
-        self.cps = sorted({Li, Lf}, reverse=True)
+        # required checkpoints: the user's cutoffs with Λi and Λf (launcher_1l.jl:59-69)
+        self.cps = sorted({float(c) for c in cps} | {Li, Lf}, reverse=True)
         for c in self.cps:
             if c < self.L:
                 self.dL = min(self.dL, 0.85 * (self.L - c))
@@ -288,6 +289,7 @@ class BS3Flow:
         if err >= 10.0:
             res.status = "unstable"
             self.done = True
+            self._finish_files()
             return False
         accept = err <= 1.0 or self.dL <= self.bmin
         if accept:
@@ -313,16 +315,21 @@ class BS3Flow:
         if self.done:
             return
         if self._accepted:
-            # divergence, measurement, monotonicity, then new meshes (launcher_1l.jl:140-178)
+            # divergence, measurement, monotonicity, then new meshes (launcher_1l.jl:140-178); a
+            # checkpoint is forced at a required cutoff (Λ ≈ one of cps) and at either stop
             stop = "diverged" if self._gmax >= 1000.0 else None
-            if chi is not None and self.measurement is not None:
-                self.measurement(self.L, self.dL, ctx.m, chi, ctx.get_sigma(A_INTER), lambda: ctx.download(A_INTER))
+            meas = (lambda force: self.measurement(self.L, self.dL, ctx.m, chi, ctx.get_sigma(A_INTER),
+                                                   lambda: ctx.download(A_INTER), force=force)) \
+                if (chi is not None and self.measurement is not None) else (lambda force: None)
+            meas(stop is not None or any(_julia_isapprox(self.L, c) for c in self.cps))
             if chi is not None and self.record_chi(chi) and stop is None:
                 stop = "non_monotone"
+                meas(True)
             if stop is not None:
                 ctx.replace_with(A, A_INTER)
                 res.status = stop
                 self.done = True
+                self._finish_files()
                 return
             if self.rescan:
                 resample_from_to(ctx, self.L, A_INTER, A, res.chi[-1], self.mesh_params)
@@ -332,6 +339,14 @@ class BS3Flow:
         if not (self.L > self.Lf):
             res.status = "finished"
             self.done = True
+            self._finish_files()
+
+    def _finish_files(self):
+        """The end of the flow loop (Λ_f reached, instability, divergence or the monotonicity
+        stop): the finished flags of the observable and checkpoint files (launcher_1l.jl:201-209).
+        A flow cut short by max_steps or max_seconds is not finished."""
+        if self.measurement is not None and hasattr(self.measurement, "finish"):
+            self.measurement.finish()
 
     def measure_chi(self):
         """compute_χ! of the accepted action (slot A_INTER) on the current χ mesh (measure,
@@ -356,18 +371,23 @@ class BS3Flow:
         return self.res
 
 
+def _julia_isapprox(x: float, y: float) -> bool:
+    """Julia's x ≈ y for Float64: |x - y| <= sqrt(eps) max(|x|, |y|)."""
+    return x == y or abs(x - y) <= 1.4901161193847656e-08 * max(abs(x), abs(y))
+
+
 def launch_flow(ctx: Context, prob: Problem, a0: Action, Li: float, Lf: float, loops: int = 1, sigma_corr=True,
                 bmin=1e-4, bmax=0.1, min_eval=10, max_eval=100, Stol=(1e-8, 1e-4), Gtol=(1e-8, 1e-4),
                 ODE_tol=(1e-8, 1e-2), max_steps: int | None = None, verbose=False, chi_tol=(1e-8, 1e-4),
                 rescan: bool = True, mesh_params: MeshParams = MeshParams(),
-                max_seconds: float | None = None, measurement=None) -> FlowResult:
+                max_seconds: float | None = None, measurement=None, cps=()) -> FlowResult:
     """BS3 flow from Li to Lf (launcher_1l.jl:72-198 and its 2l/ml twins).  Slot A holds the
     flowing action on return, on the context's current meshes.  After every accepted step χ is
     measured (chi_tol; None skips it), the flow stops when the static on-site correlation turns
     (launcher_1l.jl:160-172), and the action moves onto rescanned meshes (rescan).  A
     measurement (h5io.Measurement) writes the observable and checkpoint files as launch! does."""
     f = BS3Flow(ctx, prob, a0, Li, Lf, loops, sigma_corr, bmin, bmax, min_eval, max_eval, Stol, Gtol, ODE_tol,
-                max_steps, verbose, chi_tol, rescan, mesh_params, max_seconds, measurement)
+                max_steps, verbose, chi_tol, rescan, mesh_params, max_seconds, measurement, cps)
     while f.begin_step():
         for k in range(4):
             f.before(k)

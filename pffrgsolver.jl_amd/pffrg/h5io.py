@@ -314,15 +314,26 @@ class Measurement:
             checkpoint(cp, L, dL, m, get_action())
             self.checkpoints.append(L)
 
-    def __call__(self, L: float, dL: float, m: Mesh, chi: np.ndarray, Sigma: np.ndarray, get_action):
+    def __call__(self, L: float, dL: float, m: Mesh, chi: np.ndarray, Sigma: np.ndarray, get_action,
+                 force: bool = False):
+        """force: the flow's forced checkpoint, measure() called with ct = 0 (at a required
+        cutoff, on divergence, on the monotonicity stop; launcher_1l.jl:143-172)."""
+        ct = 0.0 if force else self.ct
         with H5File(self.obs_file, "cw") as obs, H5File(self.cp_file, "cw") as cp:
             if not obs.haskey(f"χ/{julia_float(L)}"):
                 save_self(obs, L, m, Sigma)
                 save_chi(obs, L, "su2", m, chi)
             h0 = (time.time() - self.t0) / 3600.0
             if self.wt - h0 > 0.5:
-                if (time.time() - self.t) / 3600.0 >= self.ct:
+                if (time.time() - self.t) / 3600.0 >= ct:
                     self._checkpoint(cp, L, dL, m, get_action)
                     self.t = time.time()
             elif 0.1 < self.wt - h0 <= 0.5:
                 self._checkpoint(cp, L, dL, m, get_action)
+
+    def finish(self):
+        """The finished flags of both files, set when the flow loop ends (launcher_1l.jl:201-209)."""
+        for path in (self.obs_file, self.cp_file):
+            with H5File(path, "cw") as f:
+                if not f.haskey("finished"):
+                    f.write("finished", True)

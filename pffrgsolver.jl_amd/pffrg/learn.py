@@ -167,6 +167,7 @@ class FitReport:
     losses: list
     n_coef: list
     train_rel_err: float
+    train_block_err: list = None  # per block: s, t, u channels, Σ
 
 
 class WarmStart:
@@ -189,15 +190,17 @@ class WarmStart:
             mean = X.mean(axis=0)
             U, s, Vt = np.linalg.svd(X - mean, full_matrices=False)
             svds.append((mean, U, s, Vt))
-        smax = max(float(sv[2][0]) for sv in svds) or 1.0
         for mean, U, s, Vt in svds:
             k = int(min(self.max_coef, n, int(np.sum(s > s[0] * 1e-12)) if s[0] > 0 else 1))
             k = max(k, 1)
-            # targets c_j = U_j s_j / s_max sqrt(n): a unit error in any target costs the same
-            # reconstruction error, so the squared loss tracks the decoded action's error;
-            # decoder basis = Vt_k s_max / sqrt(n)
-            targets.append(U[:, :k] * (s[:k] / smax) * math.sqrt(n))
-            self.decoders.append(Decoder(mean, Vt[:k] * (smax / math.sqrt(n)), self.device))
+            # targets c_j = U_j s_j / s_0 sqrt(n) with the block's own largest singular value s_0:
+            # the leading coefficient is of order one in every block, and a unit error in a target
+            # is a unit relative error of that block's deviation from its mean, so the squared
+            # loss weighs the four blocks (three channels and Σ) alike whatever their magnitudes;
+            # decoder basis = Vt_k s_0 / sqrt(n)
+            sb = float(s[0]) if s[0] > 0 else 1.0
+            targets.append(U[:, :k] * (s[:k] / sb) * math.sqrt(n))
+            self.decoders.append(Decoder(mean, Vt[:k] * (sb / math.sqrt(n)), self.device))
             ncoef.append(k)
         self.ncoef = ncoef
         T = np.concatenate(targets, axis=1).astype(np.float32)
@@ -213,9 +216,18 @@ class WarmStart:
             if verbose and (it % 500 == 0 or it == steps - 1):
                 print(f"  step {it}: loss {losses[-1]:.3e}", file=sys.stderr, flush=True)
         pred = [self.predict_blocks(p) for p in params]
-        err = max(float(np.max(np.abs(pb[b] - bl[b])) / max(np.max(np.abs(bl[b])), 1e-30))
-                  for pb, bl in zip(pred, blocks) for b in range(4))
-        return FitReport(losses, ncoef, err)
+        errs = self.block_errors(pred, blocks)
+        return FitReport(losses, ncoef, max(errs), errs)
+
+    @staticmethod
+    def block_errors(pred, blocks) -> list:
+        """Worst relative max-norm error over the samples, per block (s, t, u channels, Σ)."""
+        return [max(float(np.max(np.abs(pb[b] - bl[b])) / max(np.max(np.abs(bl[b])), 1e-30))
+                    for pb, bl in zip(pred, blocks)) for b in range(4)]
+
+    def held_out_errors(self, params, actions: list[Action]) -> list:
+        """block_errors of the decoded warm starts against given (converged) solutions."""
+        return self.block_errors([self.predict_blocks(p) for p in params], [_blocks(a) for a in actions])
 
     def predict_blocks(self, p):
         x = ((features(np.asarray(p, dtype=np.float64)) - self.fmean) / self.fstd).astype(np.float32)

@@ -169,3 +169,30 @@ def test_flow_measurement_on_the_oracle(tmp_path):
     octx.set_mesh(m)
     octx.upload(F.A_INTER, a)
     np.testing.assert_array_equal(octx.compute_chi(Lc, F.A_INTER, (1e-8, 1e-4)), res.chi[0])
+
+
+def test_flow_forces_checkpoints_at_required_cutoffs(tmp_path):
+    # launcher_1l.jl:59-69, 143-172, 201-209: with a checkpoint interval that never elapses
+    # (ct = 24 h), checkpoints are still written at the user's cutoffs and at Λ_f (measure with
+    # ct = 0), the steps land on those cutoffs, and both files get the finished flag
+    from pffrg import flow as F
+    from helpers import small_problem
+    from oracle_ctx import OracleContext
+    prob = small_problem("square", 2, 6, 5.0)
+    meas = H.Measurement(tmp_path / "obs.h5", tmp_path / "cp.h5", wt=24.0, ct=24.0)
+    octx = OracleContext(prob)
+    res = F.launch_flow(octx, prob, prob.bare_action(), 5.0, 3.0, measurement=meas, cps=[4.0])
+    assert res.status == "finished" and res.Lambdas[-1] == 3.0
+    assert 4.0 in res.Lambdas
+    assert meas.checkpoints == [4.0, 3.0]
+    with H.H5File(tmp_path / "cp.h5", "r") as f:
+        assert f.read("finished") is True
+        assert f.keys("a") == ["3.0", "4.0"]
+        Lc, dL, m, a = H.read_checkpoint(f, 3.0)
+        assert Lc == 3.0
+    with H.H5File(tmp_path / "obs.h5", "r") as f:
+        assert f.read("finished") is True
+        assert len(f.keys("χ")) == res.accepted
+    # the same flow without a measurement hook takes the same steps
+    res2 = F.launch_flow(OracleContext(prob), prob, prob.bare_action(), 5.0, 3.0, cps=[4.0])
+    assert res2.Lambdas == res.Lambdas
