@@ -71,3 +71,75 @@ def test_invariant_detects_planted_mistake(orig, bad, tmp_path):
 
 def test_invariant_holds_for_intact_oracle(tmp_path):
     assert _worst(None, tmp_path) < 2e-3
+
+
+# Planted mask errors (one per pass family): each must make the matching invariance of
+# tests/test_oracle_masks_cpu.py fail on the mutated oracle.
+MASK_MUTATIONS = [
+    ("test_left_pass_masks",
+     "gamma_vec(y, d, D, bs1, bt1, bu1, Mask{true, false, true}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data());",
+     "gamma_vec(y, d, D, bs1, bt1, bu1, Mask{true, true, true}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data());"),
+    ("test_central_pass_masks",
+     "case P_CENTRAL: gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, D, bs2, bt2, bu2, Mask{true, false, false}, t2.data()); break;",
+     "case P_CENTRAL: gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, D, bs2, bt2, bu2, Mask{true, true, false}, t2.data()); break;"),
+    ("test_central_pass_masks",
+     "case P_CENTRAL: gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, D, bs2, bt2, bu2, Mask{false, false, true}, t2.data()); break;",
+     "case P_CENTRAL: gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, D, bs2, bt2, bu2, Mask{false, true, true}, t2.data()); break;"),
+    ("test_bse_is_left_pass_with_the_action_as_derivative",
+     "case P_BSE: gamma_vec(y, d, A, bs1, bt1, bu1, Mask{false, true, true}, t1.data());",
+     "case P_BSE: gamma_vec(y, d, A, bs1, bt1, bu1, Mask{true, true, true}, t1.data());"),
+    ("test_reduced_bubble_left_vertex_is_bare",
+     "double v1s = A.bare[i], v1d = A.bare[ns + i];\n                    double v2s = t2[i], v2d = t2[ns + i];",
+     "double v1s = t2[i], v1d = t2[ns + i];\n                    double v2s = t2[i], v2d = t2[ns + i];"),
+    ("test_sde_self_energy_drops_the_t_channel",  # the dΣ kernel's local t buffer copied into the SDE
+     "double val = 3.0 * gamma_comp(*y, d, *a2, 0, 0, b2s, e, e, SONLY)",
+     "double val = 3.0 * gamma_comp(*y, d, *a2, 0, 0, b2s, buffer_t(*y, v - w, 0.5 * (v + w), 0.5 * (v + w)), e, ALL)"),
+    ("test_first_sigma_correction_masks",
+     "sigma_loop(*y, d, *daS, b, Mask{true, true, false}, Mask{true, false, true});",
+     "sigma_loop(*y, d, *daS, b, Mask{true, true, false}, Mask{true, true, true});"),
+]
+
+
+# Planted errors in the self-energy kernels (frequency arguments, propagator factors): each must
+# make the matching scipy comparison of tests/test_oracle_sigma_cpu.py fail.
+SIGMA_MUTATIONS = [
+    ("test_dsigma_kernel_against_scipy", "b.b1t = buffer_t(y, 0.0, w, v);", "b.b1t = buffer_t(y, 0.0, v, w);"),
+    ("test_dsigma_kernel_against_scipy", "b.b2s = buffer_s(y, v + w, 0.5 * (-v + w), 0.5 * (-v + w));",
+     "b.b2s = buffer_s(y, v + w, 0.5 * (-v + w), 0.5 * (v - w));"),
+    ("test_dsigma_kernel_against_scipy", "b.b2t = buffer_t(y, v - w, 0.5 * (v + w), 0.5 * (v + w));",
+     "b.b2t = buffer_t(y, v + w, 0.5 * (v + w), 0.5 * (v + w));"),
+    ("test_dsigma_kernel_against_scipy", "val *= S_single(*y, L, v, a->Sigma) / (2.0 * PI);",
+     "val *= -G_full(*y, L, v, a->Sigma) / (2.0 * PI);"),
+    ("test_dsigma_kernel_against_scipy", "val -= 2.0 * y.mult[j] * (2.0 * y.S) * gamma_comp",
+     "val -= 2.0 * y.mult[j] * y.S * gamma_comp"),
+    ("test_dsigma_corrections_against_scipy", "val *= G * G * get_sigma(*y, daS->Sigma, v) / (2.0 * PI);",
+     "val *= G * get_sigma(*y, daS->Sigma, v) / (2.0 * PI);"),
+    ("test_sde_sigma_kernel_against_scipy", "Buffer b1s = buffer_s(*y, v + w, INFINITY, 0.5 * (v - w));",
+     "Buffer b1s = buffer_s(*y, v + w, INFINITY, 0.5 * (-v + w));"),
+]
+
+
+def _fails_on_mutant(test_file, test, orig, bad, tmp_path):
+    text = SRC.read_text()
+    assert text.count(orig) == 1, orig
+    (tmp_path / "m.cpp").write_text(text.replace(orig, bad))
+    lib = tmp_path / "liboracle_m.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off",
+                    f"-I{ROOT / 'pffrgsolver.jl_amd' / 'csrc'}", "-shared", "-o", str(lib), str(tmp_path / "m.cpp")],
+                   check=True, timeout=300)
+    env = dict(os.environ, PFFRG_ORACLE_LIB=str(lib))
+    out = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider",
+                          f"{ROOT / 'tests' / test_file}::{test}"],
+                         env=env, capture_output=True, text=True, timeout=600, cwd=str(ROOT))
+    assert out.returncode != 0, f"{test} passed on the mutated oracle:\n{out.stdout[-2000:]}"
+    assert "failed" in out.stdout
+
+
+@pytest.mark.parametrize("test,orig,bad", MASK_MUTATIONS)
+def test_mask_invariance_detects_planted_mistake(test, orig, bad, tmp_path):
+    _fails_on_mutant("test_oracle_masks_cpu.py", test, orig, bad, tmp_path)
+
+
+@pytest.mark.parametrize("test,orig,bad", SIGMA_MUTATIONS)
+def test_sigma_pin_detects_planted_mistake(test, orig, bad, tmp_path):
+    _fails_on_mutant("test_oracle_sigma_cpu.py", test, orig, bad, tmp_path)
