@@ -9,6 +9,7 @@ as the reference's launchers do.
 """
 from __future__ import annotations
 
+import math
 import sys
 import time
 from dataclasses import dataclass, field
@@ -294,7 +295,9 @@ class BS3Flow:
         accept = err <= 1.0 or self.dL <= self.bmin
         if accept:
             self.L -= self.dL
-        self.dL = max(self.bmin, min(self.bmax * self.L, 0.85 * (1.0 / err) ** (1.0 / 3.0) * self.dL))
+        # err = 0 gives an unbounded proposal (Julia's 1/0 = Inf), so the step is bmax Λ
+        grow = 0.85 * (1.0 / err) ** (1.0 / 3.0) * self.dL if err > 0.0 else math.inf
+        self.dL = max(self.bmin, min(self.bmax * self.L, grow))
         lower = [c - self.L for c in self.cps if c - self.L < 0.0]
         L_cp = self.L + lower[0] if lower else 0.0
         self.dL = min(self.dL, self.L - L_cp)

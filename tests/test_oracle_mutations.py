@@ -118,6 +118,17 @@ SIGMA_MUTATIONS = [
      "Buffer b1s = buffer_s(*y, v + w, INFINITY, 0.5 * (-v + w));"),
 ]
 
+# Planted errors in symmetrize_add_to (action_su2.jl:176-250): an unswapped q2 and a site
+# exchange in the u channel must fail the laws of tests/test_oracle_symmetry_cpu.py.
+SYMMETRY_MUTATIONS = [
+    ("test_symmetrize_add_to_laws", "double a21 = l[d.q21 + k] + l[d.q22 + kx];",
+     "double a21 = l[d.q21 + k] + l[d.q21 + kx];"),
+    ("test_symmetrize_add_to_laws", "auto ex = [&](int i) { return (ch == 2) ? i : (int)y.exchange[i]; };",
+     "auto ex = [&](int i) { return (int)y.exchange[i]; };"),
+    ("test_symmetrize_add_to_laws", "o[d.q1 + i + (long)d.ns * w] += l[d.q1 + i + (long)d.ns * w] + l[d.q1 + ex(i) + (long)d.ns * w];",
+     "o[d.q1 + i + (long)d.ns * w] += l[d.q1 + i + (long)d.ns * w] + l[d.q1 + i + (long)d.ns * w];"),
+]
+
 
 def _fails_on_mutant(test_file, test, orig, bad, tmp_path):
     text = SRC.read_text()
@@ -143,3 +154,8 @@ def test_mask_invariance_detects_planted_mistake(test, orig, bad, tmp_path):
 @pytest.mark.parametrize("test,orig,bad", SIGMA_MUTATIONS)
 def test_sigma_pin_detects_planted_mistake(test, orig, bad, tmp_path):
     _fails_on_mutant("test_oracle_sigma_cpu.py", test, orig, bad, tmp_path)
+
+
+@pytest.mark.parametrize("test,orig,bad", SYMMETRY_MUTATIONS)
+def test_symmetry_law_detects_planted_mistake(test, orig, bad, tmp_path):
+    _fails_on_mutant("test_oracle_symmetry_cpu.py", test, orig, bad, tmp_path)
