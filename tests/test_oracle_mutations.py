@@ -129,6 +129,21 @@ SYMMETRY_MUTATIONS = [
      "o[d.q1 + i + (long)d.ns * w] += l[d.q1 + i + (long)d.ns * w] + l[d.q1 + i + (long)d.ns * w];"),
 ]
 
+# Planted errors in the bubble and in the action each side of the left, central and reduced
+# passes: each must fail the bare-only closed forms of tests/test_oracle_passes_cpu.py.
+PASS_MUTATIONS = [
+    ("test_bare_only_pass_closed_form", "    return -prop(y, c.L, x1, x2, c.a->Sigma);\n}",
+     "    return prop(y, c.L, x1, x2, c.a->Sigma);\n}"),
+    ("test_bare_only_pass_closed_form",
+     "case P_LEFT: gamma_vec(y, d, D, bs1, bt1, bu1, Mask{false, true, true}, t1.data());",
+     "case P_LEFT: gamma_vec(y, d, A, bs1, bt1, bu1, Mask{false, true, true}, t1.data());"),
+    ("test_bare_only_pass_closed_form",
+     "gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, D, bs2, bt2, bu2, Mask{false, true, false}, t2.data());",
+     "gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, Mask{false, true, false}, t2.data());"),
+    ("test_bare_only_reduced_closed_form", "double v1s = A.bare[i], v1d = A.bare[ns + i];",
+     "double v1s = 0.5 * A.bare[i], v1d = A.bare[ns + i];"),
+]
+
 
 def _fails_on_mutant(test_file, test, orig, bad, tmp_path):
     text = SRC.read_text()
@@ -159,3 +174,8 @@ def test_sigma_pin_detects_planted_mistake(test, orig, bad, tmp_path):
 @pytest.mark.parametrize("test,orig,bad", SYMMETRY_MUTATIONS)
 def test_symmetry_law_detects_planted_mistake(test, orig, bad, tmp_path):
     _fails_on_mutant("test_oracle_symmetry_cpu.py", test, orig, bad, tmp_path)
+
+
+@pytest.mark.parametrize("test,orig,bad", PASS_MUTATIONS)
+def test_pass_closed_form_detects_planted_mistake(test, orig, bad, tmp_path):
+    _fails_on_mutant("test_oracle_passes_cpu.py", test, orig, bad, tmp_path)
