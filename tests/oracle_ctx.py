@@ -75,3 +75,17 @@ class OracleContext:
 
     def compute_chi(self, L, a, tol=(1e-8, 1e-4)):
         return self.orc.compute_chi(L, self.slots[a], tol)
+
+    def set_sigma(self, s, sig):
+        self.slots[s].Sigma[:] = sig
+
+    def mult_with(self, s, fac, vertex_only=False):
+        self.slots[s].gamma[:] *= fac
+        if not vertex_only:
+            self.slots[s].Sigma[:] *= fac
+
+    def compute_sigma(self, L, a1, a2, eval_, gtol=(1e-8, 1e-4), stol=(1e-8, 1e-4)):
+        self.orc.compute_sigma(L, self.slots[a1], self.slots[a2], eval_, gtol, stol)
+
+    def compute_gamma(self, L, a1, a2, eval_, tol=(1e-8, 1e-4)):
+        self.orc.compute_gamma(L, self.slots[a1], self.slots[a2], eval_, tol)
