@@ -52,8 +52,20 @@ typedef struct {
     int32_t device;   /* HIP device ordinal */
 } pffrg_dims;
 
-/* Context lifetime.  Replaces the per-run allocations of launch_1l! (launcher_1l.jl:38-51). */
+/* Action symmetries (get_action_empty, Action/Action.jl:564-577): the SU(2) action with two vertex
+ * components (spin, density; action_lib/action_su2.jl:9-13) and the U(1)-symmetric DM action with
+ * six (xx, zz, DM, dd, zd, dz; action_lib/action_u1_dm.jl:1-11).  Slot arrays hold n_components
+ * blocks: bare[n_components * n_sites], gamma[n_components * 3 * channel length]. */
+#define PFFRG_SU2 0
+#define PFFRG_U1_DM 1
+
+/* Context lifetime.  Replaces the per-run allocations of launch_1l! (launcher_1l.jl:38-51);
+ * pffrg_create is pffrg_create_symmetry with PFFRG_SU2.  The batched entry points and the
+ * shared-node FP32 pass serve the SU(2) action; a U(1)-DM context runs the FP32-mixed mode on
+ * the per-element pass. */
 int pffrg_create(const pffrg_dims *dims, pffrg_ctx **out);
+int pffrg_create_symmetry(const pffrg_dims *dims, int32_t symmetry, pffrg_ctx **out);
+int32_t pffrg_n_components(const pffrg_ctx *ctx);
 void pffrg_destroy(pffrg_ctx *ctx);
 const char *pffrg_last_error(const pffrg_ctx *ctx);
 
@@ -164,7 +176,9 @@ int pffrg_gather_ceiling(pffrg_ctx *ctx, int32_t slot, int32_t precision, int32_
  * nine boxes of integrate_χ_boxes (correlation_lib.jl:2-30) are integrated by adaptive 2-D
  * cubature with hcubature_v's schedule (degree-7 Genz-Malik rule, several regions split per
  * round, at most 10^7 evaluations per box, as the reference's call); nreg (optional,
- * 9*n_sites*n_chi entries) receives the final region count of every box. */
+ * 9*n_sites*n_chi entries) receives the final region count of every box.  A U(1)-DM context
+ * (compute_χ!, correlation_u1_dm.jl:138-190) writes three blocks of n_sites*n_chi, χxx, χzz and
+ * χxy, each integrated on its own (and three blocks of nreg). */
 int pffrg_compute_chi(pffrg_ctx *ctx, double Lambda, int32_t a, const double *chi_mesh, int32_t n_chi,
                       const double *chi_tol, double *chi, int32_t *nreg);
 
@@ -173,7 +187,7 @@ int pffrg_compute_chi(pffrg_ctx *ctx, double Lambda, int32_t a, const double *ch
  * pffrg_resample moves an action onto the new meshes. */
 
 /* scan_channel (Action.jl:449-480), device part: for slot a and every (component c, channel
- * ch), k = 3*c + ch, the first position of max |q3| in column-major order (Julia's argmax),
+ * ch), k = 3*c + ch (k < 3 n_components), the first position of max |q3| in column-major order (Julia's argmax),
  * 0-based (site, w, v, v') into idx[4k .. 4k+3], and the four cuts through it into
  * cuts[k*(n_Omega + 3*n_nu) ...]:
  *   q3[i, :, v, v'] (n_Omega values),
@@ -185,7 +199,7 @@ int pffrg_scan_cuts(pffrg_ctx *ctx, int32_t a, int32_t *idx, double *cuts);
 /* pffrg_scan_cuts without device allocation.  No counterpart in the reference: for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
  * that forbid hipMalloc).  Same arguments and outputs as pffrg_scan_cuts; its argmax partials and
- * cuts live in one static 18.7 KB scratch per GPU (held until the process ends) instead of the
+ * cuts live in one static 54.5 KB scratch per GPU (held until the process ends) instead of the
  * context's lazily allocated one.  Every context on a GPU shares that scratch, so calls that
  * overlap on one GPU (several host threads or streams) race; each call overwrites what it reads.
  * Rejects (PFFRG_ERR_LIMIT) meshes whose cuts exceed the scratch, which the n_Omega, n_nu <= 64

@@ -39,7 +39,7 @@ class System(C.Structure):
                 ("sigma", C.POINTER(C.c_double)), ("Os", C.POINTER(C.c_double)), ("ns", C.POINTER(C.c_double)),
                 ("Ot", C.POINTER(C.c_double)), ("nt", C.POINTER(C.c_double)),
                 ("exchange", C.POINTER(C.c_int64)), ("mult", C.POINTER(C.c_double)),
-                ("ov_off", C.POINTER(C.c_int64)), ("ov_rows", C.POINTER(C.c_int64))]
+                ("ov_off", C.POINTER(C.c_int64)), ("ov_rows", C.POINTER(C.c_int64)), ("n_comp", C.c_int32)]
 
 
 class ActionS(C.Structure):
@@ -133,7 +133,7 @@ class Oracle:
     """Holds the system tables for one (lattice, mesh) pair and runs the FP64 CPU restatement
     on host Action objects (pffrg.action.Action)."""
 
-    def __init__(self, r, m, S: float = 0.5):
+    def __init__(self, r, m, S: float = 0.5, nc: int = 2):
         self.L = lib()
         off, rows = r.overlap_csr()
         self._keep = dict(
@@ -147,14 +147,16 @@ class Oracle:
         self.sys = System(r.n_sites, m.num_sigma, m.num_Omega, m.num_nu, S, _dp(k["sigma"]), _dp(k["Os"]),
                           _dp(k["ns"]), _dp(k["Ot"]), _dp(k["nt"]), k["exchange"].ctypes.data_as(C.POINTER(C.c_int64)),
                           _dp(k["mult"]), k["off"].ctypes.data_as(C.POINTER(C.c_int64)),
-                          k["rows"].ctypes.data_as(C.POINTER(C.c_int64)))
+                          k["rows"].ctypes.data_as(C.POINTER(C.c_int64)), nc)
         self.m = m
         self.ns = r.n_sites
+        self.nc = nc
         self.corrs = np.zeros(2 * 3 * m.num_Omega)
 
-    @staticmethod
-    def _act(a):
+    def _act(self, a):
         assert a.Sigma.flags.c_contiguous and a.bare.flags.c_contiguous and a.gamma.flags.c_contiguous
+        if getattr(a, "nc", 2) != self.nc:
+            raise ValueError(f"action with {getattr(a, 'nc', 2)} components on an oracle of {self.nc}")
         return ActionS(_dp(a.Sigma), _dp(a.bare), _dp(a.gamma))
 
     def set_threads(self, n: int):
@@ -203,28 +205,33 @@ class Oracle:
                                           C.byref(self._act(out)))
 
     def integrand(self, pass_, Lam, a, da, ch, kernel, w1, w2, w3, v) -> np.ndarray:
-        out = np.zeros((2, self.ns))
+        out = np.zeros((self.nc, self.ns))
         self.L.orc_integrand(C.byref(self.sys), pass_, Lam, C.byref(self._act(a)), C.byref(self._act(da)), ch, kernel,
                              w1, w2, w3, v, _dp(out))
         return out
 
     def compute_chi(self, Lam, a, tol=(1e-8, 1e-4), chi_mesh=None, with_regions=False):
-        """compute_χ! (correlation_su2.jl:58-86): chi[site, w] over the irreducible sites."""
+        """compute_χ! (correlation_su2.jl:58-86): chi[site, w] over the irreducible sites; for the
+        U(1)-DM action (correlation_u1_dm.jl:138-190) chi[c, site, w] for χxx, χzz, χxy."""
         cm = np.ascontiguousarray(self.m.chi if chi_mesh is None else chi_mesh, dtype=np.float64)
-        out = np.zeros(self.ns * len(cm))
-        nreg = np.zeros(9 * self.ns * len(cm), dtype=np.int32)
+        ncc = 1 if self.nc == 2 else 3
+        out = np.zeros(ncc * self.ns * len(cm))
+        nreg = np.zeros(ncc * 9 * self.ns * len(cm), dtype=np.int32)
         self.L.orc_compute_chi(C.byref(self.sys), Lam, C.byref(self._act(a)), _dp(cm), len(cm), _dp(_tol(tol)),
                                _dp(out), nreg.ctypes.data_as(C.POINTER(C.c_int32)))
-        chi = out.reshape(len(cm), self.ns).T.copy()
+        chi = out.reshape(ncc, len(cm), self.ns).transpose(0, 2, 1).copy()
+        if ncc == 1:
+            chi = chi[0]
         return (chi, nreg) if with_regions else chi
 
     def scan_cuts(self, a):
-        """scan_channel's argmax positions idx[6, 4] and cuts[6, nO + 3 nn] (Action.jl:449-476)."""
-        idx = np.zeros(24, dtype=np.int32)
-        cuts = np.zeros(6 * (self.m.num_Omega + 3 * self.m.num_nu))
+        """scan_channel's argmax positions idx[3 nc, 4] and cuts[3 nc, nO + 3 nn] (Action.jl:449-476)."""
+        k = 3 * self.nc
+        idx = np.zeros(4 * k, dtype=np.int32)
+        cuts = np.zeros(k * (self.m.num_Omega + 3 * self.m.num_nu))
         self.L.orc_scan_cuts(C.byref(self.sys), C.byref(self._act(a)), idx.ctypes.data_as(C.POINTER(C.c_int32)),
                              _dp(cuts))
-        return idx.reshape(6, 4), cuts.reshape(6, -1)
+        return idx.reshape(k, 4), cuts.reshape(k, -1)
 
     def resample(self, a_old, m_new, a_new):
         """resample_from_to's interpolation of a_old (on this oracle's meshes) onto m_new."""
@@ -241,19 +248,27 @@ class Oracle:
 
     # low-level probes
     def get_buffer(self, ch, w, v, vp):
-        oi = (C.c_int32 * 11)()
+        """[exch, map, sgn_mu, sgn_nu, kernel, lo1, hi1, lo2, hi2, lo3, hi3] and the weights; the
+        sgn_munu flag (read by the U(1)-DM action only) is get_buffer_munu's."""
+        oi = (C.c_int32 * 12)()
         ow = (C.c_double * 6)()
         self.L.orc_get_buffer(C.byref(self.sys), ch, w, v, vp, oi, ow)
-        return list(oi), list(ow)
+        return list(oi)[:11], list(ow)
+
+    def get_buffer_munu(self, ch, w, v, vp) -> bool:
+        oi = (C.c_int32 * 12)()
+        ow = (C.c_double * 6)()
+        self.L.orc_get_buffer(C.byref(self.sys), ch, w, v, vp, oi, ow)
+        return bool(oi[11])
 
     def get_vertex(self, a, comp, ch, w, v, vp, site) -> float:
         return self.L.orc_get_vertex(C.byref(self.sys), C.byref(self._act(a)), comp, ch, w, v, vp, site)
 
     def get_gamma_avx(self, a, freqs, mask=7) -> np.ndarray:
-        out = np.zeros(2 * self.ns)
+        out = np.zeros(self.nc * self.ns)
         f = np.ascontiguousarray(freqs, dtype=np.float64)
         self.L.orc_get_gamma_avx(C.byref(self.sys), C.byref(self._act(a)), _dp(f), mask, _dp(out))
-        return out.reshape(2, self.ns)
+        return out.reshape(self.nc, self.ns)
 
     def get_sigma(self, Sig, w) -> float:
         return self.L.orc_get_sigma(C.byref(self.sys), _dp(np.ascontiguousarray(Sig)), w)

@@ -1,4 +1,6 @@
-// pffrg_oracle.cpp -- FP64 CPU restatement of the PFFRGSolver.jl SU(2) flow right-hand side.
+// pffrg_oracle.cpp -- FP64 CPU restatement of the PFFRGSolver.jl flow right-hand side for the
+// SU(2) action (2 vertex components: spin, density) and the U(1)-symmetric Dzyaloshinskii-Moriya
+// action (6 components: xx, zz, DM, dd, zd, dz; action_u1_dm.jl:1-11).
 //
 // TEST INFRASTRUCTURE ONLY.  This file is the parity oracle and the CPU baseline
 // ("port") for the HIP library.  Only tests/, __graft_entry__.smoke() and
@@ -10,7 +12,7 @@
 // accumulation) so the GPU FP64 path can be compared to near machine precision.
 //
 // Data layout (same as the reference's column-major Julia arrays, site fastest):
-//   action.gamma = [comp 0..1][channel s,t,u][ q1 | q2_1 | q2_2 | q3 ]
+//   action.gamma = [comp 0..nc-1][channel s,t,u][ q1 | q2_1 | q2_2 | q3 ]
 //     q1  [site, w]          -> site + ns*w
 //     q2_x[site, w, v]       -> site + ns*(w + nO*v)
 //     q3  [site, w, v, vp]   -> site + ns*(w + nO*(v + nn*vp))
@@ -38,11 +40,12 @@ typedef struct {
     const double *mult;                        // [n_sites]
     const int64_t *ov_off;                     // [n_sites+1] row offsets into ov_rows
     const int64_t *ov_rows;                    // [3*rows] (j1, j2 0-based, multiplicity)
+    int32_t n_comp;                            // vertex components: 2 (SU(2)) or 6 (U(1)-DM)
 } orc_system;
 
 typedef struct {
     double *Sigma;  // [n_sigma]
-    double *bare;   // [2*n_sites]
+    double *bare;   // [n_comp*n_sites]
     double *gamma;  // packed channels, see header
 } orc_action;
 
@@ -84,16 +87,19 @@ struct Buffer {
     bool exch, map, sgn_mu, sgn_nu;
     int kernel;
     Param p1, p2, p3;
+    bool sgn_munu;  // read by the U(1)-DM flags only
 };
 
-inline Buffer buffer_empty() { return Buffer{false, false, false, false, 0, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}}; }
+inline Buffer buffer_empty() {
+    return Buffer{false, false, false, false, 0, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, false};
+}
 
 // get_buffer, Frequency/buffer.jl:51-86
 inline Buffer get_buffer(double w, double v, double vp, const double *Om, const double *nu, int nO, int nn,
-                         bool exch, bool map, bool smu, bool snu) {
+                         bool exch, bool map, bool smu, bool snu, bool smunu = false) {
     Buffer b = buffer_empty();
     if (Om[nO - 1] < std::fabs(w)) return b;
-    b.exch = exch; b.map = map; b.sgn_mu = smu; b.sgn_nu = snu;
+    b.exch = exch; b.map = map; b.sgn_mu = smu; b.sgn_nu = snu; b.sgn_munu = smunu;
     if (nu[nn - 1] < std::fabs(v)) {
         if (nu[nn - 1] < std::fabs(vp)) {
             b.kernel = 1; b.p1 = get_param(w, Om, nO);
@@ -119,33 +125,33 @@ inline Buffer buffer_s(const orc_system &y, double w, double v, double vp) {
     return get_buffer(w, v, vp, y.Os, y.ns, y.n_Omega, y.n_nu, ex, mp, smu, snu);
 }
 
-// get_buffer_t, Frequency/buffer.jl:127-160 (sgn_munu is unused by SU(2))
+// get_buffer_t, Frequency/buffer.jl:127-160 (sgn_munu is read by the U(1)-DM flags only)
 inline Buffer buffer_t(const orc_system &y, double w, double v, double vp) {
-    bool smu = false, snu = false;
-    if (w < 0.0) { w *= -1.0; }
+    bool smu = false, snu = false, smunu = false;
+    if (w < 0.0) { w *= -1.0; smunu = !smunu; }
     if (v < 0.0) { v *= -1.0; smu = !smu; }
     if (vp < 0.0) { vp *= -1.0; snu = !snu; }
-    return get_buffer(w, v, vp, y.Ot, y.nt, y.n_Omega, y.n_nu, false, false, smu, snu);
+    return get_buffer(w, v, vp, y.Ot, y.nt, y.n_Omega, y.n_nu, false, false, smu, snu, smunu);
 }
 
 // get_buffer_u, Frequency/buffer.jl:163-199
 inline Buffer buffer_u(const orc_system &y, double w, double v, double vp) {
-    bool ex = false, mp = false, snu = false;
-    if (w < 0.0) { w *= -1.0; ex = !ex; }
+    bool ex = false, mp = false, snu = false, smunu = false;
+    if (w < 0.0) { w *= -1.0; ex = !ex; smunu = !smunu; }
     if (v < 0.0) { v *= -1.0; ex = !ex; mp = !mp; snu = !snu; }
     if (vp < 0.0) { vp *= -1.0; mp = !mp; snu = !snu; }
-    return get_buffer(w, v, vp, y.Os, y.ns, y.n_Omega, y.n_nu, ex, mp, false, snu);
+    return get_buffer(w, v, vp, y.Os, y.ns, y.n_Omega, y.n_nu, ex, mp, false, snu, smunu);
 }
 
 // ---------------------------------------------------------------- Channel access (channel.jl)
 struct Dims {
-    int ns, nO, nn;
+    int ns, nO, nn, nc;
     long q1, q21, q22, q3, ch, comp;
 };
 
 inline Dims dims_of(const orc_system &y) {
     Dims d;
-    d.ns = y.n_sites; d.nO = y.n_Omega; d.nn = y.n_nu;
+    d.ns = y.n_sites; d.nO = y.n_Omega; d.nn = y.n_nu; d.nc = y.n_comp;
     d.q1 = 0;
     d.q21 = (long)d.ns * d.nO;
     d.q22 = d.q21 + (long)d.ns * d.nO * d.nn;
@@ -192,11 +198,25 @@ inline double get_channel(const Dims &d, const double *c, const Buffer &b, int s
     return 0.0;
 }
 
-// apply_flags_su2, Action/action_lib/action_su2.jl:73-92
-inline double flag_sign(const Buffer &b, int comp) {
+// apply_flags_su2 (Action/action_lib/action_su2.jl:73-92) and apply_flags_u1_dm
+// (action_u1_dm.jl:79-118): the sign of a channel's contribution to component comp and the
+// component src it is read from (U(1)-DM: the site exchange swaps zd and dz and flips DM)
+inline double flag_apply(int nc, const Buffer &b, int comp, int &src) {
     double sgn = 1.0;
-    if (comp == 1 && b.sgn_mu) sgn *= -1.0;
-    if (comp == 1 && b.sgn_nu) sgn *= -1.0;
+    src = comp;
+    if (nc == 2) {
+        if (comp == 1 && b.sgn_mu) sgn *= -1.0;
+        if (comp == 1 && b.sgn_nu) sgn *= -1.0;
+        return sgn;
+    }
+    if ((comp == 4 || comp == 5) && b.sgn_munu) sgn *= -1.0;
+    if ((comp == 3 || comp == 5) && b.sgn_mu) sgn *= -1.0;
+    if ((comp == 3 || comp == 4) && b.sgn_nu) sgn *= -1.0;
+    if (b.exch) {
+        if (comp == 2) sgn *= -1.0;
+        else if (comp == 4) src = 5;
+        else if (comp == 5) src = 4;
+    }
     return sgn;
 }
 
@@ -209,44 +229,46 @@ const Mask ALL{true, true, true};
 inline double gamma_comp(const orc_system &y, const Dims &d, const orc_action &a, int comp, int site, const Buffer &bs,
                          const Buffer &bt, const Buffer &bu, Mask m) {
     double val = a.bare[comp * d.ns + site];
+    int src;
     if (m.s) {
         int s = bs.exch ? (int)y.exchange[site] : site;
-        double sgn = flag_sign(bs, comp);
-        val += sgn * get_channel(d, chan(d, a.gamma, comp, bs.map ? 2 : 0), bs, s);
+        double sgn = flag_apply(d.nc, bs, comp, src);
+        val += sgn * get_channel(d, chan(d, a.gamma, src, bs.map ? 2 : 0), bs, s);
     }
     if (m.t) {
         int s = bt.exch ? (int)y.exchange[site] : site;
-        double sgn = flag_sign(bt, comp);
-        val += sgn * get_channel(d, chan(d, a.gamma, comp, 1), bt, s);
+        double sgn = flag_apply(d.nc, bt, comp, src);
+        val += sgn * get_channel(d, chan(d, a.gamma, src, 1), bt, s);
     }
     if (m.u) {
         int s = bu.exch ? (int)y.exchange[site] : site;
-        double sgn = flag_sign(bu, comp);
-        val += sgn * get_channel(d, chan(d, a.gamma, comp, bu.map ? 0 : 2), bu, s);
+        double sgn = flag_apply(d.nc, bu, comp, src);
+        val += sgn * get_channel(d, chan(d, a.gamma, src, bu.map ? 0 : 2), bu, s);
     }
     return val;
 }
 
-// get_Γ_comp_avx! + get_Γ_avx!, Action/Action.jl:110-167 and action_su2.jl:115-134.
-// temp[comp*ns + i]; the bare value is never permuted, the channels are.
+// get_Γ_comp_avx! + get_Γ_avx!, Action/Action.jl:110-167, action_su2.jl:115-134 and
+// action_u1_dm.jl:141-167.  temp[comp*ns + i]; the bare value is never permuted, the channels are.
 inline void gamma_vec(const orc_system &y, const Dims &d, const orc_action &a, const Buffer &bs, const Buffer &bt,
                       const Buffer &bu, Mask m, double *temp) {
-    for (int comp = 0; comp < 2; ++comp) {
+    for (int comp = 0; comp < d.nc; ++comp) {
         double *t = temp + comp * d.ns;
+        int src;
         for (int i = 0; i < d.ns; ++i) t[i] = a.bare[comp * d.ns + i];
         if (m.s && bs.kernel) {
-            double sgn = flag_sign(bs, comp);
-            const double *c = chan(d, a.gamma, comp, bs.map ? 2 : 0);
+            double sgn = flag_apply(d.nc, bs, comp, src);
+            const double *c = chan(d, a.gamma, src, bs.map ? 2 : 0);
             for (int i = 0; i < d.ns; ++i) t[i] += sgn * get_channel(d, c, bs, bs.exch ? (int)y.exchange[i] : i);
         }
         if (m.t && bt.kernel) {
-            double sgn = flag_sign(bt, comp);
-            const double *c = chan(d, a.gamma, comp, 1);
+            double sgn = flag_apply(d.nc, bt, comp, src);
+            const double *c = chan(d, a.gamma, src, 1);
             for (int i = 0; i < d.ns; ++i) t[i] += sgn * get_channel(d, c, bt, bt.exch ? (int)y.exchange[i] : i);
         }
         if (m.u && bu.kernel) {
-            double sgn = flag_sign(bu, comp);
-            const double *c = chan(d, a.gamma, comp, bu.map ? 0 : 2);
+            double sgn = flag_apply(d.nc, bu, comp, src);
+            const double *c = chan(d, a.gamma, src, bu.map ? 0 : 2);
             for (int i = 0; i < d.ns; ++i) t[i] += sgn * get_channel(d, c, bu, bu.exch ? (int)y.exchange[i] : i);
         }
     }
@@ -535,9 +557,87 @@ inline double bubble(const PassCtx &c, int ch, double W, double v) {
     return -prop(y, c.L, x1, x2, c.a->Sigma);
 }
 
-// Integrands: compute_{s,t,u}_{kat,left,central}! (Flow/flow_lib/flow_su2/flow_su2_{s,t,u}.jl),
-// compute_{s,t,u}_BSE! (Flow/parquet_lib/parquet_su2/BSE_su2_{s,t,u}.jl) and
-// compute_s_reduced! (Flow/parquet_lib/parquet_su2/SDE_su2.jl:2-43).
+// Component products of a left vertex a and a right vertex b at one site, the brackets that
+// the integrands multiply by -p.  SU(2): flow_su2_s.jl:38-39 and flow_su2_u.jl:38-39; U(1)-DM
+// (components xx, zz, DM, dd, zd, dz): flow_u1_dm_s.jl:52-57 and flow_u1_dm_u.jl:52-57.
+enum { XX = 0, ZZ = 1, DM = 2, DD = 3, ZD = 4, DZ = 5 };
+
+inline void prod_s(int nc, const double *a, const double *b, double *g) {
+    if (nc == 2) {
+        g[0] = -2.0 * a[0] * b[0] + a[0] * b[1] + a[1] * b[0];
+        g[1] = 3.0 * a[0] * b[0] + a[1] * b[1];
+        return;
+    }
+    g[XX] = a[DM] * b[DZ] - a[DM] * b[ZD] + a[XX] * b[DD] + a[DD] * b[XX] - a[XX] * b[ZZ] - a[DZ] * b[DM] + a[ZD] * b[DM] - a[ZZ] * b[XX];
+    g[ZZ] = a[ZZ] * b[DD] + a[DD] * b[ZZ] - a[DZ] * b[ZD] - 2.0 * a[XX] * b[XX] - 2.0 * a[DM] * b[DM] - a[ZD] * b[DZ];
+    g[DM] = a[DD] * b[DM] + a[DM] * b[DD] - a[ZD] * b[XX] - a[XX] * b[DZ] - a[ZZ] * b[DM] + a[XX] * b[ZD] - a[DM] * b[ZZ] + a[DZ] * b[XX];
+    g[DD] = -a[DZ] * b[DZ] + 2.0 * a[XX] * b[XX] + a[DD] * b[DD] + 2.0 * a[DM] * b[DM] - a[ZD] * b[ZD] + a[ZZ] * b[ZZ];
+    g[ZD] = a[ZD] * b[DD] + a[DD] * b[ZD] - 2.0 * a[DM] * b[XX] + a[DZ] * b[ZZ] + a[ZZ] * b[DZ] + 2.0 * a[XX] * b[DM];
+    g[DZ] = a[ZD] * b[ZZ] + a[DD] * b[DZ] + a[ZZ] * b[ZD] + 2.0 * a[DM] * b[XX] + a[DZ] * b[DD] - 2.0 * a[XX] * b[DM];
+}
+
+inline void prod_u(int nc, const double *a, const double *b, double *g) {
+    if (nc == 2) {
+        g[0] = 2.0 * a[0] * b[0] + a[0] * b[1] + a[1] * b[0];
+        g[1] = 3.0 * a[0] * b[0] + a[1] * b[1];
+        return;
+    }
+    g[XX] = -a[DM] * b[DZ] - a[DM] * b[ZD] + a[XX] * b[DD] + a[DD] * b[XX] + a[XX] * b[ZZ] + a[DZ] * b[DM] + a[ZD] * b[DM] + a[ZZ] * b[XX];
+    g[ZZ] = a[ZZ] * b[DD] + a[DD] * b[ZZ] - a[DZ] * b[ZD] + 2.0 * a[XX] * b[XX] + 2.0 * a[DM] * b[DM] - a[ZD] * b[DZ];
+    g[DM] = a[DD] * b[DM] + a[DM] * b[DD] - a[ZD] * b[XX] + a[XX] * b[DZ] + a[ZZ] * b[DM] + a[XX] * b[ZD] + a[DM] * b[ZZ] - a[DZ] * b[XX];
+    g[DD] = -a[DZ] * b[DZ] + 2.0 * a[XX] * b[XX] + a[DD] * b[DD] + 2.0 * a[DM] * b[DM] - a[ZD] * b[ZD] + a[ZZ] * b[ZZ];
+    g[ZD] = a[ZD] * b[DD] + a[DD] * b[ZD] - 2.0 * a[DM] * b[XX] + a[DZ] * b[ZZ] + a[ZZ] * b[DZ] + 2.0 * a[XX] * b[DM];
+    g[DZ] = a[ZD] * b[ZZ] + a[DD] * b[DZ] + a[ZZ] * b[ZD] - 2.0 * a[DM] * b[XX] + a[DZ] * b[DD] + 2.0 * a[XX] * b[DM];
+}
+
+// t channel, local part: non-local left vertex a (site i) with the local right vertex c4, and
+// the local left vertex c3 with the non-local right vertex b (flow_su2_t.jl:57-58,
+// flow_u1_dm_t.jl:56-67)
+inline void prod_t_local(int nc, const double *a, const double *c4, const double *c3, const double *b, double *g) {
+    if (nc == 2) {
+        g[0] = -1.0 * a[0] * c4[0] + a[0] * c4[1] - 1.0 * c3[0] * b[0] + c3[1] * b[0];
+        g[1] = 3.0 * a[1] * c4[0] + a[1] * c4[1] + 3.0 * c3[0] * b[1] + c3[1] * b[1];
+        return;
+    }
+    g[XX] = a[DM] * c4[DZ] - a[DM] * c4[ZD] + a[XX] * c4[DD] - a[XX] * c4[ZZ] + c3[DD] * b[XX] + c3[DZ] * b[DM] -
+            c3[ZD] * b[DM] - c3[ZZ] * b[XX];
+    g[ZZ] = a[ZZ] * c4[DD] + 2.0 * a[ZD] * c4[DM] - a[ZD] * c4[ZD] + a[ZZ] * c4[ZZ] - 2.0 * a[ZZ] * c4[XX] -
+            a[ZD] * c4[DZ] - 2.0 * c3[DM] * b[DZ] - c3[DZ] * b[DZ] + c3[DD] * b[ZZ] - 2.0 * c3[XX] * b[ZZ] +
+            c3[ZZ] * b[ZZ] - c3[ZD] * b[DZ];
+    g[DM] = a[DM] * c4[DD] - a[XX] * c4[DZ] + a[XX] * c4[ZD] - a[DM] * c4[ZZ] - c3[DZ] * b[XX] + c3[DD] * b[DM] +
+            c3[ZD] * b[XX] - c3[ZZ] * b[DM];
+    g[DD] = -a[DZ] * c4[DZ] + 2.0 * a[DD] * c4[XX] + a[DD] * c4[ZZ] - a[DZ] * c4[ZD] - 2.0 * a[DZ] * c4[DM] +
+            a[DD] * c4[DD] + 2.0 * c3[DM] * b[ZD] + 2.0 * c3[XX] * b[DD] + c3[ZZ] * b[DD] - c3[DZ] * b[ZD] +
+            c3[DD] * b[DD] - c3[ZD] * b[ZD];
+    g[ZD] = a[ZD] * c4[ZZ] + a[ZD] * c4[DD] + 2.0 * a[ZD] * c4[XX] + a[ZZ] * c4[ZD] + 2.0 * a[ZZ] * c4[DM] +
+            a[ZZ] * c4[DZ] + c3[ZD] * b[DD] + 2.0 * c3[DM] * b[DD] + c3[DD] * b[ZD] + c3[ZZ] * b[ZD] -
+            2.0 * c3[XX] * b[ZD] + c3[DZ] * b[DD];
+    g[DZ] = -2.0 * a[DD] * c4[DM] + a[DD] * c4[ZD] + a[DD] * c4[DZ] + a[DZ] * c4[ZZ] + a[DZ] * c4[DD] -
+            2.0 * a[DZ] * c4[XX] + c3[ZD] * b[ZZ] + 2.0 * c3[XX] * b[DZ] + c3[DD] * b[DZ] + c3[DZ] * b[ZZ] -
+            2.0 * c3[DM] * b[ZZ] + c3[ZZ] * b[DZ];
+}
+
+// t channel, one overlap row (j1, j2, m): G += -p (-2) m [(2S)] (a[j1] . b[j2])
+// (flow_su2_t.jl:70-71, flow_u1_dm_t.jl:88-93; the U(1)-DM sum carries no 2S)
+inline void add_t_row(int nc, double p, double m, double S2, const double *a, const double *b, double *G) {
+    if (nc == 2) {
+        G[0] += -p * (-2.0) * m * S2 * a[0] * b[0];
+        G[1] += -p * (-2.0) * m * S2 * a[1] * b[1];
+        return;
+    }
+    G[XX] += -p * (-2.0) * m * (a[XX] * b[XX] - a[DM] * b[DM]);
+    G[ZZ] += -p * (-2.0) * m * (a[ZZ] * b[ZZ] - a[ZD] * b[DZ]);
+    G[DM] += -p * (-2.0) * m * (a[DM] * b[XX] + a[XX] * b[DM]);
+    G[DD] += -p * (-2.0) * m * (a[DD] * b[DD] - a[DZ] * b[ZD]);
+    G[ZD] += -p * (-2.0) * m * (a[ZD] * b[DD] + a[ZZ] * b[ZD]);
+    G[DZ] += -p * (-2.0) * m * (a[DD] * b[DZ] + a[DZ] * b[ZZ]);
+}
+
+// Integrands: compute_{s,t,u}_{kat,left,central}! (Flow/flow_lib/flow_su2/flow_su2_{s,t,u}.jl,
+// flow_lib/flow_u1_dm/flow_u1_dm_{s,t,u}.jl), compute_{s,t,u}_BSE! (Flow/parquet_lib/
+// parquet_su2/BSE_su2_{s,t,u}.jl, parquet_u1_dm/BSE_u1_dm_{s,t,u}.jl) and compute_s_reduced!
+// (SDE_su2.jl:2-43, SDE_u1_dm.jl:2-43).  The buffers and channel masks are the same for both
+// actions; the component products differ.  buff[nc*i + comp].
 struct Integrand {
     const PassCtx *c;
     int ch;
@@ -548,63 +648,62 @@ struct Integrand {
         const PassCtx &C = *c;
         const orc_system &y = *C.y;
         const Dims &d = C.d;
-        const int ns = d.ns;
+        const int ns = d.ns, nc = d.nc;
         const double p = bubble(C, ch, W, v);
         const orc_action &A = *C.a;
         const orc_action &D = *C.da;
-        if (ch == 0) {
-            const double s = W, vs = vw, vsp = vwp;
-            Buffer bs2 = buffer_s(y, s, v, vsp);
-            Buffer bt2 = buffer_t(y, -v - vsp, 0.5 * (s + v - vsp), 0.5 * (s - v + vsp));
-            Buffer bu2 = buffer_u(y, v - vsp, 0.5 * (s + v + vsp), 0.5 * (s - v - vsp));
-            if (C.pass == P_REDUCED) {
-                gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data());
-                for (int i = 0; i < ns; ++i) {
-                    double v1s = A.bare[i], v1d = A.bare[ns + i];
-                    double v2s = t2[i], v2d = t2[ns + i];
-                    double Gs = -p * (-2.0 * v1s * v2s + v1s * v2d + v1d * v2s);
-                    double Gd = -p * (3.0 * v1s * v2s + v1d * v2d);
-                    buff[2 * i] += dv * Gs;
-                    buff[2 * i + 1] += dv * Gd;
+        double a[6], b[6], g[6];
+        auto site = [&](const std::vector<double> &t, int i, double *x) {
+            for (int k = 0; k < nc; ++k) x[k] = t[k * ns + i];
+        };
+        if (ch == 0 || ch == 2) {
+            if (ch == 0) {
+                const double s = W, vs = vw, vsp = vwp;
+                Buffer bs2 = buffer_s(y, s, v, vsp);
+                Buffer bt2 = buffer_t(y, -v - vsp, 0.5 * (s + v - vsp), 0.5 * (s - v + vsp));
+                Buffer bu2 = buffer_u(y, v - vsp, 0.5 * (s + v + vsp), 0.5 * (s - v - vsp));
+                if (C.pass == P_REDUCED) {
+                    // the left vertex is the bare coupling; U(1)-DM keeps xx, zz, dd (SDE_u1_dm.jl:36-42)
+                    gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data());
+                    for (int i = 0; i < ns; ++i) {
+                        for (int k = 0; k < nc; ++k) a[k] = A.bare[k * ns + i];
+                        site(t2, i, b);
+                        prod_s(nc, a, b, g);
+                        for (int k = 0; k < nc; ++k)
+                            if (nc == 2 || k == XX || k == ZZ || k == DD) buff[nc * i + k] += dv * (-p * g[k]);
+                    }
+                    return;
                 }
-                return;
-            }
-            Buffer bs1 = buffer_s(y, s, vs, -v);
-            Buffer bt1 = buffer_t(y, v - vs, 0.5 * (s + v + vs), 0.5 * (s - v - vs));
-            Buffer bu1 = buffer_u(y, v + vs, 0.5 * (s - v + vs), 0.5 * (s + v - vs));
-            switch (C.pass) {
-                case P_KAT: gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
-                case P_LEFT: gamma_vec(y, d, D, bs1, bt1, bu1, Mask{false, true, true}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
-                case P_CENTRAL: gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, D, bs2, bt2, bu2, Mask{true, false, false}, t2.data()); break;
-                case P_BSE: gamma_vec(y, d, A, bs1, bt1, bu1, Mask{false, true, true}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
-            }
-            for (int i = 0; i < ns; ++i) {
-                double v1s = t1[i], v1d = t1[ns + i], v2s = t2[i], v2d = t2[ns + i];
-                double Gs = -p * (-2.0 * v1s * v2s + v1s * v2d + v1d * v2s);
-                double Gd = -p * (3.0 * v1s * v2s + v1d * v2d);
-                buff[2 * i] += dv * Gs;
-                buff[2 * i + 1] += dv * Gd;
-            }
-        } else if (ch == 2) {
-            const double u = W, vu = vw, vup = vwp;
-            Buffer bs1 = buffer_s(y, v + vu, 0.5 * (u - v + vu), 0.5 * (-u - v + vu));
-            Buffer bt1 = buffer_t(y, v - vu, 0.5 * (u + v + vu), 0.5 * (-u + v + vu));
-            Buffer bu1 = buffer_u(y, u, vu, v);
-            Buffer bs2 = buffer_s(y, v + vup, 0.5 * (u + v - vup), 0.5 * (-u + v - vup));
-            Buffer bt2 = buffer_t(y, -v + vup, 0.5 * (u + v + vup), 0.5 * (-u + v + vup));
-            Buffer bu2 = buffer_u(y, u, v, vup);
-            switch (C.pass) {
-                case P_KAT: gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
-                case P_LEFT: gamma_vec(y, d, D, bs1, bt1, bu1, Mask{true, true, false}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
-                case P_CENTRAL: gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, D, bs2, bt2, bu2, Mask{false, false, true}, t2.data()); break;
-                case P_BSE: gamma_vec(y, d, A, bs1, bt1, bu1, Mask{true, true, false}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
+                Buffer bs1 = buffer_s(y, s, vs, -v);
+                Buffer bt1 = buffer_t(y, v - vs, 0.5 * (s + v + vs), 0.5 * (s - v - vs));
+                Buffer bu1 = buffer_u(y, v + vs, 0.5 * (s - v + vs), 0.5 * (s + v - vs));
+                switch (C.pass) {
+                    case P_KAT: gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
+                    case P_LEFT: gamma_vec(y, d, D, bs1, bt1, bu1, Mask{false, true, true}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
+                    case P_CENTRAL: gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, D, bs2, bt2, bu2, Mask{true, false, false}, t2.data()); break;
+                    case P_BSE: gamma_vec(y, d, A, bs1, bt1, bu1, Mask{false, true, true}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
+                }
+            } else {
+                const double u = W, vu = vw, vup = vwp;
+                Buffer bs1 = buffer_s(y, v + vu, 0.5 * (u - v + vu), 0.5 * (-u - v + vu));
+                Buffer bt1 = buffer_t(y, v - vu, 0.5 * (u + v + vu), 0.5 * (-u + v + vu));
+                Buffer bu1 = buffer_u(y, u, vu, v);
+                Buffer bs2 = buffer_s(y, v + vup, 0.5 * (u + v - vup), 0.5 * (-u + v - vup));
+                Buffer bt2 = buffer_t(y, -v + vup, 0.5 * (u + v + vup), 0.5 * (-u + v + vup));
+                Buffer bu2 = buffer_u(y, u, v, vup);
+                switch (C.pass) {
+                    case P_KAT: gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
+                    case P_LEFT: gamma_vec(y, d, D, bs1, bt1, bu1, Mask{true, true, false}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
+                    case P_CENTRAL: gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, D, bs2, bt2, bu2, Mask{false, false, true}, t2.data()); break;
+                    case P_BSE: gamma_vec(y, d, A, bs1, bt1, bu1, Mask{true, true, false}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data()); break;
+                }
             }
             for (int i = 0; i < ns; ++i) {
-                double v1s = t1[i], v1d = t1[ns + i], v2s = t2[i], v2d = t2[ns + i];
-                double Gs = -p * (2.0 * v1s * v2s + v1s * v2d + v1d * v2s);
-                double Gd = -p * (3.0 * v1s * v2s + v1d * v2d);
-                buff[2 * i] += dv * Gs;
-                buff[2 * i + 1] += dv * Gd;
+                site(t1, i, a);
+                site(t2, i, b);
+                if (ch == 0) prod_s(nc, a, b, g);
+                else prod_u(nc, a, b, g);
+                for (int k = 0; k < nc; ++k) buff[nc * i + k] += dv * (-p * g[k]);
             }
         } else {
             const double t = W, vt = vw, vtp = vwp;
@@ -620,43 +719,46 @@ struct Integrand {
             Buffer bs4 = buffer_s(y, v + vtp, 0.5 * (t - v + vtp), 0.5 * (-t - v + vtp));
             Buffer bt4 = buffer_t(y, v - vtp, 0.5 * (t + v + vtp), 0.5 * (-t + v + vtp));
             Buffer bu4 = buffer_u(y, t, vtp, v);
-            double v3s, v3d, v4s, v4d;
+            // local (site-0) vertices v3, v4 and the non-local caches of each pass
+            double c3[6], c4[6];
+            const orc_action *V3 = &A, *V4 = &A;
+            Mask m3 = ALL, m4 = ALL;
             switch (C.pass) {
                 case P_KAT:
-                    v3s = gamma_comp(y, d, A, 0, 0, bs3, bt3, bu3, ALL); v3d = gamma_comp(y, d, A, 1, 0, bs3, bt3, bu3, ALL);
-                    v4s = gamma_comp(y, d, A, 0, 0, bs4, bt4, bu4, ALL); v4d = gamma_comp(y, d, A, 1, 0, bs4, bt4, bu4, ALL);
                     gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data());
                     break;
                 case P_LEFT:
-                    v3s = gamma_comp(y, d, D, 0, 0, bs3, bt3, bu3, Mask{true, true, false}); v3d = gamma_comp(y, d, D, 1, 0, bs3, bt3, bu3, Mask{true, true, false});
-                    v4s = gamma_comp(y, d, A, 0, 0, bs4, bt4, bu4, ALL); v4d = gamma_comp(y, d, A, 1, 0, bs4, bt4, bu4, ALL);
+                    V3 = &D; m3 = Mask{true, true, false};
                     gamma_vec(y, d, D, bs1, bt1, bu1, Mask{true, false, true}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data());
                     break;
                 case P_CENTRAL:
-                    v3s = gamma_comp(y, d, A, 0, 0, bs3, bt3, bu3, ALL); v3d = gamma_comp(y, d, A, 1, 0, bs3, bt3, bu3, ALL);
-                    v4s = gamma_comp(y, d, D, 0, 0, bs4, bt4, bu4, Mask{false, false, true}); v4d = gamma_comp(y, d, D, 1, 0, bs4, bt4, bu4, Mask{false, false, true});
+                    V4 = &D; m4 = Mask{false, false, true};
                     gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, D, bs2, bt2, bu2, Mask{false, true, false}, t2.data());
                     break;
                 default:  // P_BSE
-                    v3s = gamma_comp(y, d, A, 0, 0, bs3, bt3, bu3, Mask{true, true, false}); v3d = gamma_comp(y, d, A, 1, 0, bs3, bt3, bu3, Mask{true, true, false});
-                    v4s = gamma_comp(y, d, A, 0, 0, bs4, bt4, bu4, ALL); v4d = gamma_comp(y, d, A, 1, 0, bs4, bt4, bu4, ALL);
+                    m3 = Mask{true, true, false};
                     gamma_vec(y, d, A, bs1, bt1, bu1, Mask{true, false, true}, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, ALL, t2.data());
                     break;
             }
+            for (int k = 0; k < nc; ++k) {
+                c3[k] = gamma_comp(y, d, *V3, k, 0, bs3, bt3, bu3, m3);
+                c4[k] = gamma_comp(y, d, *V4, k, 0, bs4, bt4, bu4, m4);
+            }
             const double S2 = 2.0 * y.S;
+            double G[6], w1v[6], w2v[6];
             for (int i = 0; i < ns; ++i) {
-                double v1s = t1[i], v1d = t1[ns + i], v2s = t2[i], v2d = t2[ns + i];
-                double Gs = -p * (-1.0 * v1s * v4s + v1s * v4d - 1.0 * v3s * v2s + v3d * v2s);
-                double Gd = -p * (3.0 * v1d * v4s + v1d * v4d + 3.0 * v3s * v2d + v3d * v2d);
+                site(t1, i, a);
+                site(t2, i, b);
+                prod_t_local(nc, a, c4, c3, b, g);
+                for (int k = 0; k < nc; ++k) G[k] = -p * g[k];
                 for (long r = y.ov_off[i]; r < y.ov_off[i + 1]; ++r) {
-                    int j1 = (int)y.ov_rows[3 * r], j2 = (int)y.ov_rows[3 * r + 1];
-                    double m = (double)y.ov_rows[3 * r + 2];
-                    double w1s = t1[j1], w1d = t1[ns + j1], w2s = t2[j2], w2d = t2[ns + j2];
-                    Gs += -p * (-2.0) * m * S2 * w1s * w2s;
-                    Gd += -p * (-2.0) * m * S2 * w1d * w2d;
+                    const int j1 = (int)y.ov_rows[3 * r], j2 = (int)y.ov_rows[3 * r + 1];
+                    const double m = (double)y.ov_rows[3 * r + 2];
+                    site(t1, j1, w1v);
+                    site(t2, j2, w2v);
+                    add_t_row(nc, p, m, S2, w1v, w2v, G);
                 }
-                buff[2 * i] += dv * Gs;
-                buff[2 * i + 1] += dv * Gd;
+                for (int k = 0; k < nc; ++k) buff[nc * i + k] += dv * G[k];
             }
         }
     }
@@ -677,7 +779,7 @@ long compute_element(const PassCtx &C, int ch, int kernel, int w1, int w2, int w
                      double *t3, int eval, double atol, double rtol) {
     const orc_system &y = *C.y;
     const Dims &d = C.d;
-    const int n = 2 * d.ns;
+    const int n = d.nc * d.ns;
     for (int i = 0; i < n; ++i) t1[i] = 0.0;
     double W, vw, vwp;
     if (C.pass == P_REDUCED) {
@@ -698,14 +800,14 @@ long compute_element(const PassCtx &C, int ch, int kernel, int w1, int w2, int w
     f(t1, val, corr_at(C, 1, ch, w1));
     nodes += 2;
     // parse result (flow_s.jl:41-51)
-    for (int comp = 0; comp < 2; ++comp) {
+    for (int comp = 0; comp < d.nc; ++comp) {
         double *c = chan(d, out.gamma, comp, ch);
         long base;
         if (kernel == 1) base = d.q1 + (long)d.ns * w1;
         else if (kernel == 2) base = d.q21 + (long)d.ns * (w1 + (long)d.nO * w2);
         else if (kernel == 3) base = d.q22 + (long)d.ns * (w1 + (long)d.nO * w3);
         else base = d.q3 + (long)d.ns * (w1 + (long)d.nO * (w2 + (long)d.nn * w3));
-        for (int i = 0; i < d.ns; ++i) c[base + i] = t1[comp + 2 * i];
+        for (int i = 0; i < d.ns; ++i) c[base + i] = t1[comp + d.nc * i];
     }
     return nodes;
 }
@@ -732,7 +834,7 @@ std::vector<Elem> elements(const Dims &d, bool full) {
 
 long run_pass(const PassCtx &C, const std::vector<Elem> &E, orc_action &out, int eval, double atol, double rtol) {
     long total = 0;
-    const int n = 2 * C.d.ns;
+    const int n = C.d.nc * C.d.ns;
 #pragma omp parallel reduction(+ : total)
     {
         std::vector<double> t1(n), t2(n), t3(n);
@@ -745,38 +847,59 @@ long run_pass(const PassCtx &C, const std::vector<Elem> &E, orc_action &out, int
     return total;
 }
 
-// symmetrize!, Action/action_lib/action_su2.jl:141-173
+// The ν <-> ν' exchange of component comp in channel ch: the right part reads component
+// sym_src with sign sym_sign (action_su2.jl:141-250; action_u1_dm.jl:170-339: DM changes sign
+// and zd, dz swap with a sign in the s and t channels, zd and dz change sign in the u channel)
+inline int sym_src(int nc, int ch, int comp) {
+    if (nc == 6 && ch != 2 && comp == ZD) return DZ;
+    if (nc == 6 && ch != 2 && comp == DZ) return ZD;
+    return comp;
+}
+inline double sym_sign(int nc, int ch, int comp) {
+    if (nc != 6) return 1.0;
+    if (ch != 2) return (comp == DM || comp == ZD || comp == DZ) ? -1.0 : 1.0;
+    return (comp == ZD || comp == DZ) ? -1.0 : 1.0;
+}
+
+// symmetrize!, Action/action_lib/action_su2.jl:141-173 and action_u1_dm.jl:170-215
 void symmetrize(const orc_system &y, const Dims &d, orc_action &a) {
-    for (int comp = 0; comp < 2; ++comp)
+    // the upper triangle (v < v') is written from lower-triangle entries only, also where the
+    // U(1)-DM s and t channels read the other component of the pair (zd, dz)
+    for (int comp = 0; comp < d.nc; ++comp)
         for (int ch = 0; ch < 3; ++ch) {
             double *q3 = chan(d, a.gamma, comp, ch) + d.q3;
+            const double sg = sym_sign(d.nc, ch, comp);
+            const double *src = chan(d, a.gamma, sym_src(d.nc, ch, comp), ch) + d.q3;
             for (int v = 0; v < d.nn; ++v)
                 for (int vp = v + 1; vp < d.nn; ++vp)
                     for (int w = 0; w < d.nO; ++w)
                         for (int i = 0; i < d.ns; ++i) {
-                            int src = (ch == 2) ? i : (int)y.exchange[i];
-                            q3[i + (long)d.ns * (w + (long)d.nO * (v + (long)d.nn * vp))] =
-                                q3[src + (long)d.ns * (w + (long)d.nO * (vp + (long)d.nn * v))];
+                            int s = (ch == 2) ? i : (int)y.exchange[i];
+                            const double x = src[s + (long)d.ns * (w + (long)d.nO * (vp + (long)d.nn * v))];
+                            q3[i + (long)d.ns * (w + (long)d.nO * (v + (long)d.nn * vp))] = (d.nc == 2) ? x : sg * x;
                         }
         }
 }
 
-// symmetrize_add_to!, Action/action_lib/action_su2.jl:176-250
+// symmetrize_add_to!, Action/action_lib/action_su2.jl:176-250 and action_u1_dm.jl:218-339
 void symmetrize_add_to(const orc_system &y, const Dims &d, const orc_action &al, orc_action &a) {
-    for (int comp = 0; comp < 2; ++comp)
+    for (int comp = 0; comp < d.nc; ++comp)
         for (int ch = 0; ch < 3; ++ch) {
             const double *l = chan(d, al.gamma, comp, ch);
+            const double *r = chan(d, al.gamma, sym_src(d.nc, ch, comp), ch);
+            const double sg = sym_sign(d.nc, ch, comp);
             double *o = chan(d, a.gamma, comp, ch);
             auto ex = [&](int i) { return (ch == 2) ? i : (int)y.exchange[i]; };
+            auto pair = [&](double x, double xr) { return (d.nc == 2) ? x + xr : (sg > 0.0 ? x + xr : x - xr); };
             for (int w = 0; w < d.nO; ++w)
                 for (int i = 0; i < d.ns; ++i)
-                    o[d.q1 + i + (long)d.ns * w] += l[d.q1 + i + (long)d.ns * w] + l[d.q1 + ex(i) + (long)d.ns * w];
+                    o[d.q1 + i + (long)d.ns * w] += pair(l[d.q1 + i + (long)d.ns * w], r[d.q1 + ex(i) + (long)d.ns * w]);
             for (int v = 0; v < d.nn; ++v)
                 for (int w = 0; w < d.nO; ++w)
                     for (int i = 0; i < d.ns; ++i) {
                         long k = i + (long)d.ns * (w + (long)d.nO * v), kx = ex(i) + (long)d.ns * (w + (long)d.nO * v);
-                        double a21 = l[d.q21 + k] + l[d.q22 + kx];
-                        double a22 = l[d.q22 + k] + l[d.q21 + kx];
+                        double a21 = pair(l[d.q21 + k], r[d.q22 + kx]);
+                        double a22 = pair(l[d.q22 + k], r[d.q21 + kx]);
                         o[d.q21 + k] += a21;
                         o[d.q22 + k] += a22;
                     }
@@ -786,12 +909,12 @@ void symmetrize_add_to(const orc_system &y, const Dims &d, const orc_action &al,
                         for (int i = 0; i < d.ns; ++i) {
                             long k = i + (long)d.ns * (w + (long)d.nO * (v + (long)d.nn * vp));
                             long kx = ex(i) + (long)d.ns * (w + (long)d.nO * (vp + (long)d.nn * v));
-                            o[d.q3 + k] += l[d.q3 + k] + l[d.q3 + kx];
+                            o[d.q3 + k] += pair(l[d.q3 + k], r[d.q3 + kx]);
                         }
         }
 }
 
-long gamma_size(const Dims &d) { return 2 * d.comp; }
+long gamma_size(const Dims &d) { return d.nc * d.comp; }
 
 // compute_corrs_kat! / compute_corrs!, Flow/bubbles.jl:99-220
 void compute_corrs(const orc_system &y, double L, const orc_action &a, const orc_action *da, bool kat, double *corrs,
@@ -992,9 +1115,10 @@ double hcubature2(const F &f, const double *lo, const double *hi, double atol, d
 }
 
 // ---------------------------------------------------------------- correlations
-// compute_χ_kernel! (correlation_su2.jl:14-55) at one point (x, y) of the unit square
+// compute_χ_kernel! (correlation_su2.jl:14-55) at one point (x, y) of the unit square, and the
+// U(1)-DM kernels compute_χ_kernel_{xx,zz,xy}! (correlation_u1_dm.jl:16-134) for comp 0, 1, 2
 inline double chi_kernel(const orc_system &y, const Dims &d, double L, int site, double w, const orc_action &a,
-                         double xx, double yy) {
+                         double xx, double yy, int comp = 0) {
     const double v = (2.0 * xx - 1.0) / ((1.0 - xx) * xx);
     const double vp = (2.0 * yy - 1.0) / ((1.0 - yy) * yy);
     const double dv = (2.0 * xx * xx - 2.0 * xx + 1) / ((1.0 - xx) * (1.0 - xx) * xx * xx);
@@ -1002,16 +1126,40 @@ inline double chi_kernel(const orc_system &y, const Dims &d, double L, int site,
     const Buffer bs1 = buffer_s(y, v + vp, 0.5 * (v - w - vp), 0.5 * (-v - w + vp));
     const Buffer bt1 = buffer_t(y, w, v, vp);
     const Buffer bu1 = buffer_u(y, v - vp, 0.5 * (v - w + vp), 0.5 * (v + w + vp));
-    const double S2 = 2.0 * y.S;
     const double tp2 = (2.0 * PI) * (2.0 * PI);
-    double val = -(S2 * S2) * gamma_comp(y, d, a, 0, site, bs1, bt1, bu1, ALL) / tp2;
-    if (site == 0) {
-        const Buffer bs2 = buffer_s(y, v + vp, 0.5 * (v - w - vp), 0.5 * (v + w - vp));
-        const Buffer bt2 = buffer_t(y, -v + vp, 0.5 * (v - w + vp), 0.5 * (v + w + vp));
-        const Buffer bu2 = buffer_u(y, -w, v, vp);
-        const double vs = gamma_comp(y, d, a, 0, site, bs2, bt2, bu2, ALL);
-        const double vd = gamma_comp(y, d, a, 1, site, bs2, bt2, bu2, ALL);
-        val -= S2 * (vs - vd) / (2.0 * tp2);
+    double val;
+    if (d.nc == 2) {
+        const double S2 = 2.0 * y.S;
+        val = -(S2 * S2) * gamma_comp(y, d, a, 0, site, bs1, bt1, bu1, ALL) / tp2;
+        if (site == 0) {
+            const Buffer bs2 = buffer_s(y, v + vp, 0.5 * (v - w - vp), 0.5 * (v + w - vp));
+            const Buffer bt2 = buffer_t(y, -v + vp, 0.5 * (v - w + vp), 0.5 * (v + w + vp));
+            const Buffer bu2 = buffer_u(y, -w, v, vp);
+            const double vs = gamma_comp(y, d, a, 0, site, bs2, bt2, bu2, ALL);
+            const double vd = gamma_comp(y, d, a, 1, site, bs2, bt2, bu2, ALL);
+            val -= S2 * (vs - vd) / (2.0 * tp2);
+        }
+    } else {
+        val = -gamma_comp(y, d, a, comp == 2 ? DM : comp, site, bs1, bt1, bu1, ALL) / tp2;
+        if (site == 0) {
+            const Buffer bs2 = buffer_s(y, v + vp, 0.5 * (v - w - vp), 0.5 * (v + w - vp));
+            const Buffer bt2 = buffer_t(y, -v + vp, 0.5 * (v - w + vp), 0.5 * (v + w + vp));
+            const Buffer bu2 = buffer_u(y, -w, v, vp);
+            if (comp == 0) {
+                const double vzz = gamma_comp(y, d, a, ZZ, site, bs2, bt2, bu2, ALL);
+                const double vdd = gamma_comp(y, d, a, DD, site, bs2, bt2, bu2, ALL);
+                val -= (vzz - vdd) / (2.0 * tp2);
+            } else if (comp == 1) {
+                const double vxx = gamma_comp(y, d, a, XX, site, bs2, bt2, bu2, ALL);
+                const double vzz = gamma_comp(y, d, a, ZZ, site, bs2, bt2, bu2, ALL);
+                const double vdd = gamma_comp(y, d, a, DD, site, bs2, bt2, bu2, ALL);
+                val -= (2.0 * vxx - vzz - vdd) / (2.0 * tp2);
+            } else {
+                const double vzd = gamma_comp(y, d, a, ZD, site, bs2, bt2, bu2, ALL);
+                const double vdz = gamma_comp(y, d, a, DZ, site, bs2, bt2, bu2, ALL);
+                val -= (vzd - vdz) / (2.0 * tp2);
+            }
+        }
     }
     val *= G_full(y, L, v - 0.5 * w, a.Sigma) * G_full(y, L, v + 0.5 * w, a.Sigma) * dv;
     val *= G_full(y, L, vp - 0.5 * w, a.Sigma) * G_full(y, L, vp + 0.5 * w, a.Sigma) * dvp;
@@ -1045,11 +1193,23 @@ inline SigmaBufs sigma_bufs(const orc_system &y, double w, double v) {
     return b;
 }
 
-// loop value 3Γs + Γd (local) - Σ_j 2 m_j (2S) Γd_j (non-local), flow_su2_self.jl:22-27
-inline double sigma_loop(const orc_system &y, const Dims &d, const orc_action &a, const SigmaBufs &b, Mask ml, Mask mn) {
-    double val = 3.0 * gamma_comp(y, d, a, 0, 0, b.b2s, b.b2t, b.b2u, ml) + gamma_comp(y, d, a, 1, 0, b.b2s, b.b2t, b.b2u, ml);
-    for (int j = 0; j < d.ns; ++j) val -= 2.0 * y.mult[j] * (2.0 * y.S) * gamma_comp(y, d, a, 1, j, b.b1s, b.b1t, b.b1u, mn);
+// loop value of the self-energy kernels from the local (b2) and non-local (b1) buffers:
+// SU(2) 3Γs + Γd - Σ_j 2 m_j (2S) Γd_j (flow_su2_self.jl:22-27); U(1)-DM 2Γxx + Γzz + Γdd -
+// Σ_j 2 m_j Γdd_j (flow_u1_dm_self.jl:22-28)
+inline double sigma_loop2(const orc_system &y, const Dims &d, const orc_action &a, const Buffer &b2s, const Buffer &b2t,
+                          const Buffer &b2u, const Buffer &b1s, const Buffer &b1t, const Buffer &b1u, Mask ml, Mask mn) {
+    if (d.nc == 2) {
+        double val = 3.0 * gamma_comp(y, d, a, 0, 0, b2s, b2t, b2u, ml) + gamma_comp(y, d, a, 1, 0, b2s, b2t, b2u, ml);
+        for (int j = 0; j < d.ns; ++j) val -= 2.0 * y.mult[j] * (2.0 * y.S) * gamma_comp(y, d, a, 1, j, b1s, b1t, b1u, mn);
+        return val;
+    }
+    double val = 2.0 * gamma_comp(y, d, a, XX, 0, b2s, b2t, b2u, ml) + gamma_comp(y, d, a, ZZ, 0, b2s, b2t, b2u, ml) +
+                 gamma_comp(y, d, a, DD, 0, b2s, b2t, b2u, ml);
+    for (int j = 0; j < d.ns; ++j) val -= 2.0 * y.mult[j] * gamma_comp(y, d, a, DD, j, b1s, b1t, b1u, mn);
     return val;
+}
+inline double sigma_loop(const orc_system &y, const Dims &d, const orc_action &a, const SigmaBufs &b, Mask ml, Mask mn) {
+    return sigma_loop2(y, d, a, b.b2s, b.b2t, b.b2u, b.b1s, b.b1t, b.b1u, ml, mn);
 }
 
 const double SIGMA_INNER_BP_SCALE[3] = {-2.0, 0.0, 2.0};
@@ -1081,11 +1241,12 @@ void orc_get_param(double val, const double *list, int n, int32_t *lo, int32_t *
     *lo = p.lo; *hi = p.hi; *wl = p.wl; *wh = p.wh;
 }
 
-// get_buffer_{s,t,u} (buffer.jl:88-199): out_i = [exch, map, sgn_mu, sgn_nu, kernel, lo1, hi1, lo2, hi2, lo3, hi3]
+// get_buffer_{s,t,u} (buffer.jl:88-199): out_i = [exch, map, sgn_mu, sgn_nu, kernel, lo1, hi1, lo2, hi2, lo3, hi3, sgn_munu]
 void orc_get_buffer(const orc_system *y, int ch, double w, double v, double vp, int32_t *out_i, double *out_w) {
     Buffer b = (ch == 0) ? buffer_s(*y, w, v, vp) : (ch == 1) ? buffer_t(*y, w, v, vp) : buffer_u(*y, w, v, vp);
     out_i[0] = b.exch; out_i[1] = b.map; out_i[2] = b.sgn_mu; out_i[3] = b.sgn_nu; out_i[4] = b.kernel;
     out_i[5] = b.p1.lo; out_i[6] = b.p1.hi; out_i[7] = b.p2.lo; out_i[8] = b.p2.hi; out_i[9] = b.p3.lo; out_i[10] = b.p3.hi;
+    out_i[11] = b.sgn_munu;
     out_w[0] = b.p1.wl; out_w[1] = b.p1.wh; out_w[2] = b.p2.wl; out_w[3] = b.p2.wh; out_w[4] = b.p3.wl; out_w[5] = b.p3.wh;
 }
 
@@ -1150,7 +1311,8 @@ long orc_compute_element(const orc_system *y, int pass, double L, const orc_acti
                          const double *corrs, int ch, int kernel, int w1, int w2, int w3, int eval, const double *tol,
                          orc_action *out) {
     PassCtx C{y, dims_of(*y), L, pass, a, da, corrs};
-    std::vector<double> t1(2 * y->n_sites), t2(2 * y->n_sites), t3(2 * y->n_sites);
+    const int n = y->n_comp * y->n_sites;
+    std::vector<double> t1(n), t2(n), t3(n);
     return compute_element(C, ch, kernel, w1, w2, w3, *out, t1.data(), t2.data(), t3.data(), eval, tol[0], tol[1]);
 }
 
@@ -1176,14 +1338,12 @@ void orc_integrand(const orc_system *y, int pass, double L, const orc_action *a,
     } else {
         kernel_args(*y, ch, kernel, w1, w2, w3, W, vw, vwp);
     }
-    const int n = 2 * y->n_sites;
+    const int nc = y->n_comp, n = nc * y->n_sites;
     Integrand f{&C, ch, W, vw, vwp, std::vector<double>(n), std::vector<double>(n)};
     std::vector<double> b(n, 0.0);
     f(b.data(), v, 1.0);
-    for (int i = 0; i < y->n_sites; ++i) {
-        out[i] = b[2 * i];
-        out[y->n_sites + i] = b[2 * i + 1];
-    }
+    for (int i = 0; i < y->n_sites; ++i)
+        for (int k = 0; k < nc; ++k) out[k * y->n_sites + i] = b[nc * i + k];
 }
 
 // a whole pass without symmetrisation: pass in {KAT, LEFT, CENTRAL, BSE, REDUCED}; full = all (w2,w3)
@@ -1296,7 +1456,7 @@ long orc_compute_dgamma_ml(const orc_system *y, double L, int loops, const orc_a
     return n;
 }
 
-// compute_Σ!, Flow/SDE.jl:58-96 with compute_Σ_kernel (SDE_su2.jl:45-77)
+// compute_Σ!, Flow/SDE.jl:58-96 with compute_Σ_kernel (SDE_su2.jl:45-77, SDE_u1_dm.jl:45-91)
 long orc_compute_sigma_sde(const orc_system *y, double L, const orc_action *a1, orc_action *a2, int eval,
                            const double *gtol, const double *stol, double *corrs) {
     Dims d = dims_of(*y);
@@ -1311,8 +1471,7 @@ long orc_compute_sigma_sde(const orc_system *y, double L, const orc_action *a1, 
             Buffer b1s = buffer_s(*y, v + w, INFINITY, 0.5 * (v - w));
             Buffer b2s = buffer_s(*y, v + w, INFINITY, 0.5 * (-v + w));
             Buffer e = buffer_empty();
-            double val = 3.0 * gamma_comp(*y, d, *a2, 0, 0, b2s, e, e, SONLY) + gamma_comp(*y, d, *a2, 1, 0, b2s, e, e, SONLY);
-            for (int j = 0; j < d.ns; ++j) val -= 2.0 * y->mult[j] * (2.0 * y->S) * gamma_comp(*y, d, *a2, 1, j, b1s, e, e, SONLY);
+            double val = sigma_loop2(*y, d, *a2, b2s, e, e, b1s, e, e, SONLY, SONLY);
             val *= -G_full(*y, L, v, a1->Sigma) / (2.0 * PI);
             return val;
         };
@@ -1332,10 +1491,14 @@ long orc_compute_gamma_bse(const orc_system *y, double L, const orc_action *a1, 
 // compute_χ! (correlation_su2.jl:58-86): chi[i + ns*w] for the irreducible sites and the χ mesh
 void orc_compute_chi(const orc_system *y, double L, const orc_action *a, const double *chi_mesh, int nchi,
                      const double *tol, double *chi, int32_t *nreg) {
+    // U(1)-DM (compute_χ!, correlation_u1_dm.jl:138-190): χxx, χzz, χxy, each its own cubature;
+    // chi[c*ns*nchi + i + ns*w], nreg[9*(c*ns*nchi + t) + k]
     Dims d = dims_of(*y);
     const int ns = d.ns;
+    const int ncc = (d.nc == 2) ? 1 : 3;
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int t = 0; t < nchi * ns; ++t) {
+    for (int tc = 0; tc < ncc * nchi * ns; ++tc) {
+        const int c = tc / (nchi * ns), t = tc % (nchi * ns);
         const int w = t / ns, i = t % ns;
         const double W = chi_mesh[w];
         const double ref = L + 0.5 * W;
@@ -1344,19 +1507,21 @@ void orc_compute_chi(const orc_system *y, double L, const orc_action *a, const d
             double lo[2], hi[2];
             chi_box(4.0 * ref, k, lo, hi);
             int nr = 0;
-            I += hcubature2([&](double xx, double yy) { return chi_kernel(*y, d, L, i, W, *a, xx, yy); }, lo, hi,
+            I += hcubature2([&](double xx, double yy) { return chi_kernel(*y, d, L, i, W, *a, xx, yy, c); }, lo, hi,
                             tol[0], tol[1], &nr);
-            if (nreg) nreg[9 * t + k] = nr;
+            if (nreg) nreg[9 * (long)tc + k] = nr;
         }
-        if (i == 0) {
+        if (i == 0 && c < 2) {
             const double inner[3] = {-2.0 * ref, 0.0, 2.0 * ref};
+            const double S2 = (d.nc == 2) ? 2.0 * y->S : 1.0;
             auto f = [&](double v) {
-                return (2.0 * y->S) * G_full(*y, L, v - 0.5 * W, a->Sigma) * G_full(*y, L, v + 0.5 * W, a->Sigma) /
-                       (4.0 * PI);
+                if (d.nc == 2)
+                    return S2 * G_full(*y, L, v - 0.5 * W, a->Sigma) * G_full(*y, L, v + 0.5 * W, a->Sigma) / (4.0 * PI);
+                return G_full(*y, L, v - 0.5 * W, a->Sigma) * G_full(*y, L, v + 0.5 * W, a->Sigma) / (4.0 * PI);
             };
             I += quadgk_inf_inf(f, inner, 3, tol[0], tol[1]);
         }
-        chi[i + ns * w] = I;
+        chi[(long)c * ns * nchi + i + ns * w] = I;
     }
 }
 
@@ -1452,12 +1617,12 @@ int orc_scan(const double *x, const double *y, int n, const double *prm, double 
     return 0;
 }
 
-// scan_channel's argmax and cuts, Action/Action.jl:449-476: k = 3*comp + channel
+// scan_channel's argmax and cuts, Action/Action.jl:449-476: k = 3*comp + channel, k < 3*n_comp
 void orc_scan_cuts(const orc_system *y, const orc_action *a, int32_t *idx, double *cuts) {
     const Dims d = dims_of(*y);
     const int ns = d.ns, nO = d.nO, nn = d.nn;
     const long n3 = (long)ns * nO * nn * nn;
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < 3 * d.nc; ++k) {
         const double *q3 = chan(d, a->gamma, k / 3, k % 3) + d.q3;
         long best = 0;
         for (long j = 1; j < n3; ++j)
@@ -1532,12 +1697,12 @@ void orc_resample(const orc_system *y_old, const double *sigma, const double *Os
                   const double *Ot, const double *nt, const orc_action *a_old, orc_action *a_new) {
     const Dims d = dims_of(*y_old);
     for (int w = 0; w < y_old->n_sigma; ++w) a_new->Sigma[w] = get_sigma(*y_old, a_old->Sigma, sigma[w]);
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < d.nc; ++c) {
         resample_channel(d, y_old->Os, y_old->ns, chan(d, a_old->gamma, c, 0), Os, ns_, chan(d, a_new->gamma, c, 0));
         resample_channel(d, y_old->Ot, y_old->nt, chan(d, a_old->gamma, c, 1), Ot, nt, chan(d, a_new->gamma, c, 1));
         resample_channel(d, y_old->Os, y_old->ns, chan(d, a_old->gamma, c, 2), Os, ns_, chan(d, a_new->gamma, c, 2));
     }
-    for (int k = 0; k < 2 * d.ns; ++k) a_new->bare[k] = a_old->bare[k];
+    for (int k = 0; k < d.nc * d.ns; ++k) a_new->bare[k] = a_old->bare[k];
 }
 
 // the shared written-out exp / expm1 (pffrg_math.h), for their accuracy test

@@ -56,7 +56,8 @@ __device__ __forceinline__ void gm_combine(Reg &R, const double *f) {
     R.split = (d1 > d0 || (d1 == d0 && R.h1 > R.h0)) ? 1 : 0;
 }
 
-// compute_χ_kernel! at (x, y) of the unit square
+// compute_χ_kernel! at (x, y) of the unit square: SU(2) (correlation_su2.jl:14-55) or the
+// U(1)-DM kernel of component Q.comp (compute_χ_kernel_{xx,zz,xy}!, correlation_u1_dm.jl:16-134)
 __device__ double chi_point(const ChiProb &Q, const Meshes<> &M, const double *sA, const int *exch, int site,
                             double w, double xx, double yy) {
     const Sys &y = Q.y;
@@ -70,16 +71,32 @@ __device__ double chi_point(const ChiProb &Q, const Meshes<> &M, const double *s
     const Buf<> bu1 = buf_u(M, y, v - vp, 0.5 * (v - w + vp), 0.5 * (v + w + vp));
     const double S2 = y.S2;
     const double tp2 = (2.0 * PI_D) * (2.0 * PI_D);
-    double vs, vd;
-    gamma_site<double, true, true, true>(Q.A, bs1, bt1, bu1, site, exch, y, vs, vd);
-    double val = -(S2 * S2) * vs / tp2;
-    if (site == 0) {
-        const Buf<> bs2 = buf_s(M, y, v + vp, 0.5 * (v - w - vp), 0.5 * (v + w - vp));
-        const Buf<> bt2 = buf_t(M, y, -v + vp, 0.5 * (v - w + vp), 0.5 * (v + w + vp));
-        const Buf<> bu2 = buf_u(M, y, -w, v, vp);
-        double s2, d2;
-        gamma_site<double, true, true, true>(Q.A, bs2, bt2, bu2, site, exch, y, s2, d2);
-        val -= S2 * (s2 - d2) / (2.0 * tp2);
+    double val;
+    if (y.nc == 6) {
+        double g6[6];
+        gamma_site_u1<true, true, true>(Q.A, bs1, bt1, bu1, site, exch, y, g6);
+        val = -g6[Q.comp == 2 ? U1_DM : Q.comp] / tp2;
+        if (site == 0) {
+            const Buf<> bs2 = buf_s(M, y, v + vp, 0.5 * (v - w - vp), 0.5 * (v + w - vp));
+            const Buf<> bt2 = buf_t(M, y, -v + vp, 0.5 * (v - w + vp), 0.5 * (v + w + vp));
+            const Buf<> bu2 = buf_u(M, y, -w, v, vp);
+            gamma_site_u1<true, true, true>(Q.A, bs2, bt2, bu2, site, exch, y, g6);
+            if (Q.comp == 0) val -= (g6[U1_ZZ] - g6[U1_DD]) / (2.0 * tp2);
+            else if (Q.comp == 1) val -= (2.0 * g6[U1_XX] - g6[U1_ZZ] - g6[U1_DD]) / (2.0 * tp2);
+            else val -= (g6[U1_ZD] - g6[U1_DZ]) / (2.0 * tp2);
+        }
+    } else {
+        double vs, vd;
+        gamma_site<double, true, true, true>(Q.A, bs1, bt1, bu1, site, exch, y, vs, vd);
+        val = -(S2 * S2) * vs / tp2;
+        if (site == 0) {
+            const Buf<> bs2 = buf_s(M, y, v + vp, 0.5 * (v - w - vp), 0.5 * (v + w - vp));
+            const Buf<> bt2 = buf_t(M, y, -v + vp, 0.5 * (v - w + vp), 0.5 * (v + w + vp));
+            const Buf<> bu2 = buf_u(M, y, -w, v, vp);
+            double s2, d2;
+            gamma_site<double, true, true, true>(Q.A, bs2, bt2, bu2, site, exch, y, s2, d2);
+            val -= S2 * (s2 - d2) / (2.0 * tp2);
+        }
     }
     val *= G_full<double>(L, v - 0.5 * w, M.sig, y.nsig, sA) * G_full<double>(L, v + 0.5 * w, M.sig, y.nsig, sA) * dv;
     val *= G_full<double>(L, vp - 0.5 * w, M.sig, y.nsig, sA) * G_full<double>(L, vp + 0.5 * w, M.sig, y.nsig, sA) *
@@ -402,6 +419,9 @@ __global__ __launch_bounds__(64) void k_chi_prop(ChiArgs Q) {
     const double L = P.L;
     const double ref = L + 0.5 * W;
     auto f = [&](double v) -> double {
+        if (y.nc == 6)  // correlation_u1_dm.jl:178-183: no spin-length factor
+            return G_full<double>(L, v - 0.5 * W, s_mesh, y.nsig, s_sA) *
+                   G_full<double>(L, v + 0.5 * W, s_mesh, y.nsig, s_sA) / (4.0 * PI_D);
         return y.S2 * G_full<double>(L, v - 0.5 * W, s_mesh, y.nsig, s_sA) *
                G_full<double>(L, v + 0.5 * W, s_mesh, y.nsig, s_sA) / (4.0 * PI_D);
     };

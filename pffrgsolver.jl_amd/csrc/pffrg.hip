@@ -100,7 +100,8 @@ struct SigmaArgs {
 
 // compute_dΣ! / compute_dΣ_corr! / compute_Σ! (Flow/flow_self.jl, Flow/SDE.jl:168-174):
 // one workgroup per σ point w_i, i >= 1; the integrand restates compute_dΣ_kernel(_corr1,
-// _corr2) (flow_su2_self.jl) and compute_Σ_kernel (SDE_su2.jl:45-77).
+// _corr2) (flow_su2_self.jl, flow_u1_dm_self.jl) and compute_Σ_kernel (SDE_su2.jl:45-77,
+// SDE_u1_dm.jl:45-91).
 template <int KIND>
 __global__ __launch_bounds__(64) void k_sigma(SigmaArgs Q) {
     __shared__ double s_mesh[5 * MAXN];
@@ -129,6 +130,48 @@ __global__ __launch_bounds__(64) void k_sigma(SigmaArgs Q) {
     const double L = Q.L;
     auto f = [&](double v) -> double {
         double val;
+        if (y.nc == 6) {
+            // U(1)-DM: 2Γxx + Γzz + Γdd (local) - Σ_j 2 m_j Γdd_j (flow_u1_dm_self.jl:22-28,
+            // SDE_u1_dm.jl:68-76)
+            double l6[6], n6[6];
+            if (KIND == SK_SDE) {
+                const double INF = __builtin_huge_val();
+                const Buf<> b1s = buf_s(M, y, v + w, INF, 0.5 * (v - w));
+                const Buf<> b2s = buf_s(M, y, v + w, INF, 0.5 * (-v + w));
+                const Buf<> e = buf_empty<double>();
+                gamma_site_u1<true, false, false>(Q.V, b2s, e, e, 0, s_exch, y, l6);
+                val = 2.0 * l6[U1_XX] + l6[U1_ZZ] + l6[U1_DD];
+                for (int j = 0; j < y.ns; ++j) {
+                    gamma_site_u1<true, false, false>(Q.V, b1s, e, e, j, s_exch, y, n6);
+                    val -= 2.0 * s_mult[j] * n6[U1_DD];
+                }
+                val *= -G_full(L, v, s_mesh, y.nsig, s_sA) / (2.0 * PI_D);
+                return val;
+            }
+            const Buf<> b1s = buf_s(M, y, v + w, 0.5 * (-v + w), 0.5 * (v - w));
+            const Buf<> b1t = buf_t(M, y, 0.0, w, v);
+            const Buf<> b1u = buf_u(M, y, -v + w, 0.5 * (v + w), 0.5 * (v + w));
+            const Buf<> b2s = buf_s(M, y, v + w, 0.5 * (-v + w), 0.5 * (-v + w));
+            const Buf<> b2t = buf_t(M, y, v - w, 0.5 * (v + w), 0.5 * (v + w));
+            const Buf<> b2u = buf_u(M, y, 0.0, w, v);
+            if (KIND == SK_CORR1) gamma_site_u1<true, true, false>(Q.V, b2s, b2t, b2u, 0, s_exch, y, l6);
+            else gamma_site_u1<true, true, true>(Q.A, b2s, b2t, b2u, 0, s_exch, y, l6);
+            val = 2.0 * l6[U1_XX] + l6[U1_ZZ] + l6[U1_DD];
+            for (int j = 0; j < y.ns; ++j) {
+                if (KIND == SK_CORR1) gamma_site_u1<true, false, true>(Q.V, b1s, b1t, b1u, j, s_exch, y, n6);
+                else gamma_site_u1<true, true, true>(Q.A, b1s, b1t, b1u, j, s_exch, y, n6);
+                val -= 2.0 * s_mult[j] * n6[U1_DD];
+            }
+            if (KIND == SK_DSIGMA) {
+                val *= S_single(L, v, s_mesh, y.nsig, s_sA) / (2.0 * PI_D);
+            } else if (KIND == SK_CORR1) {
+                val *= -G_full(L, v, s_mesh, y.nsig, s_sA) / (2.0 * PI_D);
+            } else {
+                const double G = G_full(L, v, s_mesh, y.nsig, s_sA);
+                val *= G * G * sigma_at(v, s_mesh, y.nsig, s_sV) / (2.0 * PI_D);
+            }
+            return val;
+        }
         if (KIND == SK_SDE) {
             const double INF = __builtin_huge_val();
             const Buf<> b1s = buf_s(M, y, v + w, INF, 0.5 * (v - w));
@@ -191,10 +234,25 @@ __global__ __launch_bounds__(64) void k_sigma(SigmaArgs Q) {
 }
 
 // ======================================================================= elementwise action kernels
-// symmetrize! (action_su2.jl:141-173): q3[i,w,v,v'] = q3[x(i),w,v',v] for v < v'
+// The ν <-> ν' image of component comp in channel ch reads component sym_src with sign
+// sym_sign: the identity for SU(2); for U(1)-DM DM flips and zd, dz swap with a sign in the s
+// and t channels, zd and dz flip in the u channel (action_u1_dm.jl:170-339)
+__device__ __forceinline__ int sym_src(int nc, int ch, int comp) {
+    if (nc == 6 && ch != 2 && comp == U1_ZD) return U1_DZ;
+    if (nc == 6 && ch != 2 && comp == U1_DZ) return U1_ZD;
+    return comp;
+}
+__device__ __forceinline__ bool sym_neg(int nc, int ch, int comp) {
+    if (nc != 6) return false;
+    if (ch != 2) return comp == U1_DM || comp == U1_ZD || comp == U1_DZ;
+    return comp == U1_ZD || comp == U1_DZ;
+}
+
+// symmetrize! (action_su2.jl:141-173, action_u1_dm.jl:170-215): q3[i,w,v,v'] = ±q3'[x(i),w,v',v]
+// for v < v' (the upper triangle is written from lower-triangle entries only)
 __global__ void k_symmetrize(double *g, Sys y, const int *exch) {
     const long per = (long)y.ns * y.nO * y.nn * y.nn;
-    const long total = 6 * per;
+    const long total = 3L * y.nc * per;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
         const int cc = (int)(t / per);  // comp*3 + ch
         long r = t % per;
@@ -203,22 +261,24 @@ __global__ void k_symmetrize(double *g, Sys y, const int *exch) {
         const int v = (int)(r % y.nn);
         const int vp = (int)(r / y.nn);
         if (v >= vp) continue;
-        const int ch = cc % 3;
-        double *q3 = g + (cc / 3) * y.complen + ch * y.chlen + y.q3;
+        const int ch = cc % 3, comp = cc / 3;
+        double *q3 = g + comp * y.complen + ch * y.chlen + y.q3;
+        const double *q3s = g + sym_src(y.nc, ch, comp) * y.complen + ch * y.chlen + y.q3;
         const int src = (ch == 2) ? i : exch[i];
-        q3[i + (long)y.ns * (w + (long)y.nO * (v + (long)y.nn * vp))] =
-            q3[src + (long)y.ns * (w + (long)y.nO * (vp + (long)y.nn * v))];
+        const double x = q3s[src + (long)y.ns * (w + (long)y.nO * (vp + (long)y.nn * v))];
+        q3[i + (long)y.ns * (w + (long)y.nO * (v + (long)y.nn * vp))] = sym_neg(y.nc, ch, comp) ? -x : x;
     }
 }
 
-// symmetrize_add_to! (action_su2.jl:176-250): a += a_l + image of a_l under ν <-> ν'
+// symmetrize_add_to! (action_su2.jl:176-250, action_u1_dm.jl:218-339): a += a_l ± image of
+// a_l under ν <-> ν'
 __global__ void k_symmetrize_add(const double *__restrict__ l, double *__restrict__ a, Sys y, const int *exch) {
-    const long total = 2 * y.complen;
+    const long total = y.nc * y.complen;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
         const long cc = t / y.chlen;
-        const int ch = (int)(cc % 3);
+        const int ch = (int)(cc % 3), comp = (int)(cc / 3);
         long r = t % y.chlen;
-        const long cbase = t - r;
+        const long cbase = (long)sym_src(y.nc, ch, comp) * y.complen + (long)ch * y.chlen;
         const int i = (int)(r % y.ns);
         const int ix = (ch == 2) ? i : exch[i];
         double img;
@@ -238,7 +298,7 @@ __global__ void k_symmetrize_add(const double *__restrict__ l, double *__restric
             const int vp = (int)(k / y.nn);
             img = l[cbase + y.q3 + ix + (long)y.ns * (w + (long)y.nO * (vp + (long)y.nn * v))];
         }
-        a[t] += l[t] + img;
+        a[t] += sym_neg(y.nc, ch, comp) ? l[t] - img : l[t] + img;
     }
 }
 
@@ -251,22 +311,27 @@ __global__ void k_scale(double *x, double fac, long n) {
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) x[t] *= fac;
 }
 
-// gather copy: y[2t + comp] = x[comp * complen + t]
+// gather copy: y[nc t + comp] = x[comp * complen + t]
 template <typename T>
-__global__ void k_interleave(const double *__restrict__ x, T *__restrict__ y, long complen) {
+__global__ void k_interleave(const double *__restrict__ x, T *__restrict__ y, long complen, int nc) {
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < complen; t += (long)gridDim.x * blockDim.x) {
-        y[2 * t] = (T)x[t];
-        y[2 * t + 1] = (T)x[complen + t];
+        if (nc == 2) {
+            y[2 * t] = (T)x[t];
+            y[2 * t + 1] = (T)x[complen + t];
+        } else {
+            for (int k = 0; k < nc; ++k) y[(long)nc * t + k] = (T)x[k * complen + t];
+        }
     }
 }
 
-// max |q3| over the six q3 blocks (channel.jl:406-413): block max then 64-bit atomic max
-// on the IEEE bit pattern, which orders like the value for non-negative doubles.
+// max |q3| over the q3 blocks of every component and channel (channel.jl:406-413): block max
+// then 64-bit atomic max on the IEEE bit pattern, which orders like the value for non-negative
+// doubles.
 __global__ void k_absmax_q3(const double *g, Sys y, unsigned long long *out) {
     __shared__ double red[256];
     const long per = (long)y.ns * y.nO * y.nn * y.nn;
     double m = 0.0;
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < 6 * per; t += (long)gridDim.x * blockDim.x) {
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < 3L * y.nc * per; t += (long)gridDim.x * blockDim.x) {
         const long cc = t / per;
         const double v = fabs(g[(cc / 3) * y.complen + (cc % 3) * y.chlen + y.q3 + (t % per)]);
         m = (v > m || v != v) ? v : m;
@@ -435,17 +500,20 @@ int grid_for(long n) {
 int ensure_gather(pffrg_ctx *c, int s, bool f32) {
     Slot &sl = c->slots[s];
     const long n = c->sys.complen;
+    const int nc = c->sys.nc;
     if (f32) {
-        if (!sl.gat32) HIPCHK(c, hipMalloc(&sl.gat32, 2 * n * sizeof(float)));
+        if (!sl.gat32) HIPCHK(c, hipMalloc(&sl.gat32, nc * n * sizeof(float)));
         if (!sl.gat32_valid) {
-            hipLaunchKernelGGL(k_interleave<float>, dim3(grid_for(n)), dim3(256), 0, c->stream, sl.gamma, sl.gat32, n);
+            hipLaunchKernelGGL(k_interleave<float>, dim3(grid_for(n)), dim3(256), 0, c->stream, sl.gamma, sl.gat32, n,
+                               nc);
             HIPCHK(c, hipGetLastError());
             sl.gat32_valid = true;
         }
     } else {
-        if (!sl.gat64) HIPCHK(c, hipMalloc(&sl.gat64, 2 * n * sizeof(double)));
+        if (!sl.gat64) HIPCHK(c, hipMalloc(&sl.gat64, nc * n * sizeof(double)));
         if (!sl.gat64_valid) {
-            hipLaunchKernelGGL(k_interleave<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, sl.gamma, sl.gat64, n);
+            hipLaunchKernelGGL(k_interleave<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, sl.gamma, sl.gat64, n,
+                               nc);
             HIPCHK(c, hipGetLastError());
             sl.gat64_valid = true;
         }
@@ -570,7 +638,9 @@ int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const
 // with the descriptors and interpolations).  Every problem has the same block, so a batched
 // launch runs each problem exactly as its own launch does.  Problems with more sites use the
 // per-element pass.
-bool shared_fits(const pffrg_ctx *c) { return c->sys.ns <= 8 * SH_SPT; }
+// the shared-node pass serves the SU(2) action up to 8 SH_SPT sites; U(1)-DM runs the
+// per-element pass in FP32 in that mode
+bool shared_fits(const pffrg_ctx *c) { return c->sys.nc == 2 && c->sys.ns <= 8 * SH_SPT; }
 
 void shared_shape(pffrg_ctx *c) {
     c->sh_nw = 8;
@@ -717,7 +787,7 @@ int launch_sigma(pffrg_ctx *c, double L, int a, int v, double *out, const double
 }
 
 int do_symmetrize(pffrg_ctx *c, int s) {
-    const long n = 6L * c->sys.ns * c->sys.nO * c->sys.nn * c->sys.nn;
+    const long n = 3L * c->sys.nc * c->sys.ns * c->sys.nO * c->sys.nn * c->sys.nn;
     hipLaunchKernelGGL(k_symmetrize, dim3(grid_for(n)), dim3(256), 0, c->stream, c->slots[s].gamma, c->sys, c->d_exch);
     HIPCHK(c, hipGetLastError());
     invalidate(c, s);
@@ -725,7 +795,7 @@ int do_symmetrize(pffrg_ctx *c, int s) {
 }
 
 int do_symmetrize_add(pffrg_ctx *c, int l, int a) {
-    const long n = 2 * c->sys.complen;
+    const long n = c->sys.nc * c->sys.complen;
     hipLaunchKernelGGL(k_symmetrize_add, dim3(grid_for(n)), dim3(256), 0, c->stream, c->slots[l].gamma,
                        c->slots[a].gamma, c->sys, c->d_exch);
     HIPCHK(c, hipGetLastError());
@@ -734,7 +804,7 @@ int do_symmetrize_add(pffrg_ctx *c, int l, int a) {
 }
 
 int do_axpy(pffrg_ctx *c, int src, double fac, int dst, bool vertex_only) {
-    const long n = 2 * c->sys.complen;
+    const long n = c->sys.nc * c->sys.complen;
     hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(256), 0, c->stream, c->slots[src].gamma, fac, c->slots[dst].gamma, n);
     HIPCHK(c, hipGetLastError());
     if (!vertex_only) {
@@ -747,7 +817,7 @@ int do_axpy(pffrg_ctx *c, int src, double fac, int dst, bool vertex_only) {
 }
 
 int do_scale(pffrg_ctx *c, int s, double fac, bool vertex_only) {
-    const long n = 2 * c->sys.complen;
+    const long n = c->sys.nc * c->sys.complen;
     hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(256), 0, c->stream, c->slots[s].gamma, fac, n);
     HIPCHK(c, hipGetLastError());
     if (!vertex_only) {
@@ -840,7 +910,7 @@ int ensure_scan_mem(pffrg_ctx *c) {
 }
 
 int do_copy(pffrg_ctx *c, int dst, int src, bool vertex_only) {
-    HIPCHK(c, hipMemcpyAsync(c->slots[dst].gamma, c->slots[src].gamma, 2 * c->sys.complen * sizeof(double),
+    HIPCHK(c, hipMemcpyAsync(c->slots[dst].gamma, c->slots[src].gamma, c->sys.nc * c->sys.complen * sizeof(double),
                              hipMemcpyDeviceToDevice, c->stream));
     if (!vertex_only)
         HIPCHK(c, hipMemcpyAsync(c->slots[dst].Sigma, c->slots[src].Sigma, c->sys.nsig * sizeof(double),
@@ -854,9 +924,12 @@ int do_copy(pffrg_ctx *c, int dst, int src, bool vertex_only) {
 // ======================================================================= C ABI
 extern "C" {
 
-int pffrg_create(const pffrg_dims *d, pffrg_ctx **out) {
+int pffrg_create(const pffrg_dims *d, pffrg_ctx **out) { return pffrg_create_symmetry(d, PFFRG_SU2, out); }
+
+int pffrg_create_symmetry(const pffrg_dims *d, int32_t symmetry, pffrg_ctx **out) {
     if (!d || !out) return PFFRG_ERR_ARG;
     *out = nullptr;
+    if (symmetry != PFFRG_SU2 && symmetry != PFFRG_U1_DM) return PFFRG_ERR_ARG;
     if (d->n_sites < 1 || d->n_sites > 64 || d->n_sigma < 2 || d->n_sigma > MAXN || d->n_Omega < 2 ||
         d->n_Omega > MAXN || d->n_nu < 2 || d->n_nu > MAXN || d->n_slots < 1 || d->S <= 0.0)
         return PFFRG_ERR_LIMIT;
@@ -873,6 +946,7 @@ int pffrg_create(const pffrg_dims *d, pffrg_ctx **out) {
     y.G = (y.ns <= 32) ? WAVE / y.ns : 1;
     if (y.G > MAX_GROUPS) y.G = MAX_GROUPS;
     y.S2 = 2.0 * d->S;
+    y.nc = (symmetry == PFFRG_U1_DM) ? 6 : 2;
     y.q21 = (long)y.ns * y.nO;
     y.q22 = y.q21 + (long)y.ns * y.nO * y.nn;
     y.q3 = y.q22 + (long)y.ns * y.nO * y.nn;
@@ -892,11 +966,11 @@ int pffrg_create(const pffrg_dims *d, pffrg_ctx **out) {
     c->slots.resize(d->n_slots);
     for (auto &s : c->slots) {
         ok = ok && hipMalloc(&s.Sigma, y.nsig * sizeof(double)) == hipSuccess;
-        ok = ok && hipMalloc(&s.bare, 2 * y.ns * sizeof(double)) == hipSuccess;
-        ok = ok && hipMalloc(&s.gamma, 2 * y.complen * sizeof(double)) == hipSuccess;
+        ok = ok && hipMalloc(&s.bare, y.nc * y.ns * sizeof(double)) == hipSuccess;
+        ok = ok && hipMalloc(&s.gamma, y.nc * y.complen * sizeof(double)) == hipSuccess;
         ok = ok && hipMemset(s.Sigma, 0, y.nsig * sizeof(double)) == hipSuccess;
-        ok = ok && hipMemset(s.bare, 0, 2 * y.ns * sizeof(double)) == hipSuccess;
-        ok = ok && hipMemset(s.gamma, 0, 2 * y.complen * sizeof(double)) == hipSuccess;
+        ok = ok && hipMemset(s.bare, 0, y.nc * y.ns * sizeof(double)) == hipSuccess;
+        ok = ok && hipMemset(s.gamma, 0, y.nc * y.complen * sizeof(double)) == hipSuccess;
     }
     auto upload_list = [&](uint32_t **dst, int *n, int mode) {
         std::vector<uint32_t> e = element_list(y.nO, y.nn, mode);
@@ -952,7 +1026,9 @@ void pffrg_destroy(pffrg_ctx *c) {
 
 const char *pffrg_last_error(const pffrg_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
-int64_t pffrg_gamma_length(const pffrg_ctx *c) { return c ? 2 * c->sys.complen : -1; }
+int32_t pffrg_n_components(const pffrg_ctx *c) { return c ? c->sys.nc : -1; }
+
+int64_t pffrg_gamma_length(const pffrg_ctx *c) { return c ? c->sys.nc * c->sys.complen : -1; }
 
 int pffrg_set_mesh(pffrg_ctx *c, const double *sigma, const double *Os, const double *ns, const double *Ot,
                    const double *nt) {
@@ -1036,8 +1112,8 @@ int pffrg_upload_action(pffrg_ctx *c, int32_t s, const double *Sigma, const doub
     HIPCHK(c, hipStreamSynchronize(c->stream));
     Slot &sl = c->slots[s];
     if (Sigma) HIPCHK(c, hipMemcpy(sl.Sigma, Sigma, c->sys.nsig * sizeof(double), hipMemcpyHostToDevice));
-    if (bare) HIPCHK(c, hipMemcpy(sl.bare, bare, 2 * c->sys.ns * sizeof(double), hipMemcpyHostToDevice));
-    if (gamma) HIPCHK(c, hipMemcpy(sl.gamma, gamma, 2 * c->sys.complen * sizeof(double), hipMemcpyHostToDevice));
+    if (bare) HIPCHK(c, hipMemcpy(sl.bare, bare, c->sys.nc * c->sys.ns * sizeof(double), hipMemcpyHostToDevice));
+    if (gamma) HIPCHK(c, hipMemcpy(sl.gamma, gamma, c->sys.nc * c->sys.complen * sizeof(double), hipMemcpyHostToDevice));
     invalidate(c, s);
     return 0;
 }
@@ -1049,8 +1125,8 @@ int pffrg_download_action(pffrg_ctx *c, int32_t s, double *Sigma, double *bare, 
     HIPCHK(c, hipStreamSynchronize(c->stream));
     Slot &sl = c->slots[s];
     if (Sigma) HIPCHK(c, hipMemcpy(Sigma, sl.Sigma, c->sys.nsig * sizeof(double), hipMemcpyDeviceToHost));
-    if (bare) HIPCHK(c, hipMemcpy(bare, sl.bare, 2 * c->sys.ns * sizeof(double), hipMemcpyDeviceToHost));
-    if (gamma) HIPCHK(c, hipMemcpy(gamma, sl.gamma, 2 * c->sys.complen * sizeof(double), hipMemcpyDeviceToHost));
+    if (bare) HIPCHK(c, hipMemcpy(bare, sl.bare, c->sys.nc * c->sys.ns * sizeof(double), hipMemcpyDeviceToHost));
+    if (gamma) HIPCHK(c, hipMemcpy(gamma, sl.gamma, c->sys.nc * c->sys.complen * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -1233,7 +1309,7 @@ int pffrg_abs_max(pffrg_ctx *c, int32_t s, double *out) {
     if (!out) return fail(c, PFFRG_ERR_ARG, "null output");
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipMemsetAsync(c->d_max, 0, sizeof(unsigned long long), c->stream));
-    const long n = 6L * c->sys.ns * c->sys.nO * c->sys.nn * c->sys.nn;
+    const long n = 3L * c->sys.nc * c->sys.ns * c->sys.nO * c->sys.nn * c->sys.nn;
     hipLaunchKernelGGL(k_absmax_q3, dim3(grid_for(n)), dim3(256), 0, c->stream, c->slots[s].gamma, c->sys, c->d_max);
     HIPCHK(c, hipGetLastError());
     unsigned long long bits = 0;
@@ -1333,7 +1409,7 @@ int pffrg_probe_integrand(pffrg_ctx *c, int32_t pass, double L, int32_t a, int32
     } B;
     double *&dv = B.dv, *&dout = B.dout;
     HIPCHK(c, hipMalloc(&dv, n * sizeof(double)));
-    HIPCHK(c, hipMalloc(&dout, (size_t)n * 2 * c->sys.ns * sizeof(double)));
+    HIPCHK(c, hipMalloc(&dout, (size_t)n * c->sys.nc * c->sys.ns * sizeof(double)));
     HIPCHK(c, hipMemcpy(dv, v, n * sizeof(double), hipMemcpyHostToDevice));
     const uint32_t e = enc(ch, kernel, w1, w2, w3);
     const double tol[2] = {1e-8, 1e-4};
@@ -1352,7 +1428,7 @@ int pffrg_probe_integrand(pffrg_ctx *c, int32_t pass, double L, int32_t a, int32
 #undef PROBE
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(out, dout, (size_t)n * 2 * c->sys.ns * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(out, dout, (size_t)n * c->sys.nc * c->sys.ns * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -1608,6 +1684,7 @@ struct ChiJob {
     int nchi;
     double *chi;
     int32_t *nreg;
+    int comp;  // U(1)-DM correlation component (χxx, χzz, χxy)
 };
 
 #define CHICHK(expr)                                                      \
@@ -1713,6 +1790,7 @@ int chi_run(void *&mem, size_t &bytes, hipStream_t stream, const ChiJob *jobs, i
         P.chi = d_chi + P.prop0;
         P.L = jobs[k].L;
         P.nchi = jobs[k].nchi;
+        P.comp = jobs[k].comp;
         for (int w = 0; w < jobs[k].nchi; ++w) hchi[P.prop0 + w] = jobs[k].chi_mesh[w];
     }
     CHICHK(hipMemcpyAsync(d_probs, probs.data(), njobs * sizeof(ChiProb), hipMemcpyHostToDevice, stream));
@@ -1798,7 +1876,7 @@ int chi_run(void *&mem, size_t &bytes, hipStream_t stream, const ChiJob *jobs, i
                 const long tb = t0 + ((long)w * ns + i) * 9;
                 double I = 0.0;
                 for (int b = 0; b < 9; ++b) I += box[tb + b];
-                if (i == 0) I += prop[f0 + w];
+                if (i == 0 && (jobs[k].c->sys.nc == 2 || jobs[k].comp < 2)) I += prop[f0 + w];
                 jobs[k].chi[i + (long)ns * w] = I;
                 if (jobs[k].nreg)
                     for (int b = 0; b < 9; ++b) jobs[k].nreg[((long)w * ns + i) * 9 + b] = nr[tb + b];
@@ -1822,6 +1900,10 @@ int pffrg_batch_create(pffrg_ctx *const *ctxs, int32_t n, pffrg_batch **out) {
         if (!ctxs[k] || ready(ctxs[k])) {
             delete b;
             return PFFRG_ERR_STATE;
+        }
+        if (ctxs[k]->sys.nc != 2) {  // the batched launch serves the SU(2) action
+            delete b;
+            return PFFRG_ERR_ARG;
         }
         b->order[k] = k;
     }
@@ -1906,7 +1988,7 @@ int pffrg_batch_compute_chi(pffrg_batch *b, const double *Lambda, const int32_t 
         if (!chi_mesh[q] || !chi[q] || n_chi[q] < 1 || !(Lambda[q] > 0.0))
             return bfail(b, PFFRG_ERR_ARG, "invalid chi arguments");
         if (hipStreamSynchronize(c->stream) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "stream synchronize failed");
-        jobs.push_back(ChiJob{c, Lambda[q], a[q], chi_mesh[q], n_chi[q], chi[q], nullptr});
+        jobs.push_back(ChiJob{c, Lambda[q], a[q], chi_mesh[q], n_chi[q], chi[q], nullptr, 0});
     }
     if (jobs.empty()) return 0;
     if (hipSetDevice(b->device) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "hipSetDevice failed");
@@ -1922,8 +2004,17 @@ int pffrg_compute_chi(pffrg_ctx *c, double L, int32_t a, const double *chi_mesh,
     if ((rc = ready(c)) || (rc = check_slot(c, a))) return rc;
     if (!chi_mesh || !chi || !tol || nchi < 1 || !(L > 0.0)) return fail(c, PFFRG_ERR_ARG, "invalid chi arguments");
     HIPCHK(c, hipSetDevice(c->device));
-    ChiJob job{c, L, a, chi_mesh, nchi, chi, nreg};
     std::string msg;
+    if (c->sys.nc == 6) {
+        // U(1)-DM: χxx, χzz, χxy, each its own cubature (correlation_u1_dm.jl:138-190)
+        const long blk = (long)c->sys.ns * nchi;
+        ChiJob jobs[3];
+        for (int k = 0; k < 3; ++k)
+            jobs[k] = ChiJob{c, L, a, chi_mesh, nchi, chi + k * blk, nreg ? nreg + 9 * k * blk : nullptr, k};
+        rc = chi_run(c->chi_mem, c->chi_bytes, c->stream, jobs, 3, tol, msg);
+        return rc ? fail(c, rc, msg) : 0;
+    }
+    ChiJob job{c, L, a, chi_mesh, nchi, chi, nreg, 0};
     rc = chi_run(c->chi_mem, c->chi_bytes, c->stream, &job, 1, tol, msg);
     return rc ? fail(c, rc, msg) : 0;
 }
@@ -1938,9 +2029,9 @@ static int scan_cuts_into(pffrg_ctx *c, int a, char *mem, int32_t *idx, double *
     A.idx = (int *)(mem + SCAN_OFF_IDX);
     A.cuts = (double *)(mem + SCAN_OFF_CUTS);
     HIPCHK(c, launch_scan_cuts(A, c->stream));
-    const int nc = c->sys.nO + 3 * c->sys.nn;
-    HIPCHK(c, hipMemcpyAsync(idx, A.idx, 24 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(cuts, A.cuts, 6 * (size_t)nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    const int ncut = c->sys.nO + 3 * c->sys.nn, rows = 3 * c->sys.nc;
+    HIPCHK(c, hipMemcpyAsync(idx, A.idx, 4 * rows * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cuts, A.cuts, rows * (size_t)ncut * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1960,7 +2051,7 @@ int pffrg_scan_cuts_noalloc(pffrg_ctx *c, int32_t a, int32_t *idx, double *cuts)
     int rc;
     if ((rc = ready(c)) || (rc = check_slot(c, a))) return rc;
     if (!idx || !cuts) return fail(c, PFFRG_ERR_ARG, "null output pointer");
-    const size_t need = SCAN_OFF_CUTS + 6 * (size_t)(c->sys.nO + 3 * c->sys.nn) * sizeof(double);
+    const size_t need = SCAN_OFF_CUTS + 3 * (size_t)c->sys.nc * (c->sys.nO + 3 * c->sys.nn) * sizeof(double);
     if (need > SCAN_CUTS_BYTES)
         return fail(c, PFFRG_ERR_LIMIT, "scan_cuts_noalloc: cuts of " + std::to_string(need) +
                                             " bytes exceed the static scratch of " + std::to_string(SCAN_CUTS_BYTES));
@@ -1991,7 +2082,7 @@ int pffrg_resample(pffrg_ctx *c, int32_t src, int32_t dst, const double *sigma, 
     A.dst_gamma = c->slots[dst].gamma;
     A.dst_sigma = c->slots[dst].Sigma;
     HIPCHK(c, launch_resample(A, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->slots[dst].bare, c->slots[src].bare, 2 * c->sys.ns * sizeof(double),
+    HIPCHK(c, hipMemcpyAsync(c->slots[dst].bare, c->slots[src].bare, c->sys.nc * c->sys.ns * sizeof(double),
                              hipMemcpyDeviceToDevice, c->stream));
     invalidate(c, dst);
     // the host table is released only after the copy from it has completed (apply_mesh waits)

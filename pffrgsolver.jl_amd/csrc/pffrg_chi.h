@@ -25,6 +25,7 @@ struct ChiProb {
     int nchi;
     int task0;           // first global task: local task (w * ns + site) * 9 + box
     int prop0;           // first global χ frequency
+    int comp;            // U(1)-DM: χxx (0), χzz (1), χxy (2); the SU(2) kernel ignores it
 };
 
 struct ChiArgs {

@@ -58,7 +58,11 @@ struct Sys {
     long q21, q22, q3;    // offsets of q2_1, q2_2, q3 inside a channel block
     long chlen, complen;  // channel block length, component block length
     double valS, valT;    // Ω_max + ν_max of the s/u and t meshes
+    int nc;               // vertex components: 2 (SU(2): spin, density) or 6 (U(1)-DM: xx, zz, DM, dd, zd, dz)
 };
+
+// U(1)-DM component indices (action_u1_dm.jl:1-11)
+enum { U1_XX = 0, U1_ZZ = 1, U1_DM = 2, U1_DD = 3, U1_ZD = 4, U1_DZ = 5 };
 
 // Mesh views; in the pass kernels they point into LDS.  ST is the scalar type of the
 // per-node work (double for the FP64 path, float for the FP32-mixed path).
@@ -109,6 +113,7 @@ template <typename ST = double>
 struct Buf {
     int kernel;
     bool exch, map, smu, snu;
+    bool smunu;  // read by the U(1)-DM flags only
     Param<ST> p1, p2, p3;
 };
 
@@ -116,7 +121,7 @@ template <typename ST = double>
 __device__ __forceinline__ Buf<ST> buf_empty() {
     Buf<ST> b;
     b.kernel = 0;
-    b.exch = b.map = b.smu = b.snu = false;
+    b.exch = b.map = b.smu = b.snu = b.smunu = false;
     b.p1 = b.p2 = b.p3 = Param<ST>{0, 0, (ST)0, (ST)0};
     return b;
 }
@@ -166,21 +171,25 @@ __device__ __forceinline__ Buf<ST> buf_s(const Meshes<ST> &M, const Sys &y, ST w
 // get_buffer_t (buffer.jl:127-160)
 template <typename ST>
 __device__ __forceinline__ Buf<ST> buf_t(const Meshes<ST> &M, const Sys &y, ST w, ST v, ST vp) {
-    bool smu = false, snu = false;
-    if (w < (ST)0) { w = -w; }
+    bool smu = false, snu = false, smunu = false;
+    if (w < (ST)0) { w = -w; smunu = !smunu; }
     if (v < (ST)0) { v = -v; smu = !smu; }
     if (vp < (ST)0) { vp = -vp; snu = !snu; }
-    return make_buf(w, v, vp, M.Ot, M.nt, y.nO, y.nn, false, false, smu, snu);
+    Buf<ST> b = make_buf(w, v, vp, M.Ot, M.nt, y.nO, y.nn, false, false, smu, snu);
+    b.smunu = b.kernel ? smunu : false;
+    return b;
 }
 
 // get_buffer_u (buffer.jl:163-199)
 template <typename ST>
 __device__ __forceinline__ Buf<ST> buf_u(const Meshes<ST> &M, const Sys &y, ST w, ST v, ST vp) {
-    bool ex = false, mp = false, snu = false;
-    if (w < (ST)0) { w = -w; ex = !ex; }
+    bool ex = false, mp = false, snu = false, smunu = false;
+    if (w < (ST)0) { w = -w; ex = !ex; smunu = !smunu; }
     if (v < (ST)0) { v = -v; ex = !ex; mp = !mp; snu = !snu; }
     if (vp < (ST)0) { vp = -vp; mp = !mp; snu = !snu; }
-    return make_buf(w, v, vp, M.Os, M.ns, y.nO, y.nn, ex, mp, false, snu);
+    Buf<ST> b = make_buf(w, v, vp, M.Os, M.ns, y.nO, y.nn, ex, mp, false, snu);
+    b.smunu = b.kernel ? smunu : false;
+    return b;
 }
 
 // Interpolate one channel at one site for both SU(2) components (channel.jl:42-305):
@@ -250,6 +259,27 @@ __device__ __forceinline__ double dens_sign(const Buf<> &b) {
     return s;
 }
 
+// apply_flags_u1_dm (action_u1_dm.jl:79-118) from the buffer's flags (bit 0 exchange, 1 sgn_μ,
+// 2 sgn_ν, 3 sgn_μν): the sign of the channel's contribution to component comp and the component
+// src it is read from (the site exchange flips DM and swaps zd and dz)
+__device__ __forceinline__ double flag_u1(int flags, int comp, int &src) {
+    double sgn = 1.0;
+    src = comp;
+    if ((comp == U1_ZD || comp == U1_DZ) && (flags & 8)) sgn *= -1.0;
+    if ((comp == U1_DD || comp == U1_DZ) && (flags & 2)) sgn *= -1.0;
+    if ((comp == U1_DD || comp == U1_ZD) && (flags & 4)) sgn *= -1.0;
+    if (flags & 1) {
+        if (comp == U1_DM) sgn *= -1.0;
+        else if (comp == U1_ZD) src = U1_DZ;
+        else if (comp == U1_DZ) src = U1_ZD;
+    }
+    return sgn;
+}
+
+__device__ __forceinline__ int buf_flags(const Buf<> &b) {
+    return (b.exch ? 1 : 0) | (b.smu ? 2 : 0) | (b.snu ? 4 : 0) | (b.smunu ? 8 : 0);
+}
+
 // One action as seen by a kernel: vertex channels (TV), bare and Σ (FP64).
 template <typename TV>
 struct ActV {
@@ -294,6 +324,41 @@ __device__ __forceinline__ int gamma_site(const ActV<TV> &A, const Buf<> &bs, co
         vd += (TV)dens_sign(bu) * r1;
     }
     return (MS ? corners(bs.kernel) : 0) + (MT ? corners(bt.kernel) : 0) + (MU ? corners(bu.kernel) : 0);
+}
+
+// One channel of one component at one site (channel.jl:42-305) on the plain FP64 layout.
+__device__ __forceinline__ double interp1(const double *__restrict__ c, const Buf<> &b, int site, const Sys &y) {
+    double r0, r1;
+    interp2<double>(c, c, b, site, y, r0, r1);
+    return r0;
+}
+
+// get_Γ_comp for the six U(1)-DM components at one site (Action.jl:29-107 with
+// apply_flags_u1_dm), channel masks MS/MT/MU; v[comp].
+template <bool MS, bool MT, bool MU>
+__device__ __forceinline__ void gamma_site_u1(const ActV<double> &A, const Buf<> &bs, const Buf<> &bt, const Buf<> &bu,
+                                              int i, const int *exch, const Sys &y, double *v) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = A.bare[k * y.ns + i];
+    const Buf<> *bb[3] = {&bs, &bt, &bu};
+    const bool on[3] = {MS, MT, MU};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const Buf<> &b = *bb[c];
+        if (!on[c] || !b.kernel) continue;
+        const int s = b.exch ? exch[i] : i;
+        const int chn = (c == 1) ? 1 : ((c == 0) == b.map ? 2 : 0);
+        const int fl = buf_flags(b);
+        double r[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) r[k] = interp1(A.gamma + k * y.complen + chn * y.chlen, b, s, y);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            int src;
+            const double sg = flag_u1(fl, k, src);
+            v[k] += sg * r[src];
+        }
+    }
 }
 
 // ---------------------------------------------------------------- propagators (Flow/bubbles.jl)

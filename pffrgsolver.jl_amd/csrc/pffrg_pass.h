@@ -48,7 +48,8 @@ struct PassArgs {
 template <typename TV>
 struct DescT {
     int meta;            // kernel (bits 0-2) | exchange << 3 | map << 4 | negative density sign << 5 |
-                         // upper corner one step up along Ω << 6, ν << 7, ν' << 8 (else the same point)
+                         // upper corner one step up along Ω << 6, ν << 7, ν' << 8 (else the same point) |
+                         // sgn_μ << 9 | sgn_ν << 10 | sgn_μν << 11 (the U(1)-DM flags)
     int off;             // offset of the first corner inside the channel block (site 0)
     int pad[sizeof(TV) == 4 ? 1 : 0];  // 44- or 72-byte stride (11 or 18 dwords): the descriptors of up to
                                        // 16 (FP32) or 8 (FP64) groups start in distinct LDS banks
@@ -137,10 +138,11 @@ __host__ __device__ inline PassLds pass_lds(const Sys &y, int n_ov, int desc_siz
     L.mp = (mp + 1) & ~1;
     int o = 10 * L.mp * 8;
     L.o_sig = o; o += (sig_d ? 2 : 1) * L.mp * 8;
-    // exchange rows: lane (g, i) at g*(GS+1)+i, then one row for the idle lanes; the block also
-    // holds the 12 local-vertex values per group of the t channel
+    // exchange rows: lane (g, i) at g*(GS+1)+i, then one row for the idle lanes; 2 nc rows (the
+    // Richardson check reads two values per component); the block also holds the 6 nc local-vertex
+    // values per group of the t channel
     L.sr = (y.G * (y.GS + 1) + 2) & ~1;
-    L.o_scr = o; o += (4 * L.sr > 12 * y.G ? 4 * L.sr : 12 * y.G) * 8;
+    L.o_scr = o; o += (2 * y.nc * L.sr > 6 * y.nc * y.G ? 2 * y.nc * L.sr : 6 * y.nc * y.G) * 8;
     L.o_bub = o; o += y.G * 4 * 8;
     L.o_ctl = o; o += y.G * (int)sizeof(Ctl);
     L.o_desc = o; o += y.G * MAX_DESC * desc_size;
@@ -386,8 +388,12 @@ __device__ __forceinline__ void build_desc_u(const Lds &S, const Sys &y, uint32_
     if (kernel != 4) { l3 = 0; h3 = 0; a3w = (ST)1; b3w = (ST)0; }
     if (kernel == 1) { l2 = 0; h2 = 0; a2w = (ST)1; b2w = (ST)0; }
     const int base = (kernel == 2) ? (int)y.q21 : (kernel == 3) ? (int)y.q22 : (kernel == 4) ? (int)y.q3 : 0;
+    // the flags of get_buffer_{s,t,u}: s: sgn_μ = ν < 0, sgn_ν = ν' < 0; t: also sgn_μν = Ω < 0;
+    // u: sgn_ν = (ν < 0) xor (ν' < 0), sgn_μν = Ω < 0 (buffer.jl:88-199)
+    const bool smu = (c != 2) && nx, snu = (c == 2) ? (nx != nz) : nz, smunu = (c != 0) && nw;
     d.meta = ((code & 1u) && kernel) ? (kernel | (ex ? 8 : 0) | (mp ? 16 : 0) | (negd ? 32 : 0) |
-                                        (h1 > l1 ? 64 : 0) | (h2 > l2 ? 128 : 0) | (h3 > l3 ? 256 : 0))
+                                        (h1 > l1 ? 64 : 0) | (h2 > l2 ? 128 : 0) | (h3 > l3 ? 256 : 0) |
+                                        (smu ? 512 : 0) | (snu ? 1024 : 0) | (smunu ? 2048 : 0))
                                      : 0;
     d.off = base + ns * (l1 + nO * (l2 + nn * l3));
     d.w[0] = (TV)(a1w * a2w * a3w); d.w[1] = (TV)(b1w * a2w * a3w);
@@ -529,6 +535,66 @@ __device__ __forceinline__ void gather_channel(const DescT<TV> &d, const TV *gam
     }
 }
 
+// U(1)-DM gathers.  The gather copy stores the six components of an entry next to each other,
+// three pairs per corner; r[k] is channel c's interpolation of component k before the flags.
+template <typename TV>
+__device__ __forceinline__ void corners6(const DescT<TV> &d, const TV *gamma, const int *exch, int site, const Sys &y,
+                                         int c, TV *r, int &cn) {
+    using V2 = typename Vec2<TV>::T;
+    const V2 *q = reinterpret_cast<const V2 *>(gamma);
+    const int meta = d.meta;
+    const int n = wave_corners(meta);
+    const bool on = (meta & 7) != 0;
+    const int sx = (meta & 8) ? exch[site] : site;
+    const bool map = (meta & 16) != 0;
+    const int chn = (c == 1) ? 1 : ((c == 0) == map ? 2 : 0);
+    const uint32_t o = on ? 3u * (uint32_t)(chn * y.chlen + d.off + sx) : 0u;
+    uint32_t c1, c2, c3;
+    corner_steps(meta, y, c1, c2, c3);
+    c1 *= 3u; c2 *= 3u; c3 *= 3u;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r[k] = (TV)0;
+    cn = 0;
+    if (!n) return;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        V2 x[8];
+        load_corners(q, o + (uint32_t)j, c1, c2, c3, n, x);
+        if (on) corner_sum(d, n, x, r[2 * j], r[2 * j + 1]);
+    }
+    if (on) cn = corners(meta & 7);
+}
+
+// add one channel's interpolations r to the six components v with apply_flags_u1_dm
+// (action_u1_dm.jl:79-118); the flags are the descriptor's exchange and sign bits
+template <typename TV>
+__device__ __forceinline__ void add_flags6(int meta, const TV *r, TV *v) {
+    const int fl = ((meta >> 3) & 1) | (((meta >> 9) & 7) << 1);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        int src;
+        const double sg = flag_u1(fl, k, src);
+        v[k] += (sg < 0.0) ? -r[src] : r[src];
+    }
+}
+
+// gather_slot for the six U(1)-DM components: bare + the enabled channels in s, t, u order
+template <typename TV>
+__device__ __forceinline__ void gather_slot6(const DescT<TV> *ds, const TV *gamma, const int *exch, int site,
+                                             const Sys &y, TV *v, int &cn) {
+    cn = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        TV r[6];
+        int n;
+        corners6<TV>(ds[c], gamma, exch, site, y, c, r, n);
+        if (ds[c].meta & 7) {
+            add_flags6<TV>(ds[c].meta, r, v);
+            cn += n;
+        }
+    }
+}
+
 // One leg x of the bubble: G(x) and, for the Katanin bubble, K(x) = S(x) + G(x)² dΣ(x)
 // (bubbles.jl:17-80; S reuses G as the reference's get_S recomputes the same value).  Σ and
 // dΣ are interpolated with one shared bracket of |x| (get_Σ, Action.jl:8-26).
@@ -566,16 +632,71 @@ __device__ __forceinline__ void bubble_leg(ST L, ST x, const MeshV<ST> &M, int n
     }
 }
 
-// Integrand of one element at loop frequency v for site i: returns Γs, Γd (before the
-// dv weight).  Restates compute_{s,t,u}_{kat,left,central}! (flow_lib/flow_su2/*.jl),
-// compute_{s,t,u}_BSE! (parquet_lib/parquet_su2/BSE_su2_*.jl) and compute_s_reduced!
-// (SDE_su2.jl:2-43).  The per-node scalar work (two bubble legs, one buffer per vertex
-// slot and channel) is split over the lanes of the group and published in LDS; every lane
-// then gathers its own site.  Must be executed by every lane of the wavefront.
-template <int PASS, typename TV>
+// U(1)-DM component products (components xx, zz, DM, dd, zd, dz), the brackets multiplied by -p:
+// s and u channels (flow_u1_dm_s.jl:52-57, flow_u1_dm_u.jl:52-57), the local part of the t
+// channel (flow_u1_dm_t.jl:56-67) and one overlap row of its non-local sum (:88-93, no 2S).
+template <typename TV>
+__device__ __forceinline__ void prod_su6(bool u, const TV *a, const TV *b, TV *g) {
+    const TV two = (TV)2.0;
+    const TV xx1 = a[0], zz1 = a[1], DM1 = a[2], dd1 = a[3], zd1 = a[4], dz1 = a[5];
+    const TV xx2 = b[0], zz2 = b[1], DM2 = b[2], dd2 = b[3], zd2 = b[4], dz2 = b[5];
+    if (!u) {
+        g[0] = DM1 * dz2 - DM1 * zd2 + xx1 * dd2 + dd1 * xx2 - xx1 * zz2 - dz1 * DM2 + zd1 * DM2 - zz1 * xx2;
+        g[1] = zz1 * dd2 + dd1 * zz2 - dz1 * zd2 - two * xx1 * xx2 - two * DM1 * DM2 - zd1 * dz2;
+        g[2] = dd1 * DM2 + DM1 * dd2 - zd1 * xx2 - xx1 * dz2 - zz1 * DM2 + xx1 * zd2 - DM1 * zz2 + dz1 * xx2;
+        g[3] = -dz1 * dz2 + two * xx1 * xx2 + dd1 * dd2 + two * DM1 * DM2 - zd1 * zd2 + zz1 * zz2;
+        g[4] = zd1 * dd2 + dd1 * zd2 - two * DM1 * xx2 + dz1 * zz2 + zz1 * dz2 + two * xx1 * DM2;
+        g[5] = zd1 * zz2 + dd1 * dz2 + zz1 * zd2 + two * DM1 * xx2 + dz1 * dd2 - two * xx1 * DM2;
+    } else {
+        g[0] = -DM1 * dz2 - DM1 * zd2 + xx1 * dd2 + dd1 * xx2 + xx1 * zz2 + dz1 * DM2 + zd1 * DM2 + zz1 * xx2;
+        g[1] = zz1 * dd2 + dd1 * zz2 - dz1 * zd2 + two * xx1 * xx2 + two * DM1 * DM2 - zd1 * dz2;
+        g[2] = dd1 * DM2 + DM1 * dd2 - zd1 * xx2 + xx1 * dz2 + zz1 * DM2 + xx1 * zd2 + DM1 * zz2 - dz1 * xx2;
+        g[3] = -dz1 * dz2 + two * xx1 * xx2 + dd1 * dd2 + two * DM1 * DM2 - zd1 * zd2 + zz1 * zz2;
+        g[4] = zd1 * dd2 + dd1 * zd2 - two * DM1 * xx2 + dz1 * zz2 + zz1 * dz2 + two * xx1 * DM2;
+        g[5] = zd1 * zz2 + dd1 * dz2 + zz1 * zd2 - two * DM1 * xx2 + dz1 * dd2 + two * xx1 * DM2;
+    }
+}
+
+template <typename TV>
+__device__ __forceinline__ void prod_t_local6(const TV *a, const TV *c4, const TV *c3, const TV *b, TV *g) {
+    const TV two = (TV)2.0;
+    const TV xx1 = a[0], zz1 = a[1], DM1 = a[2], dd1 = a[3], zd1 = a[4], dz1 = a[5];
+    const TV xx2 = b[0], zz2 = b[1], DM2 = b[2], dd2 = b[3], zd2 = b[4], dz2 = b[5];
+    const TV xx3 = c3[0], zz3 = c3[1], DM3 = c3[2], dd3 = c3[3], zd3 = c3[4], dz3 = c3[5];
+    const TV xx4 = c4[0], zz4 = c4[1], DM4 = c4[2], dd4 = c4[3], zd4 = c4[4], dz4 = c4[5];
+    g[0] = DM1 * dz4 - DM1 * zd4 + xx1 * dd4 - xx1 * zz4 + dd3 * xx2 + dz3 * DM2 - zd3 * DM2 - zz3 * xx2;
+    g[1] = zz1 * dd4 + two * zd1 * DM4 - zd1 * zd4 + zz1 * zz4 - two * zz1 * xx4 - zd1 * dz4 - two * DM3 * dz2 -
+           dz3 * dz2 + dd3 * zz2 - two * xx3 * zz2 + zz3 * zz2 - zd3 * dz2;
+    g[2] = DM1 * dd4 - xx1 * dz4 + xx1 * zd4 - DM1 * zz4 - dz3 * xx2 + dd3 * DM2 + zd3 * xx2 - zz3 * DM2;
+    g[3] = -dz1 * dz4 + two * dd1 * xx4 + dd1 * zz4 - dz1 * zd4 - two * dz1 * DM4 + dd1 * dd4 + two * DM3 * zd2 +
+           two * xx3 * dd2 + zz3 * dd2 - dz3 * zd2 + dd3 * dd2 - zd3 * zd2;
+    g[4] = zd1 * zz4 + zd1 * dd4 + two * zd1 * xx4 + zz1 * zd4 + two * zz1 * DM4 + zz1 * dz4 + zd3 * dd2 +
+           two * DM3 * dd2 + dd3 * zd2 + zz3 * zd2 - two * xx3 * zd2 + dz3 * dd2;
+    g[5] = -two * dd1 * DM4 + dd1 * zd4 + dd1 * dz4 + dz1 * zz4 + dz1 * dd4 - two * dz1 * xx4 + zd3 * zz2 +
+           two * xx3 * dz2 + dd3 * dz2 + dz3 * zz2 - two * DM3 * zz2 + zz3 * dz2;
+}
+
+template <typename TV>
+__device__ __forceinline__ void add_t_row6(TV cm, const TV *a, const TV *b, TV *G) {
+    G[0] += cm * (a[0] * b[0] - a[2] * b[2]);
+    G[1] += cm * (a[1] * b[1] - a[4] * b[5]);
+    G[2] += cm * (a[2] * b[0] + a[0] * b[2]);
+    G[3] += cm * (a[3] * b[3] - a[5] * b[4]);
+    G[4] += cm * (a[4] * b[3] + a[1] * b[4]);
+    G[5] += cm * (a[3] * b[5] + a[5] * b[1]);
+}
+
+// Integrand of one element at loop frequency v for site i: G[comp] (before the dv weight).
+// Restates compute_{s,t,u}_{kat,left,central}! (flow_lib/flow_su2/*.jl, flow_u1_dm/*.jl),
+// compute_{s,t,u}_BSE! (parquet_lib/parquet_su2/BSE_su2_*.jl, parquet_u1_dm/BSE_u1_dm_*.jl) and
+// compute_s_reduced! (SDE_su2.jl:2-43, SDE_u1_dm.jl:2-43); NC = 2 (SU(2)) or 6 (U(1)-DM), whose
+// buffers and channel masks are the same.  The per-node scalar work (two bubble legs, one buffer
+// per vertex slot and channel) is split over the lanes of the group and published in LDS; every
+// lane then gathers its own site.  Must be executed by every lane of the wavefront.
+template <int PASS, typename TV, int NC>
 __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, int ch, double W, double vw,
                                           double vwp, double v, int i, int slot, int g, int gbase, bool real_lane,
-                                          TV &Gs, TV &Gd, int &nc) {
+                                          TV *G, int &nc) {
     const Sys &y = P.y;
     using ST = typename PassScalar<TV>::T;
     const int nslot = (ch == 1) ? 4 : 2;
@@ -589,9 +710,9 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
             if (t < 2) {
                 ST x1, x2;
                 bubble_args<ST>(ch, (ST)W, (ST)v, x1, x2);
-                ST G, K;
-                bubble_leg<PASS == P_KAT, ST>((ST)P.L, t == 0 ? x1 : x2, meshv<ST>(S), y.nsig, G, K);
-                bg[2 * t] = (double)G;
+                ST Gl, K;
+                bubble_leg<PASS == P_KAT, ST>((ST)P.L, t == 0 ? x1 : x2, meshv<ST>(S), y.nsig, Gl, K);
+                bg[2 * t] = (double)Gl;
                 bg[2 * t + 1] = (double)K;
             } else {
                 build_desc_u<TV, ST>(S, y, S.code[t - 2], (t - 2) % 3, (ST)W, (ST)vw, (ST)vwp, (ST)v, dg[t - 2]);
@@ -602,17 +723,24 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
     const ST G1 = (ST)bg[0], K1 = (ST)bg[1], G2 = (ST)bg[2], K2 = (ST)bg[3];
     const ST twopi = (ST)2 * (ST)PI_D;
     const TV tp = (TV)((PASS == P_KAT) ? K1 * G2 / twopi + K2 * G1 / twopi : -(G1 * G2 / twopi));
-    TV v1s = 0, v1d = 0, v2s = 0, v2d = 0, v3s = 0, v3d = 0, v4s = 0, v4d = 0;
+    TV v1[NC], v2[NC], v3[NC], v4[NC];
     nc = 0;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int cfg = vertex_cfg<PASS>(ch, k);
         const ActV<TV> &V = (cfg & 8) ? P.D : P.A;
-        TV vs = (TV)V.bare[i], vd = (TV)V.bare[y.ns + i];
+        TV *vk = (k == 0) ? v1 : v2;
         int cn;
-        gather_slot<TV>(dg + 3 * k, V.gamma, S.exch, i, y, vs, vd, cn);
-        if (k == 0) { v1s = vs; v1d = vd; }
-        else { v2s = vs; v2d = vd; }
+        if constexpr (NC == 2) {
+            TV vs = (TV)V.bare[i], vd = (TV)V.bare[y.ns + i];
+            gather_slot<TV>(dg + 3 * k, V.gamma, S.exch, i, y, vs, vd, cn);
+            vk[0] = vs;
+            vk[1] = vd;
+        } else {
+#pragma unroll
+            for (int q = 0; q < NC; ++q) vk[q] = (TV)V.bare[q * y.ns + i];
+            gather_slot6<TV>(dg + 3 * k, V.gamma, S.exch, i, y, vk, cn);
+        }
         nc += cn;
     }
     if (ch == 1) {
@@ -620,17 +748,24 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
         // q < 6 (channel q % 3 of slot 2 + q / 3) goes to lane q mod GS of the group, which
         // publishes its interpolation; every lane then adds the three channels to bare[0] in
         // gather_slot's order
-        double *loc = S.scr + 12 * gs;
+        double *loc = S.scr + 6 * NC * gs;
         if (real_lane) {
 #pragma clang loop unroll(disable)
             for (int q = i; q < 6; q += y.GS) {
                 const int k = 2 + q / 3, c = q - 3 * (q / 3);
                 const ActV<TV> &V = (vertex_cfg<PASS>(ch, k) & 8) ? P.D : P.A;
-                TV r0, r1;
                 int cn;
-                gather_channel<TV>(dg[3 * k + c], V.gamma, S.exch, 0, y, c, r0, r1, cn);
-                loc[2 * q] = (double)r0;
-                loc[2 * q + 1] = (double)r1;
+                if constexpr (NC == 2) {
+                    TV r0, r1;
+                    gather_channel<TV>(dg[3 * k + c], V.gamma, S.exch, 0, y, c, r0, r1, cn);
+                    loc[2 * q] = (double)r0;
+                    loc[2 * q + 1] = (double)r1;
+                } else {
+                    TV r[6];
+                    corners6<TV>(dg[3 * k + c], V.gamma, S.exch, 0, y, c, r, cn);
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) loc[6 * q + j] = (double)r[j];
+                }
                 nc += cn;  // each local corner counted once per node
             }
         }
@@ -638,70 +773,124 @@ __device__ __forceinline__ void node_eval(const PassArgs<TV> &P, const Lds &S, i
 #pragma unroll
         for (int k = 2; k < 4; ++k) {
             const ActV<TV> &V = (vertex_cfg<PASS>(ch, k) & 8) ? P.D : P.A;
-            TV vs = (TV)V.bare[0], vd = (TV)V.bare[y.ns];
+            TV *vk = (k == 2) ? v3 : v4;
+            if constexpr (NC == 2) {
+                TV vs = (TV)V.bare[0], vd = (TV)V.bare[y.ns];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const int meta = dg[3 * k + c].meta;
-                if (meta & 7) {
-                    const TV r0 = (TV)loc[2 * (3 * (k - 2) + c)], r1 = (TV)loc[2 * (3 * (k - 2) + c) + 1];
-                    vs += r0;
-                    vd += (meta & 32) ? -r1 : r1;
+                for (int c = 0; c < 3; ++c) {
+                    const int meta = dg[3 * k + c].meta;
+                    if (meta & 7) {
+                        const TV r0 = (TV)loc[2 * (3 * (k - 2) + c)], r1 = (TV)loc[2 * (3 * (k - 2) + c) + 1];
+                        vs += r0;
+                        vd += (meta & 32) ? -r1 : r1;
+                    }
+                }
+                vk[0] = vs;
+                vk[1] = vd;
+            } else {
+#pragma unroll
+                for (int q = 0; q < NC; ++q) vk[q] = (TV)V.bare[q * y.ns];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const int meta = dg[3 * k + c].meta;
+                    if (meta & 7) {
+                        TV r[6];
+#pragma unroll
+                        for (int j = 0; j < 6; ++j) r[j] = (TV)loc[6 * (3 * (k - 2) + c) + j];
+                        add_flags6<TV>(meta, r, vk);
+                    }
                 }
             }
-            if (k == 2) { v3s = vs; v3d = vd; }
-            else { v4s = vs; v4d = vd; }
         }
         __syncthreads();  // the scratch is rewritten by the overlap exchange below
     }
-    if (ch == 0) {
-        Gs = -tp * ((TV)-2.0 * v1s * v2s + v1s * v2d + v1d * v2s);
-        Gd = -tp * ((TV)3.0 * v1s * v2s + v1d * v2d);
-    } else if (ch == 2) {
-        Gs = -tp * ((TV)2.0 * v1s * v2s + v1s * v2d + v1d * v2s);
-        Gd = -tp * ((TV)3.0 * v1s * v2s + v1d * v2d);
+    if constexpr (NC == 2) {
+        const TV v1s = v1[0], v1d = v1[1], v2s = v2[0], v2d = v2[1];
+        if (ch == 0) {
+            G[0] = -tp * ((TV)-2.0 * v1s * v2s + v1s * v2d + v1d * v2s);
+            G[1] = -tp * ((TV)3.0 * v1s * v2s + v1d * v2d);
+        } else if (ch == 2) {
+            G[0] = -tp * ((TV)2.0 * v1s * v2s + v1s * v2d + v1d * v2s);
+            G[1] = -tp * ((TV)3.0 * v1s * v2s + v1d * v2d);
+        } else {
+            const TV v3s = v3[0], v3d = v3[1], v4s = v4[0], v4d = v4[1];
+            TV gsum = -tp * ((TV)-1.0 * v1s * v4s + v1s * v4d - (TV)1.0 * v3s * v2s + v3d * v2s);
+            TV gdsum = -tp * ((TV)3.0 * v1d * v4s + v1d * v4d + (TV)3.0 * v3s * v2d + v3d * v2d);
+            // overlap gather Σ_(j1,j2,m) -p (-2) m (2S) v1[j1] v2[j2]  (flow_su2_t.jl:59-74).  The
+            // scratch holds (v1s, v1d, v2s, v2d) of each lane next to each other, so one LDS read
+            // fetches both components of a site; rows go four at a time, their reads issued
+            // together, and are added in row order.
+            using V2 = typename Vec2<TV>::T;
+            V2 *sx = reinterpret_cast<V2 *>(S.scr);
+            sx[2 * slot] = V2{v1s, v1d};
+            sx[2 * slot + 1] = V2{v2s, v2d};
+            __syncthreads();
+            const TV cc = -tp * (TV)(-2.0);
+            const TV S2 = (TV)y.S2;
+            const int r0 = S.ov_off[i], r1 = S.ov_off[i + 1];
+            const V2 *xg = sx + 2 * gbase;
+            int r = r0;
+            for (; r + 4 <= r1; r += 4) {
+                uint32_t row[4];
+                V2 e1[4], e2[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) row[k] = S.ov[r + k];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    e1[k] = xg[2 * (int)(row[k] & 255)];
+                    e2[k] = xg[2 * (int)((row[k] >> 8) & 255) + 1];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const TV m = (TV)(double)(row[k] >> 16);
+                    gsum += cc * m * S2 * e1[k].x * e2[k].x;
+                    gdsum += cc * m * S2 * e1[k].y * e2[k].y;
+                }
+            }
+            for (; r < r1; ++r) {
+                const uint32_t row = S.ov[r];
+                const V2 e1 = xg[2 * (int)(row & 255)], e2 = xg[2 * (int)((row >> 8) & 255) + 1];
+                const TV m = (TV)(double)(row >> 16);
+                gsum += cc * m * S2 * e1.x * e2.x;
+                gdsum += cc * m * S2 * e1.y * e2.y;
+            }
+            G[0] = gsum;
+            G[1] = gdsum;
+        }
     } else {
-        TV gsum = -tp * ((TV)-1.0 * v1s * v4s + v1s * v4d - (TV)1.0 * v3s * v2s + v3d * v2s);
-        TV gdsum = -tp * ((TV)3.0 * v1d * v4s + v1d * v4d + (TV)3.0 * v3s * v2d + v3d * v2d);
-        // overlap gather Σ_(j1,j2,m) -p (-2) m (2S) v1[j1] v2[j2]  (flow_su2_t.jl:59-74).  The
-        // scratch holds (v1s, v1d, v2s, v2d) of each lane next to each other, so one LDS read
-        // fetches both components of a site; rows go four at a time, their reads issued
-        // together, and are added in row order.
-        using V2 = typename Vec2<TV>::T;
-        V2 *sx = reinterpret_cast<V2 *>(S.scr);
-        sx[2 * slot] = V2{v1s, v1d};
-        sx[2 * slot + 1] = V2{v2s, v2d};
-        __syncthreads();
-        const TV cc = -tp * (TV)(-2.0);
-        const TV S2 = (TV)y.S2;
-        const int r0 = S.ov_off[i], r1 = S.ov_off[i + 1];
-        const V2 *xg = sx + 2 * gbase;
-        int r = r0;
-        for (; r + 4 <= r1; r += 4) {
-            uint32_t row[4];
-            V2 e1[4], e2[4];
+        TV gb[6];
+        if (ch == 0 || ch == 2) {
+            if (PASS == P_REDUCED) {
+                // the left vertex is the bare coupling; xx, zz and dd are kept (SDE_u1_dm.jl:36-42)
+                prod_su6<TV>(false, v1, v2, gb);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) row[k] = S.ov[r + k];
+                for (int q = 0; q < 6; ++q) G[q] = (q == 0 || q == 1 || q == 3) ? -tp * gb[q] : (TV)0;
+            } else {
+                prod_su6<TV>(ch == 2, v1, v2, gb);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                e1[k] = xg[2 * (int)(row[k] & 255)];
-                e2[k] = xg[2 * (int)((row[k] >> 8) & 255) + 1];
+                for (int q = 0; q < 6; ++q) G[q] = -tp * gb[q];
             }
+        } else {
+            prod_t_local6<TV>(v1, v4, v3, v2, gb);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const TV m = (TV)(double)(row[k] >> 16);
-                gsum += cc * m * S2 * e1[k].x * e2[k].x;
-                gdsum += cc * m * S2 * e1[k].y * e2[k].y;
+            for (int q = 0; q < 6; ++q) G[q] = -tp * gb[q];
+            // overlap gather Σ_(j1,j2,m) -p (-2) m (a[j1] . b[j2])  (flow_u1_dm_t.jl:71-94): each
+            // lane publishes its twelve components (v1 then v2) in the scratch
+            TV *xs = reinterpret_cast<TV *>(S.scr);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                xs[12 * slot + q] = v1[q];
+                xs[12 * slot + 6 + q] = v2[q];
+            }
+            __syncthreads();
+            const TV cc = -tp * (TV)(-2.0);
+            const TV *xg = xs + 12 * gbase;
+            for (int r = S.ov_off[i]; r < S.ov_off[i + 1]; ++r) {
+                const uint32_t row = S.ov[r];
+                const TV m = (TV)(double)(row >> 16);
+                add_t_row6<TV>(cc * m, xg + 12 * (int)(row & 255), xg + 12 * (int)((row >> 8) & 255) + 6, G);
             }
         }
-        for (; r < r1; ++r) {
-            const uint32_t row = S.ov[r];
-            const V2 e1 = xg[2 * (int)(row & 255)], e2 = xg[2 * (int)((row >> 8) & 255) + 1];
-            const TV m = (TV)(double)(row >> 16);
-            gsum += cc * m * S2 * e1.x * e2.x;
-            gdsum += cc * m * S2 * e1.y * e2.y;
-        }
-        Gs = gsum;
-        Gd = gdsum;
     }
     // the descriptors and bubble legs of this node are read; the next node may overwrite them
     __syncthreads();
@@ -750,8 +939,9 @@ __device__ __forceinline__ bool subinterval(int j, int eval, double ref, const d
 }
 
 // One output element of one pass (block = one wavefront).  e_idx is the element's position in
-// its pass (and the probe index in probe mode), code its encoded (ch, kernel, w1, w2, w3).
-template <int PASS, typename TV>
+// its pass (and the probe index in probe mode), code its encoded (ch, kernel, w1, w2, w3); NC
+// vertex components (2: SU(2), 6: U(1)-DM).
+template <int PASS, typename TV, int NC = 2>
 __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_idx, uint32_t code) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     __shared__ uint32_t s_code[MAX_DESC];
@@ -873,8 +1063,9 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
     int lvl = 0, cnt = 0;   // owner: nodes in the current level, contributions received
     bool init = true, done = true, act = false;
     double a = 0.0, b = 0.0, sgn = 1.0, h = 0.0;
-    double b10 = 0.0, b11 = 0.0, b20 = 0.0, b21 = 0.0, res0 = 0.0, res1 = 0.0;
-    double acc0 = 0.0, acc1 = 0.0;
+    double b1[NC], b2[NC], res[NC], acc[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) b1[k] = b2[k] = res[k] = acc[k] = 0.0;
     unsigned long long nodes = 0, corner_sum = 0;
     int sub_nodes = 0;
     while (mode != M_END) {
@@ -889,7 +1080,8 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
                 ++r;
                 continue;
             }
-            b10 = b11 = b20 = b21 = res0 = res1 = 0.0;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) b1[k] = b2[k] = res[k] = 0.0;
             done = !act;
             init = true;
             n = 4;
@@ -960,13 +1152,13 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
                 use = true;
             }
         }
-        TV Gs, Gd;
+        TV Gv[NC];
         int nc;
-        node_eval<PASS, TV>(P, S, ch, W, vw, vwp, use ? x : 0.0, i, ri, g, gbase, real_lane, Gs, Gd, nc);
+        node_eval<PASS, TV, NC>(P, S, ch, W, vw, vwp, use ? x : 0.0, i, ri, g, gbase, real_lane, Gv, nc);
         if (mode == M_PROBE) {
             if (g == 0) {
-                P.dbg_out[(2 * bid + 0) * y.ns + i] = (double)Gs;
-                P.dbg_out[(2 * bid + 1) * y.ns + i] = (double)Gd;
+#pragma unroll
+                for (int k = 0; k < NC; ++k) P.dbg_out[(NC * bid + k) * y.ns + i] = (double)Gv[k];
             }
             return;
         }
@@ -976,15 +1168,14 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
         }
         // contributions dv * Γ of every group, in group (= pooled node) order
         double *sc = S.scr;
-        const double v0 = use ? dx * (double)Gs : 0.0, v1 = use ? dx * (double)Gd : 0.0;
-        sc[ri] = v0;
-        sc[SR + ri] = v1;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) sc[k * SR + ri] = use ? dx * (double)Gv[k] : 0.0;
         if (real_lane && i == 0) S.own[g] = use ? own : -1;
         __syncthreads();
         if (mode == M_TAIL) {
             for (int e = 0; e < G && tk + e < 2; ++e) {
-                acc0 += sc[e * (GS + 1) + i];
-                acc1 += sc[SR + e * (GS + 1) + i];
+#pragma unroll
+                for (int k = 0; k < NC; ++k) acc[k] += sc[k * SR + e * (GS + 1) + i];
             }
             __syncthreads();
             tk += G;
@@ -995,12 +1186,16 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
         if (real_lane && !done) {
             for (int e = 0; e < G; ++e) {
                 if (__builtin_amdgcn_readfirstlane(S.own[e]) != g) continue;
-                const double u0 = sc[e * (GS + 1) + i], u1 = sc[SR + e * (GS + 1) + i];
                 if (init && cnt < 3) {
-                    b10 += u0; b11 += u1;
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) b1[k] += sc[k * SR + e * (GS + 1) + i];
                 } else {
-                    if (init && cnt == 3) { b20 = 0.5 * b10; b21 = 0.5 * b11; }
-                    b20 += u0; b21 += u1;
+                    if (init && cnt == 3) {
+#pragma unroll
+                        for (int k = 0; k < NC; ++k) b2[k] = 0.5 * b1[k];
+                    }
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) b2[k] += sc[k * SR + e * (GS + 1) + i];
                 }
                 ++cnt;
                 ++sub_nodes;
@@ -1012,41 +1207,53 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
         }
         if (__ballot(need_check) != 0) {
             // Richardson extrapolation and error estimate (quadrature.jl:72-88); the norms run
-            // over the owner's sites in site order, as the reference's do
-            double r10 = b10 * (-1.0 / 15.0), r11 = b11 * (-1.0 / 15.0);
-            r10 += 16.0 / 15.0 * b20;
-            r11 += 16.0 / 15.0 * b21;
+            // over the owner's sites in site order, components fastest, as the reference's do
+            double rr[NC];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                rr[k] = b1[k] * (-1.0 / 15.0);
+                rr[k] += 16.0 / 15.0 * b2[k];
+            }
             __syncthreads();
-            sc[ri] = r10;
-            sc[SR + ri] = r11;
-            sc[2 * SR + ri] = b20;
-            sc[3 * SR + ri] = b21;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                sc[k * SR + ri] = rr[k];
+                sc[(NC + k) * SR + ri] = b2[k];
+            }
             __syncthreads();
             if (need_check) {
                 double q1 = 0.0, q2 = 0.0, q3 = 0.0;
                 for (int jj = 0; jj < y.ns; ++jj) {
                     const double *e = sc + gbase + jj;
-                    const double x0 = e[0], x1 = e[SR], z0 = e[2 * SR], z1 = e[3 * SR];
-                    q1 += x0 * x0; q1 += x1 * x1;
-                    q2 += z0 * z0; q2 += z1 * z1;
-                    const double d0 = x0 - z0, d1 = x1 - z1;
-                    q3 += d0 * d0; q3 += d1 * d1;
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) {
+                        const double x0 = e[k * SR], z0 = e[(NC + k) * SR];
+                        q1 += x0 * x0;
+                        q2 += z0 * z0;
+                        const double d0 = x0 - z0;
+                        q3 += d0 * d0;
+                    }
                 }
                 const double n1 = sqrt(q1), n2 = sqrt(q2), adiff = sqrt(q3);
                 const double rdiff = adiff / fmax(n1, n2);
-                const double d0 = r10 - b20, d1 = r11 - b21;
                 if (adiff < P.atol || rdiff < P.rtol || n > (1L << 20)) {
-                    res0 = d0 + b20;
-                    res1 = d1 + b21;
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) {
+                        const double d0 = rr[k] - b2[k];
+                        res[k] = d0 + b2[k];
+                    }
                     done = true;
                     lvl = 0;
                     if (P.dbg_sub_nodes && i == 0 && (long)e_idx * nsub + j < P.dbg_cap) {
                         P.dbg_sub_nodes[(long)e_idx * nsub + j] = sub_nodes;
-                        P.dbg_sub_res[(long)e_idx * nsub + j] = res0;
+                        P.dbg_sub_res[(long)e_idx * nsub + j] = res[0];
                     }
                 } else {
-                    b10 = b20; b11 = b21;
-                    b20 = 0.5 * b10; b21 = 0.5 * b11;
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) {
+                        b1[k] = b2[k];
+                        b2[k] = 0.5 * b1[k];
+                    }
                     init = false;
                     h = 0.5 * (b - a) / n;
                     lvl = (int)(n + n / 2);
@@ -1057,15 +1264,15 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
         }
         if (__ballot(real_lane && !done) != 0) continue;
         // round complete: ordered accumulation of its sub-interval results
-        sc[ri] = res0;
-        sc[SR + ri] = res1;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) sc[k * SR + ri] = res[k];
         __syncthreads();
         for (int gg = 0; gg < G; ++gg) {
             const int jj = r * G + gg;
             double ta, tb, ts;
             if (jj < nsub && subinterval(jj, P.eval, ref, logpts, ta, tb, ts)) {
-                acc0 += sc[gg * (GS + 1) + i];
-                acc1 += sc[SR + gg * (GS + 1) + i];
+#pragma unroll
+                for (int k = 0; k < NC; ++k) acc[k] += sc[k * SR + gg * (GS + 1) + i];
             }
         }
         __syncthreads();
@@ -1080,13 +1287,13 @@ __device__ __forceinline__ void vertex_pass_body(const PassArgs<TV> &P, int e_id
         else if (kernel == 3) base = y.q22 + (long)y.ns * (w1 + (long)y.nO * w3);
         else base = y.q3 + (long)y.ns * (w1 + (long)y.nO * (w2 + (long)y.nn * w3));
         double *o = P.out + ch * y.chlen + base + i;
-        o[0] = acc0;
-        o[y.complen] = acc1;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) o[k * y.complen] = acc[k];
     }
     if (P.node_count) {
-        // nodes: one count per group; gathered bytes: every lane's corners x 2 components
+        // nodes: one count per group; gathered bytes: every lane's corners x NC components
         if (real_lane && i == 0 && nodes) atomicAdd(&s_cnt, nodes);
-        if (real_lane && corner_sum) atomicAdd(&s_bytes, corner_sum * 2ull * sizeof(TV));
+        if (real_lane && corner_sum) atomicAdd(&s_bytes, corner_sum * (unsigned long long)NC * sizeof(TV));
         __syncthreads();
         if (lane == 0) {
             atomicAdd(P.node_count, s_cnt);
@@ -1117,9 +1324,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFFRG_PASS_W
     vertex_pass_body<PASS, TV>(tab[c >> 26], bid, c & 0x03ffffffu);
 }
 
+// The U(1)-DM pass (six components): three times the accumulators and gathers of the SU(2)
+// pass, so it takes the register file of two waves per SIMD.
+template <int PASS, typename TV>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_vertex_pass_u1(PassArgs<TV> P) {
+    const int e_idx = blockIdx.x;
+    if (!P.dbg_v && e_idx >= P.n_elems) return;
+    vertex_pass_body<PASS, TV, 6>(P, e_idx, __builtin_amdgcn_readfirstlane(P.dbg_v ? P.dbg_elem : P.elems[e_idx]));
+}
+
 template <int PASS, typename TV>
 hipError_t launch_vertex_pass(const PassArgs<TV> &P, int grid, size_t lds, hipStream_t stream) {
-    hipLaunchKernelGGL((k_vertex_pass<PASS, TV>), dim3(grid), dim3(64), lds, stream, P);
+    if (P.y.nc == 6) hipLaunchKernelGGL((k_vertex_pass_u1<PASS, TV>), dim3(grid), dim3(64), lds, stream, P);
+    else hipLaunchKernelGGL((k_vertex_pass<PASS, TV>), dim3(grid), dim3(64), lds, stream, P);
     return hipGetLastError();
 }
 

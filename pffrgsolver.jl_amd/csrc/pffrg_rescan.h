@@ -10,13 +10,14 @@
 namespace pffrg {
 
 constexpr int SCAN_BLOCKS = 64;  // partial-argmax blocks per (component, channel)
+constexpr int SCAN_ROWS = 18;    // (component, channel) pairs of the largest action: 6 components x 3
 
-// mesh-rescanning scratch: argmax partials (values, positions), 6 x 4 positions, 6 cut sets of
-// n_Omega + 3 n_nu <= 4 MAXN values, the five new meshes
-constexpr size_t SCAN_OFF_PIDX = 6 * SCAN_BLOCKS * 8;
-constexpr size_t SCAN_OFF_IDX = SCAN_OFF_PIDX + 6 * SCAN_BLOCKS * 8;
-constexpr size_t SCAN_OFF_CUTS = SCAN_OFF_IDX + 256;
-constexpr size_t SCAN_OFF_MESH = SCAN_OFF_CUTS + 6 * 4 * MAXN * 8;
+// mesh-rescanning scratch: argmax partials (values, positions), SCAN_ROWS x 4 positions,
+// SCAN_ROWS cut sets of n_Omega + 3 n_nu <= 4 MAXN values, the five new meshes
+constexpr size_t SCAN_OFF_PIDX = SCAN_ROWS * SCAN_BLOCKS * 8;
+constexpr size_t SCAN_OFF_IDX = SCAN_OFF_PIDX + SCAN_ROWS * SCAN_BLOCKS * 8;
+constexpr size_t SCAN_OFF_CUTS = SCAN_OFF_IDX + 512;
+constexpr size_t SCAN_OFF_MESH = SCAN_OFF_CUTS + SCAN_ROWS * 4 * MAXN * 8;
 constexpr size_t SCAN_MEM_BYTES = SCAN_OFF_MESH + 5 * MAXN * 8;
 // what one scan_cuts call uses of it (everything before the new meshes)
 constexpr size_t SCAN_CUTS_BYTES = SCAN_OFF_MESH;
@@ -24,10 +25,10 @@ constexpr size_t SCAN_CUTS_BYTES = SCAN_OFF_MESH;
 struct ScanArgs {
     Sys y;
     const double *gamma;  // the slot's vertex (reference layout)
-    double *pval;         // [6][SCAN_BLOCKS] partial maxima of |q3|
-    long long *pidx;      // [6][SCAN_BLOCKS] their column-major positions
-    int *idx;             // [6][4] (site, w, v, v') of the first maximum
-    double *cuts;         // [6][nO + 3 nn]
+    double *pval;         // [3 nc][SCAN_BLOCKS] partial maxima of |q3|
+    long long *pidx;      // [3 nc][SCAN_BLOCKS] their column-major positions
+    int *idx;             // [3 nc][4] (site, w, v, v') of the first maximum
+    double *cuts;         // [3 nc][nO + 3 nn]
 };
 
 struct ResampleArgs {

@@ -40,7 +40,7 @@ __device__ void block_best(double &v, long long &i, double *sv, long long *si) {
     __syncthreads();
 }
 
-// grid (SCAN_BLOCKS, 6), 256 threads: y = 3*comp + channel
+// grid (SCAN_BLOCKS, 3 nc), 256 threads: y = 3*comp + channel
 __global__ __launch_bounds__(256) void k_scan_partial(ScanArgs A) {
     __shared__ double sv[256];
     __shared__ long long si[256];
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_scan_partial(ScanArgs A) {
     }
 }
 
-// 6 blocks of 256 threads: the maximum over the partials, then the cuts through it
+// 3 nc blocks of 256 threads: the maximum over the partials, then the cuts through it
 __global__ __launch_bounds__(256) void k_scan_final(ScanArgs A) {
     __shared__ double sv[256];
     __shared__ long long si[256];
@@ -134,10 +134,10 @@ __device__ __forceinline__ P1 param_of(double val, const double *m, const double
     return p;
 }
 
-// one thread per output entry: the 2 * complen vertex entries, then the n_sigma Σ entries
+// one thread per output entry: the nc * complen vertex entries, then the n_sigma Σ entries
 __global__ __launch_bounds__(256) void k_resample(ResampleArgs A) {
     const Sys &y = A.y;
-    const long nv = 2 * y.complen;
+    const long nv = (long)y.nc * y.complen;
     const long total = nv + y.nsig;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
         if (t >= nv) {
@@ -214,15 +214,15 @@ __global__ __launch_bounds__(256) void k_resample(ResampleArgs A) {
 }  // namespace
 
 hipError_t launch_scan_cuts(const ScanArgs &A, hipStream_t stream) {
-    hipLaunchKernelGGL(k_scan_partial, dim3(SCAN_BLOCKS, 6), dim3(256), 0, stream, A);
+    hipLaunchKernelGGL(k_scan_partial, dim3(SCAN_BLOCKS, 3 * A.y.nc), dim3(256), 0, stream, A);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_scan_final, dim3(6), dim3(256), 0, stream, A);
+    hipLaunchKernelGGL(k_scan_final, dim3(3 * A.y.nc), dim3(256), 0, stream, A);
     return hipGetLastError();
 }
 
 // Scratch of the allocation-free scan_cuts entry (pffrg_scan_cuts_noalloc): the argmax partials,
-// the maximum's position and the cuts of one call, SCAN_CUTS_BYTES (18.7 KB).  Each GPU holds its
+// the maximum's position and the cuts of one call, SCAN_CUTS_BYTES (54.5 KB).  Each GPU holds its
 // own copy for the life of the process, shared by every context on that GPU: calls that overlap
 // on one GPU race.  Every byte a call reads is written by the same call first, so nothing needs
 // clearing between calls.  Host code must not pass `scratch` itself to a kernel (on the host the
@@ -256,7 +256,7 @@ hipError_t scan_scratch_address(int device, void **ptr) {
 }
 
 hipError_t launch_resample(const ResampleArgs &A, hipStream_t stream) {
-    const long total = 2 * A.y.complen + A.y.nsig;
+    const long total = (long)A.y.nc * A.y.complen + A.y.nsig;
     long blocks = (total + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_resample, dim3((unsigned)blocks), dim3(256), 0, stream, A);

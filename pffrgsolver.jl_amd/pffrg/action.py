@@ -1,10 +1,11 @@
-"""Host-side SU(2) action: self energy, bare couplings and the packed vertex.
+"""Host-side action: self energy, bare couplings and the packed vertex, for the SU(2) action
+(2 components: spin, density; action_lib/action_su2.jl:9-13) and the U(1)-symmetric DM action
+(6 components: xx, zz, DM, dd, zd, dz; action_lib/action_u1_dm.jl:1-11).
 
 Packed vertex layout (identical to the reference's column-major Julia arrays and
-hence to the HDF5 checkpoint datasets, Action/channel.jl:10-35, vertex.jl:10-15,
-action_lib/action_su2.jl:9-13):
+hence to the HDF5 checkpoint datasets, Action/channel.jl:10-35, vertex.jl:10-15):
 
-    gamma = [comp (spin, dens)][channel (s, t, u)][q1 | q2_1 | q2_2 | q3]
+    gamma = [comp][channel (s, t, u)][q1 | q2_1 | q2_2 | q3]
     q1[site, w], q2_*[site, w, v], q3[site, w, v, vp]   (site fastest)
 """
 from __future__ import annotations
@@ -15,7 +16,9 @@ import numpy as np
 
 from .mesh import Mesh
 
-N_COMP = 2
+N_COMP = 2      # SU(2)
+N_COMP_U1 = 6   # U(1)-DM
+SYMMETRIES = {"su2": N_COMP, "u1-dm": N_COMP_U1}
 
 
 def channel_sizes(ns: int, nO: int, nn: int) -> tuple[int, int, int, int]:
@@ -26,28 +29,30 @@ def channel_len(ns: int, nO: int, nn: int) -> int:
     return sum(channel_sizes(ns, nO, nn))
 
 
-def gamma_len(ns: int, nO: int, nn: int) -> int:
-    return N_COMP * 3 * channel_len(ns, nO, nn)
+def gamma_len(ns: int, nO: int, nn: int, nc: int = N_COMP) -> int:
+    return nc * 3 * channel_len(ns, nO, nn)
 
 
 @dataclass
 class Action:
     S: float
     Sigma: np.ndarray   # (n_sigma,)
-    bare: np.ndarray    # (2, ns)
+    bare: np.ndarray    # (nc, ns)
     gamma: np.ndarray   # flat packed vertex
     ns: int
     nO: int
     nn: int
+    nc: int = N_COMP    # vertex components: 2 (SU(2)) or 6 (U(1)-DM)
 
     @classmethod
-    def empty(cls, ns: int, m: Mesh, S: float = 0.5) -> "Action":
+    def empty(cls, ns: int, m: Mesh, S: float = 0.5, nc: int = N_COMP) -> "Action":
         """get_action_empty (Action.jl:564-577): zero Σ, zero bare, zero channels."""
-        return cls(S, np.zeros(m.num_sigma), np.zeros((N_COMP, ns)),
-                   np.zeros(gamma_len(ns, m.num_Omega, m.num_nu)), ns, m.num_Omega, m.num_nu)
+        return cls(S, np.zeros(m.num_sigma), np.zeros((nc, ns)),
+                   np.zeros(gamma_len(ns, m.num_Omega, m.num_nu, nc)), ns, m.num_Omega, m.num_nu, nc)
 
     def copy(self) -> "Action":
-        return Action(self.S, self.Sigma.copy(), self.bare.copy(), self.gamma.copy(), self.ns, self.nO, self.nn)
+        return Action(self.S, self.Sigma.copy(), self.bare.copy(), self.gamma.copy(), self.ns, self.nO, self.nn,
+                      self.nc)
 
     def view(self, comp: int, ch: int, q: str) -> np.ndarray:
         """Column-major (Fortran-order) view of one kernel array, e.g. view(0, 0, 'q3')[site, w, v, vp]."""
@@ -61,7 +66,7 @@ class Action:
 
     def abs_max(self) -> float:
         """get_abs_max over q3 of every channel and component (Action.jl:346-359, channel.jl:406-413)."""
-        return max(float(np.max(np.abs(self.view(c, ch, "q3")))) for c in range(N_COMP) for ch in range(3))
+        return max(float(np.max(np.abs(self.view(c, ch, "q3")))) for c in range(self.nc) for ch in range(3))
 
 
 # ---------------------------------------------------------------- seeded synthetic inputs
@@ -90,13 +95,14 @@ def normals(seed: int, n: int) -> np.ndarray:
 def synthetic_action(ns: int, m: Mesh, bare: np.ndarray, seed: int = 20261203, S: float = 0.5) -> Action:
     """Seeded vertex of the single-RHS benchmark: every channel entry is
     g * 1/(1+|Ω|+|ν|+|ν'|), g ~ N(0,1), filled comp -> channel -> q1,q2_1,q2_2,q3 in
-    column-major order; asymptotic slots use ν, ν' = ν_max.  Σ_k = 0.5 σ_k/(1+σ_k²)."""
-    a = Action.empty(ns, m, S)
+    column-major order; asymptotic slots use ν, ν' = ν_max.  Σ_k = 0.5 σ_k/(1+σ_k²).  The number
+    of components is bare's first dimension."""
+    a = Action.empty(ns, m, S, nc=np.asarray(bare).shape[0])
     a.bare[:] = bare
     nO, nn = m.num_Omega, m.num_nu
     g = normals(seed, a.gamma.size)
     pos = 0
-    for comp in range(N_COMP):
+    for comp in range(a.nc):
         for ch in range(3):
             Om = m.Ot if ch == 1 else m.Os
             nu = m.nt if ch == 1 else m.ns

@@ -18,7 +18,7 @@ import numpy as np
 
 from . import lattice as LT
 from . import mesh as M
-from .action import Action
+from .action import SYMMETRIES, Action
 from .native import FP64, FP32_MIXED, Batch, Context, scan
 
 # slot roles
@@ -37,24 +37,34 @@ class Problem:
     l: LT.Lattice = None
     r: LT.ReducedLattice = None
     bare: np.ndarray = None
+    symmetry: str = "su2"   # "su2" or "u1-dm" (launch!'s symmetry argument)
+
+    @property
+    def nc(self) -> int:
+        """vertex components of the problem's action (2 or 6)."""
+        return SYMMETRIES[self.symmetry]
 
     @classmethod
     def build(cls, name, size, J, initial, num_sigma=25, num_Omega=15, num_nu=10, num_chi=10, S=0.5, A=0.0,
-              normalize=True, model="heisenberg"):
+              normalize=True, model="heisenberg", symmetry=None):
+        if symmetry is None:
+            symmetry = LT.MODEL_SYMMETRY.get(model, "su2")
+        if symmetry != LT.MODEL_SYMMETRY.get(model, symmetry):
+            raise ValueError(f"model {model} needs the {LT.MODEL_SYMMETRY[model]} action, not {symmetry}")
         if normalize:
             J, A = LT.normalize_couplings(J, A)
         l, r = LT.build(name, size, J, model)
         m = M.launch_mesh(initial, num_sigma, num_Omega, num_nu, num_chi)
-        bare = LT.bare_couplings(l, r, S, A)
-        return cls(name, size, J, S, A, m, l, r, bare)
+        bare = LT.bare_couplings(l, r, S, A, symmetry)
+        return cls(name, size, J, S, A, m, l, r, bare, symmetry)
 
     def bare_action(self) -> Action:
-        a = Action.empty(self.r.n_sites, self.mesh, self.S)
+        a = Action.empty(self.r.n_sites, self.mesh, self.S, nc=self.nc)
         a.bare[:] = self.bare
         return a
 
     def context(self, n_slots: int = 9, precision: int = FP64, device: int = 0) -> Context:
-        c = Context(self.r.n_sites, self.mesh, n_slots, self.S, device)
+        c = Context(self.r.n_sites, self.mesh, n_slots, self.S, device, nc=self.nc)
         c.set_lattice(self.r)
         c.set_precision(precision)
         return c
@@ -86,6 +96,13 @@ class MeshParams:
     p_chi: tuple = M.P_CHI
     lins: tuple = M.LINS
     bounds: tuple = M.BOUNDS
+
+
+def chi_components(chi) -> list:
+    """The χ matrices [site, ω] of a measurement: one for SU(2) (chi[site, ω]), χxx, χzz, χxy for
+    U(1)-DM (chi[c, site, ω]) (get_χ_empty, correlation_lib.jl:37-49)."""
+    chi = np.asarray(chi)
+    return [chi] if chi.ndim == 2 else [chi[k] for k in range(chi.shape[0])]
 
 
 def _jl_min(*xs):
@@ -121,7 +138,7 @@ def rescanned_mesh(ctx, L: float, a: int, chi=None, mp: MeshParams = MeshParams(
             return O_lin, v_lin
 
         Os_l, ns_l, Ot_l, nt_l = [], [], [], []
-        for comp in range(2):
+        for comp in range(len(cuts) // 3):
             Os_s, ns_s = scan_channel(3 * comp + 0, mp.p_Os, mp.p_ns, m_old.Os, m_old.ns)
             Os_u, ns_u = scan_channel(3 * comp + 2, mp.p_Os, mp.p_ns, m_old.Os, m_old.ns)
             Os_l.append(_jl_min(Os_s, Os_u))
@@ -133,10 +150,12 @@ def rescanned_mesh(ctx, L: float, a: int, chi=None, mp: MeshParams = MeshParams(
         Ot_lin, nt_lin = _jl_min(*Ot_l), _jl_min(*nt_l)
         if chi is None:
             raise ValueError("rescanning below lins[0] needs the correlations χ")
-        chi = np.asarray(chi)
-        site = int(np.argmax(np.abs(chi).T)) % chi.shape[0]  # first maximum in column-major order
         pc = mp.p_chi
-        chi_lin = scan(m_old.chi, chi[site, :], (pc[0], pc[1], pc[2], pc[3] * L, pc[4] * L))
+        chi_lins = []
+        for c in chi_components(chi):
+            site = int(np.argmax(np.abs(c).T)) % c.shape[0]  # first maximum in column-major order
+            chi_lins.append(scan(m_old.chi, c[site, :], (pc[0], pc[1], pc[2], pc[3] * L, pc[4] * L)))
+        chi_lin = _jl_min(*chi_lins)
     up = max(L, bounds[0])
     return M.Mesh(
         M.get_mesh(sig_lin, bounds[1] * up, m_old.num_sigma - 1, mp.p_sigma[0]) * mp.p_sigma[2],
@@ -197,7 +216,7 @@ class BS3Flow:
             ctx.set_mesh(prob.mesh)  # a0 lives on the problem's initial meshes
         ctx.upload(A, a0)
         # da, a_stage, a_err start empty; a_inter carries the bare couplings (launcher_1l.jl:39-44)
-        zero = Action.empty(prob.r.n_sites, prob.mesh, prob.S)
+        zero = Action.empty(prob.r.n_sites, prob.mesh, prob.S, nc=prob.nc)
         for s in (DA, A_STAGE, A_ERR, DA_L, DA_C, DA_TEMP, DA_SIGMA):
             ctx.upload(s, zero)
         ctx.upload(A_INTER, prob.bare_action())
@@ -360,13 +379,16 @@ class BS3Flow:
         return chi
 
     def record_chi(self, chi) -> bool:
-        """Store χ of the step; True when the static on-site correlation stopped growing
-        (χ/χ_prev < 0.995, the reference's stop rule, Launcher.jl:33-35)."""
+        """Store χ of the step; True when the static on-site correlation of the component with
+        the largest |χ| stopped growing (χ/χ_prev < 0.995, the reference's stop rule,
+        Launcher.jl:33-35)."""
         prev = self.res.chi[-1] if self.res.chi else None
         self.res.chi.append(chi)
-        if prev is None or prev[0, 0] == 0.0:
+        cs = chi_components(chi)
+        k = int(np.argmax([np.max(np.abs(c)) for c in cs]))
+        if prev is None or chi_components(prev)[k][0, 0] == 0.0:
             return False
-        return not (chi[0, 0] / prev[0, 0] >= 0.995)
+        return not (cs[k][0, 0] / chi_components(prev)[k][0, 0] >= 0.995)
 
     def result(self) -> FlowResult:
         self.ctx.synchronize()
@@ -473,7 +495,7 @@ def launch_parquet(ctx: Context, L: float, a0: Action, max_iter=50, min_eval=10,
     t0 = time.time()
     res = ParquetResult()
     ctx.upload(slot_a, a0)
-    zero = Action.empty(a0.ns, ctx.m, a0.S)
+    zero = Action.empty(a0.ns, ctx.m, a0.S, nc=a0.nc)
     ctx.upload(slot_ap, zero)
     ctx.upload(slot_err, zero)
     beta = min(L, 1.0)

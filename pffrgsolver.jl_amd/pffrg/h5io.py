@@ -20,7 +20,7 @@ from pathlib import Path
 
 import numpy as np
 
-from .action import N_COMP, Action
+from .action import Action
 from .mesh import Mesh
 
 LIB_PATH = Path(os.environ.get("PFFRG_H5_LIB", Path(__file__).resolve().parent / "_lib" / "libpffrg_h5.so"))
@@ -194,6 +194,10 @@ class H5File:
 _CH = ("ch_s", "ch_t", "ch_u")
 _Q = ("q1", "q2_1", "q2_2", "q3")
 _COMP = ("spin", "dens")
+# U(1)-DM vertex components as checkpoint_u1_dm.jl names them
+_COMP_U1 = ("Γxx", "Γzz", "ΓDM", "Γdd", "Γzd", "Γdz")
+# χ components of save_χ! (Observable/disk.jl:19-24)
+CHI_LABELS = {"su2": ("diag",), "u1-dm": ("xx", "zz", "xy")}
 
 
 def save_vertex(f: H5File, label: str, a: Action, comp: int):
@@ -213,35 +217,41 @@ def read_vertex(f: H5File, label: str, a: Action, comp: int):
 
 
 def checkpoint(f: H5File, L: float, dL: float, m: Mesh, a: Action):
-    """checkpoint! (checkpoint_su2.jl:2-39)."""
+    """checkpoint! (checkpoint_su2.jl:2-39; checkpoint_u1_dm.jl:2-37 for the six-component
+    action, which stores no spin length)."""
     k = julia_float(L)
     f.write(f"dΛ/{k}", float(dL))
     for name, v in (("σ", m.sigma), ("Ωs", m.Os), ("νs", m.ns), ("Ωt", m.Ot), ("νt", m.nt), ("χ", m.chi)):
         f.write(f"{name}/{k}", v)
-    if not f.haskey("S"):
+    u1 = a.nc == 6
+    if not u1 and not f.haskey("S"):
         f.write("S", float(a.S))
     if not f.haskey("symmetry"):
-        f.write("symmetry", "su2")
+        f.write("symmetry", "u1-dm" if u1 else "su2")
     f.write(f"a/{k}/Σ", a.Sigma)
-    for c in range(N_COMP):
-        save_vertex(f, f"a/{k}/Γ/{_COMP[c]}", a, c)
+    names = _COMP_U1 if u1 else _COMP
+    for c in range(a.nc):
+        save_vertex(f, f"a/{k}/Γ/{names[c]}", a, c)
 
 
 def read_checkpoint(f: H5File, L: float) -> tuple[float, float, Mesh, Action]:
-    """read_checkpoint_su2 (checkpoint_su2.jl:42-79): the checkpoint at the cutoff nearest to L
-    (first of equally near ones in key order, as argmin does)."""
+    """read_checkpoint (Action.jl:589-603) with read_checkpoint_su2 (checkpoint_su2.jl:42-79) or
+    read_checkpoint_u1_dm (checkpoint_u1_dm.jl:40-78) by the file's symmetry: the checkpoint at the
+    cutoff nearest to L (first of equally near ones in key order, as argmin does)."""
     names = f.keys("σ")
     cut = [float(n) for n in names]
     i = int(np.argmin(np.abs(np.array(cut) - L)))
     k, Lc = names[i], cut[i]
     dL = float(f.read(f"dΛ/{k}"))
     m = Mesh(*(np.asarray(f.read(f"{n}/{k}"), dtype=np.float64) for n in ("σ", "Ωs", "νs", "Ωt", "νt", "χ")))
-    S = float(f.read("S"))
-    ns = len(f.read(f"a/{k}/Γ/spin/bare"))
-    a = Action.empty(ns, m, S)
+    u1 = f.haskey("symmetry") and f.read("symmetry") == "u1-dm"
+    comps = _COMP_U1 if u1 else _COMP
+    S = 0.5 if u1 else float(f.read("S"))
+    ns = len(f.read(f"a/{k}/Γ/{comps[0]}/bare"))
+    a = Action.empty(ns, m, S, nc=len(comps))
     a.Sigma[:] = f.read(f"a/{k}/Σ")
-    for c in range(N_COMP):
-        read_vertex(f, f"a/{k}/Γ/{_COMP[c]}", a, c)
+    for c in range(len(comps)):
+        read_vertex(f, f"a/{k}/Γ/{comps[c]}", a, c)
     return Lc, dL, m, a
 
 
@@ -253,22 +263,27 @@ def save_self(f: H5File, L: float, m: Mesh, Sigma: np.ndarray):
 
 
 def save_chi(f: H5File, L: float, symmetry: str, m: Mesh, chi: np.ndarray):
-    """save_χ! (Observable/disk.jl:2-29) for the SU(2) symmetry: chi[site, ω]."""
-    if symmetry != "su2":
-        raise NotImplementedError("only the SU(2) symmetry is built")
+    """save_χ! (Observable/disk.jl:2-29): SU(2) chi[site, ω] as "diag"; U(1)-DM chi[c, site, ω] as
+    "xx", "zz", "xy"."""
+    if symmetry not in CHI_LABELS:
+        raise ValueError(f"unknown symmetry {symmetry}")
     if not f.haskey("symmetry"):
         f.write("symmetry", symmetry)
     k = julia_float(L)
     f.write(f"χ/{k}/mesh", m.chi)
-    f.write(f"χ/{k}/diag", np.asarray(chi, dtype=np.float64))
+    chi = np.asarray(chi, dtype=np.float64)
+    parts = [chi] if symmetry == "su2" else [chi[c] for c in range(3)]
+    for label, x in zip(CHI_LABELS[symmetry], parts):
+        f.write(f"χ/{k}/{label}", np.ascontiguousarray(x))
 
 
-def read_chi(f: H5File, L: float) -> tuple[float, np.ndarray]:
-    """read_χ (Observable/disk.jl:63-87) for "diag": the nearest stored cutoff and χ[site, ω]."""
+def read_chi(f: H5File, L: float, label: str = "diag") -> tuple[float, np.ndarray]:
+    """read_χ (Observable/disk.jl:63-87): the nearest stored cutoff and χ[site, ω] of one label
+    ("diag" for SU(2); "xx", "zz", "xy" for U(1)-DM)."""
     names = f.keys("χ")
     cut = np.array([float(n) for n in names])
     i = int(np.argmin(np.abs(cut - L)))
-    return float(cut[i]), f.read(f"χ/{names[i]}/diag")
+    return float(cut[i]), f.read(f"χ/{names[i]}/{label}")
 
 
 def save_lattice(f: H5File, name: str, size: int, model: str, euclidean: bool, J):
@@ -322,7 +337,7 @@ class Measurement:
         with H5File(self.obs_file, "cw") as obs, H5File(self.cp_file, "cw") as cp:
             if not obs.haskey(f"χ/{julia_float(L)}"):
                 save_self(obs, L, m, Sigma)
-                save_chi(obs, L, "su2", m, chi)
+                save_chi(obs, L, "su2" if np.asarray(chi).ndim == 2 else "u1-dm", m, chi)
             h0 = (time.time() - self.t0) / 3600.0
             if self.wt - h0 > 0.5:
                 if (time.time() - self.t) / 3600.0 >= ct:

@@ -341,12 +341,58 @@ def init_model_pyrochlore_breathing_c3(J: list[list[float]], l: Lattice) -> None
                 _add_shell(l, i, n, Jn, nbs)
 
 
-# SU(2) models of init_model! (build.jl:183-202); triangular-dm-c3 needs the U(1)-DM action, not built
+def grow_test_sites(l: Lattice, metric: int) -> None:
+    """grow_test_sites! (build.jl:87-135), bond metric: add the sites within `metric` bonds of
+    the origin to the symmetry test set if it is smaller."""
+    current = max(get_metric(l.test_sites[0], s, l.uc) for s in l.test_sites)
+    if current < metric:
+        keys = {_key(get_vec(s, l.uc)) for s in l.test_sites}
+        for s in get_sites(metric, l.uc, False):
+            k = _key(get_vec(s, l.uc))
+            if k not in keys:
+                keys.add(k)
+                l.test_sites.append(s)
+
+
+def init_model_triangular_dm_c3(J: list[list[float]], l: Lattice) -> None:
+    """Triangular lattice with out-of-plane DM interaction (model_triangular_dm_c3.jl:14-73):
+    J[n] = [Jxx, Jzz, Jxy] on the n-th neighbour shell (n <= 3); the DM coupling Jxy changes
+    sign between bonds that are not related by a C3 rotation of the shell's first bond."""
+    assert l.name == "triangular", "Model requires triangular lattice."
+    assert len(J) <= 3, "Model initialization only works up to third-nearest neighbors."
+    for n, Jn in enumerate(J, start=1):
+        assert len(Jn) == 3, f"Jxx, Jzz and Jxy all need to be specified for J[{n}]."
+    max_nbs = get_nbs(len(J), 0, l.vecs)
+    grow_test_sites(l, max(get_metric(l.ints[0], l.ints[j], l.uc) for j in max_nbs))
+    ref_vecs: list[np.ndarray] = []
+    for i in range(len(l.ints)):
+        for n, Jn in enumerate(J, start=1):
+            nbs = get_nbs(n, i, l.vecs)
+            Jxx, Jzz, Jxy = Jn
+            if len(ref_vecs) < n:
+                ref_vecs.append(l.vecs[nbs[0]] - l.vecs[i])
+            ref = ref_vecs[n - 1]
+            for j in nbs:
+                vec = l.vecs[j] - l.vecs[i]
+                c = round(float(np.dot(ref, vec) / (np.linalg.norm(ref) * np.linalg.norm(vec))), 8)
+                sg = 1.0 if math.fmod(math.acos(c), 2.0 * math.pi / 3.0) < 1e-8 else -1.0
+                l.bonds[i, j, 0, 1] += sg * Jxy
+                l.bonds[i, j, 1, 0] -= sg * Jxy
+                l.bonds[i, j, 0, 0] += Jxx
+                l.bonds[i, j, 1, 1] += Jxx
+                l.bonds[i, j, 2, 2] += Jzz
+
+
+# models of init_model! (build.jl:183-202): SU(2) models and the U(1)-DM model
 MODELS = {
     "heisenberg": init_model_heisenberg,
     "breathing": init_model_breathing,
     "pyrochlore-breathing-c3": init_model_pyrochlore_breathing_c3,
+    "triangular-dm-c3": init_model_triangular_dm_c3,
 }
+# the action each model needs (model_avail, build.jl:147-170)
+MODEL_SYMMETRY = {"heisenberg": "su2", "breathing": "su2", "pyrochlore-breathing-c3": "su2",
+                  "triangular-dm-c3": "u1-dm"}
 
 
 def _are_equal(b1, b2) -> bool:
@@ -574,7 +620,7 @@ class ReducedLattice:
 def get_reduced_lattice(model: str, J: list[list[float]], l: Lattice) -> ReducedLattice:
     """reduced.jl:716-759."""
     if model not in MODELS:
-        raise NotImplementedError(f"Model {model} unknown or not provided (SU(2) models: {', '.join(MODELS)}).")
+        raise NotImplementedError(f"Model {model} unknown or not provided (models: {', '.join(MODELS)}).")
     MODELS[model](J, l)
     reduced = get_reduced(l)
     irreducible = np.array(list(dict.fromkeys(reduced.tolist())), dtype=np.int64)
@@ -597,9 +643,21 @@ def get_reduced_lattice(model: str, J: list[list[float]], l: Lattice) -> Reduced
                           l.vecs[irreducible], overlap, mult, exchange, project)
 
 
-def bare_couplings(l: Lattice, r: ReducedLattice, S: float = 0.5, A: float = 0.0) -> np.ndarray:
-    """Bare spin vertex per irreducible site: J_xx/4/(2S) (+ level repulsion on site 0)
-    (action_su2.jl:35-66).  Returns (2, n_sites): [spin, density]."""
+def bare_couplings(l: Lattice, r: ReducedLattice, S: float = 0.5, A: float = 0.0,
+                   symmetry: str = "su2") -> np.ndarray:
+    """Bare vertex per irreducible site from the bond to the origin.  SU(2): spin J_xx/4/(2S)
+    (+ level repulsion A/4/(2S) on site 0) (action_su2.jl:35-66), returns (2, n_sites) [spin,
+    density].  U(1)-DM: xx J_xx/4, zz J_zz/4, DM J_xy/4 (+ A/4 on xx and zz of site 0)
+    (action_u1_dm.jl:32-75), returns (6, n_sites) [xx, zz, DM, dd, zd, dz]."""
+    if symmetry == "u1-dm":
+        bare = np.zeros((6, r.n_sites))
+        for i, irr in enumerate(r.irreducible):
+            bare[0, i] = l.bonds[0, irr, 0, 0] / 4.0
+            bare[1, i] = l.bonds[0, irr, 2, 2] / 4.0
+            bare[2, i] = l.bonds[0, irr, 0, 1] / 4.0
+        bare[0, 0] += A / 4.0
+        bare[1, 0] += A / 4.0
+        return bare
     bare = np.zeros((2, r.n_sites))
     for i, irr in enumerate(r.irreducible):
         bare[0, i] = l.bonds[0, irr, 0, 0] / 4.0 / (2.0 * S)

@@ -18,6 +18,9 @@ LIB_PATH = Path(os.environ.get("PFFRG_LIB", Path(__file__).resolve().parent / "_
 FP64 = 0
 FP32_MIXED = 1
 FP32_ELEMENT = 2
+# action symmetries of pffrg_create_symmetry (get_action_empty, Action.jl:564-577)
+SU2 = 0
+U1_DM = 1
 
 _lib = None
 
@@ -39,6 +42,8 @@ _d = C.c_double
 
 _SIGS = {
     "pffrg_create": ([C.POINTER(Dims), C.POINTER(_P)], C.c_int),
+    "pffrg_create_symmetry": ([C.POINTER(Dims), _i, C.POINTER(_P)], C.c_int),
+    "pffrg_n_components": ([_P], C.c_int32),
     "pffrg_destroy": ([_P], None),
     "pffrg_last_error": ([_P], C.c_char_p),
     "pffrg_set_mesh": ([_P, _D, _D, _D, _D, _D], C.c_int),
@@ -151,16 +156,20 @@ def _tol(t) -> np.ndarray:
 
 
 class Context:
-    """One device context: meshes, lattice tables and n_slots action slots on the GPU."""
+    """One device context: meshes, lattice tables and n_slots action slots on the GPU, for the
+    SU(2) action (nc = 2) or the U(1)-DM action (nc = 6)."""
 
-    def __init__(self, n_sites: int, m, n_slots: int, S: float = 0.5, device: int = 0):
+    def __init__(self, n_sites: int, m, n_slots: int, S: float = 0.5, device: int = 0, nc: int = 2):
         self.lib = load()
         self.m = m
         self.ns = n_sites
         self.S = S
+        if nc not in (2, 6):
+            raise PffrgError(f"vertex components must be 2 (SU(2)) or 6 (U(1)-DM), not {nc}")
+        self.nc = nc
         d = Dims(n_sites, m.num_sigma, m.num_Omega, m.num_nu, S, n_slots, device)
         h = _P()
-        rc = self.lib.pffrg_create(C.byref(d), C.byref(h))
+        rc = self.lib.pffrg_create_symmetry(C.byref(d), SU2 if nc == 2 else U1_DM, C.byref(h))
         if rc != 0 or not h:
             raise PffrgError(f"pffrg_create failed with code {rc} (no HIP device, or dims out of range)")
         self.h = h
@@ -210,15 +219,17 @@ class Context:
 
     # ---------------------------------------------------------------- transfer
     def upload(self, slot: int, a: Action):
+        if a.nc != self.nc:
+            raise PffrgError(f"upload: action of {a.nc} components into a context of {self.nc}")
         self._ck(self.lib.pffrg_upload_action(self.h, slot, _dp(np.ascontiguousarray(a.Sigma)),
                                               _dp(np.ascontiguousarray(a.bare.reshape(-1))),
                                               _dp(np.ascontiguousarray(a.gamma))), "upload_action")
 
     def download(self, slot: int) -> Action:
-        a = Action.empty(self.ns, self.m, self.S)
-        bare = np.zeros(2 * self.ns)
+        a = Action.empty(self.ns, self.m, self.S, nc=self.nc)
+        bare = np.zeros(self.nc * self.ns)
         self._ck(self.lib.pffrg_download_action(self.h, slot, _dp(a.Sigma), _dp(bare), _dp(a.gamma)), "download_action")
-        a.bare[:] = bare.reshape(2, self.ns)
+        a.bare[:] = bare.reshape(self.nc, self.ns)
         return a
 
     def get_sigma(self, slot: int) -> np.ndarray:
@@ -256,23 +267,28 @@ class Context:
     # ---------------------------------------------------------------- algebra
     def compute_chi(self, L, a, tol=(1e-8, 1e-4), chi_mesh=None, with_regions=False):
         """compute_χ! (correlation_su2.jl:58-86): chi[site, k] over the irreducible sites and
-        the χ frequencies (the mesh's χ by default)."""
+        the χ frequencies (the mesh's χ by default); for the U(1)-DM action (correlation_u1_dm.jl:
+        138-190) chi[c, site, k] with c = χxx, χzz, χxy."""
         cm = np.ascontiguousarray(self.m.chi if chi_mesh is None else chi_mesh, dtype=np.float64)
-        out = np.zeros(self.ns * len(cm))
-        nreg = np.zeros(9 * self.ns * len(cm), dtype=np.int32)
+        ncc = 1 if self.nc == 2 else 3
+        out = np.zeros(ncc * self.ns * len(cm))
+        nreg = np.zeros(ncc * 9 * self.ns * len(cm), dtype=np.int32)
         self._ck(self.lib.pffrg_compute_chi(self.h, L, a, _dp(cm), len(cm), _dp(_tol(tol)), _dp(out),
                                             nreg.ctypes.data_as(C.POINTER(C.c_int32))), "compute_chi")
-        chi = out.reshape(len(cm), self.ns).T.copy()
+        chi = out.reshape(ncc, len(cm), self.ns).transpose(0, 2, 1).copy()
+        if ncc == 1:
+            chi = chi[0]
         return (chi, nreg) if with_regions else chi
 
     def scan_cuts(self, a):
         """scan_channel's argmax positions idx[3*comp + ch] = (site, w, v, v') and the cuts through
         them, cuts[3*comp + ch] = (Ω cut, three ν cuts) (Action.jl:449-476)."""
-        idx = np.zeros(24, dtype=np.int32)
-        cuts = np.zeros(6 * (self.m.num_Omega + 3 * self.m.num_nu))
+        k = 3 * self.nc
+        idx = np.zeros(4 * k, dtype=np.int32)
+        cuts = np.zeros(k * (self.m.num_Omega + 3 * self.m.num_nu))
         self._ck(self.lib.pffrg_scan_cuts(self.h, a, idx.ctypes.data_as(C.POINTER(C.c_int32)), _dp(cuts)),
                  "scan_cuts")
-        return idx.reshape(6, 4), cuts.reshape(6, -1)
+        return idx.reshape(k, 4), cuts.reshape(k, -1)
 
     def resample(self, src, dst, m_new):
         """resample_from_to's interpolation (Action.jl:543-553): slot src onto the meshes m_new into
@@ -324,7 +340,7 @@ class Context:
 
     def probe_integrand(self, pass_, L, a, da, ch, kernel, w1, w2, w3, v) -> np.ndarray:
         v = np.ascontiguousarray(v, dtype=np.float64)
-        out = np.zeros((len(v), 2, self.ns))
+        out = np.zeros((len(v), self.nc, self.ns))
         self._ck(self.lib.pffrg_probe_integrand(self.h, pass_, L, a, da, ch, kernel, w1, w2, w3, len(v), _dp(v),
                                                 out.ctypes.data_as(_D)), "probe_integrand")
         return out

@@ -12,14 +12,15 @@ from pffrg.action import Action
 class OracleContext:
     def __init__(self, prob, n_slots: int = 9):
         self.r = prob.r
-        self.orc = O.Oracle(prob.r, prob.mesh)
+        self.nc = getattr(prob, "nc", 2)
+        self.orc = O.Oracle(prob.r, prob.mesh, prob.S, self.nc)
         self.m = prob.mesh
         self.ns = prob.r.n_sites
         self.S = prob.S
-        self.slots = [Action.empty(self.ns, self.m, self.S) for _ in range(n_slots)]
+        self.slots = [Action.empty(self.ns, self.m, self.S, nc=self.nc) for _ in range(n_slots)]
 
     def set_mesh(self, m):
-        self.orc = O.Oracle(self.r, m, self.S)
+        self.orc = O.Oracle(self.r, m, self.S, self.nc)
         self.m = m
 
     def get_sigma(self, s):
@@ -29,7 +30,7 @@ class OracleContext:
         return self.orc.scan_cuts(self.slots[a])
 
     def resample(self, src, dst, m_new):
-        out = Action.empty(self.ns, m_new, self.S)
+        out = Action.empty(self.ns, m_new, self.S, nc=self.nc)
         self.orc.resample(self.slots[src], m_new, out)
         self.slots[dst] = out
         self.set_mesh(m_new)
@@ -52,7 +53,7 @@ class OracleContext:
 
     def abs_max(self, s) -> float:
         a = self.slots[s]
-        return max(float(np.max(np.abs(a.view(c, ch, "q3")))) for c in range(2) for ch in range(3))
+        return max(float(np.max(np.abs(a.view(c, ch, "q3")))) for c in range(self.nc) for ch in range(3))
 
     def synchronize(self):
         pass

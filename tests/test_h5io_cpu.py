@@ -70,6 +70,35 @@ def test_checkpoint_round_trip_and_nearest_cutoff(tmp_path):
     np.testing.assert_array_equal(b.bare, ref.bare)
 
 
+def test_u1_dm_checkpoint_and_observables(tmp_path):
+    # checkpoint_u1_dm.jl:2-78: six vertices Γxx .. Γdz, symmetry "u1-dm", no spin length;
+    # save_χ! (Observable/disk.jl:19-24): χ/Λ/xx, zz, xy
+    from pffrg import flow as F
+    prob = F.Problem.build("triangular", 2, [[1.0, 0.6, 0.4]], initial=1.0, num_sigma=6, num_Omega=4, num_nu=4,
+                           normalize=False, model="triangular-dm-c3")
+    m = prob.mesh
+    a = A.synthetic_action(prob.r.n_sites, m, prob.bare, seed=5)
+    assert a.nc == 6
+    p, q = tmp_path / "cp_u1.h5", tmp_path / "obs_u1.h5"
+    chi = np.arange(3 * prob.r.n_sites * m.num_chi, dtype=np.float64).reshape(3, prob.r.n_sites, m.num_chi)
+    with H.H5File(p, "cw") as f:
+        H.checkpoint(f, 0.7, 0.07, m, a)
+    with H.H5File(q, "cw") as f:
+        H.save_chi(f, 0.7, "u1-dm", m, chi)
+    with H.H5File(q, "r") as f:
+        assert f.read("symmetry") == "u1-dm"
+        for k, label in enumerate(("xx", "zz", "xy")):
+            np.testing.assert_array_equal(H.read_chi(f, 0.7, label)[1], chi[k])
+    with H.H5File(p, "r") as f:
+        assert f.read("symmetry") == "u1-dm" and not f.haskey("S")
+        assert sorted(f.keys("a/0.7/Γ")) == sorted(["Γxx", "Γzz", "ΓDM", "Γdd", "Γzd", "Γdz"])
+        Lc, dL, m2, b = H.read_checkpoint(f, 0.6)
+    assert (Lc, dL, b.nc) == (0.7, 0.07, 6)
+    np.testing.assert_array_equal(b.gamma, a.gamma)
+    np.testing.assert_array_equal(b.bare, a.bare)
+    np.testing.assert_array_equal(b.Sigma, a.Sigma)
+
+
 @pytest.mark.skipif(H5DUMP is None, reason="h5dump not available")
 def test_layout_as_hdf5_jl_writes_it(tmp_path):
     m, a = _action()

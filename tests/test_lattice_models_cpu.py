@@ -66,4 +66,9 @@ def test_model_errors():
     with pytest.raises(AssertionError):
         LT.build("kagome", 2, [[1.0]], "breathing")
     with pytest.raises(NotImplementedError):
+        LT.build("triangular", 2, [[1.0]], "no-such-model")
+    # model_triangular_dm_c3.jl:21-26: triangular lattice only, three couplings per shell, n <= 3
+    with pytest.raises(AssertionError):
         LT.build("triangular", 2, [[1.0]], "triangular-dm-c3")
+    with pytest.raises(AssertionError):
+        LT.build("square", 2, [[1.0, 0.5, 0.2]], "triangular-dm-c3")

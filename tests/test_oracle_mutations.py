@@ -89,11 +89,11 @@ MASK_MUTATIONS = [
      "case P_BSE: gamma_vec(y, d, A, bs1, bt1, bu1, Mask{false, true, true}, t1.data());",
      "case P_BSE: gamma_vec(y, d, A, bs1, bt1, bu1, Mask{true, true, true}, t1.data());"),
     ("test_reduced_bubble_left_vertex_is_bare",
-     "double v1s = A.bare[i], v1d = A.bare[ns + i];\n                    double v2s = t2[i], v2d = t2[ns + i];",
-     "double v1s = t2[i], v1d = t2[ns + i];\n                    double v2s = t2[i], v2d = t2[ns + i];"),
+     "for (int k = 0; k < nc; ++k) a[k] = A.bare[k * ns + i];",
+     "for (int k = 0; k < nc; ++k) a[k] = t2[k * ns + i];"),
     ("test_sde_self_energy_drops_the_t_channel",  # the dΣ kernel's local t buffer copied into the SDE
-     "double val = 3.0 * gamma_comp(*y, d, *a2, 0, 0, b2s, e, e, SONLY)",
-     "double val = 3.0 * gamma_comp(*y, d, *a2, 0, 0, b2s, buffer_t(*y, v - w, 0.5 * (v + w), 0.5 * (v + w)), e, ALL)"),
+     "double val = sigma_loop2(*y, d, *a2, b2s, e, e, b1s, e, e, SONLY, SONLY);",
+     "double val = sigma_loop2(*y, d, *a2, b2s, buffer_t(*y, v - w, 0.5 * (v + w), 0.5 * (v + w)), e, b1s, e, e, ALL, SONLY);"),
     ("test_first_sigma_correction_masks",
      "sigma_loop(*y, d, *daS, b, Mask{true, true, false}, Mask{true, false, true});",
      "sigma_loop(*y, d, *daS, b, Mask{true, true, false}, Mask{true, true, true});"),
@@ -121,12 +121,13 @@ SIGMA_MUTATIONS = [
 # Planted errors in symmetrize_add_to (action_su2.jl:176-250): an unswapped q2 and a site
 # exchange in the u channel must fail the laws of tests/test_oracle_symmetry_cpu.py.
 SYMMETRY_MUTATIONS = [
-    ("test_symmetrize_add_to_laws", "double a21 = l[d.q21 + k] + l[d.q22 + kx];",
-     "double a21 = l[d.q21 + k] + l[d.q21 + kx];"),
+    ("test_symmetrize_add_to_laws", "double a21 = pair(l[d.q21 + k], r[d.q22 + kx]);",
+     "double a21 = pair(l[d.q21 + k], r[d.q21 + kx]);"),
     ("test_symmetrize_add_to_laws", "auto ex = [&](int i) { return (ch == 2) ? i : (int)y.exchange[i]; };",
      "auto ex = [&](int i) { return (int)y.exchange[i]; };"),
-    ("test_symmetrize_add_to_laws", "o[d.q1 + i + (long)d.ns * w] += l[d.q1 + i + (long)d.ns * w] + l[d.q1 + ex(i) + (long)d.ns * w];",
-     "o[d.q1 + i + (long)d.ns * w] += l[d.q1 + i + (long)d.ns * w] + l[d.q1 + i + (long)d.ns * w];"),
+    ("test_symmetrize_add_to_laws",
+     "o[d.q1 + i + (long)d.ns * w] += pair(l[d.q1 + i + (long)d.ns * w], r[d.q1 + ex(i) + (long)d.ns * w]);",
+     "o[d.q1 + i + (long)d.ns * w] += pair(l[d.q1 + i + (long)d.ns * w], r[d.q1 + i + (long)d.ns * w]);"),
 ]
 
 # Planted errors in the bubble and in the action each side of the left, central and reduced
@@ -140,8 +141,38 @@ PASS_MUTATIONS = [
     ("test_bare_only_pass_closed_form",
      "gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, D, bs2, bt2, bu2, Mask{false, true, false}, t2.data());",
      "gamma_vec(y, d, A, bs1, bt1, bu1, ALL, t1.data()); gamma_vec(y, d, A, bs2, bt2, bu2, Mask{false, true, false}, t2.data());"),
-    ("test_bare_only_reduced_closed_form", "double v1s = A.bare[i], v1d = A.bare[ns + i];",
-     "double v1s = 0.5 * A.bare[i], v1d = A.bare[ns + i];"),
+    ("test_bare_only_reduced_closed_form", "for (int k = 0; k < nc; ++k) a[k] = A.bare[k * ns + i];",
+     "for (int k = 0; k < nc; ++k) a[k] = 0.5 * A.bare[k * ns + i];"),
+]
+
+
+# Planted errors in the U(1)-DM flags, products, self-energy loop, correlations and symmetrisation:
+# each must fail its pin in tests/test_oracle_u1_cpu.py.
+U1_MUTATIONS = [
+    ('test_katanin_pass_preserves_the_u1_vertex_symmetry',
+     '        if (comp == 2) sgn *= -1.0;\n        else if (comp == 4) src = 5;',
+     '        if (comp == 2) sgn *= 1.0;\n        else if (comp == 4) src = 5;'),
+    ('test_su2_embedding_reproduces_the_su2_oracle',
+     'g[ZZ] = a[ZZ] * b[DD] + a[DD] * b[ZZ] - a[DZ] * b[ZD] - 2.0 * a[XX] * b[XX]',
+     'g[ZZ] = a[ZZ] * b[DD] + a[DD] * b[ZZ] - a[DZ] * b[ZD] + 2.0 * a[XX] * b[XX]'),
+    ('test_pi_rotation_commutes_with_every_right_hand_side',
+     'g[DM] = a[DM] * c4[DD] - a[XX] * c4[DZ]',
+     'g[DM] = a[DM] * c4[DD] - a[XX] * c4[DD]'),
+    ('test_pi_rotation_commutes_with_every_right_hand_side',
+     'G[ZD] += -p * (-2.0) * m * (a[ZD] * b[DD] + a[ZZ] * b[ZD]);',
+     'G[ZD] += -p * (-2.0) * m * (a[ZD] * b[DD] + a[ZZ] * b[DD]);'),
+    ('test_su2_embedding_reproduces_the_su2_oracle',
+     'val -= 2.0 * y.mult[j] * gamma_comp(y, d, a, DD, j, b1s, b1t, b1u, mn);',
+     'val -= 2.0 * y.mult[j] * gamma_comp(y, d, a, ZZ, j, b1s, b1t, b1u, mn);'),
+    ('test_su2_embedding_reproduces_the_su2_oracle',
+     'val -= (2.0 * vxx - vzz - vdd) / (2.0 * tp2);',
+     'val -= (vxx - vzz - vdd) / (2.0 * tp2);'),
+    ('test_u1_symmetrize_add_to_laws',
+     '    return (comp == ZD || comp == DZ) ? -1.0 : 1.0;\n}',
+     '    return 1.0;\n}'),
+    ('test_interpolation_and_flags_of_every_component',
+     '    if ((comp == 3 || comp == 4) && b.sgn_nu) sgn *= -1.0;',
+     '    if ((comp == 3 || comp == 5) && b.sgn_nu) sgn *= -1.0;'),
 ]
 
 
@@ -179,3 +210,8 @@ def test_symmetry_law_detects_planted_mistake(test, orig, bad, tmp_path):
 @pytest.mark.parametrize("test,orig,bad", PASS_MUTATIONS)
 def test_pass_closed_form_detects_planted_mistake(test, orig, bad, tmp_path):
     _fails_on_mutant("test_oracle_passes_cpu.py", test, orig, bad, tmp_path)
+
+
+@pytest.mark.parametrize("test,orig,bad", U1_MUTATIONS)
+def test_u1_pin_detects_planted_mistake(test, orig, bad, tmp_path):
+    _fails_on_mutant("test_oracle_u1_cpu.py", test, orig, bad, tmp_path)
