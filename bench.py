@@ -26,7 +26,7 @@ import numpy as np  # noqa: E402
 
 from pffrg import action as A  # noqa: E402
 from pffrg import flow as F  # noqa: E402
-from pffrg.native import FP32_MIXED, FP64  # noqa: E402
+from pffrg.native import FP32_ELEMENT, FP32_MIXED, FP64  # noqa: E402
 
 METRIC = "RHS evals/s + full-flow wall time vs plain FP64 GPU, at χ rel err ≤1e-4"
 HBM_PEAK_GBS = 8000.0
@@ -67,6 +67,66 @@ def time_mode(prob, a, mode: int, loops: int, steps: int, warmup: int, eval_: in
 
 
 L2_FOOTPRINT = 2 << 20  # bytes: half of one XCD's 4 MiB L2
+
+
+def source_hash() -> str:
+    """sha256 over the HIP sources (the library the measurement runs): a committed counter
+    profile applies to the current kernels only if it was taken from the same sources."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in sorted((ROOT / "pffrgsolver.jl_amd" / "csrc").glob("*")):
+        if f.suffix in (".h", ".hip", ".cpp"):
+            h.update(f.name.encode())
+            h.update(f.read_bytes())
+    return h.hexdigest()[:16]
+
+
+def committed_traffic(name: str):
+    """HBM bytes per launch of the named kernel from the committed rocprofv3 counter profile,
+    with its source, or (None, why) when the profile is missing or was taken from other sources."""
+    tfile = ROOT / "profiles" / name
+    if not tfile.exists():
+        return None, f"no committed profile {name}"
+    d = json.loads(tfile.read_text())
+    if d.get("source_hash") != source_hash():
+        return None, f"profiles/{name} was taken from other kernel sources ({d.get('source_hash')})"
+    return d.get("hbm_bytes_per_launch"), f"profiles/{name} ({d.get('source', '')}, same sources)"
+
+
+def per_pass_stats(prob, a, mode: int, eval_: int) -> dict:
+    """Nodes per element, gathered bytes per element-node and kernel ms of the Katanin (half) and
+    left (full) passes of the 2-loop RHS: compute_dΓ_1l! alone, then compute_dΓ_2l!, whose
+    difference is the left pass."""
+    ns, m = prob.r.n_sites, prob.mesh
+    ctx = prob.context(n_slots=4, precision=mode)
+    ctx.upload(0, a)
+    zero = A.Action.empty(ns, m)
+    for s in (1, 2, 3):
+        ctx.upload(s, zero)
+    ctx.set_timing(True)
+    out = {}
+    runs = []
+    for loops in (1, 2):
+        ctx.compute_dsigma(1.0, 0, 1)
+        ctx.synchronize()
+        ctx.reset_stats()
+        if loops == 1:
+            ctx.compute_dgamma_1l(1.0, 0, 1, eval_)
+        else:
+            ctx.compute_dgamma_2l(1.0, 0, 1, 2, eval_)
+        ctx.synchronize()
+        st = ctx.stats()
+        ms, nl = ctx.timing()
+        runs.append((st, ms))
+    (s1, ms1), (s2, ms2) = runs
+    left = {k: s2[k] - s1[k] for k in s1}
+    for name, st, ms in (("katanin", s1, ms1), ("left", left, ms2 - ms1)):
+        out[name] = {"elements": st["elements"], "element_nodes": st["nodes"],
+                     "nodes_per_element": st["nodes"] / max(1, st["elements"]),
+                     "gathered_bytes_per_element_node": st["gathered_bytes"] / max(1, st["nodes"]),
+                     "kernel_ms": ms, "ns_per_element_node": ms * 1e6 / max(1, st["nodes"])}
+    ctx.close()
+    return out
 
 
 def gather_ceilings(prob, a) -> dict:
@@ -219,6 +279,8 @@ def main():
     ap.add_argument("--flow-max-seconds", type=float, default=None, help="flow: wall-time budget per flow")
     ap.add_argument("--flow-verbose", action="store_true", help="flow: print every BS3 step")
     ap.add_argument("--batch-N", type=int, default=32, help="batch / batchflow: mesh size of every problem")
+    ap.add_argument("--batch-skip-single", action="store_true",
+                    help="batchflow: run the batched flows only (no one-at-a-time comparison)")
     ap.add_argument("--workload", choices=["single", "batch", "batchflow", "flow", "warmstart"], default="single",
                     help="single: the 2-loop square L=6 RHS (headline); batch: the J1-J2 sweep + uneven "
                          "truncations as one batched 1-loop RHS")
@@ -256,10 +318,16 @@ def main():
     log(f"workload: square L={args.L} ns={ns} N={args.N} ({m.num_Omega} points) loops={args.loops} "
         f"(setup {time.time() - t0:.1f}s)")
 
-    log("FP32-mixed leg")
+    log("FP32-mixed leg (shared-node pass)")
     dt32, st32, kms32, kl32, out32 = time_mode(prob, a, FP32_MIXED, args.loops, args.steps, args.warmup, eval_)
-    log(f"  {dt32 / args.steps * 1e3:.1f} ms/RHS, nodes/RHS {st32['nodes'] / args.steps:.3e}")
+    log(f"  {dt32 / args.steps * 1e3:.1f} ms/RHS, element-nodes/RHS {st32['nodes'] / args.steps:.3e}")
     res = {}
+    if not args.no_fp64:
+        log("FP32 per-element leg (the same arithmetic without shared nodes)")
+        dte, ste, kmse, kle, oute = time_mode(prob, a, FP32_ELEMENT, args.loops, args.steps, args.warmup, eval_)
+        log(f"  {dte / args.steps * 1e3:.1f} ms/RHS")
+        res["fp32_element_ms_per_step"] = dte / args.steps * 1e3
+        res["speedup_shared_vs_element"] = dte / dt32
     if not args.no_fp64:
         log("FP64 leg")
         dt64, st64, kms64, kl64, out64 = time_mode(prob, a, FP64, args.loops, args.steps, args.warmup, eval_)
@@ -268,27 +336,29 @@ def main():
         err_q3 = max(float(np.max(np.abs(out32.view(c, ch, "q3") - out64.view(c, ch, "q3"))) /
                            np.max(np.abs(out64.view(c, ch, "q3")))) for c in range(2) for ch in range(3))
         err_s = float(np.max(np.abs(out32.Sigma - out64.Sigma)) / np.max(np.abs(out64.Sigma)))
-        res = {"fp64_rhs_per_s": args.steps / dt64, "fp64_ms_per_step": dt64 / args.steps * 1e3,
-               "speedup_vs_fp64": dt64 / dt32, "fp32_vs_fp64_rel_err": {"vertex": err_g, "q3_worst": err_q3,
-                                                                        "sigma": err_s},
-               "fp64_kernel": {"ms_per_launch": kms64 / max(1, kl64),
-                               "gathered_GBps": st64["gathered_bytes"] / (kms64 * 1e-3) / 1e9 if kms64 else None}}
+        res.update({"fp64_rhs_per_s": args.steps / dt64, "fp64_ms_per_step": dt64 / args.steps * 1e3,
+                    "speedup_vs_fp64": dt64 / dt32,
+                    "element_nodes_ratio_fp64_vs_shared": st64["nodes"] / max(1, st32["nodes"]),
+                    "fp32_vs_fp64_rel_err": {"vertex": err_g, "q3_worst": err_q3, "sigma": err_s},
+                    "fp64_kernel": {"kernel": "k_vertex_pass<PASS, double>", "ms_per_launch": kms64 / max(1, kl64),
+                                    "gathered_GBps": st64["gathered_bytes"] / (kms64 * 1e-3) / 1e9 if kms64 else None}})
+        log("per-pass statistics")
+        res["per_pass"] = {"fp32_mixed_shared": per_pass_stats(prob, a, FP32_MIXED, eval_),
+                           "fp64": per_pass_stats(prob, a, FP64, eval_)}
         ref_out = out64
     else:
         ref_out = out32
-    # dominant kernel: the vertex pass; algorithmic bytes = interpolation corners gathered
+    # dominant kernel: the shared-node pass k_shared_pass (the timing covers it alone);
+    # algorithmic bytes = the interpolation corners it gathers (2 components x 4 B per corner)
     bytes_per_launch = st32["gathered_bytes"] / max(1, kl32)
     avg_ms = kms32 / max(1, kl32)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    traffic = None
-    tfile = ROOT / "profiles" / "vertex_pass_traffic.json"
-    if tfile.exists():
-        try:
-            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+    traffic, traffic_note = committed_traffic("shared_pass_traffic.json")
     log("gather ceiling")
     ceil = gather_ceilings(prob, a)
+    if "fp64_kernel" in res and res["fp64_kernel"]["gathered_GBps"]:
+        res["fp64_kernel"]["frac_of_l2_gather_ceiling_3_waves"] = \
+            res["fp64_kernel"]["gathered_GBps"] / ceil["fp64_l2_3_waves_GBps"]
     cpu = None
     if not args.no_cpu:
         log("CPU baseline leg (FP64 oracle, bounded sample)")
@@ -311,12 +381,15 @@ def main():
         "config": {"workload": f"single RHS: square L={args.L} ({ns} sites) Heisenberg SU(2) {args.loops}-loop, "
                                f"N={args.N} (+1 points/axis), Lambda=1, eval={eval_}",
                    "lattice": "square", "L": args.L, "n_sites": ns, "N": args.N, "loops": args.loops},
-        "nodes_per_rhs": st32["nodes"] / args.steps,
+        "element_nodes_per_rhs": st32["nodes"] / args.steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_vertex_pass", "bytes_per_launch": bytes_per_launch, "ms_per_launch": avg_ms,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
+                     "kernel": "k_shared_pass", "bytes_per_launch": bytes_per_launch, "ms_per_launch": avg_ms,
+                     "launches_per_step": kl32 / max(1, args.steps), "source_hash": source_hash(),
                      "gather_ceiling": ceil,
-                     "frac_of_l2_gather_ceiling_3_waves": achieved / ceil["fp32_l2_3_waves_GBps"]},
+                     "frac_of_l2_gather_ceiling_max_occupancy": achieved / ceil["fp32_l2_max_occupancy_GBps"],
+                     "note": "gathered bytes are served from L2 and the Infinity Cache; the pass is bound by "
+                             "gather latency and LDS traffic, not HBM"},
         "cpu_baseline": cpu,
     }
     line.update(res)
@@ -588,7 +661,7 @@ def batchflow_workload(args):
     log(f"batched flows: {n} problems, sites {[p.r.n_sites for p in probs]}")
     t0 = time.perf_counter()
     seq = []
-    for k, p in enumerate(probs):
+    for k, p in enumerate(probs if not args.batch_skip_single else []):
         ts = time.perf_counter()
         with p.context(n_slots=9, precision=FP32_MIXED) as ctx:
             r = F.launch_flow(ctx, p, p.bare_action(), 50.0, 0.3, max_seconds=args.flow_max_seconds)
@@ -604,8 +677,8 @@ def batchflow_workload(args):
     dt_b = time.perf_counter() - t1
     for c in ctxs:
         c.close()
-    same = all(r.Lambdas == q.Lambdas and all(np.array_equal(x, y) for x, y in zip(r.chi, q.chi))
-               for r, q in zip(res, seq))
+    same = None if not seq else all(r.Lambdas == q.Lambdas and all(np.array_equal(x, y) for x, y in zip(r.chi, q.chi))
+                                    for r, q in zip(res, seq))
     log(f"  batched: {dt_b:.1f} s; identical flows: {same}")
     rhs = sum(r.rhs_evals for r in res)
     line = {
@@ -617,8 +690,8 @@ def batchflow_workload(args):
         "config": {"workload": "21 x 1-loop flow: square L=5 J2/J1=k/15 (k=0..15) + square L=3..7 J2/J1=0.5, "
                                f"N={N} (+1 points/axis), Lambda 50 -> 0.3, chi and mesh rescanning every accepted step",
                    "problems": n, "sites": [p.r.n_sites for p in probs], "max_seconds": args.flow_max_seconds},
-        "batched_s": dt_b, "one_at_a_time_s": dt_seq, "batch_speedup": dt_seq / dt_b,
-        "rhs_evals": rhs, "aggregate_rhs_per_s": rhs / dt_b, "batched_equals_one_at_a_time": bool(same),
+        "batched_s": dt_b, "one_at_a_time_s": dt_seq if seq else None, "batch_speedup": dt_seq / dt_b if seq else None,
+        "rhs_evals": rhs, "aggregate_rhs_per_s": rhs / dt_b, "batched_equals_one_at_a_time": same,
         "status": [r.status for r in res], "Lambda_end": [r.Lambdas[-1] for r in res],
     }
     print(json.dumps(line), flush=True)

@@ -763,6 +763,7 @@ __global__ __launch_bounds__(256) void k_shared_finish(SharedArgs A) {
 __host__ __device__ inline int shared_threads(const SharedArgs &A) { return 64 * A.nw; }
 
 template <int PASS>
+// the work units of one pass; launch_shared_finish then writes the outputs from the partial sums
 hipError_t launch_shared_pass(const SharedArgs &A, size_t lds, hipStream_t stream) {
     if (A.n_units > 0) {
         if (lds > 64 * 1024) {
@@ -771,15 +772,9 @@ hipError_t launch_shared_pass(const SharedArgs &A, size_t lds, hipStream_t strea
             if (e != hipSuccess) return e;
         }
         hipLaunchKernelGGL((k_shared_pass<PASS>), dim3(A.n_units), dim3(shared_threads(A)), lds, stream, A);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
+        return hipGetLastError();
     }
-    long total = (long)A.P.n_elems * 2 * A.P.y.ns;
-    long blocks = (total + 255) / 256;
-    if (blocks > 16384) blocks = 16384;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL((k_shared_finish<PASS>), dim3((unsigned)blocks), dim3(256), 0, stream, A);
-    return hipGetLastError();
+    return hipSuccess;
 }
 
 // the units of all problems of a batch (problem index in bits 26-31), nthreads per block; the
