@@ -640,7 +640,11 @@ int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const
 // per-element pass.
 // the shared-node pass serves the SU(2) action up to 8 SH_SPT sites; U(1)-DM runs the
 // per-element pass in FP32 in that mode
-bool shared_fits(const pffrg_ctx *c) { return c->sys.nc == 2 && c->sys.ns <= 8 * SH_SPT; }
+bool shared_fits(const pffrg_ctx *c) {
+    if (c->sys.nc != 2 || c->sys.ns > 8 * SH_SPT) return false;
+    // the pass's LDS (tables, overlap rows, node states) within the 160 KB of a CU
+    return shared_lds(c->sys, c->n_ov, 8, true, (c->sys.ns + 7) / 8).total <= 160 * 1024;
+}
 
 void shared_shape(pffrg_ctx *c) {
     c->sh_nw = 8;
@@ -710,7 +714,7 @@ int shared_args(pffrg_ctx *c, double L, int a, int d, int out, int eval, const d
 
 template <int PASS>
 size_t shared_lds_bytes(const pffrg_ctx *c, const SharedArgs &A) {
-    return (size_t)shared_lds(c->sys, c->n_ov, A.nw, PASS == P_KAT).total;
+    return (size_t)shared_lds(c->sys, c->n_ov, A.nw, PASS == P_KAT, A.spt).total;
 }
 
 template <int PASS>
@@ -1376,7 +1380,7 @@ int pffrg_get_stats(pffrg_ctx *c, int64_t *stats) {
 }
 
 int pffrg_get_counters(pffrg_ctx *c, int64_t *out, int32_t n) {
-    if (!c || !out || n < 1 || n > N_COUNTERS) return fail(c, PFFRG_ERR_ARG, "counters: 1..16 entries");
+    if (!c || !out || n < 1 || n > N_COUNTERS) return fail(c, PFFRG_ERR_ARG, "counters: 1..32 entries");
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     unsigned long long v[N_COUNTERS];
@@ -1618,7 +1622,7 @@ int batch_rhs_1l_shared(pffrg_batch *b, const double *Lam, const int32_t *a, con
     const int n = (int)b->ctx.size();
     std::vector<SharedArgs> tab(n);
     size_t lds = 0;
-    int nthreads = 64;
+    int nthreads = 64, spt = 1;
     for (int k = 0; k < n; ++k) {
         pffrg_ctx *c = b->ctx[k];
         const int q = b->order[k];
@@ -1628,9 +1632,12 @@ int batch_rhs_1l_shared(pffrg_batch *b, const double *Lam, const int32_t *a, con
         if ((rc = launch_corrs(c, Lam[q], a[q], da[q], true, gtol))) return bfail(b, rc, c->err);
         if ((rc = ensure_gather(c, a[q], true))) return bfail(b, rc, c->err);
         if ((rc = shared_args<P_KAT>(c, Lam[q], a[q], da[q], da[q], eval[q], gtol, 0, tab[k]))) return bfail(b, rc, c->err);
-        lds = std::max(lds, shared_lds_bytes<P_KAT>(c, tab[k]));
         nthreads = std::max(nthreads, shared_threads(tab[k]));
+        spt = std::max(spt, tab[k].spt);
     }
+    // one launch shape for all: LDS of the largest problem at the batch's sites per wave
+    for (int k = 0; k < n; ++k)
+        lds = std::max(lds, (size_t)shared_lds(b->ctx[k]->sys, b->ctx[k]->n_ov, tab[k].nw, true, spt).total);
     if (!b->d_sh_units) {
         // round-robin over the problems' unit lists, each in its own order
         std::vector<std::vector<uint32_t>> per(n);
@@ -1658,7 +1665,7 @@ int batch_rhs_1l_shared(pffrg_batch *b, const double *Lam, const int32_t *a, con
     if (hipMemcpy(b->d_sh_tab, tab.data(), n * sizeof(SharedArgs), hipMemcpyHostToDevice) != hipSuccess)
         return bfail(b, PFFRG_ERR_HIP, "argument table upload failed");
     hipError_t e = launch_shared_pass_batch<P_KAT>(reinterpret_cast<const SharedArgs *>(b->d_sh_tab), b->d_sh_units,
-                                                   b->n_sh_units, nthreads, lds, b->stream);
+                                                   b->n_sh_units, nthreads, spt, lds, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) return bfail(b, PFFRG_ERR_HIP, std::string("batched shared-node pass: ") + hipGetErrorString(e));
     for (int k = 0; k < n; ++k) {
@@ -1966,8 +1973,14 @@ int pffrg_batch_rhs_1l(pffrg_batch *b, const double *Lambda, const int32_t *a, c
         if (!(Lambda[q] > 0.0) || eval[q] < 1) return bfail(b, PFFRG_ERR_ARG, "invalid Lambda or eval of problem " + std::to_string(q));
     }
     if (hipSetDevice(b->device) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "hipSetDevice failed");
+    // the shared-node batch runs every problem at the batch's largest sites per wave
     bool all_fit = true;
-    for (pffrg_ctx *c : b->ctx) all_fit = all_fit && shared_fits(c);
+    int spt = 1;
+    for (pffrg_ctx *c : b->ctx) {
+        all_fit = all_fit && shared_fits(c);
+        spt = std::max(spt, (c->sys.ns + 7) / 8);
+    }
+    for (pffrg_ctx *c : b->ctx) all_fit = all_fit && shared_lds(c->sys, c->n_ov, 8, true, spt).total <= 160 * 1024;
     if (prec == PFFRG_FP32_MIXED && all_fit) return batch_rhs_1l_shared(b, Lambda, a, da, eval, sigma_tol, gamma_tol);
     if (prec == PFFRG_FP32_MIXED) return batch_rhs_1l_t<float>(b, Lambda, a, da, eval, sigma_tol, gamma_tol);
     return prec == PFFRG_FP32_ELEMENT ? batch_rhs_1l_t<float>(b, Lambda, a, da, eval, sigma_tol, gamma_tol)

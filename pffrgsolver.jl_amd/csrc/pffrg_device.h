@@ -14,7 +14,7 @@ namespace pffrg {
 
 constexpr int MAXN = 64;      // mesh points per axis held in LDS
 constexpr int WAVE = 64;      // gfx950 wavefront
-constexpr int N_COUNTERS = 16;  // device measurement counters of a context (nodes, bytes, phase cycles)
+constexpr int N_COUNTERS = 32;  // device measurement counters of a context (nodes, bytes, phase cycles)
 
 // Bracket hints of the five meshes (σ, Ωs, νs, Ωt, νt), stored after the meshes and their
 // inverse spacings in the device mesh table.  A positive double's bits shifted right by 49 are

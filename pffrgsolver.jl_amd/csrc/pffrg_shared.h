@@ -83,7 +83,7 @@ struct SharedLds {
     int mp, o_sig, o_mf, o_ov, o_ovoff, o_exch, o_hint, o_desc, o_V, o_loc, o_red, o_nodes, o_b1, total;
 };
 
-__host__ __device__ inline SharedLds shared_lds(const Sys &y, int n_ov, int nw, bool sig_d) {
+__host__ __device__ inline SharedLds shared_lds(const Sys &y, int n_ov, int nw, bool sig_d, int spt) {
     SharedLds L;
     int mp = y.nsig;
     if (y.nO > mp) mp = y.nO;
@@ -92,7 +92,7 @@ __host__ __device__ inline SharedLds shared_lds(const Sys &y, int n_ov, int nw, 
     int o = 10 * L.mp * 8;
     L.o_sig = o; o += (sig_d ? 2 : 1) * L.mp * 8;
     L.o_mf = o; o += 12 * L.mp * 4;
-    L.o_ov = o; o += ((n_ov + 1) & ~1) * 4;
+    L.o_ov = o; o += n_ov * 8;  // (row, multiplicity factor) pairs
     L.o_ovoff = o; o += ((y.ns + 2) & ~1) * 4;
     L.o_exch = o; o += ((y.ns + 1) & ~1) * 4;
     L.o_hint = o; o += ((HINT_WORDS + 1) & ~1) * 4;
@@ -107,7 +107,7 @@ __host__ __device__ inline SharedLds shared_lds(const Sys &y, int n_ov, int nw, 
     if (red_end > o) o = red_end;
     o = (o + 15) & ~15;
     L.o_nodes = o; o += SH_NODES * (8 + 8 + 4 + 4);  // positions, weights, bubbles, reweight flags
-    L.o_b1 = o; o += nw * 64 * SH_SPT * 2 * 4 * 4;    // each lane's previous-level Simpson sums (FP32)
+    L.o_b1 = o; o += nw * 64 * spt * 2 * 4 * 4;    // each lane's previous-level Simpson sums (FP32)
     L.total = o;
     return L;
 }
@@ -144,18 +144,20 @@ __device__ __forceinline__ int gather_site(const DescT<float> *ds, const float *
 }
 
 // Phase timing of the shared-node pass (a measurement build with PFFRG_SHARED_PROFILE defined):
-// thread 0 of every block adds the shader-clock cycles of each phase to count[2 + phase].
+// thread 0 of every block adds the shader-clock cycles of each phase to count[2 + phase] (units of
+// the s and u channels) or count[2 + PR_N + phase] (t channel).
 #ifdef PFFRG_SHARED_PROFILE
 #define SH_PROF_T0() const unsigned long long _t0 = (t == 0) ? clock64() : 0ull
-#define SH_PROF_ADD(k) do { if (t == 0 && A.count) s_prof[k] += clock64() - _t0; } while (0)
+#define SH_PROF_ADD(k) do { if (t == 0 && A.count) s_prof[prof_base + (k)] += clock64() - _t0; } while (0)
 #else
 #define SH_PROF_T0() do { } while (0)
 #define SH_PROF_ADD(k) do { } while (0)
 #endif
 enum { PR_ACTIVE, PR_PREP, PR_DESC, PR_GATHER, PR_CONTRACT, PR_CHECK, PR_BATCHES, PR_TOTAL, PR_N };
 
-// One work unit (code uc) of one problem.  Every thread of the block calls it.
-template <int PASS>
+// One work unit (code uc) of one problem.  Every thread of the block calls it.  SPT: the largest
+// number of sites per wave the launch serves (the per-lane quadrature state is sized by it).
+template <int PASS, int SPT>
 __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t uc) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     __shared__ uint32_t s_code[MAX_DESC];
@@ -163,7 +165,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     __shared__ int s_act2[SH_T], s_act3[SH_T], s_nact[2];
     __shared__ int s_rowf[SH_T], s_colf[SH_T];
     __shared__ unsigned long long s_cnt[4];
-    __shared__ unsigned long long s_prof[PR_N];
+    __shared__ unsigned long long s_prof[2 * PR_N];
 
     const PassArgs<float> &P = A.P;
     const Sys &y = P.y;
@@ -173,12 +175,12 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     const int wv = t >> 6;  // this thread's wave
     const int nw = A.nw;
     const int n_ov = P.ov_off[y.ns];
-    const SharedLds Lay = shared_lds(y, n_ov, nw, PASS == P_KAT);
+    const SharedLds Lay = shared_lds(y, n_ov, nw, PASS == P_KAT, SPT);
     const int mp = Lay.mp;
     double *s_mesh = reinterpret_cast<double *>(s_dyn);
     double *s_sigA = reinterpret_cast<double *>(s_dyn + Lay.o_sig);
     double *s_sigD = (PASS == P_KAT) ? s_sigA + mp : s_sigA;
-    uint32_t *s_ov = reinterpret_cast<uint32_t *>(s_dyn + Lay.o_ov);
+    uint2 *s_ov = reinterpret_cast<uint2 *>(s_dyn + Lay.o_ov);
     int *s_ovoff = reinterpret_cast<int *>(s_dyn + Lay.o_ovoff);
     int *s_exch = reinterpret_cast<int *>(s_dyn + Lay.o_exch);
     int32_t *s_hint = reinterpret_cast<int32_t *>(s_dyn + Lay.o_hint);
@@ -207,14 +209,22 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
         mf[11 * mp + k] = (PASS == P_KAT) ? (float)P.D.Sigma[k] : 0.0f;
     }
     for (int k = t; k < y.ns; k += NT) s_exch[k] = P.exch[k];
-    for (int k = t; k < n_ov; k += NT) s_ov[k] = P.ov_rows[k];
+    {
+        // overlap rows (j1 | j2 << 8 | m << 16) beside the FP32 factor -2 S² m of their term
+        // (flow_su2_t.jl:59-74)
+        const float mfac = (float)(-2.0 * y.S2);
+        for (int k = t; k < n_ov; k += NT) {
+            const uint32_t r = P.ov_rows[k];
+            s_ov[k] = uint2{r, __float_as_uint(mfac * (float)(r >> 16))};
+        }
+    }
     for (int k = t; k <= y.ns; k += NT) s_ovoff[k] = P.ov_off[k];
     {
         const int32_t *gh = reinterpret_cast<const int32_t *>(P.mesh + 10 * MAXN);
         for (int k = t; k < HINT_WORDS; k += NT) s_hint[k] = gh[k];
     }
     if (t < 4) s_cnt[t] = 0;
-    if (t < PR_N) s_prof[t] = 0;
+    if (t < 2 * PR_N) s_prof[t] = 0;
 #ifdef PFFRG_SHARED_PROFILE
     const unsigned long long t_begin = clock64();
 #endif
@@ -231,7 +241,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     S.sigA = s_sigA;
     S.sigD = s_sigD;
     S.exch = s_exch;
-    S.ov = s_ov;
+    S.ov = nullptr;  // the contraction reads the rows from s_ov
     S.ov_off = s_ovoff;
     S.hint = reinterpret_cast<const int16_t *>(s_hint);
     S.mf = MeshV<float>{mf, mf + mp, mf + 2 * mp, mf + 3 * mp, mf + 4 * mp, mf + 5 * mp, mf + 10 * mp, mf + 11 * mp,
@@ -239,6 +249,9 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
 
     // ---- the unit
     const int ch = uc & 3, w1 = (uc >> 2) & 127, t2 = (uc >> 9) & 15, t3 = (uc >> 13) & 15, dom = (uc >> 17) & 3;
+#ifdef PFFRG_SHARED_PROFILE
+    const int prof_base = (ch == 1) ? PR_N : 0;
+#endif
     const int nn = y.nn;
     if (t < MAX_DESC) s_code[t] = slot_code<PASS>(ch, t / 3, t % 3);
     const double INF = __builtin_huge_val();
@@ -281,7 +294,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
         }
 
     // per-lane quadrature state: output o = (site slot s, component c, element r) at (s * 2 + c) * 4 + r
-    constexpr int NO = SH_SPT * 2 * 4;
+    constexpr int NO = SPT * 2 * 4;
     double b2[NO];
     float Pp[NO];
 #pragma unroll
@@ -319,8 +332,8 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     auto eval_batch = [&](int q0, int nb) {
 #ifdef PFFRG_SHARED_PROFILE
         unsigned long long _tb = (t == 0) ? clock64() : 0ull;
-        if (t == 0 && A.count) s_prof[PR_BATCHES] += 1;
-#define SH_PROF_LAP(k) do { if (t == 0 && A.count) { const unsigned long long _n = clock64(); s_prof[k] += _n - _tb; _tb = _n; } } while (0)
+        if (t == 0 && A.count) s_prof[prof_base + PR_BATCHES] += 1;
+#define SH_PROF_LAP(k) do { if (t == 0 && A.count) { const unsigned long long _n = clock64(); s_prof[prof_base + (k)] += _n - _tb; _tb = _n; } } while (0)
 #else
 #define SH_PROF_LAP(k) do { } while (0)
 #endif
@@ -363,6 +376,12 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                     int cn;
                     gather_slot<float>(s_desc + ((k * 4 + s) * SH_T + ee) * 3, V.gamma, s_exch, i, y, vs, vd, cn);
                     cb += cn;
+                    if (s == 1) {
+                        // every term's ν' factor carries the node's -tp
+                        const float mt = -s_tp[q0 + k];
+                        vs *= mt;
+                        vd *= mt;
+                    }
                     s_V[((k * 2 + s) * SH_T + ee) * VR + i] = float2{vs, vd};
                 }
             }
@@ -378,7 +397,31 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                         const int ee = act_l[u];
                         float vs = (float)V.bare[0], vd = (float)V.bare[y.ns];
                         cb += gather_site(s_desc + ((k * 4 + s) * SH_T + ee) * 3, V.gamma, s_exch, 0, y, vs, vd);
+                        if (s == 3) {
+                            const float mt = -s_tp[q0 + k];
+                            vs *= mt;
+                            vd *= mt;
+                        }
                         s_loc[(k * 2 + (s - 2)) * SH_T + ee] = float2{vs, vd};
+                    }
+                }
+                // the t contraction sums over all SH_K node slots: those beyond the batch hold zeros
+                if (nb < SH_K) {
+                    const int nz = SH_K - nb;
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        const int nA = s == 0 ? nA2 : nA3;
+                        const int *act_l = s == 0 ? s_act2 : s_act3;
+                        const int per_k = nA * y.ns;
+                        for (int tau = t; tau < nz * per_k; tau += NT) {
+                            const int k = nb + tau / per_k, r = tau % per_k;
+                            const int u = r / y.ns, i = r - u * y.ns;
+                            s_V[((k * 2 + s) * SH_T + act_l[u]) * VR + i] = float2{0.f, 0.f};
+                        }
+                        for (int tau = t; tau < nz * nA; tau += NT) {
+                            const int k = nb + tau / nA, u = tau % nA;
+                            s_loc[(k * 2 + s) * SH_T + act_l[u]] = float2{0.f, 0.f};
+                        }
                     }
                 }
             }
@@ -392,15 +435,15 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             const double wq = s_wq[q0];
             const bool addp = s_rw[q0] != 0;
             const int pos = lane & 15, kq = lane >> 4;
-            const float S2 = (float)y.S2;
 #pragma unroll
-            for (int s = 0; s < SH_SPT; ++s) {
+            for (int s = 0; s < SPT; ++s) {
                 if (s >= nsw) break;
                 const int i = wv + nw * s;
                 sh_v4f Ds = {0.f, 0.f, 0.f, 0.f}, Dd = {0.f, 0.f, 0.f, 0.f};
                 if (ch != 1) {
                     // s and u (flow_su2_s.jl:41-42, flow_su2_u.jl:41-42): Γs = v1s (-tp (∓2 v2s + v2d))
-                    // + v1d (-tp v2s), Γd = v1s (-3 tp v2s) + v1d (-tp v2d); two nodes per product
+                    // + v1d (-tp v2s), Γd = v1s (-3 tp v2s) + v1d (-tp v2d), with -tp already in
+                    // v2; two nodes per product
                     const float sgn2 = (ch == 0) ? -2.0f : 2.0f;
                     for (int st = 0; st < (nb + 1) / 2; ++st) {
                         const int kap = 4 * st + kq;
@@ -409,55 +452,59 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                         if (n < nb) {
                             const float2 v1 = s_V[((n * 2 + 0) * SH_T + pos) * VR + i];
                             const float2 v2 = s_V[((n * 2 + 1) * SH_T + pos) * VR + i];
-                            const float tp = s_tp[q0 + n];
                             as = tau ? v1.y : v1.x;
-                            bs = tau ? -tp * v2.x : -tp * (sgn2 * v2.x + v2.y);
-                            bd = tau ? -tp * v2.y : -tp * (3.0f * v2.x);
+                            bs = tau ? v2.x : sgn2 * v2.x + v2.y;
+                            bd = tau ? v2.y : 3.0f * v2.x;
                         }
                         Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(as, bs, Ds, 0, 0, 0);
                         Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(as, bd, Dd, 0, 0, 0);
                     }
                 } else {
-                    // t (flow_su2_t.jl:56-74): two local terms with the site-0 vertices v3 (ν) and
-                    // v4 (ν'), then one term per overlap row (j1, j2, m) of site i
-                    const int r0 = s_ovoff[i], R = s_ovoff[i + 1] - r0, kk = 2 + R;
-                    const int nst = (kk + 3) >> 2;
-                    for (int n = 0; n < nb; ++n) {
-                        const float tp = s_tp[q0 + n];
-                        const float cc = -tp * -2.0f * S2;
-                        const float2 *V0 = s_V + (n * 2 + 0) * SH_T * VR + pos * VR;
-                        const float2 *V1 = s_V + (n * 2 + 1) * SH_T * VR + pos * VR;
-                        const float2 v1i = V0[i], v2i = V1[i];
-                        const float2 v3 = s_loc[(n * 2 + 0) * SH_T + pos], v4 = s_loc[(n * 2 + 1) * SH_T + pos];
-                        // operands of the inner index tau: 0 and 1 the local terms, 2 .. kk-1 the rows,
-                        // beyond kk zero; every lane reads a valid row and selects (no branches, so the
-                        // loads of consecutive steps overlap)
-#pragma unroll 2
-                        for (int st = 0; st < nst; ++st) {
-                            const int tau = 4 * st + kq;
-                            int rr = tau - 2;
-                            rr = rr < 0 ? 0 : (rr >= R ? (R > 0 ? R - 1 : 0) : rr);
-                            const uint32_t row = (R > 0) ? s_ov[r0 + rr] : 0u;  // a site without rows reads none
-                            const float2 e1 = V0[(int)(row & 255)];
-                            const float2 f2 = V1[(int)((row >> 8) & 255)];
-                            const float c = cc * (float)(row >> 16);
-                            const bool isrow = tau >= 2 && tau < kk;
-                            float as = isrow ? e1.x : 0.f, ad = isrow ? e1.y : 0.f;
-                            float bs = isrow ? c * f2.x : 0.f, bd = isrow ? c * f2.y : 0.f;
-                            if (tau == 0) {
-                                as = v1i.x;
-                                ad = v1i.y;
-                                bs = -tp * (-1.0f * v4.x + v4.y);
-                                bd = -tp * (3.0f * v4.x + v4.y);
+                    // t (flow_su2_t.jl:56-74), the batch's nodes on the inner index: two local terms
+                    // with the site-0 vertices v3 (ν) and v4 (ν'), then one term per overlap row
+                    // (j1, j2, m) of site i, whose factor -2 S² m sits beside the row; the ν'
+                    // vertices carry -tp, and node slots beyond the batch hold zeros
+                    const float2 *V0 = s_V + (kq * 2 + 0) * SH_T * VR + pos * VR;
+                    const float2 *V1 = s_V + (kq * 2 + 1) * SH_T * VR + pos * VR;
+                    const float2 v1i = V0[i], v2i = V1[i];
+                    const float2 v3 = s_loc[(kq * 2 + 0) * SH_T + pos], v4 = s_loc[(kq * 2 + 1) * SH_T + pos];
+                    Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(v1i.x, -1.0f * v4.x + v4.y, Ds, 0, 0, 0);
+                    Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(v1i.y, 3.0f * v4.x + v4.y, Dd, 0, 0, 0);
+                    Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(-1.0f * v3.x + v3.y, v2i.x, Ds, 0, 0, 0);
+                    Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(3.0f * v3.x + v3.y, v2i.y, Dd, 0, 0, 0);
+                    // the rows, 64 at a time, one per lane; each step reads its row from a lane
+                    // (v_readlane), so that only the two vertex reads wait on LDS
+                    const int r0 = __builtin_amdgcn_readfirstlane(s_ovoff[i]);
+                    const int r1 = __builtin_amdgcn_readfirstlane(s_ovoff[i + 1]);
+                    for (int cb = r0; cb < r1; cb += 64) {
+                        const int nr = (r1 - cb) < 64 ? r1 - cb : 64;
+                        const uint2 rl = (lane < nr) ? s_ov[cb + lane] : uint2{0u, 0u};
+                        const int rj = (int)rl.x, rm = (int)rl.y;
+                        // four rows per step: their eight vertex reads are issued together
+                        int r = 0;
+                        for (; r + 4 <= nr; r += 4) {
+                            float2 e1[4], f2[4];
+                            float mm[4];
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                const uint32_t jj = (uint32_t)__builtin_amdgcn_readlane(rj, r + u);
+                                mm[u] = __int_as_float(__builtin_amdgcn_readlane(rm, r + u));
+                                e1[u] = V0[jj & 255];
+                                f2[u] = V1[(jj >> 8) & 255];
                             }
-                            if (tau == 1) {
-                                as = -1.0f * v3.x + v3.y;
-                                ad = 3.0f * v3.x + v3.y;
-                                bs = -tp * v2i.x;
-                                bd = -tp * v2i.y;
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(e1[u].x, mm[u] * f2[u].x, Ds, 0, 0, 0);
+                                Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(e1[u].y, mm[u] * f2[u].y, Dd, 0, 0, 0);
                             }
-                            Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(as, bs, Ds, 0, 0, 0);
-                            Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(ad, bd, Dd, 0, 0, 0);
+                        }
+                        for (; r < nr; ++r) {
+                            const uint32_t jj = (uint32_t)__builtin_amdgcn_readlane(rj, r);
+                            const float mm = __int_as_float(__builtin_amdgcn_readlane(rm, r));
+                            const float2 e1 = V0[jj & 255];
+                            const float2 f2 = V1[(jj >> 8) & 255];
+                            Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.x, mm * f2.x, Ds, 0, 0, 0);
+                            Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.y, mm * f2.y, Dd, 0, 0, 0);
                         }
                     }
                 }
@@ -557,7 +604,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                 // Richardson-extrapolated result (quadrature.jl:72-88), added in sub-interval order
                 double *pp = part_at(dom, r);
 #pragma unroll
-                for (int s = 0; s < SH_SPT; ++s) {
+                for (int s = 0; s < SPT; ++s) {
                     if (s >= nsw) break;
 #pragma unroll
                     for (int c = 0; c < 2; ++c) {
@@ -673,7 +720,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                 if (valid[r]) {
                     double *pt = part_at(4, r);
 #pragma unroll
-                    for (int s = 0; s < SH_SPT; ++s) {
+                    for (int s = 0; s < SPT; ++s) {
                         if (s >= nsw) break;
                         pt[wv + nw * s] = b2[(s * 2 + 0) * 4 + r];
                         pt[y.ns + wv + nw * s] = b2[(s * 2 + 1) * 4 + r];
@@ -703,26 +750,26 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             atomicAdd(A.count, s_cnt[0]);
             atomicAdd(A.count + 1, s_cnt[1]);
 #ifdef PFFRG_SHARED_PROFILE
-            s_prof[PR_TOTAL] = clock64() - t_begin;
-            for (int k = 0; k < PR_N; ++k) atomicAdd(A.count + 2 + k, s_prof[k]);
+            s_prof[prof_base + PR_TOTAL] = clock64() - t_begin;
+            for (int k = 0; k < 2 * PR_N; ++k) atomicAdd(A.count + 2 + k, s_prof[k]);
 #endif
         }
     }
 }
 
-template <int PASS>
+template <int PASS, int SPT>
 __global__ __launch_bounds__(SH_MAXT) void k_shared_pass(SharedArgs A) {
-    shared_pass_body<PASS>(A, __builtin_amdgcn_readfirstlane(A.units[blockIdx.x]));
+    shared_pass_body<PASS, SPT>(A, __builtin_amdgcn_readfirstlane(A.units[blockIdx.x]));
 }
 
 // The same pass of several problems in one launch: unit code bits 26-31 select the problem's
 // arguments in tab; the block has the threads of the problem that needs most, the others idle.
-template <int PASS>
+template <int PASS, int SPT>
 __global__ __launch_bounds__(SH_MAXT) void k_shared_pass_batch(const SharedArgs *__restrict__ tab,
                                                                 const uint32_t *__restrict__ units, int n) {
     if ((int)blockIdx.x >= n) return;
     const uint32_t uc = __builtin_amdgcn_readfirstlane(units[blockIdx.x]);
-    shared_pass_body<PASS>(tab[uc >> 26], uc & 0x03ffffffu);
+    shared_pass_body<PASS, SPT>(tab[uc >> 26], uc & 0x03ffffffu);
 }
 
 // Sum of an element's partial results in the reference's order (the four domains, then the two
@@ -762,34 +809,54 @@ __global__ __launch_bounds__(256) void k_shared_finish(SharedArgs A) {
 // threads of a block: nw waves, each holding the whole tile on its share of the sites
 __host__ __device__ inline int shared_threads(const SharedArgs &A) { return 64 * A.nw; }
 
-template <int PASS>
-// the work units of one pass; launch_shared_finish then writes the outputs from the partial sums
-hipError_t launch_shared_pass(const SharedArgs &A, size_t lds, hipStream_t stream) {
-    if (A.n_units > 0) {
-        if (lds > 64 * 1024) {
-            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_shared_pass<PASS>),
-                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            if (e != hipSuccess) return e;
-        }
-        hipLaunchKernelGGL((k_shared_pass<PASS>), dim3(A.n_units), dim3(shared_threads(A)), lds, stream, A);
-        return hipGetLastError();
-    }
-    return hipSuccess;
-}
-
-// the units of all problems of a batch (problem index in bits 26-31), nthreads per block; the
-// partial sums of each problem are then summed by launch_shared_finish on the same stream
-template <int PASS>
-hipError_t launch_shared_pass_batch(const SharedArgs *tab, const uint32_t *units, int n, int nthreads, size_t lds,
-                                    hipStream_t stream) {
-    if (n <= 0) return hipSuccess;
+template <int PASS, int SPT>
+hipError_t launch_shared_pass_spt(const SharedArgs &A, size_t lds, hipStream_t stream) {
     if (lds > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_shared_pass_batch<PASS>),
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_shared_pass<PASS, SPT>),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_shared_pass_batch<PASS>), dim3(n), dim3(nthreads), lds, stream, tab, units, n);
+    hipLaunchKernelGGL((k_shared_pass<PASS, SPT>), dim3(A.n_units), dim3(shared_threads(A)), lds, stream, A);
     return hipGetLastError();
+}
+
+template <int PASS>
+// the work units of one pass; launch_shared_finish then writes the outputs from the partial sums
+hipError_t launch_shared_pass(const SharedArgs &A, size_t lds, hipStream_t stream) {
+    if (A.n_units <= 0) return hipSuccess;
+    switch (A.spt) {
+    case 1: return launch_shared_pass_spt<PASS, 1>(A, lds, stream);
+    case 2: return launch_shared_pass_spt<PASS, 2>(A, lds, stream);
+    case 3: return launch_shared_pass_spt<PASS, 3>(A, lds, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int PASS, int SPT>
+hipError_t launch_shared_pass_batch_spt(const SharedArgs *tab, const uint32_t *units, int n, int nthreads, size_t lds,
+                                        hipStream_t stream) {
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_shared_pass_batch<PASS, SPT>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_shared_pass_batch<PASS, SPT>), dim3(n), dim3(nthreads), lds, stream, tab, units, n);
+    return hipGetLastError();
+}
+
+// the units of all problems of a batch (problem index in bits 26-31), nthreads per block, the
+// per-lane state sized for spt sites per wave (the most any problem of the batch has); the
+// partial sums of each problem are then summed by launch_shared_finish on the same stream
+template <int PASS>
+hipError_t launch_shared_pass_batch(const SharedArgs *tab, const uint32_t *units, int n, int nthreads, int spt,
+                                    size_t lds, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    switch (spt) {
+    case 1: return launch_shared_pass_batch_spt<PASS, 1>(tab, units, n, nthreads, lds, stream);
+    case 2: return launch_shared_pass_batch_spt<PASS, 2>(tab, units, n, nthreads, lds, stream);
+    case 3: return launch_shared_pass_batch_spt<PASS, 3>(tab, units, n, nthreads, lds, stream);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <int PASS>
@@ -804,8 +871,8 @@ hipError_t launch_shared_finish(const SharedArgs &A, hipStream_t stream) {
 
 #define PFFRG_SHARED_INSTANCES(KW, PASS)                                                                        \
     KW template hipError_t launch_shared_pass<PASS>(const SharedArgs &, size_t, hipStream_t);                  \
-    KW template hipError_t launch_shared_pass_batch<PASS>(const SharedArgs *, const uint32_t *, int, int, size_t, \
-                                                          hipStream_t);                                        \
+    KW template hipError_t launch_shared_pass_batch<PASS>(const SharedArgs *, const uint32_t *, int, int, int,    \
+                                                          size_t, hipStream_t);                                \
     KW template hipError_t launch_shared_finish<PASS>(const SharedArgs &, hipStream_t);
 
 PFFRG_SHARED_INSTANCES(extern, P_KAT)

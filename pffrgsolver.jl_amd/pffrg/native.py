@@ -333,7 +333,7 @@ class Context:
         self._ck(self.lib.pffrg_get_stats(self.h, _ip(s)), "get_stats")
         return {"nodes": int(s[0]), "elements": int(s[1]), "launches": int(s[2]), "gathered_bytes": int(s[3])}
 
-    def counters(self, n: int = 16) -> np.ndarray:
+    def counters(self, n: int = 32) -> np.ndarray:
         s = np.zeros(n, dtype=np.int64)
         self._ck(self.lib.pffrg_get_counters(self.h, _ip(s), n), "get_counters")
         return s
