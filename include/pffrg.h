@@ -143,8 +143,10 @@ int pffrg_get_stats(pffrg_ctx *ctx, int64_t *stats);
 /* the context's n <= 32 raw device counters since pffrg_reset_stats: [0] quadrature nodes
  * (element-nodes), [1] gathered bytes; [2..9] and [10..17] cycle counts of the shared-node pass's
  * phases (active set, node preparation, descriptors, gathers, contraction, convergence tests,
- * batches, total) in the units of the s and u channels and of the t channel, in a build with
- * PFFRG_SHARED_PROFILE defined (zero otherwise) */
+ * batches, total) in the units of the s and u channels and of the t channel, [18..22] batches by
+ * active elements of the tile (1-4, 5-16, 17-64, 65-128, 129-256), [23] active element-nodes,
+ * [24] active row and column vectors per node, in a build with PFFRG_SHARED_PROFILE defined
+ * (zero otherwise) */
 int pffrg_get_counters(pffrg_ctx *ctx, int64_t *out, int32_t n);
 /* boundary-correction factors corrs[Ω][channel][side] (2*3*n_Omega doubles) of the last pass */
 int pffrg_get_corrs(pffrg_ctx *ctx, double *corrs);

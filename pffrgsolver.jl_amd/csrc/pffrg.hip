@@ -1635,9 +1635,9 @@ int batch_rhs_1l_shared(pffrg_batch *b, const double *Lam, const int32_t *a, con
         nthreads = std::max(nthreads, shared_threads(tab[k]));
         spt = std::max(spt, tab[k].spt);
     }
-    // one launch shape for all: LDS of the largest problem at the batch's sites per wave
-    for (int k = 0; k < n; ++k)
-        lds = std::max(lds, (size_t)shared_lds(b->ctx[k]->sys, b->ctx[k]->n_ov, tab[k].nw, true, spt).total);
+    // one launch shape for all: the LDS of the problem that needs most (each problem keeps its own
+    // layout and batch size inside it), per-lane state for the most sites per wave
+    for (int k = 0; k < n; ++k) lds = std::max(lds, shared_lds_bytes<P_KAT>(b->ctx[k], tab[k]));
     if (!b->d_sh_units) {
         // round-robin over the problems' unit lists, each in its own order
         std::vector<std::vector<uint32_t>> per(n);
@@ -1973,14 +1973,8 @@ int pffrg_batch_rhs_1l(pffrg_batch *b, const double *Lambda, const int32_t *a, c
         if (!(Lambda[q] > 0.0) || eval[q] < 1) return bfail(b, PFFRG_ERR_ARG, "invalid Lambda or eval of problem " + std::to_string(q));
     }
     if (hipSetDevice(b->device) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "hipSetDevice failed");
-    // the shared-node batch runs every problem at the batch's largest sites per wave
     bool all_fit = true;
-    int spt = 1;
-    for (pffrg_ctx *c : b->ctx) {
-        all_fit = all_fit && shared_fits(c);
-        spt = std::max(spt, (c->sys.ns + 7) / 8);
-    }
-    for (pffrg_ctx *c : b->ctx) all_fit = all_fit && shared_lds(c->sys, c->n_ov, 8, true, spt).total <= 160 * 1024;
+    for (pffrg_ctx *c : b->ctx) all_fit = all_fit && shared_fits(c);
     if (prec == PFFRG_FP32_MIXED && all_fit) return batch_rhs_1l_shared(b, Lambda, a, da, eval, sigma_tol, gamma_tol);
     if (prec == PFFRG_FP32_MIXED) return batch_rhs_1l_t<float>(b, Lambda, a, da, eval, sigma_tol, gamma_tol);
     return prec == PFFRG_FP32_ELEMENT ? batch_rhs_1l_t<float>(b, Lambda, a, da, eval, sigma_tol, gamma_tol)

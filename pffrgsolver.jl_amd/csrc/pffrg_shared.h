@@ -36,13 +36,16 @@
 namespace pffrg {
 
 constexpr int SH_MAXT = 512;  // threads per block, at most (8 waves)
-constexpr int SH_K = 4;       // quadrature nodes evaluated per batch
 constexpr int SH_SPT = 3;     // sites per wave, at most (the pass takes up to 8 * SH_SPT sites)
 constexpr int SH_T = 16;      // tile edge (the matrix cores' 16 x 16 result)
 constexpr int SH_NPART = 5;   // partial sums per element: the four domains, then the two tail nodes
 constexpr int SH_NODES = 256; // nodes whose positions and bubbles are prepared together
 
 typedef float sh_v4f __attribute__((ext_vector_type(4)));
+
+// quadrature nodes evaluated per batch: 8, or 4 at three sites per wave (whose per-lane sums take
+// the LDS that more nodes would need)
+__host__ __device__ constexpr int shared_batch_nodes(int spt) { return spt >= 3 ? 4 : 8; }
 
 // row stride (in float2) of the site vectors in LDS: odd, so that rows of neighbouring tile
 // positions start in different LDS banks
@@ -98,10 +101,11 @@ __host__ __device__ inline SharedLds shared_lds(const Sys &y, int n_ov, int nw, 
     L.o_hint = o; o += ((HINT_WORDS + 1) & ~1) * 4;
     o = (o + 15) & ~15;
     const int base = o;
-    L.o_desc = o; o += SH_K * 4 * SH_T * 3 * (int)sizeof(DescT<float>);
+    const int K = shared_batch_nodes(spt);
+    L.o_desc = o; o += K * 4 * SH_T * 3 * (int)sizeof(DescT<float>);
     o = (o + 15) & ~15;
-    L.o_V = o; o += SH_K * 2 * SH_T * shared_vrow(y.ns) * 8;
-    L.o_loc = o; o += SH_K * 2 * SH_T * 8;
+    L.o_V = o; o += K * 2 * SH_T * shared_vrow(y.ns) * 8;
+    L.o_loc = o; o += K * 2 * SH_T * 8;
     L.o_red = base;
     const int red_end = base + nw * SH_T * SH_T * 3 * 8;
     if (red_end > o) o = red_end;
@@ -165,7 +169,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     __shared__ int s_act2[SH_T], s_act3[SH_T], s_nact[2];
     __shared__ int s_rowf[SH_T], s_colf[SH_T];
     __shared__ unsigned long long s_cnt[4];
-    __shared__ unsigned long long s_prof[2 * PR_N];
+    __shared__ unsigned long long s_prof[2 * PR_N + 7];
 
     const PassArgs<float> &P = A.P;
     const Sys &y = P.y;
@@ -175,7 +179,8 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     const int wv = t >> 6;  // this thread's wave
     const int nw = A.nw;
     const int n_ov = P.ov_off[y.ns];
-    const SharedLds Lay = shared_lds(y, n_ov, nw, PASS == P_KAT, SPT);
+    // the LDS layout and batch size are the problem's own (A.spt), whatever SPT a batched launch uses
+    const SharedLds Lay = shared_lds(y, n_ov, nw, PASS == P_KAT, A.spt);
     const int mp = Lay.mp;
     double *s_mesh = reinterpret_cast<double *>(s_dyn);
     double *s_sigA = reinterpret_cast<double *>(s_dyn + Lay.o_sig);
@@ -224,7 +229,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
         for (int k = t; k < HINT_WORDS; k += NT) s_hint[k] = gh[k];
     }
     if (t < 4) s_cnt[t] = 0;
-    if (t < 2 * PR_N) s_prof[t] = 0;
+    if (t < 2 * PR_N + 7) s_prof[t] = 0;
 #ifdef PFFRG_SHARED_PROFILE
     const unsigned long long t_begin = clock64();
 #endif
@@ -295,6 +300,8 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
 
     // per-lane quadrature state: output o = (site slot s, component c, element r) at (s * 2 + c) * 4 + r
     constexpr int NO = SPT * 2 * 4;
+    constexpr int KM = 8;                      // nodes per batch, at most
+    const int K = shared_batch_nodes(A.spt);  // nodes per batch of this problem
     double b2[NO];
     float Pp[NO];
 #pragma unroll
@@ -333,6 +340,18 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
 #ifdef PFFRG_SHARED_PROFILE
         unsigned long long _tb = (t == 0) ? clock64() : 0ull;
         if (t == 0 && A.count) s_prof[prof_base + PR_BATCHES] += 1;
+        if (wv == 0 && A.count) {
+            // active elements of the tile in this batch: a histogram and their sum
+            int na = 0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) na += __popcll(__ballot(act[r]));
+            if (t == 0) {
+                const int bk = na <= 4 ? 0 : (na <= 16 ? 1 : (na <= 64 ? 2 : (na <= 128 ? 3 : 4)));
+                s_prof[2 * PR_N + bk] += 1;
+                s_prof[2 * PR_N + 5] += (unsigned long long)na * nb;
+                s_prof[2 * PR_N + 6] += (unsigned long long)(s_nact[0] + s_nact[1]) * nb;
+            }
+        }
 #define SH_PROF_LAP(k) do { if (t == 0 && A.count) { const unsigned long long _n = clock64(); s_prof[prof_base + (k)] += _n - _tb; _tb = _n; } } while (0)
 #else
 #define SH_PROF_LAP(k) do { } while (0)
@@ -405,9 +424,11 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                         s_loc[(k * 2 + (s - 2)) * SH_T + ee] = float2{vs, vd};
                     }
                 }
-                // the t contraction sums over all SH_K node slots: those beyond the batch hold zeros
-                if (nb < SH_K) {
-                    const int nz = SH_K - nb;
+                // the t contraction sums over the node slots of whole groups of four: those beyond
+                // the batch hold zeros
+                const int ngr = 4 * ((nb + 3) / 4);
+                if (nb < ngr) {
+                    const int nz = ngr - nb;
 #pragma unroll
                     for (int s = 0; s < 2; ++s) {
                         const int nA = s == 0 ? nA2 : nA3;
@@ -464,47 +485,93 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                     // with the site-0 vertices v3 (ν) and v4 (ν'), then one term per overlap row
                     // (j1, j2, m) of site i, whose factor -2 S² m sits beside the row; the ν'
                     // vertices carry -tp, and node slots beyond the batch hold zeros
-                    const float2 *V0 = s_V + (kq * 2 + 0) * SH_T * VR + pos * VR;
-                    const float2 *V1 = s_V + (kq * 2 + 1) * SH_T * VR + pos * VR;
-                    const float2 v1i = V0[i], v2i = V1[i];
-                    const float2 v3 = s_loc[(kq * 2 + 0) * SH_T + pos], v4 = s_loc[(kq * 2 + 1) * SH_T + pos];
-                    Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(v1i.x, -1.0f * v4.x + v4.y, Ds, 0, 0, 0);
-                    Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(v1i.y, 3.0f * v4.x + v4.y, Dd, 0, 0, 0);
-                    Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(-1.0f * v3.x + v3.y, v2i.x, Ds, 0, 0, 0);
-                    Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(3.0f * v3.x + v3.y, v2i.y, Dd, 0, 0, 0);
+                    constexpr int NG = KM / 4;
+                    const int ng = (nb + 3) / 4;  // groups of four nodes holding the batch
+                    const float2 *V0[NG], *V1[NG];
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) {
+                        V0[g] = s_V + ((4 * g + kq) * 2 + 0) * SH_T * VR + pos * VR;
+                        V1[g] = s_V + ((4 * g + kq) * 2 + 1) * SH_T * VR + pos * VR;
+                        if (g < ng) {
+                            const int kn = 4 * g + kq;
+                            const float2 v1i = V0[g][i], v2i = V1[g][i];
+                            const float2 v3 = s_loc[(kn * 2 + 0) * SH_T + pos], v4 = s_loc[(kn * 2 + 1) * SH_T + pos];
+                            Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(v1i.x, -1.0f * v4.x + v4.y, Ds, 0, 0, 0);
+                            Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(v1i.y, 3.0f * v4.x + v4.y, Dd, 0, 0, 0);
+                            Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(-1.0f * v3.x + v3.y, v2i.x, Ds, 0, 0, 0);
+                            Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(3.0f * v3.x + v3.y, v2i.y, Dd, 0, 0, 0);
+                        }
+                    }
                     // the rows, 64 at a time, one per lane; each step reads its row from a lane
-                    // (v_readlane), so that only the two vertex reads wait on LDS
+                    // (v_readlane), so that only the vertex reads wait on LDS
                     const int r0 = __builtin_amdgcn_readfirstlane(s_ovoff[i]);
                     const int r1 = __builtin_amdgcn_readfirstlane(s_ovoff[i + 1]);
                     for (int cb = r0; cb < r1; cb += 64) {
                         const int nr = (r1 - cb) < 64 ? r1 - cb : 64;
                         const uint2 rl = (lane < nr) ? s_ov[cb + lane] : uint2{0u, 0u};
                         const int rj = (int)rl.x, rm = (int)rl.y;
-                        // four rows per step: their eight vertex reads are issued together
-                        int r = 0;
-                        for (; r + 4 <= nr; r += 4) {
-                            float2 e1[4], f2[4];
-                            float mm[4];
+                        if (ng == 1) {
+                            // four rows per step: their eight vertex reads are issued together
+                            int r = 0;
+                            for (; r + 4 <= nr; r += 4) {
+                                float2 e1[4], f2[4];
+                                float mm[4];
 #pragma unroll
-                            for (int u = 0; u < 4; ++u) {
-                                const uint32_t jj = (uint32_t)__builtin_amdgcn_readlane(rj, r + u);
-                                mm[u] = __int_as_float(__builtin_amdgcn_readlane(rm, r + u));
-                                e1[u] = V0[jj & 255];
-                                f2[u] = V1[(jj >> 8) & 255];
-                            }
+                                for (int u = 0; u < 4; ++u) {
+                                    const uint32_t jj = (uint32_t)__builtin_amdgcn_readlane(rj, r + u);
+                                    mm[u] = __int_as_float(__builtin_amdgcn_readlane(rm, r + u));
+                                    e1[u] = V0[0][jj & 255];
+                                    f2[u] = V1[0][(jj >> 8) & 255];
+                                }
 #pragma unroll
-                            for (int u = 0; u < 4; ++u) {
-                                Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(e1[u].x, mm[u] * f2[u].x, Ds, 0, 0, 0);
-                                Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(e1[u].y, mm[u] * f2[u].y, Dd, 0, 0, 0);
+                                for (int u = 0; u < 4; ++u) {
+                                    Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(e1[u].x, mm[u] * f2[u].x, Ds, 0, 0, 0);
+                                    Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(e1[u].y, mm[u] * f2[u].y, Dd, 0, 0, 0);
+                                }
                             }
-                        }
-                        for (; r < nr; ++r) {
-                            const uint32_t jj = (uint32_t)__builtin_amdgcn_readlane(rj, r);
-                            const float mm = __int_as_float(__builtin_amdgcn_readlane(rm, r));
-                            const float2 e1 = V0[jj & 255];
-                            const float2 f2 = V1[(jj >> 8) & 255];
-                            Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.x, mm * f2.x, Ds, 0, 0, 0);
-                            Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.y, mm * f2.y, Dd, 0, 0, 0);
+                            for (; r < nr; ++r) {
+                                const uint32_t jj = (uint32_t)__builtin_amdgcn_readlane(rj, r);
+                                const float mm = __int_as_float(__builtin_amdgcn_readlane(rm, r));
+                                const float2 e1 = V0[0][jj & 255];
+                                const float2 f2 = V1[0][(jj >> 8) & 255];
+                                Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.x, mm * f2.x, Ds, 0, 0, 0);
+                                Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.y, mm * f2.y, Dd, 0, 0, 0);
+                            }
+                        } else {
+                            // every group of four nodes per row: 2 NG vertex reads per row
+                            int r = 0;
+                            for (; r + 2 <= nr; r += 2) {
+                                float2 e1[2][NG], f2[2][NG];
+                                float mm[2];
+#pragma unroll
+                                for (int u = 0; u < 2; ++u) {
+                                    const uint32_t jj = (uint32_t)__builtin_amdgcn_readlane(rj, r + u);
+                                    mm[u] = __int_as_float(__builtin_amdgcn_readlane(rm, r + u));
+#pragma unroll
+                                    for (int g = 0; g < NG; ++g) {
+                                        e1[u][g] = V0[g][jj & 255];
+                                        f2[u][g] = V1[g][(jj >> 8) & 255];
+                                    }
+                                }
+#pragma unroll
+                                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                                    for (int g = 0; g < NG; ++g) {
+                                        Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(e1[u][g].x, mm[u] * f2[u][g].x, Ds, 0, 0, 0);
+                                        Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(e1[u][g].y, mm[u] * f2[u][g].y, Dd, 0, 0, 0);
+                                    }
+                            }
+                            for (; r < nr; ++r) {
+                                const uint32_t jj = (uint32_t)__builtin_amdgcn_readlane(rj, r);
+                                const float mm = __int_as_float(__builtin_amdgcn_readlane(rm, r));
+#pragma unroll
+                                for (int g = 0; g < NG; ++g) {
+                                    const float2 e1 = V0[g][jj & 255];
+                                    const float2 f2 = V1[g][(jj >> 8) & 255];
+                                    Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.x, mm * f2.x, Ds, 0, 0, 0);
+                                    Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.y, mm * f2.y, Dd, 0, 0, 0);
+                                }
+                            }
                         }
                     }
                 }
@@ -624,7 +691,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
         // the next level: S1 <- S2, S2 <- S2 / 2 (quadrature.jl:74-76)
 #pragma unroll
         for (int o = 0; o < NO; ++o) {
-            b1[o * NT] = (float)b2[o];
+            if ((o >> 3) < nsw) b1[o * NT] = (float)b2[o];
             b2[o] = 0.5 * b2[o];
         }
         const bool more = __syncthreads_or(any ? 1 : 0);
@@ -635,7 +702,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     // Quadrature of the unit's sub-intervals in order, then (last domain) the two tail nodes.
     // One preparation and one evaluation call site serve every step: the three nodes of the
     // initial Simpson rule (a and b, then the midpoint, whose weights differ), the new nodes of
-    // each level (SH_NODES prepared at a time, SH_K evaluated at a time), and the tail nodes
+    // each level (SH_NODES prepared at a time, K evaluated at a time), and the tail nodes
     // (one at a time: their weights differ).
     enum { ST_SUB, ST_INIT, ST_LEVEL, ST_TAIL };
     int st = ST_SUB, kk = -1, n = 0, q0 = 0;
@@ -698,7 +765,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             need_prep = false;
             q0 = 0;
         }
-        const int nb = (st == ST_INIT) ? (q0 == 0 ? 2 : 1) : (st == ST_TAIL ? 1 : ((n - q0) < SH_K ? n - q0 : SH_K));
+        const int nb = (st == ST_INIT) ? (q0 == 0 ? 2 : 1) : (st == ST_TAIL ? 1 : ((n - q0) < K ? n - q0 : K));
         eval_batch(q0, nb);
         q0 += nb;
         if (q0 < n) continue;
@@ -706,7 +773,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             // S1, then S2 starts from S1 / 2 (quadrature.jl:62-66)
 #pragma unroll
             for (int o = 0; o < NO; ++o) {
-                b1[o * NT] = (float)b2[o];
+                if ((o >> 3) < nsw) b1[o * NT] = (float)b2[o];
                 b2[o] = 0.5 * b2[o];
             }
             nlev = 2;
@@ -751,7 +818,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             atomicAdd(A.count + 1, s_cnt[1]);
 #ifdef PFFRG_SHARED_PROFILE
             s_prof[prof_base + PR_TOTAL] = clock64() - t_begin;
-            for (int k = 0; k < 2 * PR_N; ++k) atomicAdd(A.count + 2 + k, s_prof[k]);
+            for (int k = 0; k < 2 * PR_N + 7; ++k) atomicAdd(A.count + 2 + k, s_prof[k]);
 #endif
         }
     }
