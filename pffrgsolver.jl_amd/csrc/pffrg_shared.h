@@ -31,6 +31,8 @@
 // sites w, w + nw, w + 2nw, ... (nw waves): the layout of the matrix cores' result.
 #pragma once
 
+#include <type_traits>
+
 #include "pffrg_pass.h"
 
 namespace pffrg {
@@ -169,7 +171,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     __shared__ int s_act2[SH_T], s_act3[SH_T], s_nact[2];
     __shared__ int s_rowf[SH_T], s_colf[SH_T];
     __shared__ unsigned long long s_cnt[4];
-    __shared__ unsigned long long s_prof[2 * PR_N + 7];
+    __shared__ unsigned long long s_prof[2 * PR_N + 10];
 
     const PassArgs<float> &P = A.P;
     const Sys &y = P.y;
@@ -216,11 +218,23 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     for (int k = t; k < y.ns; k += NT) s_exch[k] = P.exch[k];
     {
         // overlap rows (j1 | j2 << 8 | m << 16) beside the FP32 factor -2 S² m of their term
-        // (flow_su2_t.jl:59-74)
+        // (flow_su2_t.jl:59-74), each site's rows ordered by j1 (stable: by table position
+        // within one j1) and the last row of every j1 run flagged in bit 31, so that the
+        // contraction sums the ν' vectors of a run before one product with the ν vector of j1
         const float mfac = (float)(-2.0 * y.S2);
         for (int k = t; k < n_ov; k += NT) {
             const uint32_t r = P.ov_rows[k];
-            s_ov[k] = uint2{r, __float_as_uint(mfac * (float)(r >> 16))};
+            int i = 0;
+            while (P.ov_off[i + 1] <= k) ++i;
+            const int j1 = (int)(r & 255u);
+            int pos = P.ov_off[i];
+            bool last = true;
+            for (int q = P.ov_off[i]; q < P.ov_off[i + 1]; ++q) {
+                const int jq = (int)(P.ov_rows[q] & 255u);
+                pos += (jq < j1 || (jq == j1 && q < k)) ? 1 : 0;
+                last = last && !(jq == j1 && q > k);
+            }
+            s_ov[pos] = uint2{r | (last ? 0x80000000u : 0u), __float_as_uint(mfac * (float)((r >> 16) & 0x7fffu))};
         }
     }
     for (int k = t; k <= y.ns; k += NT) s_ovoff[k] = P.ov_off[k];
@@ -229,7 +243,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
         for (int k = t; k < HINT_WORDS; k += NT) s_hint[k] = gh[k];
     }
     if (t < 4) s_cnt[t] = 0;
-    if (t < 2 * PR_N + 7) s_prof[t] = 0;
+    if (t < 2 * PR_N + 10) s_prof[t] = 0;
 #ifdef PFFRG_SHARED_PROFILE
     const unsigned long long t_begin = clock64();
 #endif
@@ -374,27 +388,92 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             }
         }
         __syncthreads();
+#if defined(PFFRG_SHARED_PROFILE) && defined(PFFRG_SHARED_PROFILE_CELLS)
+        if (A.count) {
+            // node pairs (k-1, k) of the batch whose descriptors of one (slot, vector) read the
+            // same corners in every channel: [0] pairs, [1] same in all channels, [2] channels same
+            const int per_k = (nslot / 2) * nAS;
+            unsigned long long np_ = 0, ns_ = 0, nc_ = 0;
+            for (int tau = t; tau < (nb - 1) * per_k; tau += NT) {
+                const int k = 1 + tau / per_k, q = tau % per_k;
+                const int pr = q / nAS, u = q - pr * nAS;
+                const bool row = u < nA2;
+                const int s = 2 * pr + (row ? 0 : 1);
+                const int ee = row ? s_act2[u] : s_act3[u - nA2];
+                const DescT<float> *d1 = s_desc + ((k * 4 + s) * SH_T + ee) * 3;
+                const DescT<float> *d0 = s_desc + (((k - 1) * 4 + s) * SH_T + ee) * 3;
+                int same = 0;
+                for (int c = 0; c < 3; ++c) same += (d1[c].meta == d0[c].meta && d1[c].off == d0[c].off);
+                np_ += 1; ns_ += (same == 3); nc_ += same;
+            }
+            if (np_) { atomicAdd(&s_prof[2 * PR_N + 7], np_); atomicAdd(&s_prof[2 * PR_N + 8], ns_); atomicAdd(&s_prof[2 * PR_N + 9], nc_); }
+        }
+#endif
         SH_PROF_LAP(PR_DESC);
         // site vectors of slots 0 (rows) and 1 (columns) on every site, and of the t channel's
-        // local slots 2 and 3 on site 0; rows and columns in separate loops, so that the vertex
-        // read is the same for the whole loop
+        // local slots 2 and 3 on site 0.  A thread takes one (slot, vector, site) through all
+        // nodes of the batch: consecutive nodes mostly interpolate in the same mesh cells (the
+        // same corner offsets and flags in all three channels), and then the corners loaded
+        // for the previous node are reused with the new node's weights.
         {
             unsigned long long cb = 0;
+            const int nt0 = nA2 * y.ns;
+            const int ntask = (nA2 + nA3) * y.ns;
+            const bool rowD = (vertex_cfg<PASS>(ch, 0) & 8) != 0, colD = (vertex_cfg<PASS>(ch, 1) & 8) != 0;
+            const float2 *qA = reinterpret_cast<const float2 *>(P.A.gamma);
+            const float2 *qD = reinterpret_cast<const float2 *>(P.D.gamma);
+            // the batch's nodes of one task are cut into R runs (R threads per task) when there
+            // are fewer tasks than threads, so that sparse batches keep every thread busy
+            int R = 1;
+            while (R < nb && 2 * R * ntask <= NT) R *= 2;
+            const int C = (nb + R - 1) / R;
+            for (int tau = t; tau < ntask * R; tau += NT) {
+                const int task = tau / R, run = tau - task * R;
+                const int s = task < nt0 ? 0 : 1;
+                const int r = s ? task - nt0 : task;
+                const int u = r / y.ns, i = r - u * y.ns;
+                const int ee = s ? s_act3[u] : s_act2[u];
+                const bool useD = s ? colD : rowD;
+                const float2 *q = useD ? qD : qA;
+                const double *bare = useD ? P.D.bare : P.A.bare;
+                const float b0 = (float)bare[i], b1v = (float)bare[y.ns + i];
+                const int sx0 = s_exch[i];
+                float2 x[3][8];
+                int pm[3] = {-1, -1, -1}, po[3] = {0, 0, 0};
+                const int k1 = (run + 1) * C < nb ? (run + 1) * C : nb;
+                for (int k = run * C; k < k1; ++k) {
+                    const DescT<float> *ds = s_desc + ((k * 4 + s) * SH_T + ee) * 3;
+                    int meta[3];
+                    // the corners of every channel whose cell changed are loaded first, then summed
 #pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                const int nA = s == 0 ? nA2 : nA3;
-                const int *act_l = s == 0 ? s_act2 : s_act3;
-                const ActV<float> &V = (vertex_cfg<PASS>(ch, s) & 8) ? P.D : P.A;
-                const int per_k = nA * y.ns;
-                const int ntask = nb * per_k;
-                for (int tau = t; tau < ntask; tau += NT) {
-                    const int k = tau / per_k, r = tau - k * per_k;
-                    const int u = r / y.ns, i = r - u * y.ns;
-                    const int ee = act_l[u];
-                    float vs = (float)V.bare[i], vd = (float)V.bare[y.ns + i];
-                    int cn;
-                    gather_slot<float>(s_desc + ((k * 4 + s) * SH_T + ee) * 3, V.gamma, s_exch, i, y, vs, vd, cn);
-                    cb += cn;
+                    for (int c = 0; c < 3; ++c) {
+                        meta[c] = ds[c].meta;
+                        const int off = ds[c].off;
+                        if (meta[c] != pm[c] || off != po[c]) {
+                            const int n = wave_corners(meta[c]);
+                            const bool on = (meta[c] & 7) != 0;
+                            const int sx = (meta[c] & 8) ? sx0 : i;
+                            const bool map = (meta[c] & 16) != 0;
+                            const int chn = (c == 1) ? 1 : ((c == 0) == map ? 2 : 0);
+                            const uint32_t o = on ? (uint32_t)(chn * y.chlen + off + sx) : 0u;
+                            uint32_t c1, c2, c3;
+                            corner_steps(meta[c], y, c1, c2, c3);
+                            if (n) load_corners(q, o, c1, c2, c3, n, x[c]);
+                            pm[c] = meta[c];
+                            po[c] = off;
+                        }
+                    }
+                    float vs = b0, vd = b1v;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        if (meta[c] & 7) {
+                            float r0, r1;
+                            corner_sum(ds[c], corners(meta[c] & 7), x[c], r0, r1);
+                            vs += r0;
+                            vd += (meta[c] & 32) ? -r1 : r1;
+                            cb += corners(meta[c] & 7);
+                        }
+                    }
                     if (s == 1) {
                         // every term's ν' factor carries the node's -tp
                         const float mt = -s_tp[q0 + k];
@@ -506,73 +585,61 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                     // (v_readlane), so that only the vertex reads wait on LDS
                     const int r0 = __builtin_amdgcn_readfirstlane(s_ovoff[i]);
                     const int r1 = __builtin_amdgcn_readfirstlane(s_ovoff[i + 1]);
+                    float bx[NG], by[NG];  // the current j1 run's ν' sums (a run may cross a chunk)
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) { bx[g] = 0.f; by[g] = 0.f; }
                     for (int cb = r0; cb < r1; cb += 64) {
                         const int nr = (r1 - cb) < 64 ? r1 - cb : 64;
                         const uint2 rl = (lane < nr) ? s_ov[cb + lane] : uint2{0u, 0u};
                         const int rj = (int)rl.x, rm = (int)rl.y;
-                        if (ng == 1) {
-                            // four rows per step: their eight vertex reads are issued together
+                        // rows in j1 runs: the ν' factors m v(j2) of a run are summed on the
+                        // vector unit, then one product per run and group of four nodes with
+                        // the ν vector of j1; the vertex reads of four rows are issued together
+                        auto rows = [&](auto ngc) {
+                            constexpr int G = decltype(ngc)::value;
+                            auto step = [&](uint32_t jj, float mm, const float2 *f2) {
+#pragma unroll
+                                for (int g = 0; g < G; ++g) {
+                                    bx[g] += mm * f2[g].x;
+                                    by[g] += mm * f2[g].y;
+                                }
+                                if (jj >> 31) {
+#pragma unroll
+                                    for (int g = 0; g < G; ++g) {
+                                        const float2 e1 = V0[g][jj & 255];
+                                        Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.x, bx[g], Ds, 0, 0, 0);
+                                        Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.y, by[g], Dd, 0, 0, 0);
+                                        bx[g] = 0.f;
+                                        by[g] = 0.f;
+                                    }
+                                }
+                            };
                             int r = 0;
                             for (; r + 4 <= nr; r += 4) {
-                                float2 e1[4], f2[4];
+                                uint32_t jj[4];
                                 float mm[4];
+                                float2 f2[4][G];
 #pragma unroll
                                 for (int u = 0; u < 4; ++u) {
-                                    const uint32_t jj = (uint32_t)__builtin_amdgcn_readlane(rj, r + u);
+                                    jj[u] = (uint32_t)__builtin_amdgcn_readlane(rj, r + u);
                                     mm[u] = __int_as_float(__builtin_amdgcn_readlane(rm, r + u));
-                                    e1[u] = V0[0][jj & 255];
-                                    f2[u] = V1[0][(jj >> 8) & 255];
+#pragma unroll
+                                    for (int g = 0; g < G; ++g) f2[u][g] = V1[g][(jj[u] >> 8) & 255];
                                 }
 #pragma unroll
-                                for (int u = 0; u < 4; ++u) {
-                                    Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(e1[u].x, mm[u] * f2[u].x, Ds, 0, 0, 0);
-                                    Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(e1[u].y, mm[u] * f2[u].y, Dd, 0, 0, 0);
-                                }
+                                for (int u = 0; u < 4; ++u) step(jj[u], mm[u], f2[u]);
                             }
                             for (; r < nr; ++r) {
                                 const uint32_t jj = (uint32_t)__builtin_amdgcn_readlane(rj, r);
                                 const float mm = __int_as_float(__builtin_amdgcn_readlane(rm, r));
-                                const float2 e1 = V0[0][jj & 255];
-                                const float2 f2 = V1[0][(jj >> 8) & 255];
-                                Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.x, mm * f2.x, Ds, 0, 0, 0);
-                                Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.y, mm * f2.y, Dd, 0, 0, 0);
+                                float2 f2[G];
+#pragma unroll
+                                for (int g = 0; g < G; ++g) f2[g] = V1[g][(jj >> 8) & 255];
+                                step(jj, mm, f2);
                             }
-                        } else {
-                            // every group of four nodes per row: 2 NG vertex reads per row
-                            int r = 0;
-                            for (; r + 2 <= nr; r += 2) {
-                                float2 e1[2][NG], f2[2][NG];
-                                float mm[2];
-#pragma unroll
-                                for (int u = 0; u < 2; ++u) {
-                                    const uint32_t jj = (uint32_t)__builtin_amdgcn_readlane(rj, r + u);
-                                    mm[u] = __int_as_float(__builtin_amdgcn_readlane(rm, r + u));
-#pragma unroll
-                                    for (int g = 0; g < NG; ++g) {
-                                        e1[u][g] = V0[g][jj & 255];
-                                        f2[u][g] = V1[g][(jj >> 8) & 255];
-                                    }
-                                }
-#pragma unroll
-                                for (int u = 0; u < 2; ++u)
-#pragma unroll
-                                    for (int g = 0; g < NG; ++g) {
-                                        Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(e1[u][g].x, mm[u] * f2[u][g].x, Ds, 0, 0, 0);
-                                        Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(e1[u][g].y, mm[u] * f2[u][g].y, Dd, 0, 0, 0);
-                                    }
-                            }
-                            for (; r < nr; ++r) {
-                                const uint32_t jj = (uint32_t)__builtin_amdgcn_readlane(rj, r);
-                                const float mm = __int_as_float(__builtin_amdgcn_readlane(rm, r));
-#pragma unroll
-                                for (int g = 0; g < NG; ++g) {
-                                    const float2 e1 = V0[g][jj & 255];
-                                    const float2 f2 = V1[g][(jj >> 8) & 255];
-                                    Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.x, mm * f2.x, Ds, 0, 0, 0);
-                                    Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.y, mm * f2.y, Dd, 0, 0, 0);
-                                }
-                            }
-                        }
+                        };
+                        if (ng == 1) rows(std::integral_constant<int, 1>{});
+                        else rows(std::integral_constant<int, NG>{});
                     }
                 }
 #pragma unroll
@@ -818,7 +885,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             atomicAdd(A.count + 1, s_cnt[1]);
 #ifdef PFFRG_SHARED_PROFILE
             s_prof[prof_base + PR_TOTAL] = clock64() - t_begin;
-            for (int k = 0; k < 2 * PR_N + 7; ++k) atomicAdd(A.count + 2 + k, s_prof[k]);
+            for (int k = 0; k < 2 * PR_N + 10; ++k) atomicAdd(A.count + 2 + k, s_prof[k]);
 #endif
         }
     }
