@@ -42,11 +42,13 @@ constexpr int SH_SPT = 3;     // sites per wave, at most (the pass takes up to 8
 constexpr int SH_T = 16;      // tile edge (the matrix cores' 16 x 16 result)
 constexpr int SH_NPART = 5;   // partial sums per element: the four domains, then the two tail nodes
 constexpr int SH_NODES = 256; // nodes whose positions and bubbles are prepared together
+constexpr int SH_KMAX = 64;   // nodes per batch, at most (levels with few active vectors)
 
 typedef float sh_v4f __attribute__((ext_vector_type(4)));
 
-// quadrature nodes evaluated per batch: 8, or 4 at three sites per wave (whose per-lane sums take
-// the LDS that more nodes would need)
+// quadrature nodes of a batch with all 32 row and column vectors active: 8, or 4 at three sites
+// per wave (whose per-lane sums take the LDS that more nodes would need); a batch holds this
+// many nodes times 32 site vectors, so levels with fewer active vectors take more nodes per batch
 __host__ __device__ constexpr int shared_batch_nodes(int spt) { return spt >= 3 ? 4 : 8; }
 
 // row stride (in float2) of the site vectors in LDS: odd, so that rows of neighbouring tile
@@ -85,7 +87,7 @@ __host__ __device__ inline bool shared_in_pass(int mode, int w2x, int w3x, int n
 // vectors and local vectors of a batch share their space with the cross-wave norm sums of a
 // convergence test, which never overlaps a batch.
 struct SharedLds {
-    int mp, o_sig, o_mf, o_ov, o_ovoff, o_exch, o_hint, o_desc, o_V, o_loc, o_red, o_nodes, o_b1, total;
+    int mp, o_sig, o_mf, o_ov, o_ovoff, o_exch, o_hint, o_desc, o_V, o_Z, o_loc, o_red, o_nodes, o_b1, total;
 };
 
 __host__ __device__ inline SharedLds shared_lds(const Sys &y, int n_ov, int nw, bool sig_d, int spt) {
@@ -107,6 +109,7 @@ __host__ __device__ inline SharedLds shared_lds(const Sys &y, int n_ov, int nw, 
     L.o_desc = o; o += K * 4 * SH_T * 3 * (int)sizeof(DescT<float>);
     o = (o + 15) & ~15;
     L.o_V = o; o += K * 2 * SH_T * shared_vrow(y.ns) * 8;
+    L.o_Z = o; o += shared_vrow(y.ns) * 8;  // a row of zeros: the site vector of an inactive position
     L.o_loc = o; o += K * 2 * SH_T * 8;
     L.o_red = base;
     const int red_end = base + nw * SH_T * SH_T * 3 * 8;
@@ -169,7 +172,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     __shared__ uint32_t s_code[MAX_DESC];
     __shared__ double s_a1[SH_T], s_a2[SH_T];
     __shared__ int s_act2[SH_T], s_act3[SH_T], s_nact[2];
-    __shared__ int s_rowf[SH_T], s_colf[SH_T];
+    __shared__ int s_rowf[SH_T], s_colf[SH_T], s_rmap[SH_T], s_cmap[SH_T], s_kb;
     __shared__ unsigned long long s_cnt[4];
     __shared__ unsigned long long s_prof[2 * PR_N + 10];
 
@@ -194,6 +197,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     float *mf = reinterpret_cast<float *>(s_dyn + Lay.o_mf);
     DescT<float> *s_desc = reinterpret_cast<DescT<float> *>(s_dyn + Lay.o_desc);
     float2 *s_V = reinterpret_cast<float2 *>(s_dyn + Lay.o_V);
+    float2 *s_Z = reinterpret_cast<float2 *>(s_dyn + Lay.o_Z);
     float2 *s_loc = reinterpret_cast<float2 *>(s_dyn + Lay.o_loc);
     double *s_red = reinterpret_cast<double *>(s_dyn + Lay.o_red);
     double *s_xs = reinterpret_cast<double *>(s_dyn + Lay.o_nodes);  // node positions
@@ -243,6 +247,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
         for (int k = t; k < HINT_WORDS; k += NT) s_hint[k] = gh[k];
     }
     if (t < 4) s_cnt[t] = 0;
+    for (int k = t; k < VR; k += NT) s_Z[k] = float2{0.f, 0.f};
     if (t < 2 * PR_N + 10) s_prof[t] = 0;
 #ifdef PFFRG_SHARED_PROFILE
     const unsigned long long t_begin = clock64();
@@ -314,8 +319,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
 
     // per-lane quadrature state: output o = (site slot s, component c, element r) at (s * 2 + c) * 4 + r
     constexpr int NO = SPT * 2 * 4;
-    constexpr int KM = 8;                      // nodes per batch, at most
-    const int K = shared_batch_nodes(A.spt);  // nodes per batch of this problem
+    const int Kcap = shared_batch_nodes(A.spt) * 32;  // site vectors of one batch (node x vector)
     double b2[NO];
     float Pp[NO];
 #pragma unroll
@@ -349,7 +353,9 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
 
     // Evaluate the integrands of the tile at the prepared nodes q0 .. q0+nb-1, which share one
     // weight w and reweight flag, and accumulate b2 += w Σ_q f_q (FP64) and, when flagged,
-    // Pp += Σ_q f_q.
+    // Pp += Σ_q f_q.  The active rows and columns are numbered compactly (rows first): vector
+    // v of node k has its site vector at s_V[(k nAS + v) VR + i], its descriptors (slot pair pr)
+    // at s_desc[((k npair + pr) nAS + v) 3 + c] and its t-channel local vertex at s_loc[k nAS + v].
     auto eval_batch = [&](int q0, int nb) {
 #ifdef PFFRG_SHARED_PROFILE
         unsigned long long _tb = (t == 0) ? clock64() : 0ull;
@@ -372,19 +378,20 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
 #endif
         const int nA2 = s_nact[0], nA3 = s_nact[1];
         const int nAS = nA2 + nA3;
-        // descriptors (node, slot, row or column, channel)
+        const int npair = nslot / 2;
+        // descriptors (node, slot pair, vector, channel)
         {
-            const int per_k = 3 * (nslot / 2) * nAS;
+            const int per_k = 3 * npair * nAS;
             const int ntask = nb * per_k;
             for (int tau = t; tau < ntask; tau += NT) {
                 const int k = tau / per_k, r = tau - k * per_k;
                 const int c = r % 3, q = r / 3;
-                const int pr = q / nAS, u = q - pr * nAS;
-                const bool row = u < nA2;
+                const int pr = q / nAS, v = q - pr * nAS;
+                const bool row = v < nA2;
                 const int s = 2 * pr + (row ? 0 : 1);
-                const int ee = row ? s_act2[u] : s_act3[u - nA2];
+                const int ee = row ? s_act2[v] : s_act3[v - nA2];
                 build_desc_u<float, double>(S, y, s_code[3 * s + c], c, W, row ? s_a1[ee] : 0.0, row ? 0.0 : s_a2[ee],
-                                            s_xs[q0 + k], s_desc[((k * 4 + s) * SH_T + ee) * 3 + c]);
+                                            s_xs[q0 + k], s_desc[((k * npair + pr) * nAS + v) * 3 + c]);
             }
         }
         __syncthreads();
@@ -392,16 +399,12 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
         if (A.count) {
             // node pairs (k-1, k) of the batch whose descriptors of one (slot, vector) read the
             // same corners in every channel: [0] pairs, [1] same in all channels, [2] channels same
-            const int per_k = (nslot / 2) * nAS;
+            const int per_k = npair * nAS;
             unsigned long long np_ = 0, ns_ = 0, nc_ = 0;
             for (int tau = t; tau < (nb - 1) * per_k; tau += NT) {
                 const int k = 1 + tau / per_k, q = tau % per_k;
-                const int pr = q / nAS, u = q - pr * nAS;
-                const bool row = u < nA2;
-                const int s = 2 * pr + (row ? 0 : 1);
-                const int ee = row ? s_act2[u] : s_act3[u - nA2];
-                const DescT<float> *d1 = s_desc + ((k * 4 + s) * SH_T + ee) * 3;
-                const DescT<float> *d0 = s_desc + (((k - 1) * 4 + s) * SH_T + ee) * 3;
+                const DescT<float> *d1 = s_desc + (k * per_k + q) * 3;
+                const DescT<float> *d0 = s_desc + ((k - 1) * per_k + q) * 3;
                 int same = 0;
                 for (int c = 0; c < 3; ++c) same += (d1[c].meta == d0[c].meta && d1[c].off == d0[c].off);
                 np_ += 1; ns_ += (same == 3); nc_ += same;
@@ -411,14 +414,13 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
 #endif
         SH_PROF_LAP(PR_DESC);
         // site vectors of slots 0 (rows) and 1 (columns) on every site, and of the t channel's
-        // local slots 2 and 3 on site 0.  A thread takes one (slot, vector, site) through all
-        // nodes of the batch: consecutive nodes mostly interpolate in the same mesh cells (the
-        // same corner offsets and flags in all three channels), and then the corners loaded
-        // for the previous node are reused with the new node's weights.
+        // local slots 2 and 3 on site 0.  A thread takes one (vector, site) through a run of the
+        // batch's nodes: consecutive nodes mostly interpolate in the same mesh cells (the same
+        // corner offsets and flags in all three channels), and then the corners loaded for the
+        // previous node are reused with the new node's weights.
         {
             unsigned long long cb = 0;
-            const int nt0 = nA2 * y.ns;
-            const int ntask = (nA2 + nA3) * y.ns;
+            const int ntask = nAS * y.ns;
             const bool rowD = (vertex_cfg<PASS>(ch, 0) & 8) != 0, colD = (vertex_cfg<PASS>(ch, 1) & 8) != 0;
             const float2 *qA = reinterpret_cast<const float2 *>(P.A.gamma);
             const float2 *qD = reinterpret_cast<const float2 *>(P.D.gamma);
@@ -429,10 +431,8 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             const int C = (nb + R - 1) / R;
             for (int tau = t; tau < ntask * R; tau += NT) {
                 const int task = tau / R, run = tau - task * R;
-                const int s = task < nt0 ? 0 : 1;
-                const int r = s ? task - nt0 : task;
-                const int u = r / y.ns, i = r - u * y.ns;
-                const int ee = s ? s_act3[u] : s_act2[u];
+                const int v = task / y.ns, i = task - v * y.ns;
+                const int s = v < nA2 ? 0 : 1;
                 const bool useD = s ? colD : rowD;
                 const float2 *q = useD ? qD : qA;
                 const double *bare = useD ? P.D.bare : P.A.bare;
@@ -442,7 +442,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                 int pm[3] = {-1, -1, -1}, po[3] = {0, 0, 0};
                 const int k1 = (run + 1) * C < nb ? (run + 1) * C : nb;
                 for (int k = run * C; k < k1; ++k) {
-                    const DescT<float> *ds = s_desc + ((k * 4 + s) * SH_T + ee) * 3;
+                    const DescT<float> *ds = s_desc + (k * npair * nAS + v) * 3;
                     int meta[3];
                     // the corners of every channel whose cell changed are loaded first, then summed
 #pragma unroll
@@ -480,49 +480,35 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                         vs *= mt;
                         vd *= mt;
                     }
-                    s_V[((k * 2 + s) * SH_T + ee) * VR + i] = float2{vs, vd};
+                    s_V[(k * nAS + v) * VR + i] = float2{vs, vd};
                 }
             }
             if (ch == 1) {
-#pragma unroll
-                for (int s = 2; s < 4; ++s) {
-                    const int nA = s == 2 ? nA2 : nA3;
-                    const int *act_l = s == 2 ? s_act2 : s_act3;
-                    const ActV<float> &V = (vertex_cfg<PASS>(ch, s) & 8) ? P.D : P.A;
-                    const int ntl = nb * nA;
-                    for (int tau = t; tau < ntl; tau += NT) {
-                        const int k = tau / nA, u = tau - k * nA;
-                        const int ee = act_l[u];
-                        float vs = (float)V.bare[0], vd = (float)V.bare[y.ns];
-                        cb += gather_site(s_desc + ((k * 4 + s) * SH_T + ee) * 3, V.gamma, s_exch, 0, y, vs, vd);
-                        if (s == 3) {
-                            const float mt = -s_tp[q0 + k];
-                            vs *= mt;
-                            vd *= mt;
-                        }
-                        s_loc[(k * 2 + (s - 2)) * SH_T + ee] = float2{vs, vd};
+                const bool locD2 = (vertex_cfg<PASS>(ch, 2) & 8) != 0, locD3 = (vertex_cfg<PASS>(ch, 3) & 8) != 0;
+                const int ntl = nb * nAS;
+                for (int tau = t; tau < ntl; tau += NT) {
+                    const int k = tau / nAS, v = tau - k * nAS;
+                    const int s = v < nA2 ? 2 : 3;
+                    const ActV<float> &V = (s == 2 ? locD2 : locD3) ? P.D : P.A;
+                    float vs = (float)V.bare[0], vd = (float)V.bare[y.ns];
+                    cb += gather_site(s_desc + ((k * 2 + 1) * nAS + v) * 3, V.gamma, s_exch, 0, y, vs, vd);
+                    if (s == 3) {
+                        const float mt = -s_tp[q0 + k];
+                        vs *= mt;
+                        vd *= mt;
                     }
+                    s_loc[k * nAS + v] = float2{vs, vd};
                 }
                 // the t contraction sums over the node slots of whole groups of four: those beyond
                 // the batch hold zeros
                 const int ngr = 4 * ((nb + 3) / 4);
                 if (nb < ngr) {
-                    const int nz = ngr - nb;
-#pragma unroll
-                    for (int s = 0; s < 2; ++s) {
-                        const int nA = s == 0 ? nA2 : nA3;
-                        const int *act_l = s == 0 ? s_act2 : s_act3;
-                        const int per_k = nA * y.ns;
-                        for (int tau = t; tau < nz * per_k; tau += NT) {
-                            const int k = nb + tau / per_k, r = tau % per_k;
-                            const int u = r / y.ns, i = r - u * y.ns;
-                            s_V[((k * 2 + s) * SH_T + act_l[u]) * VR + i] = float2{0.f, 0.f};
-                        }
-                        for (int tau = t; tau < nz * nA; tau += NT) {
-                            const int k = nb + tau / nA, u = tau % nA;
-                            s_loc[(k * 2 + s) * SH_T + act_l[u]] = float2{0.f, 0.f};
-                        }
+                    const int nz = (ngr - nb) * nAS;
+                    for (int tau = t; tau < nz * y.ns; tau += NT) {
+                        const int kv = tau / y.ns, i = tau - kv * y.ns;
+                        s_V[(nb * nAS + kv) * VR + i] = float2{0.f, 0.f};
                     }
+                    for (int tau = t; tau < nz; tau += NT) s_loc[nb * nAS + tau] = float2{0.f, 0.f};
                 }
             }
             if (A.count && cb) atomicAdd(&s_cnt[1], cb * 2ull * sizeof(float));
@@ -530,11 +516,16 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
         __syncthreads();
         SH_PROF_LAP(PR_GATHER);
         // the tile's integrands on this wave's sites, summed over the batch on the matrix cores:
-        // A rows = ν positions, B columns = ν' positions, inner index = (node, term)
+        // A rows = ν positions, B columns = ν' positions, inner index = (node, term).  A row or
+        // column without an active element reads the zero row (node stride 0).
         {
             const double wq = s_wq[q0];
             const bool addp = s_rw[q0] != 0;
             const int pos = lane & 15, kq = lane >> 4;
+            const int vr = s_rmap[pos], vc = s_cmap[pos];
+            const float2 *Rb = vr >= 0 ? s_V + vr * VR : s_Z;
+            const float2 *Cb = vc >= 0 ? s_V + (nA2 + vc) * VR : s_Z;
+            const int Rs = vr >= 0 ? nAS * VR : 0, Cs = vc >= 0 ? nAS * VR : 0;
 #pragma unroll
             for (int s = 0; s < SPT; ++s) {
                 if (s >= nsw) break;
@@ -550,8 +541,8 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                         const int n = kap >> 1, tau = kap & 1;
                         float as = 0.f, bs = 0.f, bd = 0.f;
                         if (n < nb) {
-                            const float2 v1 = s_V[((n * 2 + 0) * SH_T + pos) * VR + i];
-                            const float2 v2 = s_V[((n * 2 + 1) * SH_T + pos) * VR + i];
+                            const float2 v1 = Rb[n * Rs + i];
+                            const float2 v2 = Cb[n * Cs + i];
                             as = tau ? v1.y : v1.x;
                             bs = tau ? v2.x : sgn2 * v2.x + v2.y;
                             bd = tau ? v2.y : 3.0f * v2.x;
@@ -563,83 +554,85 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                     // t (flow_su2_t.jl:56-74), the batch's nodes on the inner index: two local terms
                     // with the site-0 vertices v3 (ν) and v4 (ν'), then one term per overlap row
                     // (j1, j2, m) of site i, whose factor -2 S² m sits beside the row; the ν'
-                    // vertices carry -tp, and node slots beyond the batch hold zeros
-                    constexpr int NG = KM / 4;
+                    // vertices carry -tp, and node slots beyond the batch hold zeros.  The nodes
+                    // go through the rows eight at a time (two groups of four).
                     const int ng = (nb + 3) / 4;  // groups of four nodes holding the batch
-                    const float2 *V0[NG], *V1[NG];
-#pragma unroll
-                    for (int g = 0; g < NG; ++g) {
-                        V0[g] = s_V + ((4 * g + kq) * 2 + 0) * SH_T * VR + pos * VR;
-                        V1[g] = s_V + ((4 * g + kq) * 2 + 1) * SH_T * VR + pos * VR;
-                        if (g < ng) {
-                            const int kn = 4 * g + kq;
-                            const float2 v1i = V0[g][i], v2i = V1[g][i];
-                            const float2 v3 = s_loc[(kn * 2 + 0) * SH_T + pos], v4 = s_loc[(kn * 2 + 1) * SH_T + pos];
-                            Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(v1i.x, -1.0f * v4.x + v4.y, Ds, 0, 0, 0);
-                            Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(v1i.y, 3.0f * v4.x + v4.y, Dd, 0, 0, 0);
-                            Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(-1.0f * v3.x + v3.y, v2i.x, Ds, 0, 0, 0);
-                            Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(3.0f * v3.x + v3.y, v2i.y, Dd, 0, 0, 0);
-                        }
-                    }
-                    // the rows, 64 at a time, one per lane; each step reads its row from a lane
-                    // (v_readlane), so that only the vertex reads wait on LDS
                     const int r0 = __builtin_amdgcn_readfirstlane(s_ovoff[i]);
                     const int r1 = __builtin_amdgcn_readfirstlane(s_ovoff[i + 1]);
-                    float bx[NG], by[NG];  // the current j1 run's ν' sums (a run may cross a chunk)
+                    for (int g0 = 0; g0 < ng; g0 += 2) {
+                        const int gn = (ng - g0) < 2 ? ng - g0 : 2;
+                        const float2 *V0[2], *V1[2];
 #pragma unroll
-                    for (int g = 0; g < NG; ++g) { bx[g] = 0.f; by[g] = 0.f; }
-                    for (int cb = r0; cb < r1; cb += 64) {
-                        const int nr = (r1 - cb) < 64 ? r1 - cb : 64;
-                        const uint2 rl = (lane < nr) ? s_ov[cb + lane] : uint2{0u, 0u};
-                        const int rj = (int)rl.x, rm = (int)rl.y;
-                        // rows in j1 runs: the ν' factors m v(j2) of a run are summed on the
-                        // vector unit, then one product per run and group of four nodes with
-                        // the ν vector of j1; the vertex reads of four rows are issued together
-                        auto rows = [&](auto ngc) {
-                            constexpr int G = decltype(ngc)::value;
-                            auto step = [&](uint32_t jj, float mm, const float2 *f2) {
-#pragma unroll
-                                for (int g = 0; g < G; ++g) {
-                                    bx[g] += mm * f2[g].x;
-                                    by[g] += mm * f2[g].y;
-                                }
-                                if (jj >> 31) {
+                        for (int g = 0; g < 2; ++g) {
+                            const int kn = 4 * (g0 + g) + kq;
+                            V0[g] = Rb + kn * Rs;
+                            V1[g] = Cb + kn * Cs;
+                            if (g < gn) {
+                                const float2 v1i = V0[g][i], v2i = V1[g][i];
+                                const float2 v3 = vr >= 0 ? s_loc[kn * nAS + vr] : float2{0.f, 0.f};
+                                const float2 v4 = vc >= 0 ? s_loc[kn * nAS + nA2 + vc] : float2{0.f, 0.f};
+                                Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(v1i.x, -1.0f * v4.x + v4.y, Ds, 0, 0, 0);
+                                Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(v1i.y, 3.0f * v4.x + v4.y, Dd, 0, 0, 0);
+                                Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(-1.0f * v3.x + v3.y, v2i.x, Ds, 0, 0, 0);
+                                Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(3.0f * v3.x + v3.y, v2i.y, Dd, 0, 0, 0);
+                            }
+                        }
+                        // the rows, 64 at a time, one per lane; each step reads its row from a
+                        // lane (v_readlane), so that only the vertex reads wait on LDS
+                        float bx[2] = {0.f, 0.f}, by[2] = {0.f, 0.f};  // the current j1 run's ν' sums
+                        for (int cb = r0; cb < r1; cb += 64) {
+                            const int nr = (r1 - cb) < 64 ? r1 - cb : 64;
+                            const uint2 rl = (lane < nr) ? s_ov[cb + lane] : uint2{0u, 0u};
+                            const int rj = (int)rl.x, rm = (int)rl.y;
+                            // rows in j1 runs: the ν' factors m v(j2) of a run are summed on the
+                            // vector unit, then one product per run and group of four nodes with
+                            // the ν vector of j1; the vertex reads of four rows are issued together
+                            auto rows = [&](auto ngc) {
+                                constexpr int G = decltype(ngc)::value;
+                                auto step = [&](uint32_t jj, float mm, const float2 *f2) {
 #pragma unroll
                                     for (int g = 0; g < G; ++g) {
-                                        const float2 e1 = V0[g][jj & 255];
-                                        Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.x, bx[g], Ds, 0, 0, 0);
-                                        Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.y, by[g], Dd, 0, 0, 0);
-                                        bx[g] = 0.f;
-                                        by[g] = 0.f;
+                                        bx[g] += mm * f2[g].x;
+                                        by[g] += mm * f2[g].y;
                                     }
+                                    if (jj >> 31) {
+#pragma unroll
+                                        for (int g = 0; g < G; ++g) {
+                                            const float2 e1 = V0[g][jj & 255];
+                                            Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.x, bx[g], Ds, 0, 0, 0);
+                                            Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.y, by[g], Dd, 0, 0, 0);
+                                            bx[g] = 0.f;
+                                            by[g] = 0.f;
+                                        }
+                                    }
+                                };
+                                int r = 0;
+                                for (; r + 4 <= nr; r += 4) {
+                                    uint32_t jj[4];
+                                    float mm[4];
+                                    float2 f2[4][G];
+#pragma unroll
+                                    for (int u = 0; u < 4; ++u) {
+                                        jj[u] = (uint32_t)__builtin_amdgcn_readlane(rj, r + u);
+                                        mm[u] = __int_as_float(__builtin_amdgcn_readlane(rm, r + u));
+#pragma unroll
+                                        for (int g = 0; g < G; ++g) f2[u][g] = V1[g][(jj[u] >> 8) & 255];
+                                    }
+#pragma unroll
+                                    for (int u = 0; u < 4; ++u) step(jj[u], mm[u], f2[u]);
+                                }
+                                for (; r < nr; ++r) {
+                                    const uint32_t jj = (uint32_t)__builtin_amdgcn_readlane(rj, r);
+                                    const float mm = __int_as_float(__builtin_amdgcn_readlane(rm, r));
+                                    float2 f2[G];
+#pragma unroll
+                                    for (int g = 0; g < G; ++g) f2[g] = V1[g][(jj >> 8) & 255];
+                                    step(jj, mm, f2);
                                 }
                             };
-                            int r = 0;
-                            for (; r + 4 <= nr; r += 4) {
-                                uint32_t jj[4];
-                                float mm[4];
-                                float2 f2[4][G];
-#pragma unroll
-                                for (int u = 0; u < 4; ++u) {
-                                    jj[u] = (uint32_t)__builtin_amdgcn_readlane(rj, r + u);
-                                    mm[u] = __int_as_float(__builtin_amdgcn_readlane(rm, r + u));
-#pragma unroll
-                                    for (int g = 0; g < G; ++g) f2[u][g] = V1[g][(jj[u] >> 8) & 255];
-                                }
-#pragma unroll
-                                for (int u = 0; u < 4; ++u) step(jj[u], mm[u], f2[u]);
-                            }
-                            for (; r < nr; ++r) {
-                                const uint32_t jj = (uint32_t)__builtin_amdgcn_readlane(rj, r);
-                                const float mm = __int_as_float(__builtin_amdgcn_readlane(rm, r));
-                                float2 f2[G];
-#pragma unroll
-                                for (int g = 0; g < G; ++g) f2[g] = V1[g][(jj >> 8) & 255];
-                                step(jj, mm, f2);
-                            }
-                        };
-                        if (ng == 1) rows(std::integral_constant<int, 1>{});
-                        else rows(std::integral_constant<int, NG>{});
+                            if (gn == 1) rows(std::integral_constant<int, 1>{});
+                            else rows(std::integral_constant<int, 2>{});
+                        }
                     }
                 }
 #pragma unroll
@@ -664,7 +657,10 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     };
 
     // active rows and columns of the tile: those holding at least one unconverged element,
-    // compacted by the first wave with a ballot
+    // compacted by the first wave with a ballot (s_act2/s_act3: tile positions of the active
+    // rows/columns in order; s_rmap/s_cmap: compact index of each position, -1 if inactive), and
+    // the level's batch size: the most nodes (a power of two, at most SH_KMAX) whose site
+    // vectors fit the batch's capacity of shared_batch_nodes(spt) nodes of 32 vectors
     auto set_active = [&]() {
         SH_PROF_T0();
         if (t < SH_T) { s_rowf[t] = 0; s_colf[t] = 0; }
@@ -684,7 +680,14 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             const unsigned long long mine = isrow ? (bal & 0xffffffffull) : (bal >> 32);
             const int p = __popcll(mine & ((1ull << q) - 1ull));
             if (on) (isrow ? s_act2 : s_act3)[p] = q;
-            if (t == 0) s_nact[0] = __popcll(bal & 0xffffffffull);
+            if (q < SH_T) (isrow ? s_rmap : s_cmap)[q] = on ? p : -1;
+            if (t == 0) {
+                const int nas = __popcll(bal);
+                s_nact[0] = __popcll(bal & 0xffffffffull);
+                int kb = 1;
+                while (2 * kb <= SH_KMAX && 2 * kb * nas <= Kcap) kb *= 2;
+                s_kb = kb;
+            }
             if (t == 32) s_nact[1] = __popcll(bal >> 32);
         }
         __syncthreads();
@@ -832,7 +835,8 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             need_prep = false;
             q0 = 0;
         }
-        const int nb = (st == ST_INIT) ? (q0 == 0 ? 2 : 1) : (st == ST_TAIL ? 1 : ((n - q0) < K ? n - q0 : K));
+        const int kbu = __builtin_amdgcn_readfirstlane(s_kb);
+        const int nb = (st == ST_INIT) ? (q0 == 0 ? 2 : 1) : (st == ST_TAIL ? 1 : ((n - q0) < kbu ? n - q0 : kbu));
         eval_batch(q0, nb);
         q0 += nb;
         if (q0 < n) continue;
