@@ -146,9 +146,11 @@ int pffrg_get_stats(pffrg_ctx *ctx, int64_t *stats);
  * batches, total) in the units of the s and u channels and of the t channel, [18..22] batches by
  * active elements of the tile (1-4, 5-16, 17-64, 65-128, 129-256), [23] active element-nodes,
  * [24] active row and column vectors per node, in a build with PFFRG_SHARED_PROFILE defined
- * (zero otherwise); with PFFRG_SHARED_PROFILE_CELLS also defined, [25] consecutive node pairs of
- * one batch and vector, [26] those reading the same corners in all three channels, [27] channels
- * reading the same corners */
+ * (zero otherwise); with PFFRG_SHARED_PROFILE_CELLS also defined, [25] wave steps of the gather
+ * loop (one node of one site vector per lane), [26] those in which a lane loaded corners, [27] their
+ * cycles; with PFFRG_SHARED_PROFILE_PAIRS instead, [25] consecutive node pairs of one batch and
+ * vector, [26] those reading the same corners in all three channels, [27] channels reading the same
+ * corners */
 int pffrg_get_counters(pffrg_ctx *ctx, int64_t *out, int32_t n);
 /* boundary-correction factors corrs[Ω][channel][side] (2*3*n_Omega doubles) of the last pass */
 int pffrg_get_corrs(pffrg_ctx *ctx, double *corrs);

@@ -28,7 +28,8 @@
 // differ from the per-element FP64 pass by FP32 rounding and summation order only.
 //
 // Lane l of wave w owns elements (ν row 4 (l/16) + r, ν' column l mod 16), r = 0..3, on the
-// sites w, w + nw, w + 2nw, ... (nw waves): the layout of the matrix cores' result.
+// wave's sites (at most spt per wave, assigned so that the waves' overlap rows balance): the
+// layout of the matrix cores' result.
 #pragma once
 
 #include <type_traits>
@@ -173,6 +174,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     __shared__ double s_a1[SH_T], s_a2[SH_T];
     __shared__ int s_act2[SH_T], s_act3[SH_T], s_nact[2];
     __shared__ int s_rowf[SH_T], s_colf[SH_T], s_rmap[SH_T], s_cmap[SH_T], s_kb;
+    __shared__ int s_site[8 * SH_SPT], s_nsw[8];
     __shared__ unsigned long long s_cnt[4];
     __shared__ unsigned long long s_prof[2 * PR_N + 10];
 
@@ -278,6 +280,29 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
 #endif
     const int nn = y.nn;
     if (t < MAX_DESC) s_code[t] = slot_code<PASS>(ch, t / 3, t % 3);
+    if (t == 0) {
+        // sites to waves, at most spt each, balancing the overlap rows that the t-channel
+        // contraction of a site runs through: the site with the most rows first, each to the
+        // least-loaded wave with room (lowest index on ties)
+        int load[8], cntw[8];
+        uint32_t used = 0u;
+        for (int w = 0; w < nw; ++w) { load[w] = 0; cntw[w] = 0; }
+        for (int n = 0; n < y.ns; ++n) {
+            int best = 0, br = -1;
+            for (int j = 0; j < y.ns; ++j) {
+                const int rj = P.ov_off[j + 1] - P.ov_off[j];
+                if (!((used >> j) & 1u) && rj > br) { br = rj; best = j; }
+            }
+            used |= 1u << best;
+            int bw = -1;
+            for (int w = 0; w < nw; ++w)
+                if (cntw[w] < A.spt && (bw < 0 || load[w] < load[bw])) bw = w;
+            s_site[bw * SH_SPT + cntw[bw]] = best;
+            cntw[bw] += 1;
+            load[bw] += br;
+        }
+        for (int w = 0; w < nw; ++w) s_nsw[w] = cntw[w];
+    }
     const double INF = __builtin_huge_val();
     const double *Om = (PASS == P_REDUCED || ch != 1) ? S.Os : S.Ot;
     const double *nu = (PASS == P_REDUCED || ch != 1) ? S.ns : S.nt;
@@ -305,15 +330,20 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     auto part_at = [&](int d, int r) {
         return A.part + shared_part_index(d, ch, w1, t2 * SH_T + r2 + r, w3x, y.nO, A.NX2, A.NX3, y.ns);
     };
-    // sites of this wave: wv, wv + nw, ... (wave-uniform)
-    const int nsw = (y.ns - wv + nw - 1) / nw < A.spt ? (y.ns - wv + nw - 1) / nw : A.spt;
+    // sites of this wave (wave-uniform)
+    const int nsw = __builtin_amdgcn_readfirstlane(s_nsw[wv]);
+    int msite[SPT];
+#pragma unroll
+    for (int s = 0; s < SPT; ++s) msite[s] = __builtin_amdgcn_readfirstlane(s < nsw ? s_site[wv * SH_SPT + s] : 0);
 #pragma unroll
     for (int r = 0; r < 4; ++r)
         if (valid[r]) {
             double *pp = part_at(dom, r);
-            for (int s = 0; s < nsw; ++s) {
-                pp[wv + nw * s] = 0.0;
-                pp[y.ns + wv + nw * s] = 0.0;
+#pragma unroll
+            for (int s = 0; s < SPT; ++s) {
+                if (s >= nsw) break;
+                pp[msite[s]] = 0.0;
+                pp[y.ns + msite[s]] = 0.0;
             }
         }
 
@@ -395,7 +425,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             }
         }
         __syncthreads();
-#if defined(PFFRG_SHARED_PROFILE) && defined(PFFRG_SHARED_PROFILE_CELLS)
+#if defined(PFFRG_SHARED_PROFILE) && defined(PFFRG_SHARED_PROFILE_PAIRS)
         if (A.count) {
             // node pairs (k-1, k) of the batch whose descriptors of one (slot, vector) read the
             // same corners in every channel: [0] pairs, [1] same in all channels, [2] channels same
@@ -442,6 +472,10 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                 int pm[3] = {-1, -1, -1}, po[3] = {0, 0, 0};
                 const int k1 = (run + 1) * C < nb ? (run + 1) * C : nb;
                 for (int k = run * C; k < k1; ++k) {
+#if defined(PFFRG_SHARED_PROFILE) && defined(PFFRG_SHARED_PROFILE_CELLS)
+                    const unsigned long long _g0 = clock64();
+                    bool _rl = false;
+#endif
                     const DescT<float> *ds = s_desc + (k * npair * nAS + v) * 3;
                     int meta[3];
                     // the corners of every channel whose cell changed are loaded first, then summed
@@ -461,6 +495,9 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                             if (n) load_corners(q, o, c1, c2, c3, n, x[c]);
                             pm[c] = meta[c];
                             po[c] = off;
+#if defined(PFFRG_SHARED_PROFILE) && defined(PFFRG_SHARED_PROFILE_CELLS)
+                            _rl = true;
+#endif
                         }
                     }
                     float vs = b0, vd = b1v;
@@ -481,6 +518,17 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                         vd *= mt;
                     }
                     s_V[(k * nAS + v) * VR + i] = float2{vs, vd};
+#if defined(PFFRG_SHARED_PROFILE) && defined(PFFRG_SHARED_PROFILE_CELLS)
+                    if (A.count) {
+                        // wave node steps, those where a lane reloaded corners, and their cycles
+                        const unsigned long long am = __ballot(1), rb = __ballot(_rl);
+                        if (lane == __ffsll((long long)am) - 1) {
+                            atomicAdd(&s_prof[2 * PR_N + 7], 1ull);
+                            atomicAdd(&s_prof[2 * PR_N + 8], rb ? 1ull : 0ull);
+                            atomicAdd(&s_prof[2 * PR_N + 9], clock64() - _g0);
+                        }
+                    }
+#endif
                 }
             }
             if (ch == 1) {
@@ -529,7 +577,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
 #pragma unroll
             for (int s = 0; s < SPT; ++s) {
                 if (s >= nsw) break;
-                const int i = wv + nw * s;
+                const int i = msite[s];
                 sh_v4f Ds = {0.f, 0.f, 0.f, 0.f}, Dd = {0.f, 0.f, 0.f, 0.f};
                 if (ch != 1) {
                     // s and u (flow_su2_s.jl:41-42, flow_su2_u.jl:41-42): Γs = v1s (-tp (∓2 v2s + v2d))
@@ -557,6 +605,8 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                     // vertices carry -tp, and node slots beyond the batch hold zeros.  The nodes
                     // go through the rows eight at a time (two groups of four).
                     const int ng = (nb + 3) / 4;  // groups of four nodes holding the batch
+                    // one accumulator pair per group: four independent product chains
+                    sh_v4f Dsg[2] = {Ds, Ds}, Ddg[2] = {Dd, Dd};
                     const int r0 = __builtin_amdgcn_readfirstlane(s_ovoff[i]);
                     const int r1 = __builtin_amdgcn_readfirstlane(s_ovoff[i + 1]);
                     for (int g0 = 0; g0 < ng; g0 += 2) {
@@ -571,10 +621,10 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                                 const float2 v1i = V0[g][i], v2i = V1[g][i];
                                 const float2 v3 = vr >= 0 ? s_loc[kn * nAS + vr] : float2{0.f, 0.f};
                                 const float2 v4 = vc >= 0 ? s_loc[kn * nAS + nA2 + vc] : float2{0.f, 0.f};
-                                Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(v1i.x, -1.0f * v4.x + v4.y, Ds, 0, 0, 0);
-                                Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(v1i.y, 3.0f * v4.x + v4.y, Dd, 0, 0, 0);
-                                Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(-1.0f * v3.x + v3.y, v2i.x, Ds, 0, 0, 0);
-                                Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(3.0f * v3.x + v3.y, v2i.y, Dd, 0, 0, 0);
+                                Dsg[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(v1i.x, -1.0f * v4.x + v4.y, Dsg[g], 0, 0, 0);
+                                Ddg[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(v1i.y, 3.0f * v4.x + v4.y, Ddg[g], 0, 0, 0);
+                                Dsg[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(-1.0f * v3.x + v3.y, v2i.x, Dsg[g], 0, 0, 0);
+                                Ddg[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(3.0f * v3.x + v3.y, v2i.y, Ddg[g], 0, 0, 0);
                             }
                         }
                         // the rows, 64 at a time, one per lane; each step reads its row from a
@@ -599,8 +649,8 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
 #pragma unroll
                                         for (int g = 0; g < G; ++g) {
                                             const float2 e1 = V0[g][jj & 255];
-                                            Ds = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.x, bx[g], Ds, 0, 0, 0);
-                                            Dd = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.y, by[g], Dd, 0, 0, 0);
+                                            Dsg[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.x, bx[g], Dsg[g], 0, 0, 0);
+                                            Ddg[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.y, by[g], Ddg[g], 0, 0, 0);
                                             bx[g] = 0.f;
                                             by[g] = 0.f;
                                         }
@@ -634,6 +684,8 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                             else rows(std::integral_constant<int, 2>{});
                         }
                     }
+                    Ds = Dsg[0] + Dsg[1];
+                    Dd = Ddg[0] + Ddg[1];
                 }
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
@@ -748,7 +800,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                         const int o = (s * 2 + c) * 4 + r;
                         double R = (double)b1[o * NT] * (-1.0 / 15.0);
                         R += 16.0 / 15.0 * b2[o];
-                        pp[c * y.ns + wv + nw * s] += (R - b2[o]) + b2[o];
+                        pp[c * y.ns + msite[s]] += (R - b2[o]) + b2[o];
                         // one read-modify-write at a time (once per element and sub-interval): keeps
                         // the compiler from holding every load of the loop in registers at once
                         asm volatile("" ::: "memory");
@@ -860,8 +912,8 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
 #pragma unroll
                     for (int s = 0; s < SPT; ++s) {
                         if (s >= nsw) break;
-                        pt[wv + nw * s] = b2[(s * 2 + 0) * 4 + r];
-                        pt[y.ns + wv + nw * s] = b2[(s * 2 + 1) * 4 + r];
+                        pt[msite[s]] = b2[(s * 2 + 0) * 4 + r];
+                        pt[y.ns + msite[s]] = b2[(s * 2 + 1) * 4 + r];
                     }
                 }
             break;
