@@ -358,9 +358,17 @@ __device__ __forceinline__ void bracket_k(ST val, const MeshV<ST> &M, const ST *
 // interpolation (channel.jl:42-305) in the reference's product order.  The unused dimensions of
 // q1/q2 carry the neutral bracket (index 0, weights 1 and 0), which reproduces the reference's
 // weights exactly and leaves the extra corners at weight zero.
-template <typename TV, typename ST>
-__device__ __forceinline__ void build_desc_u(const Lds &S, const Sys &y, uint32_t code, int c, ST W, ST a1, ST a2, ST v,
-                                             DescT<TV> &d) {
+// The descriptor's parts: meta, the offset of the first corner and the three 1-D brackets'
+// weights (lower, upper) along Ω, ν and ν'.
+template <typename ST>
+struct DescParts {
+    int meta, off;
+    ST a1w, b1w, a2w, b2w, a3w, b3w;
+};
+
+template <typename ST>
+__device__ __forceinline__ DescParts<ST> desc_parts(const Lds &S, const Sys &y, uint32_t code, int c, ST W, ST a1, ST a2,
+                                                     ST v) {
     const MeshV<ST> M = meshv<ST>(S);
     ST w = coord<ST>(code >> 8 & 255u, W, v, a1, a2);
     ST x = coord<ST>(code >> 16 & 255u, W, v, a1, a2);
@@ -391,15 +399,26 @@ __device__ __forceinline__ void build_desc_u(const Lds &S, const Sys &y, uint32_
     // the flags of get_buffer_{s,t,u}: s: sgn_μ = ν < 0, sgn_ν = ν' < 0; t: also sgn_μν = Ω < 0;
     // u: sgn_ν = (ν < 0) xor (ν' < 0), sgn_μν = Ω < 0 (buffer.jl:88-199)
     const bool smu = (c != 2) && nx, snu = (c == 2) ? (nx != nz) : nz, smunu = (c != 0) && nw;
-    d.meta = ((code & 1u) && kernel) ? (kernel | (ex ? 8 : 0) | (mp ? 16 : 0) | (negd ? 32 : 0) |
+    DescParts<ST> p;
+    p.meta = ((code & 1u) && kernel) ? (kernel | (ex ? 8 : 0) | (mp ? 16 : 0) | (negd ? 32 : 0) |
                                         (h1 > l1 ? 64 : 0) | (h2 > l2 ? 128 : 0) | (h3 > l3 ? 256 : 0) |
                                         (smu ? 512 : 0) | (snu ? 1024 : 0) | (smunu ? 2048 : 0))
                                      : 0;
-    d.off = base + ns * (l1 + nO * (l2 + nn * l3));
-    d.w[0] = (TV)(a1w * a2w * a3w); d.w[1] = (TV)(b1w * a2w * a3w);
-    d.w[2] = (TV)(a1w * b2w * a3w); d.w[3] = (TV)(b1w * b2w * a3w);
-    d.w[4] = (TV)(a1w * a2w * b3w); d.w[5] = (TV)(b1w * a2w * b3w);
-    d.w[6] = (TV)(a1w * b2w * b3w); d.w[7] = (TV)(b1w * b2w * b3w);
+    p.off = base + ns * (l1 + nO * (l2 + nn * l3));
+    p.a1w = a1w; p.b1w = b1w; p.a2w = a2w; p.b2w = b2w; p.a3w = a3w; p.b3w = b3w;
+    return p;
+}
+
+template <typename TV, typename ST>
+__device__ __forceinline__ void build_desc_u(const Lds &S, const Sys &y, uint32_t code, int c, ST W, ST a1, ST a2, ST v,
+                                             DescT<TV> &d) {
+    const DescParts<ST> p = desc_parts<ST>(S, y, code, c, W, a1, a2, v);
+    d.meta = p.meta;
+    d.off = p.off;
+    d.w[0] = (TV)(p.a1w * p.a2w * p.a3w); d.w[1] = (TV)(p.b1w * p.a2w * p.a3w);
+    d.w[2] = (TV)(p.a1w * p.b2w * p.a3w); d.w[3] = (TV)(p.b1w * p.b2w * p.a3w);
+    d.w[4] = (TV)(p.a1w * p.a2w * p.b3w); d.w[5] = (TV)(p.b1w * p.a2w * p.b3w);
+    d.w[6] = (TV)(p.a1w * p.b2w * p.b3w); d.w[7] = (TV)(p.b1w * p.b2w * p.b3w);
 }
 
 // Corners any group of the wavefront needs for one channel (wave-uniform): 8, 4, 2 or 0.

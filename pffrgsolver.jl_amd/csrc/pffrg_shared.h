@@ -52,6 +52,77 @@ typedef float sh_v4f __attribute__((ext_vector_type(4)));
 // many nodes times 32 site vectors, so levels with fewer active vectors take more nodes per batch
 __host__ __device__ constexpr int shared_batch_nodes(int spt) { return spt >= 3 ? 4 : 8; }
 
+// Descriptor of one buffer lookup in this pass, with everything that does not depend on the site
+// resolved once per (node, vector, channel) where it is built: meta as in DescT (bits 0-11) with
+// the step to the upper Ω corner in bits 16-23, the offset of the first corner inside the vertex
+// copy (channel block included, bit 31 set when the site is taken through the exchange table),
+// the steps to the upper ν and ν' corners, and the eight corner weights, zero beyond the
+// buffer's own corners and for a disabled or out-of-range channel.  48 bytes, three 16-byte
+// LDS reads; a lookup then always reads eight corners (repeated ones at step zero).
+struct __align__(16) DescG {
+    int meta, base, c2, c3;
+    float w[8];
+};
+
+__device__ __forceinline__ void desc_shared(const Lds &S, const Sys &y, uint32_t code, int c, double W, double a1,
+                                            double a2, double v, DescG &d) {
+    const DescParts<double> p = desc_parts<double>(S, y, code, c, W, a1, a2, v);
+    const int mc = p.meta;
+    const bool map = (mc & 16) != 0;
+    const int chn = (c == 1) ? 1 : ((c == 0) == map ? 2 : 0);
+    uint32_t c1, c2, c3;
+    corner_steps(mc, y, c1, c2, c3);
+    const int n = corners(mc & 7);
+    d.meta = mc | (int)(c1 << 16);
+    d.base = (mc & 7) ? (int)((uint32_t)(chn * y.chlen + p.off) | ((mc & 8) ? 0x80000000u : 0u)) : 0;
+    d.c2 = (int)c2;
+    d.c3 = (int)c3;
+    d.w[0] = n ? (float)(p.a1w * p.a2w * p.a3w) : 0.f;
+    d.w[1] = n ? (float)(p.b1w * p.a2w * p.a3w) : 0.f;
+    d.w[2] = n >= 4 ? (float)(p.a1w * p.b2w * p.a3w) : 0.f;
+    d.w[3] = n >= 4 ? (float)(p.b1w * p.b2w * p.a3w) : 0.f;
+    d.w[4] = n >= 8 ? (float)(p.a1w * p.a2w * p.b3w) : 0.f;
+    d.w[5] = n >= 8 ? (float)(p.b1w * p.a2w * p.b3w) : 0.f;
+    d.w[6] = n >= 8 ? (float)(p.a1w * p.b2w * p.b3w) : 0.f;
+    d.w[7] = n >= 8 ? (float)(p.b1w * p.b2w * p.b3w) : 0.f;
+}
+
+// offset of the first corner of descriptor d for site i (exchange partner sx)
+__device__ __forceinline__ uint32_t desc_site_offset(const DescG &d, int i, int sx) {
+    return ((uint32_t)d.base & 0x7fffffffu) + (uint32_t)(d.base < 0 ? sx : i);
+}
+
+// the eight corners of descriptor d at corner offset o (repeated corners where a step is zero)
+__device__ __forceinline__ void load_corners_g(const float2 *__restrict__ q, const DescG &d, uint32_t o, float2 *x) {
+    const uint32_t c1 = ((uint32_t)d.meta >> 16) & 255u, c2 = (uint32_t)d.c2, c3 = (uint32_t)d.c3;
+    x[0] = q[o];
+    x[1] = q[o + c1];
+    x[2] = q[o + c2];
+    x[3] = q[o + c1 + c2];
+    x[4] = q[o + c3];
+    x[5] = q[o + c1 + c3];
+    x[6] = q[o + c2 + c3];
+    x[7] = q[o + c1 + c2 + c3];
+}
+
+// Interpolated (spin, density) pair from n corners, added in the reference's order
+// (channel.jl:42-305); corners beyond the buffer's own count carry weight zero.
+__device__ __forceinline__ void corner_sum_g(const DescG &d, int n, const float2 *x, float &r0, float &r1) {
+    r0 = d.w[0] * x[0].x;
+    r1 = d.w[0] * x[0].y;
+    r0 += d.w[1] * x[1].x; r1 += d.w[1] * x[1].y;
+    if (n >= 4) {
+        r0 += d.w[2] * x[2].x; r1 += d.w[2] * x[2].y;
+        r0 += d.w[3] * x[3].x; r1 += d.w[3] * x[3].y;
+    }
+    if (n >= 8) {
+        r0 += d.w[4] * x[4].x; r1 += d.w[4] * x[4].y;
+        r0 += d.w[5] * x[5].x; r1 += d.w[5] * x[5].y;
+        r0 += d.w[6] * x[6].x; r1 += d.w[6] * x[6].y;
+        r0 += d.w[7] * x[7].x; r1 += d.w[7] * x[7].y;
+    }
+}
+
 // row stride (in float2) of the site vectors in LDS: odd, so that rows of neighbouring tile
 // positions start in different LDS banks
 __host__ __device__ inline int shared_vrow(int ns) { return ns | 1; }
@@ -107,7 +178,7 @@ __host__ __device__ inline SharedLds shared_lds(const Sys &y, int n_ov, int nw, 
     o = (o + 15) & ~15;
     const int base = o;
     const int K = shared_batch_nodes(spt);
-    L.o_desc = o; o += K * 4 * SH_T * 3 * (int)sizeof(DescT<float>);
+    L.o_desc = o; o += K * 4 * SH_T * 3 * (int)sizeof(DescG);
     o = (o + 15) & ~15;
     L.o_V = o; o += K * 2 * SH_T * shared_vrow(y.ns) * 8;
     L.o_Z = o; o += shared_vrow(y.ns) * 8;  // a row of zeros: the site vector of an inactive position
@@ -125,27 +196,24 @@ __host__ __device__ inline SharedLds shared_lds(const Sys &y, int n_ov, int nw, 
 // Vertex of one slot at one site from its three channel descriptors (get_Γ_avx!,
 // action_su2.jl:95-134): bare + Σ_channels sign·interpolation, one channel at a time to bound
 // registers; the corners of a channel are loaded together.  Returns the corners read.
-__device__ __forceinline__ int gather_site(const DescT<float> *ds, const float *gamma, const int *exch, int site,
+__device__ __forceinline__ int gather_site(const DescG *ds, const float *gamma, const int *exch, int site,
                                            const Sys &y, float &vs, float &vd) {
     const float2 *q = reinterpret_cast<const float2 *>(gamma);
     int cn = 0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const DescT<float> &d = ds[c];
+        const DescG &d = ds[c];
         const int meta = d.meta;
         const int kern = meta & 7;
         if (!kern) continue;
-        const int sx = (meta & 8) ? exch[site] : site;
-        const bool map = (meta & 16) != 0;
-        const int chn = (c == 1) ? 1 : ((c == 0) == map ? 2 : 0);
-        const uint32_t o = (uint32_t)(chn * y.chlen + d.off + sx);
+        const uint32_t o = desc_site_offset(d, site, exch[site]);
         uint32_t c1, c2, c3;
-        corner_steps(meta, y, c1, c2, c3);
+        corner_steps(meta & 0xffff, y, c1, c2, c3);
         const int n = corners(kern);
         float2 x[8];
         load_corners(q, o, c1, c2, c3, n, x);
         float r0, r1;
-        corner_sum(d, n, x, r0, r1);
+        corner_sum_g(d, n, x, r0, r1);
         vs += r0;
         vd += (meta & 32) ? -r1 : r1;
         cn += n;
@@ -176,7 +244,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     __shared__ int s_rowf[SH_T], s_colf[SH_T], s_rmap[SH_T], s_cmap[SH_T], s_kb;
     __shared__ int s_site[8 * SH_SPT], s_nsw[8];
     __shared__ unsigned long long s_cnt[4];
-    __shared__ unsigned long long s_prof[2 * PR_N + 10];
+    __shared__ unsigned long long s_prof[2 * PR_N + 11];
 
     const PassArgs<float> &P = A.P;
     const Sys &y = P.y;
@@ -197,7 +265,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     int *s_exch = reinterpret_cast<int *>(s_dyn + Lay.o_exch);
     int32_t *s_hint = reinterpret_cast<int32_t *>(s_dyn + Lay.o_hint);
     float *mf = reinterpret_cast<float *>(s_dyn + Lay.o_mf);
-    DescT<float> *s_desc = reinterpret_cast<DescT<float> *>(s_dyn + Lay.o_desc);
+    DescG *s_desc = reinterpret_cast<DescG *>(s_dyn + Lay.o_desc);
     float2 *s_V = reinterpret_cast<float2 *>(s_dyn + Lay.o_V);
     float2 *s_Z = reinterpret_cast<float2 *>(s_dyn + Lay.o_Z);
     float2 *s_loc = reinterpret_cast<float2 *>(s_dyn + Lay.o_loc);
@@ -250,7 +318,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     }
     if (t < 4) s_cnt[t] = 0;
     for (int k = t; k < VR; k += NT) s_Z[k] = float2{0.f, 0.f};
-    if (t < 2 * PR_N + 10) s_prof[t] = 0;
+    if (t < 2 * PR_N + 11) s_prof[t] = 0;
 #ifdef PFFRG_SHARED_PROFILE
     const unsigned long long t_begin = clock64();
 #endif
@@ -420,8 +488,8 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                 const bool row = v < nA2;
                 const int s = 2 * pr + (row ? 0 : 1);
                 const int ee = row ? s_act2[v] : s_act3[v - nA2];
-                build_desc_u<float, double>(S, y, s_code[3 * s + c], c, W, row ? s_a1[ee] : 0.0, row ? 0.0 : s_a2[ee],
-                                            s_xs[q0 + k], s_desc[((k * npair + pr) * nAS + v) * 3 + c]);
+                desc_shared(S, y, s_code[3 * s + c], c, W, row ? s_a1[ee] : 0.0, row ? 0.0 : s_a2[ee], s_xs[q0 + k],
+                            s_desc[((k * npair + pr) * nAS + v) * 3 + c]);
             }
         }
         __syncthreads();
@@ -433,10 +501,10 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             unsigned long long np_ = 0, ns_ = 0, nc_ = 0;
             for (int tau = t; tau < (nb - 1) * per_k; tau += NT) {
                 const int k = 1 + tau / per_k, q = tau % per_k;
-                const DescT<float> *d1 = s_desc + (k * per_k + q) * 3;
-                const DescT<float> *d0 = s_desc + ((k - 1) * per_k + q) * 3;
+                const DescG *d1 = s_desc + (k * per_k + q) * 3;
+                const DescG *d0 = s_desc + ((k - 1) * per_k + q) * 3;
                 int same = 0;
-                for (int c = 0; c < 3; ++c) same += (d1[c].meta == d0[c].meta && d1[c].off == d0[c].off);
+                for (int c = 0; c < 3; ++c) same += (d1[c].meta == d0[c].meta && d1[c].base == d0[c].base);
                 np_ += 1; ns_ += (same == 3); nc_ += same;
             }
             if (np_) { atomicAdd(&s_prof[2 * PR_N + 7], np_); atomicAdd(&s_prof[2 * PR_N + 8], ns_); atomicAdd(&s_prof[2 * PR_N + 9], nc_); }
@@ -476,41 +544,38 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                     const unsigned long long _g0 = clock64();
                     bool _rl = false;
 #endif
-                    const DescT<float> *ds = s_desc + (k * npair * nAS + v) * 3;
-                    int meta[3];
-                    // the corners of every channel whose cell changed are loaded first, then summed
+                    // the three descriptors are read together, then the corners of every channel
+                    // whose cell changed are loaded (all channels' loads in flight together), then
+                    // summed
+                    // the three descriptors are read together; when any channel's cell changed,
+                    // all 24 corners are loaded (in flight together), then summed with the node's
+                    // weights in straight-line code
+                    const DescG *ds = s_desc + (k * npair * nAS + v) * 3;
+                    DescG d[3];
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        meta[c] = ds[c].meta;
-                        const int off = ds[c].off;
-                        if (meta[c] != pm[c] || off != po[c]) {
-                            const int n = wave_corners(meta[c]);
-                            const bool on = (meta[c] & 7) != 0;
-                            const int sx = (meta[c] & 8) ? sx0 : i;
-                            const bool map = (meta[c] & 16) != 0;
-                            const int chn = (c == 1) ? 1 : ((c == 0) == map ? 2 : 0);
-                            const uint32_t o = on ? (uint32_t)(chn * y.chlen + off + sx) : 0u;
-                            uint32_t c1, c2, c3;
-                            corner_steps(meta[c], y, c1, c2, c3);
-                            if (n) load_corners(q, o, c1, c2, c3, n, x[c]);
-                            pm[c] = meta[c];
-                            po[c] = off;
-#if defined(PFFRG_SHARED_PROFILE) && defined(PFFRG_SHARED_PROFILE_CELLS)
-                            _rl = true;
-#endif
+                    for (int c = 0; c < 3; ++c) d[c] = ds[c];
+                    const bool rl = d[0].meta != pm[0] || d[0].base != po[0] || d[1].meta != pm[1] ||
+                                    d[1].base != po[1] || d[2].meta != pm[2] || d[2].base != po[2];
+                    if (rl) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            load_corners_g(q, d[c], desc_site_offset(d[c], i, sx0), x[c]);
+                            pm[c] = d[c].meta;
+                            po[c] = d[c].base;
                         }
+#if defined(PFFRG_SHARED_PROFILE) && defined(PFFRG_SHARED_PROFILE_CELLS)
+                        _rl = true;
+#endif
                     }
                     float vs = b0, vd = b1v;
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
-                        if (meta[c] & 7) {
-                            float r0, r1;
-                            corner_sum(ds[c], corners(meta[c] & 7), x[c], r0, r1);
-                            vs += r0;
-                            vd += (meta[c] & 32) ? -r1 : r1;
-                            cb += corners(meta[c] & 7);
-                        }
+                        float r0, r1;
+                        corner_sum_g(d[c], 8, x[c], r0, r1);
+                        vs += r0;
+                        vd += (d[c].meta & 32) ? -r1 : r1;
                     }
+                    if (A.count) cb += corners(d[0].meta & 7) + corners(d[1].meta & 7) + corners(d[2].meta & 7);
                     if (s == 1) {
                         // every term's ν' factor carries the node's -tp
                         const float mt = -s_tp[q0 + k];
@@ -525,7 +590,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                         if (lane == __ffsll((long long)am) - 1) {
                             atomicAdd(&s_prof[2 * PR_N + 7], 1ull);
                             atomicAdd(&s_prof[2 * PR_N + 8], rb ? 1ull : 0ull);
-                            atomicAdd(&s_prof[2 * PR_N + 9], clock64() - _g0);
+                            atomicAdd(&s_prof[2 * PR_N + (rb ? 9 : 10)], clock64() - _g0);
                         }
                     }
 #endif
@@ -941,7 +1006,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             atomicAdd(A.count + 1, s_cnt[1]);
 #ifdef PFFRG_SHARED_PROFILE
             s_prof[prof_base + PR_TOTAL] = clock64() - t_begin;
-            for (int k = 0; k < 2 * PR_N + 10; ++k) atomicAdd(A.count + 2 + k, s_prof[k]);
+            for (int k = 0; k < 2 * PR_N + 11; ++k) atomicAdd(A.count + 2 + k, s_prof[k]);
 #endif
         }
     }
