@@ -245,6 +245,14 @@ const char *pffrg_batch_last_error(const pffrg_batch *batch);
 int pffrg_batch_rhs_1l(pffrg_batch *batch, const double *Lambda, const int32_t *a, const int32_t *da,
                        const int32_t *eval, const double *sigma_tol, const double *gamma_tol);
 
+/* replace_with! / mult_with_add_to! / get_abs_max (Action.jl:181-359) on the same slots of every
+ * problem of the batch, with fac[k] for problem k (caller's order) and out[k] its |q3| maximum:
+ * the BS3 stage algebra of launch_1l! (launcher_1l.jl:84-116) for all flows of a batch in one
+ * call each.  Each problem's result equals the per-context call's bitwise. */
+int pffrg_batch_replace_with(pffrg_batch *batch, int32_t dst, int32_t src, int32_t vertex_only);
+int pffrg_batch_mult_with_add_to(pffrg_batch *batch, int32_t src, const double *fac, int32_t dst, int32_t vertex_only);
+int pffrg_batch_abs_max(pffrg_batch *batch, int32_t slot, double *out);
+
 /* compute_χ! (correlation_su2.jl:58-86) of every problem k of the batch (caller's order): slot
  * a[k] at Lambda[k], χ frequencies chi_mesh[k][0 .. n_chi[k]-1], result chi[k][i + n_sites*w] as
  * pffrg_compute_chi; all problems' cubature tasks run through one set of rounds.  Each

@@ -103,7 +103,7 @@ struct SigmaArgs {
 // _corr2) (flow_su2_self.jl, flow_u1_dm_self.jl) and compute_Σ_kernel (SDE_su2.jl:45-77,
 // SDE_u1_dm.jl:45-91).
 template <int KIND>
-__global__ __launch_bounds__(64) void k_sigma(SigmaArgs Q) {
+__device__ __forceinline__ void sigma_point(const SigmaArgs &Q, int si) {
     __shared__ double s_mesh[5 * MAXN];
     __shared__ double s_sA[MAXN], s_sV[MAXN];
     __shared__ int s_exch[64];
@@ -125,7 +125,6 @@ __global__ __launch_bounds__(64) void k_sigma(SigmaArgs Q) {
     }
     __syncthreads();
     const Meshes<> M{s_mesh, s_mesh + MAXN, s_mesh + 2 * MAXN, s_mesh + 3 * MAXN, s_mesh + 4 * MAXN};
-    const int si = blockIdx.x + 1;
     const double w = s_mesh[si];
     const double L = Q.L;
     auto f = [&](double v) -> double {
@@ -231,6 +230,19 @@ __global__ __launch_bounds__(64) void k_sigma(SigmaArgs Q) {
             Q.out[si] = I;
         }
     }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(64) void k_sigma(SigmaArgs Q) {
+    sigma_point<KIND>(Q, blockIdx.x + 1);
+}
+
+// The σ points of several problems in one launch: block b takes point pts[b] & 0xffff of problem
+// pts[b] >> 16, whose arguments are tab[problem]; each point is computed exactly as k_sigma does.
+template <int KIND>
+__global__ __launch_bounds__(64) void k_sigma_batch(const SigmaArgs *__restrict__ tab, const uint32_t *__restrict__ pts) {
+    const uint32_t pt = pts[blockIdx.x];
+    sigma_point<KIND>(tab[pt >> 16], (int)(pt & 0xffffu));
 }
 
 // ======================================================================= elementwise action kernels
@@ -433,6 +445,10 @@ struct pffrg_batch {
     uint32_t *d_sh_units = nullptr;
     int n_sh_units = 0;
     void *d_sh_tab = nullptr;
+    // dΣ of all problems in one launch: the σ points (problem << 16 | point) and the arguments
+    uint32_t *d_sig_pts = nullptr;
+    int n_sig_pts = 0;
+    void *d_sig_tab = nullptr;
 };
 
 namespace {
@@ -769,9 +785,8 @@ int launch_corrs(pffrg_ctx *c, double L, int a, int da, bool kat, const double *
     return 0;
 }
 
-template <int KIND>
-int launch_sigma(pffrg_ctx *c, double L, int a, int v, double *out, const double *sigV, const double *add,
-                 const double *tol) {
+SigmaArgs sigma_args(const pffrg_ctx *c, double L, int a, int v, double *out, const double *sigV, const double *add,
+                     const double *tol) {
     SigmaArgs Q;
     Q.y = c->sys;
     Q.mesh = c->d_mesh;
@@ -785,6 +800,13 @@ int launch_sigma(pffrg_ctx *c, double L, int a, int v, double *out, const double
     Q.L = L;
     Q.atol = tol[0];
     Q.rtol = tol[1];
+    return Q;
+}
+
+template <int KIND>
+int launch_sigma(pffrg_ctx *c, double L, int a, int v, double *out, const double *sigV, const double *add,
+                 const double *tol) {
+    const SigmaArgs Q = sigma_args(c, L, a, v, out, sigV, add, tol);
     if (c->sys.nsig > 1) {
         hipLaunchKernelGGL((k_sigma<KIND>), dim3(c->sys.nsig - 1), dim3(64), 0, c->stream, Q);
         HIPCHK(c, hipGetLastError());
@@ -1557,10 +1579,47 @@ int batch_layout(pffrg_batch *b) {
     return 0;
 }
 
+// compute_dΣ! of every problem of a batch as one launch over all their σ points (each point as
+// k_sigma computes it), on the batch stream, after the contexts' streams have drained
+int batch_dsigma(pffrg_batch *b, const double *Lam, const int32_t *a, const int32_t *da, const double *stol) {
+    const int n = (int)b->ctx.size();
+    std::vector<SigmaArgs> tab(n);
+    for (int k = 0; k < n; ++k) {
+        pffrg_ctx *c = b->ctx[k];
+        const int q = b->order[k];
+        tab[k] = sigma_args(c, Lam[q], a[q], a[q], c->slots[da[q]].Sigma, nullptr, nullptr, stol);
+    }
+    if (!b->d_sig_pts) {
+        std::vector<uint32_t> pts;
+        for (int k = 0; k < n; ++k)
+            for (int si = 1; si < b->ctx[k]->sys.nsig; ++si) pts.push_back(((uint32_t)k << 16) | (uint32_t)si);
+        if (hipMalloc(&b->d_sig_pts, std::max<size_t>(1, pts.size()) * sizeof(uint32_t)) != hipSuccess ||
+            hipMalloc(&b->d_sig_tab, n * sizeof(SigmaArgs)) != hipSuccess)
+            return bfail(b, PFFRG_ERR_HIP, "hipMalloc failed");
+        if (!pts.empty() &&
+            hipMemcpy(b->d_sig_pts, pts.data(), pts.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+            return bfail(b, PFFRG_ERR_HIP, "σ point list upload failed");
+        b->n_sig_pts = (int)pts.size();
+    }
+    for (pffrg_ctx *c : b->ctx)
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "stream synchronize failed");
+    if (hipMemcpy(b->d_sig_tab, tab.data(), n * sizeof(SigmaArgs), hipMemcpyHostToDevice) != hipSuccess)
+        return bfail(b, PFFRG_ERR_HIP, "σ argument table upload failed");
+    if (b->n_sig_pts > 0) {
+        hipLaunchKernelGGL((k_sigma_batch<SK_DSIGMA>), dim3(b->n_sig_pts), dim3(64), 0, b->stream,
+                           reinterpret_cast<const SigmaArgs *>(b->d_sig_tab), b->d_sig_pts);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+        if (e != hipSuccess) return bfail(b, PFFRG_ERR_HIP, std::string("batched dΣ: ") + hipGetErrorString(e));
+    }
+    return 0;
+}
+
 template <typename TV>
 int batch_rhs_1l_t(pffrg_batch *b, const double *Lam, const int32_t *a, const int32_t *da, const int32_t *eval,
                    const double *stol, const double *gtol) {
     if (int rc = batch_layout<TV>(b)) return rc;
+    if (int rc = batch_dsigma(b, Lam, a, da, stol)) return rc;
     const int n = (int)b->ctx.size();
     const bool f32 = std::is_same<TV, float>::value;
     std::vector<PassArgs<TV>> tab(n);
@@ -1569,8 +1628,6 @@ int batch_rhs_1l_t(pffrg_batch *b, const double *Lam, const int32_t *a, const in
         pffrg_ctx *c = b->ctx[k];
         const int q = b->order[k];
         int rc;
-        if ((rc = launch_sigma<SK_DSIGMA>(c, Lam[q], a[q], a[q], c->slots[da[q]].Sigma, nullptr, nullptr, stol)))
-            return bfail(b, rc, c->err);
         if ((rc = launch_corrs(c, Lam[q], a[q], da[q], true, gtol))) return bfail(b, rc, c->err);
         if ((rc = ensure_gather(c, a[q], f32))) return bfail(b, rc, c->err);
         if ((rc = pass_args<P_KAT, TV>(c, Lam[q], a[q], da[q], da[q], eval[q], gtol, 0, tab[k])))
@@ -1619,6 +1676,7 @@ int batch_rhs_1l_t(pffrg_batch *b, const double *Lam, const int32_t *a, const in
 // grid mixes problems), then each problem's partial sums and symmetrisation on its own stream.
 int batch_rhs_1l_shared(pffrg_batch *b, const double *Lam, const int32_t *a, const int32_t *da, const int32_t *eval,
                         const double *stol, const double *gtol) {
+    if (int rc = batch_dsigma(b, Lam, a, da, stol)) return rc;
     const int n = (int)b->ctx.size();
     std::vector<SharedArgs> tab(n);
     size_t lds = 0;
@@ -1627,8 +1685,6 @@ int batch_rhs_1l_shared(pffrg_batch *b, const double *Lam, const int32_t *a, con
         pffrg_ctx *c = b->ctx[k];
         const int q = b->order[k];
         int rc;
-        if ((rc = launch_sigma<SK_DSIGMA>(c, Lam[q], a[q], a[q], c->slots[da[q]].Sigma, nullptr, nullptr, stol)))
-            return bfail(b, rc, c->err);
         if ((rc = launch_corrs(c, Lam[q], a[q], da[q], true, gtol))) return bfail(b, rc, c->err);
         if ((rc = ensure_gather(c, a[q], true))) return bfail(b, rc, c->err);
         if ((rc = shared_args<P_KAT>(c, Lam[q], a[q], da[q], da[q], eval[q], gtol, 0, tab[k]))) return bfail(b, rc, c->err);
@@ -1951,6 +2007,8 @@ void pffrg_batch_destroy(pffrg_batch *b) {
     if (b->d_tab) (void)hipFree(b->d_tab);
     if (b->d_sh_units) (void)hipFree(b->d_sh_units);
     if (b->d_sh_tab) (void)hipFree(b->d_sh_tab);
+    if (b->d_sig_pts) (void)hipFree(b->d_sig_pts);
+    if (b->d_sig_tab) (void)hipFree(b->d_sig_tab);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->chi_mem) (void)hipFree(b->chi_mem);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1979,6 +2037,63 @@ int pffrg_batch_rhs_1l(pffrg_batch *b, const double *Lambda, const int32_t *a, c
     if (prec == PFFRG_FP32_MIXED) return batch_rhs_1l_t<float>(b, Lambda, a, da, eval, sigma_tol, gamma_tol);
     return prec == PFFRG_FP32_ELEMENT ? batch_rhs_1l_t<float>(b, Lambda, a, da, eval, sigma_tol, gamma_tol)
                                       : batch_rhs_1l_t<double>(b, Lambda, a, da, eval, sigma_tol, gamma_tol);
+}
+
+// The BS3 stage algebra of every problem of a batch (Action.jl:181-359): each problem's kernels go
+// to its own stream and all streams are synchronised once at the end, where the per-context
+// calls synchronise after every problem.  Every slot is checked before anything is queued.
+int pffrg_batch_replace_with(pffrg_batch *b, int32_t dst, int32_t src, int32_t vertex_only) {
+    if (!b) return PFFRG_ERR_ARG;
+    for (pffrg_ctx *c : b->ctx)
+        if (check_slot(c, dst) || check_slot(c, src)) return bfail(b, PFFRG_ERR_ARG, "invalid slot");
+    if (dst == src) return 0;
+    if (hipSetDevice(b->device) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "hipSetDevice failed");
+    for (pffrg_ctx *c : b->ctx)
+        if (int rc = do_copy(c, dst, src, vertex_only != 0)) return bfail(b, rc, c->err);
+    for (pffrg_ctx *c : b->ctx)
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "stream synchronize failed");
+    return 0;
+}
+
+int pffrg_batch_mult_with_add_to(pffrg_batch *b, int32_t src, const double *fac, int32_t dst, int32_t vertex_only) {
+    if (!b) return PFFRG_ERR_ARG;
+    if (!fac) return bfail(b, PFFRG_ERR_ARG, "null factors");
+    if (src == dst) return bfail(b, PFFRG_ERR_ARG, "source and destination must differ");
+    for (pffrg_ctx *c : b->ctx)
+        if (check_slot(c, dst) || check_slot(c, src)) return bfail(b, PFFRG_ERR_ARG, "invalid slot");
+    if (hipSetDevice(b->device) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "hipSetDevice failed");
+    for (size_t k = 0; k < b->ctx.size(); ++k)
+        if (int rc = do_axpy(b->ctx[k], src, fac[b->order[k]], dst, vertex_only != 0)) return bfail(b, rc, b->ctx[k]->err);
+    for (pffrg_ctx *c : b->ctx)
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "stream synchronize failed");
+    return 0;
+}
+
+int pffrg_batch_abs_max(pffrg_batch *b, int32_t s, double *out) {
+    if (!b) return PFFRG_ERR_ARG;
+    if (!out) return bfail(b, PFFRG_ERR_ARG, "null output");
+    for (pffrg_ctx *c : b->ctx)
+        if (check_slot(c, s)) return bfail(b, PFFRG_ERR_ARG, "invalid slot");
+    if (hipSetDevice(b->device) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "hipSetDevice failed");
+    const size_t n = b->ctx.size();
+    std::vector<unsigned long long> bits(n, 0ull);
+    for (size_t k = 0; k < n; ++k) {
+        pffrg_ctx *c = b->ctx[k];
+        const long m = 3L * c->sys.nc * c->sys.ns * c->sys.nO * c->sys.nn * c->sys.nn;
+        if (hipMemsetAsync(c->d_max, 0, sizeof(unsigned long long), c->stream) != hipSuccess)
+            return bfail(b, PFFRG_ERR_HIP, "hipMemsetAsync failed");
+        hipLaunchKernelGGL(k_absmax_q3, dim3(grid_for(m)), dim3(256), 0, c->stream, c->slots[s].gamma, c->sys, c->d_max);
+        if (hipGetLastError() != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "k_absmax_q3 launch failed");
+    }
+    for (size_t k = 0; k < n; ++k) {
+        pffrg_ctx *c = b->ctx[k];
+        if (hipMemcpyAsync(&bits[k], c->d_max, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            return bfail(b, PFFRG_ERR_HIP, "read-back failed");
+    }
+    for (pffrg_ctx *c : b->ctx)
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "stream synchronize failed");
+    for (size_t k = 0; k < n; ++k) std::memcpy(&out[b->order[k]], &bits[k], sizeof(double));
+    return 0;
 }
 
 // compute_χ! of every problem of a batch in one set of cubature rounds

@@ -249,16 +249,42 @@ class BS3Flow:
         self.ev = M.get_eval(self.L, self.min_eval, self.max_eval)
         return True
 
-    def before(self, k: int):
-        ctx, L, dL = self.ctx, self.L, self.dL
+    # The stage algebra (launcher_1l.jl:84-116) as lists of slot operations, ("copy", dst, src)
+    # for replace_with! and ("axpy", src, fac, dst) for mult_with_add_to!: every flow has the same
+    # operations on the same slots at a stage, only the factors differ, so that a batch of flows
+    # can run each operation for all of them in one call.
+    def before_ops(self, k: int) -> list:
+        dL = self.dL
         if k == 0:
-            ctx.replace_with(DA, A)
-            ctx.replace_with(A_ERR, A)
-        elif k in (1, 2):
-            ctx.replace_with(A_INTER, A)
-            ctx.mult_with_add_to(A_STAGE, (-0.5 if k == 1 else -0.75) * dL, A_INTER)
-        else:
-            ctx.replace_with(A_INTER, DA)
+            return [("copy", DA, A), ("copy", A_ERR, A)]
+        if k in (1, 2):
+            return [("copy", A_INTER, A), ("axpy", A_STAGE, (-0.5 if k == 1 else -0.75) * dL, A_INTER)]
+        return [("copy", A_INTER, DA)]
+
+    def after_ops(self, k: int) -> list:
+        dL = self.dL
+        if k == 0:
+            return [("axpy", A_STAGE, -2.0 * dL / 9.0, DA), ("axpy", A_STAGE, -7.0 * dL / 24.0, A_ERR)]
+        if k == 1:
+            return [("axpy", A_STAGE, -1.0 * dL / 3.0, DA), ("axpy", A_STAGE, -1.0 * dL / 4.0, A_ERR)]
+        if k == 2:
+            return [("axpy", A_STAGE, -4.0 * dL / 9.0, DA), ("axpy", A_STAGE, -1.0 * dL / 3.0, A_ERR)]
+        return [("axpy", A_STAGE, -1.0 * dL / 8.0, A_ERR)]
+
+    @staticmethod
+    def finish_ops() -> list:
+        # the embedded error a_err -= a_inter (launcher_1l.jl:113-116)
+        return [("axpy", A_INTER, -1.0, A_ERR)]
+
+    def run_ops(self, ops):
+        for op in ops:
+            if op[0] == "copy":
+                self.ctx.replace_with(op[1], op[2])
+            else:
+                self.ctx.mult_with_add_to(op[1], op[2], op[3])
+
+    def before(self, k: int):
+        self.run_ops(self.before_ops(k))
 
     def stage_input(self, k: int):
         """(Λ, input slot, eval) of stage k; its right-hand side goes to slot A_STAGE."""
@@ -266,18 +292,7 @@ class BS3Flow:
         return ((L, A), (L - 0.5 * dL, A_INTER), (L - 0.75 * dL, A_INTER), (L - dL, A_INTER))[k] + (self.ev,)
 
     def after(self, k: int):
-        ctx, dL = self.ctx, self.dL
-        if k == 0:
-            ctx.mult_with_add_to(A_STAGE, -2.0 * dL / 9.0, DA)
-            ctx.mult_with_add_to(A_STAGE, -7.0 * dL / 24.0, A_ERR)
-        elif k == 1:
-            ctx.mult_with_add_to(A_STAGE, -1.0 * dL / 3.0, DA)
-            ctx.mult_with_add_to(A_STAGE, -1.0 * dL / 4.0, A_ERR)
-        elif k == 2:
-            ctx.mult_with_add_to(A_STAGE, -4.0 * dL / 9.0, DA)
-            ctx.mult_with_add_to(A_STAGE, -1.0 * dL / 3.0, A_ERR)
-        else:
-            ctx.mult_with_add_to(A_STAGE, -1.0 * dL / 8.0, A_ERR)
+        self.run_ops(self.after_ops(k))
 
     def rhs(self, k: int):
         L, a, ev = self.stage_input(k)
@@ -291,17 +306,19 @@ class BS3Flow:
         else:
             self.finish_step_b(None)
 
-    def finish_step_a(self) -> bool:
+    def finish_step_a(self, maxima=None) -> bool:
         """First half of finish_step: the embedded error, acceptance and the next step size.
-        True when an accepted step now needs χ (measure_chi, or a batched compute_chi)."""
+        True when an accepted step now needs χ (measure_chi, or a batched compute_chi).  A
+        batch passes maxima = (|A_ERR|, |A_INTER|, |A|) after running finish_ops() itself."""
         ctx, res = self.ctx, self.res
         res.rhs_evals += 4
         self._accepted = False
         # embedded error (launcher_1l.jl:113-116)
-        ctx.mult_with_add_to(A_INTER, -1.0, A_ERR)
-        delta = ctx.abs_max(A_ERR)
-        gmax = ctx.abs_max(A_INTER)
-        scale = self.ODE_tol[0] + max(gmax, ctx.abs_max(A)) * self.ODE_tol[1]
+        if maxima is None:
+            self.run_ops(self.finish_ops())
+            maxima = (ctx.abs_max(A_ERR), ctx.abs_max(A_INTER), ctx.abs_max(A))
+        delta, gmax, amax = maxima
+        scale = self.ODE_tol[0] + max(gmax, amax) * self.ODE_tol[1]
         err = delta / scale
         if self.verbose:
             print(f"L={self.L:.6g} dL={self.dL:.3g} eval={self.ev} err={err:.3g} Gmax={gmax:.4g} "
@@ -422,6 +439,19 @@ def launch_flow(ctx: Context, prob: Problem, a0: Action, Li: float, Lf: float, l
     return f.result()
 
 
+def _batch_ops(batch, per_flow):
+    """Run the same list of slot operations for every flow of a batch, one batched call per
+    operation, with each flow's own factor."""
+    for col in zip(*per_flow):
+        op = col[0]
+        if any(c[0] != op[0] or c[1] != op[1] or c[-1] != op[-1] for c in col):
+            raise ValueError("the flows of a batch must run the same slot operations")
+        if op[0] == "copy":
+            batch.replace_with(op[1], op[2])
+        else:
+            batch.mult_with_add_to(op[1], [c[2] for c in col], op[3])
+
+
 def launch_flow_batch(ctxs, probs, a0s, Li, Lf, max_steps: int | None = None, progress: int = 0,
                       **kw) -> list[FlowResult]:
     """1-loop BS3 flows of several problems advanced together: every stage evaluates the
@@ -450,15 +480,14 @@ def launch_flow_batch(ctxs, probs, a0s, Li, Lf, max_steps: int | None = None, pr
                     batch.close()
                 batch, members = Batch([ctxs[k] for k in live]), live
             for k in range(4):
-                ins = []
-                for q in live:
-                    flows[q].before(k)
-                    ins.append(flows[q].stage_input(k))
+                _batch_ops(batch, [flows[q].before_ops(k) for q in live])
+                ins = [flows[q].stage_input(k) for q in live]
                 batch.rhs_1l([x[0] for x in ins], [x[1] for x in ins], A_STAGE, [x[2] for x in ins],
                              flows[live[0]].Stol, flows[live[0]].Gtol)
-                for q in live:
-                    flows[q].after(k)
-            need = [flows[q].finish_step_a() for q in live]
+                _batch_ops(batch, [flows[q].after_ops(k) for q in live])
+            _batch_ops(batch, [flows[q].finish_ops() for q in live])
+            mx = [batch.abs_max(s) for s in (A_ERR, A_INTER, A)]
+            need = [flows[q].finish_step_a((mx[0][i], mx[1][i], mx[2][i])) for i, q in enumerate(live)]
             chis = [None] * len(live)
             if any(need):
                 t = time.time()

@@ -102,6 +102,9 @@ _SIGS = {
                            C.c_int),
     "pffrg_batch_compute_chi": ([_P, _D, C.POINTER(C.c_int32), C.POINTER(_D), C.POINTER(C.c_int32), _D,
                                  C.POINTER(_D)], C.c_int),
+    "pffrg_batch_replace_with": ([_P, _i, _i, _i], C.c_int),
+    "pffrg_batch_mult_with_add_to": ([_P, _i, _D, _i, _i], C.c_int),
+    "pffrg_batch_abs_max": ([_P, _i, _D], C.c_int),
 }
 
 
@@ -415,6 +418,27 @@ class Batch:
         if rc != 0:
             msg = self.lib.pffrg_batch_last_error(self.h)
             raise PffrgError(f"batch_rhs_1l failed ({rc}): {msg.decode() if msg else ''}")
+
+    def _bck(self, rc: int, what: str):
+        if rc != 0:
+            msg = self.lib.pffrg_batch_last_error(self.h)
+            raise PffrgError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def replace_with(self, dst, src, vertex_only=False):
+        """replace_with! of every problem: slot dst <- slot src."""
+        self._bck(self.lib.pffrg_batch_replace_with(self.h, dst, src, int(vertex_only)), "batch_replace_with")
+
+    def mult_with_add_to(self, src, facs, dst, vertex_only=False):
+        """mult_with_add_to! of every problem: slot dst += facs[k] * slot src."""
+        f = np.ascontiguousarray(np.broadcast_to(np.asarray(facs, dtype=np.float64), (len(self.ctxs),)))
+        self._bck(self.lib.pffrg_batch_mult_with_add_to(self.h, src, _dp(f), dst, int(vertex_only)),
+                  "batch_mult_with_add_to")
+
+    def abs_max(self, slot) -> np.ndarray:
+        """get_abs_max of every problem's slot."""
+        out = np.zeros(len(self.ctxs))
+        self._bck(self.lib.pffrg_batch_abs_max(self.h, slot, _dp(out)), "batch_abs_max")
+        return out
 
     def compute_chi(self, Lambdas, a, tol=(1e-8, 1e-4), chi_meshes=None, active=None):
         """compute_χ! of every problem (slot a[k] at Lambdas[k], the context's χ mesh by default)

@@ -147,3 +147,41 @@ def test_batch_workload_size_equals_single_and_oracle_sample():
         finally:
             for c in ctxs:
                 c.close()
+
+
+def test_batch_stage_algebra_equals_per_context():
+    # pffrg_batch_replace_with / _mult_with_add_to / _abs_max (the BS3 stage algebra of all flows
+    # of a batch, Action.jl:181-359) against the per-context calls: bitwise, with each problem's
+    # own factor; a bad slot is refused before any problem is touched
+    probs = _problems()
+    ctxs = [p.context(n_slots=4) for p, _, _, _ in probs]
+    ref = [p.context(n_slots=4) for p, _, _, _ in probs]
+    try:
+        for k, (p, a, _, _) in enumerate(probs):
+            b2 = A.synthetic_action(p.r.n_sites, p.mesh, p.bare, seed=90 + k)
+            for c in (ctxs[k], ref[k]):
+                c.upload(0, a)
+                c.upload(1, b2)
+        facs = [0.37, -1.25, 2.5e-3, -0.8]
+        for c, f in zip(ref, facs):
+            c.replace_with(2, 0)
+            c.mult_with_add_to(1, f, 2)
+            c.mult_with_add_to(0, -f, 1, vertex_only=True)
+        want_max = [c.abs_max(2) for c in ref]
+        with Batch(ctxs) as b:
+            b.replace_with(2, 0)
+            b.mult_with_add_to(1, facs, 2)
+            b.mult_with_add_to(0, [-f for f in facs], 1, vertex_only=True)
+            got_max = b.abs_max(2)
+            with pytest.raises(Exception):
+                b.mult_with_add_to(1, facs, 9)
+        for k in range(len(probs)):
+            for s in (1, 2):
+                g, w = ctxs[k].download(s), ref[k].download(s)
+                np.testing.assert_array_equal(g.gamma, w.gamma)
+                np.testing.assert_array_equal(g.Sigma, w.Sigma)
+        np.testing.assert_array_equal(got_max, np.array(want_max))
+        print(f"batched stage algebra = per-context, abs max {got_max}", flush=True)
+    finally:
+        for c in ctxs + ref:
+            c.close()
