@@ -99,6 +99,23 @@ int pffrg_compute_dsigma_corr(pffrg_ctx *ctx, double Lambda, int32_t a, int32_t 
 int pffrg_compute_dgamma_1l(pffrg_ctx *ctx, double Lambda, int32_t a, int32_t da, int32_t eval,
                             const double *gamma_tol);
 
+/* Launch shape of the FP32-mixed shared-node vertex pass (no reference counterpart: the
+ * reference has no kernels): shape[0] waves per block (sites split over them), shape[1] the
+ * batch capacity in nodes of 32 site vectors (4 or 8), shape[2] the most quadrature nodes per
+ * batch (16..64).  Every shape gives results within FP32 rounding of the others; a context keeps
+ * one shape for all its shared-node passes.  pffrg_set_shared_shape refuses a shape the problem
+ * cannot take (at most 3 sites per wave, 160 KB of LDS).
+ *
+ * pffrg_tune_shared is the autotuner: it times this problem's own Katanin pass (compute_dΣ! and
+ * compute_dΓ_1l! of slot a into slot da at Lambda; slot da is overwritten, and the runs count in
+ * the statistics) for each candidate of a fixed list of shapes, keeps the fastest for the
+ * context and returns it in best[3]; ms[k] is candidate k's time (-1 where infeasible), for at
+ * most *n_cand entries, and *n_cand is set to the list's length. */
+int pffrg_set_shared_shape(pffrg_ctx *ctx, int32_t waves, int32_t batch_nodes, int32_t max_nodes);
+int pffrg_get_shared_shape(pffrg_ctx *ctx, int32_t *shape);
+int pffrg_tune_shared(pffrg_ctx *ctx, double Lambda, int32_t a, int32_t da, int32_t eval, const double *gamma_tol,
+                      int32_t *best, double *ms, int32_t *n_cand);
+
 /* compute_dΓ_2l!  (Flow/compute.jl:115-174). */
 int pffrg_compute_dgamma_2l(pffrg_ctx *ctx, double Lambda, int32_t a, int32_t da, int32_t da_l, int32_t eval,
                             const double *gamma_tol);

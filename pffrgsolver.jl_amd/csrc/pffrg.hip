@@ -394,7 +394,8 @@ struct pffrg_ctx {
     // shared-node pass (FP32-mixed): work units per element-list mode, tile shape, partial sums
     uint32_t *d_units[3] = {nullptr, nullptr, nullptr};
     int n_units[3] = {0, 0, 0};
-    int sh_nw = 1, sh_spt = 1;
+    int sh_nw = 1, sh_spt = 1, sh_kb = 8, sh_kmax = SH_KMAX;
+    bool sh_shaped = false;  // the shape above is set (defaults or the autotuner's choice)
     double *d_part = nullptr;
     size_t part_bytes = 0;
     int precision = PFFRG_FP64;
@@ -656,15 +657,33 @@ int launch_pass_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, const
 // per-element pass.
 // the shared-node pass serves the SU(2) action up to 8 SH_SPT sites; U(1)-DM runs the
 // per-element pass in FP32 in that mode
+// the shape (waves per block, batch capacity, most nodes per batch) is feasible for the problem:
+// at most SH_SPT sites per wave and the pass's LDS (tables, overlap rows, node states, the batch)
+// within the 160 KB of a CU
+bool shared_shape_ok(const pffrg_ctx *c, int nw, int kb, int kmax) {
+    // kb >= 4: a batch of the initial Simpson rule pads its nodes to a group of four
+    if (nw < 1 || nw > 8 || kb < 4 || kb > 8 || kmax < kb || kmax > SH_KMAX) return false;
+    const int spt = (c->sys.ns + nw - 1) / nw;
+    if (spt > SH_SPT) return false;
+    return shared_lds(c->sys, c->n_ov, nw, true, spt, kb).total <= 160 * 1024;
+}
+
 bool shared_fits(const pffrg_ctx *c) {
     if (c->sys.nc != 2 || c->sys.ns > 8 * SH_SPT) return false;
-    // the pass's LDS (tables, overlap rows, node states) within the 160 KB of a CU
-    return shared_lds(c->sys, c->n_ov, 8, true, (c->sys.ns + 7) / 8).total <= 160 * 1024;
+    return shared_shape_ok(c, 8, shared_batch_nodes((c->sys.ns + 7) / 8), SH_KMAX);
+}
+
+void shared_set_shape(pffrg_ctx *c, int nw, int kb, int kmax) {
+    c->sh_nw = nw;
+    c->sh_spt = (c->sys.ns + nw - 1) / nw;
+    c->sh_kb = kb;
+    c->sh_kmax = kmax;
+    c->sh_shaped = true;
 }
 
 void shared_shape(pffrg_ctx *c) {
-    c->sh_nw = 8;
-    c->sh_spt = (c->sys.ns + 7) / 8;
+    if (!c->sh_shaped || !shared_shape_ok(c, c->sh_nw, c->sh_kb, c->sh_kmax))
+        shared_set_shape(c, 8, shared_batch_nodes((c->sys.ns + 7) / 8), SH_KMAX);
 }
 
 // Work units of the shared-node pass for element-list mode m: every (channel, Ω, 16 x 16 tile)
@@ -698,8 +717,8 @@ template <int PASS>
 int shared_args(pffrg_ctx *c, double L, int a, int d, int out, int eval, const double *tol, int mode, SharedArgs &A) {
     int rc;
     if ((rc = pass_args<PASS, float>(c, L, a, d, out, eval, tol, mode, A.P))) return rc;
+    shared_shape(c);
     if (!c->d_units[mode]) {
-        shared_shape(c);
         std::vector<uint32_t> u = shared_units(c, mode);
         HIPCHK(c, hipMalloc(&c->d_units[mode], std::max<size_t>(1, u.size()) * sizeof(uint32_t)));
         if (!u.empty())
@@ -714,6 +733,8 @@ int shared_args(pffrg_ctx *c, double L, int a, int d, int out, int eval, const d
     A.NX3 = nn + 1;
     A.nw = c->sh_nw;
     A.spt = c->sh_spt;
+    A.kb = c->sh_kb;
+    A.kmax = c->sh_kmax;
     A.count = c->d_count;
     const size_t need = (size_t)SH_NPART * 3 * c->sys.nO * (size_t)A.NX2 * A.NX3 * 2 * c->sys.ns * sizeof(double);
     if (need > c->part_bytes) {
@@ -730,7 +751,7 @@ int shared_args(pffrg_ctx *c, double L, int a, int d, int out, int eval, const d
 
 template <int PASS>
 size_t shared_lds_bytes(const pffrg_ctx *c, const SharedArgs &A) {
-    return (size_t)shared_lds(c->sys, c->n_ov, A.nw, PASS == P_KAT, A.spt).total;
+    return (size_t)shared_lds(c->sys, c->n_ov, A.nw, PASS == P_KAT, A.spt, A.kb).total;
 }
 
 template <int PASS>
@@ -1212,6 +1233,78 @@ int pffrg_compute_dgamma_1l(pffrg_ctx *c, double L, int32_t a, int32_t da, int32
     HIPCHK(c, hipSetDevice(c->device));
     if ((rc = dgamma_1l(c, L, a, da, eval, tol))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pffrg_set_shared_shape(pffrg_ctx *c, int32_t nw, int32_t kb, int32_t kmax) {
+    int rc;
+    if ((rc = ready(c))) return rc;
+    if (c->precision != PFFRG_FP32_MIXED || !shared_fits(c))
+        return fail(c, PFFRG_ERR_STATE, "the shared-node pass does not serve this context");
+    if (!shared_shape_ok(c, nw, kb, kmax)) return fail(c, PFFRG_ERR_ARG, "infeasible shape for this problem");
+    shared_set_shape(c, nw, kb, kmax);
+    return 0;
+}
+
+int pffrg_get_shared_shape(pffrg_ctx *c, int32_t *shape) {
+    if (!c || !shape) return PFFRG_ERR_ARG;
+    shared_shape(c);
+    shape[0] = c->sh_nw;
+    shape[1] = c->sh_kb;
+    shape[2] = c->sh_kmax;
+    return 0;
+}
+
+// The autotuner: every feasible (waves per block, batch capacity, most nodes per batch) from a
+// fixed candidate list, timed on this problem's own Katanin pass (compute_dΣ! once, then
+// compute_dΓ_1l! of slot a into slot da at Lambda, one untimed and two timed runs each, the
+// faster counted); the fastest is kept for every later shared-node pass of the context.
+int pffrg_tune_shared(pffrg_ctx *c, double L, int32_t a, int32_t da, int32_t eval, const double *tol, int32_t *best,
+                      double *ms, int32_t *n_cand) {
+    int rc;
+    if ((rc = ready(c)) || (rc = check_slot(c, a)) || (rc = check_slot(c, da))) return rc;
+    if (!tol || !best || !n_cand || !(L > 0.0) || a == da || eval < 1) return fail(c, PFFRG_ERR_ARG, "invalid arguments");
+    if (c->precision != PFFRG_FP32_MIXED || !shared_fits(c))
+        return fail(c, PFFRG_ERR_STATE, "the shared-node pass does not serve this context");
+    HIPCHK(c, hipSetDevice(c->device));
+    static const int cand[][3] = {{8, 8, 64}, {8, 8, 32}, {8, 8, 16}, {8, 4, 64}, {8, 4, 32},
+                                  {4, 8, 64}, {4, 8, 32}, {4, 4, 64}, {4, 4, 32}, {4, 4, 16}};
+    const int nc = (int)(sizeof(cand) / sizeof(cand[0]));
+    const int cap = *n_cand;
+    if ((rc = launch_sigma<SK_DSIGMA>(c, L, a, a, c->slots[da].Sigma, nullptr, nullptr, tol))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const bool timing = c->timing;
+    c->timing = true;
+    int k_best = -1, n_run = 0;
+    double t_best = 0.0;
+    for (int k = 0; k < nc; ++k) {
+        double t = -1.0;
+        if (shared_shape_ok(c, cand[k][0], cand[k][1], cand[k][2])) {
+            shared_set_shape(c, cand[k][0], cand[k][1], cand[k][2]);
+            for (int r = 0; r < 3; ++r) {
+                const double t0 = c->time_ms;
+                if ((rc = dgamma_1l(c, L, a, da, eval, tol))) {
+                    c->timing = timing;
+                    return rc;
+                }
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                const double dt = c->time_ms - t0;
+                if (r > 0 && (t < 0.0 || dt < t)) t = dt;
+            }
+            if (k_best < 0 || t < t_best) {
+                k_best = k;
+                t_best = t;
+            }
+        }
+        if (ms && n_run < cap) ms[n_run] = t;
+        ++n_run;
+    }
+    c->timing = timing;
+    *n_cand = n_run;
+    shared_set_shape(c, cand[k_best][0], cand[k_best][1], cand[k_best][2]);
+    best[0] = cand[k_best][0];
+    best[1] = cand[k_best][1];
+    best[2] = cand[k_best][2];
     return 0;
 }
 

@@ -43,13 +43,14 @@ constexpr int SH_SPT = 3;     // sites per wave, at most (the pass takes up to 8
 constexpr int SH_T = 16;      // tile edge (the matrix cores' 16 x 16 result)
 constexpr int SH_NPART = 5;   // partial sums per element: the four domains, then the two tail nodes
 constexpr int SH_NODES = 256; // nodes whose positions and bubbles are prepared together
-constexpr int SH_KMAX = 64;   // nodes per batch, at most (levels with few active vectors)
+constexpr int SH_KMAX = 64;   // nodes per batch, at most (levels with few active vectors), by default
 
 typedef float sh_v4f __attribute__((ext_vector_type(4)));
 
-// quadrature nodes of a batch with all 32 row and column vectors active: 8, or 4 at three sites
-// per wave (whose per-lane sums take the LDS that more nodes would need); a batch holds this
-// many nodes times 32 site vectors, so levels with fewer active vectors take more nodes per batch
+// quadrature nodes of a batch with all 32 row and column vectors active, by default: 8, or 4 at
+// three sites per wave (whose per-lane sums take the LDS that more nodes would need); a batch
+// holds this many nodes (SharedArgs::kb) times 32 site vectors, so levels with fewer active
+// vectors take more nodes per batch, up to SharedArgs::kmax
 __host__ __device__ constexpr int shared_batch_nodes(int spt) { return spt >= 3 ? 4 : 8; }
 
 // Descriptor of one buffer lookup in this pass, with everything that does not depend on the site
@@ -134,6 +135,7 @@ struct SharedArgs {
     int mode;                // 0 half pass (q3 with ν >= ν'), 1 full pass, 2 reduced s bubble
     int NX2, NX3;            // extents of the extended ν and ν' indices (n_ν + 1; reduced pass: NX2 = 1)
     int nw, spt;             // waves per block and sites per wave
+    int kb, kmax;            // batch capacity in nodes of 32 vectors, most nodes per batch
     double *part;            // [SH_NPART][3][nO][NX2][NX3][2][ns] partial sums
     unsigned long long *count;  // optional: element-nodes, gathered bytes, phase cycles
 };
@@ -162,7 +164,7 @@ struct SharedLds {
     int mp, o_sig, o_mf, o_ov, o_ovoff, o_exch, o_hint, o_desc, o_V, o_Z, o_loc, o_red, o_nodes, o_b1, total;
 };
 
-__host__ __device__ inline SharedLds shared_lds(const Sys &y, int n_ov, int nw, bool sig_d, int spt) {
+__host__ __device__ inline SharedLds shared_lds(const Sys &y, int n_ov, int nw, bool sig_d, int spt, int kb) {
     SharedLds L;
     int mp = y.nsig;
     if (y.nO > mp) mp = y.nO;
@@ -177,7 +179,7 @@ __host__ __device__ inline SharedLds shared_lds(const Sys &y, int n_ov, int nw, 
     L.o_hint = o; o += ((HINT_WORDS + 1) & ~1) * 4;
     o = (o + 15) & ~15;
     const int base = o;
-    const int K = shared_batch_nodes(spt);
+    const int K = kb;
     L.o_desc = o; o += K * 4 * SH_T * 3 * (int)sizeof(DescG);
     o = (o + 15) & ~15;
     L.o_V = o; o += K * 2 * SH_T * shared_vrow(y.ns) * 8;
@@ -250,12 +252,13 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     const Sys &y = P.y;
     const int t = threadIdx.x;
     const int NT = blockDim.x;
+    const int NTp = 64 * A.nw;
     const int lane = t & 63;
     const int wv = t >> 6;  // this thread's wave
     const int nw = A.nw;
     const int n_ov = P.ov_off[y.ns];
     // the LDS layout and batch size are the problem's own (A.spt), whatever SPT a batched launch uses
-    const SharedLds Lay = shared_lds(y, n_ov, nw, PASS == P_KAT, A.spt);
+    const SharedLds Lay = shared_lds(y, n_ov, nw, PASS == P_KAT, A.spt, A.kb);
     const int mp = Lay.mp;
     double *s_mesh = reinterpret_cast<double *>(s_dyn);
     double *s_sigA = reinterpret_cast<double *>(s_dyn + Lay.o_sig);
@@ -275,7 +278,8 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     float *s_tp = reinterpret_cast<float *>(s_wq + SH_NODES);         // bubbles
     int *s_rw = reinterpret_cast<int *>(s_tp + SH_NODES);             // node sums kept for the next level
     const int VR = shared_vrow(y.ns);
-    // this lane's previous-level Simpson sums, output o at b1[o * NT] (consecutive lanes adjacent)
+    // this lane's previous-level Simpson sums, output o at b1[o * NTp] (consecutive lanes adjacent);
+    // NTp = the problem's own threads (a batched launch may give the block more, which hold no sites)
     float *b1 = reinterpret_cast<float *>(s_dyn + Lay.o_b1) + t;
 
     // ---- tables in LDS (as the per-element pass stages them)
@@ -369,7 +373,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             cntw[bw] += 1;
             load[bw] += br;
         }
-        for (int w = 0; w < nw; ++w) s_nsw[w] = cntw[w];
+        for (int w = 0; w < 8; ++w) s_nsw[w] = w < nw ? cntw[w] : 0;
     }
     const double INF = __builtin_huge_val();
     const double *Om = (PASS == P_REDUCED || ch != 1) ? S.Os : S.Ot;
@@ -417,7 +421,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
 
     // per-lane quadrature state: output o = (site slot s, component c, element r) at (s * 2 + c) * 4 + r
     constexpr int NO = SPT * 2 * 4;
-    const int Kcap = shared_batch_nodes(A.spt) * 32;  // site vectors of one batch (node x vector)
+    const int Kcap = A.kb * 32;  // site vectors of one batch (node x vector)
     double b2[NO];
     float Pp[NO];
 #pragma unroll
@@ -802,7 +806,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                 const int nas = __popcll(bal);
                 s_nact[0] = __popcll(bal & 0xffffffffull);
                 int kb = 1;
-                while (2 * kb <= SH_KMAX && 2 * kb * nas <= Kcap) kb *= 2;
+                while (2 * kb <= A.kmax && 2 * kb * nas <= Kcap) kb *= 2;
                 s_kb = kb;
             }
             if (t == 32) s_nact[1] = __popcll(bal >> 32);
@@ -823,7 +827,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
         for (int o = 0; o < NO; ++o) {
             const int s = o >> 3, r = o & 3;
             if (s < nsw) {
-                double R = (double)b1[o * NT] * (-1.0 / 15.0);
+                double R = (double)b1[o * NTp] * (-1.0 / 15.0);
                 R += 16.0 / 15.0 * b2[o];
                 q[r][0] += R * R;
                 q[r][1] += b2[o] * b2[o];
@@ -831,13 +835,15 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                 q[r][2] += dd * dd;
             }
         }
+        if (wv < nw) {  // the problem's own waves (a batched launch's block may have more)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int e = (r2 + r) * SH_T + e3;
-            double *d = s_red + ((long)wv * SH_T * SH_T + e) * 3;
-            d[0] = q[r][0];
-            d[1] = q[r][1];
-            d[2] = q[r][2];
+            for (int r = 0; r < 4; ++r) {
+                const int e = (r2 + r) * SH_T + e3;
+                double *d = s_red + ((long)wv * SH_T * SH_T + e) * 3;
+                d[0] = q[r][0];
+                d[1] = q[r][1];
+                d[2] = q[r][2];
+            }
         }
         __syncthreads();
         bool any = false;
@@ -863,7 +869,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
 #pragma unroll
                     for (int c = 0; c < 2; ++c) {
                         const int o = (s * 2 + c) * 4 + r;
-                        double R = (double)b1[o * NT] * (-1.0 / 15.0);
+                        double R = (double)b1[o * NTp] * (-1.0 / 15.0);
                         R += 16.0 / 15.0 * b2[o];
                         pp[c * y.ns + msite[s]] += (R - b2[o]) + b2[o];
                         // one read-modify-write at a time (once per element and sub-interval): keeps
@@ -878,7 +884,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
         // the next level: S1 <- S2, S2 <- S2 / 2 (quadrature.jl:74-76)
 #pragma unroll
         for (int o = 0; o < NO; ++o) {
-            if ((o >> 3) < nsw) b1[o * NT] = (float)b2[o];
+            if ((o >> 3) < nsw) b1[o * NTp] = (float)b2[o];
             b2[o] = 0.5 * b2[o];
         }
         const bool more = __syncthreads_or(any ? 1 : 0);
@@ -961,7 +967,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             // S1, then S2 starts from S1 / 2 (quadrature.jl:62-66)
 #pragma unroll
             for (int o = 0; o < NO; ++o) {
-                if ((o >> 3) < nsw) b1[o * NT] = (float)b2[o];
+                if ((o >> 3) < nsw) b1[o * NTp] = (float)b2[o];
                 b2[o] = 0.5 * b2[o];
             }
             nlev = 2;

@@ -102,6 +102,9 @@ _SIGS = {
                            C.c_int),
     "pffrg_batch_compute_chi": ([_P, _D, C.POINTER(C.c_int32), C.POINTER(_D), C.POINTER(C.c_int32), _D,
                                  C.POINTER(_D)], C.c_int),
+    "pffrg_set_shared_shape": ([_P, _i, _i, _i], C.c_int),
+    "pffrg_get_shared_shape": ([_P, C.POINTER(C.c_int32)], C.c_int),
+    "pffrg_tune_shared": ([_P, C.c_double, _i, _i, _i, _D, C.POINTER(C.c_int32), _D, C.POINTER(C.c_int32)], C.c_int),
     "pffrg_batch_replace_with": ([_P, _i, _i, _i], C.c_int),
     "pffrg_batch_mult_with_add_to": ([_P, _i, _D, _i, _i], C.c_int),
     "pffrg_batch_abs_max": ([_P, _i, _D], C.c_int),
@@ -252,6 +255,25 @@ class Context:
 
     def compute_dgamma_1l(self, L, a, da, eval_, tol=(1e-8, 1e-4)):
         self._ck(self.lib.pffrg_compute_dgamma_1l(self.h, L, a, da, eval_, _dp(_tol(tol))), "compute_dgamma_1l")
+
+    def shared_shape(self) -> tuple:
+        """(waves per block, batch capacity, most nodes per batch) of the shared-node pass."""
+        out = np.zeros(3, dtype=np.int32)
+        self._ck(self.lib.pffrg_get_shared_shape(self.h, out.ctypes.data_as(C.POINTER(C.c_int32))), "get_shared_shape")
+        return tuple(int(x) for x in out)
+
+    def set_shared_shape(self, waves, batch_nodes, max_nodes):
+        self._ck(self.lib.pffrg_set_shared_shape(self.h, waves, batch_nodes, max_nodes), "set_shared_shape")
+
+    def tune_shared(self, L, a, da, eval_, tol=(1e-8, 1e-4)):
+        """The autotuner: times every candidate shape on this problem's Katanin pass (slot a ->
+        slot da, overwritten) and keeps the fastest; returns (best shape, [ms per candidate])."""
+        best = np.zeros(3, dtype=np.int32)
+        ms = np.zeros(32)
+        n = C.c_int32(len(ms))
+        self._ck(self.lib.pffrg_tune_shared(self.h, L, a, da, eval_, _dp(_tol(tol)), best.ctypes.data_as(C.POINTER(C.c_int32)),
+                                            _dp(ms), C.byref(n)), "tune_shared")
+        return tuple(int(x) for x in best), [float(x) for x in ms[:n.value]]
 
     def compute_dgamma_2l(self, L, a, da, da_l, eval_, tol=(1e-8, 1e-4)):
         self._ck(self.lib.pffrg_compute_dgamma_2l(self.h, L, a, da, da_l, eval_, _dp(_tol(tol))), "compute_dgamma_2l")

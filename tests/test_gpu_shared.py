@@ -102,3 +102,85 @@ def test_shared_parquet_passes():
     print(f"   SDE Σ vs FP64 {es:.2e}; BSE worst {w[0]} {w[1]:.2e}", flush=True)
     assert es < TOL32
     assert w[1] < TOL32
+
+
+SHAPES = [(8, 8, 64), (8, 8, 16), (8, 4, 32), (4, 8, 64), (4, 4, 16)]
+
+
+def test_shared_shapes_agree():
+    # every launch shape of the shared-node pass (waves per block, batch capacity, most nodes per
+    # batch) computes the same integrals: within FP32 rounding of each other, within the mode's
+    # bound of FP64
+    prob = small_problem("square", 3, 8, 1.0)
+    ns, m = prob.r.n_sites, prob.mesh
+    a = A.synthetic_action(ns, m, prob.bare, seed=61)
+    ref = _rhs_modes(prob, a, 2, (FP64,))[FP64][0]
+    outs = {}
+    for shape in SHAPES:
+        with prob.context(n_slots=3, precision=FP32_MIXED) as ctx:
+            ctx.set_shared_shape(*shape)
+            assert ctx.shared_shape() == shape
+            ctx.upload(0, a)
+            ctx.upload(1, A.Action.empty(ns, m))
+            ctx.upload(2, A.Action.empty(ns, m))
+            F.rhs(ctx, 1.0, 2, 0, 1, 10, da_l=2)
+            outs[shape] = ctx.download(1)
+    for shape, g in outs.items():
+        e64 = max(per_array_err(g, ref).values())
+        e0 = max(per_array_err(g, outs[SHAPES[0]]).values())
+        print(f"shape {shape}: vs FP64 {e64:.2e}, vs default shape {e0:.2e}", flush=True)
+        assert e64 < TOL32
+        assert e0 < TOL_ELEM
+
+
+def test_shared_shape_refused_and_tuned():
+    prob = small_problem("square", 6, 6, 1.0)  # 16 sites: 4 waves would need 4 sites per wave
+    ns, m = prob.r.n_sites, prob.mesh
+    a = A.synthetic_action(ns, m, prob.bare, seed=62)
+    with prob.context(n_slots=3, precision=FP32_MIXED) as ctx:
+        with pytest.raises(Exception):
+            ctx.set_shared_shape(4, 8, 64)
+        ctx.upload(0, a)
+        ctx.upload(1, A.Action.empty(ns, m))
+        best, ms = ctx.tune_shared(1.0, 0, 1, 10)
+        print(f"tuned shape {best}, candidate ms {['%.2f' % x for x in ms]}", flush=True)
+        assert best[0] == 8 and ctx.shared_shape() == best
+        assert all(x < 0 for x in ms[5:]) and all(x > 0 for x in ms[:3])
+        F.rhs(ctx, 1.0, 1, 0, 1, 10)
+        tuned = ctx.download(1)
+    with prob.context(n_slots=3, precision=FP32_MIXED) as ctx:
+        ctx.set_shared_shape(*best)
+        ctx.upload(0, a)
+        ctx.upload(1, A.Action.empty(ns, m))
+        F.rhs(ctx, 1.0, 1, 0, 1, 10)
+        np.testing.assert_array_equal(ctx.download(1).gamma, tuned.gamma)
+
+
+def test_batch_of_mixed_shapes_equals_single():
+    # a batched launch runs every problem in its own shape (the block has the threads of the
+    # widest): each problem's result equals its own run bitwise
+    from pffrg.mesh import get_eval
+    from pffrg.native import Batch
+    specs = [("square", 3, 8, (4, 4, 16)), ("square", 5, 6, (8, 8, 64)), ("kagome", 3, 6, (4, 8, 32))]
+    probs, acts, ctxs, single = [], [], [], []
+    try:
+        for k, (name, size, N, shape) in enumerate(specs):
+            p = small_problem(name, size, N, 1.0)
+            a = A.synthetic_action(p.r.n_sites, p.mesh, p.bare, seed=70 + k)
+            c = p.context(n_slots=3, precision=FP32_MIXED)
+            c.set_shared_shape(*shape)
+            c.upload(0, a)
+            F.rhs(c, 0.8, 1, 0, 1, get_eval(0.8))
+            single.append(c.download(1))
+            probs.append(p)
+            ctxs.append(c)
+        with Batch(ctxs) as b:
+            b.rhs_1l([0.8] * len(ctxs), 0, 2, [get_eval(0.8)] * len(ctxs))
+        for k, c in enumerate(ctxs):
+            got = c.download(2)
+            np.testing.assert_array_equal(got.gamma, single[k].gamma)
+            np.testing.assert_array_equal(got.Sigma, single[k].Sigma)
+        print("batched mixed shapes = single runs", flush=True)
+    finally:
+        for c in ctxs:
+            c.close()
