@@ -41,13 +41,29 @@ def build_problem(L: int, N: int):
     return F.Problem.build("square", L, [[1.0]], initial=1.0, num_sigma=N, num_Omega=N, num_nu=N, num_chi=10)
 
 
-def time_mode(prob, a, mode: int, loops: int, steps: int, warmup: int, eval_: int):
+# the autotuner's candidate list (pffrg_tune_shared): (waves per block, batch capacity in nodes of
+# 32 site vectors, most nodes per batch)
+TUNE_CANDIDATES = [(8, 8, 64), (8, 8, 32), (8, 8, 16), (8, 4, 64), (8, 4, 32),
+                   (4, 8, 64), (4, 8, 32), (4, 4, 64), (4, 4, 32), (4, 4, 16)]
+
+
+def time_mode(prob, a, mode: int, loops: int, steps: int, warmup: int, eval_: int, tune: dict | None = None):
     ns, m = prob.r.n_sites, prob.mesh
     ctx = prob.context(n_slots=3, precision=mode)
     ctx.upload(0, a)
     zero = A.Action.empty(ns, m)
     ctx.upload(1, zero)
     ctx.upload(2, zero)
+    if tune is not None and tune.get("fixed"):
+        ctx.set_shared_shape(*tune["fixed"])
+        tune["shape"] = dict(zip(("waves", "batch_nodes", "max_nodes"), tune["fixed"]))
+    elif tune is not None:
+        # the shared-node pass's launch shape, picked by the autotuner on this problem's own
+        # Katanin pass (slot 1 is an output and is overwritten below)
+        shape, ms = ctx.tune_shared(1.0, 0, 1, eval_)
+        tune.update({"shape": {"waves": shape[0], "batch_nodes": shape[1], "max_nodes": shape[2]},
+                     "candidates_ms": {str(c): (t if t >= 0 else None) for c, t in zip(TUNE_CANDIDATES, ms)}})
+        log(f"  autotuner: shape {shape}, Katanin pass ms {['%.1f' % t for t in ms]}")
     for _ in range(warmup):
         F.rhs(ctx, 1.0, loops, 0, 1, eval_, da_l=2)
     ctx.synchronize()
@@ -93,12 +109,14 @@ def committed_traffic(name: str):
     return d.get("hbm_bytes_per_launch"), f"profiles/{name} ({d.get('source', '')}, same sources)"
 
 
-def per_pass_stats(prob, a, mode: int, eval_: int) -> dict:
+def per_pass_stats(prob, a, mode: int, eval_: int, shape=None) -> dict:
     """Nodes per element, gathered bytes per element-node and kernel ms of the Katanin (half) and
     left (full) passes of the 2-loop RHS: compute_dΓ_1l! alone, then compute_dΓ_2l!, whose
     difference is the left pass."""
     ns, m = prob.r.n_sites, prob.mesh
     ctx = prob.context(n_slots=4, precision=mode)
+    if shape is not None:
+        ctx.set_shared_shape(*shape)
     ctx.upload(0, a)
     zero = A.Action.empty(ns, m)
     for s in (1, 2, 3):
@@ -284,6 +302,9 @@ def main():
     ap.add_argument("--workload", choices=["single", "batch", "batchflow", "flow", "warmstart"], default="single",
                     help="single: the 2-loop square L=6 RHS (headline); batch: the J1-J2 sweep + uneven "
                          "truncations as one batched 1-loop RHS")
+    ap.add_argument("--shape", default="", metavar="W,K,M",
+                    help="single: the shared-node pass's launch shape (waves per block, batch capacity, most nodes "
+                         "per batch) instead of the autotuner's choice")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (3 times), the known-size reference "
                          "for scaling hardware byte counters")
@@ -319,7 +340,9 @@ def main():
         f"(setup {time.time() - t0:.1f}s)")
 
     log("FP32-mixed leg (shared-node pass)")
-    dt32, st32, kms32, kl32, out32 = time_mode(prob, a, FP32_MIXED, args.loops, args.steps, args.warmup, eval_)
+    tune = {"fixed": tuple(int(x) for x in args.shape.split(","))} if args.shape else {}
+    dt32, st32, kms32, kl32, out32 = time_mode(prob, a, FP32_MIXED, args.loops, args.steps, args.warmup, eval_,
+                                               tune=tune)
     log(f"  {dt32 / args.steps * 1e3:.1f} ms/RHS, element-nodes/RHS {st32['nodes'] / args.steps:.3e}")
     res = {}
     if not args.no_fp64:
@@ -343,7 +366,8 @@ def main():
                     "fp64_kernel": {"kernel": "k_vertex_pass<PASS, double>", "ms_per_launch": kms64 / max(1, kl64),
                                     "gathered_GBps": st64["gathered_bytes"] / (kms64 * 1e-3) / 1e9 if kms64 else None}})
         log("per-pass statistics")
-        res["per_pass"] = {"fp32_mixed_shared": per_pass_stats(prob, a, FP32_MIXED, eval_),
+        tshape = tuple(tune["shape"][k] for k in ("waves", "batch_nodes", "max_nodes")) if tune else None
+        res["per_pass"] = {"fp32_mixed_shared": per_pass_stats(prob, a, FP32_MIXED, eval_, tshape),
                            "fp64": per_pass_stats(prob, a, FP64, eval_)}
         ref_out = out64
     else:
@@ -382,6 +406,7 @@ def main():
                                f"N={args.N} (+1 points/axis), Lambda=1, eval={eval_}",
                    "lattice": "square", "L": args.L, "n_sites": ns, "N": args.N, "loops": args.loops},
         "element_nodes_per_rhs": st32["nodes"] / args.steps,
+        "autotune": tune,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
                      "kernel": "k_shared_pass", "bytes_per_launch": bytes_per_launch, "ms_per_launch": avg_ms,
