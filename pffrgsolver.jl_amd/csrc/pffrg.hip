@@ -18,6 +18,7 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <array>
 #include <vector>
 
 #include "pffrg.h"
@@ -446,6 +447,8 @@ struct pffrg_batch {
     uint32_t *d_sh_units = nullptr;
     int n_sh_units = 0;
     void *d_sh_tab = nullptr;
+    std::vector<int> sh_spts;                 // sites per wave of each problem when the list was made
+    std::vector<std::array<int, 3>> sh_groups;  // (sites per wave, first unit, units) per launch
     // dΣ of all problems in one launch: the σ points (problem << 16 | point) and the arguments
     uint32_t *d_sig_pts = nullptr;
     int n_sig_pts = 0;
@@ -1772,8 +1775,7 @@ int batch_rhs_1l_shared(pffrg_batch *b, const double *Lam, const int32_t *a, con
     if (int rc = batch_dsigma(b, Lam, a, da, stol)) return rc;
     const int n = (int)b->ctx.size();
     std::vector<SharedArgs> tab(n);
-    size_t lds = 0;
-    int nthreads = 64, spt = 1;
+    std::vector<int> spts(n);
     for (int k = 0; k < n; ++k) {
         pffrg_ctx *c = b->ctx[k];
         const int q = b->order[k];
@@ -1781,41 +1783,70 @@ int batch_rhs_1l_shared(pffrg_batch *b, const double *Lam, const int32_t *a, con
         if ((rc = launch_corrs(c, Lam[q], a[q], da[q], true, gtol))) return bfail(b, rc, c->err);
         if ((rc = ensure_gather(c, a[q], true))) return bfail(b, rc, c->err);
         if ((rc = shared_args<P_KAT>(c, Lam[q], a[q], da[q], da[q], eval[q], gtol, 0, tab[k]))) return bfail(b, rc, c->err);
-        nthreads = std::max(nthreads, shared_threads(tab[k]));
-        spt = std::max(spt, tab[k].spt);
+        spts[k] = tab[k].spt;
     }
-    // one launch shape for all: the LDS of the problem that needs most (each problem keeps its own
-    // layout and batch size inside it), per-lane state for the most sites per wave
-    for (int k = 0; k < n; ++k) lds = std::max(lds, shared_lds_bytes<P_KAT>(b->ctx[k], tab[k]));
-    if (!b->d_sh_units) {
-        // round-robin over the problems' unit lists, each in its own order
-        std::vector<std::vector<uint32_t>> per(n);
-        size_t total = 0;
-        for (int k = 0; k < n; ++k) {
-            per[k] = shared_units(b->ctx[k], 0);
-            total += per[k].size();
-        }
+    // The problems are grouped by sites per wave, which sizes the kernel's per-lane state: one
+    // launch per group (each launch holds every problem of its group, its units interleaved
+    // round-robin so that each part of the grid mixes problems), so that a problem with three
+    // sites per wave does not give every other problem its larger state.  Each problem keeps its
+    // own launch shape, LDS layout and batch size inside its group's launch.
+    if (!b->d_sh_units || b->sh_spts != spts) {
+        if (b->d_sh_units) (void)hipFree(b->d_sh_units);
+        b->d_sh_units = nullptr;
         std::vector<uint32_t> all;
-        all.reserve(total);
-        std::vector<size_t> pos(n, 0);
-        while (all.size() < total)
+        b->sh_groups.clear();
+        for (int g = 1; g <= SH_SPT; ++g) {
+            std::vector<int> mem;
             for (int k = 0; k < n; ++k)
-                if (pos[k] < per[k].size()) all.push_back(per[k][pos[k]++] | ((uint32_t)k << 26));
+                if (spts[k] == g) mem.push_back(k);
+            if (mem.empty()) continue;
+            std::vector<std::vector<uint32_t>> per;
+            size_t total = 0;
+            for (int k : mem) {
+                per.push_back(shared_units(b->ctx[k], 0));
+                total += per.back().size();
+            }
+            const size_t first = all.size();
+            std::vector<size_t> pos(mem.size(), 0);
+            while (all.size() - first < total)
+                for (size_t j = 0; j < mem.size(); ++j)
+                    if (pos[j] < per[j].size()) all.push_back(per[j][pos[j]++] | ((uint32_t)mem[j] << 26));
+            if (total) b->sh_groups.push_back({g, (int)first, (int)total});
+        }
         if (hipMalloc(&b->d_sh_units, std::max<size_t>(1, all.size()) * sizeof(uint32_t)) != hipSuccess ||
-            hipMalloc(&b->d_sh_tab, n * sizeof(SharedArgs)) != hipSuccess)
+            (!b->d_sh_tab && hipMalloc(&b->d_sh_tab, n * sizeof(SharedArgs)) != hipSuccess))
             return bfail(b, PFFRG_ERR_HIP, "hipMalloc failed");
         if (!all.empty() && hipMemcpy(b->d_sh_units, all.data(), all.size() * sizeof(uint32_t), hipMemcpyHostToDevice) !=
                                 hipSuccess)
             return bfail(b, PFFRG_ERR_HIP, "unit list upload failed");
         b->n_sh_units = (int)all.size();
+        b->sh_spts = spts;
     }
     for (pffrg_ctx *c : b->ctx)
         if (hipStreamSynchronize(c->stream) != hipSuccess) return bfail(b, PFFRG_ERR_HIP, "stream synchronize failed");
     if (hipMemcpy(b->d_sh_tab, tab.data(), n * sizeof(SharedArgs), hipMemcpyHostToDevice) != hipSuccess)
         return bfail(b, PFFRG_ERR_HIP, "argument table upload failed");
-    hipError_t e = launch_shared_pass_batch<P_KAT>(reinterpret_cast<const SharedArgs *>(b->d_sh_tab), b->d_sh_units,
-                                                   b->n_sh_units, nthreads, spt, lds, b->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    hipError_t e = hipSuccess;
+    std::vector<hipStream_t> used;
+    for (const auto &grp : b->sh_groups) {
+        // the group's launch: the threads and LDS of the member that needs most; the groups go to
+        // different streams (the batch's, then a member's own) so that they can overlap
+        size_t lds = 0;
+        int nthreads = 64, first = -1;
+        for (int k = 0; k < n; ++k)
+            if (spts[k] == grp[0]) {
+                nthreads = std::max(nthreads, shared_threads(tab[k]));
+                lds = std::max(lds, shared_lds_bytes<P_KAT>(b->ctx[k], tab[k]));
+                if (first < 0) first = k;
+            }
+        const hipStream_t st = used.empty() ? b->stream : b->ctx[first]->stream;
+        used.push_back(st);
+        e = launch_shared_pass_batch<P_KAT>(reinterpret_cast<const SharedArgs *>(b->d_sh_tab), b->d_sh_units + grp[1],
+                                            grp[2], nthreads, grp[0], lds, st);
+        if (e != hipSuccess) break;
+    }
+    for (hipStream_t st : used)
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return bfail(b, PFFRG_ERR_HIP, std::string("batched shared-node pass: ") + hipGetErrorString(e));
     for (int k = 0; k < n; ++k) {
         pffrg_ctx *c = b->ctx[k];
