@@ -165,7 +165,8 @@ int pffrg_get_stats(pffrg_ctx *ctx, int64_t *stats);
  * [24] active row and column vectors per node, in a build with PFFRG_SHARED_PROFILE defined
  * (zero otherwise); with PFFRG_SHARED_PROFILE_CELLS also defined, [25] wave steps of the gather
  * loop (one node of one site vector per lane), [26] those in which a lane loaded corners, [27] their
- * cycles, [28] the cycles of the others; with PFFRG_SHARED_PROFILE_PAIRS instead, [25] consecutive node pairs of one batch and
+ * cycles, [28] the cycles of the others, [29] contraction cycles of batches with at most 4 active
+ * rows and columns, [30] those batches; with PFFRG_SHARED_PROFILE_PAIRS instead, [25] consecutive node pairs of one batch and
  * vector, [26] those reading the same corners in all three channels, [27] channels reading the same
  * corners */
 int pffrg_get_counters(pffrg_ctx *ctx, int64_t *out, int32_t n);

@@ -246,7 +246,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     __shared__ int s_rowf[SH_T], s_colf[SH_T], s_rmap[SH_T], s_cmap[SH_T], s_kb;
     __shared__ int s_site[8 * SH_SPT], s_nsw[8];
     __shared__ unsigned long long s_cnt[4];
-    __shared__ unsigned long long s_prof[2 * PR_N + 11];
+    __shared__ unsigned long long s_prof[2 * PR_N + 13];
 
     const PassArgs<float> &P = A.P;
     const Sys &y = P.y;
@@ -322,7 +322,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     }
     if (t < 4) s_cnt[t] = 0;
     for (int k = t; k < VR; k += NT) s_Z[k] = float2{0.f, 0.f};
-    if (t < 2 * PR_N + 11) s_prof[t] = 0;
+    if (t < 2 * PR_N + 13) s_prof[t] = 0;
 #ifdef PFFRG_SHARED_PROFILE
     const unsigned long long t_begin = clock64();
 #endif
@@ -643,12 +643,106 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             const float2 *Rb = vr >= 0 ? s_V + vr * VR : s_Z;
             const float2 *Cb = vc >= 0 ? s_V + (nA2 + vc) * VR : s_Z;
             const int Rs = vr >= 0 ? nAS * VR : 0, Cs = vc >= 0 ? nAS * VR : 0;
+            // the t channel with at most 8 active rows and 8 active columns runs on 4 x 4 blocks
+            // (v_mfma_f32_4x4x1f32, 16 blocks per product): block b = lane / 4 holds node
+            // kb4 = b / nsub4 of the products' node group and 4 x 4 sub-tile st4 = b mod nsub4 of the
+            // compact active rows and columns; lane 4b + e supplies row u4 = 4 p + e to A and column
+            // v4 = 4 q + e to B, and receives D[b][j][e] (row 4 p + j, column 4 q + e) in register j
+            const bool sparse4 = ch == 1 && nA2 <= 8 && nA3 <= 8;
+            const int nq4 = (nA3 + 3) >> 2, nsub4 = ((nA2 + 3) >> 2) * nq4, npi4 = 16 / (nsub4 > 0 ? nsub4 : 1);
+            const int kb4 = (lane >> 2) / (nsub4 > 0 ? nsub4 : 1), st4 = (lane >> 2) - kb4 * nsub4;
+            const int u4 = 4 * (st4 / (nq4 > 0 ? nq4 : 1)) + (lane & 3), v4 = 4 * (st4 % (nq4 > 0 ? nq4 : 1)) + (lane & 3);
+            const bool ua4 = u4 < nA2, va4 = v4 < nA3;
 #pragma unroll
             for (int s = 0; s < SPT; ++s) {
                 if (s >= nsw) break;
                 const int i = msite[s];
                 sh_v4f Ds = {0.f, 0.f, 0.f, 0.f}, Dd = {0.f, 0.f, 0.f, 0.f};
-                if (ch != 1) {
+                if (sparse4) {
+                    sh_v4f Es = {0.f, 0.f, 0.f, 0.f}, Ed = {0.f, 0.f, 0.f, 0.f};
+                    const int r0 = __builtin_amdgcn_readfirstlane(s_ovoff[i]);
+                    const int r1 = __builtin_amdgcn_readfirstlane(s_ovoff[i + 1]);
+                    for (int k0 = 0; k0 < nb; k0 += npi4) {
+                        const int k = k0 + kb4;
+                        const bool kon = k < nb;
+                        const float2 *R0 = (kon && ua4) ? s_V + (k * nAS + u4) * VR : s_Z;
+                        const float2 *C0 = (kon && va4) ? s_V + (k * nAS + nA2 + v4) * VR : s_Z;
+                        {
+                            // the two local terms (flow_su2_t.jl:56-58)
+                            const float2 v1i = R0[i], v2i = C0[i];
+                            const float2 l3 = (kon && ua4) ? s_loc[k * nAS + u4] : float2{0.f, 0.f};
+                            const float2 l4 = (kon && va4) ? s_loc[k * nAS + nA2 + v4] : float2{0.f, 0.f};
+                            Es = __builtin_amdgcn_mfma_f32_4x4x1f32(v1i.x, -1.0f * l4.x + l4.y, Es, 0, 0, 0);
+                            Ed = __builtin_amdgcn_mfma_f32_4x4x1f32(v1i.y, 3.0f * l4.x + l4.y, Ed, 0, 0, 0);
+                            Es = __builtin_amdgcn_mfma_f32_4x4x1f32(-1.0f * l3.x + l3.y, v2i.x, Es, 0, 0, 0);
+                            Ed = __builtin_amdgcn_mfma_f32_4x4x1f32(3.0f * l3.x + l3.y, v2i.y, Ed, 0, 0, 0);
+                        }
+                        // the overlap rows in j1 runs, as in the 16 x 16 form
+                        float bx = 0.f, by = 0.f;
+                        for (int cb = r0; cb < r1; cb += 64) {
+                            const int nr = (r1 - cb) < 64 ? r1 - cb : 64;
+                            const uint2 rl = (lane < nr) ? s_ov[cb + lane] : uint2{0u, 0u};
+                            const int rj = (int)rl.x, rm = (int)rl.y;
+                            auto step = [&](uint32_t jj, float mm, float2 f2) {
+                                bx += mm * f2.x;
+                                by += mm * f2.y;
+                                if (jj >> 31) {
+                                    const float2 e1 = R0[jj & 255];
+                                    Es = __builtin_amdgcn_mfma_f32_4x4x1f32(e1.x, bx, Es, 0, 0, 0);
+                                    Ed = __builtin_amdgcn_mfma_f32_4x4x1f32(e1.y, by, Ed, 0, 0, 0);
+                                    bx = 0.f;
+                                    by = 0.f;
+                                }
+                            };
+                            int r = 0;
+                            for (; r + 4 <= nr; r += 4) {
+                                uint32_t jj[4];
+                                float mm[4];
+                                float2 f2[4];
+#pragma unroll
+                                for (int u = 0; u < 4; ++u) {
+                                    jj[u] = (uint32_t)__builtin_amdgcn_readlane(rj, r + u);
+                                    mm[u] = __int_as_float(__builtin_amdgcn_readlane(rm, r + u));
+                                    f2[u] = C0[(jj[u] >> 8) & 255];
+                                }
+#pragma unroll
+                                for (int u = 0; u < 4; ++u) step(jj[u], mm[u], f2[u]);
+                            }
+                            for (; r < nr; ++r) {
+                                const uint32_t jj = (uint32_t)__builtin_amdgcn_readlane(rj, r);
+                                const float mm = __int_as_float(__builtin_amdgcn_readlane(rm, r));
+                                step(jj, mm, C0[(jj >> 8) & 255]);
+                            }
+                        }
+                    }
+                    // the blocks of one sub-tile (lanes 4 nsub4 apart) summed over their nodes
+                    for (int off = 4 * nsub4; off < 64; off <<= 1) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            Es[r] += __shfl_xor(Es[r], off);
+                            Ed[r] += __shfl_xor(Ed[r], off);
+                        }
+                    }
+                    // each owner lane (tile row 4 (lane / 16) + r, column lane mod 16) reads its element
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int ur = s_rmap[4 * (lane >> 4) + r], vcol = s_cmap[lane & 15];
+                        const bool on = ur >= 0 && vcol >= 0;
+                        const int src = on ? 4 * ((ur >> 2) * nq4 + (vcol >> 2)) + (vcol & 3) : 0;
+                        const int jr = ur & 3;
+                        float gs = 0.f, gd = 0.f;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const float ts = __shfl(Es[j], src), td = __shfl(Ed[j], src);
+                            if (j == jr) {
+                                gs = ts;
+                                gd = td;
+                            }
+                        }
+                        Ds[r] = on ? gs : 0.f;
+                        Dd[r] = on ? gd : 0.f;
+                    }
+                } else if (ch != 1) {
                     // s and u (flow_su2_s.jl:41-42, flow_su2_u.jl:41-42): Γs = v1s (-tp (∓2 v2s + v2d))
                     // + v1d (-tp v2s), Γd = v1s (-3 tp v2s) + v1d (-tp v2d), with -tp already in
                     // v2; two nodes per product
@@ -773,6 +867,13 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             }
         }
         __syncthreads();  // the descriptors and site vectors are rewritten by the next batch
+#if defined(PFFRG_SHARED_PROFILE) && defined(PFFRG_SHARED_PROFILE_CELLS)
+        // contraction cycles and batches with at most 4 active rows and 4 active columns
+        if (t == 0 && A.count && nA2 <= 4 && nA3 <= 4) {
+            s_prof[2 * PR_N + 11] += clock64() - _tb;
+            s_prof[2 * PR_N + 12] += 1;
+        }
+#endif
         SH_PROF_LAP(PR_CONTRACT);
 #undef SH_PROF_LAP
     };
@@ -1012,7 +1113,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             atomicAdd(A.count + 1, s_cnt[1]);
 #ifdef PFFRG_SHARED_PROFILE
             s_prof[prof_base + PR_TOTAL] = clock64() - t_begin;
-            for (int k = 0; k < 2 * PR_N + 11; ++k) atomicAdd(A.count + 2 + k, s_prof[k]);
+            for (int k = 0; k < 2 * PR_N + 13; ++k) atomicAdd(A.count + 2 + k, s_prof[k]);
 #endif
         }
     }
