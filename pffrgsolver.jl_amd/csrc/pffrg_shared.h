@@ -485,6 +485,9 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
         {
             const int per_k = 3 * npair * nAS;
             const int ntask = nb * per_k;
+            // the gathered-bytes counter: each descriptor of pair 0 is read on every site, of pair
+            // 1 (the t channel's local vertices) on site 0
+            unsigned long long cbd = 0;
             for (int tau = t; tau < ntask; tau += NT) {
                 const int k = tau / per_k, r = tau - k * per_k;
                 const int c = r % 3, q = r / 3;
@@ -492,9 +495,11 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                 const bool row = v < nA2;
                 const int s = 2 * pr + (row ? 0 : 1);
                 const int ee = row ? s_act2[v] : s_act3[v - nA2];
-                desc_shared(S, y, s_code[3 * s + c], c, W, row ? s_a1[ee] : 0.0, row ? 0.0 : s_a2[ee], s_xs[q0 + k],
-                            s_desc[((k * npair + pr) * nAS + v) * 3 + c]);
+                DescG &dg = s_desc[((k * npair + pr) * nAS + v) * 3 + c];
+                desc_shared(S, y, s_code[3 * s + c], c, W, row ? s_a1[ee] : 0.0, row ? 0.0 : s_a2[ee], s_xs[q0 + k], dg);
+                if (A.count) cbd += (unsigned long long)corners(dg.meta & 7) * (pr == 0 ? y.ns : 1);
             }
+            if (A.count && cbd) atomicAdd(&s_cnt[1], cbd * 2ull * sizeof(float));
         }
         __syncthreads();
 #if defined(PFFRG_SHARED_PROFILE) && defined(PFFRG_SHARED_PROFILE_PAIRS)
@@ -521,7 +526,6 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
         // corner offsets and flags in all three channels), and then the corners loaded for the
         // previous node are reused with the new node's weights.
         {
-            unsigned long long cb = 0;
             const int ntask = nAS * y.ns;
             const bool rowD = (vertex_cfg<PASS>(ch, 0) & 8) != 0, colD = (vertex_cfg<PASS>(ch, 1) & 8) != 0;
             const float2 *qA = reinterpret_cast<const float2 *>(P.A.gamma);
@@ -579,7 +583,6 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                         vs += r0;
                         vd += (d[c].meta & 32) ? -r1 : r1;
                     }
-                    if (A.count) cb += corners(d[0].meta & 7) + corners(d[1].meta & 7) + corners(d[2].meta & 7);
                     if (s == 1) {
                         // every term's ν' factor carries the node's -tp
                         const float mt = -s_tp[q0 + k];
@@ -608,7 +611,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                     const int s = v < nA2 ? 2 : 3;
                     const ActV<float> &V = (s == 2 ? locD2 : locD3) ? P.D : P.A;
                     float vs = (float)V.bare[0], vd = (float)V.bare[y.ns];
-                    cb += gather_site(s_desc + ((k * 2 + 1) * nAS + v) * 3, V.gamma, s_exch, 0, y, vs, vd);
+                    gather_site(s_desc + ((k * 2 + 1) * nAS + v) * 3, V.gamma, s_exch, 0, y, vs, vd);
                     if (s == 3) {
                         const float mt = -s_tp[q0 + k];
                         vs *= mt;
@@ -628,7 +631,6 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                     for (int tau = t; tau < nz; tau += NT) s_loc[nb * nAS + tau] = float2{0.f, 0.f};
                 }
             }
-            if (A.count && cb) atomicAdd(&s_cnt[1], cb * 2ull * sizeof(float));
         }
         __syncthreads();
         SH_PROF_LAP(PR_GATHER);
