@@ -196,31 +196,25 @@ __host__ __device__ inline SharedLds shared_lds(const Sys &y, int n_ov, int nw, 
 }
 
 // Vertex of one slot at one site from its three channel descriptors (get_Γ_avx!,
-// action_su2.jl:95-134): bare + Σ_channels sign·interpolation, one channel at a time to bound
-// registers; the corners of a channel are loaded together.  Returns the corners read.
-__device__ __forceinline__ int gather_site(const DescG *ds, const float *gamma, const int *exch, int site,
-                                           const Sys &y, float &vs, float &vd) {
+// action_su2.jl:95-134): bare + Σ_channels sign·interpolation.
+__device__ __forceinline__ void gather_site(const DescG *ds, const float *gamma, const int *exch, int site,
+                                            float &vs, float &vd) {
     const float2 *q = reinterpret_cast<const float2 *>(gamma);
-    int cn = 0;
+    DescG d[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) d[c] = ds[c];
+    // all 24 corners in flight together (eight per channel, zero weights beyond a buffer's own)
+    float2 x[3][8];
+    const int sx = exch[site];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) load_corners_g(q, d[c], desc_site_offset(d[c], site, sx), x[c]);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const DescG &d = ds[c];
-        const int meta = d.meta;
-        const int kern = meta & 7;
-        if (!kern) continue;
-        const uint32_t o = desc_site_offset(d, site, exch[site]);
-        uint32_t c1, c2, c3;
-        corner_steps(meta & 0xffff, y, c1, c2, c3);
-        const int n = corners(kern);
-        float2 x[8];
-        load_corners(q, o, c1, c2, c3, n, x);
         float r0, r1;
-        corner_sum_g(d, n, x, r0, r1);
+        corner_sum_g(d[c], 8, x[c], r0, r1);
         vs += r0;
-        vd += (meta & 32) ? -r1 : r1;
-        cn += n;
+        vd += (d[c].meta & 32) ? -r1 : r1;
     }
-    return cn;
 }
 
 // Phase timing of the shared-node pass (a measurement build with PFFRG_SHARED_PROFILE defined):
@@ -611,7 +605,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                     const int s = v < nA2 ? 2 : 3;
                     const ActV<float> &V = (s == 2 ? locD2 : locD3) ? P.D : P.A;
                     float vs = (float)V.bare[0], vd = (float)V.bare[y.ns];
-                    gather_site(s_desc + ((k * 2 + 1) * nAS + v) * 3, V.gamma, s_exch, 0, y, vs, vd);
+                    gather_site(s_desc + ((k * 2 + 1) * nAS + v) * 3, V.gamma, s_exch, 0, vs, vd);
                     if (s == 3) {
                         const float mt = -s_tp[q0 + k];
                         vs *= mt;
