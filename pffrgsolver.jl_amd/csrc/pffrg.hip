@@ -668,7 +668,7 @@ bool shared_shape_ok(const pffrg_ctx *c, int nw, int kb, int kmax) {
     if (nw < 1 || nw > 8 || kb < 4 || kb > 8 || kmax < kb || kmax > SH_KMAX) return false;
     const int spt = (c->sys.ns + nw - 1) / nw;
     if (spt > SH_SPT) return false;
-    return shared_lds(c->sys, c->n_ov, nw, true, spt, kb).total <= 160 * 1024;
+    return shared_lds(c->sys, c->n_ov, nw, true, spt, kb).total + SH_STATIC_LDS <= 160 * 1024;
 }
 
 bool shared_fits(const pffrg_ctx *c) {

@@ -44,6 +44,9 @@ constexpr int SH_T = 16;      // tile edge (the matrix cores' 16 x 16 result)
 constexpr int SH_NPART = 5;   // partial sums per element: the four domains, then the two tail nodes
 constexpr int SH_NODES = 256; // nodes whose positions and bubbles are prepared together
 constexpr int SH_KMAX = 64;   // nodes per batch, at most (levels with few active vectors), by default
+// static LDS of k_shared_pass beside the dynamic block (small tables and flags), with some slack:
+// the launch needs dynamic + static <= 160 KB
+constexpr int SH_STATIC_LDS = 2048;
 
 typedef float sh_v4f __attribute__((ext_vector_type(4)));
 
@@ -239,6 +242,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     __shared__ int s_act2[SH_T], s_act3[SH_T], s_nact[2];
     __shared__ int s_rowf[SH_T], s_colf[SH_T], s_rmap[SH_T], s_cmap[SH_T], s_kb;
     __shared__ int s_site[8 * SH_SPT], s_nsw[8];
+    __shared__ uint8_t s_conv[SH_T * SH_T];
     __shared__ unsigned long long s_cnt[4];
     __shared__ unsigned long long s_prof[2 * PR_N + 13];
 
@@ -943,20 +947,25 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             }
         }
         __syncthreads();
-        bool any = false;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int e = (r2 + r) * SH_T + e3;
+        // each element's test once: thread e adds the waves' partial sums in wave order
+        if (t < SH_T * SH_T) {
             double q1 = 0.0, q2 = 0.0, q3 = 0.0;
             for (int w = 0; w < nw; ++w) {
-                const double *d = s_red + ((long)w * SH_T * SH_T + e) * 3;
+                const double *d = s_red + ((long)w * SH_T * SH_T + t) * 3;
                 q1 += d[0];
                 q2 += d[1];
                 q3 += d[2];
             }
             const double n1 = sqrt(q1), n2 = sqrt(q2), adiff = sqrt(q3);
             const double rdiff = adiff / fmax(n1, n2);
-            const bool conv = act[r] && (adiff < P.atol || rdiff < P.rtol || nn_ > (1L << 20));
+            s_conv[t] = (adiff < P.atol || rdiff < P.rtol || nn_ > (1L << 20)) ? 1 : 0;
+        }
+        __syncthreads();
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int e = (r2 + r) * SH_T + e3;
+            const bool conv = act[r] && s_conv[e] != 0;
             if (conv) {
                 // Richardson-extrapolated result (quadrature.jl:72-88), added in sub-interval order
                 double *pp = part_at(dom, r);
