@@ -395,6 +395,12 @@ struct pffrg_ctx {
     // shared-node pass (FP32-mixed): work units per element-list mode, tile shape, partial sums
     uint32_t *d_units[3] = {nullptr, nullptr, nullptr};
     int n_units[3] = {0, 0, 0};
+    // dispatch order of the units per (pass, element-list mode): the list the last launch ran and
+    // each unit's run time there (a prior before the first launch); the next launch of that pass
+    // and mode dispatches its units longest first, from the other list
+    uint32_t *d_ord[9][2] = {};
+    uint32_t *d_cost[9] = {};
+    int ord_cur[9] = {};
     int sh_nw = 1, sh_spt = 1, sh_kb = 8, sh_kmax = SH_KMAX;
     bool sh_shaped = false;  // the shape above is set (defaults or the autotuner's choice)
     double *d_part = nullptr;
@@ -446,6 +452,8 @@ struct pffrg_batch {
     // shared-node pass (FP32-mixed): the work units of all problems, problem index in bits 26-31
     uint32_t *d_sh_units = nullptr;
     int n_sh_units = 0;
+    uint32_t *d_sh_units2 = nullptr;  // the other unit list: launches alternate, dispatching longest first
+    uint32_t *d_sh_cost = nullptr;    // each unit's run time at its place in the list last run
     void *d_sh_tab = nullptr;
     std::vector<int> sh_spts;                 // sites per wave of each problem when the list was made
     std::vector<std::array<int, 3>> sh_groups;  // (sites per wave, first unit, units) per launch
@@ -714,6 +722,49 @@ std::vector<uint32_t> shared_units(const pffrg_ctx *c, int m) {
     return u;
 }
 
+// Run-time prior of a unit before its first measured launch (clock ticks): t-channel tiles carry
+// the overlap-row contraction and the two inner loop-frequency domains hold most nodes.
+uint32_t shared_unit_prior(uint32_t u) {
+    const int ch = u & 3, dom = (u >> 17) & 3;
+    return 100000u * (ch == 1 ? 3u : 1u) * ((dom == 1 || dom == 2) ? 2u : 1u);
+}
+
+// Order key of a unit's run time: 1/8 octave of clock ticks, the longest bucket first.
+__device__ __forceinline__ int unit_bucket(uint32_t ticks) {
+    const int b = (int)(8.0f * log2f((float)ticks + 1.0f));
+    return 255 - (b < 0 ? 0 : (b > 255 ? 255 : b));
+}
+
+// out = the units of in, longest run time first (a counting sort by unit_bucket; the order
+// inside a bucket is the order the atomics serve, which no result depends on: every partial sum
+// has one unit that owns it).  One block.
+__global__ __launch_bounds__(1024) void k_order_units(const uint32_t *__restrict__ in,
+                                                      const uint32_t *__restrict__ cost, uint32_t *__restrict__ out,
+                                                      int n) {
+    __shared__ int s_off[256];
+    const int t = threadIdx.x;
+    if (t < 256) s_off[t] = 0;
+    __syncthreads();
+    for (int i = t; i < n; i += blockDim.x) atomicAdd(&s_off[unit_bucket(cost[i])], 1);
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int b = 0; b < 256; ++b) {
+            const int h = s_off[b];
+            s_off[b] = acc;
+            acc += h;
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < n; i += blockDim.x) out[atomicAdd(&s_off[unit_bucket(cost[i])], 1)] = in[i];
+}
+
+hipError_t launch_order_units(const uint32_t *in, const uint32_t *cost, uint32_t *out, int n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_order_units, dim3(1), dim3(1024), 0, stream, in, cost, out, n);
+    return hipGetLastError();
+}
+
 // Arguments of the shared-node pass of one context: the pass's tables, the work units of the
 // element-list mode (built on first use) and the partial-sum buffer (grown on demand).
 template <int PASS>
@@ -739,6 +790,7 @@ int shared_args(pffrg_ctx *c, double L, int a, int d, int out, int eval, const d
     A.kb = c->sh_kb;
     A.kmax = c->sh_kmax;
     A.count = c->d_count;
+    A.cost = nullptr;
     const size_t need = (size_t)SH_NPART * 3 * c->sys.nO * (size_t)A.NX2 * A.NX3 * 2 * c->sys.ns * sizeof(double);
     if (need > c->part_bytes) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -763,6 +815,27 @@ int launch_shared_t(pffrg_ctx *c, double L, int a, int d, int out, int eval, con
     int rc;
     if ((rc = shared_args<PASS>(c, L, a, d, out, eval, tol, mode, A))) return rc;
     const size_t lds = shared_lds_bytes<PASS>(c, A);
+    // dispatch order: the units of the last launch of this pass and mode, longest first
+    const int slot = (PASS == P_KAT ? 0 : PASS == P_LEFT ? 1 : 2) * 3 + mode;
+    if (!c->d_cost[slot] && A.n_units > 0) {
+        const std::vector<uint32_t> u = shared_units(c, mode);
+        std::vector<uint32_t> prior(u.size());
+        for (size_t k = 0; k < u.size(); ++k) prior[k] = shared_unit_prior(u[k]);
+        const size_t bytes = u.size() * sizeof(uint32_t);
+        HIPCHK(c, hipMalloc(&c->d_ord[slot][0], bytes));
+        HIPCHK(c, hipMalloc(&c->d_ord[slot][1], bytes));
+        HIPCHK(c, hipMalloc(&c->d_cost[slot], bytes));
+        HIPCHK(c, hipMemcpy(c->d_ord[slot][0], u.data(), bytes, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_cost[slot], prior.data(), bytes, hipMemcpyHostToDevice));
+        c->ord_cur[slot] = 0;
+    }
+    if (c->d_cost[slot]) {
+        const int cur = c->ord_cur[slot];
+        HIPCHK(c, launch_order_units(c->d_ord[slot][cur], c->d_cost[slot], c->d_ord[slot][1 - cur], A.n_units, c->stream));
+        c->ord_cur[slot] = 1 - cur;
+        A.units = c->d_ord[slot][1 - cur];
+        A.cost = c->d_cost[slot];
+    }
     // the timing covers k_shared_pass alone (the pass's dominant kernel), not the finish
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, launch_shared_pass<PASS>(A, lds, c->stream));
@@ -1010,10 +1083,10 @@ int pffrg_create_symmetry(const pffrg_dims *d, int32_t symmetry, pffrg_ctx **out
     ok = ok && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipMalloc(&c->d_mesh, MESH_TABLE * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_corrs, 2 * 3 * y.nO * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&c->d_count, N_COUNTERS * sizeof(unsigned long long)) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_count, N_COUNTERS_ALLOC * sizeof(unsigned long long)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_max, sizeof(unsigned long long)) == hipSuccess;
     ok = ok && hipMemset(c->d_corrs, 0, 2 * 3 * y.nO * sizeof(double)) == hipSuccess;
-    ok = ok && hipMemset(c->d_count, 0, N_COUNTERS * sizeof(unsigned long long)) == hipSuccess;
+    ok = ok && hipMemset(c->d_count, 0, N_COUNTERS_ALLOC * sizeof(unsigned long long)) == hipSuccess;
     ok = ok && hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
     c->slots.resize(d->n_slots);
     for (auto &s : c->slots) {
@@ -1066,6 +1139,11 @@ void pffrg_destroy(pffrg_ctx *c) {
     (void)hipFree(c->d_el_full);
     (void)hipFree(c->d_el_red);
     for (int m = 0; m < 3; ++m) (void)hipFree(c->d_units[m]);
+    for (int k = 0; k < 9; ++k) {
+        (void)hipFree(c->d_ord[k][0]);
+        (void)hipFree(c->d_ord[k][1]);
+        (void)hipFree(c->d_cost[k]);
+    }
     (void)hipFree(c->d_part);
     (void)hipFree(c->d_logpts);
     (void)hipFree(c->d_dbg_nodes);
@@ -1476,7 +1554,7 @@ int pffrg_reset_stats(pffrg_ctx *c) {
     if (!c) return PFFRG_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemset(c->d_count, 0, N_COUNTERS * sizeof(unsigned long long)));
+    HIPCHK(c, hipMemset(c->d_count, 0, N_COUNTERS_ALLOC * sizeof(unsigned long long)));
     c->stat_elems = 0;
     c->stat_launches = 0;
     c->time_ms = 0.0;
@@ -1506,6 +1584,17 @@ int pffrg_get_counters(pffrg_ctx *c, int64_t *out, int32_t n) {
     for (int k = 0; k < n; ++k) out[k] = (int64_t)v[k];
     return 0;
 }
+
+#ifdef PFFRG_SHARED_PROFILE_UNITS
+// measurement build only: the counters followed by the record count and the unit records
+int pffrg_get_counters_ext(pffrg_ctx *c, int64_t *out, int32_t n) {
+    if (!c || !out || n < 1 || n > N_COUNTERS_ALLOC) return PFFRG_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, c->d_count, (size_t)n * sizeof(int64_t), hipMemcpyDeviceToHost));
+    return 0;
+}
+#endif
 
 int pffrg_get_corrs(pffrg_ctx *c, double *out) {
     if (!c || !out) return PFFRG_ERR_ARG;
@@ -1792,7 +1881,11 @@ int batch_rhs_1l_shared(pffrg_batch *b, const double *Lam, const int32_t *a, con
     // own launch shape, LDS layout and batch size inside its group's launch.
     if (!b->d_sh_units || b->sh_spts != spts) {
         if (b->d_sh_units) (void)hipFree(b->d_sh_units);
+        if (b->d_sh_units2) (void)hipFree(b->d_sh_units2);
+        if (b->d_sh_cost) (void)hipFree(b->d_sh_cost);
         b->d_sh_units = nullptr;
+        b->d_sh_units2 = nullptr;
+        b->d_sh_cost = nullptr;
         std::vector<uint32_t> all;
         b->sh_groups.clear();
         for (int g = 1; g <= SH_SPT; ++g) {
@@ -1813,11 +1906,16 @@ int batch_rhs_1l_shared(pffrg_batch *b, const double *Lam, const int32_t *a, con
                     if (pos[j] < per[j].size()) all.push_back(per[j][pos[j]++] | ((uint32_t)mem[j] << 26));
             if (total) b->sh_groups.push_back({g, (int)first, (int)total});
         }
-        if (hipMalloc(&b->d_sh_units, std::max<size_t>(1, all.size()) * sizeof(uint32_t)) != hipSuccess ||
+        const size_t ubytes = std::max<size_t>(1, all.size()) * sizeof(uint32_t);
+        if (hipMalloc(&b->d_sh_units, ubytes) != hipSuccess || hipMalloc(&b->d_sh_units2, ubytes) != hipSuccess ||
+            hipMalloc(&b->d_sh_cost, ubytes) != hipSuccess ||
             (!b->d_sh_tab && hipMalloc(&b->d_sh_tab, n * sizeof(SharedArgs)) != hipSuccess))
             return bfail(b, PFFRG_ERR_HIP, "hipMalloc failed");
-        if (!all.empty() && hipMemcpy(b->d_sh_units, all.data(), all.size() * sizeof(uint32_t), hipMemcpyHostToDevice) !=
-                                hipSuccess)
+        std::vector<uint32_t> prior(all.size());
+        for (size_t k = 0; k < all.size(); ++k) prior[k] = shared_unit_prior(all[k]);
+        if (!all.empty() &&
+            (hipMemcpy(b->d_sh_units, all.data(), all.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(b->d_sh_cost, prior.data(), all.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess))
             return bfail(b, PFFRG_ERR_HIP, "unit list upload failed");
         b->n_sh_units = (int)all.size();
         b->sh_spts = spts;
@@ -1841,10 +1939,15 @@ int batch_rhs_1l_shared(pffrg_batch *b, const double *Lam, const int32_t *a, con
             }
         const hipStream_t st = used.empty() ? b->stream : b->ctx[first]->stream;
         used.push_back(st);
-        e = launch_shared_pass_batch<P_KAT>(reinterpret_cast<const SharedArgs *>(b->d_sh_tab), b->d_sh_units + grp[1],
-                                            grp[2], nthreads, grp[0], lds, st);
+        // the group's units of the last launch, longest first, into the other list
+        e = launch_order_units(b->d_sh_units + grp[1], b->d_sh_cost + grp[1], b->d_sh_units2 + grp[1], grp[2], st);
+        if (e != hipSuccess) break;
+        e = launch_shared_pass_batch<P_KAT>(reinterpret_cast<const SharedArgs *>(b->d_sh_tab), b->d_sh_units2 + grp[1],
+                                            b->d_sh_cost + grp[1], grp[2], nthreads, grp[0], lds, st);
         if (e != hipSuccess) break;
     }
+    // the next launch orders the lists just run (every group was reordered when no launch failed)
+    if (e == hipSuccess) std::swap(b->d_sh_units, b->d_sh_units2);
     for (hipStream_t st : used)
         if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return bfail(b, PFFRG_ERR_HIP, std::string("batched shared-node pass: ") + hipGetErrorString(e));
@@ -2130,6 +2233,8 @@ void pffrg_batch_destroy(pffrg_batch *b) {
     if (b->d_codes) (void)hipFree(b->d_codes);
     if (b->d_tab) (void)hipFree(b->d_tab);
     if (b->d_sh_units) (void)hipFree(b->d_sh_units);
+    if (b->d_sh_units2) (void)hipFree(b->d_sh_units2);
+    if (b->d_sh_cost) (void)hipFree(b->d_sh_cost);
     if (b->d_sh_tab) (void)hipFree(b->d_sh_tab);
     if (b->d_sig_pts) (void)hipFree(b->d_sig_pts);
     if (b->d_sig_tab) (void)hipFree(b->d_sig_tab);

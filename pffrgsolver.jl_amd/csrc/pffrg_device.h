@@ -15,6 +15,12 @@ namespace pffrg {
 constexpr int MAXN = 64;      // mesh points per axis held in LDS
 constexpr int WAVE = 64;      // gfx950 wavefront
 constexpr int N_COUNTERS = 32;  // device measurement counters of a context (nodes, bytes, phase cycles)
+#ifdef PFFRG_SHARED_PROFILE_UNITS
+// measurement build: the counters, a record count and four words per shared-pass unit run
+constexpr int N_COUNTERS_ALLOC = N_COUNTERS + 1 + 4 * 16384;
+#else
+constexpr int N_COUNTERS_ALLOC = N_COUNTERS;
+#endif
 
 // Bracket hints of the five meshes (σ, Ωs, νs, Ωt, νt), stored after the meshes and their
 // inverse spacings in the device mesh table.  A positive double's bits shifted right by 49 are

@@ -141,6 +141,7 @@ struct SharedArgs {
     int kb, kmax;            // batch capacity in nodes of 32 vectors, most nodes per batch
     double *part;            // [SH_NPART][3][nO][NX2][NX3][2][ns] partial sums
     unsigned long long *count;  // optional: element-nodes, gathered bytes, phase cycles
+    uint32_t *cost;          // optional: each unit's run time (clock ticks) at its place in units
 };
 
 // work unit: ch 2 bits | Ω index 7 | tile along ν 4 | tile along ν' 4 | domain 2
@@ -235,7 +236,8 @@ enum { PR_ACTIVE, PR_PREP, PR_DESC, PR_GATHER, PR_CONTRACT, PR_CHECK, PR_BATCHES
 // One work unit (code uc) of one problem.  Every thread of the block calls it.  SPT: the largest
 // number of sites per wave the launch serves (the per-lane quadrature state is sized by it).
 template <int PASS, int SPT>
-__device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t uc) {
+__device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t uc, uint32_t *cost_slot) {
+    const unsigned long long rt_unit = wall_clock64();
     extern __shared__ __align__(16) unsigned char s_dyn[];
     __shared__ uint32_t s_code[MAX_DESC];
     __shared__ double s_a1[SH_T], s_a2[SH_T];
@@ -322,7 +324,7 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
     for (int k = t; k < VR; k += NT) s_Z[k] = float2{0.f, 0.f};
     if (t < 2 * PR_N + 13) s_prof[t] = 0;
 #ifdef PFFRG_SHARED_PROFILE
-    const unsigned long long t_begin = clock64();
+    const unsigned long long t_begin = clock64(), rt_begin = wall_clock64();
 #endif
 
     Lds S;
@@ -1110,6 +1112,11 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
         nlev *= 2;
         start_level();
     }
+    // the unit's run time in constant-rate clock ticks, which orders the next launch's units
+    if (cost_slot && t == 0) {
+        const unsigned long long dt = wall_clock64() - rt_unit;
+        *cost_slot = dt < 0xffffffffull ? (uint32_t)dt : 0xffffffffu;
+    }
     if (A.count) {
         if (my_nodes) atomicAdd(&s_cnt[0], my_nodes);
         __syncthreads();
@@ -1119,6 +1126,19 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
 #ifdef PFFRG_SHARED_PROFILE
             s_prof[prof_base + PR_TOTAL] = clock64() - t_begin;
             for (int k = 0; k < 2 * PR_N + 13; ++k) atomicAdd(A.count + 2 + k, s_prof[k]);
+            // the block's duration in constant-rate (wall clock) ticks
+            const unsigned long long rt_end = wall_clock64();
+            atomicAdd(A.count + 31, rt_end - rt_begin);
+#ifdef PFFRG_SHARED_PROFILE_UNITS
+            const unsigned long long r = atomicAdd(A.count + N_COUNTERS, 1ull);
+            if (r < 16384) {
+                unsigned long long *rec = A.count + N_COUNTERS + 1 + 4 * r;
+                rec[0] = (unsigned long long)uc | ((unsigned long long)PASS << 32);
+                rec[1] = rt_begin;
+                rec[2] = rt_end;
+                rec[3] = (unsigned long long)__smid() | ((unsigned long long)blockIdx.x << 32);
+            }
+#endif
 #endif
         }
     }
@@ -1126,17 +1146,20 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
 
 template <int PASS, int SPT>
 __global__ __launch_bounds__(SH_MAXT) void k_shared_pass(SharedArgs A) {
-    shared_pass_body<PASS, SPT>(A, __builtin_amdgcn_readfirstlane(A.units[blockIdx.x]));
+    shared_pass_body<PASS, SPT>(A, __builtin_amdgcn_readfirstlane(A.units[blockIdx.x]),
+                                A.cost ? A.cost + blockIdx.x : nullptr);
 }
 
 // The same pass of several problems in one launch: unit code bits 26-31 select the problem's
 // arguments in tab; the block has the threads of the problem that needs most, the others idle.
+// cost (may be null) receives each unit's run time at the unit's place in units.
 template <int PASS, int SPT>
 __global__ __launch_bounds__(SH_MAXT) void k_shared_pass_batch(const SharedArgs *__restrict__ tab,
-                                                                const uint32_t *__restrict__ units, int n) {
+                                                                const uint32_t *__restrict__ units, uint32_t *cost,
+                                                                int n) {
     if ((int)blockIdx.x >= n) return;
     const uint32_t uc = __builtin_amdgcn_readfirstlane(units[blockIdx.x]);
-    shared_pass_body<PASS, SPT>(tab[uc >> 26], uc & 0x03ffffffu);
+    shared_pass_body<PASS, SPT>(tab[uc >> 26], uc & 0x03ffffffu, cost ? cost + blockIdx.x : nullptr);
 }
 
 // Sum of an element's partial results in the reference's order (the four domains, then the two
@@ -1200,14 +1223,14 @@ hipError_t launch_shared_pass(const SharedArgs &A, size_t lds, hipStream_t strea
 }
 
 template <int PASS, int SPT>
-hipError_t launch_shared_pass_batch_spt(const SharedArgs *tab, const uint32_t *units, int n, int nthreads, size_t lds,
-                                        hipStream_t stream) {
+hipError_t launch_shared_pass_batch_spt(const SharedArgs *tab, const uint32_t *units, uint32_t *cost, int n,
+                                        int nthreads, size_t lds, hipStream_t stream) {
     if (lds > 64 * 1024) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_shared_pass_batch<PASS, SPT>),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_shared_pass_batch<PASS, SPT>), dim3(n), dim3(nthreads), lds, stream, tab, units, n);
+    hipLaunchKernelGGL((k_shared_pass_batch<PASS, SPT>), dim3(n), dim3(nthreads), lds, stream, tab, units, cost, n);
     return hipGetLastError();
 }
 
@@ -1215,13 +1238,13 @@ hipError_t launch_shared_pass_batch_spt(const SharedArgs *tab, const uint32_t *u
 // per-lane state sized for spt sites per wave (the most any problem of the batch has); the
 // partial sums of each problem are then summed by launch_shared_finish on the same stream
 template <int PASS>
-hipError_t launch_shared_pass_batch(const SharedArgs *tab, const uint32_t *units, int n, int nthreads, int spt,
-                                    size_t lds, hipStream_t stream) {
+hipError_t launch_shared_pass_batch(const SharedArgs *tab, const uint32_t *units, uint32_t *cost, int n, int nthreads,
+                                    int spt, size_t lds, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     switch (spt) {
-    case 1: return launch_shared_pass_batch_spt<PASS, 1>(tab, units, n, nthreads, lds, stream);
-    case 2: return launch_shared_pass_batch_spt<PASS, 2>(tab, units, n, nthreads, lds, stream);
-    case 3: return launch_shared_pass_batch_spt<PASS, 3>(tab, units, n, nthreads, lds, stream);
+    case 1: return launch_shared_pass_batch_spt<PASS, 1>(tab, units, cost, n, nthreads, lds, stream);
+    case 2: return launch_shared_pass_batch_spt<PASS, 2>(tab, units, cost, n, nthreads, lds, stream);
+    case 3: return launch_shared_pass_batch_spt<PASS, 3>(tab, units, cost, n, nthreads, lds, stream);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1238,8 +1261,8 @@ hipError_t launch_shared_finish(const SharedArgs &A, hipStream_t stream) {
 
 #define PFFRG_SHARED_INSTANCES(KW, PASS)                                                                        \
     KW template hipError_t launch_shared_pass<PASS>(const SharedArgs &, size_t, hipStream_t);                  \
-    KW template hipError_t launch_shared_pass_batch<PASS>(const SharedArgs *, const uint32_t *, int, int, int,    \
-                                                          size_t, hipStream_t);                                \
+    KW template hipError_t launch_shared_pass_batch<PASS>(const SharedArgs *, const uint32_t *, uint32_t *, int,  \
+                                                          int, int, size_t, hipStream_t);                      \
     KW template hipError_t launch_shared_finish<PASS>(const SharedArgs &, hipStream_t);
 
 PFFRG_SHARED_INSTANCES(extern, P_KAT)
