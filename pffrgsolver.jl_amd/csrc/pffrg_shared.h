@@ -488,7 +488,14 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
             // the gathered-bytes counter: each descriptor of pair 0 is read on every site, of pair
             // 1 (the t channel's local vertices) on site 0
             unsigned long long cbd = 0;
+#if defined(PFFRG_SHARED_PROFILE) && defined(PFFRG_SHARED_PROFILE_DESC)
+            const unsigned long long _d0 = clock64();
+            int _nd = 0;
+#endif
             for (int tau = t; tau < ntask; tau += NT) {
+#if defined(PFFRG_SHARED_PROFILE) && defined(PFFRG_SHARED_PROFILE_DESC)
+                ++_nd;
+#endif
                 const int k = tau / per_k, r = tau - k * per_k;
                 const int c = r % 3, q = r / 3;
                 const int pr = q / nAS, v = q - pr * nAS;
@@ -500,6 +507,13 @@ __device__ __forceinline__ void shared_pass_body(const SharedArgs &A, uint32_t u
                 if (A.count) cbd += (unsigned long long)corners(dg.meta & 7) * (pr == 0 ? y.ns : 1);
             }
             if (A.count && cbd) atomicAdd(&s_cnt[1], cbd * 2ull * sizeof(float));
+#if defined(PFFRG_SHARED_PROFILE) && defined(PFFRG_SHARED_PROFILE_DESC)
+            // thread 0's own descriptor work (no barrier) and its task count; the most tasks a thread has
+            if (t == 0 && A.count) {
+                s_prof[2 * PR_N + 7 + (ch == 1 ? 2 : 0)] += clock64() - _d0;
+                s_prof[2 * PR_N + 8 + (ch == 1 ? 2 : 0)] += (unsigned long long)_nd;
+            }
+#endif
         }
         __syncthreads();
 #if defined(PFFRG_SHARED_PROFILE) && defined(PFFRG_SHARED_PROFILE_PAIRS)
