@@ -307,7 +307,7 @@ def main():
     ap.add_argument("--batch-N", type=int, default=32, help="batch / batchflow: mesh size of every problem")
     ap.add_argument("--batch-skip-single", action="store_true",
                     help="batchflow: run the batched flows only (no one-at-a-time comparison)")
-    ap.add_argument("--workload", choices=["single", "batch", "batchflow", "flow", "warmstart"], default="single",
+    ap.add_argument("--workload", choices=["single", "batch", "batchflow", "flow", "warmstart", "mlp"], default="single",
                     help="single: the 2-loop square L=6 RHS (headline); batch: the J1-J2 sweep + uneven "
                          "truncations as one batched 1-loop RHS")
     ap.add_argument("--shape", default="", metavar="W,K,M",
@@ -330,6 +330,8 @@ def main():
         return flow_workload(args)
     if args.workload == "warmstart":
         return warmstart_workload(args)
+    if args.workload == "mlp":
+        return mlp_workload(args)
     if args.copy_calibration > 0:
         import torch
         n = args.copy_calibration * (1 << 20) // 4
@@ -553,6 +555,34 @@ def flow_workload(args):
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def mlp_workload(args):
+    """Training steps (backward + Adam) of the warm start's network at the size it trains with:
+    2 inputs, three hidden layers of 512 (SiLU), 256 outputs, a batch of 64 samples; BF16 MFMA
+    products (k_gemm_bf16) with FP32 accumulation."""
+    from pffrg import learn
+    net = learn.MLP(2, 512, 4, 256, seed=3)
+    rng = np.random.default_rng(SEED)
+    x = rng.standard_normal((64, 2)).astype(np.float32)
+    t = rng.standard_normal((64, 256)).astype(np.float32)
+    steps = max(args.steps, 1) * 100
+    for _ in range(max(args.warmup, 1) * 20):
+        net.backward(x, t)
+        net.adam(1e-3)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss = net.backward(x, t)
+        net.adam(1e-3)
+    dt = time.perf_counter() - t0
+    log(f"  {dt / steps * 1e3:.3f} ms per training step, loss {loss:.3e}")
+    print(json.dumps({
+        "metric": "warm-start network training steps/s (backward + Adam, batch 64)", "value": steps / dt,
+        "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": max(args.warmup, 1) * 20,
+        "ms_per_step": dt / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 products, fp32 accumulation and parameters", "data": "synthetic (seeded N(0,1) inputs and targets)",
+        "config": {"workload": "MLP 2 -> 512 x3 (SiLU) -> 256, batch 64, backward + Adam per step"},
+        "final_loss": float(loss)}), flush=True)
 
 
 def warmstart_workload(args):

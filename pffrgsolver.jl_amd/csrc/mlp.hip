@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -46,7 +47,6 @@ struct pffrg_decoder {
 
 namespace {
 
-typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline uint16_t f2bf(float f) {
@@ -57,8 +57,7 @@ __host__ __device__ inline uint16_t f2bf(float f) {
 }
 
 // C(m, n) (+)= Σ_k A(m, k) B(k, n) (+ bias[n]); A(m, k) = a[m*sam + k*sak], B(k, n) = b[k*sbk + n*sbn]
-// (BF16 bits), C(m, n) = c[m*scm + n*scn] (FP32).  One wavefront per 16 x 16 tile of C, one
-// v_mfma_f32_16x16x16bf16 per 16-deep slice of K; out-of-range operands read as zero.
+// (BF16 bits), C(m, n) = c[m*scm + n*scn] (FP32); out-of-range operands read as zero.
 struct GemmArgs {
     const uint16_t *a;
     long sam, sak;
@@ -71,29 +70,115 @@ struct GemmArgs {
     int accumulate;
 };
 
-__global__ __launch_bounds__(64) void k_gemm_bf16(GemmArgs g) {
-    const int tm = blockIdx.y * 16, tn = blockIdx.x * 16;
-    const int l = threadIdx.x;
-    const int r = l % 16, kq = (l / 16) * 4;
-    v4f acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < g.K; k0 += 16) {
-        v4s av, bv;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int k = k0 + kq + i;
-            const int m = tm + r, n = tn + r;
-            av[i] = (m < g.M && k < g.K) ? (short)g.a[m * g.sam + k * g.sak] : (short)0;
-            bv[i] = (n < g.N && k < g.K) ? (short)g.b[k * g.sbk + n * g.sbn] : (short)0;
-        }
-        // A operand: lane (r, kq) holds A(tm + r, k0 + kq .. +3); B operand: B(k0 + kq .. +3, tn + r)
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+// One 32 x 32 tile of C per 256-thread block.  K goes in chunks of 128 staged through LDS (A as
+// [m][k] and B as [n][k], k contiguous, rows padded to 136); the four waves split a chunk's K
+// into 32-deep slices, each wave forming the whole tile's partial product on its slice with
+// 2 x 2 v_mfma_f32_16x16x32_bf16 (lane l: A row l%16 and B column l%16 at k = 8 (l/16) .. +7;
+// result rows 4 (l/16) .. +3 of column l%16).  The next chunk is loaded into registers while the
+// current one is multiplied, and the waves' partial tiles are summed in wave order at the end.
+// The small GEMMs of the warm-start network (batch 64, widths up to 1024) fill more CUs this way
+// than with 64-wide tiles, and each wave's K loop is a quarter as long.
+constexpr int GT = 32, GKC = 128, GLD = GKC + 8;
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+// Stage one operand chunk: rows r of the tile (m for A, n for B) and k of the chunk.  kcont: the
+// operand is contiguous along k (stride sk == 1); then thread t takes row t/8 and 16 consecutive
+// k, else k = t/2 and 16 consecutive rows.  Element (row, k) is p[row*sr + k*sk].
+__device__ __forceinline__ void gemm_load(const uint16_t *__restrict__ p, long sr, long sk, int r0, int R, int k0, int K,
+                                          bool kcont, int t, uint4 (&v)[2]) {
+    const int a0 = kcont ? (t >> 3) : 16 * (t & 1);   // first row
+    const int b0 = kcont ? 16 * (t & 7) : (t >> 1);   // first k
+    const int row = r0 + a0, k = k0 + b0;
+    const long step = kcont ? sk : sr;                 // stride along the 16 elements
+    const uint16_t *q = p + (long)row * sr + (long)k * sk;
+    const bool full = kcont ? (row < R && k + 15 < K) : (k < K && row + 15 < R);
+    if (full && step == 1 && ((reinterpret_cast<uintptr_t>(q) & 15u) == 0)) {
+        v[0] = reinterpret_cast<const uint4 *>(q)[0];
+        v[1] = reinterpret_cast<const uint4 *>(q)[1];
+        return;
     }
-    // result: lane (r, kq) holds C(tm + kq .. +3, tn + r)
+    uint16_t e[16];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int m = tm + kq + i, n = tn + r;
+    for (int i = 0; i < 16; ++i) {
+        const int rr = kcont ? row : row + i, kk = kcont ? k + i : k;
+        e[i] = (rr < R && kk < K) ? p[(long)rr * sr + (long)kk * sk] : (uint16_t)0;
+    }
+    uint32_t w[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) w[i] = (uint32_t)e[2 * i] | ((uint32_t)e[2 * i + 1] << 16);
+    v[0] = uint4{w[0], w[1], w[2], w[3]};
+    v[1] = uint4{w[4], w[5], w[6], w[7]};
+}
+
+// the staged elements into the LDS tile s ([row][k], row stride GLD)
+__device__ __forceinline__ void gemm_store(uint16_t *s, bool kcont, int t, const uint4 (&v)[2]) {
+    if (kcont) {
+        uint4 *d = reinterpret_cast<uint4 *>(s + (t >> 3) * GLD + 16 * (t & 7));
+        d[0] = v[0];
+        d[1] = v[1];
+        return;
+    }
+    const int a0 = 16 * (t & 1), k = t >> 1;
+    const uint32_t w[8] = {v[0].x, v[0].y, v[0].z, v[0].w, v[1].x, v[1].y, v[1].z, v[1].w};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        s[(a0 + 2 * i) * GLD + k] = (uint16_t)(w[i] & 0xffffu);
+        s[(a0 + 2 * i + 1) * GLD + k] = (uint16_t)(w[i] >> 16);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
+    __shared__ __align__(16) uint16_t sA[GT * GLD];
+    __shared__ __align__(16) uint16_t sB[GT * GLD];
+    __shared__ float sR[4 * GT * GT];
+    const int t = threadIdx.x, wv = t >> 6, l = t & 63;
+    const int tm = blockIdx.y * GT, tn = blockIdx.x * GT;
+    const bool akc = g.sak == 1, bkc = g.sbk == 1;
+    v4f acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    uint4 ra[2], rb[2];
+    gemm_load(g.a, g.sam, g.sak, tm, g.M, 0, g.K, akc, t, ra);
+    gemm_load(g.b, g.sbn, g.sbk, tn, g.N, 0, g.K, bkc, t, rb);
+    for (int k0 = 0; k0 < g.K; k0 += GKC) {
+        __syncthreads();  // the previous chunk's products have read the tiles
+        gemm_store(sA, akc, t, ra);
+        gemm_store(sB, bkc, t, rb);
+        __syncthreads();
+        if (k0 + GKC < g.K) {
+            gemm_load(g.a, g.sam, g.sak, tm, g.M, k0 + GKC, g.K, akc, t, ra);
+            gemm_load(g.b, g.sbn, g.sbk, tn, g.N, k0 + GKC, g.K, bkc, t, rb);
+        }
+        const int ko = 32 * wv + 8 * (l >> 4);
+        bf8 av[2], bv[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            av[i] = *reinterpret_cast<const bf8 *>(sA + (16 * i + (l & 15)) * GLD + ko);
+            bv[i] = *reinterpret_cast<const bf8 *>(sB + (16 * i + (l & 15)) * GLD + ko);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+    }
+    // the waves' partial tiles, then their sum in wave order
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                sR[wv * GT * GT + (16 * i + 4 * (l >> 4) + q) * GT + 16 * j + (l & 15)] = acc[i][j][q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < GT * GT / 256; ++q) {
+        const int e = t + 256 * q, mr = e / GT, nc = e % GT;
+        const int m = tm + mr, n = tn + nc;
         if (m < g.M && n < g.N) {
-            float v = acc[i];
+            float v = ((sR[e] + sR[GT * GT + e]) + sR[2 * GT * GT + e]) + sR[3 * GT * GT + e];
             if (g.bias) v += g.bias[n];
             float *p = g.c + m * g.scm + n * g.scn;
             *p = g.accumulate ? *p + v : v;
@@ -124,12 +209,16 @@ __global__ void k_to_bf(const float *x, uint16_t *y, long n) {
         y[i] = f2bf(x[i]);
 }
 
-// mean-squared error L = Σ (y - t)^2 / n and dL/dy = 2 (y - t) / n (one block)
+// mean-squared error L = Σ (y - t)^2 / n and dL/dy = 2 (y - t) / n: each block's partial sum of
+// squares (a fixed grid for a given n, reduced in a fixed order) into part[blockIdx.x], then
+// k_mse_sum adds the partials in block order, so the loss does not depend on scheduling
+constexpr int MSE_BLOCKS = 64;
+
 __global__ __launch_bounds__(256) void k_mse(const float *y, const float *t, long n, float *dy, uint16_t *dybf,
-                                             float *loss) {
+                                             float *part) {
     __shared__ float red[256];
     float s = 0.f;
-    for (long i = threadIdx.x; i < n; i += 256) {
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
         const float d = y[i] - t[i];
         s += d * d;
         const float g = 2.0f * d / (float)n;
@@ -142,16 +231,34 @@ __global__ __launch_bounds__(256) void k_mse(const float *y, const float *t, lon
         if ((int)threadIdx.x < m) red[threadIdx.x] += red[threadIdx.x + m];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *loss = red[0] / (float)n;
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// db[n] = Σ_b dz[b][n]
-__global__ void k_colsum(const float *dz, int B, int N, float *db) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+__global__ __launch_bounds__(64) void k_mse_sum(const float *part, int np, long n, float *loss) {
+    if (threadIdx.x == 0) {
+        float s = 0.f;
+        for (int b = 0; b < np; ++b) s += part[b];
+        *loss = s / (float)n;
+    }
+}
+
+// db[n] = Σ_b dz[b][n]: 32 columns per 256-thread block, eight row slices per column summed in
+// slice order
+__global__ __launch_bounds__(256) void k_colsum(const float *dz, int B, int N, float *db) {
+    __shared__ float red[8][33];
+    const int c = threadIdx.x & 31, r = threadIdx.x >> 5;
+    const int n = blockIdx.x * 32 + c;
     float s = 0.f;
-    for (int b = 0; b < B; ++b) s += dz[(long)b * N + n];
-    db[n] = s;
+    if (n < N)
+        for (int b = r; b < B; b += 8) s += dz[(long)b * N + n];
+    red[r][c] = s;
+    __syncthreads();
+    if (r == 0 && n < N) {
+        float v = red[0][c];
+#pragma unroll
+        for (int k = 1; k < 8; ++k) v += red[k][c];
+        db[n] = v;
+    }
 }
 
 __global__ void k_adam(float *p, const float *g, float *m1, float *m2, uint16_t *pbf, long n, float lr, float b1,
@@ -197,7 +304,7 @@ int mfail(pffrg_mlp *m, int code, const std::string &s) {
 hipError_t gemm(hipStream_t s, const uint16_t *a, long sam, long sak, const uint16_t *b, long sbk, long sbn, int M, int N,
                 int K, float *c, long scm, long scn, const float *bias, int accumulate) {
     GemmArgs g{a, sam, sak, b, sbk, sbn, M, N, K, c, scm, scn, bias, accumulate};
-    hipLaunchKernelGGL(k_gemm_bf16, dim3((N + 15) / 16, (M + 15) / 16), dim3(64), 0, s, g);
+    hipLaunchKernelGGL(k_gemm_bf16, dim3((N + GT - 1) / GT, (M + GT - 1) / GT), dim3(256), 0, s, g);
     return hipGetLastError();
 }
 
@@ -303,7 +410,7 @@ int pffrg_mlp_create(int32_t n_in, int32_t width, int32_t n_layers, int32_t n_ou
     ok = ok && hipMalloc(&m->m1, off * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&m->m2, off * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&m->wbf, off * sizeof(uint16_t)) == hipSuccess;
-    ok = ok && hipMalloc(&m->loss, sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc(&m->loss, (1 + MSE_BLOCKS) * sizeof(float)) == hipSuccess;  // the loss, then k_mse's partials
     ok = ok && hipMemcpy(m->params, p.data(), off * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMemcpy(m->wbf, pb.data(), off * sizeof(uint16_t), hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMemset(m->grads, 0, off * sizeof(float)) == hipSuccess;
@@ -373,13 +480,15 @@ int pffrg_mlp_backward(pffrg_mlp *m, const float *x, const float *target, int32_
     const int B = batch;
     const long no = (long)B * m->dims[L];
     MCHK(m, hipMemcpyAsync(m->t, target, no * sizeof(float), hipMemcpyHostToDevice, m->stream));
-    hipLaunchKernelGGL(k_mse, dim3(1), dim3(256), 0, m->stream, m->z[L - 1], m->t, no, m->dz, m->dzbf, m->loss);
+    const int nb = (int)std::min<long>(MSE_BLOCKS, (no + 255) / 256);
+    hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, m->stream, m->z[L - 1], m->t, no, m->dz, m->dzbf, m->loss + 1);
+    hipLaunchKernelGGL(k_mse_sum, dim3(1), dim3(64), 0, m->stream, m->loss + 1, nb, no, m->loss);
     MCHK(m, hipGetLastError());
     for (int l = L - 1; l >= 0; --l) {
         const int in = m->dims[l], out = m->dims[l + 1];
         // dW_l[n][k] = Σ_b dz[b][n] h_{l-1}[b][k]
         MCHK(m, gemm(m->stream, m->dzbf, 1, out, m->hbf[l], in, 1, out, in, B, m->grads + m->woff[l], in, 1, nullptr, 0));
-        hipLaunchKernelGGL(k_colsum, dim3((out + 255) / 256), dim3(256), 0, m->stream, m->dz, B, out,
+        hipLaunchKernelGGL(k_colsum, dim3((out + 31) / 32), dim3(256), 0, m->stream, m->dz, B, out,
                            m->grads + m->boff[l]);
         MCHK(m, hipGetLastError());
         if (l > 0) {
