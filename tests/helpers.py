@@ -44,6 +44,18 @@ def entries(a: A.Action, ch, k, w1, w2, w3):
     return np.concatenate([a.view(c, ch, q)[(slice(None),) + idx] for c in range(2)])
 
 
+def class_sample(elems, nO, nn, stride):
+    """Every stride-th element, plus every element of each (channel, kernel) class at the ends of
+    the Ω and ν ranges (w1 in {0, nO-1}; each used frequency index in {0, nn-1})."""
+    pick = set(elems[::stride])
+    eo, en = {0, nO - 1}, {0, nn - 1}
+    for e in elems:
+        _, _, w1, w2, w3 = e
+        if w1 in eo and (w2 < 0 or w2 in en) and (w3 < 0 or w3 in en):
+            pick.add(e)
+    return sorted(pick)
+
+
 def full_elements(nO, nn):
     """Output elements of a full pass: every q3 entry (compute.jl:115-174)."""
     return [(ch, k, w1, w2, w3) for ch in range(3) for w1 in range(nO)

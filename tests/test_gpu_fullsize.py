@@ -1,6 +1,7 @@
 """Parity at the single-RHS bench size (square L=6, 16 sites, N=40, Λ=1, eval=10, the seeded
 synthetic vertex): the GPU's compute_dΣ!, its Katanin pass and the left pass of the 2-loop
-right-hand side against the FP64 oracle on strided samples of the output elements, and
+right-hand side against the FP64 oracle on strided samples of the output elements (with every
+(channel, kernel) class at the ends of its Ω and ν ranges), and
 FP32-mixed against FP64 on every element; at the pyrochlore 3-loop flow's shape (L=4, N=32),
 the central pass of the multiloop right-hand side; at the warm-start shape (kagome L=5, N=32),
 the parquet SDE and BSE passes; at the largest meshes the library takes (64 points per axis), the
@@ -17,11 +18,13 @@ from pffrg import action as A
 from pffrg import flow as F
 from pffrg.native import FP32_MIXED
 
-from helpers import entries, full_elements, half_elements, rel_err
+from helpers import class_sample, entries, full_elements, half_elements, rel_err
 
 pytestmark = pytest.mark.gpu
 
-STRIDE = 997  # every 997th element of the half pass: 117 elements over all channels and kernels
+# every 997th element of the half pass (117 over all channels and kernels), plus each (channel,
+# kernel) class at the ends of its Ω and ν ranges
+STRIDE = 997
 
 
 def test_bench_size_katanin_rhs_matches_oracle_sample():
@@ -45,7 +48,7 @@ def test_bench_size_katanin_rhs_matches_oracle_sample():
     orc.compute_corrs(1.0, a, dd, True)
     out = A.Action.empty(ns, m)
     elems = half_elements(m.num_Omega, m.num_nu)
-    sample = elems[::STRIDE]
+    sample = class_sample(elems, m.num_Omega, m.num_nu, STRIDE)
     for e in sample:
         orc.compute_element(O.P_KAT, 1.0, a, dd, *e, 10, out)
     scale = float(np.max(np.abs(got.gamma)))
@@ -74,7 +77,7 @@ def test_bench_size_left_pass_matches_oracle_sample():
     orc = O.Oracle(prob.r, m)
     orc.compute_corrs(1.0, a, a, False)
     out = A.Action.empty(ns, m)
-    sample = full_elements(m.num_Omega, m.num_nu)[::STRIDE]
+    sample = class_sample(full_elements(m.num_Omega, m.num_nu), m.num_Omega, m.num_nu, STRIDE)
     for e in sample:
         orc.compute_element(O.P_LEFT, 1.0, a, da1, *e, 10, out)
     scale = float(np.max(np.abs(left.gamma)))
@@ -101,7 +104,7 @@ def test_pyrochlore_flow_size_central_pass_matches_oracle_sample():
     orc = O.Oracle(prob.r, m)
     orc.compute_corrs(1.0, a, a, False)
     out = A.Action.empty(ns, m)
-    sample = half_elements(m.num_Omega, m.num_nu)[::499]
+    sample = class_sample(half_elements(m.num_Omega, m.num_nu), m.num_Omega, m.num_nu, 499)
     for e in sample:
         orc.compute_element(O.P_CENTRAL, 1.0, a, left2, *e, 10, out)
     scale = float(np.max(np.abs(central.gamma)))
@@ -129,7 +132,7 @@ def test_warmstart_size_parquet_passes_match_oracle():
     e_sde = max(rel_err(sde.view(c, 0, q), ref.view(c, 0, q)) for c in range(2) for q in ("q1", "q2_2"))
     e_sig = rel_err(sde.Sigma, ref.Sigma)
     out = A.Action.empty(ns, m)
-    sample = half_elements(m.num_Omega, m.num_nu)[::499]
+    sample = class_sample(half_elements(m.num_Omega, m.num_nu), m.num_Omega, m.num_nu, 499)
     for e in sample:
         orc.compute_element(O.P_BSE, 1.0, a, a, *e, 10, out)
     scale = float(np.max(np.abs(bse.gamma)))

@@ -107,6 +107,26 @@ def test_shared_parquet_passes():
 SHAPES = [(8, 8, 64), (8, 8, 16), (8, 4, 32), (4, 8, 64), (4, 4, 16)]
 
 
+def test_shared_unit_order_changes_no_result():
+    # every launch of a pass dispatches its units in the order of the previous launch's unit run
+    # times; each partial sum has one owner unit, so repeated right-hand sides on one context are
+    # bitwise equal whatever order each launch ran in
+    prob = small_problem("square", 4, 8, 1.0)
+    ns, m = prob.r.n_sites, prob.mesh
+    a = A.synthetic_action(ns, m, prob.bare, seed=77)
+    with prob.context(n_slots=3, precision=FP32_MIXED) as ctx:
+        ctx.upload(0, a)
+        ctx.upload(1, A.Action.empty(ns, m))
+        ctx.upload(2, A.Action.empty(ns, m))
+        outs = []
+        for _ in range(3):
+            F.rhs(ctx, 1.0, 2, 0, 1, 10, da_l=2)
+            outs.append(ctx.download(1))
+    print(f"square L=4 sites {ns} N=8: three 2-loop right-hand sides in measured unit order", flush=True)
+    for o in outs[1:]:
+        assert np.array_equal(o.gamma, outs[0].gamma) and np.array_equal(o.Sigma, outs[0].Sigma)
+
+
 def test_shared_shapes_agree():
     # every launch shape of the shared-node pass (waves per block, batch capacity, most nodes per
     # batch) computes the same integrals: within FP32 rounding of each other, within the mode's
