@@ -103,6 +103,10 @@ struct SigmaArgs {
 // one workgroup per σ point w_i, i >= 1; the integrand restates compute_dΣ_kernel(_corr1,
 // _corr2) (flow_su2_self.jl, flow_u1_dm_self.jl) and compute_Σ_kernel (SDE_su2.jl:45-77,
 // SDE_u1_dm.jl:45-91).
+// Threads of a σ-point block: the rule nodes' site terms are spread over them (a bisection's 30
+// nodes on 16 sites in one round).
+constexpr int SIGMA_T = 512;
+
 template <int KIND>
 __device__ __forceinline__ void sigma_point(const SigmaArgs &Q, int si) {
     __shared__ double s_mesh[5 * MAXN];
@@ -110,13 +114,14 @@ __device__ __forceinline__ void sigma_point(const SigmaArgs &Q, int si) {
     __shared__ int s_exch[64];
     __shared__ double s_mult[64];
     __shared__ Seg segs[GK_MAX_SEGS];
-    __shared__ double fv[64];
+    __shared__ double fv[64], ts[64];
+    __shared__ double s_terms[60 * 65];  // per rule node: the local term, then each site's term
     __shared__ int ctl[4];
     __shared__ double ctld[2];
     const Sys &y = Q.y;
     const int lane = threadIdx.x;
-    for (int k = lane; k < 5 * MAXN; k += 64) s_mesh[k] = Q.mesh[k];
-    for (int k = lane; k < y.nsig; k += 64) {
+    for (int k = lane; k < 5 * MAXN; k += blockDim.x) s_mesh[k] = Q.mesh[k];
+    for (int k = lane; k < y.nsig; k += blockDim.x) {
         s_sA[k] = Q.A.Sigma[k];
         s_sV[k] = (KIND == SK_CORR2) ? Q.sigV[k] : 0.0;
     }
@@ -128,91 +133,99 @@ __device__ __forceinline__ void sigma_point(const SigmaArgs &Q, int si) {
     const Meshes<> M{s_mesh, s_mesh + MAXN, s_mesh + 2 * MAXN, s_mesh + 3 * MAXN, s_mesh + 4 * MAXN};
     const double w = s_mesh[si];
     const double L = Q.L;
-    auto f = [&](double v) -> double {
-        double val;
+    // The integrand at loop frequency v is ((local - Σ_j term_j) · factor): piece(v, -1) is the local
+    // term of site 0, piece(v, j) site j's term with its coefficient, factor(v) the propagators.
+    // SU(2): local = 3 Γs + Γd, term_j = 2 m_j S² Γd_j (flow_su2_self.jl:2-101, SDE_su2.jl:45-77);
+    // U(1)-DM: local = 2 Γxx + Γzz + Γdd, term_j = 2 m_j Γdd_j (flow_u1_dm_self.jl:22-28,
+    // SDE_u1_dm.jl:68-76).
+    auto piece = [&](double v, int j) -> double {
         if (y.nc == 6) {
-            // U(1)-DM: 2Γxx + Γzz + Γdd (local) - Σ_j 2 m_j Γdd_j (flow_u1_dm_self.jl:22-28,
-            // SDE_u1_dm.jl:68-76)
-            double l6[6], n6[6];
+            double l6[6];
             if (KIND == SK_SDE) {
                 const double INF = __builtin_huge_val();
-                const Buf<> b1s = buf_s(M, y, v + w, INF, 0.5 * (v - w));
-                const Buf<> b2s = buf_s(M, y, v + w, INF, 0.5 * (-v + w));
                 const Buf<> e = buf_empty<double>();
-                gamma_site_u1<true, false, false>(Q.V, b2s, e, e, 0, s_exch, y, l6);
-                val = 2.0 * l6[U1_XX] + l6[U1_ZZ] + l6[U1_DD];
-                for (int j = 0; j < y.ns; ++j) {
-                    gamma_site_u1<true, false, false>(Q.V, b1s, e, e, j, s_exch, y, n6);
-                    val -= 2.0 * s_mult[j] * n6[U1_DD];
+                if (j < 0) {
+                    const Buf<> b2s = buf_s(M, y, v + w, INF, 0.5 * (-v + w));
+                    gamma_site_u1<true, false, false>(Q.V, b2s, e, e, 0, s_exch, y, l6);
+                    return 2.0 * l6[U1_XX] + l6[U1_ZZ] + l6[U1_DD];
                 }
-                val *= -G_full(L, v, s_mesh, y.nsig, s_sA) / (2.0 * PI_D);
-                return val;
+                const Buf<> b1s = buf_s(M, y, v + w, INF, 0.5 * (v - w));
+                gamma_site_u1<true, false, false>(Q.V, b1s, e, e, j, s_exch, y, l6);
+                return 2.0 * s_mult[j] * l6[U1_DD];
+            }
+            if (j < 0) {
+                const Buf<> b2s = buf_s(M, y, v + w, 0.5 * (-v + w), 0.5 * (-v + w));
+                const Buf<> b2t = buf_t(M, y, v - w, 0.5 * (v + w), 0.5 * (v + w));
+                const Buf<> b2u = buf_u(M, y, 0.0, w, v);
+                if (KIND == SK_CORR1) gamma_site_u1<true, true, false>(Q.V, b2s, b2t, b2u, 0, s_exch, y, l6);
+                else gamma_site_u1<true, true, true>(Q.A, b2s, b2t, b2u, 0, s_exch, y, l6);
+                return 2.0 * l6[U1_XX] + l6[U1_ZZ] + l6[U1_DD];
             }
             const Buf<> b1s = buf_s(M, y, v + w, 0.5 * (-v + w), 0.5 * (v - w));
             const Buf<> b1t = buf_t(M, y, 0.0, w, v);
             const Buf<> b1u = buf_u(M, y, -v + w, 0.5 * (v + w), 0.5 * (v + w));
+            if (KIND == SK_CORR1) gamma_site_u1<true, false, true>(Q.V, b1s, b1t, b1u, j, s_exch, y, l6);
+            else gamma_site_u1<true, true, true>(Q.A, b1s, b1t, b1u, j, s_exch, y, l6);
+            return 2.0 * s_mult[j] * l6[U1_DD];
+        }
+        double sj, dj;
+        if (KIND == SK_SDE) {
+            const double INF = __builtin_huge_val();
+            const Buf<> e = buf_empty<double>();
+            if (j < 0) {
+                const Buf<> b2s = buf_s(M, y, v + w, INF, 0.5 * (-v + w));
+                gamma_site<double, true, false, false>(Q.V, b2s, e, e, 0, s_exch, y, sj, dj);
+                return 3.0 * sj + dj;
+            }
+            const Buf<> b1s = buf_s(M, y, v + w, INF, 0.5 * (v - w));
+            gamma_site<double, true, false, false>(Q.V, b1s, e, e, j, s_exch, y, sj, dj);
+            return 2.0 * s_mult[j] * y.S2 * dj;
+        }
+        if (j < 0) {
             const Buf<> b2s = buf_s(M, y, v + w, 0.5 * (-v + w), 0.5 * (-v + w));
             const Buf<> b2t = buf_t(M, y, v - w, 0.5 * (v + w), 0.5 * (v + w));
             const Buf<> b2u = buf_u(M, y, 0.0, w, v);
-            if (KIND == SK_CORR1) gamma_site_u1<true, true, false>(Q.V, b2s, b2t, b2u, 0, s_exch, y, l6);
-            else gamma_site_u1<true, true, true>(Q.A, b2s, b2t, b2u, 0, s_exch, y, l6);
-            val = 2.0 * l6[U1_XX] + l6[U1_ZZ] + l6[U1_DD];
-            for (int j = 0; j < y.ns; ++j) {
-                if (KIND == SK_CORR1) gamma_site_u1<true, false, true>(Q.V, b1s, b1t, b1u, j, s_exch, y, n6);
-                else gamma_site_u1<true, true, true>(Q.A, b1s, b1t, b1u, j, s_exch, y, n6);
-                val -= 2.0 * s_mult[j] * n6[U1_DD];
-            }
-            if (KIND == SK_DSIGMA) {
-                val *= S_single(L, v, s_mesh, y.nsig, s_sA) / (2.0 * PI_D);
-            } else if (KIND == SK_CORR1) {
-                val *= -G_full(L, v, s_mesh, y.nsig, s_sA) / (2.0 * PI_D);
-            } else {
-                const double G = G_full(L, v, s_mesh, y.nsig, s_sA);
-                val *= G * G * sigma_at(v, s_mesh, y.nsig, s_sV) / (2.0 * PI_D);
-            }
-            return val;
-        }
-        if (KIND == SK_SDE) {
-            const double INF = __builtin_huge_val();
-            const Buf<> b1s = buf_s(M, y, v + w, INF, 0.5 * (v - w));
-            const Buf<> b2s = buf_s(M, y, v + w, INF, 0.5 * (-v + w));
-            const Buf<> e = buf_empty<double>();
-            double s0, d0;
-            gamma_site<double, true, false, false>(Q.V, b2s, e, e, 0, s_exch, y, s0, d0);
-            val = 3.0 * s0 + d0;
-            for (int j = 0; j < y.ns; ++j) {
-                double sj, dj;
-                gamma_site<double, true, false, false>(Q.V, b1s, e, e, j, s_exch, y, sj, dj);
-                val -= 2.0 * s_mult[j] * y.S2 * dj;
-            }
-            val *= -G_full(L, v, s_mesh, y.nsig, s_sA) / (2.0 * PI_D);
-            return val;
+            if (KIND == SK_CORR1) gamma_site<double, true, true, false>(Q.V, b2s, b2t, b2u, 0, s_exch, y, sj, dj);
+            else gamma_site<double, true, true, true>(Q.A, b2s, b2t, b2u, 0, s_exch, y, sj, dj);
+            return 3.0 * sj + dj;
         }
         const Buf<> b1s = buf_s(M, y, v + w, 0.5 * (-v + w), 0.5 * (v - w));
         const Buf<> b1t = buf_t(M, y, 0.0, w, v);
         const Buf<> b1u = buf_u(M, y, -v + w, 0.5 * (v + w), 0.5 * (v + w));
-        const Buf<> b2s = buf_s(M, y, v + w, 0.5 * (-v + w), 0.5 * (-v + w));
-        const Buf<> b2t = buf_t(M, y, v - w, 0.5 * (v + w), 0.5 * (v + w));
-        const Buf<> b2u = buf_u(M, y, 0.0, w, v);
-        double s0, d0;
-        if (KIND == SK_CORR1) gamma_site<double, true, true, false>(Q.V, b2s, b2t, b2u, 0, s_exch, y, s0, d0);
-        else gamma_site<double, true, true, true>(Q.A, b2s, b2t, b2u, 0, s_exch, y, s0, d0);
-        val = 3.0 * s0 + d0;
-        for (int j = 0; j < y.ns; ++j) {
-            double sj, dj;
-            if (KIND == SK_CORR1) gamma_site<double, true, false, true>(Q.V, b1s, b1t, b1u, j, s_exch, y, sj, dj);
-            else gamma_site<double, true, true, true>(Q.A, b1s, b1t, b1u, j, s_exch, y, sj, dj);
-            val -= 2.0 * s_mult[j] * y.S2 * dj;
+        if (KIND == SK_CORR1) gamma_site<double, true, false, true>(Q.V, b1s, b1t, b1u, j, s_exch, y, sj, dj);
+        else gamma_site<double, true, true, true>(Q.A, b1s, b1t, b1u, j, s_exch, y, sj, dj);
+        return 2.0 * s_mult[j] * y.S2 * dj;
+    };
+    auto factor = [&](double v) -> double {
+        if (KIND == SK_DSIGMA) return S_single(L, v, s_mesh, y.nsig, s_sA) / (2.0 * PI_D);
+        if (KIND == SK_CORR1 || KIND == SK_SDE) return -G_full(L, v, s_mesh, y.nsig, s_sA) / (2.0 * PI_D);
+        const double G = G_full(L, v, s_mesh, y.nsig, s_sA);
+        return G * G * sigma_at(v, s_mesh, y.nsig, s_sV) / (2.0 * PI_D);
+    };
+    // the rule nodes' pieces over all threads, then each node's sum in site order by one thread,
+    // mapped from t ∈ (-1, 1) to v = t/(1-t²) with the weight (1+t²)/(1-t²)²
+    const int per = y.ns + 1;
+    auto ev = [&](int n, const double *tq, double *fo) {
+        // the n ns site terms first, then the n local terms, so that a wave's lanes take one path
+        const int nsite = n * y.ns;
+        for (int it = lane; it < nsite + n; it += blockDim.x) {
+            const bool site = it < nsite;
+            const int q = site ? it / y.ns : it - nsite, j = site ? it - q * y.ns : -1;
+            const double t = tq[q];
+            const double den = 1.0 / (1.0 - t * t);
+            s_terms[q * per + 1 + j] = piece(t * den, j);
         }
-        if (KIND == SK_DSIGMA) {
-            val *= S_single(L, v, s_mesh, y.nsig, s_sA) / (2.0 * PI_D);
-        } else if (KIND == SK_CORR1) {
-            val *= -G_full(L, v, s_mesh, y.nsig, s_sA) / (2.0 * PI_D);
-        } else {
-            const double G = G_full(L, v, s_mesh, y.nsig, s_sA);
-            val *= G * G * sigma_at(v, s_mesh, y.nsig, s_sV) / (2.0 * PI_D);
+        __syncthreads();
+        if (lane < n) {
+            const double t = tq[lane];
+            const double t2 = t * t;
+            const double den = 1.0 / (1.0 - t2);
+            const double v = t * den;
+            double val = s_terms[lane * per];
+            for (int jj = 0; jj < y.ns; ++jj) val -= s_terms[lane * per + 1 + jj];
+            val *= factor(v);
+            fo[lane] = val * (1.0 + t2) * den * den;
         }
-        return val;
     };
     // breakpoints (-Inf, -2Λ, 0, 2Λ, Inf) in the t/(1-t²) variable
     double bp[5];
@@ -220,7 +233,7 @@ __device__ __forceinline__ void sigma_point(const SigmaArgs &Q, int si) {
     const double inner[3] = {-2.0 * L, 0.0, 2.0 * L};
     for (int k = 0; k < 3; ++k) bp[k + 1] = 2.0 * inner[k] / (1.0 + hypot(1.0, 2.0 * inner[k]));
     bp[4] = 1.0;
-    const double I = gk_block(f, 0, 0.0, bp, 5, Q.atol, Q.rtol, segs, fv, ctl, ctld);
+    const double I = gk_block_coop(ev, bp, 5, Q.atol, Q.rtol, segs, fv, ts, ctl, ctld);
     if (lane == 0) {
         if (KIND == SK_CORR2) {
             double o = Q.out[si];
@@ -234,14 +247,14 @@ __device__ __forceinline__ void sigma_point(const SigmaArgs &Q, int si) {
 }
 
 template <int KIND>
-__global__ __launch_bounds__(64) void k_sigma(SigmaArgs Q) {
+__global__ __launch_bounds__(SIGMA_T) void k_sigma(SigmaArgs Q) {
     sigma_point<KIND>(Q, blockIdx.x + 1);
 }
 
 // The σ points of several problems in one launch: block b takes point pts[b] & 0xffff of problem
 // pts[b] >> 16, whose arguments are tab[problem]; each point is computed exactly as k_sigma does.
 template <int KIND>
-__global__ __launch_bounds__(64) void k_sigma_batch(const SigmaArgs *__restrict__ tab, const uint32_t *__restrict__ pts) {
+__global__ __launch_bounds__(SIGMA_T) void k_sigma_batch(const SigmaArgs *__restrict__ tab, const uint32_t *__restrict__ pts) {
     const uint32_t pt = pts[blockIdx.x];
     sigma_point<KIND>(tab[pt >> 16], (int)(pt & 0xffffu));
 }
@@ -905,7 +918,7 @@ int launch_sigma(pffrg_ctx *c, double L, int a, int v, double *out, const double
                  const double *tol) {
     const SigmaArgs Q = sigma_args(c, L, a, v, out, sigV, add, tol);
     if (c->sys.nsig > 1) {
-        hipLaunchKernelGGL((k_sigma<KIND>), dim3(c->sys.nsig - 1), dim3(64), 0, c->stream, Q);
+        hipLaunchKernelGGL((k_sigma<KIND>), dim3(c->sys.nsig - 1), dim3(SIGMA_T), 0, c->stream, Q);
         HIPCHK(c, hipGetLastError());
     }
     return 0;
@@ -1791,7 +1804,7 @@ int batch_dsigma(pffrg_batch *b, const double *Lam, const int32_t *a, const int3
     if (hipMemcpy(b->d_sig_tab, tab.data(), n * sizeof(SigmaArgs), hipMemcpyHostToDevice) != hipSuccess)
         return bfail(b, PFFRG_ERR_HIP, "σ argument table upload failed");
     if (b->n_sig_pts > 0) {
-        hipLaunchKernelGGL((k_sigma_batch<SK_DSIGMA>), dim3(b->n_sig_pts), dim3(64), 0, b->stream,
+        hipLaunchKernelGGL((k_sigma_batch<SK_DSIGMA>), dim3(b->n_sig_pts), dim3(SIGMA_T), 0, b->stream,
                            reinterpret_cast<const SigmaArgs *>(b->d_sig_tab), b->d_sig_pts);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(b->stream);

@@ -570,4 +570,105 @@ __device__ double gk_block(const F &f, int mode, double s0, const double *bp, in
     return I;  // valid on lane 0
 }
 
+// gk_block with the rule nodes evaluated by the whole block: ev(n, ts, fv) is called by every
+// thread and writes fv[q], the mapped integrand at rule node ts[q] (q < n; ts and fv in LDS, room
+// for 60 nodes).  The segment list, the bisection choice and every sum are gk_block's (thread 0).
+template <typename EV>
+__device__ double gk_block_coop(const EV &ev, const double *bp, int nbp, double atol, double rtol, Seg *segs,
+                                double *fv, double *ts, int *ctl, double *ctld) {
+    const int lane = threadIdx.x;
+    const int nseg0 = nbp - 1;
+    if (lane < 15 * nseg0) {
+        const int q = lane / 15, k = lane % 15;
+        ts[lane] = gk_node(bp[q], bp[q + 1], k);
+    }
+    __syncthreads();
+    ev(15 * nseg0, ts, fv);
+    __syncthreads();
+    if (lane == 0) {
+        double I = 0.0, E = 0.0;
+        for (int q = 0; q < nseg0; ++q) {
+            segs[q] = gk_combine(bp[q], bp[q + 1], fv + 15 * q);
+            if (q == 0) { I = segs[0].I; E = segs[0].E; }
+            else { I += segs[q].I; E += segs[q].E; }
+        }
+        ctl[0] = nseg0;
+        ctld[0] = I;
+        ctld[1] = E;
+        ctl[1] = (E <= atol || E <= rtol * fabs(I)) ? 0 : 1;
+    }
+    __syncthreads();
+    __shared__ double s_be[32];
+    __shared__ int s_bk[32];
+    const int nwv = (blockDim.x + 63) / 64;
+    while (true) {
+        // the segment with the largest error, the first one on ties (gk_block's serial choice):
+        // each thread scans its stride, then the waves and the block reduce the (error, index) pairs
+        const int nseg = ctl[0];
+        double eb = 0.0;
+        int kb = -1;
+        for (int j = lane; j < nseg; j += blockDim.x)
+            if (kb < 0 || segs[j].E > eb) {
+                eb = segs[j].E;
+                kb = j;
+            }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double eo = __shfl_xor(eb, off);
+            const int ko = __shfl_xor(kb, off);
+            if (ko >= 0 && (kb < 0 || eo > eb || (eo == eb && ko < kb))) {
+                eb = eo;
+                kb = ko;
+            }
+        }
+        if ((lane & 63) == 0) {
+            s_be[lane >> 6] = eb;
+            s_bk[lane >> 6] = kb;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            const double I = ctld[0], E = ctld[1];
+            int go = ctl[1] && (E > atol && E > rtol * fabs(I) && ctl[0] < GK_MAX_SEGS);
+            if (go) {
+                int k = s_bk[0];
+                double ek = s_be[0];
+                for (int q = 1; q < nwv; ++q)
+                    if (s_bk[q] >= 0 && (k < 0 || s_be[q] > ek || (s_be[q] == ek && s_bk[q] < k))) {
+                        ek = s_be[q];
+                        k = s_bk[q];
+                    }
+                ctl[2] = k;
+            }
+            ctl[1] = go;
+        }
+        __syncthreads();
+        if (!ctl[1]) break;
+        const int k = ctl[2];
+        const double a = segs[k].a, b = segs[k].b;
+        const double mid = 0.5 * (a + b);
+        if (lane < 30) {
+            const int q = lane / 15, kk = lane % 15;
+            ts[lane] = q == 0 ? gk_node(a, mid, kk) : gk_node(mid, b, kk);
+        }
+        __syncthreads();
+        ev(30, ts, fv);
+        __syncthreads();
+        if (lane == 0) {
+            const Seg s = segs[k];
+            const Seg s1 = gk_combine(a, mid, fv), s2 = gk_combine(mid, b, fv + 15);
+            ctld[0] = (ctld[0] - s.I) + s1.I + s2.I;
+            ctld[1] = (ctld[1] - s.E) + s1.E + s2.E;
+            segs[k] = s1;
+            segs[ctl[0]] = s2;
+            ctl[0] += 1;
+        }
+        __syncthreads();
+    }
+    double I = 0.0;
+    if (lane == 0) {
+        I = segs[0].I;
+        for (int j = 1; j < ctl[0]; ++j) I += segs[j].I;
+    }
+    return I;  // valid on thread 0
+}
+
 }  // namespace pffrg
